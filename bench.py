@@ -1,0 +1,190 @@
+"""Benchmark: frequency points per second of the per-frequency solve
+(Green's tables + FP64 fill + LU + port de-embedding / S) on one MI355X.
+
+Workload (default): the 40 x 20 mm via-stitched grounded-coplanar pour
+(`workloads.gcpw40`, 12 104 RWG + 324 via unknowns = 12 428, 51 frequencies
+1-20 GHz).  A step is one frequency point of the sweep.  The headline path
+is the mixed-precision one (FP64 fill, complex64 LU on the matrix cores,
+FP64 iterative refinement); the plain FP64 path (complex128 LU) is timed
+beside it on the same frequencies, and |dS| between the two is reported.
+Inputs (mesh, fill plan) are resident on the GPU before the timed region;
+the Green's tables of the timed frequencies are built inside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+Prints one JSON line (rank 0).  More than one GPU is not part of this build.
+"""
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"fp64_mfma": 78.6e12, "fp32_mfma": 157.3e12, "fp64_valu": 78.6e12, "hbm": 8e12}
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def time_path(mesh, sub, freqs, precision, warmup, steps, torch):
+    from mom25d_amd import Solver
+
+    s = Solver(mesh, sub, verbose=False, precision=precision)
+    s.geometry().upload(s.device)
+    nf = len(freqs)
+    pick = lambda a, b: freqs[np.arange(a, b) % nf]  # noqa: E731
+    if warmup:
+        s.sweep(pick(0, warmup))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = s.sweep(pick(warmup, warmup + steps))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    return s, r, t1 - t0
+
+
+def gemm_roofline(solver, freq, torch):
+    """Event-timed trailing-update GEMM launches of one more solve."""
+    from mom25d_amd import _lib
+
+    g = solver.tables([freq])[0]
+    _lib.call("mom_gemm_timing_begin", 4096)
+    solver._solve_dev(g, solver.port_excitations())
+    torch.cuda.synchronize()
+    out = np.zeros(3)
+    _lib.call("mom_gemm_timing_end", out.ctypes.data_as(__import__("ctypes").c_void_p))
+    return int(out[0]), out[1] * 1e-3, out[2]
+
+
+def fill_time(solver, freq, torch, reps=3):
+    g = solver.tables([freq])[0]
+    n = solver.mesh.n_unknowns
+    Z = solver._buf("Z", (n, n), torch.complex128)
+    solver._fill(g, Z)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    for _ in range(reps):
+        solver._fill(g, Z)
+    ev[1].record()
+    torch.cuda.synchronize()
+    return ev[0].elapsed_time(ev[1]) * 1e-3 / reps
+
+
+def cpu_baseline(mesh, sub, freqs, npts):
+    from threadpoolctl import threadpool_limits
+
+    from oracle.solve_ref import OracleSolver
+
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    o = OracleSolver(mesh, sub, use_c=True)
+    o.geometry()
+    with threadpool_limits(cores):
+        t0 = time.perf_counter()
+        o.sweep(freqs[:npts])
+        dt = time.perf_counter() - t0
+    return dict(value=npts / dt, unit="freq points/s", cores=cores, kind="port",
+                sample=f"{npts} frequency point(s) ({', '.join(f'{f / 1e9:g}' for f in freqs[:npts])} GHz) of the same "
+                       f"workload: oracle tables (scipy J0), C/OpenMP pair kernel + serial scatter, "
+                       f"LAPACK zgetrf/zgetrs (OpenBLAS), numpy de-embedding; per-mesh set-up untimed; "
+                       f"CPU {_cpu_model()}",
+                split={k: round(v, 3) for k, v in o.timing.items()})
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="gcpw40")
+    ap.add_argument("--cpu-sample", type=int, default=1, help="frequency points timed on the CPU oracle (0: skip)")
+    ap.add_argument("--no-fp64", action="store_true", help="skip timing the plain FP64 path")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build runs on one GPU only (frequency sharding across GPUs is a design note, "
+              "not implemented); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from mom25d_amd import workloads as W
+
+    lay, sub, mk, freqs = W.ALL[args.workload]()
+    mesh = lay.mesh(**mk)
+    N = mesh.n_unknowns
+    print(f"workload {args.workload}: N={N} ({mesh.n_basis} RWG + {mesh.n_vias} vias), {len(freqs)} f", file=sys.stderr,
+          flush=True)
+
+    sm, rm, tm = time_path(mesh, sub, freqs, "mixed", args.warmup, args.steps, torch)
+    print(f"mixed: {args.steps} points in {tm:.3f} s, refinement its {sm.last_refine}", file=sys.stderr, flush=True)
+    res = {}
+    if not args.no_fp64:
+        s64, r64, t64 = time_path(mesh, sub, freqs, "fp64", args.warmup, args.steps, torch)
+        res["fp64"] = dict(value=args.steps / t64, ms_per_step=1e3 * t64 / args.steps,
+                           dS_max=float(np.max(np.abs(rm.S - r64.S))))
+        print(f"fp64: {args.steps} points in {t64:.3f} s; |dS| mixed vs fp64 {res['fp64']['dS_max']:.2e}",
+              file=sys.stderr, flush=True)
+
+    f_probe = freqs[len(freqs) // 2]
+    calls, secs, flops = gemm_roofline(sm, f_probe, torch)
+    achieved = flops / secs if secs > 0 else 0.0
+    roof = dict(bound="mfma", achieved=achieved / 1e12, peak=PEAK["fp32_mfma"] / 1e12, unit="TFLOP/s",
+                frac=achieved / PEAK["fp32_mfma"], traffic=None,
+                kernel="cgemm_sub_kernel<float> (LU trailing update, complex64)",
+                launches=calls, avg_launch_ms=1e3 * secs / max(calls, 1), flops_per_launch=flops / max(calls, 1))
+    tf = fill_time(sm, f_probe, torch)
+    fill_flops = sm.geometry().algorithmic_flops()
+    extra = dict(fill=dict(seconds=tf, algorithmic_flops=fill_flops, achieved_tflops=fill_flops / tf / 1e12,
+                           frac_fp64_valu=fill_flops / tf / PEAK["fp64_valu"]))
+    if not args.no_fp64:
+        c2, s2, f2 = gemm_roofline(s64, f_probe, torch)
+        extra["lu_fp64_gemm"] = dict(achieved_tflops=f2 / s2 / 1e12, frac=f2 / s2 / PEAK["fp64_mfma"],
+                                     launches=c2, seconds=s2)
+    cpu = None
+    if args.cpu_sample > 0:
+        cpu = cpu_baseline(mesh, sub, freqs, args.cpu_sample)
+        print(f"cpu baseline: {cpu['value']:.4f} points/s {cpu['split']}", file=sys.stderr, flush=True)
+
+    line = {
+        "metric": "freq points/s (fill+LU+S) at N~12k RWG, |dS|<1e-4 vs FP64; fill FLOP/s",
+        "value": args.steps / tm,
+        "unit": "freq points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * tm / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 fill / c64 LU + f64 refinement",
+        "data": "synthetic (seeded / deterministic layouts)",
+        "config": {"workload": f"{args.workload}: 40x20 mm via-stitched GCPW pour, N={N} unknowns, "
+                               f"{len(freqs)} f 1-20 GHz" if args.workload == "gcpw40" else f"{args.workload}: N={N}",
+                   "N": N, "precision": "mixed", "parallelism": "1 GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "fp64_path": res.get("fp64"),
+        "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,118 @@
+/* C-ABI of libmom25d.so: the gfx950 kernels of mom25d's per-frequency solve.
+ *
+ * Conventions
+ *   - every pointer argument except `args` / host structs is a device pointer
+ *     (the caller, e.g. PyTorch, owns every buffer; the library allocates
+ *     nothing);
+ *   - complex numbers are interleaved (re, im) doubles (complex128) or floats
+ *     (complex64); matrices are row-major with a leading dimension counted in
+ *     complex elements;
+ *   - `stream` is a hipStream_t; all calls are asynchronous on it;
+ *   - the return value is 0 on success and < 0 on error, with the text in
+ *     mom_last_error() (thread-local).  No C++ exception crosses the ABI.
+ *
+ * Each entry point names the reference interface (mom25d, pure Python) whose
+ * work it takes over.
+ */
+#ifndef MOM25D_H
+#define MOM25D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+const char* mom_last_error(void);
+int mom_abi_version(void);
+
+/* Sommerfeld integrals of the Green's-function remainders, batched over the
+ * table rows of every frequency of a sweep:
+ *   out[row, c] = sum_k J0(rho[row] kc_k) Cc[k, c] + sum_k J0(rho[row] kr_k) Cr[k, c]
+ * over the nodes of problem row_prob[row] (kc: complex lifted path, kr: real
+ * tail; offsets / lengths per problem).  ncol = 3 * levels^2 (3, 12 or 27).
+ * Replaces GroundedSlabGreens._sommerfeld / residual
+ * (Engine/mom25d/greens.py:157-173) and LayeredGreens._sommerfeld / _residual
+ * (Engine/mom25d/layered.py:360-377). */
+int mom_sommerfeld(int n_rows, const double* rho, const int32_t* row_prob, const int64_t* kc_off,
+                   const int32_t* kc_len, const double* kc, const double* Cc, const int64_t* kr_off,
+                   const int32_t* kr_len, const double* kr, const double* Cr, int ncol, double* out, void* stream);
+
+/* Element-wise Bessel J0 (complex z, or real part only when real_axis != 0):
+ * the scipy.special.jv(0, .) / j0(.) calls of Engine/mom25d/greens.py:164. */
+int mom_j0(int n, const double* z, double* out, int real_axis, void* stream);
+
+/* FP64 impedance-matrix fill of one tier (0 near, 1 mid, 2 far) for n_items
+ * work items; `args` points to a host FillArgs block (layout: csrc/fill.hip,
+ * size mom_fill_args_size()).  Adds into Z (zeroed by the caller).
+ * Replaces _fast.chunk_vals / group_vals / scatter (Engine/mom25d/_fast.py:82-158,
+ * 190-211) and the block loop of assemble (Engine/mom25d/solver.py:474-509). */
+int mom_fill_tier(int tier, int n_items, const void* args, void* stream);
+int mom_fill_args_size(void);
+int mom_pair_tab_size(void);
+
+/* Z[b_i, b_j] += zs * g[t][i][j]: surface-impedance (conductor loss) term,
+ * Engine/mom25d/solver.py:511-513. */
+int mom_gram_add(int nt, const double* g, const int32_t* tb, double zs_re, double zs_im, double* Z, int64_t ldz,
+                 void* stream);
+
+/* Via rows / columns and the via-via block of Z from IVpad (pad triangles x
+ * all triangles) and the host part E (nv x nv), Engine/mom25d/solver.py:527-566. */
+int mom_via_blocks(int nb, int nv, int nt, const double* IVpad, const int32_t* d_off, const int32_t* d_pad,
+                   const double* d_val, const int32_t* pad_tri, const int32_t* tp, const int32_t* vp,
+                   const int32_t* tm, const int32_t* vm, const double* cV, const double* E, double ijwe_re,
+                   double ijwe_im, double* Z, int64_t ldz, void* stream);
+
+/* Z <- (Z + Z^T)/2 in place; a complex64 copy into Z32 when non-null
+ * (Engine/mom25d/solver.py:522-524). */
+int mom_symmetrize(int n, double* Z, int64_t ldz, float* Z32, int64_t ld32, void* stream);
+
+/* Line voltage at points of level li from the triangle charges q (nt x nrhs),
+ * Engine/mom25d/solver.py:1101-1125 (Solver.line_voltage). */
+int mom_line_voltage(int npts, const double* pts, int li, int nl, int nt, int nrhs, const int32_t* tri_layer,
+                     const double* V, const double* qp7, const double* qw7, const void* pairs, const double* tab_x,
+                     const double* tab_cw, const int32_t* tab_b, const double* q, double inv_eps0, double* out,
+                     void* stream);
+
+/* LU with partial pivoting of an n x n complex matrix in place (prec 64:
+ * complex128, 32: complex64), panel width jb; ipiv[k] = row swapped with
+ * row k (0-based); info[0] = first zero pivot + 1 or 0.  `work` holds
+ * mom_getrf_work_bytes(prec, n, jb) bytes.  Replaces scipy.linalg.lu_factor
+ * (LAPACK zgetrf) of _dense_solve, Engine/mom25d/solver.py:11-23. */
+int64_t mom_getrf_work_bytes(int prec, int n, int jb);
+int mom_getrf(int prec, int n, void* A, int64_t lda, int32_t* ipiv, int32_t* info, void* work, int64_t work_bytes,
+              int jb, void* stream);
+
+/* Row permutation of the factorisation: perm[i] = original row of row i. */
+int mom_build_perm(int n, const int32_t* ipiv, int32_t* perm, void* stream);
+
+/* X = (LU)^-1 P B for nrhs <= 4 right-hand sides (n x nrhs, row-major);
+ * `work` holds mom_getrs_work_bytes(n) bytes.  Replaces scipy.linalg.lu_solve
+ * (LAPACK zgetrs), Engine/mom25d/solver.py:23. */
+int64_t mom_getrs_work_bytes(int n);
+int mom_getrs(int prec, int n, int nrhs, const void* LU, int64_t lda, const int32_t* perm, const void* B, void* X,
+              void* work, void* stream);
+
+/* C (m x n) -= A (m x k) B (k x n), complex, on the matrix cores (the LU's
+ * trailing update, exposed for testing and tuning). */
+int mom_cgemm_sub(int prec, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb, void* C,
+                  int64_t ldc, void* stream);
+
+/* Event timing of the LU's trailing-update GEMM launches (roofline report):
+ * begin(capacity) starts recording up to `capacity` launches; end(out) waits
+ * and writes out[0] = launches, out[1] = total ms, out[2] = total flops. */
+int mom_gemm_timing_begin(int capacity);
+int mom_gemm_timing_end(double* out);
+
+/* FP64 iterative refinement pieces: R = B - Z X (nrhs <= 4), complex128 ->
+ * complex64 conversion, and X += D with the column maxima |D|, |X| (|re|+|im|,
+ * as IEEE bit patterns in uint64) for the stopping test. */
+int mom_residual(int n, int nrhs, const void* Z, int64_t ldz, const void* X, const void* B, void* R, void* stream);
+int mom_cvt_z_to_c(int64_t n_reals, const double* src, float* dst, void* stream);
+int mom_refine_update(int n, int nrhs, const void* D, void* X, void* dmax, void* xmax, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MOM25D_H */

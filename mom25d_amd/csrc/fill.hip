@@ -1,0 +1,536 @@
+// FP64 MPIE impedance-matrix fill for RWG bases on triangles.
+//
+// Work is organised as in Engine/mom25d/_fast.py:82-158 (group_vals +
+// scatter) and Engine/mom25d/solver.py:429-524 (assemble): test triangles
+// come in Morton-ordered blocks of up to 32, and every source triangle has
+// one of three tiers per (test block, source triangle):
+//   near  7-point source rule on the smooth part, plus the closed-form
+//         integrals of 1/R and (r'-r)/R (Wilton et al. 1984) at each test point
+//   mid   3-point source rule on the full kernel
+//   far   1-point (centroid) source rule on the full kernel
+// The test side always uses the 7-point Dunavant rule.  For a triangle pair
+// (p, s) the kernel forms the 3x3 half-basis block
+//   vals[i][j] = jw mu0 cA_p,i cA_s,j IA[i][j] + cV_p,i cV_s,j IV / (jw eps0)
+// with IA[i][j] = sum_k w_k (r_k - a_i).(K1A(r_k) - b_j K0A(r_k)),
+//      IV       = sum_k w_k K0V(r_k),
+// and adds it to Z[basis(p,i), basis(s,j)] (boundary edges carry no basis).
+//
+// GPU mapping: one workgroup (4 wavefronts) per work item = (test block,
+// source layer, tier, chunk of <= 64 source triangles).  Lane l of every
+// wavefront owns source triangle l of the chunk; wavefront w walks test
+// triangles w, w+4, ... of the block, whose data is staged in LDS.  The
+// smooth kernel is a cubic spline read from global memory (L1/L2 resident);
+// its interval comes from a uniform bucket table plus a one-step fix-up,
+// giving exactly scipy's find_interval result (clamped end intervals).
+// Contributions are summed into Z with hardware FP64 atomics; the matrix is
+// symmetrised by a later pass, as the reference does.
+
+#include "common.h"
+
+namespace mom {
+
+constexpr int kBlock = 32;     // test triangles per block
+constexpr int kChunk = 64;     // source triangles per work item (one per lane)
+constexpr double kFourPi = 12.566370614359172954;
+
+struct PairTab {       // spline of one (observer level, source level) pair at one frequency
+  int64_t x_off;       // offset of the rho nodes
+  int64_t c_off;       // offset (in doubles) of the coefficients, 16 per interval
+  int64_t b_off;       // offset of the bucket table
+  int32_t n;           // number of nodes
+  int32_t nbucket;
+  double inv_dx;       // 1 / bucket width
+  double dz;           // vertical distance of the two levels
+  double cA_re, cA_im; // coefficient of 1/(4 pi R) in gA
+  double cV_re, cV_im; // coefficient of 1/(4 pi R) in gV
+  double cW_re, cW_im; // coefficient of 1/(4 pi R) in gW (line voltage)
+  int32_t zero;        // levels in different stack sections: no coupling
+  int32_t pad_;
+};
+
+struct FillArgs {
+  // mesh plan
+  const double* V;     // (Nt,3,2) vertices
+  const double* qp7;   // (Nt,7,2)
+  const double* qw7;   // (Nt,7)
+  const double* qp3;   // (Nt,3,2)
+  const double* qw3;   // (Nt,3)
+  const double* cen;   // (Nt,2)
+  const double* area;  // (Nt)
+  const double* cA;    // (Nt,3)
+  const double* cV;    // (Nt,3)
+  const int32_t* tb;   // (Nt,3) basis index or -1
+  const int32_t* pad_pos;   // (Nt) row in IVpad or -1
+  const int32_t* blk_tris;  // (nblk, 32), -1 padded
+  const int32_t* items;     // (n_items, 4): block, pair index, src offset, src count
+  const int32_t* src;       // source triangle lists
+  int32_t nt;
+  // frequency data
+  const PairTab* pairs;
+  const double* tab_x;
+  const double* tab_c;
+  const int32_t* tab_b;
+  double jwmu_re, jwmu_im;
+  double ijwe_re, ijwe_im;
+  double* Z;      // complex, row-major
+  int64_t ldz;    // leading dimension in complex elements
+  double* IVpad;  // complex (npad, Nt) or null
+};
+
+__device__ inline int spline_interval(const double* __restrict__ x, const int32_t* __restrict__ bucket, int n,
+                                      int nbucket, double inv_dx, double R) {
+  int b = (int)(R * inv_dx);
+  b = b < 0 ? 0 : (b >= nbucket ? nbucket - 1 : b);
+  int i = bucket[b];
+  while (i > 0 && R < x[i]) --i;
+  while (i < n - 2 && R >= x[i + 1]) ++i;
+  return i;
+}
+
+// S_A(R), S_V(R) of the tabulated smooth part (scipy PPoly evaluation).
+__device__ inline void spline_eval(const PairTab& pt, const double* __restrict__ x, const double* __restrict__ c,
+                                   const int32_t* __restrict__ bucket, double R, z64& sa, z64& sv) {
+  int i = spline_interval(x, bucket, pt.n, pt.nbucket, pt.inv_dx, R);
+  double d = R - x[i];
+  const double* q = c + (int64_t)i * 16;  // [k][kernel][re,im], k = 0..3 (cubic .. constant)
+  double z = d;
+  double ar = q[12] + q[8] * z, ai = q[13] + q[9] * z;
+  double vr = q[14] + q[10] * z, vi = q[15] + q[11] * z;
+  z *= d;
+  ar += q[4] * z; ai += q[5] * z; vr += q[6] * z; vi += q[7] * z;
+  z *= d;
+  ar += q[0] * z; ai += q[1] * z; vr += q[2] * z; vi += q[3] * z;
+  sa = {ar, ai};
+  sv = {vr, vi};
+}
+
+// Closed-form I0 = int_T 1/R dS' and I1 = int_T (r'-r)/R dS' (in-plane part)
+// for an observer at (rx, ry) a height d above the triangle's plane
+// (Engine/mom25d/integrals.py:24-67).
+__device__ inline void wilton(const double* Vs, double rx, double ry, double d, double& I0, double& I1x,
+                              double& I1y) {
+  const double d2 = d * d;
+  I0 = 0.0; I1x = 0.0; I1y = 0.0;
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    const int a = (e + 1) % 3, b = (e + 2) % 3;
+    double pmx = Vs[2 * a], pmy = Vs[2 * a + 1];
+    double ppx = Vs[2 * b], ppy = Vs[2 * b + 1];
+    double Lx = ppx - pmx, Ly = ppy - pmy;
+    double ln = sqrt(Lx * Lx + Ly * Ly);
+    double lx = Lx / ln, ly = Ly / ln;
+    double ux = ly, uy = -lx;
+    double dmx = pmx - rx, dmy = pmy - ry;
+    double dpx = ppx - rx, dpy = ppy - ry;
+    double lm = dmx * lx + dmy * ly;
+    double lp = dpx * lx + dpy * ly;
+    double t = dmx * ux + dmy * uy;
+    double Rm = sqrt(dmx * dmx + dmy * dmy + d2);
+    double Rp = sqrt(dpx * dpx + dpy * dpy + d2);
+    double R02 = t * t + d2;
+    double logt = 0.0;
+    if (!(sqrt(R02) <= 1e-12 * ln)) {
+      double fp = lp >= 0.0 ? Rp + lp : R02 / (Rp - lp);
+      double fm = lm >= 0.0 ? Rm + lm : R02 / (Rm - lm);
+      logt = log(fp / fm);
+    }
+    I0 += t * logt;
+    if (d != 0.0) I0 -= d * (atan2(t * lp, R02 + d * Rp) - atan2(t * lm, R02 + d * Rm));
+    double g = 0.5 * (R02 * logt + lp * Rp - lm * Rm);
+    I1x += ux * g;
+    I1y += uy * g;
+  }
+}
+
+template <int TIER>
+__global__ __launch_bounds__(256) void fill_kernel(FillArgs a) {
+  constexpr int L = TIER == 0 ? 7 : (TIER == 1 ? 3 : 1);
+  __shared__ double s_qp[kBlock][14];
+  __shared__ double s_qw[kBlock][7];
+  __shared__ double s_V[kBlock][6];
+  __shared__ double s_cA[kBlock][3];
+  __shared__ double s_cV[kBlock][3];
+  __shared__ int32_t s_tb[kBlock][3];
+  __shared__ int32_t s_tri[kBlock];
+
+  const int* item = a.items + 4 * (int64_t)blockIdx.x;
+  const int blk = item[0], pp = item[1], soff = item[2], scnt = item[3];
+  const int tid = threadIdx.x;
+  if (tid < kBlock) {
+    int p = a.blk_tris[blk * kBlock + tid];
+    s_tri[tid] = p;
+    if (p >= 0) {
+      for (int q = 0; q < 14; ++q) s_qp[tid][q] = a.qp7[14 * (int64_t)p + q];
+      for (int q = 0; q < 7; ++q) s_qw[tid][q] = a.qw7[7 * (int64_t)p + q];
+      for (int q = 0; q < 6; ++q) s_V[tid][q] = a.V[6 * (int64_t)p + q];
+      for (int q = 0; q < 3; ++q) {
+        s_cA[tid][q] = a.cA[3 * (int64_t)p + q];
+        s_cV[tid][q] = a.cV[3 * (int64_t)p + q];
+        s_tb[tid][q] = a.tb[3 * (int64_t)p + q];
+      }
+    }
+  }
+  __syncthreads();
+
+  const PairTab pt = a.pairs[pp];
+  const double* __restrict__ tx = a.tab_x + pt.x_off;
+  const double* __restrict__ tc = a.tab_c + pt.c_off;
+  const int32_t* __restrict__ tbk = a.tab_b + pt.b_off;
+  const z64 kA = {pt.cA_re, pt.cA_im}, kV = {pt.cV_re, pt.cV_im};
+  const double dz = pt.dz;
+  const z64 jwmu = {a.jwmu_re, a.jwmu_im}, ijwe = {a.ijwe_re, a.ijwe_im};
+
+  const int lane = tid & 63, wave = tid >> 6;
+  if (lane >= scnt) return;
+  const int s = a.src[soff + lane];
+
+  double Vs[6], px[L], py[L], w[L], cAs[3], cVs[3];
+  int tbs[3];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) Vs[q] = a.V[6 * (int64_t)s + q];
+#pragma unroll
+  for (int l = 0; l < L; ++l) {
+    if (TIER == 0) {
+      px[l] = a.qp7[14 * (int64_t)s + 2 * l];
+      py[l] = a.qp7[14 * (int64_t)s + 2 * l + 1];
+      w[l] = a.qw7[7 * (int64_t)s + l];
+    } else if (TIER == 1) {
+      px[l] = a.qp3[6 * (int64_t)s + 2 * l];
+      py[l] = a.qp3[6 * (int64_t)s + 2 * l + 1];
+      w[l] = a.qw3[3 * (int64_t)s + l];
+    } else {
+      px[l] = a.cen[2 * (int64_t)s];
+      py[l] = a.cen[2 * (int64_t)s + 1];
+      w[l] = a.area[s];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    cAs[q] = a.cA[3 * (int64_t)s + q];
+    cVs[q] = a.cV[3 * (int64_t)s + q];
+    tbs[q] = a.tb[3 * (int64_t)s + q];
+  }
+  const int pad = a.IVpad ? 0 : -1;
+
+  for (int pl = wave; pl < kBlock; pl += 4) {
+    const int p = s_tri[pl];
+    if (p < 0) break;
+    z64 IA[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) IA[i][j] = {0.0, 0.0};
+    z64 IV = {0.0, 0.0};
+    for (int k = 0; k < 7; ++k) {
+      const double rx = s_qp[pl][2 * k], ry = s_qp[pl][2 * k + 1];
+      z64 k0a, k1x, k1y, k0v;
+      if (TIER == 0) {
+        double I0, I1x, I1y;
+        wilton(Vs, rx, ry, dz, I0, I1x, I1y);
+        const double S0 = I0 / kFourPi;
+        const double S1x = (rx * I0 + I1x) / kFourPi, S1y = (ry * I0 + I1y) / kFourPi;
+        k0a = kA * S0; k1x = kA * S1x; k1y = kA * S1y; k0v = kV * S0;
+      } else {
+        k0a = {0.0, 0.0}; k1x = k0a; k1y = k0a; k0v = k0a;
+      }
+#pragma unroll
+      for (int l = 0; l < L; ++l) {
+        const double ddx = rx - px[l], ddy = ry - py[l];
+        const double R = sqrt(ddx * ddx + ddy * ddy);
+        z64 sa, sv;
+        spline_eval(pt, tx, tc, tbk, R, sa, sv);
+        if (TIER != 0) {
+          const double inv = (dz == 0.0) ? 1.0 / (kFourPi * R) : 1.0 / (kFourPi * sqrt(R * R + dz * dz));
+          sa += kA * inv;
+          sv += kV * inv;
+        }
+        const z64 saw = sa * w[l];
+        k0a += saw;
+        k1x += saw * px[l];
+        k1y += saw * py[l];
+        k0v += sv * w[l];
+      }
+      const double wk = s_qw[pl][k];
+      IV += wk * k0v;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const double ax = wk * (rx - s_V[pl][2 * i]);
+        const double ay = wk * (ry - s_V[pl][2 * i + 1]);
+        const z64 t1 = ax * k1x + ay * k1y;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) IA[i][j] += t1 - (ax * Vs[2 * j] + ay * Vs[2 * j + 1]) * k0a;
+      }
+    }
+    if (pad == 0) {
+      const int pr = a.pad_pos[p];
+      if (pr >= 0) {
+        z64* dst = reinterpret_cast<z64*>(a.IVpad) + (int64_t)pr * a.nt + s;
+        *dst = IV;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const int row = s_tb[pl][i];
+      if (row < 0) continue;
+      const double cAi = s_cA[pl][i], cVi = s_cV[pl][i];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const int col = tbs[j];
+        if (col < 0) continue;
+        const z64 v = (jwmu * (cAi * cAs[j])) * IA[i][j] + (ijwe * (cVi * cVs[j])) * IV;
+        double* dst = a.Z + 2 * ((int64_t)row * a.ldz + col);
+        unsafeAtomicAdd(dst, v.re);
+        unsafeAtomicAdd(dst + 1, v.im);
+      }
+    }
+  }
+}
+
+// Z[b_i, b_j] += zs * g[t][i][j]: conductor loss through the Gram matrix of
+// the RWG functions on each triangle (Engine/mom25d/solver.py:385-391, 511-513).
+__global__ void gram_kernel(int nt, const double* __restrict__ g, const int32_t* __restrict__ tb, double zs_re,
+                            double zs_im, double* __restrict__ Z, int64_t ldz) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  for (int i = 0; i < 3; ++i) {
+    int bi = tb[3 * t + i];
+    if (bi < 0) continue;
+    for (int j = 0; j < 3; ++j) {
+      int bj = tb[3 * t + j];
+      if (bj < 0) continue;
+      double v = g[9 * t + 3 * i + j];
+      double* dst = Z + 2 * ((int64_t)bi * ldz + bj);
+      unsafeAtomicAdd(dst, zs_re * v);
+      unsafeAtomicAdd(dst + 1, zs_im * v);
+    }
+  }
+}
+
+// Via rows / columns (Engine/mom25d/solver.py:527-566):
+//   Zvb[k, n] = 1/(jw eps0) sum_pad D[pad, k] (cV+ IVpad[pad, T+_n] + cV- IVpad[pad, T-_n])
+// written to Z[nb + k, n] and Z[n, nb + k].
+__global__ void via_rows_kernel(int nb, int nv, int nt, const double* __restrict__ IVpad,
+                                const int32_t* __restrict__ d_off, const int32_t* __restrict__ d_pad,
+                                const double* __restrict__ d_val, const int32_t* __restrict__ tp,
+                                const int32_t* __restrict__ vp, const int32_t* __restrict__ tm,
+                                const int32_t* __restrict__ vm, const double* __restrict__ cV, double ijwe_re,
+                                double ijwe_im, double* __restrict__ Z, int64_t ldz) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  int k = blockIdx.y;
+  if (n >= nb) return;
+  const z64* IV = reinterpret_cast<const z64*>(IVpad);
+  const int a = tp[n], b = tm[n];
+  const double ca = cV[3 * a + vp[n]], cb = cV[3 * b + vm[n]];
+  z64 acc = {0.0, 0.0};
+  for (int q = d_off[k]; q < d_off[k + 1]; ++q) {
+    const int64_t row = (int64_t)d_pad[q] * nt;
+    z64 y = ca * IV[row + a] + cb * IV[row + b];
+    acc += d_val[q] * y;
+  }
+  const z64 v = z64{ijwe_re, ijwe_im} * acc;
+  z64* Zc = reinterpret_cast<z64*>(Z);
+  Zc[(int64_t)(nb + k) * ldz + n] = v;
+  Zc[(int64_t)n * ldz + nb + k] = v;
+}
+
+//   Zvv[k, m] = 1/(jw eps0) sum D[pad,k] IVpad[pad, tri(pad')] D[pad',m] + E[k, m]
+__global__ void via_block_kernel(int nb, int nv, int nt, const double* __restrict__ IVpad,
+                                 const int32_t* __restrict__ d_off, const int32_t* __restrict__ d_pad,
+                                 const double* __restrict__ d_val, const int32_t* __restrict__ pad_tri,
+                                 const double* __restrict__ E, double ijwe_re, double ijwe_im,
+                                 double* __restrict__ Z, int64_t ldz) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  int k = blockIdx.y;
+  if (m >= nv) return;
+  const z64* IV = reinterpret_cast<const z64*>(IVpad);
+  z64 acc = {0.0, 0.0};
+  for (int q = d_off[k]; q < d_off[k + 1]; ++q) {
+    const int64_t row = (int64_t)d_pad[q] * nt;
+    z64 inner = {0.0, 0.0};
+    for (int r = d_off[m]; r < d_off[m + 1]; ++r) inner += d_val[r] * IV[row + pad_tri[d_pad[r]]];
+    acc += d_val[q] * inner;
+  }
+  const z64 e = reinterpret_cast<const z64*>(E)[(int64_t)k * nv + m];
+  reinterpret_cast<z64*>(Z)[(int64_t)(nb + k) * ldz + nb + m] = z64{ijwe_re, ijwe_im} * acc + e;
+}
+
+// Z <- 0.5 (Z + Z^T) in place (Engine/mom25d/solver.py:522-524); optionally
+// also writes a complex64 copy Z32 (leading dimension ld32) of the result.
+__global__ __launch_bounds__(256) void symmetrize_kernel(int n, double* __restrict__ Z, int64_t ldz,
+                                                         float* __restrict__ Z32, int64_t ld32) {
+  const int I = blockIdx.y, J = blockIdx.x;
+  if (I > J) return;
+  __shared__ z64 A[32][33];
+  __shared__ z64 B[32][33];
+  z64* Zc = reinterpret_cast<z64*>(Z);
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int r = ty; r < 32; r += 8) {
+    int gi = I * 32 + r, gj = J * 32 + tx;
+    if (gi < n && gj < n) A[r][tx] = Zc[(int64_t)gi * ldz + gj];
+    gi = J * 32 + r; gj = I * 32 + tx;
+    if (gi < n && gj < n) B[r][tx] = Zc[(int64_t)gi * ldz + gj];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    int gi = I * 32 + r, gj = J * 32 + tx;
+    if (gi < n && gj < n) {
+      z64 a = A[r][tx], b = B[tx][r];
+      z64 v = {0.5 * (a.re + b.re), 0.5 * (a.im + b.im)};
+      Zc[(int64_t)gi * ldz + gj] = v;
+      if (Z32) {
+        Z32[2 * ((int64_t)gi * ld32 + gj)] = (float)v.re;
+        Z32[2 * ((int64_t)gi * ld32 + gj) + 1] = (float)v.im;
+      }
+    }
+    gi = J * 32 + r; gj = I * 32 + tx;
+    if (I != J && gi < n && gj < n) {
+      z64 a = B[r][tx], b = A[tx][r];
+      z64 v = {0.5 * (a.re + b.re), 0.5 * (a.im + b.im)};
+      Zc[(int64_t)gi * ldz + gj] = v;
+      if (Z32) {
+        Z32[2 * ((int64_t)gi * ld32 + gj)] = (float)v.re;
+        Z32[2 * ((int64_t)gi * ld32 + gj) + 1] = (float)v.im;
+      }
+    }
+  }
+}
+
+// Line voltage to the plane below at observation points on level `li`
+// (Engine/mom25d/solver.py:1101-1125):
+//   phi(pt) = sum_t [ sum_k S_W(|pt - r_k|) w_k + cW I0(t, pt) / (4 pi) ] q_t / eps0
+// One workgroup per point; threads stride over the triangles.  The S_W
+// spline has 8 doubles per interval (c0..c3, re / im) at c_off / 2.
+constexpr int kMaxRhs = 4;
+
+__global__ __launch_bounds__(256) void line_voltage_kernel(int npts, const double* __restrict__ pts, int li, int nl,
+                                                           int nt, int nrhs, const int32_t* __restrict__ tri_layer,
+                                                           const double* __restrict__ V,
+                                                           const double* __restrict__ qp7,
+                                                           const double* __restrict__ qw7,
+                                                           const PairTab* __restrict__ pairs,
+                                                           const double* __restrict__ tab_x,
+                                                           const double* __restrict__ tab_cw,
+                                                           const int32_t* __restrict__ tab_b,
+                                                           const double* __restrict__ q, double inv_eps0,
+                                                           double* __restrict__ out) {
+  const int m = blockIdx.x, tid = threadIdx.x;
+  const double rx = pts[2 * m], ry = pts[2 * m + 1];
+  z64 acc[kMaxRhs];
+#pragma unroll
+  for (int c = 0; c < kMaxRhs; ++c) acc[c] = {0.0, 0.0};
+  const z64* Q = reinterpret_cast<const z64*>(q);
+  for (int t = tid; t < nt; t += 256) {
+    const PairTab pt = pairs[li * nl + tri_layer[t]];
+    if (pt.zero) continue;
+    const double* x = tab_x + pt.x_off;
+    const double* cw = tab_cw + pt.c_off / 2;
+    const int32_t* bk = tab_b + pt.b_off;
+    z64 k = {0.0, 0.0};
+    for (int l = 0; l < 7; ++l) {
+      const double dx = rx - qp7[14 * (int64_t)t + 2 * l], dy = ry - qp7[14 * (int64_t)t + 2 * l + 1];
+      const double R = sqrt(dx * dx + dy * dy);
+      const int i = spline_interval(x, bk, pt.n, pt.nbucket, pt.inv_dx, R);
+      const double d = R - x[i];
+      const double* c = cw + (int64_t)i * 8;
+      double z = d;
+      double sr = c[6] + c[4] * z, si = c[7] + c[5] * z;
+      z *= d;
+      sr += c[2] * z; si += c[3] * z;
+      z *= d;
+      sr += c[0] * z; si += c[1] * z;
+      const double w = qw7[7 * (int64_t)t + l];
+      k += z64{sr * w, si * w};
+    }
+    if (pt.cW_re != 0.0 || pt.cW_im != 0.0) {
+      double I0, I1x, I1y;
+      wilton(V + 6 * (int64_t)t, rx, ry, fabs(pt.dz), I0, I1x, I1y);
+      k += z64{pt.cW_re, pt.cW_im} * (I0 / kFourPi);
+    }
+    for (int c = 0; c < nrhs; ++c) acc[c] += k * Q[(int64_t)t * nrhs + c];
+  }
+  __shared__ z64 part[4][kMaxRhs];
+  const int lane = tid & 63, wave = tid >> 6;
+  for (int c = 0; c < nrhs; ++c) {
+    double re = acc[c].re, im = acc[c].im;
+    for (int off = 32; off > 0; off >>= 1) {
+      re += __shfl_down(re, off, 64);
+      im += __shfl_down(im, off, 64);
+    }
+    if (lane == 0) part[wave][c] = {re, im};
+  }
+  __syncthreads();
+  if (tid < nrhs) {
+    z64 s = part[0][tid];
+    s += part[1][tid];
+    s += part[2][tid];
+    s += part[3][tid];
+    reinterpret_cast<z64*>(out)[(int64_t)m * nrhs + tid] = s * inv_eps0;
+  }
+}
+
+}  // namespace mom
+
+using namespace mom;
+
+extern "C" int mom_fill_tier(int tier, int n_items, const void* args, void* stream) {
+  if (n_items <= 0) return n_items == 0 ? 0 : fail("mom_fill_tier: negative item count");
+  if (!args) return fail("mom_fill_tier: null argument block");
+  const FillArgs a = *reinterpret_cast<const FillArgs*>(args);
+  if (!a.Z || !a.items || !a.src || !a.pairs) return fail("mom_fill_tier: null device pointer");
+  hipStream_t s = (hipStream_t)stream;
+  dim3 grid(n_items), block(256);
+  switch (tier) {
+    case 0: hipLaunchKernelGGL(fill_kernel<0>, grid, block, 0, s, a); break;
+    case 1: hipLaunchKernelGGL(fill_kernel<1>, grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL(fill_kernel<2>, grid, block, 0, s, a); break;
+    default: return fail("mom_fill_tier: tier must be 0 (near), 1 (mid) or 2 (far)");
+  }
+  return check_launch("fill_kernel");
+}
+
+extern "C" int mom_fill_args_size(void) { return (int)sizeof(FillArgs); }
+extern "C" int mom_pair_tab_size(void) { return (int)sizeof(PairTab); }
+
+extern "C" int mom_gram_add(int nt, const double* g, const int32_t* tb, double zs_re, double zs_im, double* Z,
+                            int64_t ldz, void* stream) {
+  if (nt <= 0) return 0;
+  hipLaunchKernelGGL(gram_kernel, dim3((nt + 255) / 256), dim3(256), 0, (hipStream_t)stream, nt, g, tb, zs_re,
+                     zs_im, Z, ldz);
+  return check_launch("gram_kernel");
+}
+
+extern "C" int mom_via_blocks(int nb, int nv, int nt, const double* IVpad, const int32_t* d_off,
+                              const int32_t* d_pad, const double* d_val, const int32_t* pad_tri, const int32_t* tp,
+                              const int32_t* vp, const int32_t* tm, const int32_t* vm, const double* cV,
+                              const double* E, double ijwe_re, double ijwe_im, double* Z, int64_t ldz,
+                              void* stream) {
+  if (nv <= 0) return 0;
+  if (nv > 65535) return fail("mom_via_blocks: more than 65535 via segments");
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(via_rows_kernel, dim3((nb + 255) / 256, nv), dim3(256), 0, s, nb, nv, nt, IVpad, d_off,
+                     d_pad, d_val, tp, vp, tm, vm, cV, ijwe_re, ijwe_im, Z, ldz);
+  int rc = check_launch("via_rows_kernel");
+  if (rc) return rc;
+  hipLaunchKernelGGL(via_block_kernel, dim3((nv + 63) / 64, nv), dim3(64), 0, s, nb, nv, nt, IVpad, d_off, d_pad,
+                     d_val, pad_tri, E, ijwe_re, ijwe_im, Z, ldz);
+  return check_launch("via_block_kernel");
+}
+
+extern "C" int mom_symmetrize(int n, double* Z, int64_t ldz, float* Z32, int64_t ld32, void* stream) {
+  if (n <= 0) return 0;
+  int T = (n + 31) / 32;
+  if (T > 65535) return fail("mom_symmetrize: matrix too large");
+  hipLaunchKernelGGL(symmetrize_kernel, dim3(T, T), dim3(256), 0, (hipStream_t)stream, n, Z, ldz, Z32, ld32);
+  return check_launch("symmetrize_kernel");
+}
+
+extern "C" int mom_line_voltage(int npts, const double* pts, int li, int nl, int nt, int nrhs,
+                                const int32_t* tri_layer, const double* V, const double* qp7, const double* qw7,
+                                const void* pairs, const double* tab_x, const double* tab_cw, const int32_t* tab_b,
+                                const double* q, double inv_eps0, double* out, void* stream) {
+  if (npts <= 0) return 0;
+  if (nrhs < 1 || nrhs > kMaxRhs) return fail("mom_line_voltage: 1..4 right-hand sides");
+  hipLaunchKernelGGL(line_voltage_kernel, dim3(npts), dim3(256), 0, (hipStream_t)stream, npts, pts, li, nl, nt, nrhs,
+                     tri_layer, V, qp7, qw7, (const PairTab*)pairs, tab_x, tab_cw, tab_b, q, inv_eps0, out);
+  return check_launch("line_voltage_kernel");
+}

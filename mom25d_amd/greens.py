@@ -1,0 +1,386 @@
+"""Tabulated spatial Green's functions of a sweep, built with one GPU launch.
+
+For each frequency and each pair of metal levels the MPIE kernels are split
+into a closed-form leading part and a tabulated smooth remainder
+(Engine/mom25d/greens.py:68-228, Engine/mom25d/layered.py:212-415):
+
+    g(R) = c exp(-j k R) / (4 pi R) + S(rho),   R = sqrt(rho^2 + dz^2)
+
+  * path   : a lifted path kr = t + j a sin(pi t / T) on [0, T] (12-point
+             Gauss-Legendre panels), then the real axis up to kmax (8-point
+             panels), as in Engine/mom25d/greens.py:138-155;
+  * spectra: g~(kr) minus its asymptote c exp(-u dz) / (2u), on the host;
+  * Sommerfeld: R(rho) = (1/2 pi) sum J0(kr rho) kr w g~ -- the one
+             expensive step, run by the `mom_sommerfeld` GPU kernel for all
+             rows of all frequencies of the sweep in one launch;
+  * table  : S = c expm1(-j k R)/(4 pi R) + R(rho) (+ the ground-plane loss
+             term in S_A), on the grid r_{i+1} = r_i + min(lam/80, max(h/25, r_i/25));
+  * spline : not-a-knot cubic spline of [S_A, S_V] and of S_W (scipy
+             CubicSpline semantics), solved for all frequencies at once.
+
+The splines are uploaded in the layout the fill kernel reads (16 doubles per
+interval: c0..c3 of S_A, S_V, re / im) together with a bucket table that
+turns the interval search into one lookup and a one-step fix-up.
+"""
+
+import numpy as np
+import torch
+
+from . import _lib
+from .constants import C0, MU0
+from .layered import csqrt_re_pos, layered_spectra
+
+PAIR_DTYPE = np.dtype([("x_off", "<i8"), ("c_off", "<i8"), ("b_off", "<i8"), ("n", "<i4"), ("nbucket", "<i4"),
+                       ("inv_dx", "<f8"), ("dz", "<f8"), ("cA_re", "<f8"), ("cA_im", "<f8"), ("cV_re", "<f8"),
+                       ("cV_im", "<f8"), ("cW_re", "<f8"), ("cW_im", "<f8"), ("zero", "<i4"), ("pad_", "<i4")])
+
+
+def gauss_panels(a, b, n_panels, order):
+    """Composite Gauss-Legendre nodes / weights on [a, b]."""
+    x, w = np.polynomial.legendre.leggauss(order)
+    edges = np.linspace(a, b, n_panels + 1)
+    half = 0.5 * np.diff(edges)
+    mid = 0.5 * (edges[1:] + edges[:-1])
+    return (mid[:, None] + half[:, None] * x[None, :]).ravel(), (half[:, None] * w[None, :]).ravel()
+
+
+def rho_grid(lam, h, rmax):
+    """Table nodes: fine near the source, never coarser than lam / 80."""
+    r = [0.0]
+    while r[-1] < rmax:
+        r.append(r[-1] + min(lam / 80, max(h / 25, r[-1] / 25)))
+    return np.array(r)
+
+
+def spline_coeffs(x, Y):
+    """Not-a-knot cubic spline coefficients c (4, n-1, m) of Y (n, m) on
+    nodes x (n >= 4), in scipy PPoly order (c[0] multiplies dx^3)."""
+    n = x.size
+    dx = np.diff(x)
+    slope = np.diff(Y, axis=0) / dx[:, None]
+    sub = np.zeros(n)
+    dia = np.zeros(n)
+    sup = np.zeros(n)
+    rhs = np.empty(Y.shape, complex)
+    dia[1:-1] = 2 * (dx[:-1] + dx[1:])
+    sup[1:-1] = dx[:-1]
+    sub[1:-1] = dx[1:]
+    rhs[1:-1] = 3 * (dx[1:, None] * slope[:-1] + dx[:-1, None] * slope[1:])
+    d0 = x[2] - x[0]
+    dia[0], sup[0] = dx[1], d0
+    rhs[0] = ((dx[0] + 2 * d0) * dx[1] * slope[0] + dx[0] ** 2 * slope[1]) / d0
+    d1 = x[-1] - x[-3]
+    dia[-1], sub[-1] = dx[-2], d1
+    rhs[-1] = (dx[-1] ** 2 * slope[-2] + (2 * d1 + dx[-1]) * dx[-2] * slope[-1]) / d1
+    # Thomas elimination (the eliminated diagonal stays positive for these systems)
+    cp = np.empty(n)
+    dp = np.empty(Y.shape, complex)
+    cp[0] = sup[0] / dia[0]
+    dp[0] = rhs[0] / dia[0]
+    for i in range(1, n):
+        den = dia[i] - sub[i] * cp[i - 1]
+        cp[i] = sup[i] / den
+        dp[i] = (rhs[i] - sub[i] * dp[i - 1]) / den
+    s = np.empty(Y.shape, complex)
+    s[-1] = dp[-1]
+    for i in range(n - 2, -1, -1):
+        s[i] = dp[i] - cp[i] * s[i + 1]
+    t = (s[:-1] + s[1:] - 2 * slope) / dx[:, None]
+    c = np.empty((4, n - 1, Y.shape[1]), complex)
+    c[0] = t / dx[:, None]
+    c[1] = (slope - s[:-1]) / dx[:, None] - t
+    c[2] = s[:-1]
+    c[3] = Y[:-1]
+    return c
+
+
+def ppoly_eval(x, c, R):
+    """Evaluate PPoly coefficients c (4, n-1, ...) at R (interval by search,
+    clamped to the end intervals)."""
+    R = np.asarray(R, float)
+    i = np.clip(np.searchsorted(x, R, side="right") - 1, 0, x.size - 2)
+    d = (R - x[i])[(...,) + (None,) * (c.ndim - 2)]
+    return ((c[0, i] * d + c[1, i]) * d + c[2, i]) * d + c[3, i]
+
+
+class KernelPair:
+    """Kernels between an observer and a source level at one frequency:
+    g = c / (4 pi R) + S(rho) with R = sqrt(rho^2 + dz^2)."""
+
+    zero = False
+
+    def __init__(self, dz, cA, cV, cW, x, c, cw):
+        self.dz, self.cA, self.cV, self.cW = float(dz), complex(cA), complex(cV), complex(cW)
+        self.cv = self.cV
+        self.x, self.c, self.cw = x, c, cw
+
+    def smooth(self, rho):
+        S = ppoly_eval(self.x, self.c, rho)
+        return S[..., 0], S[..., 1]
+
+    def smooth_w(self, rho):
+        return ppoly_eval(self.x, self.cw, rho)
+
+    def _R(self, rho):
+        return np.sqrt(np.asarray(rho) ** 2 + self.dz ** 2)
+
+    def gA(self, rho):
+        return self.cA / (4 * np.pi * self._R(rho)) + self.smooth(rho)[0]
+
+    def gV(self, rho):
+        return self.cV / (4 * np.pi * self._R(rho)) + self.smooth(rho)[1]
+
+    def gW(self, rho):
+        return self.cW / (4 * np.pi * self._R(rho)) + self.smooth_w(rho)
+
+
+class FreqGreens:
+    """Green's functions of all level pairs at one frequency (host view plus
+    the location of its tables in the sweep's device arrays)."""
+
+    def __init__(self, freq, h, levels, pairs, stack, tables, index):
+        self.freq = freq
+        self.omega = 2 * np.pi * freq
+        self.k0 = self.omega / C0
+        self.h = h
+        self.levels = list(levels)
+        self._pairs = pairs
+        self.stack = stack
+        self.tables = tables
+        self.index = index
+
+    def index_of(self, level):
+        return level if isinstance(level, (int, np.integer)) else self.levels.index(level)
+
+    def pair(self, i=0, j=0):
+        return self._pairs[self.index_of(i), self.index_of(j)]
+
+    @property
+    def pairs_dev(self):
+        """Device PairTab array (L*L entries, observer-major) of this frequency."""
+        return self.tables.pairs_dev[self.index]
+
+    def via_sections(self):
+        return self.stack.via_sections(self.freq)
+
+
+class SweepTables:
+    """Tabulated kernels of a stack at every frequency of a sweep.
+
+    single : one dielectric with copper on top and only the "top" level:
+             the grounded-slab closed-form spectra (Engine/mom25d/greens.py:111-135);
+             otherwise the transmission-line spectra of the stack.
+    """
+
+    def __init__(self, stack, freqs, rho_max, levels, single, device="cuda", gl_order=12):
+        self.stack = stack
+        self.freqs = np.atleast_1d(np.asarray(freqs, float))
+        self.rho_max = float(rho_max)
+        self.levels = list(levels)
+        self.single = bool(single)
+        self.device = torch.device(device)
+        L = len(self.levels)
+        self.L = L
+        st = stack
+        self.il = np.array([st.metals[n] for n in self.levels])
+        self.zl = np.array([st.level_z(n) for n in self.levels])
+        self.epsc = np.array([d.eps_r * (1 - 1j * d.tand) for d in st.layers])
+        self.t = np.array([d.t for d in st.layers])
+        self.h = float(st.z[-1])
+        jobs = [self._prepare(f, gl_order) for f in self.freqs]
+        res = self._sommerfeld(jobs)
+        self._finish(jobs, res)
+
+    # ---------------------------------------------------------- per frequency
+    def _prepare(self, f, order):
+        st, L = self.stack, self.L
+        omega = 2 * np.pi * f
+        k0 = omega / C0
+        kk = k0 * np.sqrt(self.epsc)
+        tmin = self.t.min()
+        kmaxr = np.abs(kk).max()
+        T = 1.3 * kmaxr + k0
+        a = min(0.5 * k0, 1.0 / self.rho_max)
+        n1 = int(np.clip(np.ceil(2 * T / a), 100, 3000))
+        tt, ww = gauss_panels(0.0, T, n1, order)
+        kc = tt + 1j * a * np.sin(np.pi * tt / T)
+        wc = ww * (1 + 1j * a * np.pi / T * np.cos(np.pi * tt / T))
+        kmax = max(40 * kmaxr, 20.0 / tmin, T * 1.5)
+        dk = min(np.pi / self.rho_max, 0.5 / tmin, kmaxr)
+        n2 = int(np.ceil((kmax - T) / dk))
+        kr, wr = gauss_panels(T, kmax, n2, 8)
+
+        dz = np.abs(self.zl[:, None] - self.zl[None, :])
+        c = np.zeros((3, L, L), complex)
+        kx = np.zeros((2, L, L), complex)
+        if self.single:
+            epsc, k1 = self.epsc[0], kk[0]
+            cv = 2.0 / (epsc + 1.0)
+            c[:, 0, 0] = (1.0, cv, cv)
+            kx[0, 0, 0] = np.sqrt(0.5 * (k0 ** 2 + k1 ** 2))
+            kx[1, 0, 0] = np.sqrt((epsc * k0 ** 2 + k1 ** 2) / (epsc + 1.0))
+        else:
+            self._leading(k0, kk, dz, c, kx)
+
+        def remainders(k):
+            if self.single:
+                g = _slab_spectra(k, k0, kk[0], self.epsc[0], self.t[0])
+                g = [gi[None, None] for gi in g]
+            else:
+                g = layered_spectra(k, k0, kk, self.epsc, self.t, self.il, st.top == "pec")
+            out = []
+            for m in range(3):
+                ux = np.sqrt(k[None, None] ** 2 - kx[min(m, 1)][..., None] ** 2)
+                asym = c[m][..., None] / (2 * ux) if not dz.any() else c[m][..., None] * np.exp(-ux * dz[..., None]) / (2 * ux)
+                out.append(g[m] - asym)
+            return np.stack(out).reshape(3 * L * L, k.size)
+
+        Cc = (wc * kc / (2 * np.pi))[None] * remainders(kc.astype(complex))
+        Cr = (wr.astype(complex) * kr / (2 * np.pi))[None] * remainders(kr.astype(complex))
+        lam = 2 * np.pi / kmaxr
+        rho = rho_grid(lam, tmin, self.rho_max)
+        zs = st.surface_impedance(f) if st.ground_loss else 0.0
+        return dict(f=f, omega=omega, k0=k0, kc=kc, Cc=Cc.T.copy(), kr=kr, Cr=Cr.T.copy(), rho=rho, c=c, kx=kx,
+                    dz=dz, zs=zs)
+
+    def _leading(self, k0, kk, dz, c, kx):
+        """Leading terms per pair (Engine/mom25d/layered.py:292-331)."""
+        L, n = self.L, len(self.t)
+        Kbig = 15.0 / self.t.min()
+        big = None
+        for i in range(L):
+            for j in range(L):
+                if i == j:
+                    m = self.il[i]
+                    eb, kb = self.epsc[m - 1], kk[m - 1]
+                    ea, ka = (1.0, k0) if m == n else (self.epsc[m], kk[m])
+                    cv = 2.0 / (ea + eb)
+                    c[:, i, j] = (1.0, cv, cv)
+                    kx[0, i, j] = np.sqrt(0.5 * (ka ** 2 + kb ** 2))
+                    kx[1, i, j] = np.sqrt((eb * ka ** 2 + ea * kb ** 2) / (ea + eb))
+                    continue
+                if big is None:
+                    g = layered_spectra(np.array([Kbig], complex), k0, kk, self.epsc, self.t, self.il,
+                                        self.stack.top == "pec")
+                    big = [gi[..., 0] for gi in g]
+                ux = np.sqrt(Kbig ** 2 - k0 ** 2 + 0j)
+                f = 2 * ux * np.exp(ux * dz[i, j])
+                cA, cV, cW = (f * g[i, j] for g in big)
+                if self.zl[j] < self.zl[i]:
+                    cW = 0.0
+                c[:, i, j] = (cA, cV, cW)
+                kx[:, i, j] = k0
+        small = np.abs(c.imag) < 1e-9 * np.abs(c)
+        c[small] = c[small].real
+
+    # ------------------------------------------------------------ Sommerfeld
+    def _sommerfeld(self, jobs):
+        dev = self.device
+        ncol = 3 * self.L * self.L
+        rho = np.concatenate([j["rho"] for j in jobs])
+        prob = np.concatenate([np.full(j["rho"].size, p, np.int32) for p, j in enumerate(jobs)])
+        kc_len = np.array([j["kc"].size for j in jobs], np.int32)
+        kr_len = np.array([j["kr"].size for j in jobs], np.int32)
+        kc_off = np.concatenate([[0], np.cumsum(kc_len)[:-1]]).astype(np.int64)
+        kr_off = np.concatenate([[0], np.cumsum(kr_len)[:-1]]).astype(np.int64)
+        t = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a)).to(dev, dt)  # noqa: E731
+        d_rho = t(rho, torch.float64)
+        d_prob = t(prob, torch.int32)
+        d_kc_off, d_kc_len = t(kc_off, torch.int64), t(kc_len, torch.int32)
+        d_kr_off, d_kr_len = t(kr_off, torch.int64), t(kr_len, torch.int32)
+        d_kc = t(np.concatenate([j["kc"] for j in jobs]).astype(complex), torch.complex128)
+        d_Cc = t(np.concatenate([j["Cc"] for j in jobs]), torch.complex128)
+        d_kr = t(np.concatenate([j["kr"] for j in jobs]), torch.float64)
+        d_Cr = t(np.concatenate([j["Cr"] for j in jobs]), torch.complex128)
+        out = torch.empty((rho.size, ncol), dtype=torch.complex128, device=dev)
+        P = _lib.ptr
+        _lib.call("mom_sommerfeld", int(rho.size), P(d_rho), P(d_prob), P(d_kc_off), P(d_kc_len), P(d_kc), P(d_Cc),
+                  P(d_kr_off), P(d_kr_len), P(d_kr), P(d_Cr), ncol, P(out), _lib.stream_ptr())
+        res = out.cpu().numpy()
+        parts, o = [], 0
+        for j in jobs:
+            parts.append(res[o:o + j["rho"].size])
+            o += j["rho"].size
+        return parts
+
+    # ----------------------------------------------------------------- tables
+    def _finish(self, jobs, res):
+        L = self.L
+        xs, cs, cws, bks = [], [], [], []
+        x_off = c_off = b_off = 0
+        pair_rows = []
+        self.greens = []
+        for fi, (j, RES) in enumerate(zip(jobs, res)):
+            rho = j["rho"]
+            RES = RES.T.reshape(3, L, L, rho.size)
+            pairs = {}
+            recs = np.zeros(L * L, PAIR_DTYPE)
+            dxb = 0.5 * np.diff(rho).min()
+            nbucket = int(rho[-1] / dxb) + 2
+            edges = np.arange(nbucket) * dxb
+            bucket = np.clip(np.searchsorted(rho, edges, side="right") - 1, 0, rho.size - 2).astype(np.int32)
+            xs.append(rho)
+            bks.append(bucket)
+            for i in range(L):
+                for k in range(L):
+                    dz = j["dz"][i, k]
+                    R = np.sqrt(rho ** 2 + dz ** 2)
+                    S = []
+                    for m in range(3):
+                        kw = j["kx"][min(m, 1), i, k]
+                        with np.errstate(divide="ignore", invalid="ignore"):
+                            f = np.where(R > 0, np.expm1(-1j * kw * R) / (4 * np.pi * R), -1j * kw / (4 * np.pi))
+                        S.append(j["c"][m, i, k] * f + RES[m, i, k])
+                    if j["zs"]:
+                        d = self.zl[i] + self.zl[k]
+                        S[0] = S[0] + j["zs"] / (1j * j["omega"] * MU0) * d / (2 * np.pi * (rho ** 2 + d ** 2) ** 1.5)
+                    coef = spline_coeffs(rho, np.stack(S, -1))  # (4, n-1, 3)
+                    c2, cw = coef[..., :2], coef[..., 2]
+                    pairs[i, k] = KernelPair(dz, *j["c"][:, i, k], rho, c2, cw)
+                    cs.append(np.ascontiguousarray(c2.transpose(1, 0, 2)).view(np.float64).reshape(-1))
+                    cws.append(np.ascontiguousarray(cw.T).view(np.float64).reshape(-1))
+                    r = recs[i * L + k]
+                    r["x_off"], r["c_off"], r["b_off"] = x_off, c_off, b_off
+                    r["n"], r["nbucket"], r["inv_dx"], r["dz"] = rho.size, nbucket, 1.0 / dxb, dz
+                    cA, cV = complex(j["c"][0, i, k]), complex(j["c"][1, i, k])
+                    cW = complex(j["c"][2, i, k])
+                    r["cA_re"], r["cA_im"], r["cV_re"], r["cV_im"] = cA.real, cA.imag, cV.real, cV.imag
+                    r["cW_re"], r["cW_im"] = cW.real, cW.imag
+                    c_off += 16 * (rho.size - 1)
+            x_off += rho.size
+            b_off += nbucket
+            pair_rows.append(recs)
+            self.greens.append(FreqGreens(j["f"], self.h, self.levels, pairs, self.stack, self, fi))
+        dev = self.device
+        self.tab_x = torch.from_numpy(np.concatenate(xs)).to(dev)
+        self.tab_c = torch.from_numpy(np.concatenate(cs)).to(dev)
+        self.tab_cw = torch.from_numpy(np.concatenate(cws)).to(dev)
+        self.tab_b = torch.from_numpy(np.concatenate(bks)).to(dev)
+        allrec = np.concatenate(pair_rows)
+        self._pairs_blob = torch.from_numpy(allrec.view(np.uint8).copy()).to(dev)
+        rec_bytes = PAIR_DTYPE.itemsize * L * L
+        self.pairs_dev = [self._pairs_blob[fi * rec_bytes:(fi + 1) * rec_bytes] for fi in range(len(jobs))]
+        self.pair_records = pair_rows
+
+    def __getitem__(self, fi):
+        return self.greens[fi]
+
+    def __len__(self):
+        return len(self.greens)
+
+
+def _slab_spectra(kr, k0, k1, epsc, h):
+    """(gA~, gV~, gW~) of a grounded slab with conductors on top
+    (Engine/mom25d/greens.py:111-129)."""
+    u0 = np.sqrt(kr ** 2 - k0 ** 2)
+    u1 = csqrt_re_pos(kr ** 2 - k1 ** 2)
+    x = u1 * h
+    e = np.exp(-2 * x)
+    small = np.abs(x) < 1e-4
+    with np.errstate(divide="ignore", invalid="ignore"):
+        u1coth = np.where(small, 1 / h + u1 ** 2 * h / 3, u1 * (1 + e) / (1 - e))
+        u1tanh = np.where(small, u1 ** 2 * h, u1 * (1 - e) / (1 + e))
+        tanh_over_u1 = np.where(small, h, u1tanh / u1 ** 2)
+    dte = u0 + u1coth
+    dtm = epsc * u0 + u1tanh
+    return 1.0 / dte, (u0 + u1tanh) / (dte * dtm), u0 * tanh_over_u1 / dtm

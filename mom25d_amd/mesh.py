@@ -1,0 +1,409 @@
+"""Manhattan layouts, ports, vias and the structured (tensor-grid) RWG mesh.
+
+This is the input side of the per-frequency solve: it must produce exactly
+the mesh the reference builds from the same `Layout` calls, because the
+impedance matrix is compared entry by entry.  The behaviour follows
+Engine/mom25d/mesh.py (Port :62-188, Via :191-241, Layout :309-693,
+_grid_lines :696-703, Mesh :706-1116); pours, polygons, thick metal,
+apertures, lumped gaps and triangular meshing are not part of this build.
+
+Conventions kept from the reference because the numbers depend on them:
+  * grid lines: every rectangle edge and every port's reference / gap
+    coordinate, optional refinement lines +-edge_cell, then uniform
+    subdivision of each interval into ceil(len / max_cell) parts;
+  * a cell is copper if its centre lies strictly inside a rectangle or a
+    via pad of the layer; it is split into two triangles along a diagonal
+    that alternates in a checkerboard;
+  * node numbering: per layer, node (i, j) of the grid has id
+    layer * nx * ny + i * ny + j, then the used ids are renumbered in
+    ascending order;
+  * local edge i of a triangle is the one opposite vertex i; an RWG basis
+    lives on every edge shared by two triangles, T+ being the first of the
+    two in the stable sort of the (3 Nt) half-edges by node-pair key.
+"""
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_AXIS_DIRS = {"+x": (1, 0), "-x": (-1, 0), "+y": (0, 1), "-y": (0, -1)}
+
+
+@dataclass
+class Port:
+    """Microstrip feed with a delta-gap source (Engine/mom25d/mesh.py:62-188).
+
+    ref        : (x, y) of the line centre at the reference plane [m]
+    direction  : '+x', '-x', '+y' or '-y', pointing away from the circuit
+    width      : line width [m]
+    length     : feed length from the reference plane to its open end [m]
+    gap_from_end : delta-gap position measured from the open end (default width)
+    margin     : distance kept free of near fields in the wave fit (default
+                 max(1.5 width, 4 h), applied by the solver)
+    layer      : metal layer of the feed
+    z0_from    : "voltage" or "reciprocity" (how the line impedance is fixed)
+    """
+
+    name: str
+    ref: tuple
+    direction: str
+    width: float
+    length: float
+    gap_from_end: float = None
+    margin: float = None
+    layer: str = "top"
+    ground: object = None
+    z0_from: str = "voltage"
+
+    def __post_init__(self):
+        if self.z0_from not in ("voltage", "reciprocity"):
+            raise ValueError("z0_from must be 'voltage' or 'reciprocity'")
+        if self.ground is not None:
+            raise ValueError("coplanar feed grounds are not supported by this build")
+        if not isinstance(self.direction, str) or self.direction not in _AXIS_DIRS:
+            raise ValueError(f"direction must be one of {list(_AXIS_DIRS)} (axis-aligned feeds only)")
+        self.u = np.array(_AXIS_DIRS[self.direction], float)
+        self.n = np.array([-self.u[1], self.u[0]])
+        self.ref = (float(self.ref[0]), float(self.ref[1]))
+        if self.gap_from_end is None:
+            self.gap_from_end = self.width
+        self.aligned = True
+        self.axis = 0 if self.direction[1] == "x" else 1
+        self.sgn = _AXIS_DIRS[self.direction][self.axis]
+
+    @property
+    def s_gap(self):
+        return self.length - self.gap_from_end
+
+    def coord(self, s):
+        """Grid coordinate along the feed axis at outward distance s."""
+        return self.ref[self.axis] + self.sgn * s
+
+    def point(self, s, t=0.0):
+        """(x, y) at outward distance s along the feed and t across it."""
+        s, t = np.broadcast_arrays(np.asarray(s, float), np.asarray(t, float))
+        return np.asarray(self.ref) + s[..., None] * self.u + t[..., None] * self.n
+
+    def local(self, xy):
+        """Feed coordinates (s, t) of points (..., 2)."""
+        d = np.asarray(xy, float) - np.asarray(self.ref)
+        return d @ self.u, d @ self.n
+
+    def rect(self):
+        a = self.axis
+        c0, c1 = sorted((self.ref[a], self.coord(self.length)))
+        t = self.ref[1 - a]
+        lo, hi = t - 0.5 * self.width, t + 0.5 * self.width
+        return (c0, lo, c1, hi) if a == 0 else (lo, c0, hi, c1)
+
+
+@dataclass
+class Via:
+    """Plated via between two levels (Engine/mom25d/mesh.py:191-241).
+
+    xy : centre [m]; drill : hole diameter [m]; pad : pad diameter (default
+    2 drill), added on `top` and on `bottom` unless that is "gnd"."""
+
+    xy: tuple
+    drill: float
+    pad: float = None
+    name: str = None
+    top: str = "top"
+    bottom: str = "gnd"
+    connect: tuple = ()
+    antipad: float = None
+
+    def __post_init__(self):
+        self.xy = (float(self.xy[0]), float(self.xy[1]))
+        self.connect = tuple(self.connect)
+        if self.antipad is not None:
+            raise ValueError("antipads are not supported by this build")
+        if self.pad is None:
+            self.pad = 2 * self.drill
+        if self.pad < self.drill:
+            raise ValueError("via pad smaller than its drill")
+        if self.top == self.bottom:
+            raise ValueError("via top and bottom layer are the same")
+
+    def pad_layers(self):
+        return [n for n in (self.top, self.bottom) if n != "gnd"]
+
+
+@dataclass
+class Layout:
+    """Axis-aligned copper rectangles, feed ports and vias on named metal
+    layers (Engine/mom25d/mesh.py:309-693, rectangle / port / via subset)."""
+
+    rects: list = field(default_factory=list)
+    ports: list = field(default_factory=list)
+    vias: list = field(default_factory=list)
+    rect_layers: list = field(default_factory=list)
+    planes: list = field(default_factory=list)
+    lumped: list = field(default_factory=list)
+    internal_ports: list = field(default_factory=list)
+
+    def add_rect(self, x0, y0, x1, y1, layer="top"):
+        self.rects.append((min(x0, x1), min(y0, y1), max(x0, x1), max(y0, y1)))
+        self.rect_layers.append(layer)
+        return self
+
+    def add_port(self, name, ref, direction, width, length, **kw):
+        p = Port(name, tuple(ref), direction, width, length, **kw)
+        self.ports.append(p)
+        self.add_rect(*p.rect(), layer=p.layer)
+        return p
+
+    def add_via(self, x, y, drill=0.3e-3, pad=None, name=None, top="top", bottom="gnd", connect=(),
+                antipad=None):
+        v = Via((x, y), drill, pad, name, top, bottom, connect, antipad)
+        self.vias.append(v)
+        return v
+
+    def add_plane(self, name):
+        if name not in self.planes:
+            self.planes.append(name)
+        return self
+
+    def layer_names(self):
+        """Metal layers in order of first appearance (rectangles, then via pads)."""
+        names = list(self.rect_layers) + [n for v in self.vias for n in v.pad_layers()]
+        return [n for n in dict.fromkeys(names) if n not in self.planes]
+
+    def all_vias(self):
+        return list(self.vias)
+
+    def bbox(self):
+        boxes = [tuple(r) for r in self.rects]
+        for v in self.vias:
+            r = 0.5 * v.pad
+            boxes.append((v.xy[0] - r, v.xy[1] - r, v.xy[0] + r, v.xy[1] + r))
+        b = np.array(boxes)
+        return b[:, 0].min(), b[:, 1].min(), b[:, 2].max(), b[:, 3].max()
+
+    def contains(self, x, y, layer=None):
+        """Points strictly inside the copper of `layer` (None: any layer)."""
+        hit = np.zeros(np.broadcast(x, y).shape, bool)
+        for (x0, y0, x1, y1), n in zip(self.rects, self.rect_layers):
+            if layer is None or n == layer:
+                hit |= (x > x0) & (x < x1) & (y > y0) & (y < y1)
+        for v in self.vias:
+            if layer is None or layer in v.pad_layers():
+                hit |= (x - v.xy[0]) ** 2 + (y - v.xy[1]) ** 2 < (0.5 * v.pad) ** 2
+        return hit
+
+    def mesh(self, max_cell, edge_cell=None, snap=None, kind="grid"):
+        if kind != "grid":
+            raise ValueError("only kind='grid' meshes are supported by this build")
+        return Mesh.from_layout(self, max_cell, edge_cell)
+
+
+def _grid_lines(fixed, refine, lo, hi, max_cell, tol):
+    """Sorted unique grid coordinates (snapped to tol) subdivided so that no
+    interval exceeds max_cell (Engine/mom25d/mesh.py:696-703)."""
+    c = np.unique(np.round(np.concatenate([np.asarray(fixed, float), np.asarray(refine, float)]) / tol) * tol)
+    c = c[(c >= lo - tol) & (c <= hi + tol)]
+    lines = [c[0]]
+    for a, b in zip(c[:-1], c[1:]):
+        n = int(np.ceil((b - a) / max_cell - 1e-9))
+        lines.extend(np.linspace(a, b, n + 1)[1:])
+    return np.array(lines)
+
+
+class Mesh:
+    """Triangles of every metal layer on one shared tensor grid; each layer
+    has its own nodes, so RWG edges never join layers."""
+
+    def __init__(self, nodes, tris, layout=None, xg=None, yg=None, tri_layer=None, layer_names=("top",)):
+        self.nodes = np.asarray(nodes, float)
+        self.tris = np.asarray(tris, int)
+        self.layout = layout
+        self.xg, self.yg = xg, yg
+        self.layer_names = list(layer_names)
+        self.tri_layer = np.zeros(len(self.tris), int) if tri_layer is None else np.asarray(tri_layer, int)
+        self.twin_layer = -np.ones(len(self.layer_names), int)
+        self.n_frills = 0
+        self.n_ap = 0
+        self._orient()
+        self._edges()
+        self.edge_layer = self.tri_layer[self.tp]
+        self.planes = list(getattr(layout, "planes", []))
+        self._attach_vias(layout.all_vias() if layout is not None else [])
+
+    @classmethod
+    def from_layout(cls, layout, max_cell, edge_cell=None):
+        """Structured mesh of a layout (Engine/mom25d/mesh.py:728-781)."""
+        x0, y0, x1, y1 = layout.bbox()
+        tol = 1e-9 * max(x1 - x0, y1 - y0)
+        fx = [c for r in layout.rects for c in (r[0], r[2])]
+        fy = [c for r in layout.rects for c in (r[1], r[3])]
+        for p in layout.ports:
+            (fx if p.axis == 0 else fy).extend([p.coord(0), p.coord(p.s_gap)])
+        rx, ry = [], []
+        if edge_cell:
+            ax, ay = np.array(fx), np.array(fy)
+            rx = [c + s for c in fx for s in (-edge_cell, edge_cell)]
+            ry = [c + s for c in fy for s in (-edge_cell, edge_cell)]
+            rx = [c for c in rx if np.min(np.abs(ax - c)) > 0.5 * edge_cell]
+            ry = [c for c in ry if np.min(np.abs(ay - c)) > 0.5 * edge_cell]
+        xg = _grid_lines(fx, rx, x0, x1, max_cell, tol)
+        yg = _grid_lines(fy, ry, y0, y1, max_cell, tol)
+        nx, ny = len(xg), len(yg)
+        XC, YC = np.meshgrid(0.5 * (xg[1:] + xg[:-1]), 0.5 * (yg[1:] + yg[:-1]), indexing="ij")
+        names = layout.layer_names() or ["top"]
+        first, second, lay = [], [], []
+        for li, name in enumerate(names):
+            i, j = np.nonzero(layout.contains(XC, YC, name))
+            base = li * nx * ny
+            n00 = base + i * ny + j
+            n10 = n00 + ny
+            n11 = n10 + 1
+            n01 = n00 + 1
+            odd = ((i + j) % 2 == 1)[:, None]
+            first_l = np.where(odd, np.stack([n00, n10, n01], 1), np.stack([n00, n10, n11], 1))
+            second_l = np.where(odd, np.stack([n10, n11, n01], 1), np.stack([n00, n11, n01], 1))
+            first.append(first_l)
+            second.append(second_l)
+            lay.append(np.full(2 * i.size, li))
+        tris = np.concatenate([t for pair in zip(first, second) for t in pair])
+        used, inv = np.unique(tris, return_inverse=True)
+        X, Y = np.meshgrid(xg, yg, indexing="ij")
+        gx = np.tile(X.ravel(), len(names))[used]
+        gy = np.tile(Y.ravel(), len(names))[used]
+        return cls(np.stack([gx, gy], 1), inv.reshape(tris.shape), layout, xg, yg, np.concatenate(lay), names)
+
+    # ------------------------------------------------------------ geometry
+    def _orient(self):
+        """Counter-clockwise vertex order, areas and centroids."""
+        v = self.nodes[self.tris]
+        e1, e2 = v[:, 1] - v[:, 0], v[:, 2] - v[:, 0]
+        cross = e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0]
+        cw = cross < 0
+        self.tris[cw] = self.tris[cw][:, [0, 2, 1]]
+        self.verts = self.nodes[self.tris]
+        self.area = 0.5 * np.abs(cross)
+        self.centroid = self.verts.mean(1)
+
+    def _edges(self):
+        """RWG bases on shared edges (Engine/mom25d/mesh.py:850-898)."""
+        nt = len(self.tris)
+        opposite = [(1, 2), (2, 0), (0, 1)]
+        half = np.concatenate([np.sort(self.tris[:, list(e)], 1) for e in opposite])
+        owner = np.tile(np.arange(nt), 3)
+        local = np.repeat(np.arange(3), nt)
+        key = half[:, 0] * len(self.nodes) + half[:, 1]
+        order = np.argsort(key, kind="stable")
+        ks = key[order]
+        dup = ks[1:] == ks[:-1]
+        if np.any(dup[1:] & dup[:-1]):
+            raise ValueError("non-manifold edge (more than two triangles)")
+        a, b = order[:-1][dup], order[1:][dup]
+        self.edge_nodes = half[a]
+        self.tp, self.vp = owner[a], local[a]
+        self.tm, self.vm = owner[b], local[b]
+        d = self.nodes[self.edge_nodes[:, 1]] - self.nodes[self.edge_nodes[:, 0]]
+        self.edge_len = np.hypot(d[:, 0], d[:, 1])
+        self.edge_mid = 0.5 * (self.nodes[self.edge_nodes[:, 0]] + self.nodes[self.edge_nodes[:, 1]])
+        self.n_basis = len(self.tp)
+        self.wall = np.zeros(self.n_basis, bool)
+        self.n_walls = 0
+        self.tri_basis = -np.ones((nt, 3), int)
+        self.tri_sign = np.zeros((nt, 3))
+        idx = np.arange(self.n_basis)
+        self.tri_basis[self.tp, self.vp] = idx
+        self.tri_sign[self.tp, self.vp] = 1.0
+        self.tri_basis[self.tm, self.vm] = idx
+        self.tri_sign[self.tm, self.vm] = -1.0
+
+    # ---------------------------------------------------------------- vias
+    def _attach_vias(self, vias):
+        """Triangles inside each via hole per layer (Engine/mom25d/mesh.py:900-930)."""
+        self.vias = list(vias)
+        self.via_attach = []
+        for v in self.vias:
+            att = {}
+            for li, name in enumerate(self.layer_names):
+                t = self._hole_tris(v, li)
+                if t.size:
+                    att[name] = t
+            for n in v.pad_layers():
+                if n in self.layer_names and n not in att:
+                    raise ValueError(f"via at ({v.xy[0] * 1e3:.4g}, {v.xy[1] * 1e3:.4g}) mm: "
+                                     f"no meshed copper on layer {n}")
+            self.via_attach.append(att)
+        self.assign_levels(self.layer_names + self.planes)
+
+    def _hole_tris(self, v, li):
+        on = self.tri_layer == li
+        c = self.centroid
+        d2 = (c[:, 0] - v.xy[0]) ** 2 + (c[:, 1] - v.xy[1]) ** 2
+        t = np.nonzero(on & (d2 < (0.5 * v.drill) ** 2))[0]
+        if t.size == 0:
+            t = self._cell_tris(*v.xy)
+            t = t[on[t]]
+        return t
+
+    def _cell_tris(self, x, y):
+        """Triangles of the grid cell(s) holding (x, y); every touching cell
+        when the point lies on a grid line (Engine/mom25d/mesh.py:1011-1032)."""
+        def span(g, c):
+            tol = 1e-6 * (g[-1] - g[0])
+            k = np.nonzero(np.abs(g - c) < tol)[0]
+            if k.size:
+                return g[max(k[0] - 1, 0)], g[min(k[0] + 1, len(g) - 1)]
+            i = np.searchsorted(g, c) - 1
+            return (g[i], g[i + 1]) if 0 <= i < len(g) - 1 else (c, c)
+
+        (xa, xb), (ya, yb) = span(self.xg, x), span(self.yg, y)
+        c = self.centroid
+        return np.nonzero((c[:, 0] > xa) & (c[:, 0] < xb) & (c[:, 1] > ya) & (c[:, 1] < yb))[0]
+
+    def assign_levels(self, order):
+        """Split vias into vertical segments for a stack order, bottom to top
+        (Engine/mom25d/mesh.py:932-976; antipad frills excluded).  Unknown
+        n_basis + k is the downward current of segment k."""
+        rank = {n: i for i, n in enumerate(order)}
+        rank["gnd"] = -1
+        self.via_segments = []
+        for k, (v, att) in enumerate(zip(self.vias, self.via_attach)):
+            for n in (v.top, v.bottom) + v.connect:
+                if n not in rank:
+                    raise ValueError(f"via layer {n!r} is not in the stack order {order}")
+            hi, lo = sorted((rank[v.top], rank[v.bottom]), reverse=True)
+            mid = [n for n in order if lo < rank[n] < hi and (n in att or n in v.connect)]
+            chain = [order[hi]] + sorted(mid, key=lambda n: -rank[n]) + [order[lo] if lo >= 0 else "gnd"]
+            for a, b in zip(chain[:-1], chain[1:]):
+                self.via_segments.append(dict(via=k, top=a, bottom=b, top_tris=att.get(a),
+                                              bottom_tris=att.get(b)))
+        self.level_order = list(order)
+        self.frills = []
+        self.n_vias = len(self.via_segments)
+        self.via_tris = [g["top_tris"] for g in self.via_segments]
+        self.n_unknowns = self.n_basis + self.n_vias
+
+    # --------------------------------------------------------------- ports
+    def flow_direction(self):
+        """Unit vector of positive RWG flow (T+ -> T-) across each edge."""
+        d = self.centroid[self.tm] - self.centroid[self.tp]
+        n = np.linalg.norm(d, axis=1)
+        return d / np.where(n > 0, n, 1.0)[:, None]
+
+    def cut_edges(self, port, s):
+        """RWG edges on the transverse cut at outward distance s, with signs
+        making positive current flow toward the circuit."""
+        es, _ = port.local(self.nodes[self.edge_nodes])
+        _, tm = port.local(self.edge_mid)
+        tol = 1e-6 * port.width
+        on = (np.abs(es[:, 0] - s) < tol) & (np.abs(es[:, 1] - s) < tol)
+        on &= np.abs(tm) < 0.5 * port.width
+        li = self.layer_names.index(port.layer)
+        on &= self.edge_layer == li
+        on &= ~self.wall
+        idx = np.nonzero(on)[0]
+        sign = np.sign(self.flow_direction()[idx] @ -port.u)
+        return idx, sign
+
+    def cut_positions(self, port):
+        g = self.xg if port.axis == 0 else self.yg
+        s = port.sgn * (g - port.ref[port.axis])
+        s = s[(s > -1e-12) & (s <= port.length + 1e-12)]
+        return np.sort(s)

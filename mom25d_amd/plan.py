@@ -1,0 +1,184 @@
+"""Frequency-independent fill plan of one mesh (host set-up, device arrays).
+
+Mirrors the set-up of Engine/mom25d/solver.py:294-426 (`_morton_order`,
+`AssemblyGeometry`): quadrature points and area-scaled weights, RWG
+coefficients cA = s l / (2A) and cV = s l / A, Morton-ordered test blocks of
+32 triangles per metal layer, and the near / mid / far tier of every
+(test block, source triangle) pair:
+
+    rel = min over the block of |c_p - c_s|_3D / max(size_p, size_s)
+    near: rel < 3,   mid: 3 <= rel < 10,   far: rel >= 10
+
+The tier decides which source rule enters Z, so it is computed here with the
+same floating-point expressions as the reference (the 3D distance is the
+planar norm, squared again and combined with dz) and never re-derived on the
+device.  The plan is then flattened into work items of <= 64 source
+triangles for the GPU fill, and uploaded once.
+"""
+
+import numpy as np
+import torch
+
+from .rules import TRI3_BARY, TRI3_W, TRI7_BARY, TRI7_W, quad_points
+
+BLOCK = 32
+CHUNK = 64
+NEAR, MID, FAR = 0, 1, 2
+
+
+def morton_order(xy):
+    """Stable argsort of the Z-order (Morton) code of 16-bit quantised points."""
+    span = max(np.ptp(xy, 0).max(), 1e-30)
+    q = ((xy - xy.min(0)) / span * 65535).astype(np.uint64)
+
+    def spread(v):
+        for shift, mask in ((8, 0x00FF00FF), (4, 0x0F0F0F0F), (2, 0x33333333), (1, 0x55555555)):
+            v = (v | (v << np.uint64(shift))) & np.uint64(mask)
+        return v
+
+    return np.argsort(spread(q[:, 0]) | (spread(q[:, 1]) << np.uint64(1)), kind="stable")
+
+
+class FillPlan:
+    """Host arrays of the plan; `upload()` makes the device copies."""
+
+    def __init__(self, mesh, zl=None, near_factor=3.0, far_factor=10.0, block=BLOCK):
+        V, A = mesh.verts, mesh.area
+        nt = len(A)
+        self.mesh = mesh
+        self.nt, self.nb = nt, mesh.n_basis
+        self.V = V
+        self.area = A
+        self.cen = mesh.centroid
+        self.qp7 = quad_points(V, TRI7_BARY)
+        self.qw7 = TRI7_W[None, :] * A[:, None]
+        self.qp3 = quad_points(V, TRI3_BARY)
+        self.qw3 = TRI3_W[None, :] * A[:, None]
+        size = np.max(np.linalg.norm(V - np.roll(V, 1, axis=1), axis=-1), axis=1)
+        self.size = size
+        tl = mesh.tri_layer
+        nl = len(mesh.layer_names)
+        self.zl = np.zeros(nl) if zl is None else np.asarray(zl, float)
+        b = mesh.tri_basis
+        self.tb = b
+        lab = np.where(b >= 0, mesh.edge_len[np.maximum(b, 0)], 0.0)
+        self.cA = mesh.tri_sign * lab / (2 * A[:, None])
+        self.cV = mesh.tri_sign * lab / A[:, None]
+
+        blocks, blk_layer = [], []
+        items = []     # (block, pair index, src offset, count) per tier
+        src_lists = []
+        per_tier = {NEAR: [], MID: [], FAR: []}
+        n_src = 0
+        self.tier_pairs = np.zeros(3, np.int64)  # triangle pairs per tier
+        cen = self.cen
+        for lt in range(nl):
+            on_t = np.nonzero(tl == lt)[0]
+            if not on_t.size:
+                continue
+            order = on_t[morton_order(cen[on_t])]
+            for p0 in range(0, order.size, block):
+                pb = order[p0:p0 + block]
+                bi = len(blocks)
+                blocks.append(pb)
+                blk_layer.append(lt)
+                for ls in range(nl):
+                    src = np.nonzero(tl == ls)[0]
+                    if not src.size:
+                        continue
+                    dz = abs(self.zl[lt] - self.zl[ls])
+                    dist = np.linalg.norm(cen[pb][:, None, :] - cen[None, src, :], axis=-1)
+                    dist = np.sqrt(dist ** 2 + dz ** 2)
+                    rel = np.min(dist / np.maximum(size[pb][:, None], size[None, src]), axis=0)
+                    tiers = (src[rel < near_factor], src[(rel >= near_factor) & (rel < far_factor)],
+                             src[rel >= far_factor])
+                    for tier, sel in zip((NEAR, MID, FAR), tiers):
+                        if not sel.size:
+                            continue
+                        self.tier_pairs[tier] += sel.size * pb.size
+                        src_lists.append(sel.astype(np.int32))
+                        for c0 in range(0, sel.size, CHUNK):
+                            cnt = min(CHUNK, sel.size - c0)
+                            per_tier[tier].append((bi, lt * nl + ls, n_src + c0, cnt))
+                        n_src += sel.size
+        self.n_blocks = len(blocks)
+        bt = -np.ones((len(blocks), block), np.int32)
+        for i, pb in enumerate(blocks):
+            bt[i, :pb.size] = pb
+        self.blk_tris = bt
+        self.blk_layer = np.array(blk_layer, np.int32)
+        self.blocks = blocks
+        self.src = np.concatenate(src_lists) if src_lists else np.zeros(0, np.int32)
+        self.items = {t: np.array(v, np.int32).reshape(-1, 4) for t, v in per_tier.items()}
+        self.n_levels = nl
+
+        # conductor-loss Gram matrix per triangle, times cA_i cA_j
+        X = self.qp7[:, None, :, :] - V[:, :, None, :]
+        G = np.einsum("tk,tikd,tjkd->tij", self.qw7, X, X)
+        self.gram = (self.cA[:, :, None] * self.cA[:, None, :] * G).reshape(nt, 9)
+
+        # via segments: pad triangles and the drain matrix D (pad x segment)
+        segs = getattr(mesh, "via_segments", [])
+        self.nv = nv = len(segs)
+        self.pad_pos = -np.ones(nt, np.int32)
+        if nv:
+            self.vias = [mesh.vias[g["via"]] for g in segs]
+            tris = [g[e] for g in segs for e in ("top_tris", "bottom_tris") if g[e] is not None]
+            pad = np.unique(np.concatenate(tris)) if tris else np.zeros(0, int)
+            self.pad_tris = pad
+            self.pad_pos[pad] = np.arange(pad.size)
+            D = np.zeros((pad.size, nv))
+            self.via_ends = []
+            for k, g in enumerate(segs):
+                ends = []
+                t = g["top_tris"]
+                if t is not None:
+                    D[self.pad_pos[t], k] += 1 / A[t].sum()
+                    ends.append((tl[t[0]], 1.0))
+                t = g["bottom_tris"]
+                if t is not None:
+                    D[self.pad_pos[t], k] -= 1 / A[t].sum()
+                    ends.append((tl[t[0]], -1.0))
+                self.via_ends.append(ends)
+            self.Dvia = D
+            # CSR by segment
+            off, rows, vals = [0], [], []
+            for k in range(nv):
+                nz = np.nonzero(D[:, k])[0]
+                rows.extend(nz.tolist())
+                vals.extend(D[nz, k].tolist())
+                off.append(len(rows))
+            self.d_off = np.array(off, np.int32)
+            self.d_pad = np.array(rows, np.int32)
+            self.d_val = np.array(vals, float)
+        else:
+            self.pad_tris = np.zeros(0, int)
+        self.dev = None
+
+    def upload(self, device="cuda"):
+        """Device copies of every array the kernels read (kept on the plan)."""
+        if self.dev is not None:
+            return self.dev
+        dev = torch.device(device)
+        f64 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(dev)  # noqa: E731
+        i32 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.int32)).to(dev)  # noqa: E731
+        m = self.mesh
+        d = dict(
+            V=f64(self.V), qp7=f64(self.qp7), qw7=f64(self.qw7), qp3=f64(self.qp3), qw3=f64(self.qw3),
+            cen=f64(self.cen), area=f64(self.area), cA=f64(self.cA), cV=f64(self.cV), tb=i32(self.tb),
+            pad_pos=i32(self.pad_pos), blk_tris=i32(self.blk_tris), src=i32(self.src if self.src.size else [0]),
+            gram=f64(self.gram), tp=i32(m.tp), vp=i32(m.vp), tm=i32(m.tm), vm=i32(m.vm),
+        )
+        for t, it in self.items.items():
+            d[f"items{t}"] = i32(it if it.size else np.zeros((1, 4)))
+        if self.nv:
+            d.update(d_off=i32(self.d_off), d_pad=i32(self.d_pad), d_val=f64(self.d_val),
+                     pad_tri=i32(self.pad_tris))
+        self.dev = d
+        return d
+
+    def algorithmic_flops(self):
+        """Algorithmic FP64 flops of one fill, counted per triangle pair from
+        the loop body (sqrt and divide as one flop): near ~3.5k (7-point
+        rule plus closed-form integrals), mid ~2.27k, far ~1.41k."""
+        return float(3500 * self.tier_pairs[NEAR] + 2270 * self.tier_pairs[MID] + 1410 * self.tier_pairs[FAR])

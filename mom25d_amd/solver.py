@@ -1,0 +1,699 @@
+"""MPIE / RWG method-of-moments solver on one MI355X: the reference's Python
+API (Engine/mom25d/solver.py:769-1335) over the gfx950 kernels of libmom25d.
+
+Per frequency of a sweep:
+  tables (all frequencies at once, greens.SweepTables)
+  -> FP64 fill of Z on the GPU (tiers near / mid / far, atomics), Gram loss
+     term, via rows / columns, symmetrisation Z <- (Z + Z^T) / 2
+  -> LU with partial pivoting on the GPU: complex128 ("fp64"), or complex64
+     on the matrix cores with FP64 iterative refinement against the FP64 Z
+     ("mixed")
+  -> triangle charges, line voltages at the port sample points (GPU)
+  -> shared-gamma wave fit per port group and S (host, FP64).
+"""
+
+import ctypes
+import time
+import warnings
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+from scipy.optimize import minimize, minimize_scalar
+
+from . import _lib, analytic
+from .constants import C0, EPS0, MU0
+from .greens import SweepTables
+from .plan import FillPlan
+
+
+@dataclass
+class Substrate:
+    """Grounded dielectric slab with conductors on top (Engine/mom25d/solver.py:42-76)."""
+
+    eps_r: float
+    h: float
+    tand: float = 0.0
+    sigma: float = None
+    t: float = 35e-6
+    ground_loss: bool = True
+
+    @property
+    def lossless(self):
+        return not self.tand and not self.sigma
+
+    def surface_impedance(self, freq):
+        if not self.sigma:
+            return 0.0
+        w = 2 * np.pi * freq
+        kc = np.sqrt(1j * w * MU0 * self.sigma)
+        return kc / self.sigma / np.tanh(kc * self.t)
+
+    def to_stackup(self):
+        from .layered import Stackup
+
+        return Stackup.microstrip(self.eps_r, self.h, self.tand, self.sigma, self.t, self.ground_loss)
+
+
+# --------------------------------------------------------------------- vias
+def filament_mutual(l, d):
+    """Partial mutual inductance of two parallel aligned filaments of length l
+    at distance d (d = radius: partial self inductance of a tube)."""
+    return MU0 * l / (2 * np.pi) * (np.arcsinh(l / d) - np.sqrt(1 + (d / l) ** 2) + d / l)
+
+
+def filament_mutual_z(a1, a2, b1, b2, d):
+    """Partial mutual inductance (Grover) of parallel z-filaments spanning
+    [a1, a2] and [b1, b2] at lateral distance d."""
+    def f(s):
+        return s * np.arcsinh(s / d) - np.sqrt(s * s + d * d)
+
+    return MU0 / (4 * np.pi) * (f(a2 - b1) - f(a2 - b2) - f(a1 - b1) + f(a1 - b2))
+
+
+def via_inductance(vias, h, spans=None):
+    """Quasi-static partial inductance matrix [H] of vertical via currents
+    with their ground-plane images (Engine/mom25d/solver.py:93-122, k = 0)."""
+    xy = np.array([v.xy for v in vias], float).reshape(-1, 2)
+    d = np.linalg.norm(xy[:, None] - xy[None], axis=-1)
+    np.fill_diagonal(d, [0.5 * v.drill for v in vias])
+    if spans is not None:
+        z = np.asarray(spans, float).reshape(-1, 2)
+        a1, a2 = z[:, None, 0], z[:, None, 1]
+        b1, b2 = z[None, :, 0], z[None, :, 1]
+        return filament_mutual_z(a1, a2, b1, b2, d) + filament_mutual_z(a1, a2, -b2, -b1, d)
+    return 0.5 * filament_mutual(2 * h, d)
+
+
+def via_impedance(vias, spans, omega, sections):
+    """Vector-potential impedance [ohm] of the via segments per stack section
+    (Engine/mom25d/solver.py:125-199): open sections by quasi-static partial
+    inductances of piece + image, PEC-closed sections by parallel-plate TEM
+    and TM_n modes."""
+    from scipy.special import hankel2, kv
+
+    xy = np.array([v.xy for v in vias], float).reshape(-1, 2)
+    rho = np.linalg.norm(xy[:, None] - xy[None], axis=-1)
+    rad = np.array([0.5 * v.drill for v in vias])
+    same = rho < 1e-12
+    rho = np.where(same, np.minimum(rad[:, None], rad[None, :]), rho)
+    np.fill_diagonal(rho, rad)
+    spans = np.asarray(spans, float).reshape(-1, 2)
+    nv = len(spans)
+    Z = np.zeros((nv, nv), complex)
+    for zb, zt, k in sections:
+        a1 = np.clip(spans[:, 0], zb, zt) - zb
+        a2 = np.clip(spans[:, 1], zb, zt) - zb
+        on = np.nonzero(a2 - a1 > 1e-12 * (zt - zb))[0]
+        if not on.size:
+            continue
+        a1, a2, r = a1[on], a2[on], rho[np.ix_(on, on)]
+        d = zt - zb
+        if k is None:
+            p1, p2 = a1[:, None], a2[:, None]
+            q1, q2 = a1[None, :], a2[None, :]
+            L = filament_mutual_z(p1, p2, q1, q2, r) + filament_mutual_z(p1, p2, -q2, -q1, r)
+            Zs = 1j * omega * L
+        else:
+            la = a2 - a1
+            Zs = omega * MU0 / (4 * d) * np.outer(la, la) * hankel2(0, k * r)
+            nmax = int(np.ceil(40 * d / (np.pi * r.min()))) + 1
+            kn = np.arange(1, nmax + 1) * np.pi / d
+            c = (np.sin(kn * a2[:, None]) - np.sin(kn * a1[:, None])) / kn
+            if np.abs(c).max() > 1e-12 * d:
+                qn = np.sqrt(kn ** 2 - k ** 2 + 0j)
+                K = kv(0, np.multiply.outer(r, qn))
+                Zs = Zs + 1j * omega * MU0 / (np.pi * d) * np.einsum("ijn,in,jn->ij", K, c, c)
+        Z[np.ix_(on, on)] += Zs
+    return Z
+
+
+# -------------------------------------------------------------------- ports
+class PortModel:
+    """Delta-gap excitation and current / voltage sample cuts of one port
+    (Engine/mom25d/solver.py:570-610)."""
+
+    def __init__(self, mesh, port, h):
+        self.port = port
+        margin = port.margin if port.margin is not None else max(1.5 * port.width, 4 * h)
+        s = mesh.cut_positions(port)
+        s = s[(s >= margin - 1e-12) & (s <= port.s_gap - margin + 1e-12)]
+        if s.size < 4:
+            raise ValueError(f"port {port.name}: feed line too short or mesh too coarse "
+                             f"({s.size} sample cuts in the fitting window "
+                             f"[{margin * 1e3:.3g}, {(port.s_gap - margin) * 1e3:.3g}] mm)")
+        self.s = s
+        self.cuts = [mesh.cut_edges(port, si) for si in s]
+        idx, sgn = mesh.cut_edges(port, port.s_gap)
+        if idx.size == 0:
+            raise ValueError(f"port {port.name}: no mesh edges on the gap cut")
+        self.excitation = np.zeros(mesh.n_unknowns)
+        self.excitation[idx] = sgn * mesh.edge_len[idx]
+        self.len = mesh.edge_len
+        allcuts = mesh.cut_positions(port)
+        k = np.searchsorted(allcuts, s)
+        lo = s - allcuts[np.maximum(k - 1, 0)]
+        hi = allcuts[np.minimum(k + 1, allcuts.size - 1)] - s
+        d = 0.5 * np.minimum(np.where(lo > 0, lo, np.inf), np.where(hi > 0, hi, np.inf))
+        frac = np.array([-0.3, -0.1, 0.1, 0.3])
+        along = np.stack([s - d, s + d], 1)
+        self.vpts = port.point(along[:, :, None], frac * port.width).reshape(-1, 2)
+        self.nper = 2 * frac.size
+
+    def currents(self, x):
+        """Total current toward the circuit at each sample cut, (Ns, Nexc)."""
+        return np.array([(sg * self.len[i]) @ x[i] for i, sg in self.cuts])
+
+    def voltages(self, phi):
+        return phi.reshape(self.s.size, self.nper, -1).mean(1)
+
+
+VOLTAGE_WEIGHT = 1.0
+
+
+def fit_waves(s_list, I_list, V_list, gamma0, lossless=False, v_weight=VOLTAGE_WEIGHT, bounds=(0.7, 1.3)):
+    """Shared-gamma / Z0 fit of I(s) = A e^{gs} + B e^{-gs}, V = Z0 (A e^{gs} - B e^{-gs})
+    over all data sets (Engine/mom25d/solver.py:618-670): bounded scalar search
+    of beta for lossless stacks, Nelder-Mead on (alpha, beta) otherwise.
+    Returns gamma, Z0, [(A, B), ...], relative residual."""
+    scale = abs(gamma0.imag)
+    Inrm = sum(np.sum(np.abs(I) ** 2) for I in I_list)
+    Vnrm = sum(np.sum(np.abs(V) ** 2) for V in V_list)
+
+    def solve(g):
+        coefs, res, U_all, V_all = [], 0.0, [], []
+        for s, I, V in zip(s_list, I_list, V_list):
+            ep, em = np.exp(g * s), np.exp(-g * s)
+            M = np.stack([ep, em], 1)
+            c, *_ = np.linalg.lstsq(M, I, rcond=None)
+            coefs.append(c)
+            res += np.sum(np.abs(M @ c - I) ** 2) / Inrm
+            U_all.append(c[0] * ep - c[1] * em)
+            V_all.append(V)
+        U, V = np.concatenate(U_all), np.concatenate(V_all)
+        z0 = np.vdot(U, V) / np.vdot(U, U)
+        res += v_weight * np.sum(np.abs(V - z0 * U) ** 2) / Vnrm
+        return coefs, z0, res
+
+    if lossless:
+        lo, hi = bounds
+        opt = minimize_scalar(lambda b: solve(1j * scale * b)[2], bounds=(lo, hi), method="bounded",
+                              options=dict(xatol=1e-12))
+        if min(opt.x - lo, hi - opt.x) < 1e-3:
+            warnings.warn(f"beta fit hit its search bound ({opt.x:.4f} x the Kirschning-Jansen estimate)",
+                          RuntimeWarning, stacklevel=2)
+        g = 1j * scale * opt.x
+    else:
+        p0 = np.array([gamma0.real, gamma0.imag]) / scale
+        opt = minimize(lambda p: solve(scale * (p[0] + 1j * p[1]))[2], p0, method="Nelder-Mead",
+                       options=dict(xatol=1e-10, fatol=1e-16, maxiter=4000))
+        g = scale * (opt.x[0] + 1j * opt.x[1])
+    coefs, z0, res = solve(g)
+    return g, z0, coefs, np.sqrt(res)
+
+
+@dataclass
+class SweepResult:
+    """Result of a sweep (Engine/mom25d/solver.py:673-765)."""
+
+    freqs: np.ndarray
+    S: np.ndarray
+    z0: np.ndarray
+    gamma: np.ndarray
+    z0_line: np.ndarray
+    fit_residual: np.ndarray
+    port_names: list
+    currents: list = None
+    waves: np.ndarray = None
+    timing: dict = None
+
+    def matched_currents(self, fi, port):
+        if self.currents is None:
+            raise ValueError("run sweep(..., keep_currents=True)")
+        e = np.zeros(self.waves.shape[1])
+        e[port] = 1.0
+        return self.currents[fi] @ np.linalg.solve(self.waves[fi], e)
+
+    @property
+    def eeff(self):
+        k0 = 2 * np.pi * self.freqs / C0
+        return (self.gamma.imag / k0[:, None]) ** 2
+
+    def renormalize(self, zref=50.0):
+        """S referenced to a common real impedance zref."""
+        out = np.empty_like(self.S)
+        eye = np.eye(self.S.shape[1])
+        for f, S in enumerate(self.S):
+            sq = np.diag(np.sqrt(self.z0[f]))
+            Zm = sq @ (eye + S) @ np.linalg.inv(eye - S) @ sq
+            out[f] = (Zm - zref * eye) @ np.linalg.inv(Zm + zref * eye)
+        return out
+
+    def db(self, i, j, S=None):
+        S = self.S if S is None else S
+        return 20 * np.log10(np.abs(S[:, i, j]))
+
+    def write_touchstone(self, path, zref=50.0):
+        S = self.S if zref is None else self.renormalize(zref)
+        z = zref if zref is not None else float(np.mean(self.z0))
+        n = S.shape[1]
+        with open(path, "w") as fh:
+            fh.write(f"! mom25d-mi355x 2.5D MoM. ports: {', '.join(self.port_names)}\n")
+            fh.write(f"# HZ S RI R {z:g}\n")
+            for f, Sf in zip(self.freqs, S):
+                vals = Sf.T.ravel() if n == 2 else Sf.ravel()
+                fh.write(f"{f:.9e} " + " ".join(f"{v.real:.9e} {v.imag:.9e}" for v in vals) + "\n")
+
+
+_PTR_FIELDS = ("V", "qp7", "qw7", "qp3", "qw3", "cen", "area", "cA", "cV", "tb", "pad_pos", "blk_tris", "items",
+               "src")
+
+
+class _FillArgs(ctypes.Structure):
+    """Mirror of the FillArgs block of csrc/fill.hip (size checked at first use)."""
+
+    _fields_ = [(n, ctypes.c_void_p) for n in _PTR_FIELDS] + [
+        ("nt", ctypes.c_int32), ("pairs", ctypes.c_void_p), ("tab_x", ctypes.c_void_p), ("tab_c", ctypes.c_void_p),
+        ("tab_b", ctypes.c_void_p), ("jwmu_re", ctypes.c_double), ("jwmu_im", ctypes.c_double),
+        ("ijwe_re", ctypes.c_double), ("ijwe_im", ctypes.c_double), ("Z", ctypes.c_void_p), ("ldz", ctypes.c_int64),
+        ("IVpad", ctypes.c_void_p)]
+
+
+def _check_abi():
+    L = _lib.lib()
+    if L.mom_fill_args_size() != ctypes.sizeof(_FillArgs):
+        raise RuntimeError("libmom25d FillArgs layout differs from the Python mirror")
+    from .greens import PAIR_DTYPE
+
+    if L.mom_pair_tab_size() != PAIR_DTYPE.itemsize:
+        raise RuntimeError("libmom25d PairTab layout differs from the Python mirror")
+
+
+class Solver:
+    """Solver(mesh, Substrate | Stackup).sweep(freqs) -> SweepResult.
+
+    precision : "fp64" -- complex128 LU (the plain FP64 path);
+                "mixed" -- complex64 LU on the matrix cores with FP64
+                iterative refinement against the FP64 impedance matrix.
+    method    : "auto" / "dense" (the H-matrix solver is not part of this build).
+    """
+
+    OPEN = 1e12
+    HMATRIX_N = 7000
+
+    def __init__(self, mesh, substrate, z0=None, verbose=True, method="auto", v_weight=None, z0_recip_min=None,
+                 tand_floor=None, precision="fp64", device="cuda", panel=None, refine_tol=1e-12, refine_maxit=30):
+        from .layered import Stackup
+
+        if method not in ("auto", "dense"):
+            raise ValueError("method must be 'auto' or 'dense' (no H-matrix solver in this build)")
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.mesh = mesh
+        self.sub = substrate
+        self.stack = substrate if isinstance(substrate, Stackup) else substrate.to_stackup()
+        st = self.stack
+        missing = [n for n in mesh.layer_names if n not in st.metals]
+        if missing:
+            raise ValueError(f"mesh layers {missing} are not metal levels of the stackup {list(st.metals)}")
+        self.single = len(st.layers) == 1 and st.top == "air" and mesh.layer_names == ["top"]
+        self.zl = np.array([st.level_z(n) for n in mesh.layer_names])
+        order = sorted(mesh.layer_names, key=st.level_z)
+        if getattr(mesh, "level_order", None) != order:
+            mesh.assign_levels(order)
+        self.ports = mesh.layout.ports
+        if not self.ports:
+            raise ValueError("layout has no ports")
+        if getattr(mesh.layout, "lumped", None) or getattr(mesh.layout, "internal_ports", None):
+            raise ValueError("lumped components / internal ports are not supported by this build")
+        self.pm = [PortModel(mesh, p, st.level_h(p.layer)) for p in self.ports]
+        self.method = "dense"
+        self.precision = precision
+        self.z0_fixed = None if z0 is None else np.asarray(z0, float)
+        v = mesh.nodes
+        self.rho_max = 1.02 * np.hypot(*(v.max(0) - v.min(0)))
+        self.verbose = verbose
+        self.v_weight = VOLTAGE_WEIGHT if v_weight is None else float(v_weight)
+        self.z0_recip_min = z0_recip_min
+        self.device = torch.device(device)
+        self.panel = panel or (64 if precision == "fp64" else 64)
+        self.refine_tol = refine_tol
+        self.refine_maxit = refine_maxit
+        self._plan = None
+        self._bufs = {}
+        _check_abi()
+        self.groups = {}
+        for p, port in enumerate(self.ports):
+            hp = st.level_h(port.layer)
+            key = (round(port.width / hp, 9), port.layer, None, port.z0_from)
+            self.groups.setdefault(key, []).append(p)
+
+    @property
+    def lossless(self):
+        return self.stack.lossless
+
+    def surface_impedance(self, freq):
+        return self.stack.surface_impedance(freq)
+
+    # ------------------------------------------------------------- set-up
+    def geometry(self):
+        if self._plan is None:
+            self._plan = FillPlan(self.mesh, zl=self.zl)
+        return self._plan
+
+    def tables(self, freqs):
+        """Green's-function tables of all frequencies (one GPU launch)."""
+        return SweepTables(self.stack, freqs, self.rho_max, self.mesh.layer_names, self.single, self.device)
+
+    def greens(self, freq):
+        return self.tables([freq])[0]
+
+    def _buf(self, name, shape, dtype):
+        b = self._bufs.get(name)
+        if b is None or tuple(b.shape) != tuple(shape) or b.dtype != dtype:
+            self._bufs.pop(name, None)
+            b = torch.empty(shape, dtype=dtype, device=self.device)
+            self._bufs[name] = b
+        return b
+
+    # --------------------------------------------------------------- fill
+    def _via_extra(self, g, zs_via):
+        """Host part of the via-via block: -Im gV(0) correction, the
+        vector-potential impedance of the segments, the via-wall loss."""
+        plan = self.geometry()
+        omega = g.omega
+        inv_jwe = 1 / (1j * omega * EPS0)
+        nv = plan.nv
+        E = np.zeros((nv, nv), complex)
+        im0 = {}
+        for k, ek in enumerate(plan.via_ends):
+            for m, em in enumerate(plan.via_ends):
+                acc = 0.0
+                for la, sa in ek:
+                    for lb, sb in em:
+                        if (la, lb) not in im0:
+                            pr = g.pair(la, lb)
+                            rho = 0.0 if pr.dz else 1e-6 * g.h
+                            im0[la, lb] = pr.gV(np.array([rho]))[0].imag
+                        acc += sa * sb * im0[la, lb]
+                E[k, m] -= inv_jwe * 1j * acc
+        st = self.stack
+        segs = self.mesh.via_segments
+        vz = {n: st.level_z(n) for n in self.mesh.layer_names}
+        vz["gnd"] = 0.0
+        spans = [(vz[s["bottom"]], vz[s["top"]]) for s in segs]
+        E += via_impedance(plan.vias, spans, omega, g.via_sections())
+        if zs_via:
+            lengths = np.array([z1 - z0 for z0, z1 in spans])
+            E += np.diag([zs_via * l / (np.pi * v.drill) for l, v in zip(lengths, plan.vias)])
+        return E
+
+    def _fill(self, g, Z, Z32=None):
+        """Z (device complex128, zeroed here) <- the symmetric impedance matrix
+        at g.freq; Z32 (complex64) receives a copy when given."""
+        plan = self.geometry()
+        d = plan.upload(self.device)
+        n = self.mesh.n_unknowns
+        omega = g.omega
+        jwmu = 1j * omega * MU0
+        ijwe = 1 / (1j * omega * EPS0)
+        Z.zero_()
+        nv = plan.nv
+        ivpad = None
+        if nv:
+            ivpad = self._buf("ivpad", (plan.pad_tris.size, plan.nt), torch.complex128)
+            ivpad.zero_()
+        P = _lib.ptr
+        tabs = g.tables
+        a = _FillArgs()
+        for name in ("V", "qp7", "qw7", "qp3", "qw3", "cen", "area", "cA", "cV", "tb", "pad_pos", "blk_tris",
+                     "src"):
+            setattr(a, name, d[name].data_ptr())
+        a.nt = plan.nt
+        a.pairs = g.pairs_dev.data_ptr()
+        a.tab_x, a.tab_c, a.tab_b = tabs.tab_x.data_ptr(), tabs.tab_c.data_ptr(), tabs.tab_b.data_ptr()
+        a.jwmu_re, a.jwmu_im = jwmu.real, jwmu.imag
+        a.ijwe_re, a.ijwe_im = ijwe.real, ijwe.imag
+        a.Z = Z.data_ptr()
+        a.ldz = Z.stride(0)
+        a.IVpad = ivpad.data_ptr() if ivpad is not None else None
+        stream = _lib.stream_ptr()
+        for tier in (0, 1, 2):
+            items = plan.items[tier]
+            if not items.size:
+                continue
+            a.items = d[f"items{tier}"].data_ptr()
+            _lib.call("mom_fill_tier", tier, int(items.shape[0]), ctypes.byref(a), stream)
+        zs = self.stack.surface_impedance(g.freq)
+        if zs:
+            _lib.call("mom_gram_add", plan.nt, P(d["gram"]), P(d["tb"]), float(np.real(zs)), float(np.imag(zs)),
+                      P(Z), Z.stride(0), stream)
+        if nv:
+            E = torch.from_numpy(self._via_extra(g, zs)).to(self.device)
+            _lib.call("mom_via_blocks", plan.nb, nv, plan.nt, P(ivpad), P(d["d_off"]), P(d["d_pad"]), P(d["d_val"]),
+                      P(d["pad_tri"]), P(d["tp"]), P(d["vp"]), P(d["tm"]), P(d["vm"]), P(d["cV"]), P(E),
+                      ijwe.real, ijwe.imag, P(Z), Z.stride(0), stream)
+            self._keep = E
+        _lib.call("mom_symmetrize", n, P(Z), Z.stride(0), P(Z32), Z32.stride(0) if Z32 is not None else 0, stream)
+        return Z
+
+    def impedance(self, freq, greens=None):
+        """The impedance matrix at freq (numpy complex128)."""
+        g = greens or self.greens(freq)
+        n = self.mesh.n_unknowns
+        Z = self._buf("Z", (n, n), torch.complex128)
+        self._fill(g, Z)
+        return Z.cpu().numpy()
+
+    # -------------------------------------------------------------- solve
+    def port_excitations(self):
+        return np.stack([pm.excitation for pm in self.pm], 1)
+
+    def _lu_solve(self, Z, B, Z32=None):
+        """X = Z^-1 B on the device; Z (complex128) is overwritten by its LU
+        in the fp64 path; in the mixed path Z32 holds the complex64 LU and Z
+        stays intact for the refinement residuals."""
+        n, nrhs = B.shape
+        stream = _lib.stream_ptr()
+        P = _lib.ptr
+        jb = self.panel
+        prec = 64 if Z32 is None else 32
+        A = Z if Z32 is None else Z32
+        ipiv = self._buf("ipiv", (n,), torch.int32)
+        perm = self._buf("perm", (n,), torch.int32)
+        info = self._buf("info", (4,), torch.int32)
+        wbytes = int(_lib.lib().mom_getrf_work_bytes(prec, n, jb))
+        work = self._buf(f"work{prec}", (wbytes,), torch.uint8)
+        _lib.call("mom_getrf", prec, n, P(A), A.stride(0), P(ipiv), P(info), P(work), wbytes, jb, stream)
+        _lib.call("mom_build_perm", n, P(ipiv), P(perm), stream)
+        sw = self._buf("swork", (int(_lib.lib().mom_getrs_work_bytes(n)),), torch.uint8)
+        X = torch.empty((n, nrhs), dtype=torch.complex128, device=self.device)
+        if Z32 is None:
+            _lib.call("mom_getrs", 64, n, nrhs, P(A), A.stride(0), P(perm), P(B), P(X), P(sw), stream)
+            self.last_refine = 0
+            return X, info
+        B32 = B.to(torch.complex64)
+        X32 = torch.empty_like(B32)
+        _lib.call("mom_getrs", 32, n, nrhs, P(A), A.stride(0), P(perm), P(B32), P(X32), P(sw), stream)
+        X.copy_(X32)
+        Rz = torch.empty_like(X)
+        nrm = torch.zeros(2 * nrhs, dtype=torch.int64, device=self.device)
+        it = 0
+        for it in range(1, self.refine_maxit + 1):
+            _lib.call("mom_residual", n, nrhs, P(Z), Z.stride(0), P(X), P(B), P(Rz), stream)
+            R32 = Rz.to(torch.complex64)
+            _lib.call("mom_getrs", 32, n, nrhs, P(A), A.stride(0), P(perm), P(R32), P(X32), P(sw), stream)
+            nrm.zero_()
+            _lib.call("mom_refine_update", n, nrhs, P(X32), P(X), P(nrm), P(nrm[nrhs:]), stream)
+            v = nrm.cpu().view(torch.float64).numpy()
+            if np.all(v[:nrhs] <= self.refine_tol * v[nrhs:]):
+                break
+        self.last_refine = it
+        return X, info
+
+    def solve_currents(self, freq, excitations=None, greens=None):
+        g = greens or self.greens(freq)
+        if excitations is None:
+            excitations = self.port_excitations()
+        return self._solve_dev(g, excitations).cpu().numpy()
+
+    def _solve_dev(self, g, excitations):
+        n = self.mesh.n_unknowns
+        Z = self._buf("Z", (n, n), torch.complex128)
+        Z32 = self._buf("Z32", (n, n), torch.complex64) if self.precision == "mixed" else None
+        self._fill(g, Z, Z32)
+        B = torch.from_numpy(np.ascontiguousarray(excitations, complex)).to(self.device)
+        X, info = self._lu_solve(Z, B, Z32)
+        return X
+
+    # ----------------------------------------------------------- de-embed
+    def charges(self, x, omega):
+        """Surface charge per triangle (Nt, Nexc) (Engine/mom25d/solver.py:1076-1091)."""
+        m = self.mesh
+        b = m.tri_basis
+        lab = np.where(b >= 0, m.edge_len[np.maximum(b, 0)], 0.0)
+        cV = m.tri_sign * lab / m.area[:, None]
+        div = np.einsum("ti,tie->te", cV, np.where((b >= 0)[..., None], x[np.maximum(b, 0)], 0))
+        for k, g in enumerate(m.via_segments):
+            t = g["top_tris"]
+            if t is not None:
+                div[t] += x[m.n_basis + k] / m.area[t].sum()
+            t = g["bottom_tris"]
+            if t is not None:
+                div[t] -= x[m.n_basis + k] / m.area[t].sum()
+        return -div / (1j * omega)
+
+    def line_voltage(self, greens, q, pts, layer="top"):
+        """Voltage to the ground plane at points of a metal layer from the
+        triangle charges q (Nt, Nexc) (GPU kernel)."""
+        m = self.mesh
+        plan = self.geometry()
+        d = plan.upload(self.device)
+        li = m.layer_names.index(layer)
+        tabs = greens.tables
+        pts = np.ascontiguousarray(pts, float)
+        qd = torch.from_numpy(np.ascontiguousarray(q, complex)).to(self.device)
+        pd = torch.from_numpy(pts).to(self.device)
+        tl = self._buf("tri_layer", (plan.nt,), torch.int32)
+        tl.copy_(torch.from_numpy(m.tri_layer.astype(np.int32)))
+        out = torch.empty((pts.shape[0], q.shape[1]), dtype=torch.complex128, device=self.device)
+        P = _lib.ptr
+        _lib.call("mom_line_voltage", pts.shape[0], P(pd), li, len(m.layer_names), plan.nt, q.shape[1], P(tl),
+                  P(d["V"]), P(d["qp7"]), P(d["qw7"]), P(greens.pairs_dev), P(tabs.tab_x), P(tabs.tab_cw),
+                  P(tabs.tab_b), P(qd), 1.0 / EPS0, P(out), _lib.stream_ptr())
+        return out.cpu().numpy()
+
+    def _sparams(self, f, g, x, S, gam, zl, zr, res, waves, fi):
+        npt = len(self.ports)
+        n = npt
+        q = self.charges(x, g.omega)
+        phi = [None] * npt
+        layers = list(dict.fromkeys(p.layer for p in self.ports))
+        for lay in layers:
+            ps = [p for p in range(npt) if self.ports[p].layer == lay]
+            vp = self.line_voltage(g, q, np.concatenate([self.pm[p].vpts for p in ps]), lay)
+            for p, v in zip(ps, np.split(vp, np.cumsum([len(self.pm[p].vpts) for p in ps])[:-1])):
+                phi[p] = v
+        Acoef = np.zeros((n, n), complex)
+        Bcoef = np.zeros((n, n), complex)
+        for members in self.groups.values():
+            port = self.ports[members[0]]
+            hp = self.stack.level_h(port.layer)
+            b0 = analytic.beta(port.width, hp, self.stack.eps_below(port.layer), f)
+            bnd = (0.7, 1.3) if not self.stack.covered(port.layer) else (0.7, 1.8)
+            s_list, I_list, V_list, keys = [], [], [], []
+            for p in members:
+                Ip = self.pm[p].currents(x)
+                Vp = self.pm[p].voltages(phi[p])
+                for e in range(n):
+                    s_list.append(self.pm[p].s)
+                    I_list.append(Ip[:, e])
+                    V_list.append(Vp[:, e])
+                    keys.append((p, e))
+            gm, z0, coefs, r = fit_waves(s_list, I_list, V_list, 1e-3 * b0 + 1j * b0, lossless=self.lossless,
+                                         bounds=bnd, v_weight=self.v_weight)
+            for (p, e), (a, bb) in zip(keys, coefs):
+                Acoef[p, e], Bcoef[p, e] = a, bb
+            for p in members:
+                gam[fi, p], zl[fi, p], res[fi, p] = gm, z0, r
+        waves[fi] = Acoef
+        SI = -Bcoef @ np.linalg.inv(Acoef)
+        if self.z0_fixed is None:
+            zr[fi] = self._reciprocal_z0(SI, zl[fi].real)
+        else:
+            zr[fi] = self.z0_fixed
+        dd = np.sqrt(zr[fi])
+        S[fi] = dd[:, None] * SI / dd[None, :]
+
+    def sweep(self, freqs, keep_currents=False, interp=None, values=None, checkpoint=None, wait_memory=None):
+        """S-parameters at freqs (every frequency filled and factored; the
+        reference's interpolated-Z mode is not part of this build, so
+        interp="auto" behaves like None)."""
+        if interp not in (None, "auto"):
+            raise ValueError("interp must be None or 'auto' (no frequency interpolation in this build)")
+        freqs = np.atleast_1d(np.asarray(freqs, float))
+        nf, n = len(freqs), len(self.ports)
+        S = np.zeros((nf, n, n), complex)
+        gam = np.zeros((nf, n), complex)
+        zl = np.zeros((nf, n), complex)
+        zr = np.zeros((nf, n))
+        res = np.zeros((nf, n))
+        waves = np.zeros((nf, n, n), complex)
+        cur = []
+        t0 = time.time()
+        tabs = self.tables(freqs)
+        self.geometry().upload(self.device)
+        exc = self.port_excitations()
+        t1 = time.time()
+        for fi, f in enumerate(freqs):
+            g = tabs[fi]
+            x = self._solve_dev(g, exc).cpu().numpy()
+            if keep_currents:
+                cur.append(x)
+            self._sparams(f, g, x, S, gam, zl, zr, res, waves, fi)
+            if self.verbose:
+                print(f"PROGRESS {fi + 1}/{nf}", flush=True)
+        t2 = time.time()
+        return SweepResult(freqs, S, zr, gam, zl, res, [p.name for p in self.ports], cur if keep_currents else None,
+                           waves, timing=dict(tables=t1 - t0, solve=t2 - t1))
+
+    # ---------------------------------------------------------- reference Z0
+    def _reciprocal_z0(self, SI, zphi, prior=1e-4):
+        """Port reference impedances from reciprocity ratios plus the
+        scalar-potential level (Engine/mom25d/solver.py:1357-1397)."""
+        groups = list(self.groups.values())
+        recip = np.array([self.ports[m[0]].z0_from == "reciprocity" for m in groups])
+        if recip.all():
+            raise ValueError("every port has z0_from='reciprocity': at least one needs 'voltage'")
+        cmin = self.z0_recip_min
+        if not recip.any():
+            return self._voltage_z0(SI, zphi, groups, prior, cmin)
+        gid = np.empty(len(self.ports), int)
+        for k, members in enumerate(groups):
+            gid[members] = k
+        vport = ~recip[gid]
+        z = np.array(zphi, float)
+        vidx = np.nonzero(vport)[0]
+        vgroups = [np.nonzero(np.isin(vidx, g))[0] for g, r in zip(groups, recip) if not r]
+        z[vidx] = self._voltage_z0(SI[np.ix_(vidx, vidx)], z[vidx], vgroups, prior, cmin)
+        for k in np.nonzero(recip)[0]:
+            num = den = 0.0
+            for i in groups[k]:
+                for j in np.nonzero(vport)[0]:
+                    w = min(abs(SI[i, j]), abs(SI[j, i])) ** 2
+                    if w > 0 and (cmin is None or w >= cmin ** 2):
+                        num += w * np.log(z[j] * abs(SI[j, i] / SI[i, j]))
+                        den += w
+            if den == 0:
+                raise ValueError(f"port group {k} (z0_from='reciprocity') does not couple to any voltage port")
+            z[groups[k]] = np.exp(num / den)
+        return z
+
+    @staticmethod
+    def _voltage_z0(SI, zphi, groups, prior, cmin=None):
+        if len(groups) == 1:
+            return zphi
+        gid = np.empty(len(zphi), int)
+        for k, members in enumerate(groups):
+            gid[members] = k
+        rows, rhs, wts = [], [], []
+        for i in range(len(gid)):
+            for j in range(i + 1, len(gid)):
+                if gid[i] != gid[j]:
+                    w = min(abs(SI[i, j]), abs(SI[j, i])) ** 2
+                    if w > 0 and (cmin is None or w >= cmin ** 2):
+                        r = np.zeros(len(groups))
+                        r[gid[i]], r[gid[j]] = 1, -1
+                        rows.append(r)
+                        rhs.append(np.log(abs(SI[j, i] / SI[i, j])))
+                        wts.append(w)
+        for k, members in enumerate(groups):
+            r = np.zeros(len(groups))
+            r[k] = 1
+            rows.append(r)
+            rhs.append(np.log(np.mean(zphi[members])))
+            wts.append(prior)
+        sw = np.sqrt(wts)
+        logz, *_ = np.linalg.lstsq(np.array(rows) * sw[:, None], np.array(rhs) * sw, rcond=None)
+        logz += np.mean(np.log(zphi)) - np.mean(logz[gid])
+        return np.exp(logz[gid])

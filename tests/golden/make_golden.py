@@ -1,0 +1,156 @@
+"""Write golden vectors of the reference implementation (run in the build
+container only, where the reference source tree is mounted read-only at
+/root/reference; the GPU box never runs this).
+
+The reference is imported from its source tree and used only to compute
+inputs / outputs that no closed form pins: meshes of the workloads, the
+Sommerfeld tables, impedance matrices (a small mesh in full, selected rows
+of the tee and of the via transition), and S / gamma / z0_line of sweeps with
+method="dense", interp=None.  Results are written as small .npz fixtures
+next to this script; nothing of the reference itself is stored.
+
+    python tests/golden/make_golden.py [set ...]     (sets: mesh tables z sweep)
+"""
+
+import os
+import sys
+import time
+import types
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, REPO)
+sys.path.insert(0, "/root/reference/Engine")
+
+import mom25d as ref  # noqa: E402
+
+from mom25d_amd import workloads as W  # noqa: E402
+
+NS = types.SimpleNamespace(Layout=ref.Layout, Substrate=ref.Substrate, Stackup=ref.Stackup,
+                           Dielectric=ref.Dielectric, analytic=ref.analytic)
+mm = 1e-3
+
+
+def mesh_digest(m):
+    import hashlib
+
+    h = hashlib.sha256()
+    for a in (m.nodes, m.tris, m.tri_basis, m.tri_sign, m.tri_layer, m.edge_len):
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def do_mesh():
+    out = {}
+    builders = dict(tee=W.tee, lpf7=W.lpf7, via2=W.via2, gcpw40=W.gcpw40, tiny=W.tiny)
+    for name, fn in builders.items():
+        lay, st, mk, _ = fn(NS)
+        m = lay.mesh(**mk)
+        out[f"{name}_n_unknowns"] = m.n_unknowns
+        out[f"{name}_n_vias"] = m.n_vias
+        out[f"{name}_digest"] = mesh_digest(m)
+        if name in ("tee", "tiny"):
+            out[f"{name}_nodes"] = m.nodes
+            out[f"{name}_tris"] = m.tris
+            out[f"{name}_tri_basis"] = m.tri_basis
+    for seed in (0, 5, 11):
+        lay, st, mk, _ = W.random_chain(seed, NS)
+        m = lay.mesh(**mk)
+        out[f"random{seed}_n_unknowns"] = m.n_unknowns
+        out[f"random{seed}_digest"] = mesh_digest(m)
+    np.savez_compressed(os.path.join(HERE, "meshes.npz"), **out)
+
+
+def do_tables():
+    out = {}
+    for f in (1e9, 7.3e9, 20e9):
+        g = ref.GroundedSlabGreens(f, 3.55, 0.508 * mm, 0.0027, rho_max=0.05,
+                                   zs_ground=ref.Substrate(3.55, 0.508 * mm, 0.0027, 5.8e7).surface_impedance(f))
+        key = f"slab_{f / 1e9:g}"
+        out[key + "_rho"] = g.rho_tab
+        out[key + "_SA"], out[key + "_SV"], out[key + "_SW"] = g.SA_tab, g.SV_tab, g.SW_tab
+    lay, st, mk, _ = W.via2(NS)
+    for f in (2e9, 12e9):
+        zg = st.surface_impedance(f)
+        g = ref.LayeredGreens(f, st, rho_max=0.04, zs_ground=zg, levels=["inner", "top"])
+        R = np.geomspace(1e-6, 0.039, 60)
+        key = f"layered_{f / 1e9:g}"
+        out[key + "_R"] = R
+        for a in ("inner", "top"):
+            for b in ("inner", "top"):
+                p = g.pair(a, b)
+                out[f"{key}_{a}_{b}_gA"] = p.gA(R)
+                out[f"{key}_{a}_{b}_gV"] = p.gV(R)
+                out[f"{key}_{a}_{b}_gW"] = p.gW(R)
+                out[f"{key}_{a}_{b}_c"] = np.array([p.cA, p.cV, p.cW, p.dz], complex)
+    np.savez_compressed(os.path.join(HERE, "tables.npz"), **out)
+
+
+def do_z():
+    out = {}
+    lay, sub, mk, _ = W.tiny(NS)
+    m = lay.mesh(**mk)
+    s = ref.Solver(m, sub, verbose=False, method="dense")
+    # 9 GHz: the upper triangle of the (symmetric) matrix; 1.5 GHz: every 5th row
+    Z = s.impedance(9e9)
+    iu = np.triu_indices(Z.shape[0])
+    out["tiny_n"] = Z.shape[0]
+    out["tiny_Zupper_9"] = Z[iu]
+    rows = np.arange(0, m.n_unknowns, 5)
+    out["tiny_rows"] = rows
+    out["tiny_Zrows_1.5"] = s.impedance(1.5e9)[rows]
+    lay, sub, mk, _ = W.tee(NS)
+    m = lay.mesh(**mk)
+    s = ref.Solver(m, sub, verbose=False, method="dense")
+    rows = np.unique(np.linspace(0, m.n_unknowns - 1, 12).astype(int))
+    out["tee_rows"] = rows
+    for f in (1e9, 10e9):
+        out[f"tee_Zrows_{f / 1e9:g}"] = s.impedance(f)[rows]
+    lay, st, mk, _ = W.via2(NS)
+    m = lay.mesh(**mk)
+    s = ref.Solver(m, st, verbose=False, method="dense")
+    n = m.n_unknowns
+    rows = np.unique(np.r_[np.linspace(0, n - 2, 8).astype(int), n - 1])
+    out["via2_rows"] = rows
+    t = time.time()
+    out["via2_Zrows_5"] = s.impedance(5e9)[rows]
+    print(f"via2 Z {time.time() - t:.0f}s", flush=True)
+    np.savez_compressed(os.path.join(HERE, "impedance.npz"), **out)
+
+
+def do_sweep():
+    out = {}
+    lay, sub, mk, f = W.tee(NS)
+    m = lay.mesh(**mk)
+    fs = f[[0, 9, 18, 27, 36]]
+    r = ref.Solver(m, sub, verbose=False, method="dense").sweep(fs, interp=None)
+    for k in ("S", "z0", "gamma", "z0_line", "fit_residual"):
+        out[f"tee_{k}"] = getattr(r, k)
+    out["tee_freqs"] = fs
+    lay, sub, mk, _ = W.tiny(NS)
+    m = lay.mesh(**mk)
+    fs = np.array([1.5e9, 9e9])
+    r = ref.Solver(m, sub, verbose=False, method="dense").sweep(fs, interp=None)
+    for k in ("S", "z0", "gamma", "z0_line"):
+        out[f"tiny_{k}"] = getattr(r, k)
+    out["tiny_freqs"] = fs
+    lay, st, mk, f = W.via2(NS)
+    m = lay.mesh(**mk)
+    fs = np.array([3e9, 11e9])
+    t = time.time()
+    r = ref.Solver(m, st, verbose=False, method="dense").sweep(fs, interp=None)
+    print(f"via2 sweep {time.time() - t:.0f}s", flush=True)
+    for k in ("S", "z0", "gamma", "z0_line"):
+        out[f"via2_{k}"] = getattr(r, k)
+    out["via2_freqs"] = fs
+    np.savez_compressed(os.path.join(HERE, "sweeps.npz"), **out)
+
+
+if __name__ == "__main__":
+    sets = sys.argv[1:] or ["mesh", "tables", "z", "sweep"]
+    for s in sets:
+        t = time.time()
+        globals()[f"do_{s}"]()
+        print(f"{s}: {time.time() - t:.1f}s", flush=True)

@@ -1,0 +1,117 @@
+"""Host-side pieces of the product that run without a GPU: the spline
+solver, the table grid, the fill plan's tiers against the oracle's, and the
+C-ABI library (loads, exports every declared symbol, struct layouts)."""
+
+import os
+import re
+
+import numpy as np
+import pytest
+from scipy.interpolate import CubicSpline
+
+from mom25d_amd import workloads as W
+from mom25d_amd.greens import PAIR_DTYPE, ppoly_eval, rho_grid, spline_coeffs
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_spline_matches_scipy_not_a_knot():
+    rng = np.random.default_rng(1)
+    x = np.cumsum(rng.uniform(0.1, 1.0, 300))
+    Y = rng.standard_normal((300, 3)) + 1j * rng.standard_normal((300, 3))
+    c = spline_coeffs(x, Y)
+    ref = CubicSpline(x, Y).c
+    assert np.max(np.abs(c - ref)) < 1e-9 * np.max(np.abs(ref))
+    xs = np.linspace(x[0] - 0.5, x[-1] + 0.5, 1000)
+    assert np.max(np.abs(ppoly_eval(x, c, xs) - CubicSpline(x, Y)(xs))) < 1e-9 * np.max(np.abs(Y))
+
+
+def test_spline_reproduces_cubics_exactly():
+    """A not-a-knot spline is exact on cubic polynomials."""
+    x = np.sort(np.random.default_rng(2).uniform(0, 3, 40))
+    p = lambda t: 1.5 - 2 * t + 0.7 * t ** 2 - 0.3 * t ** 3  # noqa: E731
+    Y = (p(x) * (1 + 0.5j))[:, None]
+    c = spline_coeffs(x, Y)
+    xs = np.linspace(0, 3, 200)
+    np.testing.assert_allclose(ppoly_eval(x, c, xs)[:, 0], p(xs) * (1 + 0.5j), atol=1e-11)
+
+
+def test_rho_grid_matches_reference(golden):
+    t = golden("tables.npz")
+    from oracle.greens_ref import SlabGreens
+
+    for f in (1e9, 7.3e9, 20e9):
+        g = SlabGreens(f, 3.55, 0.508e-3, 0.0027, rho_max=0.05)
+        np.testing.assert_array_equal(rho_grid(2 * np.pi / abs(g.k1), 0.508e-3, 0.05), t[f"slab_{f / 1e9:g}_rho"])
+
+
+@pytest.mark.parametrize("name", ["tee", "via2"])
+def test_plan_tiers_match_oracle_groups(name):
+    from mom25d_amd.plan import FillPlan
+    from oracle.assemble_ref import Geometry
+
+    lay, st, mk, _ = getattr(W, name)()
+    m = lay.mesh(**mk)
+    zl = np.array([(st.to_stackup() if hasattr(st, "to_stackup") else st).level_z(n) for n in m.layer_names])
+    plan = FillPlan(m, zl=zl)
+    geo = Geometry(m, zl=zl)
+    assert len(geo.blocks) == plan.n_blocks
+    nl = len(m.layer_names)
+    for bi, (lt, pb, r, groups, wo, Xw) in enumerate(geo.blocks):
+        np.testing.assert_array_equal(plan.blk_tris[bi, :pb.size], pb)
+        oracle_sets = {}
+        for ls, sel, pts, w, static in groups:
+            tier = 0 if static is not None else (1 if pts.shape[1] == 3 else 2)
+            oracle_sets[(ls, tier)] = set(sel.tolist())
+        mine = {}
+        for tier, items in plan.items.items():
+            for b, pp, off, cnt in items[items[:, 0] == bi]:
+                mine.setdefault((pp % nl, tier), set()).update(plan.src[off:off + cnt].tolist())
+        assert mine == oracle_sets
+
+
+def _declared_symbols():
+    text = open(os.path.join(ROOT, "include", "mom25d.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:const char\*|int64_t|int)\s+(mom_\w+)\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    import ctypes
+
+    from mom25d_amd import _lib
+
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libmom25d.so not built")
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    names = _declared_symbols()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(L, n), n
+        assert n in _lib.SIGNATURES, n
+
+
+def test_struct_layouts_match():
+    import ctypes
+
+    from mom25d_amd import _lib
+    from mom25d_amd.solver import _FillArgs
+
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libmom25d.so not built")
+    L = _lib.lib()
+    assert L.mom_fill_args_size() == ctypes.sizeof(_FillArgs)
+    assert L.mom_pair_tab_size() == PAIR_DTYPE.itemsize
+    assert L.mom_abi_version() == 1
+    assert L.mom_getrf_work_bytes(64, 1000, 64) > 0
+
+
+def test_error_path_reports_message():
+    """A bad argument is rejected on the host with a message (no device call)."""
+    from mom25d_amd import _lib
+
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libmom25d.so not built")
+    with pytest.raises(RuntimeError, match="prec must be"):
+        _lib.call("mom_cgemm_sub", 16, 4, 4, 4, None, 4, None, 4, None, 4, None)
+    with pytest.raises(RuntimeError, match="ncol must be"):
+        _lib.call("mom_sommerfeld", 1, None, None, None, None, None, None, None, None, None, None, 5, None, None)

@@ -1,0 +1,19 @@
+"""The oracle's sweep against the reference's S / gamma / z0_line (golden)."""
+
+import numpy as np
+import pytest
+
+from mom25d_amd import workloads as W
+from oracle.solve_ref import OracleSolver
+
+
+@pytest.mark.parametrize("name", ["tiny", "tee"])
+def test_sweep_matches_reference(name, golden):
+    g = golden("sweeps.npz")
+    lay, sub, mk, _ = getattr(W, name)()
+    s = OracleSolver(lay.mesh(**mk), sub, use_c=True)
+    r = s.sweep(g[f"{name}_freqs"])
+    assert np.max(np.abs(r["S"] - g[f"{name}_S"])) < 1e-7
+    np.testing.assert_allclose(r["gamma"], g[f"{name}_gamma"], rtol=1e-7)
+    np.testing.assert_allclose(r["z0_line"], g[f"{name}_z0_line"], rtol=1e-7)
+    np.testing.assert_allclose(r["z0"], g[f"{name}_z0"], rtol=1e-7)
