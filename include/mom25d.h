@@ -63,9 +63,13 @@ int mom_via_blocks(int nb, int nv, int nt, const double* IVpad, const int32_t* d
                    const int32_t* tm, const int32_t* vm, const double* cV, const double* E, double ijwe_re,
                    double ijwe_im, double* Z, int64_t ldz, void* stream);
 
-/* Z <- (Z + Z^T)/2 in place; a complex64 copy into Z32 when non-null
- * (Engine/mom25d/solver.py:522-524). */
-int mom_symmetrize(int n, double* Z, int64_t ldz, float* Z32, int64_t ld32, void* stream);
+/* Z <- (Z + Z^T)/2 in place; a complex64 copy into Z32 when non-null, scaled
+ * to d_i Z_ij d_j when d is non-null (Engine/mom25d/solver.py:522-524). */
+int mom_symmetrize(int n, double* Z, int64_t ldz, float* Z32, int64_t ld32, const double* d, void* stream);
+
+/* d_i = 1 / sqrt(|Z_ii|): symmetric diagonal scaling for the complex64 LU
+ * (no counterpart in the reference, which factors in complex128 only). */
+int mom_diag_scale(int n, const double* Z, int64_t ldz, double* d, void* stream);
 
 /* Line voltage at points of level li from the triangle charges q (nt x nrhs),
  * Engine/mom25d/solver.py:1101-1125 (Solver.line_voltage). */
@@ -104,10 +108,12 @@ int mom_cgemm_sub(int prec, int m, int n, int k, const void* A, int64_t lda, con
 int mom_gemm_timing_begin(int capacity);
 int mom_gemm_timing_end(double* out);
 
-/* FP64 iterative refinement pieces: R = B - Z X (nrhs <= 4), complex128 ->
- * complex64 conversion, and X += D with the column maxima |D|, |X| (|re|+|im|,
- * as IEEE bit patterns in uint64) for the stopping test. */
-int mom_residual(int n, int nrhs, const void* Z, int64_t ldz, const void* X, const void* B, void* R, void* stream);
+/* FP64 iterative refinement pieces: R = B - Z X (nrhs <= 4; rowsum, when
+ * non-null, receives the row sums of |Re z| + |Im z| of diag(d) Z diag(d),
+ * d = 1 when null), complex128 -> complex64 conversion, and X += D with the
+ * column maxima |D|, |X| (|re|+|im|, as IEEE bit patterns in uint64). */
+int mom_residual(int n, int nrhs, const void* Z, int64_t ldz, const void* X, const void* B, void* R, double* rowsum,
+                 const double* d, void* stream);
 int mom_cvt_z_to_c(int64_t n_reals, const double* src, float* dst, void* stream);
 int mom_refine_update(int n, int nrhs, const void* D, void* X, void* dmax, void* xmax, void* stream);
 

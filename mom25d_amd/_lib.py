@@ -27,14 +27,15 @@ SIGNATURES = {
     "mom_pair_tab_size": (_i, []),
     "mom_gram_add": (_i, [_i, _p, _p, _d, _d, _p, _i64, _p]),
     "mom_via_blocks": (_i, [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _d, _d, _p, _i64, _p]),
-    "mom_symmetrize": (_i, [_i, _p, _i64, _p, _i64, _p]),
+    "mom_symmetrize": (_i, [_i, _p, _i64, _p, _i64, _p, _p]),
+    "mom_diag_scale": (_i, [_i, _p, _i64, _p, _p]),
     "mom_getrf_work_bytes": (_i64, [_i, _i, _i]),
     "mom_getrf": (_i, [_i, _i, _p, _i64, _p, _p, _p, _i64, _i, _p]),
     "mom_build_perm": (_i, [_i, _p, _p, _p]),
     "mom_getrs_work_bytes": (_i64, [_i]),
     "mom_getrs": (_i, [_i, _i, _i, _p, _i64, _p, _p, _p, _p, _p]),
     "mom_cgemm_sub": (_i, [_i, _i, _i, _i, _p, _i64, _p, _i64, _p, _i64, _p]),
-    "mom_residual": (_i, [_i, _i, _p, _i64, _p, _p, _p, _p]),
+    "mom_residual": (_i, [_i, _i, _p, _i64, _p, _p, _p, _p, _p, _p]),
     "mom_cvt_z_to_c": (_i, [_i64, _p, _p, _p]),
     "mom_refine_update": (_i, [_i, _i, _p, _p, _p, _p, _p]),
     "mom_gemm_timing_begin": (_i, [_i]),

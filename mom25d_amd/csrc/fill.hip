@@ -355,9 +355,11 @@ __global__ void via_block_kernel(int nb, int nv, int nt, const double* __restric
 }
 
 // Z <- 0.5 (Z + Z^T) in place (Engine/mom25d/solver.py:522-524); optionally
-// also writes a complex64 copy Z32 (leading dimension ld32) of the result.
+// also writes a complex64 copy Z32 (leading dimension ld32) of the result,
+// scaled to d_i Z_ij d_j when a scaling vector d is given.
 __global__ __launch_bounds__(256) void symmetrize_kernel(int n, double* __restrict__ Z, int64_t ldz,
-                                                         float* __restrict__ Z32, int64_t ld32) {
+                                                         float* __restrict__ Z32, int64_t ld32,
+                                                         const double* __restrict__ dsc) {
   const int I = blockIdx.y, J = blockIdx.x;
   if (I > J) return;
   __shared__ z64 A[32][33];
@@ -378,8 +380,9 @@ __global__ __launch_bounds__(256) void symmetrize_kernel(int n, double* __restri
       z64 v = {0.5 * (a.re + b.re), 0.5 * (a.im + b.im)};
       Zc[(int64_t)gi * ldz + gj] = v;
       if (Z32) {
-        Z32[2 * ((int64_t)gi * ld32 + gj)] = (float)v.re;
-        Z32[2 * ((int64_t)gi * ld32 + gj) + 1] = (float)v.im;
+        const double sc = dsc ? dsc[gi] * dsc[gj] : 1.0;
+        Z32[2 * ((int64_t)gi * ld32 + gj)] = (float)(sc * v.re);
+        Z32[2 * ((int64_t)gi * ld32 + gj) + 1] = (float)(sc * v.im);
       }
     }
     gi = J * 32 + r; gj = I * 32 + tx;
@@ -388,11 +391,22 @@ __global__ __launch_bounds__(256) void symmetrize_kernel(int n, double* __restri
       z64 v = {0.5 * (a.re + b.re), 0.5 * (a.im + b.im)};
       Zc[(int64_t)gi * ldz + gj] = v;
       if (Z32) {
-        Z32[2 * ((int64_t)gi * ld32 + gj)] = (float)v.re;
-        Z32[2 * ((int64_t)gi * ld32 + gj) + 1] = (float)v.im;
+        const double sc = dsc ? dsc[gi] * dsc[gj] : 1.0;
+        Z32[2 * ((int64_t)gi * ld32 + gj)] = (float)(sc * v.re);
+        Z32[2 * ((int64_t)gi * ld32 + gj) + 1] = (float)(sc * v.im);
       }
     }
   }
+}
+
+// d[i] = 1 / sqrt(|Z[i, i]|) (1 where the diagonal vanishes): the symmetric
+// diagonal scaling applied to the complex64 copy of Z.
+__global__ void diag_scale_kernel(int n, const double* __restrict__ Z, int64_t ldz, double* __restrict__ d) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double re = Z[2 * ((int64_t)i * ldz + i)], im = Z[2 * ((int64_t)i * ldz + i) + 1];
+  const double a = sqrt(re * re + im * im);
+  d[i] = a > 0.0 ? 1.0 / sqrt(a) : 1.0;
 }
 
 // Line voltage to the plane below at observation points on level `li`
@@ -516,12 +530,19 @@ extern "C" int mom_via_blocks(int nb, int nv, int nt, const double* IVpad, const
   return check_launch("via_block_kernel");
 }
 
-extern "C" int mom_symmetrize(int n, double* Z, int64_t ldz, float* Z32, int64_t ld32, void* stream) {
+extern "C" int mom_symmetrize(int n, double* Z, int64_t ldz, float* Z32, int64_t ld32, const double* d,
+                              void* stream) {
   if (n <= 0) return 0;
   int T = (n + 31) / 32;
   if (T > 65535) return fail("mom_symmetrize: matrix too large");
-  hipLaunchKernelGGL(symmetrize_kernel, dim3(T, T), dim3(256), 0, (hipStream_t)stream, n, Z, ldz, Z32, ld32);
+  hipLaunchKernelGGL(symmetrize_kernel, dim3(T, T), dim3(256), 0, (hipStream_t)stream, n, Z, ldz, Z32, ld32, d);
   return check_launch("symmetrize_kernel");
+}
+
+extern "C" int mom_diag_scale(int n, const double* Z, int64_t ldz, double* d, void* stream) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(diag_scale_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n, Z, ldz, d);
+  return check_launch("diag_scale_kernel");
 }
 
 extern "C" int mom_line_voltage(int npts, const double* pts, int li, int nl, int nt, int nrhs,

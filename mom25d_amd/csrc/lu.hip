@@ -31,18 +31,6 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 template <typename R>
 __device__ inline R cabs1(cplx<R> a) { return fabs(a.re) + fabs(a.im); }
 
-// ---------------------------------------------------------------- grid barrier
-__device__ inline void grid_barrier(unsigned* counter, unsigned target) {
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target)
-      __builtin_amdgcn_s_sleep(1);
-  }
-  __syncthreads();
-}
-
 template <typename T>
 __device__ inline T load_coherent(const T* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -54,31 +42,79 @@ __device__ inline cplx<R> load_coherent_c(const cplx<R>* p) {
   return {load_coherent(q), load_coherent(q + 1)};
 }
 
+// Cross-workgroup publication without cache maintenance: data and flags are
+// written and read with device-coherent (relaxed, agent-scope) accesses, and
+// a writer waits for its stores to complete (s_waitcnt 0) before raising the
+// flag.  Agent-scope release / acquire would instead write back / invalidate
+// the whole L2 of the XCD on every flag, which costs microseconds when other
+// kernels keep the L2 busy.
+template <typename T>
+__device__ inline void store_coherent(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename R>
+__device__ inline void store_coherent_c(cplx<R>* p, cplx<R> v) {
+  R* q = reinterpret_cast<R*>(p);
+  store_coherent(q, v.re);
+  store_coherent(q + 1, v.im);
+}
+
+__device__ inline void wait_stores() { __builtin_amdgcn_s_waitcnt(0); }
+
 // ------------------------------------------------------------------- panel
+// Per-workgroup publication slot, one per (column parity, workgroup): the
+// candidate pivot value / row index and a generation number, written with
+// release order after the candidate row itself.  Every workgroup polls the G
+// slots of the other workgroups (one thread per slot) instead of a shared
+// counter, so no address sees more than one writer.
+struct alignas(16) PivotSlot {
+  double val;
+  int32_t idx;
+  uint32_t gen;
+};
+
 template <typename R>
 struct PanelArgs {
   cplx<R>* A;
   int64_t lda;
   int n, k0, jb, rows_per;  // rows_per: rows owned by one workgroup
+  uint32_t gen0;            // generation of this panel's first column
   int32_t* ipiv;
   int32_t* info;
-  unsigned* counter;
-  R* slot_val;          // [2][G]
-  int32_t* slot_idx;    // [2][G]
-  cplx<R>* slot_row;    // [2][G][jb]
-  cplx<R>* diag_row;    // [2][jb]
+  PivotSlot* slots;         // [2][256]
+  cplx<R>* slot_row;        // [2][256][jb]
+  cplx<R>* diag_row;        // [2][jb]
 };
+
+// (value, index) maximum over the workgroup, ties to the smaller index.
+template <typename R>
+__device__ inline void block_argmax(R& v, int& i, R* sv, int* si) {
+  for (int off = 32; off > 0; off >>= 1) {
+    R v2 = __shfl_xor(v, off, 64);
+    int i2 = __shfl_xor(i, off, 64);
+    if (v2 > v || (v2 == v && i2 < i)) { v = v2; i = i2; }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) { sv[wave] = v; si[wave] = i; }
+  __syncthreads();
+  v = sv[0];
+  i = si[0];
+#pragma unroll
+  for (int w = 1; w < 4; ++w)
+    if (sv[w] > v || (sv[w] == v && si[w] < i)) { v = sv[w]; i = si[w]; }
+  __syncthreads();
+}
 
 template <typename R>
 __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<R> a) {
   extern __shared__ unsigned char smem_raw[];
   cplx<R>* rows = reinterpret_cast<cplx<R>*>(smem_raw);  // [rows_per][jb]
-  __shared__ cplx<R> prow[256];
-  __shared__ cplx<R> drow[256];
-  __shared__ R red_v[256];
-  __shared__ int red_i[256];
-  __shared__ cplx<R> s_inv;
-  __shared__ int s_piv, s_win;
+  __shared__ cplx<R> prow[128];
+  __shared__ cplx<R> drow[128];
+  __shared__ R red_v[4];
+  __shared__ int red_i[4];
+  __shared__ int s_win;
 
   const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
   const int jb = a.jb, rp = a.rows_per;
@@ -95,72 +131,53 @@ __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<R> a) {
   for (int j = 0; j < jb; ++j) {
     const int c = a.k0 + j;
     const int par = j & 1;
-    // phase A: local candidate among owned rows >= c
+    const uint32_t gen = a.gen0 + (uint32_t)j;
+    // phase A: local candidate among owned rows >= c, published with its row
     R best = (R)-1;
     int bidx = 0x7fffffff;
     for (int lr = max(0, c - r0) + tid; lr < nown; lr += 256) {
       R v = cabs1(rows[lr * jb + j]);
       if (v > best) { best = v; bidx = r0 + lr; }
     }
-    red_v[tid] = best;
-    red_i[tid] = bidx;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-      if (tid < off) {
-        R v2 = red_v[tid + off];
-        int i2 = red_i[tid + off];
-        if (v2 > red_v[tid] || (v2 == red_v[tid] && i2 < red_i[tid])) { red_v[tid] = v2; red_i[tid] = i2; }
-      }
-      __syncthreads();
-    }
-    const int lbest = red_i[0];
-    if (tid == 0) {
-      a.slot_val[par * G + g] = red_v[0];
-      a.slot_idx[par * G + g] = lbest;
-    }
-    if (lbest != 0x7fffffff)
-      for (int q = tid; q < jb; q += 256) a.slot_row[((int64_t)par * G + g) * jb + q] = rows[(lbest - r0) * jb + q];
+    block_argmax(best, bidx, red_v, red_i);
+    if (bidx != 0x7fffffff)
+      for (int q = tid; q < jb; q += 256)
+        store_coherent_c(a.slot_row + ((int64_t)par * 256 + g) * jb + q, rows[(bidx - r0) * jb + q]);
     if (c >= r0 && c < r1)
-      for (int q = tid; q < jb; q += 256) a.diag_row[par * jb + q] = rows[(c - r0) * jb + q];
-    grid_barrier(a.counter, (unsigned)G * (unsigned)(j + 1));
-
-    // phase B: global pivot, interchange, scale, rank-1 update of owned rows
-    {
-      R v = (R)-1;
-      int i = 0x7fffffff, w = 0;
-      if (tid < G) {
-        v = load_coherent(a.slot_val + par * G + tid);
-        i = load_coherent(a.slot_idx + par * G + tid);
-        w = tid;
-      }
-      red_v[tid] = v;
-      red_i[tid] = i;
-      __syncthreads();
-      // reduce keeping the owner in the low bits of a separate pass
-      for (int off = 128; off > 0; off >>= 1) {
-        if (tid < off) {
-          R v2 = red_v[tid + off];
-          int i2 = red_i[tid + off];
-          if (v2 > red_v[tid] || (v2 == red_v[tid] && i2 < red_i[tid])) { red_v[tid] = v2; red_i[tid] = i2; }
-        }
-        __syncthreads();
-      }
-      if (tid == 0) s_piv = red_i[0];
-      __syncthreads();
-      if (tid < G && i == s_piv) s_win = w;
-      __syncthreads();
+      for (int q = tid; q < jb; q += 256) store_coherent_c(a.diag_row + par * jb + q, rows[(c - r0) * jb + q]);
+    if (tid == 0) {
+      PivotSlot* sl = a.slots + par * 256 + g;
+      store_coherent(&sl->val, (double)best);
+      store_coherent(&sl->idx, bidx);
     }
-    const int piv = s_piv, win = s_win;
+    wait_stores();
+    __syncthreads();
+    if (tid == 0) store_coherent(&a.slots[par * 256 + g].gen, gen);
+    // phase B: wait for every workgroup's slot, global pivot
+    R v = (R)-1;
+    int i = 0x7fffffff;
+    if (tid < G) {
+      PivotSlot* sl = a.slots + par * 256 + tid;
+      while (load_coherent(&sl->gen) != gen) __builtin_amdgcn_s_sleep(1);
+      v = (R)load_coherent(&sl->val);
+      i = load_coherent(&sl->idx);
+    }
+    const int mine_i = i;
+    block_argmax(v, i, red_v, red_i);
+    const int piv = i;
+    if (tid < G && mine_i == piv) s_win = tid;
+    __syncthreads();
+    const int win = s_win;
     for (int q = tid; q < jb; q += 256) {
-      prow[q] = load_coherent_c(a.slot_row + ((int64_t)par * G + win) * jb + q);
+      prow[q] = load_coherent_c(a.slot_row + ((int64_t)par * 256 + win) * jb + q);
       drow[q] = load_coherent_c(a.diag_row + par * jb + q);
     }
     __syncthreads();
     const bool nonzero = (prow[j].re != (R)0) || (prow[j].im != (R)0);
+    const cplx<R> inv = nonzero ? cdiv(cplx<R>{(R)1, (R)0}, prow[j]) : cplx<R>{(R)0, (R)0};
     if (tid == 0) {
       if (g == 0) a.ipiv[c] = piv;
       if (g == 0 && !nonzero && *a.info == 0) *a.info = c + 1;
-      if (nonzero) s_inv = cdiv(cplx<R>{(R)1, (R)0}, prow[j]);
     }
     if (nonzero) {
       if (c >= r0 && c < r1)
@@ -169,11 +186,12 @@ __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<R> a) {
         for (int q = tid; q < jb; q += 256) rows[(piv - r0) * jb + q] = drow[q];
     }
     __syncthreads();
+    // scale the pivot column and apply the rank-1 update, row by row
     const int lstart = max(0, c + 1 - r0);
-    if (nonzero)
-      for (int lr = lstart + tid; lr < nown; lr += 256) rows[lr * jb + j] = rows[lr * jb + j] * s_inv;
-    __syncthreads();
     const int w = jb - j - 1;
+    if (nonzero)
+      for (int lr = lstart + tid; lr < nown; lr += 256) rows[lr * jb + j] = rows[lr * jb + j] * inv;
+    __syncthreads();
     if (w > 0)
       for (int q = tid; q < (nown - lstart) * w; q += 256) {
         int lr = lstart + q / w, cc = j + 1 + q % w;
@@ -188,20 +206,62 @@ __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<R> a) {
 }
 
 // ----------------------------------------------------------------- laswp
-// Row interchanges ipiv[k0 .. k0+jb) applied, in order, to columns
-// [c0, c1) of A (one thread per column).
+// Row interchanges ipiv[k0 .. k0+b) applied, in order, to columns [c0, c1)
+// of A.  The first wavefront folds the b interchanges into one permutation
+// of the (at most 2b) rows they touch; every workgroup then moves its 16
+// columns of those rows through LDS: all loads, then all stores.
 template <typename R>
-__global__ void laswp_kernel(cplx<R>* A, int64_t lda, int c0, int c1, int k0, int jb, const int32_t* ipiv) {
-  int col = c0 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= c1) return;
-  for (int k = k0; k < k0 + jb; ++k) {
-    int p = ipiv[k];
-    if (p != k) {
-      cplx<R> t = A[(int64_t)k * lda + col];
-      A[(int64_t)k * lda + col] = A[(int64_t)p * lda + col];
-      A[(int64_t)p * lda + col] = t;
+__global__ __launch_bounds__(256) void laswp_kernel(cplx<R>* A, int64_t lda, int c0, int c1, int k0, int b,
+                                                    const int32_t* ipiv) {
+  __shared__ int rows[256];
+  __shared__ int src[256];
+  __shared__ int s_nrows;
+  __shared__ cplx<R> buf[256][17];
+  const int tid = threadIdx.x, lane = tid & 63;
+  if (tid < 64) {
+    for (int i = lane; i < 2 * b; i += 64) {
+      rows[i] = i < b ? k0 + i : -1;
+      src[i] = i;
     }
+    int nout = 0;
+    for (int k = 0; k < b; ++k) {
+      const int p = ipiv[k0 + k];
+      int pp;
+      if (p < k0 + b) {
+        pp = p - k0;
+      } else {
+        int found = -1;
+        for (int base = 0; base < nout; base += 64) {
+          const int j = base + lane;
+          const bool hit = j < nout && rows[b + j] == p;
+          const unsigned long long m = __ballot(hit);
+          if (m) { found = base + __ffsll((long long)m) - 1; break; }
+        }
+        if (found < 0) {
+          found = nout;
+          if (lane == 0) rows[b + nout] = p;
+          ++nout;
+        }
+        pp = b + found;
+      }
+      if (lane == 0 && pp != k) {
+        int t = src[k];
+        src[k] = src[pp];
+        src[pp] = t;
+      }
+    }
+    if (lane == 0) s_nrows = b + nout;
   }
+  __syncthreads();
+  const int nrows = s_nrows;
+  const int tx = tid & 15, ty = tid >> 4;
+  const int col = c0 + blockIdx.x * 16 + tx;
+  if (col < c1)
+    for (int i = ty; i < nrows; i += 16) buf[i][tx] = A[(int64_t)rows[src[i]] * lda + col];
+  __syncthreads();
+  if (col < c1)
+    for (int i = ty; i < nrows; i += 16)
+      if (src[i] != i) A[(int64_t)rows[i] * lda + col] = buf[i][tx];
 }
 
 // ------------------------------------------------------------------ trsm
@@ -232,12 +292,16 @@ __global__ __launch_bounds__(256) void trsm_lower_unit_kernel(cplx<R>* A, int64_
 // MFMA 16x16 tiles; K' processed in chunks of 16 real (8 complex).
 template <typename R>
 struct mfma_traits;
+// Operand layout of the 16x16x4 MFMAs: lane l holds A[l % 16][l / 16] and
+// B[l / 16][l % 16]; accumulator element v of lane l is D[row(l, v)][l % 16]
+// with row = l / 16 + 4 v for f64 and 4 (l / 16) + v for f32.
 template <>
 struct mfma_traits<double> {
   typedef d4 acc_t;
   static __device__ inline acc_t mfma(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ inline int row(int lane, int v) { return (lane >> 4) + 4 * v; }
 };
 template <>
 struct mfma_traits<float> {
@@ -245,6 +309,7 @@ struct mfma_traits<float> {
   static __device__ inline acc_t mfma(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ inline int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
 template <typename R>
@@ -305,7 +370,7 @@ __global__ __launch_bounds__(256) void cgemm_sub_kernel(int m, int n, int kk, co
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        int r = row0 + wr + 16 * a + 4 * (lane >> 4) + v;
+        int r = row0 + wr + 16 * a + mfma_traits<R>::row(lane, v);
         int cr = colr0 + wc + 16 * b + (lane & 15);
         if (r < m && cr < 2 * n) Cr[(int64_t)r * 2 * ldc + cr] -= acc[a][b][v];
       }
@@ -347,12 +412,12 @@ __global__ void gather_rows_kernel(int n, int nrhs, const int32_t* perm, const c
 // so a workgroup only ever waits for workgroups that are already running.
 constexpr int kTrsvBS = 64;
 
-template <typename R, bool LOWER>
-__global__ __launch_bounds__(256) void trsv_kernel(int n, int nrhs, const cplx<R>* __restrict__ A, int64_t lda,
-                                                   cplx<R>* X, unsigned* flags, unsigned* ticket) {
+template <typename R, bool LOWER, int NR>
+__global__ __launch_bounds__(256) void trsv_kernel(int n, const cplx<R>* __restrict__ A, int64_t lda, cplx<R>* X,
+                                                   unsigned* flags, unsigned* ticket) {
   constexpr int BS = kTrsvBS;
   __shared__ cplx<R> D[BS][BS + 1];
-  __shared__ cplx<R> part[4][BS / 4][4];
+  __shared__ cplx<R> part[4][BS / 4][NR];
   __shared__ int s_blk;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nblk = (n + BS - 1) / BS;
@@ -363,43 +428,42 @@ __global__ __launch_bounds__(256) void trsv_kernel(int n, int nrhs, const cplx<R
   __syncthreads();
   const int blk = s_blk;
   const int k0 = blk * BS, b = min(BS, n - k0);
-  cplx<R> acc[BS / 4][4];
+  cplx<R> acc[BS / 4][NR];
 #pragma unroll
   for (int r = 0; r < BS / 4; ++r)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) acc[r][c] = {0, 0};
-  // diagonal block to LDS
+    for (int c = 0; c < NR; ++c) acc[r][c] = {0, 0};
   for (int q = tid; q < BS * BS; q += 256) {
     int r = q / BS, c = q % BS;
     D[r][c] = (r < b && c < b) ? A[(int64_t)(k0 + r) * lda + k0 + c] : cplx<R>{0, 0};
   }
-  const int first = LOWER ? 0 : blk + 1, last = LOWER ? blk : nblk;
-  for (int jb = first; jb < last; ++jb) {
+  // dependencies in the order they were solved: blocks 0.. (lower) or nblk-1.. (upper)
+  const int ndep = LOWER ? blk : nblk - 1 - blk;
+  for (int d = 0; d < ndep; ++d) {
+    const int jb = LOWER ? d : nblk - 1 - d;
     if (tid == 0)
-      while (__hip_atomic_load(flags + jb, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0u)
-        __builtin_amdgcn_s_sleep(1);
+      while (load_coherent(flags + jb) == 0u) __builtin_amdgcn_s_sleep(1);
     __syncthreads();
-    const int j0 = jb * BS, jcol = j0 + lane;
+    const int jcol = jb * BS + lane;
     if (jcol < n) {
-      cplx<R> xv[4];
+      cplx<R> xv[NR];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) xv[c] = c < nrhs ? load_coherent_c(X + (int64_t)jcol * nrhs + c) : cplx<R>{0, 0};
+      for (int c = 0; c < NR; ++c) xv[c] = load_coherent_c(X + (int64_t)jcol * NR + c);
 #pragma unroll
       for (int r = 0; r < BS / 4; ++r) {
         const int row = wave * (BS / 4) + r;
         if (row < b) {
           const cplx<R> l = A[(int64_t)(k0 + row) * lda + jcol];
 #pragma unroll
-          for (int c = 0; c < 4; ++c) acc[r][c] += l * xv[c];
+          for (int c = 0; c < NR; ++c) acc[r][c] += l * xv[c];
         }
       }
     }
   }
-  // reduce partial sums over the 64 lanes of each wavefront
 #pragma unroll
   for (int r = 0; r < BS / 4; ++r)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < NR; ++c) {
       R re = acc[r][c].re, im = acc[r][c].im;
       for (int off = 32; off > 0; off >>= 1) {
         re += __shfl_down(re, off, 64);
@@ -408,11 +472,11 @@ __global__ __launch_bounds__(256) void trsv_kernel(int n, int nrhs, const cplx<R
       if (lane == 0) part[wave][r][c] = {re, im};
     }
   __syncthreads();
-  // substitution by the first wavefront: lane = row of the block
   if (wave == 0) {
-    for (int c = 0; c < nrhs; ++c) {
+#pragma unroll
+    for (int c = 0; c < NR; ++c) {
       cplx<R> v = {0, 0};
-      if (lane < b) v = X[(int64_t)(k0 + lane) * nrhs + c] - part[lane / (BS / 4)][lane % (BS / 4)][c];
+      if (lane < b) v = X[(int64_t)(k0 + lane) * NR + c] - part[lane / (BS / 4)][lane % (BS / 4)][c];
       if (LOWER) {
         for (int k = 0; k < b; ++k) {
           cplx<R> xk = {__shfl(v.re, k, 64), __shfl(v.im, k, 64)};
@@ -425,24 +489,29 @@ __global__ __launch_bounds__(256) void trsv_kernel(int n, int nrhs, const cplx<R
           if (lane < k) v -= D[lane][k] * xk;
         }
       }
-      if (lane < b) X[(int64_t)(k0 + lane) * nrhs + c] = v;
+      if (lane < b) store_coherent_c(X + (int64_t)(k0 + lane) * NR + c, v);
     }
   }
-  __threadfence();
+  wait_stores();
   __syncthreads();
-  if (tid == 0) __hip_atomic_store(flags + blk, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  if (tid == 0) store_coherent(flags + blk, 1u);
 }
 
-// R = B - Z X (FP64), one wavefront per row.
+// R = B - Z X (FP64), one wavefront per row; when rowsum is given it also
+// receives d_r sum_k (|Re z_rk| + |Im z_rk|) d_k (d = 1 when dsc is null),
+// the row sums of the diagonally scaled matrix (for its norm).
 __global__ __launch_bounds__(256) void residual_kernel(int n, int nrhs, const z64* __restrict__ Z, int64_t ldz,
                                                        const z64* __restrict__ X, const z64* __restrict__ B,
-                                                       z64* __restrict__ Rout) {
+                                                       z64* __restrict__ Rout, double* __restrict__ rowsum,
+                                                       const double* __restrict__ dsc) {
   const int lane = threadIdx.x & 63;
   const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (r >= n) return;
   z64 s[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  double a = 0.0;
   for (int k = lane; k < n; k += 64) {
     z64 zz = Z[(int64_t)r * ldz + k];
+    if (rowsum) a += (fabs(zz.re) + fabs(zz.im)) * (dsc ? dsc[k] : 1.0);
     for (int c = 0; c < nrhs && c < 4; ++c) s[c] += zz * X[(int64_t)k * nrhs + c];
   }
   for (int c = 0; c < nrhs && c < 4; ++c) {
@@ -452,6 +521,10 @@ __global__ __launch_bounds__(256) void residual_kernel(int n, int nrhs, const z6
       im += __shfl_down(im, off, 64);
     }
     if (lane == 0) Rout[(int64_t)r * nrhs + c] = B[(int64_t)r * nrhs + c] - z64{re, im};
+  }
+  if (rowsum) {
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
+    if (lane == 0) rowsum[r] = a * (dsc ? dsc[r] : 1.0);
   }
 }
 
@@ -494,11 +567,11 @@ static GemmTiming g_timing;
 
 static void timing_mark_begin(hipStream_t s) {
   if (!g_timing.on || g_timing.used + 2 > g_timing.ev.size()) return;
-  hipEventRecord(g_timing.ev[g_timing.used], s);
+  (void)hipEventRecord(g_timing.ev[g_timing.used], s);
 }
 static void timing_mark_end(hipStream_t s, double flops) {
   if (!g_timing.on || g_timing.used + 2 > g_timing.ev.size()) return;
-  hipEventRecord(g_timing.ev[g_timing.used + 1], s);
+  (void)hipEventRecord(g_timing.ev[g_timing.used + 1], s);
   g_timing.flops.push_back(flops);
   g_timing.used += 2;
 }
@@ -514,15 +587,82 @@ static int device_cus() {
 
 template <typename R>
 static size_t getrf_work_bytes(int n, int jb) {
-  const int G = 256;
   size_t b = 0;
-  b += sizeof(unsigned) * ((n + jb - 1) / jb + 1);
-  b += sizeof(R) * 2 * G + sizeof(int32_t) * 2 * G;
-  b += sizeof(cplx<R>) * 2 * (size_t)G * jb + sizeof(cplx<R>) * 2 * jb;
-  b += 1024;
-  return b;
+  b += sizeof(PivotSlot) * 2 * 256 + 256;
+  b += sizeof(cplx<R>) * 2 * (size_t)256 * jb + 256;
+  b += sizeof(cplx<R>) * 2 * (size_t)jb + 256;
+  return b + 1024;
 }
 
+// Side stream and events of the look-ahead (one set per process, created on
+// first use; the side stream has the default priority).
+struct Lookahead {
+  hipStream_t side = nullptr;
+  std::vector<hipEvent_t> ev;
+};
+static Lookahead g_la;
+
+static int lookahead_events(size_t need) {
+  if (!g_la.side) MOM_TRY(hipStreamCreateWithFlags(&g_la.side, hipStreamNonBlocking));
+  while (g_la.ev.size() < need) {
+    hipEvent_t e;
+    MOM_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    g_la.ev.push_back(e);
+  }
+  return 0;
+}
+
+static size_t g_panel_lds_max = 0;
+
+template <typename R>
+static int launch_panel(cplx<R>* A, int64_t lda, int n, int k0, int b, int32_t* ipiv, int32_t* info,
+                        PivotSlot* slots, cplx<R>* slot_row, cplx<R>* diag_row, int ncu, hipStream_t s) {
+  const int m = n - k0;
+  const size_t lds_max = 120 * 1024;
+  if (g_panel_lds_max == 0) {
+    MOM_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(panel_kernel<double>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max));
+    MOM_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(panel_kernel<float>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max));
+    g_panel_lds_max = lds_max;
+  }
+  const int gmax = std::min(ncu, 256);  // slot arrays hold 256 workgroups
+  // as few workgroups as the LDS allows (fewer slots to poll per column)
+  int rp = std::max(1, (int)(lds_max / ((size_t)b * sizeof(cplx<R>))));
+  int G = (m + rp - 1) / rp;
+  if (G > gmax) return fail("mom_getrf: matrix too large for the LDS-resident panel (reduce the panel width)");
+  G = std::max(G, std::min(gmax, (m + 255) / 256));  // at least one row per thread block of work
+  rp = (m + G - 1) / G;
+  G = (m + rp - 1) / rp;
+  const uint32_t gen0 = 1u + (uint32_t)k0;  // unique per column of the factorisation
+  PanelArgs<R> pa{A, lda, n, k0, b, rp, gen0, ipiv, info, slots, slot_row, diag_row};
+  void* args[] = {&pa};
+  MOM_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(panel_kernel<R>), dim3(G), dim3(256), args,
+                                     (size_t)rp * b * sizeof(cplx<R>), s));
+  return 0;
+}
+
+template <typename R>
+static int launch_update(cplx<R>* A, int64_t lda, int k0, int b, int r0, int r1, int c0, int c1, hipStream_t s) {
+  // trsm on the block row k0..k0+b of columns [c0, c1), then the GEMM on rows [r0, r1) x columns [c0, c1)
+  if (c1 <= c0) return 0;
+  int rc;
+  hipLaunchKernelGGL(trsm_lower_unit_kernel<R>, dim3((c1 - c0 + 15) / 16), dim3(256), 0, s, A, lda, k0, b, c0, c1);
+  if ((rc = check_launch("trsm_lower_unit_kernel"))) return rc;
+  const int mm = r1 - r0, nn = c1 - c0;
+  if (mm <= 0) return 0;
+  dim3 grid((2 * nn + 63) / 64, (mm + 63) / 64);
+  timing_mark_begin(s);
+  hipLaunchKernelGGL(cgemm_sub_kernel<R>, grid, dim3(256), 0, s, mm, nn, b, A + (int64_t)r0 * lda + k0, lda,
+                     A + (int64_t)k0 * lda + c0, lda, A + (int64_t)r0 * lda + c0, lda);
+  if ((rc = check_launch("cgemm_sub_kernel"))) return rc;
+  timing_mark_end(s, 8.0 * (double)mm * (double)nn * (double)b);
+  return 0;
+}
+
+// Right-looking blocked LU with one panel of look-ahead: while the trailing
+// update of step p runs on the caller's stream, panel p+1 (whose columns
+// were updated first) is factored on the side stream.
 template <typename R>
 static int getrf(int n, cplx<R>* A, int64_t lda, int32_t* ipiv, int32_t* info, void* work, size_t work_bytes, int jb,
                  hipStream_t s) {
@@ -538,52 +678,40 @@ static int getrf(int n, cplx<R>* A, int64_t lda, int32_t* ipiv, int32_t* info, v
     return p;
   };
   const int npanel = (n + jb - 1) / jb;
-  unsigned* counters = reinterpret_cast<unsigned*>(take(sizeof(unsigned) * (npanel + 1)));
-  R* slot_val = reinterpret_cast<R*>(take(sizeof(R) * 2 * 256));
-  int32_t* slot_idx = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * 2 * 256));
+  PivotSlot* slots = reinterpret_cast<PivotSlot*>(take(sizeof(PivotSlot) * 2 * 256));
   cplx<R>* slot_row = reinterpret_cast<cplx<R>*>(take(sizeof(cplx<R>) * 2 * 256 * (size_t)jb));
   cplx<R>* diag_row = reinterpret_cast<cplx<R>*>(take(sizeof(cplx<R>) * 2 * (size_t)jb));
-  hipLaunchKernelGGL(zero_u32_kernel, dim3((npanel + 1 + 255) / 256), dim3(256), 0, s, counters, npanel + 1);
-  int rc = check_launch("zero_u32_kernel");
-  if (rc) return rc;
+  int rc;
+  hipLaunchKernelGGL(zero_u32_kernel, dim3((2 * 256 * 4 + 255) / 256), dim3(256), 0, s,
+                     reinterpret_cast<unsigned*>(slots), 2 * 256 * 4);
+  if ((rc = check_launch("zero_u32_kernel"))) return rc;
   hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, reinterpret_cast<unsigned*>(info), 1);
   if ((rc = check_launch("zero_u32_kernel"))) return rc;
-  const size_t lds_max = 64 * 1024;
+  if ((rc = lookahead_events(2 * (size_t)npanel + 2))) return rc;
+  hipStream_t side = g_la.side;
+  // panel 0 on the caller's stream
+  if ((rc = launch_panel<R>(A, lda, n, 0, std::min(jb, n), ipiv, info, slots, slot_row, diag_row, ncu, s)))
+    return rc;
   for (int p = 0; p < npanel; ++p) {
-    const int k0 = p * jb, b = std::min(jb, n - k0), m = n - k0;
-    const int gmax = std::min(ncu, 256);  // slot arrays hold 256 workgroups
-    int G = std::min(gmax, std::max(1, (m + 31) / 32));
-    int rp = (m + G - 1) / G;
-    while ((size_t)rp * b * sizeof(cplx<R>) > lds_max && G < gmax) {
-      G = std::min(gmax, G * 2);
-      rp = (m + G - 1) / G;
-    }
-    if ((size_t)rp * b * sizeof(cplx<R>) > lds_max)
-      return fail("mom_getrf: matrix too large for the LDS-resident panel (reduce the panel width)");
-    G = (m + rp - 1) / rp;
-    PanelArgs<R> pa{A, lda, n, k0, b, rp, ipiv, info, counters + p, slot_val, slot_idx, slot_row, diag_row};
-    void* args[] = {&pa};
-    MOM_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(panel_kernel<R>), dim3(G), dim3(256), args,
-                                       (size_t)rp * b * sizeof(cplx<R>), s));
-    if (k0 > 0) {
-      hipLaunchKernelGGL(laswp_kernel<R>, dim3((k0 + 255) / 256), dim3(256), 0, s, A, lda, 0, k0, k0, b, ipiv);
-      if ((rc = check_launch("laswp_kernel"))) return rc;
-    }
+    const int k0 = p * jb, b = std::min(jb, n - k0);
     const int c0 = k0 + b;
-    if (c0 < n) {
-      hipLaunchKernelGGL(laswp_kernel<R>, dim3((n - c0 + 255) / 256), dim3(256), 0, s, A, lda, c0, n, k0, b, ipiv);
+    if (p > 0) MOM_TRY(hipStreamWaitEvent(s, g_la.ev[2 * p], 0));  // panel p done on the side stream
+    if (k0 > 0) {
+      hipLaunchKernelGGL(laswp_kernel<R>, dim3((k0 + 15) / 16), dim3(256), 0, s, A, lda, 0, k0, k0, b, ipiv);
       if ((rc = check_launch("laswp_kernel"))) return rc;
-      hipLaunchKernelGGL(trsm_lower_unit_kernel<R>, dim3((n - c0 + 15) / 16), dim3(256), 0, s, A, lda, k0, b, c0, n);
-      if ((rc = check_launch("trsm_lower_unit_kernel"))) return rc;
-      const int mm = n - c0, nn = n - c0;
-      dim3 grid((2 * nn + 63) / 64, (mm + 63) / 64);
-      timing_mark_begin(s);
-      hipLaunchKernelGGL(cgemm_sub_kernel<R>, grid, dim3(256), 0, s, mm, nn, b,
-                         A + (int64_t)c0 * lda + k0, lda, A + (int64_t)k0 * lda + c0, lda,
-                         A + (int64_t)c0 * lda + c0, lda);
-      if ((rc = check_launch("cgemm_sub_kernel"))) return rc;
-      timing_mark_end(s, 8.0 * (double)mm * (double)nn * (double)b);
     }
+    if (c0 >= n) break;
+    hipLaunchKernelGGL(laswp_kernel<R>, dim3((n - c0 + 15) / 16), dim3(256), 0, s, A, lda, c0, n, k0, b, ipiv);
+    if ((rc = check_launch("laswp_kernel"))) return rc;
+    const int b1 = std::min(jb, n - c0);
+    // next panel's columns first, then hand the panel to the side stream
+    if ((rc = launch_update<R>(A, lda, k0, b, c0, n, c0, c0 + b1, s))) return rc;
+    MOM_TRY(hipEventRecord(g_la.ev[2 * p + 1], s));
+    MOM_TRY(hipStreamWaitEvent(side, g_la.ev[2 * p + 1], 0));
+    if ((rc = launch_panel<R>(A, lda, n, c0, b1, ipiv, info, slots, slot_row, diag_row, ncu, side))) return rc;
+    MOM_TRY(hipEventRecord(g_la.ev[2 * (p + 1)], side));
+    // the rest of the trailing matrix
+    if ((rc = launch_update<R>(A, lda, k0, b, c0, n, c0 + b1, n, s))) return rc;
   }
   return 0;
 }
@@ -599,11 +727,21 @@ static int getrs(int n, int nrhs, const cplx<R>* LU, int64_t lda, const int32_t*
   if ((rc = check_launch("zero_u32_kernel"))) return rc;
   hipLaunchKernelGGL(gather_rows_kernel<R>, dim3((n * nrhs + 255) / 256), dim3(256), 0, s, n, nrhs, perm, B, X);
   if ((rc = check_launch("gather_rows_kernel"))) return rc;
-  hipLaunchKernelGGL((trsv_kernel<R, true>), dim3(nblk), dim3(256), 0, s, n, nrhs, LU, lda, X, flags,
-                     flags + 2 * nblk);
-  if ((rc = check_launch("trsv_kernel"))) return rc;
-  hipLaunchKernelGGL((trsv_kernel<R, false>), dim3(nblk), dim3(256), 0, s, n, nrhs, LU, lda, X, flags + nblk,
-                     flags + 2 * nblk + 1);
+#define MOM_TRSV(NR)                                                                                    \
+  case NR:                                                                                              \
+    hipLaunchKernelGGL((trsv_kernel<R, true, NR>), dim3(nblk), dim3(256), 0, s, n, LU, lda, X, flags,      \
+                       flags + 2 * nblk);                                                               \
+    if ((rc = check_launch("trsv_kernel"))) return rc;                                                  \
+    hipLaunchKernelGGL((trsv_kernel<R, false, NR>), dim3(nblk), dim3(256), 0, s, n, LU, lda, X, flags + nblk, \
+                       flags + 2 * nblk + 1);                                                           \
+    break;
+  switch (nrhs) {
+    MOM_TRSV(1)
+    MOM_TRSV(2)
+    MOM_TRSV(3)
+    MOM_TRSV(4)
+  }
+#undef MOM_TRSV
   return check_launch("trsv_kernel");
 }
 
@@ -664,11 +802,11 @@ extern "C" int mom_cgemm_sub(int prec, int m, int n, int k, const void* A, int64
 }
 
 extern "C" int mom_residual(int n, int nrhs, const void* Z, int64_t ldz, const void* X, const void* B, void* R,
-                            void* stream) {
+                            double* rowsum, const double* dsc, void* stream) {
   if (n <= 0) return 0;
   if (nrhs > 4) return fail("mom_residual: at most 4 right-hand sides");
   hipLaunchKernelGGL(residual_kernel, dim3((n + 3) / 4), dim3(256), 0, (hipStream_t)stream, n, nrhs, (const z64*)Z,
-                     ldz, (const z64*)X, (const z64*)B, (z64*)R);
+                     ldz, (const z64*)X, (const z64*)B, (z64*)R, rowsum, dsc);
   return check_launch("residual_kernel");
 }
 
@@ -691,7 +829,7 @@ extern "C" int mom_refine_update(int n, int nrhs, const void* D, void* X, void* 
 // out[0..2], the number of launches, their total duration [ms] and their
 // total algorithmic flops (8 m n k per complex update).
 extern "C" int mom_gemm_timing_begin(int capacity) {
-  for (hipEvent_t e : g_timing.ev) hipEventDestroy(e);
+  for (hipEvent_t e : g_timing.ev) (void)hipEventDestroy(e);
   g_timing.ev.assign(2 * (size_t)capacity, nullptr);
   for (auto& e : g_timing.ev) MOM_TRY(hipEventCreate(&e));
   g_timing.flops.clear();

@@ -302,7 +302,7 @@ class Solver:
     HMATRIX_N = 7000
 
     def __init__(self, mesh, substrate, z0=None, verbose=True, method="auto", v_weight=None, z0_recip_min=None,
-                 tand_floor=None, precision="fp64", device="cuda", panel=None, refine_tol=1e-12, refine_maxit=30):
+                 tand_floor=None, precision="fp64", device="cuda", panel=None, refine_tol=2e-16, refine_maxit=10):
         from .layered import Stackup
 
         if method not in ("auto", "dense"):
@@ -339,6 +339,7 @@ class Solver:
         self.panel = panel or (64 if precision == "fp64" else 64)
         self.refine_tol = refine_tol
         self.refine_maxit = refine_maxit
+        self.gmres_restart = 30
         self._plan = None
         self._bufs = {}
         _check_abi()
@@ -454,7 +455,12 @@ class Solver:
                       P(d["pad_tri"]), P(d["tp"]), P(d["vp"]), P(d["tm"]), P(d["vm"]), P(d["cV"]), P(E),
                       ijwe.real, ijwe.imag, P(Z), Z.stride(0), stream)
             self._keep = E
-        _lib.call("mom_symmetrize", n, P(Z), Z.stride(0), P(Z32), Z32.stride(0) if Z32 is not None else 0, stream)
+        dsc = None
+        if Z32 is not None:
+            dsc = self._buf("dscale", (n,), torch.float64)
+            _lib.call("mom_diag_scale", n, P(Z), Z.stride(0), P(dsc), stream)
+        _lib.call("mom_symmetrize", n, P(Z), Z.stride(0), P(Z32), Z32.stride(0) if Z32 is not None else 0, P(dsc),
+                  stream)
         return Z
 
     def impedance(self, freq, greens=None):
@@ -492,24 +498,99 @@ class Solver:
             _lib.call("mom_getrs", 64, n, nrhs, P(A), A.stride(0), P(perm), P(B), P(X), P(sw), stream)
             self.last_refine = 0
             return X, info
-        B32 = B.to(torch.complex64)
-        X32 = torch.empty_like(B32)
-        _lib.call("mom_getrs", 32, n, nrhs, P(A), A.stride(0), P(perm), P(B32), P(X32), P(sw), stream)
-        X.copy_(X32)
-        Rz = torch.empty_like(X)
-        nrm = torch.zeros(2 * nrhs, dtype=torch.int64, device=self.device)
-        it = 0
-        for it in range(1, self.refine_maxit + 1):
-            _lib.call("mom_residual", n, nrhs, P(Z), Z.stride(0), P(X), P(B), P(Rz), stream)
-            R32 = Rz.to(torch.complex64)
-            _lib.call("mom_getrs", 32, n, nrhs, P(A), A.stride(0), P(perm), P(R32), P(X32), P(sw), stream)
-            nrm.zero_()
-            _lib.call("mom_refine_update", n, nrhs, P(X32), P(X), P(nrm), P(nrm[nrhs:]), stream)
-            v = nrm.cpu().view(torch.float64).numpy()
-            if np.all(v[:nrhs] <= self.refine_tol * v[nrhs:]):
+        return self._gmres_ir(Z, A, perm, sw, B), info
+
+    def _gmres_ir(self, Z, A32, perm, sw, B):
+        """Restarted GMRES in FP64 on the diagonally scaled system
+        (D Z D) y = D b, x = D y, right-preconditioned by the complex64 LU of
+        D Z D.  All right-hand sides advance together (one Krylov basis per
+        column).  Target: normwise backward error of the scaled system
+        |D b - D Z D y| <= refine_tol |D Z D|_inf |y| (FP64 round-off); a
+        restart cycle that does not halve the true residual ends the
+        iteration (the FP64 residual floor is reached)."""
+        n, nrhs = B.shape
+        P = _lib.ptr
+        stream = _lib.stream_ptr()
+        dv = self._bufs["dscale"]
+        d = dv[:, None]
+        m = self.gmres_restart
+        zeros = torch.zeros_like(B)
+        rows = self._buf("rowsum", (n,), torch.float64)
+        Bs = d * B
+
+        def precond(V):
+            W32 = V.to(torch.complex64)
+            Y32 = torch.empty_like(W32)
+            _lib.call("mom_getrs", 32, n, nrhs, P(A32), A32.stride(0), P(perm), P(W32), P(Y32), P(sw), stream)
+            return Y32.to(torch.complex128)
+
+        def matvec(Y):  # (D Z D) Y
+            R = torch.empty_like(B)
+            _lib.call("mom_residual", n, nrhs, P(Z), Z.stride(0), P(d * Y), P(zeros), P(R), None, None, stream)
+            return -(d * R)
+
+        def residual(Y, rowsum=None):  # D (b - Z D Y)
+            R = torch.empty_like(B)
+            _lib.call("mom_residual", n, nrhs, P(Z), Z.stride(0), P(d * Y), P(B), P(R), P(rowsum), P(dv), stream)
+            return d * R
+
+        Y = precond(Bs)
+        R = residual(Y, rows)
+        znorm = float(rows.max())
+        its = 0
+        prev = None
+        Vb = torch.empty((m + 1, n, nrhs), dtype=torch.complex128, device=self.device)
+        Wb = torch.empty((m, n, nrhs), dtype=torch.complex128, device=self.device)
+        for _restart in range(self.refine_maxit):
+            beta = torch.linalg.vector_norm(R, dim=0)
+            bh = beta.cpu().numpy()
+            target = self.refine_tol * znorm * torch.linalg.vector_norm(Y, dim=0).cpu().numpy()
+            if np.all(bh <= target) or (prev is not None and np.all(bh > 0.5 * prev)):
                 break
-        self.last_refine = it
-        return X, info
+            prev = bh
+            Vb[0] = R / beta
+            H = np.zeros((m + 1, m, nrhs), complex)
+            g = np.zeros((m + 1, nrhs), complex)
+            g[0] = bh
+            cs = np.zeros((m, nrhs), complex)
+            sn = np.zeros((m, nrhs), complex)
+            k = 0
+            for j in range(m):
+                Wb[j] = precond(Vb[j])
+                w = matvec(Wb[j])
+                hj = torch.zeros((j + 2, nrhs), dtype=torch.complex128, device=self.device)
+                for _ in range(2):  # classical Gram-Schmidt, twice
+                    h = torch.einsum("inp,np->ip", Vb[:j + 1].conj(), w)
+                    w = w - torch.einsum("inp,ip->np", Vb[:j + 1], h)
+                    hj[:j + 1] += h
+                nw = torch.linalg.vector_norm(w, dim=0)
+                hj[j + 1] = nw
+                Vb[j + 1] = w / nw
+                hc = hj.cpu().numpy()
+                its += 1
+                for i in range(j):  # Givens rotations, per column
+                    t = cs[i] * hc[i] + sn[i] * hc[i + 1]
+                    hc[i + 1] = -np.conj(sn[i]) * hc[i] + np.conj(cs[i]) * hc[i + 1]
+                    hc[i] = t
+                den = np.sqrt(np.abs(hc[j]) ** 2 + np.abs(hc[j + 1]) ** 2)
+                den = np.where(den == 0, 1.0, den)
+                cs[j] = hc[j] / den
+                sn[j] = hc[j + 1] / den
+                hc[j] = cs[j].conj() * hc[j] + sn[j].conj() * hc[j + 1]
+                hc[j + 1] = 0
+                g[j + 1] = -sn[j] * g[j]
+                g[j] = cs[j].conj() * g[j]
+                H[:j + 2, j] = hc
+                k = j + 1
+                if np.all(np.abs(g[j + 1]) <= 0.25 * target):
+                    break
+            y = np.zeros((k, nrhs), complex)
+            for c in range(nrhs):
+                y[:, c] = np.linalg.solve(np.triu(H[:k, :k, c]), g[:k, c])
+            Y = Y + torch.einsum("inp,ip->np", Wb[:k], torch.from_numpy(y).to(self.device))
+            R = residual(Y)
+        self.last_refine = its
+        return d * Y
 
     def solve_currents(self, freq, excitations=None, greens=None):
         g = greens or self.greens(freq)

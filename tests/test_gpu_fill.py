@@ -55,17 +55,33 @@ def test_tiny_fill_matches_reference_golden(f, golden):
 
     rows, R = golden_tiny(golden("impedance.npz"), f)
     lay, sub, mk, _ = W.tiny()
-    Z = Solver(lay.mesh(**mk), sub, verbose=False).impedance(f)
+    m = lay.mesh(**mk)
+    Z = Solver(m, sub, verbose=False).impedance(f)
+    nb = m.n_basis
+    rwg = rows < nb
+    Zr, Rr = Z[rows[rwg]][:, :nb], R[rwg][:, :nb]
+    assert np.max(np.abs(Zr - Rr)) < 1e-10 * np.max(np.abs(Rr))
     assert np.max(np.abs(Z[rows] - R)) < 1e-10 * np.max(np.abs(R))
     assert np.array_equal(Z, Z.T)
 
 
-@pytest.mark.parametrize("name,f", [("tee", 1e9), ("tee", 10e9), ("via2", 5e9)])
+def blocks_close(Z, R, nb, tol):
+    """Relative error per block (RWG-RWG, via-RWG, via-via): the via-via
+    block can be orders of magnitude larger than the rest."""
+    errs = []
+    for rows, cols in ((slice(0, nb), slice(0, nb)), (slice(nb, None), slice(0, nb)), (slice(nb, None), slice(nb, None))):
+        r = R[rows, cols]
+        if r.size:
+            errs.append(np.max(np.abs(Z[rows, cols] - r)) / np.max(np.abs(r)))
+    return max(errs)
+
+
+@pytest.mark.parametrize("name,f", [("tee", 1e9), ("tee", 10e9), ("via2", 5e9), ("tiny", 3e9)])
 def test_fill_matches_oracle(name, f):
     lay, st, mk, _ = getattr(W, name)()
     m = lay.mesh(**mk)
     Z = Solver(m, st, verbose=False).impedance(f)
     R = OracleSolver(m, st, use_c=True).impedance(f)
-    err = np.max(np.abs(Z - R)) / np.max(np.abs(R))
-    print(f"{name} {f / 1e9:g} GHz N={m.n_unknowns} max rel err {err:.2e}")
+    err = blocks_close(Z, R, m.n_basis, 1e-10)
+    print(f"{name} {f / 1e9:g} GHz N={m.n_unknowns} max blockwise rel err {err:.2e}")
     assert err < 1e-10
