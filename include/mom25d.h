@@ -90,10 +90,11 @@ int mom_getrf(int prec, int n, void* A, int64_t lda, int32_t* ipiv, int32_t* inf
 /* Row permutation of the factorisation: perm[i] = original row of row i. */
 int mom_build_perm(int n, const int32_t* ipiv, int32_t* perm, void* stream);
 
-/* X = (LU)^-1 P B for nrhs <= 4 right-hand sides (n x nrhs, row-major);
- * `work` holds mom_getrs_work_bytes(n) bytes.  Replaces scipy.linalg.lu_solve
+/* X = (LU)^-1 P B for nrhs <= 4 right-hand sides (n x nrhs, row-major).
+ * `work` holds mom_getrs_work_bytes(prec, n) bytes of device memory (progress
+ * counters, cleared by the call itself).  Replaces scipy.linalg.lu_solve
  * (LAPACK zgetrs), Engine/mom25d/solver.py:23. */
-int64_t mom_getrs_work_bytes(int n);
+int64_t mom_getrs_work_bytes(int prec, int n);
 int mom_getrs(int prec, int n, int nrhs, const void* LU, int64_t lda, const int32_t* perm, const void* B, void* X,
               void* work, void* stream);
 
@@ -116,6 +117,12 @@ int mom_residual(int n, int nrhs, const void* Z, int64_t ldz, const void* X, con
                  const double* d, void* stream);
 int mom_cvt_z_to_c(int64_t n_reals, const double* src, float* dst, void* stream);
 int mom_refine_update(int n, int nrhs, const void* D, void* X, void* dmax, void* xmax, void* stream);
+
+/* GMRES orthogonalisation against a basis V (j1 x n x nrhs, complex128):
+ * h = V^H w per column, and w -= V h (no counterpart in the reference, whose
+ * dense solve is a direct LAPACK factorisation). */
+int mom_basis_dot(int n, int nrhs, int j1, const void* V, const void* w, void* h, void* stream);
+int mom_basis_sub(int n, int nrhs, int j1, const void* V, const void* h, void* w, void* stream);
 
 #ifdef __cplusplus
 }

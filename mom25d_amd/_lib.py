@@ -32,7 +32,7 @@ SIGNATURES = {
     "mom_getrf_work_bytes": (_i64, [_i, _i, _i]),
     "mom_getrf": (_i, [_i, _i, _p, _i64, _p, _p, _p, _i64, _i, _p]),
     "mom_build_perm": (_i, [_i, _p, _p, _p]),
-    "mom_getrs_work_bytes": (_i64, [_i]),
+    "mom_getrs_work_bytes": (_i64, [_i, _i]),
     "mom_getrs": (_i, [_i, _i, _i, _p, _i64, _p, _p, _p, _p, _p]),
     "mom_cgemm_sub": (_i, [_i, _i, _i, _i, _p, _i64, _p, _i64, _p, _i64, _p]),
     "mom_residual": (_i, [_i, _i, _p, _i64, _p, _p, _p, _p, _p, _p]),
@@ -40,6 +40,8 @@ SIGNATURES = {
     "mom_refine_update": (_i, [_i, _i, _p, _p, _p, _p, _p]),
     "mom_gemm_timing_begin": (_i, [_i]),
     "mom_gemm_timing_end": (_i, [_p]),
+    "mom_basis_dot": (_i, [_i, _i, _i, _p, _p, _p, _p]),
+    "mom_basis_sub": (_i, [_i, _i, _i, _p, _p, _p, _p]),
     "mom_line_voltage": (_i, [_i, _p, _i, _i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _d, _p, _p]),
 }
 

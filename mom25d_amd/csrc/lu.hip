@@ -62,6 +62,12 @@ __device__ inline void store_coherent_c(cplx<R>* p, cplx<R> v) {
 
 __device__ inline void wait_stores() { __builtin_amdgcn_s_waitcnt(0); }
 
+// Back-off between polls of a flag in memory (units of 64 clocks).
+#ifndef MOM_POLL_SLEEP
+#define MOM_POLL_SLEEP 1
+#endif
+__device__ inline void poll_pause() { __builtin_amdgcn_s_sleep(MOM_POLL_SLEEP); }
+
 // ------------------------------------------------------------------- panel
 // Per-workgroup publication slot, one per (column parity, workgroup): the
 // candidate pivot value / row index and a generation number, written with
@@ -158,7 +164,7 @@ __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<R> a) {
     int i = 0x7fffffff;
     if (tid < G) {
       PivotSlot* sl = a.slots + par * 256 + tid;
-      while (load_coherent(&sl->gen) != gen) __builtin_amdgcn_s_sleep(1);
+      while (load_coherent(&sl->gen) != gen) poll_pause();
       v = (R)load_coherent(&sl->val);
       i = load_coherent(&sl->idx);
     }
@@ -312,68 +318,137 @@ struct mfma_traits<float> {
   static __device__ inline int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
-template <typename R>
+// Workgroup tile: BM rows x BN real columns (BN/2 complex); the 4 wavefronts
+// form a 2 x 2 grid, each owning (BM/2) x (BN/2) as MFMA 16x16 tiles.  K' is
+// streamed in stages of 16 real columns through double-buffered LDS: the
+// global loads of stage s+1 are in flight while stage s runs on the matrix
+// cores.  Tile sizes are chosen per precision (gemm_tiles below).
+constexpr int kGemmBK = 16;
+
+template <typename R, int BM, int BN>
 __global__ __launch_bounds__(256) void cgemm_sub_kernel(int m, int n, int kk, const cplx<R>* __restrict__ Ap,
                                                         int64_t lda, const cplx<R>* __restrict__ Bp, int64_t ldb,
                                                         cplx<R>* __restrict__ Cp, int64_t ldc) {
   typedef typename mfma_traits<R>::acc_t acc_t;
-  __shared__ R As[64][17];   // [row][k']
-  __shared__ R Bs[16][65];   // [k'][col']
+  constexpr int BK = kGemmBK;
+  constexpr int TM = BM / 32, TN = BN / 32;          // MFMA tiles per wavefront
+  constexpr int AE = BM * BK / 256;                  // A' reals per thread per stage
+  constexpr int BE = (BK / 2) * (BN / 2) / 256;      // B complex per thread per stage
+  static_assert(AE >= 1 && BE >= 1, "tile too small for 256 threads");
+  __shared__ R As[2][BM][BK + 1];
+  __shared__ R Bs[2][BK][BN + 1];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int row0 = blockIdx.y * 64;
-  const int colr0 = blockIdx.x * 64;  // real column offset (complex column colr0 / 2)
-  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-  acc_t acc[2][2];
+  const int row0 = blockIdx.y * BM;
+  const int colr0 = blockIdx.x * BN;
+  const int wr = (wave >> 1) * (BM / 2), wc = (wave & 1) * (BN / 2);
+  acc_t acc[TM][TN];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < TM; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
+    for (int b = 0; b < TN; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
 
   const R* Ar = reinterpret_cast<const R*>(Ap);
   const int K2 = 2 * kk;
-  for (int k0 = 0; k0 < K2; k0 += 16) {
-    // A' chunk: 64 rows x 16 real
-    for (int q = tid; q < 64 * 16; q += 256) {
-      int r = q >> 4, c = q & 15;
-      int gr = row0 + r, gk = k0 + c;
-      As[r][c] = (gr < m && gk < K2) ? Ar[(int64_t)gr * 2 * lda + gk] : (R)0;
+  const int nst = (K2 + BK - 1) / BK;
+  constexpr int APR = BK / AE;                        // threads per A' row
+  const int a_row = tid / APR, a_col = (tid % APR) * AE;
+  constexpr int BPR = (BN / 2) / BE;                  // threads per complex B row
+  const int b_row = tid / BPR, b_col = (tid % BPR) * BE;
+  R ra[AE];
+  cplx<R> rb[BE];
+  auto load = [&](int st) {
+    const int k0 = st * BK;
+    const int gr = row0 + a_row;
+#pragma unroll
+    for (int e = 0; e < AE; ++e) {
+      const int gk = k0 + a_col + e;
+      ra[e] = (gr < m && gk < K2) ? Ar[(int64_t)gr * 2 * lda + gk] : (R)0;
     }
-    // B'' chunk: 8 complex rows x 32 complex columns -> 16 x 64 real
-    {
-      int kr = tid >> 5, jc = tid & 31;
-      int gk = (k0 >> 1) + kr, gj = (colr0 >> 1) + jc;
-      cplx<R> b = (gk < kk && gj < n) ? Bp[(int64_t)gk * ldb + gj] : cplx<R>{0, 0};
-      Bs[2 * kr][2 * jc] = b.re;
-      Bs[2 * kr][2 * jc + 1] = b.im;
-      Bs[2 * kr + 1][2 * jc] = -b.im;
-      Bs[2 * kr + 1][2 * jc + 1] = b.re;
+    const int gk = (k0 >> 1) + b_row;
+#pragma unroll
+    for (int e = 0; e < BE; ++e) {
+      const int gj = (colr0 >> 1) + b_col + e;
+      rb[e] = (gk < kk && gj < n) ? Bp[(int64_t)gk * ldb + gj] : cplx<R>{0, 0};
     }
-    __syncthreads();
+  };
+  auto store = [&](int buf) {
 #pragma unroll
-    for (int ks = 0; ks < 16; ks += 4) {
-      R af[2], bf[2];
+    for (int e = 0; e < AE; ++e) As[buf][a_row][a_col + e] = ra[e];
 #pragma unroll
-      for (int a = 0; a < 2; ++a) af[a] = As[wr + 16 * a + (lane & 15)][ks + (lane >> 4)];
-#pragma unroll
-      for (int b = 0; b < 2; ++b) bf[b] = Bs[ks + (lane >> 4)][wc + 16 * b + (lane & 15)];
-#pragma unroll
-      for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = mfma_traits<R>::mfma(af[a], bf[b], acc[a][b]);
+    for (int e = 0; e < BE; ++e) {
+      const int c = 2 * (b_col + e);
+      Bs[buf][2 * b_row][c] = rb[e].re;
+      Bs[buf][2 * b_row][c + 1] = rb[e].im;
+      Bs[buf][2 * b_row + 1][c] = -rb[e].im;
+      Bs[buf][2 * b_row + 1][c + 1] = rb[e].re;
     }
+  };
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int st = 0; st < nst; ++st) {
+    const int cur = st & 1;
+    if (st + 1 < nst) load(st + 1);
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 4) {
+      R af[TM], bf[TN];
+#pragma unroll
+      for (int a = 0; a < TM; ++a) af[a] = As[cur][wr + 16 * a + (lane & 15)][ks + (lane >> 4)];
+#pragma unroll
+      for (int b = 0; b < TN; ++b) bf[b] = Bs[cur][ks + (lane >> 4)][wc + 16 * b + (lane & 15)];
+#pragma unroll
+      for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b) acc[a][b] = mfma_traits<R>::mfma(af[a], bf[b], acc[a][b]);
+    }
+    if (st + 1 < nst) store(cur ^ 1);
     __syncthreads();
   }
   R* Cr = reinterpret_cast<R*>(Cp);
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < TM; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < TN; ++b)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        int r = row0 + wr + 16 * a + mfma_traits<R>::row(lane, v);
-        int cr = colr0 + wc + 16 * b + (lane & 15);
+        const int r = row0 + wr + 16 * a + mfma_traits<R>::row(lane, v);
+        const int cr = colr0 + wc + 16 * b + (lane & 15);
         if (r < m && cr < 2 * n) Cr[(int64_t)r * 2 * ldc + cr] -= acc[a][b][v];
       }
+}
+
+#ifndef MOM_GEMM64_BM
+#define MOM_GEMM64_BM 64
+#endif
+#ifndef MOM_GEMM64_BN
+#define MOM_GEMM64_BN 128
+#endif
+#ifndef MOM_GEMM32_BM
+#define MOM_GEMM32_BM 64
+#endif
+#ifndef MOM_GEMM32_BN
+#define MOM_GEMM32_BN 128
+#endif
+
+template <typename R>
+struct gemm_tiles;
+template <>
+struct gemm_tiles<double> {
+  static constexpr int BM = MOM_GEMM64_BM, BN = MOM_GEMM64_BN;
+};
+template <>
+struct gemm_tiles<float> {
+  static constexpr int BM = MOM_GEMM32_BM, BN = MOM_GEMM32_BN;
+};
+
+template <typename R>
+static int launch_cgemm(int m, int n, int k, const cplx<R>* A, int64_t lda, const cplx<R>* B, int64_t ldb,
+                        cplx<R>* C, int64_t ldc, hipStream_t s) {
+  constexpr int BM = gemm_tiles<R>::BM, BN = gemm_tiles<R>::BN;
+  if ((m + BM - 1) / BM > 65535) return fail("cgemm: too many row tiles");
+  dim3 grid((2 * n + BN - 1) / BN, (m + BM - 1) / BM);
+  hipLaunchKernelGGL((cgemm_sub_kernel<R, BM, BN>), grid, dim3(256), 0, s, m, n, k, A, lda, B, ldb, C, ldc);
+  return check_launch("cgemm_sub_kernel");
 }
 
 // ------------------------------------------------------------ solve phase
@@ -403,98 +478,138 @@ __global__ void gather_rows_kernel(int n, int nrhs, const int32_t* perm, const c
   dst[q] = src[(int64_t)perm[i] * nrhs + c];
 }
 
-// Blocked triangular solve with the block rows handed out by ticket: the
-// workgroup holding ticket t solves block row t (lower) or nblk-1-t (upper)
-// after the blocks it depends on have published their solution (flags).
-// Every lane keeps partial sums over the finished blocks in registers; one
-// cross-lane reduction per row, then forward / backward substitution of the
-// 64 x 64 diagonal block staged in LDS.  Tickets are taken in launch order,
-// so a workgroup only ever waits for workgroups that are already running.
+// Triangular solves as a chain of 64-row blocks: block k needs x_0 .. x_{k-1}
+// (lower; the mirror for upper).  Workgroup k, given block k by ticket, first
+// copies the triangle of its diagonal block into LDS (packed by columns, the
+// reciprocal of U's diagonal in place of the diagonal), then keeps catching
+// up with the blocks already finished (one coherent progress counter,
+// advanced in order) and accumulates A[k, j] x_j for all of them in
+// registers.  Once its last dependency is done, wavefront 0 runs the
+// substitution on the diagonal block (row i in lane i, x_k broadcast by lane
+// shuffle) -- the backward-stable form of LAPACK's ztrsv -- and publishes x_k.
+// Tickets are taken in launch order, so a workgroup only waits for
+// workgroups that are already running.
 constexpr int kTrsvBS = 64;
+
+template <bool LOWER>
+__device__ __forceinline__ int packed_index(int i, int k) {
+  // column k of the strict lower triangle holds rows k+1 .. 63; of the upper
+  // triangle (with diagonal) rows 0 .. k
+  return LOWER ? k * (2 * kTrsvBS - k - 1) / 2 + (i - k - 1) : k * (k + 1) / 2 + i;
+}
 
 template <typename R, bool LOWER, int NR>
 __global__ __launch_bounds__(256) void trsv_kernel(int n, const cplx<R>* __restrict__ A, int64_t lda, cplx<R>* X,
-                                                   unsigned* flags, unsigned* ticket) {
+                                                   unsigned* progress, unsigned* ticket) {
   constexpr int BS = kTrsvBS;
-  __shared__ cplx<R> D[BS][BS + 1];
-  __shared__ cplx<R> part[4][BS / 4][NR];
+  constexpr int TRI = LOWER ? BS * (BS - 1) / 2 : BS * (BS + 1) / 2;
+  __shared__ cplx<R> Dk[TRI];
+  __shared__ cplx<R> xs[BS][NR];
+  __shared__ cplx<R> part[4][BS][NR];
   __shared__ int s_blk;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  __shared__ unsigned s_ready;
+  const int tid = threadIdx.x;
+  const int r = tid & 63, q = tid >> 6;  // row of the block, column quarter
   const int nblk = (n + BS - 1) / BS;
   if (tid == 0) {
-    unsigned t = atomicAdd(ticket, 1u);
+    const unsigned t = atomicAdd(ticket, 1u);
     s_blk = LOWER ? (int)t : nblk - 1 - (int)t;
   }
   __syncthreads();
   const int blk = s_blk;
   const int k0 = blk * BS, b = min(BS, n - k0);
-  cplx<R> acc[BS / 4][NR];
-#pragma unroll
-  for (int r = 0; r < BS / 4; ++r)
-#pragma unroll
-    for (int c = 0; c < NR; ++c) acc[r][c] = {0, 0};
-  for (int q = tid; q < BS * BS; q += 256) {
-    int r = q / BS, c = q % BS;
-    D[r][c] = (r < b && c < b) ? A[(int64_t)(k0 + r) * lda + k0 + c] : cplx<R>{0, 0};
+  const int row = k0 + r;
+  for (int e = tid; e < BS * BS; e += 256) {
+    const int i = e / BS, k = e % BS;
+    if (LOWER ? i > k : i <= k) {
+      cplx<R> v = (i < b && k < b) ? A[(int64_t)(k0 + i) * lda + k0 + k] : cplx<R>{(R)(i == k), 0};
+      if (!LOWER && i == k) v = cdiv(cplx<R>{1, 0}, v);
+      Dk[packed_index<LOWER>(i, k)] = v;
+    }
   }
-  // dependencies in the order they were solved: blocks 0.. (lower) or nblk-1.. (upper)
-  const int ndep = LOWER ? blk : nblk - 1 - blk;
-  for (int d = 0; d < ndep; ++d) {
-    const int jb = LOWER ? d : nblk - 1 - d;
-    if (tid == 0)
-      while (load_coherent(flags + jb) == 0u) __builtin_amdgcn_s_sleep(1);
+  cplx<R> acc[NR];
+#pragma unroll
+  for (int c = 0; c < NR; ++c) acc[c] = {0, 0};
+  const unsigned need = LOWER ? (unsigned)blk : (unsigned)(nblk - 1 - blk);  // finished blocks needed
+  unsigned done = 0;
+  while (done < need) {
+    if (tid == 0) {
+      unsigned v;
+      while ((v = load_coherent(progress)) <= done) poll_pause();
+      s_ready = v < need ? v : need;
+    }
     __syncthreads();
-    const int jcol = jb * BS + lane;
-    if (jcol < n) {
-      cplx<R> xv[NR];
+    const unsigned upto = s_ready;
+    for (unsigned d = done; d < upto; ++d) {
+      const int jb = LOWER ? (int)d : nblk - 1 - (int)d;
+      const int j0 = jb * BS;
+      if (tid < BS * NR) {
+        const int i = tid / NR, c = tid % NR;
+        xs[i][c] = (j0 + i < n) ? load_coherent_c(X + (int64_t)(j0 + i) * NR + c) : cplx<R>{0, 0};
+      }
+      __syncthreads();
+      if (r < b) {
+        const cplx<R>* Lrow = A + (int64_t)row * lda + j0 + 16 * q;
 #pragma unroll
-      for (int c = 0; c < NR; ++c) xv[c] = load_coherent_c(X + (int64_t)jcol * NR + c);
+        for (int e = 0; e < 16; ++e) {
+          if (j0 + 16 * q + e < n) {
+            const cplx<R> l = Lrow[e];
 #pragma unroll
-      for (int r = 0; r < BS / 4; ++r) {
-        const int row = wave * (BS / 4) + r;
-        if (row < b) {
-          const cplx<R> l = A[(int64_t)(k0 + row) * lda + jcol];
-#pragma unroll
-          for (int c = 0; c < NR; ++c) acc[r][c] += l * xv[c];
+            for (int c = 0; c < NR; ++c) acc[c] += l * xs[16 * q + e][c];
+          }
         }
+      }
+      __syncthreads();
+    }
+    done = upto;
+  }
+#pragma unroll
+  for (int c = 0; c < NR; ++c) part[q][r][c] = acc[c];
+  __syncthreads();
+  if (tid >= 64) return;
+  // wavefront 0: v = b - sum, then substitution with the diagonal block
+  const int i = tid;
+  cplx<R> v[NR];
+#pragma unroll
+  for (int c = 0; c < NR; ++c)
+    v[c] = i < b ? X[(int64_t)(k0 + i) * NR + c] - part[0][i][c] - part[1][i][c] - part[2][i][c] - part[3][i][c]
+                 : cplx<R>{0, 0};
+  if (LOWER) {
+#pragma unroll
+    for (int k = 0; k < BS - 1; ++k) {
+      cplx<R> xk[NR];
+#pragma unroll
+      for (int c = 0; c < NR; ++c) xk[c] = {__shfl(v[c].re, k, 64), __shfl(v[c].im, k, 64)};
+      if (i > k) {
+        const cplx<R> l = Dk[packed_index<true>(i, k)];
+#pragma unroll
+        for (int c = 0; c < NR; ++c) v[c] -= l * xk[c];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int k = BS - 1; k >= 0; --k) {
+      if (i == k) {
+        const cplx<R> rd = Dk[packed_index<false>(k, k)];
+#pragma unroll
+        for (int c = 0; c < NR; ++c) v[c] = v[c] * rd;
+      }
+      cplx<R> xk[NR];
+#pragma unroll
+      for (int c = 0; c < NR; ++c) xk[c] = {__shfl(v[c].re, k, 64), __shfl(v[c].im, k, 64)};
+      if (i < k) {
+        const cplx<R> u = Dk[packed_index<false>(i, k)];
+#pragma unroll
+        for (int c = 0; c < NR; ++c) v[c] -= u * xk[c];
       }
     }
   }
+  if (i < b) {
 #pragma unroll
-  for (int r = 0; r < BS / 4; ++r)
-#pragma unroll
-    for (int c = 0; c < NR; ++c) {
-      R re = acc[r][c].re, im = acc[r][c].im;
-      for (int off = 32; off > 0; off >>= 1) {
-        re += __shfl_down(re, off, 64);
-        im += __shfl_down(im, off, 64);
-      }
-      if (lane == 0) part[wave][r][c] = {re, im};
-    }
-  __syncthreads();
-  if (wave == 0) {
-#pragma unroll
-    for (int c = 0; c < NR; ++c) {
-      cplx<R> v = {0, 0};
-      if (lane < b) v = X[(int64_t)(k0 + lane) * NR + c] - part[lane / (BS / 4)][lane % (BS / 4)][c];
-      if (LOWER) {
-        for (int k = 0; k < b; ++k) {
-          cplx<R> xk = {__shfl(v.re, k, 64), __shfl(v.im, k, 64)};
-          if (lane > k && lane < b) v -= D[lane][k] * xk;
-        }
-      } else {
-        for (int k = b - 1; k >= 0; --k) {
-          if (lane == k) v = cdiv(v, D[k][k]);
-          cplx<R> xk = {__shfl(v.re, k, 64), __shfl(v.im, k, 64)};
-          if (lane < k) v -= D[lane][k] * xk;
-        }
-      }
-      if (lane < b) store_coherent_c(X + (int64_t)(k0 + lane) * NR + c, v);
-    }
+    for (int c = 0; c < NR; ++c) store_coherent_c(X + (int64_t)(k0 + i) * NR + c, v[c]);
   }
   wait_stores();
-  __syncthreads();
-  if (tid == 0) store_coherent(flags + blk, 1u);
+  if (i == 0) store_coherent(progress, need + 1u);
 }
 
 // R = B - Z X (FP64), one wavefront per row; when rowsum is given it also
@@ -526,6 +641,49 @@ __global__ __launch_bounds__(256) void residual_kernel(int n, int nrhs, const z6
     for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
     if (lane == 0) rowsum[r] = a * (dsc ? dsc[r] : 1.0);
   }
+}
+
+// GMRES orthogonalisation helpers over a basis V (j1 x n x nrhs, complex128):
+// h[i, c] = sum_r conj(V[i, r, c]) w[r, c]   (one workgroup per basis vector)
+__global__ __launch_bounds__(256) void basis_dot_kernel(int n, int nrhs, const z64* __restrict__ V,
+                                                        const z64* __restrict__ w, z64* __restrict__ h) {
+  const int i = blockIdx.x, tid = threadIdx.x;
+  z64 acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  const z64* Vi = V + (int64_t)i * n * nrhs;
+  for (int r = tid; r < n; r += 256)
+    for (int c = 0; c < nrhs; ++c) {
+      const z64 v = Vi[(int64_t)r * nrhs + c], x = w[(int64_t)r * nrhs + c];
+      acc[c] += z64{v.re * x.re + v.im * x.im, v.re * x.im - v.im * x.re};
+    }
+  __shared__ z64 part[4][4];
+  const int lane = tid & 63, wave = tid >> 6;
+  for (int c = 0; c < nrhs; ++c) {
+    double re = acc[c].re, im = acc[c].im;
+    for (int off = 32; off > 0; off >>= 1) {
+      re += __shfl_down(re, off, 64);
+      im += __shfl_down(im, off, 64);
+    }
+    if (lane == 0) part[wave][c] = {re, im};
+  }
+  __syncthreads();
+  if (tid < nrhs) {
+    z64 t = part[0][tid];
+    t += part[1][tid];
+    t += part[2][tid];
+    t += part[3][tid];
+    h[(int64_t)i * nrhs + tid] = t;
+  }
+}
+
+// w[r, c] -= sum_{i < j1} V[i, r, c] h[i, c]
+__global__ void basis_sub_kernel(int n, int nrhs, int j1, const z64* __restrict__ V, const z64* __restrict__ h,
+                                 z64* __restrict__ w) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n * nrhs) return;
+  const int c = q % nrhs;
+  z64 s = {0, 0};
+  for (int i = 0; i < j1; ++i) s += V[(int64_t)i * n * nrhs + q] * h[(int64_t)i * nrhs + c];
+  w[q] -= s;
 }
 
 __global__ void cvt_z_to_c_kernel(int64_t n, const double* src, float* dst) {
@@ -651,11 +809,10 @@ static int launch_update(cplx<R>* A, int64_t lda, int k0, int b, int r0, int r1,
   if ((rc = check_launch("trsm_lower_unit_kernel"))) return rc;
   const int mm = r1 - r0, nn = c1 - c0;
   if (mm <= 0) return 0;
-  dim3 grid((2 * nn + 63) / 64, (mm + 63) / 64);
   timing_mark_begin(s);
-  hipLaunchKernelGGL(cgemm_sub_kernel<R>, grid, dim3(256), 0, s, mm, nn, b, A + (int64_t)r0 * lda + k0, lda,
-                     A + (int64_t)k0 * lda + c0, lda, A + (int64_t)r0 * lda + c0, lda);
-  if ((rc = check_launch("cgemm_sub_kernel"))) return rc;
+  if ((rc = launch_cgemm<R>(mm, nn, b, A + (int64_t)r0 * lda + k0, lda, A + (int64_t)k0 * lda + c0, lda,
+                            A + (int64_t)r0 * lda + c0, lda, s)))
+    return rc;
   timing_mark_end(s, 8.0 * (double)mm * (double)nn * (double)b);
   return 0;
 }
@@ -723,17 +880,16 @@ static int getrs(int n, int nrhs, const cplx<R>* LU, int64_t lda, const int32_t*
   if (nrhs > 4) return fail("mom_getrs: at most 4 right-hand sides");
   const int nblk = (n + kTrsvBS - 1) / kTrsvBS;
   int rc;
-  hipLaunchKernelGGL(zero_u32_kernel, dim3((2 * nblk + 2 + 255) / 256), dim3(256), 0, s, flags, 2 * nblk + 2);
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, flags, 4);
   if ((rc = check_launch("zero_u32_kernel"))) return rc;
   hipLaunchKernelGGL(gather_rows_kernel<R>, dim3((n * nrhs + 255) / 256), dim3(256), 0, s, n, nrhs, perm, B, X);
   if ((rc = check_launch("gather_rows_kernel"))) return rc;
-#define MOM_TRSV(NR)                                                                                    \
-  case NR:                                                                                              \
-    hipLaunchKernelGGL((trsv_kernel<R, true, NR>), dim3(nblk), dim3(256), 0, s, n, LU, lda, X, flags,      \
-                       flags + 2 * nblk);                                                               \
-    if ((rc = check_launch("trsv_kernel"))) return rc;                                                  \
-    hipLaunchKernelGGL((trsv_kernel<R, false, NR>), dim3(nblk), dim3(256), 0, s, n, LU, lda, X, flags + nblk, \
-                       flags + 2 * nblk + 1);                                                           \
+#define MOM_TRSV(NR)                                                                                            \
+  case NR:                                                                                                      \
+    hipLaunchKernelGGL((trsv_kernel<R, true, NR>), dim3(nblk), dim3(256), 0, s, n, LU, lda, X, flags, flags + 2); \
+    if ((rc = check_launch("trsv_kernel"))) return rc;                                                          \
+    hipLaunchKernelGGL((trsv_kernel<R, false, NR>), dim3(nblk), dim3(256), 0, s, n, LU, lda, X, flags + 1,       \
+                       flags + 3);                                                                              \
     break;
   switch (nrhs) {
     MOM_TRSV(1)
@@ -770,8 +926,12 @@ extern "C" int mom_build_perm(int n, const int32_t* ipiv, int32_t* perm, void* s
   return check_launch("build_perm_kernel");
 }
 
-extern "C" int64_t mom_getrs_work_bytes(int n) {
-  return (int64_t)sizeof(unsigned) * (2 * ((n + kTrsvBS - 1) / kTrsvBS) + 2);
+// Workspace of mom_getrs: the four progress / ticket counters of the two
+// triangular solves (the same for every n and precision).
+extern "C" int64_t mom_getrs_work_bytes(int prec, int n) {
+  (void)prec;
+  (void)n;
+  return 256;
 }
 
 extern "C" int mom_getrs(int prec, int n, int nrhs, const void* LU, int64_t lda, const int32_t* perm, const void* B,
@@ -788,17 +948,10 @@ extern "C" int mom_getrs(int prec, int n, int nrhs, const void* LU, int64_t lda,
 extern "C" int mom_cgemm_sub(int prec, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb,
                              void* C, int64_t ldc, void* stream) {
   if (m <= 0 || n <= 0 || k <= 0) return 0;
-  dim3 grid((2 * n + 63) / 64, (m + 63) / 64);
   hipStream_t s = (hipStream_t)stream;
-  if (prec == 64)
-    hipLaunchKernelGGL(cgemm_sub_kernel<double>, grid, dim3(256), 0, s, m, n, k, (const z64*)A, lda, (const z64*)B,
-                       ldb, (z64*)C, ldc);
-  else if (prec == 32)
-    hipLaunchKernelGGL(cgemm_sub_kernel<float>, grid, dim3(256), 0, s, m, n, k, (const c32*)A, lda, (const c32*)B,
-                       ldb, (c32*)C, ldc);
-  else
-    return fail("mom_cgemm_sub: prec must be 64 or 32");
-  return check_launch("cgemm_sub_kernel");
+  if (prec == 64) return launch_cgemm<double>(m, n, k, (const z64*)A, lda, (const z64*)B, ldb, (z64*)C, ldc, s);
+  if (prec == 32) return launch_cgemm<float>(m, n, k, (const c32*)A, lda, (const c32*)B, ldb, (c32*)C, ldc, s);
+  return fail("mom_cgemm_sub: prec must be 64 or 32");
 }
 
 extern "C" int mom_residual(int n, int nrhs, const void* Z, int64_t ldz, const void* X, const void* B, void* R,
@@ -852,4 +1005,20 @@ extern "C" int mom_gemm_timing_end(double* out) {
   out[1] = ms;
   out[2] = fl;
   return 0;
+}
+
+extern "C" int mom_basis_dot(int n, int nrhs, int j1, const void* V, const void* w, void* h, void* stream) {
+  if (j1 <= 0 || n <= 0) return 0;
+  if (nrhs < 1 || nrhs > 4) return fail("mom_basis_dot: 1..4 columns");
+  hipLaunchKernelGGL(basis_dot_kernel, dim3(j1), dim3(256), 0, (hipStream_t)stream, n, nrhs, (const z64*)V,
+                     (const z64*)w, (z64*)h);
+  return check_launch("basis_dot_kernel");
+}
+
+extern "C" int mom_basis_sub(int n, int nrhs, int j1, const void* V, const void* h, void* w, void* stream) {
+  if (j1 <= 0 || n <= 0) return 0;
+  if (nrhs < 1 || nrhs > 4) return fail("mom_basis_sub: 1..4 columns");
+  hipLaunchKernelGGL(basis_sub_kernel, dim3((n * nrhs + 255) / 256), dim3(256), 0, (hipStream_t)stream, n, nrhs, j1,
+                     (const z64*)V, (const z64*)h, (z64*)w);
+  return check_launch("basis_sub_kernel");
 }

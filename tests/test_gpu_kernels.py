@@ -89,7 +89,7 @@ def _lu_solve(A, B, prec, jb):
     info = torch.zeros(4, dtype=torch.int32, device=DEV)
     wb = int(_lib.lib().mom_getrf_work_bytes(prec, n, jb))
     work = torch.empty(wb, dtype=torch.uint8, device=DEV)
-    sw = torch.empty(int(_lib.lib().mom_getrs_work_bytes(n)), dtype=torch.uint8, device=DEV)
+    sw = torch.empty(int(_lib.lib().mom_getrs_work_bytes(prec, n)), dtype=torch.uint8, device=DEV)
     s = _lib.stream_ptr()
     P = _lib.ptr
     _lib.call("mom_getrf", prec, n, P(Ad), n, P(ipiv), P(info), P(work), wb, jb, s)
@@ -98,13 +98,14 @@ def _lu_solve(A, B, prec, jb):
     return X.cpu().numpy(), ipiv.cpu().numpy(), int(info[0])
 
 
-@pytest.mark.parametrize("n,jb", [(1, 8), (37, 8), (200, 32), (777, 64), (1500, 64)])
-def test_lu_fp64_matches_lapack(n, jb):
+@pytest.mark.parametrize("n,jb,nrhs", [(1, 8, 1), (37, 8, 3), (200, 32, 2), (777, 64, 4), (1500, 64, 2),
+                                       (2049, 64, 1)])
+def test_lu_fp64_matches_lapack(n, jb, nrhs):
     import scipy.linalg as sla
 
     rng = np.random.default_rng(n)
     A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
-    B = rng.standard_normal((n, 2)) + 1j * rng.standard_normal((n, 2))
+    B = rng.standard_normal((n, nrhs)) + 1j * rng.standard_normal((n, nrhs))
     X, ipiv, info = _lu_solve(A, B, 64, jb)
     assert info == 0
     ref = np.linalg.solve(A, B)
