@@ -3,14 +3,16 @@
 
 Workload (default): the 40 x 20 mm via-stitched grounded-coplanar pour
 (`workloads.gcpw40`, 12 104 RWG + 324 via unknowns = 12 428, 51 frequencies
-1-20 GHz).  A step is one frequency point of the sweep.  The headline path
-is the mixed-precision one (FP64 fill, complex64 LU on the matrix cores,
-FP64 iterative refinement); the plain FP64 path (complex128 LU) is timed
-beside it on the same frequencies, and |dS| between the two is reported.
-Inputs (mesh, fill plan) are resident on the GPU before the timed region;
-the Green's tables of the timed frequencies are built inside it.
+1-20 GHz).  A step is one frequency point of the sweep: Green's tables, FP64
+fill, LU + port solve, de-embedding to S.  The headline path is chosen with
+--precision (default: the plain FP64 path, complex128 LU); the other path
+(FP64 fill, complex64 LU on the matrix cores as the preconditioner of FP64
+GMRES refinement) is timed beside it on the same frequencies and |dS|
+between the two is reported.  Inputs (mesh, fill plan) are resident on the
+GPU before the timed region; the Green's tables of the timed frequencies are
+built inside it.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--precision fp64|mixed]
 
 Prints one JSON line (rank 0).  More than one GPU is not part of this build.
 """
@@ -114,7 +116,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="gcpw40")
     ap.add_argument("--cpu-sample", type=int, default=1, help="frequency points timed on the CPU oracle (0: skip)")
-    ap.add_argument("--no-fp64", action="store_true", help="skip timing the plain FP64 path")
+    ap.add_argument("--precision", default="fp64", choices=("fp64", "mixed"), help="headline path")
+    ap.add_argument("--no-alt", action="store_true", help="skip timing the other path")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -132,31 +135,31 @@ def main():
     print(f"workload {args.workload}: N={N} ({mesh.n_basis} RWG + {mesh.n_vias} vias), {len(freqs)} f", file=sys.stderr,
           flush=True)
 
-    sm, rm, tm = time_path(mesh, sub, freqs, "mixed", args.warmup, args.steps, torch)
-    print(f"mixed: {args.steps} points in {tm:.3f} s, refinement its {sm.last_refine}", file=sys.stderr, flush=True)
-    res = {}
-    if not args.no_fp64:
-        s64, r64, t64 = time_path(mesh, sub, freqs, "fp64", args.warmup, args.steps, torch)
-        res["fp64"] = dict(value=args.steps / t64, ms_per_step=1e3 * t64 / args.steps,
-                           dS_max=float(np.max(np.abs(rm.S - r64.S))))
-        print(f"fp64: {args.steps} points in {t64:.3f} s; |dS| mixed vs fp64 {res['fp64']['dS_max']:.2e}",
-              file=sys.stderr, flush=True)
+    prec = args.precision
+    alt = "mixed" if prec == "fp64" else "fp64"
+    sm, rm, tm = time_path(mesh, sub, freqs, prec, args.warmup, args.steps, torch)
+    print(f"{prec}: {args.steps} points in {tm:.3f} s ({sm.last_refine} refinement steps)", file=sys.stderr, flush=True)
+    other = None
+    if not args.no_alt:
+        sa, ra, ta = time_path(mesh, sub, freqs, alt, args.warmup, args.steps, torch)
+        other = dict(precision=alt, value=args.steps / ta, ms_per_step=1e3 * ta / args.steps,
+                     dS_max_vs_headline=float(np.max(np.abs(rm.S - ra.S))), refinement_steps_last=sa.last_refine)
+        print(f"{alt}: {args.steps} points in {ta:.3f} s; |dS| {other['dS_max_vs_headline']:.2e}", file=sys.stderr,
+              flush=True)
+        del sa
 
     f_probe = freqs[len(freqs) // 2]
     calls, secs, flops = gemm_roofline(sm, f_probe, torch)
     achieved = flops / secs if secs > 0 else 0.0
-    roof = dict(bound="mfma", achieved=achieved / 1e12, peak=PEAK["fp32_mfma"] / 1e12, unit="TFLOP/s",
-                frac=achieved / PEAK["fp32_mfma"], traffic=None,
-                kernel="cgemm_sub_kernel<float> (LU trailing update, complex64)",
+    peak = PEAK["fp64_mfma"] if prec == "fp64" else PEAK["fp32_mfma"]
+    kname = "cgemm_sub_kernel<double, 64, 128>" if prec == "fp64" else "cgemm_sub_kernel<float, 64, 128>"
+    roof = dict(bound="mfma", achieved=achieved / 1e12, peak=peak / 1e12, unit="TFLOP/s", frac=achieved / peak,
+                traffic=None, kernel=f"{kname} (LU trailing update)",
                 launches=calls, avg_launch_ms=1e3 * secs / max(calls, 1), flops_per_launch=flops / max(calls, 1))
     tf = fill_time(sm, f_probe, torch)
     fill_flops = sm.geometry().algorithmic_flops()
     extra = dict(fill=dict(seconds=tf, algorithmic_flops=fill_flops, achieved_tflops=fill_flops / tf / 1e12,
                            frac_fp64_valu=fill_flops / tf / PEAK["fp64_valu"]))
-    if not args.no_fp64:
-        c2, s2, f2 = gemm_roofline(s64, f_probe, torch)
-        extra["lu_fp64_gemm"] = dict(achieved_tflops=f2 / s2 / 1e12, frac=f2 / s2 / PEAK["fp64_mfma"],
-                                     launches=c2, seconds=s2)
     cpu = None
     if args.cpu_sample > 0:
         cpu = cpu_baseline(mesh, sub, freqs, args.cpu_sample)
@@ -173,14 +176,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 fill / c64 LU + f64 refinement",
+        "dtype": "f64 (complex128 fill and LU)" if prec == "fp64" else "f64 fill / c64 LU + f64 GMRES refinement",
         "data": "synthetic (seeded / deterministic layouts)",
         "config": {"workload": f"{args.workload}: 40x20 mm via-stitched GCPW pour, N={N} unknowns, "
                                f"{len(freqs)} f 1-20 GHz" if args.workload == "gcpw40" else f"{args.workload}: N={N}",
-                   "N": N, "precision": "mixed", "parallelism": "1 GPU"},
+                   "N": N, "precision": prec, "parallelism": "1 GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "fp64_path": res.get("fp64"),
+        "other_path": other,
         "extra": extra,
     }
     print(json.dumps(line), flush=True)

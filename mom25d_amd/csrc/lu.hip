@@ -498,6 +498,16 @@ __device__ __forceinline__ int packed_index(int i, int k) {
   return LOWER ? k * (2 * kTrsvBS - k - 1) / 2 + (i - k - 1) : k * (k + 1) / 2 + i;
 }
 
+// Value of lane k (uniform k) in every lane, through the scalar unit.
+__device__ __forceinline__ double lane_value(double v, int k) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), k);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), k);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float lane_value(float v, int k) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+
 template <typename R, bool LOWER, int NR>
 __global__ __launch_bounds__(256) void trsv_kernel(int n, const cplx<R>* __restrict__ A, int64_t lda, cplx<R>* X,
                                                    unsigned* progress, unsigned* ticket) {
@@ -531,6 +541,15 @@ __global__ __launch_bounds__(256) void trsv_kernel(int n, const cplx<R>* __restr
 #pragma unroll
   for (int c = 0; c < NR; ++c) acc[c] = {0, 0};
   const unsigned need = LOWER ? (unsigned)blk : (unsigned)(nblk - 1 - blk);  // finished blocks needed
+  // This thread's 16 entries of A[row, block d] for the next dependency d,
+  // loaded while the workgroup waits for block d to be published.
+  cplx<R> la[16];
+  auto fetch = [&](unsigned d) {
+    const int j0 = (LOWER ? (int)d : nblk - 1 - (int)d) * BS + 16 * q;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) la[e] = (r < b && j0 + e < n) ? A[(int64_t)row * lda + j0 + e] : cplx<R>{0, 0};
+  };
+  if (need) fetch(0);
   unsigned done = 0;
   while (done < need) {
     if (tid == 0) {
@@ -541,24 +560,17 @@ __global__ __launch_bounds__(256) void trsv_kernel(int n, const cplx<R>* __restr
     __syncthreads();
     const unsigned upto = s_ready;
     for (unsigned d = done; d < upto; ++d) {
-      const int jb = LOWER ? (int)d : nblk - 1 - (int)d;
-      const int j0 = jb * BS;
+      const int j0 = (LOWER ? (int)d : nblk - 1 - (int)d) * BS;
       if (tid < BS * NR) {
         const int i = tid / NR, c = tid % NR;
         xs[i][c] = (j0 + i < n) ? load_coherent_c(X + (int64_t)(j0 + i) * NR + c) : cplx<R>{0, 0};
       }
       __syncthreads();
-      if (r < b) {
-        const cplx<R>* Lrow = A + (int64_t)row * lda + j0 + 16 * q;
 #pragma unroll
-        for (int e = 0; e < 16; ++e) {
-          if (j0 + 16 * q + e < n) {
-            const cplx<R> l = Lrow[e];
+      for (int e = 0; e < 16; ++e)
 #pragma unroll
-            for (int c = 0; c < NR; ++c) acc[c] += l * xs[16 * q + e][c];
-          }
-        }
-      }
+        for (int c = 0; c < NR; ++c) acc[c] += la[e] * xs[16 * q + e][c];
+      if (d + 1 < need) fetch(d + 1);
       __syncthreads();
     }
     done = upto;
@@ -567,7 +579,8 @@ __global__ __launch_bounds__(256) void trsv_kernel(int n, const cplx<R>* __restr
   for (int c = 0; c < NR; ++c) part[q][r][c] = acc[c];
   __syncthreads();
   if (tid >= 64) return;
-  // wavefront 0: v = b - sum, then substitution with the diagonal block
+  // wavefront 0: v = b - sum, then substitution with the diagonal block; x_k
+  // is read from lane k by every lane (uniform k: scalar broadcast)
   const int i = tid;
   cplx<R> v[NR];
 #pragma unroll
@@ -577,30 +590,22 @@ __global__ __launch_bounds__(256) void trsv_kernel(int n, const cplx<R>* __restr
   if (LOWER) {
 #pragma unroll
     for (int k = 0; k < BS - 1; ++k) {
-      cplx<R> xk[NR];
+      const cplx<R> l = i > k ? Dk[packed_index<true>(i, k)] : cplx<R>{0, 0};
 #pragma unroll
-      for (int c = 0; c < NR; ++c) xk[c] = {__shfl(v[c].re, k, 64), __shfl(v[c].im, k, 64)};
-      if (i > k) {
-        const cplx<R> l = Dk[packed_index<true>(i, k)];
-#pragma unroll
-        for (int c = 0; c < NR; ++c) v[c] -= l * xk[c];
+      for (int c = 0; c < NR; ++c) {
+        const cplx<R> xk = {lane_value(v[c].re, k), lane_value(v[c].im, k)};
+        v[c] -= l * xk;
       }
     }
   } else {
 #pragma unroll
     for (int k = BS - 1; k >= 0; --k) {
-      if (i == k) {
-        const cplx<R> rd = Dk[packed_index<false>(k, k)];
+      const cplx<R> rd = Dk[packed_index<false>(k, k)];
+      const cplx<R> u = i < k ? Dk[packed_index<false>(i, k)] : cplx<R>{0, 0};
 #pragma unroll
-        for (int c = 0; c < NR; ++c) v[c] = v[c] * rd;
-      }
-      cplx<R> xk[NR];
-#pragma unroll
-      for (int c = 0; c < NR; ++c) xk[c] = {__shfl(v[c].re, k, 64), __shfl(v[c].im, k, 64)};
-      if (i < k) {
-        const cplx<R> u = Dk[packed_index<false>(i, k)];
-#pragma unroll
-        for (int c = 0; c < NR; ++c) v[c] -= u * xk[c];
+      for (int c = 0; c < NR; ++c) {
+        const cplx<R> xk = cplx<R>{lane_value(v[c].re, k), lane_value(v[c].im, k)} * rd;
+        v[c] = i == k ? xk : v[c] - u * xk;
       }
     }
   }

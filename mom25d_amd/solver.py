@@ -289,6 +289,26 @@ def _check_abi():
         raise RuntimeError("libmom25d PairTab layout differs from the Python mirror")
 
 
+def givens_column(h, cs, sn, g, j):
+    """Bring column j of an upper-Hessenberg least-squares problem to upper
+    triangular form in place (GMRES; one problem per right-hand side along the
+    last axis): apply the earlier rotations G_i = [[c_i*, s_i*], [-s_i, c_i]]
+    to h (length j + 2), make the rotation that zeroes h[j + 1], and apply it
+    to the right-hand side g.  |g[j + 1]| is then the least-squares residual."""
+    for i in range(j):
+        t = np.conj(cs[i]) * h[i] + np.conj(sn[i]) * h[i + 1]
+        h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1]
+        h[i] = t
+    den = np.sqrt(np.abs(h[j]) ** 2 + np.abs(h[j + 1]) ** 2)
+    den = np.where(den == 0, 1.0, den)
+    cs[j] = h[j] / den
+    sn[j] = h[j + 1] / den
+    h[j] = np.conj(cs[j]) * h[j] + np.conj(sn[j]) * h[j + 1]
+    h[j + 1] = 0
+    g[j + 1] = -sn[j] * g[j]
+    g[j] = np.conj(cs[j]) * g[j]
+
+
 class Solver:
     """Solver(mesh, Substrate | Stackup).sweep(freqs) -> SweepResult.
 
@@ -570,18 +590,7 @@ class Solver:
                 Vb[j + 1] = w / nw
                 hc = hj.cpu().numpy()
                 its += 1
-                for i in range(j):  # Givens rotations, per column
-                    t = cs[i] * hc[i] + sn[i] * hc[i + 1]
-                    hc[i + 1] = -np.conj(sn[i]) * hc[i] + np.conj(cs[i]) * hc[i + 1]
-                    hc[i] = t
-                den = np.sqrt(np.abs(hc[j]) ** 2 + np.abs(hc[j + 1]) ** 2)
-                den = np.where(den == 0, 1.0, den)
-                cs[j] = hc[j] / den
-                sn[j] = hc[j + 1] / den
-                hc[j] = cs[j].conj() * hc[j] + sn[j].conj() * hc[j + 1]
-                hc[j + 1] = 0
-                g[j + 1] = -sn[j] * g[j]
-                g[j] = cs[j].conj() * g[j]
+                givens_column(hc, cs, sn, g, j)
                 H[:j + 2, j] = hc
                 k = j + 1
                 if np.all(np.abs(g[j + 1]) <= 0.25 * target):

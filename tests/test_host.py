@@ -115,3 +115,33 @@ def test_error_path_reports_message():
         _lib.call("mom_cgemm_sub", 16, 4, 4, 4, None, 4, None, 4, None, 4, None)
     with pytest.raises(RuntimeError, match="ncol must be"):
         _lib.call("mom_sommerfeld", 1, None, None, None, None, None, None, None, None, None, None, 5, None, None)
+
+
+def test_gmres_givens_solves_hessenberg_least_squares():
+    """The rotations of the GMRES host loop reduce a complex upper-Hessenberg
+    least-squares problem min |beta e1 - H y| column by column; the
+    triangular solve must match numpy's lstsq, and |g[k]| its residual."""
+    from mom25d_amd.solver import givens_column
+
+    rng = np.random.default_rng(3)
+    m, nrhs = 12, 2
+    H = rng.standard_normal((m + 1, m, nrhs)) + 1j * rng.standard_normal((m + 1, m, nrhs))
+    for c in range(m):
+        H[c + 2:, c] = 0
+    beta = np.array([1.7, 0.4])
+    g = np.zeros((m + 1, nrhs), complex)
+    g[0] = beta
+    cs = np.zeros((m, nrhs), complex)
+    sn = np.zeros((m, nrhs), complex)
+    R = np.zeros_like(H)
+    for j in range(m):
+        h = H[:j + 2, j].copy()
+        givens_column(h, cs, sn, g, j)
+        R[:j + 2, j] = h
+    for c in range(nrhs):
+        y = np.linalg.solve(np.triu(R[:m, :m, c]), g[:m, c])
+        e1 = np.zeros(m + 1, complex)
+        e1[0] = beta[c]
+        ref, res, *_ = np.linalg.lstsq(H[:, :, c], e1, rcond=None)
+        np.testing.assert_allclose(y, ref, rtol=1e-10, atol=1e-12)
+        assert abs(abs(g[m, c]) - np.linalg.norm(e1 - H[:, :, c] @ ref)) < 1e-12
