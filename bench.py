@@ -43,10 +43,10 @@ def _cpu_model():
     return platform.processor() or "unknown"
 
 
-def time_path(mesh, sub, freqs, precision, warmup, steps, torch):
+def time_path(mesh, sub, freqs, precision, warmup, steps, torch, lanes=None):
     from mom25d_amd import Solver
 
-    s = Solver(mesh, sub, verbose=False, precision=precision)
+    s = Solver(mesh, sub, verbose=False, precision=precision, lanes=lanes)
     s.geometry().upload(s.device)
     nf = len(freqs)
     pick = lambda a, b: freqs[np.arange(a, b) % nf]  # noqa: E731
@@ -118,6 +118,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1, help="frequency points timed on the CPU oracle (0: skip)")
     ap.add_argument("--precision", default="fp64", choices=("fp64", "mixed"), help="headline path")
     ap.add_argument("--no-alt", action="store_true", help="skip timing the other path")
+    ap.add_argument("--lanes", type=int, default=None, help="frequency points in flight (default: automatic)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -137,11 +138,12 @@ def main():
 
     prec = args.precision
     alt = "mixed" if prec == "fp64" else "fp64"
-    sm, rm, tm = time_path(mesh, sub, freqs, prec, args.warmup, args.steps, torch)
+    sm, rm, tm = time_path(mesh, sub, freqs, prec, args.warmup, args.steps, torch, args.lanes)
+    lanes = sm._lane_count()
     print(f"{prec}: {args.steps} points in {tm:.3f} s ({sm.last_refine} refinement steps)", file=sys.stderr, flush=True)
     other = None
     if not args.no_alt:
-        sa, ra, ta = time_path(mesh, sub, freqs, alt, args.warmup, args.steps, torch)
+        sa, ra, ta = time_path(mesh, sub, freqs, alt, args.warmup, args.steps, torch, args.lanes)
         other = dict(precision=alt, value=args.steps / ta, ms_per_step=1e3 * ta / args.steps,
                      dS_max_vs_headline=float(np.max(np.abs(rm.S - ra.S))), refinement_steps_last=sa.last_refine)
         print(f"{alt}: {args.steps} points in {ta:.3f} s; |dS| {other['dS_max_vs_headline']:.2e}", file=sys.stderr,
@@ -180,7 +182,7 @@ def main():
         "data": "synthetic (seeded / deterministic layouts)",
         "config": {"workload": f"{args.workload}: 40x20 mm via-stitched GCPW pour, N={N} unknowns, "
                                f"{len(freqs)} f 1-20 GHz" if args.workload == "gcpw40" else f"{args.workload}: N={N}",
-                   "N": N, "precision": prec, "parallelism": "1 GPU"},
+                   "N": N, "precision": prec, "lanes": lanes, "parallelism": "1 GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "other_path": other,

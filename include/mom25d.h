@@ -81,11 +81,15 @@ int mom_line_voltage(int npts, const double* pts, int li, int nl, int nt, int nr
 /* LU with partial pivoting of an n x n complex matrix in place (prec 64:
  * complex128, 32: complex64), panel width jb; ipiv[k] = row swapped with
  * row k (0-based); info[0] = first zero pivot + 1 or 0.  `work` holds
- * mom_getrf_work_bytes(prec, n, jb) bytes.  Replaces scipy.linalg.lu_factor
- * (LAPACK zgetrf) of _dense_solve, Engine/mom25d/solver.py:11-23. */
+ * mom_getrf_work_bytes(prec, n, jb) bytes.  max_groups bounds the
+ * workgroups of the panel kernel, whose workgroups must all be resident at
+ * once (0: all compute units); a caller that factors L matrices concurrently
+ * on L streams passes CUs / L so that every panel can be resident together.
+ * Replaces scipy.linalg.lu_factor (LAPACK zgetrf) of _dense_solve,
+ * Engine/mom25d/solver.py:11-23. */
 int64_t mom_getrf_work_bytes(int prec, int n, int jb);
 int mom_getrf(int prec, int n, void* A, int64_t lda, int32_t* ipiv, int32_t* info, void* work, int64_t work_bytes,
-              int jb, void* stream);
+              int jb, int max_groups, void* stream);
 
 /* Row permutation of the factorisation: perm[i] = original row of row i. */
 int mom_build_perm(int n, const int32_t* ipiv, int32_t* perm, void* stream);

@@ -30,7 +30,7 @@ SIGNATURES = {
     "mom_symmetrize": (_i, [_i, _p, _i64, _p, _i64, _p, _p]),
     "mom_diag_scale": (_i, [_i, _p, _i64, _p, _p]),
     "mom_getrf_work_bytes": (_i64, [_i, _i, _i]),
-    "mom_getrf": (_i, [_i, _i, _p, _i64, _p, _p, _p, _i64, _i, _p]),
+    "mom_getrf": (_i, [_i, _i, _p, _i64, _p, _p, _p, _i64, _i, _i, _p]),
     "mom_build_perm": (_i, [_i, _p, _p, _p]),
     "mom_getrs_work_bytes": (_i64, [_i, _i]),
     "mom_getrs": (_i, [_i, _i, _i, _p, _i64, _p, _p, _p, _p, _p]),

@@ -21,6 +21,8 @@
 #include "common.h"
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <vector>
 
 namespace mom {
@@ -757,21 +759,26 @@ static size_t getrf_work_bytes(int n, int jb) {
   return b + 1024;
 }
 
-// Side stream and events of the look-ahead (one set per process, created on
-// first use; the side stream has the default priority).
+// Side stream and events of the look-ahead, one set per caller stream
+// (created on first use and kept until the process ends), so that
+// factorisations enqueued on different streams never share a side stream.
 struct Lookahead {
   hipStream_t side = nullptr;
   std::vector<hipEvent_t> ev;
 };
-static Lookahead g_la;
+static std::mutex g_la_mu;
+static std::map<hipStream_t, Lookahead> g_la;
 
-static int lookahead_events(size_t need) {
-  if (!g_la.side) MOM_TRY(hipStreamCreateWithFlags(&g_la.side, hipStreamNonBlocking));
-  while (g_la.ev.size() < need) {
+static int lookahead_for(hipStream_t s, size_t need, Lookahead** out) {
+  std::lock_guard<std::mutex> lock(g_la_mu);
+  Lookahead& la = g_la[s];
+  if (!la.side) MOM_TRY(hipStreamCreateWithFlags(&la.side, hipStreamNonBlocking));
+  while (la.ev.size() < need) {
     hipEvent_t e;
     MOM_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    g_la.ev.push_back(e);
+    la.ev.push_back(e);
   }
+  *out = &la;
   return 0;
 }
 
@@ -779,7 +786,7 @@ static size_t g_panel_lds_max = 0;
 
 template <typename R>
 static int launch_panel(cplx<R>* A, int64_t lda, int n, int k0, int b, int32_t* ipiv, int32_t* info,
-                        PivotSlot* slots, cplx<R>* slot_row, cplx<R>* diag_row, int ncu, hipStream_t s) {
+                        PivotSlot* slots, cplx<R>* slot_row, cplx<R>* diag_row, int gmax, hipStream_t s) {
   const int m = n - k0;
   const size_t lds_max = 120 * 1024;
   if (g_panel_lds_max == 0) {
@@ -789,7 +796,6 @@ static int launch_panel(cplx<R>* A, int64_t lda, int n, int k0, int b, int32_t* 
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max));
     g_panel_lds_max = lds_max;
   }
-  const int gmax = std::min(ncu, 256);  // slot arrays hold 256 workgroups
   // as few workgroups as the LDS allows (fewer slots to poll per column)
   int rp = std::max(1, (int)(lds_max / ((size_t)b * sizeof(cplx<R>))));
   int G = (m + rp - 1) / rp;
@@ -827,12 +833,14 @@ static int launch_update(cplx<R>* A, int64_t lda, int k0, int b, int r0, int r1,
 // were updated first) is factored on the side stream.
 template <typename R>
 static int getrf(int n, cplx<R>* A, int64_t lda, int32_t* ipiv, int32_t* info, void* work, size_t work_bytes, int jb,
-                 hipStream_t s) {
+                 int max_groups, hipStream_t s) {
   if (n <= 0) return 0;
   if (jb <= 0 || jb > 128 || (jb % 8) != 0) return fail("mom_getrf: panel width must be a multiple of 8 in [8, 128]");
   if (work_bytes < getrf_work_bytes<R>(n, jb)) return fail("mom_getrf: workspace too small");
   const int ncu = device_cus();
   if (ncu <= 0) return fail("mom_getrf: no device");
+  // slot arrays hold 256 workgroups
+  const int gmax = std::min(std::min(ncu, 256), max_groups > 0 ? max_groups : 256);
   unsigned char* w = reinterpret_cast<unsigned char*>(work);
   auto take = [&](size_t bytes) {
     unsigned char* p = w;
@@ -849,15 +857,16 @@ static int getrf(int n, cplx<R>* A, int64_t lda, int32_t* ipiv, int32_t* info, v
   if ((rc = check_launch("zero_u32_kernel"))) return rc;
   hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, reinterpret_cast<unsigned*>(info), 1);
   if ((rc = check_launch("zero_u32_kernel"))) return rc;
-  if ((rc = lookahead_events(2 * (size_t)npanel + 2))) return rc;
-  hipStream_t side = g_la.side;
+  Lookahead* la = nullptr;
+  if ((rc = lookahead_for(s, 2 * (size_t)npanel + 2, &la))) return rc;
+  hipStream_t side = la->side;
   // panel 0 on the caller's stream
-  if ((rc = launch_panel<R>(A, lda, n, 0, std::min(jb, n), ipiv, info, slots, slot_row, diag_row, ncu, s)))
+  if ((rc = launch_panel<R>(A, lda, n, 0, std::min(jb, n), ipiv, info, slots, slot_row, diag_row, gmax, s)))
     return rc;
   for (int p = 0; p < npanel; ++p) {
     const int k0 = p * jb, b = std::min(jb, n - k0);
     const int c0 = k0 + b;
-    if (p > 0) MOM_TRY(hipStreamWaitEvent(s, g_la.ev[2 * p], 0));  // panel p done on the side stream
+    if (p > 0) MOM_TRY(hipStreamWaitEvent(s, la->ev[2 * p], 0));  // panel p done on the side stream
     if (k0 > 0) {
       hipLaunchKernelGGL(laswp_kernel<R>, dim3((k0 + 15) / 16), dim3(256), 0, s, A, lda, 0, k0, k0, b, ipiv);
       if ((rc = check_launch("laswp_kernel"))) return rc;
@@ -868,10 +877,10 @@ static int getrf(int n, cplx<R>* A, int64_t lda, int32_t* ipiv, int32_t* info, v
     const int b1 = std::min(jb, n - c0);
     // next panel's columns first, then hand the panel to the side stream
     if ((rc = launch_update<R>(A, lda, k0, b, c0, n, c0, c0 + b1, s))) return rc;
-    MOM_TRY(hipEventRecord(g_la.ev[2 * p + 1], s));
-    MOM_TRY(hipStreamWaitEvent(side, g_la.ev[2 * p + 1], 0));
-    if ((rc = launch_panel<R>(A, lda, n, c0, b1, ipiv, info, slots, slot_row, diag_row, ncu, side))) return rc;
-    MOM_TRY(hipEventRecord(g_la.ev[2 * (p + 1)], side));
+    MOM_TRY(hipEventRecord(la->ev[2 * p + 1], s));
+    MOM_TRY(hipStreamWaitEvent(side, la->ev[2 * p + 1], 0));
+    if ((rc = launch_panel<R>(A, lda, n, c0, b1, ipiv, info, slots, slot_row, diag_row, gmax, side))) return rc;
+    MOM_TRY(hipEventRecord(la->ev[2 * (p + 1)], side));
     // the rest of the trailing matrix
     if ((rc = launch_update<R>(A, lda, k0, b, c0, n, c0 + b1, n, s))) return rc;
   }
@@ -915,12 +924,12 @@ extern "C" int64_t mom_getrf_work_bytes(int prec, int n, int jb) {
 }
 
 extern "C" int mom_getrf(int prec, int n, void* A, int64_t lda, int32_t* ipiv, int32_t* info, void* work,
-                         int64_t work_bytes, int jb, void* stream) {
+                         int64_t work_bytes, int jb, int max_groups, void* stream) {
   if (!A || !ipiv || !info || !work) return fail("mom_getrf: null device pointer");
   if (lda < n) return fail("mom_getrf: lda < n");
   hipStream_t s = (hipStream_t)stream;
-  if (prec == 64) return getrf<double>(n, (z64*)A, lda, ipiv, info, work, (size_t)work_bytes, jb, s);
-  if (prec == 32) return getrf<float>(n, (c32*)A, lda, ipiv, info, work, (size_t)work_bytes, jb, s);
+  if (prec == 64) return getrf<double>(n, (z64*)A, lda, ipiv, info, work, (size_t)work_bytes, jb, max_groups, s);
+  if (prec == 32) return getrf<float>(n, (c32*)A, lda, ipiv, info, work, (size_t)work_bytes, jb, max_groups, s);
   return fail("mom_getrf: prec must be 64 (complex128) or 32 (complex64)");
 }
 

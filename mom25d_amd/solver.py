@@ -316,13 +316,19 @@ class Solver:
                 "mixed" -- complex64 LU on the matrix cores with FP64
                 iterative refinement against the FP64 impedance matrix.
     method    : "auto" / "dense" (the H-matrix solver is not part of this build).
+    lanes     : frequency points in flight during a sweep, each on its own
+                stream with its own matrices (None: 2 when they fit in a
+                third of the free device memory, else 1).  The latency-bound
+                steps of one point (LU panels, triangular solves, host
+                de-embedding) then overlap the matrix-core work of another.
     """
 
     OPEN = 1e12
     HMATRIX_N = 7000
 
     def __init__(self, mesh, substrate, z0=None, verbose=True, method="auto", v_weight=None, z0_recip_min=None,
-                 tand_floor=None, precision="fp64", device="cuda", panel=None, refine_tol=1e-15, refine_maxit=20):
+                 tand_floor=None, precision="fp64", device="cuda", panel=None, refine_tol=1e-15, refine_maxit=20,
+                 lanes=None):
         from .layered import Stackup
 
         if method not in ("auto", "dense"):
@@ -360,8 +366,10 @@ class Solver:
         self.refine_tol = refine_tol
         self.refine_maxit = refine_maxit
         self.gmres_restart = 30
+        self.lanes = lanes
         self._plan = None
         self._bufs = {}
+        self._keep = {}
         _check_abi()
         self.groups = {}
         for p, port in enumerate(self.ports):
@@ -389,13 +397,43 @@ class Solver:
     def greens(self, freq):
         return self.tables([freq])[0]
 
-    def _buf(self, name, shape, dtype):
-        b = self._bufs.get(name)
+    def _buf(self, name, shape, dtype, lane=0):
+        """Device buffer kept between frequencies (one set per lane)."""
+        key = (name, lane)
+        b = self._bufs.get(key)
         if b is None or tuple(b.shape) != tuple(shape) or b.dtype != dtype:
-            self._bufs.pop(name, None)
+            self._bufs.pop(key, None)
             b = torch.empty(shape, dtype=dtype, device=self.device)
-            self._bufs[name] = b
+            self._bufs[key] = b
         return b
+
+    PANEL_LDS = 120 * 1024  # LDS of one LU panel workgroup (the library's bound)
+
+    def _lane_count(self):
+        """Lanes of a sweep: as asked (default 2 when the matrices of two
+        lanes fit in a third of the free device memory), and no more than
+        allow every lane's LU panel to be resident at once."""
+        n = self.mesh.n_unknowns
+        if self.lanes is not None:
+            lanes = max(1, int(self.lanes))
+        else:
+            per = n * n * (16 + (8 if self.precision == "mixed" else 0))
+            free, _total = torch.cuda.mem_get_info(self.device)
+            lanes = 2 if 2 * per < free / 3 else 1
+        elem = 16 if self.precision == "fp64" else 8
+        rows = max(1, self.PANEL_LDS // (self.panel * elem))
+        groups = -(-n // rows)
+        cus = torch.cuda.get_device_properties(self.device).multi_processor_count
+        return max(1, min(lanes, min(cus, 256) // groups))
+
+    @staticmethod
+    def _run(gen):
+        """Drive a solve generator to its end on the current stream."""
+        try:
+            while True:
+                next(gen)
+        except StopIteration as stop:
+            return stop.value
 
     # --------------------------------------------------------------- fill
     def _via_extra(self, g, zs_via):
@@ -429,9 +467,10 @@ class Solver:
             E += np.diag([zs_via * l / (np.pi * v.drill) for l, v in zip(lengths, plan.vias)])
         return E
 
-    def _fill(self, g, Z, Z32=None):
+    def _fill(self, g, Z, Z32=None, lane=0):
         """Z (device complex128, zeroed here) <- the symmetric impedance matrix
-        at g.freq; Z32 (complex64) receives a copy when given."""
+        at g.freq; Z32 (complex64) receives the diagonally scaled copy when
+        given.  Enqueued on the current stream."""
         plan = self.geometry()
         d = plan.upload(self.device)
         n = self.mesh.n_unknowns
@@ -442,7 +481,7 @@ class Solver:
         nv = plan.nv
         ivpad = None
         if nv:
-            ivpad = self._buf("ivpad", (plan.pad_tris.size, plan.nt), torch.complex128)
+            ivpad = self._buf("ivpad", (plan.pad_tris.size, plan.nt), torch.complex128, lane)
             ivpad.zero_()
         P = _lib.ptr
         tabs = g.tables
@@ -474,10 +513,10 @@ class Solver:
             _lib.call("mom_via_blocks", plan.nb, nv, plan.nt, P(ivpad), P(d["d_off"]), P(d["d_pad"]), P(d["d_val"]),
                       P(d["pad_tri"]), P(d["tp"]), P(d["vp"]), P(d["tm"]), P(d["vm"]), P(d["cV"]), P(E),
                       ijwe.real, ijwe.imag, P(Z), Z.stride(0), stream)
-            self._keep = E
+            self._keep[lane] = E
         dsc = None
         if Z32 is not None:
-            dsc = self._buf("dscale", (n,), torch.float64)
+            dsc = self._buf("dscale", (n,), torch.float64, lane)
             _lib.call("mom_diag_scale", n, P(Z), Z.stride(0), P(dsc), stream)
         _lib.call("mom_symmetrize", n, P(Z), Z.stride(0), P(Z32), Z32.stride(0) if Z32 is not None else 0, P(dsc),
                   stream)
@@ -495,32 +534,42 @@ class Solver:
     def port_excitations(self):
         return np.stack([pm.excitation for pm in self.pm], 1)
 
-    def _lu_solve(self, Z, B, Z32=None):
-        """X = Z^-1 B on the device; Z (complex128) is overwritten by its LU
-        in the fp64 path; in the mixed path Z32 holds the complex64 LU and Z
-        stays intact for the refinement residuals."""
+    def _lu_solve(self, Z, B, Z32=None, lane=0):
+        """Generator: X = Z^-1 B on the device (the value it returns).  Z
+        (complex128) is overwritten by its LU in the fp64 path; in the mixed
+        path Z32 holds the complex64 LU and Z stays intact for the refinement
+        residuals.  It yields wherever the host would wait for the GPU."""
         n, nrhs = B.shape
         stream = _lib.stream_ptr()
         P = _lib.ptr
         jb = self.panel
         prec = 64 if Z32 is None else 32
         A = Z if Z32 is None else Z32
-        ipiv = self._buf("ipiv", (n,), torch.int32)
-        perm = self._buf("perm", (n,), torch.int32)
-        info = self._buf("info", (4,), torch.int32)
+        ipiv = self._buf("ipiv", (n,), torch.int32, lane)
+        perm = self._buf("perm", (n,), torch.int32, lane)
+        info = self._buf("info", (4,), torch.int32, lane)
         wbytes = int(_lib.lib().mom_getrf_work_bytes(prec, n, jb))
-        work = self._buf(f"work{prec}", (wbytes,), torch.uint8)
-        _lib.call("mom_getrf", prec, n, P(A), A.stride(0), P(ipiv), P(info), P(work), wbytes, jb, stream)
+        work = self._buf(f"work{prec}", (wbytes,), torch.uint8, lane)
+        groups = self._panel_groups()
+        _lib.call("mom_getrf", prec, n, P(A), A.stride(0), P(ipiv), P(info), P(work), wbytes, jb, groups, stream)
         _lib.call("mom_build_perm", n, P(ipiv), P(perm), stream)
-        sw = self._buf(f"swork{prec}", (int(_lib.lib().mom_getrs_work_bytes(prec, n)),), torch.uint8)
+        sw = self._buf(f"swork{prec}", (int(_lib.lib().mom_getrs_work_bytes(prec, n)),), torch.uint8, lane)
         X = torch.empty((n, nrhs), dtype=torch.complex128, device=self.device)
         if Z32 is None:
             _lib.call("mom_getrs", 64, n, nrhs, P(A), A.stride(0), P(perm), P(B), P(X), P(sw), stream)
             self.last_refine = 0
-            return X, info
-        return self._gmres_ir(Z, A, perm, sw, B), info
+            return X
+        return (yield from self._gmres_ir(Z, A, perm, sw, B, lane))
 
-    def _gmres_ir(self, Z, A32, perm, sw, B):
+    def _panel_groups(self):
+        """Workgroup bound of the LU panel: every lane's panel resident at once."""
+        lanes = getattr(self, "_active_lanes", 1)
+        if lanes <= 1:
+            return 0
+        cus = torch.cuda.get_device_properties(self.device).multi_processor_count
+        return max(1, cus // lanes)
+
+    def _gmres_ir(self, Z, A32, perm, sw, B, lane=0):
         """Restarted GMRES in FP64 on the diagonally scaled system
         (D Z D) y = D b, x = D y, right-preconditioned by the complex64 LU of
         D Z D.  All right-hand sides advance together (one Krylov basis per
@@ -531,11 +580,11 @@ class Solver:
         n, nrhs = B.shape
         P = _lib.ptr
         stream = _lib.stream_ptr()
-        dv = self._bufs["dscale"]
+        dv = self._bufs[("dscale", lane)]
         d = dv[:, None]
         m = self.gmres_restart
         zeros = torch.zeros_like(B)
-        rows = self._buf("rowsum", (n,), torch.float64)
+        rows = self._buf("rowsum", (n,), torch.float64, lane)
         Bs = d * B
 
         def precond(V):
@@ -556,6 +605,7 @@ class Solver:
 
         Y = precond(Bs)
         R = residual(Y, rows)
+        yield
         znorm = float(rows.max())
         its = 0
         prev = None
@@ -563,8 +613,10 @@ class Solver:
         Wb = torch.empty((m, n, nrhs), dtype=torch.complex128, device=self.device)
         for _restart in range(self.refine_maxit):
             beta = torch.linalg.vector_norm(R, dim=0)
+            ynorm = torch.linalg.vector_norm(Y, dim=0)
+            yield
             bh = beta.cpu().numpy()
-            target = self.refine_tol * znorm * torch.linalg.vector_norm(Y, dim=0).cpu().numpy()
+            target = self.refine_tol * znorm * ynorm.cpu().numpy()
             # converged, or stalled at the FP64 residual floor (within 1e3 of the target)
             if np.all(bh <= target) or (prev is not None and np.all((bh > 0.5 * prev) & (bh < 1e3 * target))):
                 break
@@ -588,6 +640,7 @@ class Solver:
                 nw = torch.linalg.vector_norm(w, dim=0)
                 hj[j + 1] = nw
                 Vb[j + 1] = w / nw
+                yield
                 hc = hj.cpu().numpy()
                 its += 1
                 givens_column(hc, cs, sn, g, j)
@@ -611,13 +664,17 @@ class Solver:
         return self._solve_dev(g, excitations).cpu().numpy()
 
     def _solve_dev(self, g, excitations):
-        n = self.mesh.n_unknowns
-        Z = self._buf("Z", (n, n), torch.complex128)
-        Z32 = self._buf("Z32", (n, n), torch.complex64) if self.precision == "mixed" else None
-        self._fill(g, Z, Z32)
         B = torch.from_numpy(np.ascontiguousarray(excitations, complex)).to(self.device)
-        X, info = self._lu_solve(Z, B, Z32)
-        return X
+        return self._run(self._solve_task(g, B, 0))
+
+    def _solve_task(self, g, B, lane):
+        """Generator: fill, LU and port solve of one frequency on the current
+        stream with the buffers of `lane`; returns the device currents."""
+        n = self.mesh.n_unknowns
+        Z = self._buf("Z", (n, n), torch.complex128, lane)
+        Z32 = self._buf("Z32", (n, n), torch.complex64, lane) if self.precision == "mixed" else None
+        self._fill(g, Z, Z32, lane)
+        return (yield from self._lu_solve(Z, B, Z32, lane))
 
     # ----------------------------------------------------------- de-embed
     def charges(self, x, omega):
@@ -698,6 +755,45 @@ class Solver:
         dd = np.sqrt(zr[fi])
         S[fi] = dd[:, None] * SI / dd[None, :]
 
+    def _schedule(self, task, count, lanes):
+        """Run task(i, lane) for i < count, `lanes` at a time, round robin.
+        Each task is a generator that enqueues GPU work on the current stream
+        and yields before it waits for it; the lane's stream is made current
+        whenever its task resumes, so the GPU work of all lanes overlaps.
+        Tasks complete in order within a lane, and a lane takes the next index
+        as soon as its task ends."""
+        prev = torch.cuda.current_stream(self.device)
+        streams = [prev] if lanes == 1 else [torch.cuda.Stream(self.device) for _ in range(lanes)]
+        for st in streams:
+            if st is not prev:
+                st.wait_stream(prev)
+        self._active_lanes = lanes
+        active = [None] * lanes
+        nxt = 0
+        try:
+            while True:
+                busy = False
+                for lane in range(lanes):
+                    if active[lane] is None and nxt < count:
+                        active[lane] = task(nxt, lane)
+                        nxt += 1
+                    if active[lane] is None:
+                        continue
+                    busy = True
+                    torch.cuda.set_stream(streams[lane])
+                    try:
+                        next(active[lane])
+                    except StopIteration:
+                        active[lane] = None
+                if not busy:
+                    break
+        finally:
+            torch.cuda.set_stream(prev)
+            for st in streams:
+                if st is not prev:
+                    prev.wait_stream(st)
+            self._active_lanes = 1
+
     def sweep(self, freqs, keep_currents=False, interp=None, values=None, checkpoint=None, wait_memory=None):
         """S-parameters at freqs (every frequency filled and factored; the
         reference's interpolated-Z mode is not part of this build, so
@@ -712,20 +808,27 @@ class Solver:
         zr = np.zeros((nf, n))
         res = np.zeros((nf, n))
         waves = np.zeros((nf, n, n), complex)
-        cur = []
+        cur = [None] * nf
         t0 = time.time()
         tabs = self.tables(freqs)
         self.geometry().upload(self.device)
-        exc = self.port_excitations()
+        B = torch.from_numpy(np.ascontiguousarray(self.port_excitations(), complex)).to(self.device)
+        torch.cuda.synchronize(self.device)
         t1 = time.time()
-        for fi, f in enumerate(freqs):
-            g = tabs[fi]
-            x = self._solve_dev(g, exc).cpu().numpy()
+        done = [0]
+
+        def point(fi, lane):
+            x_dev = yield from self._solve_task(tabs[fi], B, lane)
+            yield  # let the other lanes enqueue their work before waiting for this one
+            x = x_dev.cpu().numpy()
             if keep_currents:
-                cur.append(x)
-            self._sparams(f, g, x, S, gam, zl, zr, res, waves, fi)
+                cur[fi] = x
+            self._sparams(freqs[fi], tabs[fi], x, S, gam, zl, zr, res, waves, fi)
+            done[0] += 1
             if self.verbose:
-                print(f"PROGRESS {fi + 1}/{nf}", flush=True)
+                print(f"PROGRESS {done[0]}/{nf}", flush=True)
+
+        self._schedule(point, nf, min(self._lane_count(), nf))
         t2 = time.time()
         return SweepResult(freqs, S, zr, gam, zl, res, [p.name for p in self.ports], cur if keep_currents else None,
                            waves, timing=dict(tables=t1 - t0, solve=t2 - t1))

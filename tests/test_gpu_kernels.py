@@ -92,7 +92,7 @@ def _lu_solve(A, B, prec, jb):
     sw = torch.empty(int(_lib.lib().mom_getrs_work_bytes(prec, n)), dtype=torch.uint8, device=DEV)
     s = _lib.stream_ptr()
     P = _lib.ptr
-    _lib.call("mom_getrf", prec, n, P(Ad), n, P(ipiv), P(info), P(work), wb, jb, s)
+    _lib.call("mom_getrf", prec, n, P(Ad), n, P(ipiv), P(info), P(work), wb, jb, 0, s)
     _lib.call("mom_build_perm", n, P(ipiv), P(perm), s)
     _lib.call("mom_getrs", prec, n, B.shape[1], P(Ad), n, P(perm), P(Bd), P(X), P(sw), s)
     return X.cpu().numpy(), ipiv.cpu().numpy(), int(info[0])

@@ -59,3 +59,18 @@ def test_via_transition_sweep_matches_oracle():
     r = Solver(m, st, verbose=False).sweep(f)
     o = OracleSolver(m, st, use_c=True).sweep(f)
     assert np.max(np.abs(r.S - o["S"])) < 1e-6
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_lanes_match_single_lane(precision):
+    """Three frequency points in flight on three streams give the S and the
+    currents of the one-at-a-time sweep (the fill accumulates with atomics,
+    so the two agree to round-off, not bit for bit)."""
+    lay, sub, mk, freqs = W.tee()
+    m = lay.mesh(**mk)
+    f = freqs[:7]
+    r1 = Solver(m, sub, verbose=False, precision=precision, lanes=1).sweep(f, keep_currents=True)
+    r3 = Solver(m, sub, verbose=False, precision=precision, lanes=3).sweep(f, keep_currents=True)
+    for a, b in zip(r1.currents, r3.currents):
+        assert np.max(np.abs(a - b)) <= 1e-9 * np.max(np.abs(a))
+    assert np.max(np.abs(r1.S - r3.S)) < 1e-8
