@@ -30,7 +30,8 @@ int mom_abi_version(void);
  * table rows of every frequency of a sweep:
  *   out[row, c] = sum_k J0(rho[row] kc_k) Cc[k, c] + sum_k J0(rho[row] kr_k) Cr[k, c]
  * over the nodes of problem row_prob[row] (kc: complex lifted path, kr: real
- * tail; offsets / lengths per problem).  ncol = 3 * levels^2 (3, 12 or 27).
+ * tail; offsets / lengths per problem).  ncol = kernels * levels^2 with 1, 2
+ * or 3 kernels and 1 to 3 levels.
  * Replaces GroundedSlabGreens._sommerfeld / residual
  * (Engine/mom25d/greens.py:157-173) and LayeredGreens._sommerfeld / _residual
  * (Engine/mom25d/layered.py:360-377). */
@@ -127,6 +128,30 @@ int mom_refine_update(int n, int nrhs, const void* D, void* X, void* dmax, void*
  * dense solve is a direct LAPACK factorisation). */
 int mom_basis_dot(int n, int nrhs, int j1, const void* V, const void* w, void* h, void* stream);
 int mom_basis_sub(int n, int nrhs, int j1, const void* V, const void* h, void* w, void* stream);
+
+/* Surrogate of the smooth Green's-function remainders R_A, R_V: an MLP
+ * 6 -> 128 -> 128 -> 128 -> 4 with SiLU (FP32 parameter vector of
+ * mom_mlp_param_count() values: W1[128][6] b1 W2[128][128] b2 W3 b3 W4[4][128]
+ * b4; BF16 copies of W2, W3 from mom_mlp_pack).  It stands in for the
+ * Sommerfeld integrals of Engine/mom25d/layered.py:333-377 (`_remainder`,
+ * the Bessel sums) when the stack's tables are built; the integrals stay the
+ * judge of its error.
+ * mom_mlp_forward: out[n][4] from X[n][6]; precision 16 = BF16 matrix cores
+ * (the training arithmetic), 32 = FP32 throughout.
+ * mom_mlp_train_step: squared-error gradient of one batch, one partial
+ * gradient of mom_mlp_partial_stride() floats per 64-sample tile
+ * (nslots = ceil(n / 64)); mom_mlp_adam sums the partials (and the loss,
+ * into *loss) and, when step_lr != 0, applies Adam to P, m1, m2 and the
+ * BF16 copies; grad, when non-null, receives the summed gradient. */
+int mom_mlp_param_count(void);
+int mom_mlp_partial_stride(void);
+int mom_mlp_pack(const float* P, uint16_t* Wbf, void* stream);
+int mom_mlp_forward(int precision, int n, const float* X, const float* P, const uint16_t* Wbf, float* out,
+                    void* stream);
+int mom_mlp_train_step(int n, const float* X, const float* Y, const float* wout, float grad_scale, const float* P,
+                       const uint16_t* Wbf, float* part, int nslots, void* stream);
+int mom_mlp_adam(int nslots, const float* part, float* P, float* m1, float* m2, uint16_t* Wbf, float* grad,
+                 float* loss, float beta1, float beta2, float eps, float step_lr, void* stream);
 
 #ifdef __cplusplus
 }

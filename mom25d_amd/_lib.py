@@ -15,6 +15,7 @@ _p = ctypes.c_void_p
 _i = ctypes.c_int
 _i64 = ctypes.c_int64
 _d = ctypes.c_double
+_f = ctypes.c_float
 
 # name -> (restype, argtypes); the C declarations are in include/mom25d.h
 SIGNATURES = {
@@ -43,6 +44,12 @@ SIGNATURES = {
     "mom_basis_dot": (_i, [_i, _i, _i, _p, _p, _p, _p]),
     "mom_basis_sub": (_i, [_i, _i, _i, _p, _p, _p, _p]),
     "mom_line_voltage": (_i, [_i, _p, _i, _i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _d, _p, _p]),
+    "mom_mlp_param_count": (_i, []),
+    "mom_mlp_partial_stride": (_i, []),
+    "mom_mlp_pack": (_i, [_p, _p, _p]),
+    "mom_mlp_forward": (_i, [_i, _i, _p, _p, _p, _p, _p]),
+    "mom_mlp_train_step": (_i, [_i, _p, _p, _p, _f, _p, _p, _p, _i, _p]),
+    "mom_mlp_adam": (_i, [_i, _p, _p, _p, _p, _p, _p, _p, _f, _f, _f, _f, _p]),
 }
 
 _lib = None

@@ -193,11 +193,17 @@ extern "C" int mom_sommerfeld(int n_rows, const double* rho, const int32_t* row_
                        (const z64*)Cc, kr_off, kr_len, kr, (const z64*)Cr, (z64*)out);                      \
     break;
   switch (ncol) {
+    MOM_SOMMERFELD_CASE(1)
+    MOM_SOMMERFELD_CASE(2)
     MOM_SOMMERFELD_CASE(3)
+    MOM_SOMMERFELD_CASE(4)
+    MOM_SOMMERFELD_CASE(8)
+    MOM_SOMMERFELD_CASE(9)
     MOM_SOMMERFELD_CASE(12)
+    MOM_SOMMERFELD_CASE(18)
     MOM_SOMMERFELD_CASE(27)
     default:
-      return fail("mom_sommerfeld: ncol must be 3, 12 or 27 (3 kernels x levels^2)");
+      return fail("mom_sommerfeld: ncol must be k * levels^2 with k = 1, 2 or 3 and 1 to 3 levels");
   }
 #undef MOM_SOMMERFELD_CASE
   return check_launch("sommerfeld_kernel");

@@ -172,9 +172,20 @@ class SweepTables:
              otherwise the transmission-line spectra of the stack.
     """
 
-    def __init__(self, stack, freqs, rho_max, levels, single, device="cuda", gl_order=12):
-        self.stack = stack
+    def __init__(self, stack, freqs, rho_max, levels, single, device="cuda", gl_order=12, surrogate=None,
+                 surrogate_tol=2e-3):
+        self._setup(stack, rho_max, levels, single, device)
         self.freqs = np.atleast_1d(np.asarray(freqs, float))
+        jobs = [self._prepare(f, gl_order) for f in self.freqs]
+        if surrogate is None:
+            res = self._sommerfeld(jobs)
+            self.surrogate_report = None
+        else:
+            res = self._surrogate_remainders(jobs, surrogate, surrogate_tol)
+        self._finish(jobs, res)
+
+    def _setup(self, stack, rho_max, levels, single, device):
+        self.stack = stack
         self.rho_max = float(rho_max)
         self.levels = list(levels)
         self.single = bool(single)
@@ -187,9 +198,14 @@ class SweepTables:
         self.epsc = np.array([d.eps_r * (1 - 1j * d.tand) for d in st.layers])
         self.t = np.array([d.t for d in st.layers])
         self.h = float(st.z[-1])
-        jobs = [self._prepare(f, gl_order) for f in self.freqs]
-        res = self._sommerfeld(jobs)
-        self._finish(jobs, res)
+
+    @classmethod
+    def spectra_only(cls, stack, rho_max, levels, single=False, device="cuda"):
+        """An instance with the stack set up and no tables (for sampling the
+        remainders at arbitrary points)."""
+        obj = cls.__new__(cls)
+        obj._setup(stack, rho_max, levels, single, device)
+        return obj
 
     # ---------------------------------------------------------- per frequency
     def _prepare(self, f, order):
@@ -243,6 +259,14 @@ class SweepTables:
         return dict(f=f, omega=omega, k0=k0, kc=kc, Cc=Cc.T.copy(), kr=kr, Cr=Cr.T.copy(), rho=rho, c=c, kx=kx,
                     dz=dz, zs=zs)
 
+    def remainders_at(self, freqs, rhos, gl_order=12):
+        """Sommerfeld remainders (n_i, 3, L, L) at rhos[i] for freqs[i], and
+        the leading-term wavenumbers kx (2, L, L) of each frequency."""
+        jobs = [self._prepare(f, gl_order) for f in np.atleast_1d(freqs)]
+        parts = self._sommerfeld(jobs, rhos=rhos)
+        L = self.L
+        return [p.reshape(-1, 3, L, L) for p in parts], [j["kx"] for j in jobs]
+
     def _leading(self, k0, kk, dz, c, kx):
         """Leading terms per pair (Engine/mom25d/layered.py:292-331)."""
         L, n = self.L, len(self.t)
@@ -274,11 +298,18 @@ class SweepTables:
         c[small] = c[small].real
 
     # ------------------------------------------------------------ Sommerfeld
-    def _sommerfeld(self, jobs):
+    def _sommerfeld(self, jobs, cols=None, rhos=None):
+        """Remainders of every job at its table nodes (or at rhos[i] for job
+        i), columns cols of the 3 L^2 kernels (default: all), one launch."""
         dev = self.device
-        ncol = 3 * self.L * self.L
-        rho = np.concatenate([j["rho"] for j in jobs])
-        prob = np.concatenate([np.full(j["rho"].size, p, np.int32) for p, j in enumerate(jobs)])
+        ncol_all = 3 * self.L * self.L
+        cols = np.arange(ncol_all) if cols is None else np.asarray(cols)
+        ncol = cols.size
+        rl = [j["rho"] for j in jobs] if rhos is None else [np.asarray(r, float) for r in rhos]
+        rho = np.concatenate(rl)
+        if rho.size == 0:
+            return [np.zeros((0, ncol), complex) for _ in jobs]
+        prob = np.concatenate([np.full(r.size, p, np.int32) for p, r in enumerate(rl)])
         kc_len = np.array([j["kc"].size for j in jobs], np.int32)
         kr_len = np.array([j["kr"].size for j in jobs], np.int32)
         kc_off = np.concatenate([[0], np.cumsum(kc_len)[:-1]]).astype(np.int64)
@@ -289,19 +320,82 @@ class SweepTables:
         d_kc_off, d_kc_len = t(kc_off, torch.int64), t(kc_len, torch.int32)
         d_kr_off, d_kr_len = t(kr_off, torch.int64), t(kr_len, torch.int32)
         d_kc = t(np.concatenate([j["kc"] for j in jobs]).astype(complex), torch.complex128)
-        d_Cc = t(np.concatenate([j["Cc"] for j in jobs]), torch.complex128)
+        d_Cc = t(np.concatenate([j["Cc"][:, cols] for j in jobs]), torch.complex128)
         d_kr = t(np.concatenate([j["kr"] for j in jobs]), torch.float64)
-        d_Cr = t(np.concatenate([j["Cr"] for j in jobs]), torch.complex128)
+        d_Cr = t(np.concatenate([j["Cr"][:, cols] for j in jobs]), torch.complex128)
         out = torch.empty((rho.size, ncol), dtype=torch.complex128, device=dev)
         P = _lib.ptr
         _lib.call("mom_sommerfeld", int(rho.size), P(d_rho), P(d_prob), P(d_kc_off), P(d_kc_len), P(d_kc), P(d_Cc),
                   P(d_kr_off), P(d_kr_len), P(d_kr), P(d_Cr), ncol, P(out), _lib.stream_ptr())
         res = out.cpu().numpy()
         parts, o = [], 0
-        for j in jobs:
-            parts.append(res[o:o + j["rho"].size])
-            o += j["rho"].size
+        for r in rl:
+            parts.append(res[o:o + r.size])
+            o += r.size
         return parts
+
+    def _surrogate_remainders(self, jobs, sur, tol, n_check=16):
+        """Remainder columns of every job from the surrogate networks (R_A,
+        R_V, R_W at every table node), judged by the Sommerfeld integrals at
+        n_check nodes per frequency (log-spaced over the table, ends
+        included): a frequency whose largest error there, relative to the
+        largest |R| of the same kernel and level pair, exceeds tol takes all
+        its columns from the integrals."""
+        import time
+
+        L = self.L
+        LL = L * L
+        f_lo, f_hi = sur.f_range
+        if self.rho_max > sur.rho_range[1] or any(not (f_lo * (1 - 1e-9) <= j["f"] <= f_hi * (1 + 1e-9)) for j in jobs):
+            self.surrogate_report = dict(check_err=np.full(len(jobs), np.inf), fallback=list(range(len(jobs))),
+                                         tol=tol, n_check=n_check, seconds={}, integral_nodes=None,
+                                         reason="outside the trained rho / frequency range")
+            return self._sommerfeld(jobs)
+        t0 = time.perf_counter()
+        chk_idx = [np.unique(np.round(np.geomspace(1, j["rho"].size, n_check)).astype(int) - 1) for j in jobs]
+        resC = self._sommerfeld(jobs, rhos=[j["rho"][c] for j, c in zip(jobs, chk_idx)])
+        t1 = time.perf_counter()
+        cols = []
+        for j in jobs:
+            n = j["rho"].size
+            for i in range(L):
+                for k in range(L):
+                    cols.append((j["rho"], np.full(n, self.zl[i]), np.full(n, self.zl[k]), np.full(n, j["k0"]),
+                                 np.full(n, j["kx"][0, i, k]), np.full(n, j["kx"][1, i, k])))
+        rho, zi, zj, k0, kxA, kxV = (np.concatenate(c) for c in zip(*cols))
+        RA, RV, RW = sur.remainders(rho, zi, zj, k0, kxA, kxV)
+        t2 = time.perf_counter()
+        res, errs, fallback = [], [], []
+        o = 0
+        for fi, j in enumerate(jobs):
+            n = j["rho"].size
+            R = np.empty((n, 3 * LL), complex)
+            for p in range(LL):
+                sl = slice(o + p * n, o + (p + 1) * n)
+                R[:, p], R[:, LL + p], R[:, 2 * LL + p] = RA[sl], RV[sl], RW[sl]
+            o += LL * n
+            ref = resC[fi]
+            scale = np.maximum(np.abs(ref).max(0), 1e-300)
+            err = float(np.max(np.abs(R[chk_idx[fi]] - ref).max(0) / scale))
+            errs.append(err)
+            if not err <= tol:
+                fallback.append(fi)
+            res.append(R)
+        if fallback:
+            full = self._sommerfeld([jobs[fi] for fi in fallback])
+            for fi, F in zip(fallback, full):
+                res[fi] = F
+        t3 = time.perf_counter()
+        nodes = sum(c.size * (j["kc"].size + j["kr"].size) for j, c in zip(jobs, chk_idx))
+        nodes += sum(jobs[fi]["rho"].size * (jobs[fi]["kc"].size + jobs[fi]["kr"].size) for fi in fallback)
+        self.surrogate_report = dict(check_err=np.array(errs), fallback=fallback, tol=tol, n_check=n_check,
+                                     seconds=dict(integrals=t1 - t0 + (t3 - t2), network=t2 - t1),
+                                     integral_nodes=nodes, reason=None)
+        return res
+
+    def integral_nodes(self, jobs):
+        """Bessel evaluations of the plain tables (table nodes x spectral nodes)."""
+        return sum(j["rho"].size * (j["kc"].size + j["kr"].size) for j in jobs)
 
     # ----------------------------------------------------------------- tables
     def _finish(self, jobs, res):
@@ -367,6 +461,13 @@ class SweepTables:
 
     def __len__(self):
         return len(self.greens)
+
+
+def sommerfeld_remainders(stack, levels, freqs, rhos, rho_max, device="cuda"):
+    """Remainders of a (multi-level) stack at arbitrary points: list of
+    (n_i, 3, L, L) arrays for rhos[i] at freqs[i], and the kx (2, L, L)."""
+    tabs = SweepTables.spectra_only(stack, rho_max, levels, False, device)
+    return tabs.remainders_at(freqs, rhos)
 
 
 def _slab_spectra(kr, k0, k1, epsc, h):

@@ -316,6 +316,11 @@ class Solver:
                 "mixed" -- complex64 LU on the matrix cores with FP64
                 iterative refinement against the FP64 impedance matrix.
     method    : "auto" / "dense" (the H-matrix solver is not part of this build).
+    surrogate : a trained GreensSurrogate of this stack: the remainders of
+                the Green's tables come from its networks, corrected and
+                judged per frequency by the Sommerfeld integrals at a few
+                nodes (relative tolerance surrogate_tol; the integrals are
+                used throughout where it misses).
     lanes     : frequency points in flight during a sweep, each on its own
                 stream with its own matrices (None: 2 when they fit in a
                 third of the free device memory, else 1).  The latency-bound
@@ -328,7 +333,7 @@ class Solver:
 
     def __init__(self, mesh, substrate, z0=None, verbose=True, method="auto", v_weight=None, z0_recip_min=None,
                  tand_floor=None, precision="fp64", device="cuda", panel=None, refine_tol=1e-15, refine_maxit=20,
-                 lanes=None):
+                 lanes=None, surrogate=None, surrogate_tol=2e-3):
         from .layered import Stackup
 
         if method not in ("auto", "dense"):
@@ -367,6 +372,9 @@ class Solver:
         self.refine_maxit = refine_maxit
         self.gmres_restart = 30
         self.lanes = lanes
+        self.surrogate = surrogate
+        self.surrogate_tol = surrogate_tol
+        self.last_tables = None
         self._plan = None
         self._bufs = {}
         self._keep = {}
@@ -392,7 +400,10 @@ class Solver:
 
     def tables(self, freqs):
         """Green's-function tables of all frequencies (one GPU launch)."""
-        return SweepTables(self.stack, freqs, self.rho_max, self.mesh.layer_names, self.single, self.device)
+        t = SweepTables(self.stack, freqs, self.rho_max, self.mesh.layer_names, self.single, self.device,
+                        surrogate=self.surrogate, surrogate_tol=self.surrogate_tol)
+        self.last_tables = t
+        return t
 
     def greens(self, freq):
         return self.tables([freq])[0]
