@@ -129,6 +129,16 @@ int mom_refine_update(int n, int nrhs, const void* D, void* X, void* dmax, void*
 int mom_basis_dot(int n, int nrhs, int j1, const void* V, const void* w, void* h, void* stream);
 int mom_basis_sub(int n, int nrhs, int j1, const void* V, const void* h, void* w, void* stream);
 
+/* HIP graphs of library calls (no counterpart in the reference, which has no
+ * GPU path): calls issued on `stream` between mom_graph_begin and
+ * mom_graph_end are captured (the stream must not be the default stream,
+ * nothing else may be enqueued on it meanwhile, and the calls must not
+ * allocate); mom_graph_launch replays them with the same arguments. */
+int mom_graph_begin(void* stream);
+int mom_graph_end(void* stream, void** graph_exec);
+int mom_graph_launch(void* graph_exec, void* stream);
+int mom_graph_destroy(void* graph_exec);
+
 /* Surrogate of the smooth Green's-function remainders R_A, R_V: an MLP
  * 6 -> 128 -> 128 -> 128 -> 4 with SiLU (FP32 parameter vector of
  * mom_mlp_param_count() values: W1[128][6] b1 W2[128][128] b2 W3 b3 W4[4][128]

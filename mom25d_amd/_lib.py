@@ -44,6 +44,10 @@ SIGNATURES = {
     "mom_basis_dot": (_i, [_i, _i, _i, _p, _p, _p, _p]),
     "mom_basis_sub": (_i, [_i, _i, _i, _p, _p, _p, _p]),
     "mom_line_voltage": (_i, [_i, _p, _i, _i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _d, _p, _p]),
+    "mom_graph_begin": (_i, [_p]),
+    "mom_graph_end": (_i, [_p, _p]),
+    "mom_graph_launch": (_i, [_p, _p]),
+    "mom_graph_destroy": (_i, [_p]),
     "mom_mlp_param_count": (_i, []),
     "mom_mlp_partial_stride": (_i, []),
     "mom_mlp_pack": (_i, [_p, _p, _p]),

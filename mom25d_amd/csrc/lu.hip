@@ -5,10 +5,11 @@
 // Replaces scipy.linalg.lu_factor / lu_solve (LAPACK zgetrf / zgetrs) of
 // Engine/mom25d/solver.py:11-23.  Blocked right-looking factorisation with
 // panel width jb:
-//   1. panel  : GEPP of the (n - k0) x jb column panel by a cooperative grid
-//               (rows split over workgroups, held in LDS; one grid barrier per
-//               column: every workgroup publishes its local pivot candidate
-//               row, and after the barrier all of them know the winner);
+//   1. panel  : GEPP of the (n - k0) x jb column panel by a grid whose
+//               workgroups are all resident (rows split over workgroups, held
+//               in LDS; one exchange per column: every workgroup publishes its
+//               local pivot candidate row in its own slot, and once all slots
+//               carry the column's generation every workgroup knows the winner);
 //               pivots are chosen by |re| + |im|, first maximum wins, as
 //               LAPACK's izamax does;
 //   2. laswp  : the panel's row interchanges applied to the other columns;
@@ -805,10 +806,17 @@ static int launch_panel(cplx<R>* A, int64_t lda, int n, int k0, int b, int32_t* 
   G = (m + rp - 1) / rp;
   const uint32_t gen0 = 1u + (uint32_t)k0;  // unique per column of the factorisation
   PanelArgs<R> pa{A, lda, n, k0, b, rp, gen0, ipiv, info, slots, slot_row, diag_row};
-  void* args[] = {&pa};
-  MOM_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(panel_kernel<R>), dim3(G), dim3(256), args,
-                                     (size_t)rp * b * sizeof(cplx<R>), s));
-  return 0;
+  // An ordinary launch, not a cooperative one: cooperative launches are
+  // serialised across streams, which would stop the panels of concurrent
+  // factorisations from overlapping.  Every workgroup of the panel is
+  // resident together all the same: the grid is at most G <= max_groups <=
+  // CUs / (factorisations in flight) workgroups; the other kernels that can
+  // hold a CU meanwhile either run to completion on their own (updates,
+  // swaps, fills) or wait only for workgroups that are already running (the
+  // ticket-ordered triangular solves), so they release their CUs and the
+  // dispatcher places the rest of the panel.
+  hipLaunchKernelGGL(panel_kernel<R>, dim3(G), dim3(256), (size_t)rp * b * sizeof(cplx<R>), s, pa);
+  return check_launch("panel_kernel");
 }
 
 template <typename R>

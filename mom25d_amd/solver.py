@@ -13,16 +13,20 @@ Per frequency of a sweep:
 """
 
 import ctypes
+import multiprocessing
+import os
+import sys
 import time
-import warnings
+import types
+from concurrent.futures import ProcessPoolExecutor
 from dataclasses import dataclass
 
 import numpy as np
 import torch
-from scipy.optimize import minimize, minimize_scalar
 
 from . import _lib, analytic
 from .constants import C0, EPS0, MU0
+from .deembed import VOLTAGE_WEIGHT, FitContext, fit_waves, reciprocal_z0, sparams_point
 from .greens import SweepTables
 from .plan import FillPlan
 
@@ -37,6 +41,13 @@ class Substrate:
     sigma: float = None
     t: float = 35e-6
     ground_loss: bool = True
+
+    def __del__(self):
+        for _sig, h in getattr(self, "_graphs", {}).values():
+            try:
+                _lib.call("mom_graph_destroy", h)
+            except Exception:  # interpreter shutdown: the library may be gone
+                pass
 
     @property
     def lossless(self):
@@ -168,48 +179,40 @@ class PortModel:
         return phi.reshape(self.s.size, self.nper, -1).mean(1)
 
 
-VOLTAGE_WEIGHT = 1.0
+class FitPool:
+    """Worker processes for the wave fits of a sweep.  They are spawned (not
+    forked: this process holds a GPU context) and import numpy, scipy and the
+    de-embedding module only.  The caller's main module is hidden while a
+    worker starts, so that a script without a `__main__` guard is not run
+    again in the workers."""
+
+    def __init__(self, workers):
+        self.workers = workers
+        self.ex = ProcessPoolExecutor(workers, mp_context=multiprocessing.get_context("spawn"))
+
+    def submit(self, fn, *args):
+        main = sys.modules.get("__main__")
+        sys.modules["__main__"] = types.ModuleType("__main__")
+        try:
+            return self.ex.submit(fn, *args)
+        finally:
+            sys.modules["__main__"] = main
 
 
-def fit_waves(s_list, I_list, V_list, gamma0, lossless=False, v_weight=VOLTAGE_WEIGHT, bounds=(0.7, 1.3)):
-    """Shared-gamma / Z0 fit of I(s) = A e^{gs} + B e^{-gs}, V = Z0 (A e^{gs} - B e^{-gs})
-    over all data sets (Engine/mom25d/solver.py:618-670): bounded scalar search
-    of beta for lossless stacks, Nelder-Mead on (alpha, beta) otherwise.
-    Returns gamma, Z0, [(A, B), ...], relative residual."""
-    scale = abs(gamma0.imag)
-    Inrm = sum(np.sum(np.abs(I) ** 2) for I in I_list)
-    Vnrm = sum(np.sum(np.abs(V) ** 2) for V in V_list)
+_POOL = None
 
-    def solve(g):
-        coefs, res, U_all, V_all = [], 0.0, [], []
-        for s, I, V in zip(s_list, I_list, V_list):
-            ep, em = np.exp(g * s), np.exp(-g * s)
-            M = np.stack([ep, em], 1)
-            c, *_ = np.linalg.lstsq(M, I, rcond=None)
-            coefs.append(c)
-            res += np.sum(np.abs(M @ c - I) ** 2) / Inrm
-            U_all.append(c[0] * ep - c[1] * em)
-            V_all.append(V)
-        U, V = np.concatenate(U_all), np.concatenate(V_all)
-        z0 = np.vdot(U, V) / np.vdot(U, U)
-        res += v_weight * np.sum(np.abs(V - z0 * U) ** 2) / Vnrm
-        return coefs, z0, res
 
-    if lossless:
-        lo, hi = bounds
-        opt = minimize_scalar(lambda b: solve(1j * scale * b)[2], bounds=(lo, hi), method="bounded",
-                              options=dict(xatol=1e-12))
-        if min(opt.x - lo, hi - opt.x) < 1e-3:
-            warnings.warn(f"beta fit hit its search bound ({opt.x:.4f} x the Kirschning-Jansen estimate)",
-                          RuntimeWarning, stacklevel=2)
-        g = 1j * scale * opt.x
-    else:
-        p0 = np.array([gamma0.real, gamma0.imag]) / scale
-        opt = minimize(lambda p: solve(scale * (p[0] + 1j * p[1]))[2], p0, method="Nelder-Mead",
-                       options=dict(xatol=1e-10, fatol=1e-16, maxiter=4000))
-        g = scale * (opt.x[0] + 1j * opt.x[1])
-    coefs, z0, res = solve(g)
-    return g, z0, coefs, np.sqrt(res)
+def fit_pool(workers):
+    """The process's FitPool (kept for its lifetime, replaced by a larger one
+    when more workers are asked for); None for workers == 0."""
+    global _POOL
+    if workers <= 0:
+        return None
+    if _POOL is None or _POOL.workers < workers:
+        if _POOL is not None:
+            _POOL.ex.shutdown(wait=False)
+        _POOL = FitPool(workers)
+    return _POOL
 
 
 @dataclass
@@ -321,9 +324,15 @@ class Solver:
                 judged per frequency by the Sommerfeld integrals at a few
                 nodes (relative tolerance surrogate_tol; the integrals are
                 used throughout where it misses).
+    graphs    : in a sweep, replay each lane's factorisation (and the FP64
+                solve) from a HIP graph captured once per lane (off by
+                default: on this ROCm the replay of these multi-stream graphs
+                is slower than issuing the calls).
+    workers   : processes that run the wave fits of a sweep beside the GPU
+                work (None: up to 8 for sweeps of 16 points or more; 0: none).
     lanes     : frequency points in flight during a sweep, each on its own
-                stream with its own matrices (None: 2 when they fit in a
-                third of the free device memory, else 1).  The latency-bound
+                stream with its own matrices (None: 4 up to N = 4000, else 2,
+                as many as fit in a third of the free device memory).  The latency-bound
                 steps of one point (LU panels, triangular solves, host
                 de-embedding) then overlap the matrix-core work of another.
     """
@@ -333,7 +342,7 @@ class Solver:
 
     def __init__(self, mesh, substrate, z0=None, verbose=True, method="auto", v_weight=None, z0_recip_min=None,
                  tand_floor=None, precision="fp64", device="cuda", panel=None, refine_tol=1e-15, refine_maxit=20,
-                 lanes=None, surrogate=None, surrogate_tol=2e-3):
+                 lanes=None, surrogate=None, surrogate_tol=2e-3, workers=None, graphs=False):
         from .layered import Stackup
 
         if method not in ("auto", "dense"):
@@ -372,6 +381,10 @@ class Solver:
         self.refine_maxit = refine_maxit
         self.gmres_restart = 30
         self.lanes = lanes
+        self.workers = workers
+        self.graphs = graphs
+        self._graphs = {}
+        self._graph_seen = set()
         self.surrogate = surrogate
         self.surrogate_tol = surrogate_tol
         self.last_tables = None
@@ -384,6 +397,13 @@ class Solver:
             hp = st.level_h(port.layer)
             key = (round(port.width / hp, 9), port.layer, None, port.z0_from)
             self.groups.setdefault(key, []).append(p)
+
+    def __del__(self):
+        for _sig, h in getattr(self, "_graphs", {}).values():
+            try:
+                _lib.call("mom_graph_destroy", h)
+            except Exception:  # interpreter shutdown: the library may be gone
+                pass
 
     @property
     def lossless(self):
@@ -420,9 +440,15 @@ class Solver:
 
     PANEL_LDS = 120 * 1024  # LDS of one LU panel workgroup (the library's bound)
 
+    def _worker_count(self, nf):
+        """Processes for the wave fits of a sweep (0: in this process)."""
+        if self.workers is not None:
+            return max(0, int(self.workers))
+        return min(8, max(1, (os.cpu_count() or 2) // 2)) if nf >= 16 else 0
+
     def _lane_count(self):
-        """Lanes of a sweep: as asked (default 2 when the matrices of two
-        lanes fit in a third of the free device memory), and no more than
+        """Lanes of a sweep: as asked (default 4 up to N = 4000, else 2, as
+        many as fit in a third of the free device memory), and no more than
         allow every lane's LU panel to be resident at once."""
         n = self.mesh.n_unknowns
         if self.lanes is not None:
@@ -430,7 +456,8 @@ class Solver:
         else:
             per = n * n * (16 + (8 if self.precision == "mixed" else 0))
             free, _total = torch.cuda.mem_get_info(self.device)
-            lanes = 2 if 2 * per < free / 3 else 1
+            want = 4 if n <= 4000 else 2
+            lanes = max(1, min(want, int(free / 3 // per)))
         elem = 16 if self.precision == "fp64" else 8
         rows = max(1, self.PANEL_LDS // (self.panel * elem))
         groups = -(-n // rows)
@@ -545,11 +572,15 @@ class Solver:
     def port_excitations(self):
         return np.stack([pm.excitation for pm in self.pm], 1)
 
-    def _lu_solve(self, Z, B, Z32=None, lane=0):
+    def _lu_solve(self, Z, B, Z32=None, lane=0, graph=False):
         """Generator: X = Z^-1 B on the device (the value it returns).  Z
         (complex128) is overwritten by its LU in the fp64 path; in the mixed
         path Z32 holds the complex64 LU and Z stays intact for the refinement
-        residuals.  It yields wherever the host would wait for the GPU."""
+        residuals.  It yields wherever the host would wait for the GPU.
+        graph: replay the factorisation (and the FP64 solve) from a HIP graph
+        of this lane's buffers, captured on the second call (the first runs
+        the same calls directly); the current stream must not be the default
+        stream."""
         n, nrhs = B.shape
         stream = _lib.stream_ptr()
         P = _lib.ptr
@@ -562,12 +593,39 @@ class Solver:
         wbytes = int(_lib.lib().mom_getrf_work_bytes(prec, n, jb))
         work = self._buf(f"work{prec}", (wbytes,), torch.uint8, lane)
         groups = self._panel_groups()
-        _lib.call("mom_getrf", prec, n, P(A), A.stride(0), P(ipiv), P(info), P(work), wbytes, jb, groups, stream)
-        _lib.call("mom_build_perm", n, P(ipiv), P(perm), stream)
         sw = self._buf(f"swork{prec}", (int(_lib.lib().mom_getrs_work_bytes(prec, n)),), torch.uint8, lane)
-        X = torch.empty((n, nrhs), dtype=torch.complex128, device=self.device)
+        X = self._buf("X", (n, nrhs), torch.complex128, lane)
+
+        def calls():
+            _lib.call("mom_getrf", prec, n, P(A), A.stride(0), P(ipiv), P(info), P(work), wbytes, jb, groups, stream)
+            _lib.call("mom_build_perm", n, P(ipiv), P(perm), stream)
+            if Z32 is None:
+                _lib.call("mom_getrs", 64, n, nrhs, P(A), A.stride(0), P(perm), P(B), P(X), P(sw), stream)
+
+        if graph:
+            key = (lane, prec)
+            sig = (stream, A.data_ptr(), B.data_ptr(), X.data_ptr(), work.data_ptr(), sw.data_ptr(), ipiv.data_ptr(),
+                   groups, jb, n, nrhs)
+            g = self._graphs.get(key)
+            if g is not None and g[0] != sig:
+                _lib.call("mom_graph_destroy", g[1])
+                g = None
+            if g is None and key in self._graph_seen:
+                h = ctypes.c_void_p()
+                _lib.call("mom_graph_begin", stream)
+                try:
+                    calls()
+                finally:
+                    _lib.call("mom_graph_end", stream, ctypes.byref(h))
+                g = self._graphs[key] = (sig, h.value)
+            self._graph_seen.add(key)
+            if g is not None:
+                _lib.call("mom_graph_launch", g[1], stream)
+            else:
+                calls()
+        else:
+            calls()
         if Z32 is None:
-            _lib.call("mom_getrs", 64, n, nrhs, P(A), A.stride(0), P(perm), P(B), P(X), P(sw), stream)
             self.last_refine = 0
             return X
         return (yield from self._gmres_ir(Z, A, perm, sw, B, lane))
@@ -676,16 +734,17 @@ class Solver:
 
     def _solve_dev(self, g, excitations):
         B = torch.from_numpy(np.ascontiguousarray(excitations, complex)).to(self.device)
-        return self._run(self._solve_task(g, B, 0))
+        return self._run(self._solve_task(g, B, 0)).clone()
 
-    def _solve_task(self, g, B, lane):
+    def _solve_task(self, g, B, lane, graph=False):
         """Generator: fill, LU and port solve of one frequency on the current
-        stream with the buffers of `lane`; returns the device currents."""
+        stream with the buffers of `lane`; returns the device currents (a
+        buffer of the lane: read it before the lane's next solve)."""
         n = self.mesh.n_unknowns
         Z = self._buf("Z", (n, n), torch.complex128, lane)
         Z32 = self._buf("Z32", (n, n), torch.complex64, lane) if self.precision == "mixed" else None
         self._fill(g, Z, Z32, lane)
-        return (yield from self._lu_solve(Z, B, Z32, lane))
+        return (yield from self._lu_solve(Z, B, Z32, lane, graph))
 
     # ----------------------------------------------------------- de-embed
     def charges(self, x, omega):
@@ -724,86 +783,42 @@ class Solver:
                   P(tabs.tab_b), P(qd), 1.0 / EPS0, P(out), _lib.stream_ptr())
         return out.cpu().numpy()
 
-    def _sparams(self, f, g, x, S, gam, zl, zr, res, waves, fi):
+    def _fit_context(self):
+        """The port description the wave fits need (picklable)."""
+        groups = list(self.groups.values())
+        first = [self.ports[m[0]] for m in groups]
+        st = self.stack
+        return FitContext(groups=groups, width=[p.width for p in first], h=[st.level_h(p.layer) for p in first],
+                          eps=[st.eps_below(p.layer) for p in first], covered=[st.covered(p.layer) for p in first],
+                          z0_from=[p.z0_from for p in self.ports], s=[pm.s for pm in self.pm],
+                          lossless=bool(self.lossless), v_weight=self.v_weight, z0_fixed=self.z0_fixed,
+                          z0_recip_min=self.z0_recip_min)
+
+    def _port_iv(self, g, x):
+        """Currents and voltages at every port's sample cuts (the voltages
+        from the charges through the line-voltage kernel)."""
         npt = len(self.ports)
-        n = npt
         q = self.charges(x, g.omega)
         phi = [None] * npt
-        layers = list(dict.fromkeys(p.layer for p in self.ports))
-        for lay in layers:
+        for lay in dict.fromkeys(p.layer for p in self.ports):
             ps = [p for p in range(npt) if self.ports[p].layer == lay]
             vp = self.line_voltage(g, q, np.concatenate([self.pm[p].vpts for p in ps]), lay)
             for p, v in zip(ps, np.split(vp, np.cumsum([len(self.pm[p].vpts) for p in ps])[:-1])):
                 phi[p] = v
-        Acoef = np.zeros((n, n), complex)
-        Bcoef = np.zeros((n, n), complex)
-        for members in self.groups.values():
-            port = self.ports[members[0]]
-            hp = self.stack.level_h(port.layer)
-            b0 = analytic.beta(port.width, hp, self.stack.eps_below(port.layer), f)
-            bnd = (0.7, 1.3) if not self.stack.covered(port.layer) else (0.7, 1.8)
-            s_list, I_list, V_list, keys = [], [], [], []
-            for p in members:
-                Ip = self.pm[p].currents(x)
-                Vp = self.pm[p].voltages(phi[p])
-                for e in range(n):
-                    s_list.append(self.pm[p].s)
-                    I_list.append(Ip[:, e])
-                    V_list.append(Vp[:, e])
-                    keys.append((p, e))
-            gm, z0, coefs, r = fit_waves(s_list, I_list, V_list, 1e-3 * b0 + 1j * b0, lossless=self.lossless,
-                                         bounds=bnd, v_weight=self.v_weight)
-            for (p, e), (a, bb) in zip(keys, coefs):
-                Acoef[p, e], Bcoef[p, e] = a, bb
-            for p in members:
-                gam[fi, p], zl[fi, p], res[fi, p] = gm, z0, r
-        waves[fi] = Acoef
-        SI = -Bcoef @ np.linalg.inv(Acoef)
-        if self.z0_fixed is None:
-            zr[fi] = self._reciprocal_z0(SI, zl[fi].real)
-        else:
-            zr[fi] = self.z0_fixed
-        dd = np.sqrt(zr[fi])
-        S[fi] = dd[:, None] * SI / dd[None, :]
+        I = [self.pm[p].currents(x) for p in range(npt)]
+        V = [self.pm[p].voltages(phi[p]) for p in range(npt)]
+        return I, V
+
+    @staticmethod
+    def _store_point(out, fi, S, gam, zl, zr, res, waves):
+        S[fi], gam[fi], zl[fi], res[fi], zr[fi], waves[fi] = out
+
+    def _sparams(self, f, g, x, S, gam, zl, zr, res, waves, fi):
+        I, V = self._port_iv(g, x)
+        self._store_point(sparams_point(self._fit_context(), f, I, V), fi, S, gam, zl, zr, res, waves)
 
     def _schedule(self, task, count, lanes):
-        """Run task(i, lane) for i < count, `lanes` at a time, round robin.
-        Each task is a generator that enqueues GPU work on the current stream
-        and yields before it waits for it; the lane's stream is made current
-        whenever its task resumes, so the GPU work of all lanes overlaps.
-        Tasks complete in order within a lane, and a lane takes the next index
-        as soon as its task ends."""
-        prev = torch.cuda.current_stream(self.device)
-        streams = [prev] if lanes == 1 else [torch.cuda.Stream(self.device) for _ in range(lanes)]
-        for st in streams:
-            if st is not prev:
-                st.wait_stream(prev)
-        self._active_lanes = lanes
-        active = [None] * lanes
-        nxt = 0
-        try:
-            while True:
-                busy = False
-                for lane in range(lanes):
-                    if active[lane] is None and nxt < count:
-                        active[lane] = task(nxt, lane)
-                        nxt += 1
-                    if active[lane] is None:
-                        continue
-                    busy = True
-                    torch.cuda.set_stream(streams[lane])
-                    try:
-                        next(active[lane])
-                    except StopIteration:
-                        active[lane] = None
-                if not busy:
-                    break
-        finally:
-            torch.cuda.set_stream(prev)
-            for st in streams:
-                if st is not prev:
-                    prev.wait_stream(st)
-            self._active_lanes = 1
+        run_lanes(task, count, lanes, self.device, [self], self.graphs)
 
     def sweep(self, freqs, keep_currents=False, interp=None, values=None, checkpoint=None, wait_memory=None):
         """S-parameters at freqs (every frequency filled and factored; the
@@ -812,95 +827,131 @@ class Solver:
         if interp not in (None, "auto"):
             raise ValueError("interp must be None or 'auto' (no frequency interpolation in this build)")
         freqs = np.atleast_1d(np.asarray(freqs, float))
-        nf, n = len(freqs), len(self.ports)
-        S = np.zeros((nf, n, n), complex)
-        gam = np.zeros((nf, n), complex)
-        zl = np.zeros((nf, n), complex)
-        zr = np.zeros((nf, n))
-        res = np.zeros((nf, n))
-        waves = np.zeros((nf, n, n), complex)
-        cur = [None] * nf
-        t0 = time.time()
-        tabs = self.tables(freqs)
-        self.geometry().upload(self.device)
-        B = torch.from_numpy(np.ascontiguousarray(self.port_excitations(), complex)).to(self.device)
-        torch.cuda.synchronize(self.device)
-        t1 = time.time()
-        done = [0]
-
-        def point(fi, lane):
-            x_dev = yield from self._solve_task(tabs[fi], B, lane)
-            yield  # let the other lanes enqueue their work before waiting for this one
-            x = x_dev.cpu().numpy()
-            if keep_currents:
-                cur[fi] = x
-            self._sparams(freqs[fi], tabs[fi], x, S, gam, zl, zr, res, waves, fi)
-            done[0] += 1
-            if self.verbose:
-                print(f"PROGRESS {done[0]}/{nf}", flush=True)
-
-        self._schedule(point, nf, min(self._lane_count(), nf))
-        t2 = time.time()
-        return SweepResult(freqs, S, zr, gam, zl, res, [p.name for p in self.ports], cur if keep_currents else None,
-                           waves, timing=dict(tables=t1 - t0, solve=t2 - t1))
+        lanes = min(self._lane_count(), len(freqs))
+        return _sweep_many([(self, freqs)], lanes, self._worker_count(len(freqs)), keep_currents, self.graphs)[0]
 
     # ---------------------------------------------------------- reference Z0
     def _reciprocal_z0(self, SI, zphi, prior=1e-4):
-        """Port reference impedances from reciprocity ratios plus the
-        scalar-potential level (Engine/mom25d/solver.py:1357-1397)."""
-        groups = list(self.groups.values())
-        recip = np.array([self.ports[m[0]].z0_from == "reciprocity" for m in groups])
-        if recip.all():
-            raise ValueError("every port has z0_from='reciprocity': at least one needs 'voltage'")
-        cmin = self.z0_recip_min
-        if not recip.any():
-            return self._voltage_z0(SI, zphi, groups, prior, cmin)
-        gid = np.empty(len(self.ports), int)
-        for k, members in enumerate(groups):
-            gid[members] = k
-        vport = ~recip[gid]
-        z = np.array(zphi, float)
-        vidx = np.nonzero(vport)[0]
-        vgroups = [np.nonzero(np.isin(vidx, g))[0] for g, r in zip(groups, recip) if not r]
-        z[vidx] = self._voltage_z0(SI[np.ix_(vidx, vidx)], z[vidx], vgroups, prior, cmin)
-        for k in np.nonzero(recip)[0]:
-            num = den = 0.0
-            for i in groups[k]:
-                for j in np.nonzero(vport)[0]:
-                    w = min(abs(SI[i, j]), abs(SI[j, i])) ** 2
-                    if w > 0 and (cmin is None or w >= cmin ** 2):
-                        num += w * np.log(z[j] * abs(SI[j, i] / SI[i, j]))
-                        den += w
-            if den == 0:
-                raise ValueError(f"port group {k} (z0_from='reciprocity') does not couple to any voltage port")
-            z[groups[k]] = np.exp(num / den)
-        return z
+        """Port reference impedances (Engine/mom25d/solver.py:1357-1397)."""
+        return reciprocal_z0(self._fit_context(), SI, zphi, prior)
 
-    @staticmethod
-    def _voltage_z0(SI, zphi, groups, prior, cmin=None):
-        if len(groups) == 1:
-            return zphi
-        gid = np.empty(len(zphi), int)
-        for k, members in enumerate(groups):
-            gid[members] = k
-        rows, rhs, wts = [], [], []
-        for i in range(len(gid)):
-            for j in range(i + 1, len(gid)):
-                if gid[i] != gid[j]:
-                    w = min(abs(SI[i, j]), abs(SI[j, i])) ** 2
-                    if w > 0 and (cmin is None or w >= cmin ** 2):
-                        r = np.zeros(len(groups))
-                        r[gid[i]], r[gid[j]] = 1, -1
-                        rows.append(r)
-                        rhs.append(np.log(abs(SI[j, i] / SI[i, j])))
-                        wts.append(w)
-        for k, members in enumerate(groups):
-            r = np.zeros(len(groups))
-            r[k] = 1
-            rows.append(r)
-            rhs.append(np.log(np.mean(zphi[members])))
-            wts.append(prior)
-        sw = np.sqrt(wts)
-        logz, *_ = np.linalg.lstsq(np.array(rows) * sw[:, None], np.array(rhs) * sw, rcond=None)
-        logz += np.mean(np.log(zphi)) - np.mean(logz[gid])
-        return np.exp(logz[gid])
+
+def run_lanes(task, count, lanes, device, solvers, graphs=False):
+    """Run task(i, lane) for i < count, `lanes` at a time, round robin.
+    Each task is a generator that enqueues GPU work on the current stream and
+    yields before it waits for it; the lane's stream is made current whenever
+    its task resumes, so the GPU work of all lanes overlaps.  Tasks complete
+    in order within a lane, and a lane takes the next index as soon as its
+    task ends.  The solvers bound their LU panels for `lanes` factorisations
+    in flight meanwhile."""
+    prev = torch.cuda.current_stream(device)
+    own = lanes > 1 or graphs  # graphs are captured on a stream of their own
+    streams = [torch.cuda.Stream(device) for _ in range(lanes)] if own else [prev]
+    for st in streams:
+        if st is not prev:
+            st.wait_stream(prev)
+    for s in solvers:
+        s._active_lanes = lanes
+    active = [None] * lanes
+    nxt = 0
+    try:
+        while True:
+            busy = False
+            for lane in range(lanes):
+                if active[lane] is None and nxt < count:
+                    active[lane] = task(nxt, lane)
+                    nxt += 1
+                if active[lane] is None:
+                    continue
+                busy = True
+                torch.cuda.set_stream(streams[lane])
+                try:
+                    next(active[lane])
+                except StopIteration:
+                    active[lane] = None
+            if not busy:
+                break
+    finally:
+        torch.cuda.set_stream(prev)
+        for st in streams:
+            if st is not prev:
+                prev.wait_stream(st)
+        for s in solvers:
+            s._active_lanes = 1
+
+
+def _sweep_many(jobs, lanes, workers, keep_currents=False, graphs=False):
+    """Sweeps of (solver, freqs) jobs as one schedule of (job, frequency)
+    points on `lanes` streams; wave fits in `workers` processes (0: here)."""
+    out = []
+    points = []
+    t0 = time.time()
+    for ji, (s, freqs) in enumerate(jobs):
+        nf, n = len(freqs), len(s.ports)
+        o = dict(solver=s, freqs=freqs, S=np.zeros((nf, n, n), complex), gam=np.zeros((nf, n), complex),
+                 zl=np.zeros((nf, n), complex), zr=np.zeros((nf, n)), res=np.zeros((nf, n)),
+                 waves=np.zeros((nf, n, n), complex), cur=[None] * nf, futures={}, host=0.0)
+        o["tabs"] = s.tables(freqs)
+        s.geometry().upload(s.device)
+        o["B"] = torch.from_numpy(np.ascontiguousarray(s.port_excitations(), complex)).to(s.device)
+        o["ctx"] = s._fit_context()
+        out.append(o)
+        points += [(ji, fi) for fi in range(nf)]
+    device = jobs[0][0].device
+    torch.cuda.synchronize(device)
+    t1 = time.time()
+    pool = fit_pool(workers)
+    done = [0]
+
+    def store(o, fi, r):
+        Solver._store_point(r, fi, o["S"], o["gam"], o["zl"], o["zr"], o["res"], o["waves"])
+
+    def point(k, lane):
+        ji, fi = points[k]
+        o = out[ji]
+        s = o["solver"]
+        g = o["tabs"][fi]
+        x_dev = yield from s._solve_task(g, o["B"], lane, graphs)
+        yield  # let the other lanes enqueue their work before waiting for this one
+        x = x_dev.cpu().numpy()
+        if keep_currents:
+            o["cur"][fi] = x
+        h0 = time.perf_counter()
+        I, V = s._port_iv(g, x)
+        f = o["freqs"][fi]
+        if pool is None:
+            store(o, fi, sparams_point(o["ctx"], f, I, V))
+        else:
+            o["futures"][fi] = pool.submit(sparams_point, o["ctx"], f, I, V)
+        o["host"] += time.perf_counter() - h0
+        done[0] += 1
+        if s.verbose:
+            print(f"PROGRESS {done[0]}/{len(points)}", flush=True)
+
+    run_lanes(point, len(points), lanes, device, [o["solver"] for o in out], graphs)
+    for o in out:
+        for fi, fut in o["futures"].items():
+            store(o, fi, fut.result())
+    t2 = time.time()
+    return [SweepResult(o["freqs"], o["S"], o["zr"], o["gam"], o["zl"], o["res"], [p.name for p in o["solver"].ports],
+                        o["cur"] if keep_currents else None, o["waves"],
+                        timing=dict(tables=t1 - t0, solve=t2 - t1, sparams=o["host"])) for o in out]
+
+
+def sweep_batch(problems, precision="mixed", lanes=None, workers=None, graphs=False, keep_currents=False, **kw):
+    """Sweeps of several meshes of uneven size in one schedule.
+
+    problems : list of (mesh, Substrate | Stackup, freqs).
+    Every (mesh, frequency) point is a task of one lane scheduler, so the
+    latency-bound steps of the small problems (LU panels, triangular solves,
+    host de-embedding) overlap the matrix-core work of the others (graphs:
+    each lane replays its factorisation from a HIP graph per mesh).  Returns one
+    SweepResult per problem (Solver(mesh, stack).sweep(freqs) for each)."""
+    solvers = [Solver(m, st, verbose=False, precision=precision, graphs=graphs, **kw) for m, st, _ in problems]
+    jobs = [(s, np.atleast_1d(np.asarray(f, float))) for s, (_, _, f) in zip(solvers, problems)]
+    npts = sum(len(f) for _, f in jobs)
+    if lanes is None:
+        lanes = min(s._lane_count() for s in solvers)
+    if workers is None:
+        workers = solvers[0]._worker_count(npts)
+    return _sweep_many(jobs, max(1, min(lanes, npts)), workers, keep_currents, graphs)

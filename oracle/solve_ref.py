@@ -27,12 +27,10 @@ class OracleSolver:
     """CPU restatement of Solver(mesh, stack) with method='dense', interp=None."""
 
     def __init__(self, mesh, substrate, use_c=True, lapack_threads=None):
-        from mom25d_amd.solver import Solver
-
         # host-side port / grouping set-up shared with the product (pure numpy)
-        self.host = Solver.__new__(Solver)
+        from mom25d_amd.deembed import VOLTAGE_WEIGHT, FitContext
         from mom25d_amd.layered import Stackup
-        from mom25d_amd.solver import PortModel, VOLTAGE_WEIGHT
+        from mom25d_amd.solver import PortModel
 
         self.mesh = mesh
         self.stack = substrate if isinstance(substrate, Stackup) else substrate.to_stackup()
@@ -54,8 +52,8 @@ class OracleSolver:
         for p, port in enumerate(self.ports):
             key = (round(port.width / st.level_h(port.layer), 9), port.layer, None, port.z0_from)
             self.groups.setdefault(key, []).append(p)
-        h = self.host
-        h.ports, h.groups, h.z0_recip_min = self.ports, self.groups, None
+        self.z0_ctx = FitContext(groups=list(self.groups.values()), width=None, h=None, eps=None, covered=None,
+                                 z0_from=[p.z0_from for p in self.ports], s=None, lossless=None, v_weight=None)
         self.v_weight = VOLTAGE_WEIGHT
         self.timing = {}
 
@@ -114,7 +112,7 @@ class OracleSolver:
 
     def sweep(self, freqs):
         from mom25d_amd import analytic
-        from mom25d_amd.solver import fit_waves
+        from mom25d_amd.deembed import fit_waves, reciprocal_z0
 
         freqs = np.atleast_1d(np.asarray(freqs, float))
         nf, n = len(freqs), len(self.ports)
@@ -161,7 +159,7 @@ class OracleSolver:
                 for p in members:
                     gam[fi, p], zl[fi, p], res[fi, p] = gm, z0, r
             SI = -B @ np.linalg.inv(A)
-            zr[fi] = self.host._reciprocal_z0(SI, zl[fi].real)
+            zr[fi] = reciprocal_z0(self.z0_ctx, SI, zl[fi].real)
             d = np.sqrt(zr[fi])
             S[fi] = d[:, None] * SI / d[None, :]
             t4 = time.time()

@@ -74,3 +74,36 @@ def test_lanes_match_single_lane(precision):
     for a, b in zip(r1.currents, r3.currents):
         assert np.max(np.abs(a - b)) <= 1e-9 * np.max(np.abs(a))
     assert np.max(np.abs(r1.S - r3.S)) < 1e-8
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_replays_match_direct_calls(precision):
+    """Each lane's factorisation replayed from its HIP graph (captured on the
+    lane's second point, replayed on the later ones with the fill and the
+    host work in between) gives the currents of direct calls."""
+    lay, sub, mk, freqs = W.tee()
+    m = lay.mesh(**mk)
+    f = freqs[:9]
+    a = Solver(m, sub, verbose=False, precision=precision, lanes=2, graphs=False).sweep(f, keep_currents=True)
+    s = Solver(m, sub, verbose=False, precision=precision, lanes=2, graphs=True)
+    b = s.sweep(f, keep_currents=True)
+    assert len(s._graphs) == 2
+    for x, y in zip(a.currents, b.currents):
+        assert np.max(np.abs(x - y)) <= 1e-9 * np.max(np.abs(x))
+    assert np.max(np.abs(a.S - b.S)) < 1e-8
+
+
+def test_sweep_batch_matches_separate_sweeps():
+    """Three random layouts of uneven size in one schedule give the S of
+    their separate sweeps (to the wave fit's own noise: the fill accumulates
+    with atomics and the Nelder-Mead stop moves S by a few 1e-8 for inputs
+    that differ in the last digits)."""
+    from mom25d_amd import sweep_batch
+
+    probs = [W.random_chain(seed) for seed in (9, 3, 0)]
+    problems = [(lay.mesh(**mk), st, freqs[::16]) for lay, st, mk, freqs in probs]
+    batch = sweep_batch(problems, precision="mixed", lanes=3, workers=2)
+    for (m, st, f), r in zip(problems, batch):
+        ref = Solver(m, st, verbose=False, precision="mixed", lanes=1, workers=0, graphs=False).sweep(f)
+        print(f"N={m.n_unknowns}: |dS| batch vs separate {np.max(np.abs(r.S - ref.S)):.1e}")
+        assert np.max(np.abs(r.S - ref.S)) < 1e-7

@@ -145,3 +145,23 @@ def test_gmres_givens_solves_hessenberg_least_squares():
         ref, res, *_ = np.linalg.lstsq(H[:, :, c], e1, rcond=None)
         np.testing.assert_allclose(y, ref, rtol=1e-10, atol=1e-12)
         assert abs(abs(g[m, c]) - np.linalg.norm(e1 - H[:, :, c] @ ref)) < 1e-12
+
+
+def test_fit_pool_does_not_rerun_unguarded_scripts(tmp_path):
+    """The wave-fit workers are spawned processes; a caller script without a
+    __main__ guard must run once, not once per worker."""
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    script = tmp_path / "unguarded.py"
+    script.write_text(
+        "import sys\n"
+        f"sys.path.insert(0, {root!r})\n"
+        "print('MAIN RUN', flush=True)\n"
+        "import numpy as np\n"
+        "from mom25d_amd.solver import fit_pool\n"
+        "p = fit_pool(2)\n"
+        "print('sum', p.submit(np.add, 1, 2).result() + p.submit(np.add, 3, 4).result(), flush=True)\n")
+    out = subprocess.run([sys.executable, str(script)], capture_output=True, text=True, timeout=300).stdout
+    assert out.count("MAIN RUN") == 1 and "sum 10" in out
