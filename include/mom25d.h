@@ -92,6 +92,13 @@ int64_t mom_getrf_work_bytes(int prec, int n, int jb);
 int mom_getrf(int prec, int n, void* A, int64_t lda, int32_t* ipiv, int32_t* info, void* work, int64_t work_bytes,
               int jb, int max_groups, void* stream);
 
+/* Workgroup tile (rows x real columns) of the trailing-update GEMM of
+ * mom_getrf and mom_cgemm_sub, per precision: 64x64, 64x128, 128x64 or
+ * 128x128 (the autotuner's choice; default 64x128).  Applies to the calls
+ * issued after it, process-wide. */
+int mom_set_gemm_tile(int prec, int bm, int bn);
+int mom_get_gemm_tile(int prec, int* bm, int* bn);
+
 /* Row permutation of the factorisation: perm[i] = original row of row i. */
 int mom_build_perm(int n, const int32_t* ipiv, int32_t* perm, void* stream);
 

@@ -1,0 +1,116 @@
+"""Autotuner of the LU's launch shapes on the device at hand.
+
+For a matrix size and precision it times mom_getrf on a synthetic complex
+matrix of that size (random entries plus a dominant diagonal, so that the
+pivoting does the same work as on an impedance matrix) for every candidate
+panel width and trailing-update tile, and keeps the fastest.  Results are
+cached per (precision, size rounded up to a multiple of 512) in memory and,
+when MOM25D_TUNE_CACHE names a file, in that JSON file.
+"""
+
+import json
+import os
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+
+PANELS = (32, 64, 128)
+TILES = ((64, 64), (64, 128), (128, 64), (128, 128))
+_CACHE = {}
+
+
+def _key(prec, n):
+    return f"{prec}:{-(-n // 512) * 512}"
+
+
+def _load():
+    path = os.environ.get("MOM25D_TUNE_CACHE")
+    if path and os.path.exists(path) and not _CACHE.get("_loaded"):
+        with open(path) as fh:
+            _CACHE.update(json.load(fh))
+        _CACHE["_loaded"] = True
+
+
+def _save():
+    path = os.environ.get("MOM25D_TUNE_CACHE")
+    if path:
+        with open(path, "w") as fh:
+            json.dump({k: v for k, v in _CACHE.items() if not k.startswith("_")}, fh, indent=1)
+
+
+def set_tile(prec, tile):
+    _lib.call("mom_set_gemm_tile", prec, int(tile[0]), int(tile[1]))
+
+
+def get_tile(prec):
+    import ctypes
+
+    bm, bn = ctypes.c_int(), ctypes.c_int()
+    _lib.call("mom_get_gemm_tile", prec, ctypes.byref(bm), ctypes.byref(bn))
+    return bm.value, bn.value
+
+
+def time_getrf(n, prec, jb, tile, device="cuda", reps=2, seed=0):
+    """Median milliseconds of mom_getrf (n, prec, jb, tile) on a synthetic matrix."""
+    dev = torch.device(device)
+    dt = torch.complex128 if prec == 64 else torch.complex64
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    A0 = torch.complex(torch.randn(n, n, device=dev, generator=g, dtype=torch.float64),
+                       torch.randn(n, n, device=dev, generator=g, dtype=torch.float64)).to(dt)
+    A0.diagonal().add_(float(np.sqrt(n)) * 4)
+    A = torch.empty_like(A0)
+    ipiv = torch.empty(n, dtype=torch.int32, device=dev)
+    info = torch.zeros(4, dtype=torch.int32, device=dev)
+    wb = int(_lib.lib().mom_getrf_work_bytes(prec, n, jb))
+    work = torch.empty(wb, dtype=torch.uint8, device=dev)
+    old = get_tile(prec)
+    set_tile(prec, tile)
+    P = _lib.ptr
+    s = _lib.stream_ptr()
+    times = []
+    try:
+        for _ in range(reps + 1):
+            A.copy_(A0)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            _lib.call("mom_getrf", prec, n, P(A), n, P(ipiv), P(info), P(work), wb, jb, 0, s)
+            e1.record()
+            torch.cuda.synchronize(dev)
+            times.append(e0.elapsed_time(e1))
+    finally:
+        set_tile(prec, old)
+    return float(np.median(times[1:]))
+
+
+def tune_lu(n, prec, device="cuda", panels=PANELS, tiles=TILES, reps=2, verbose=False):
+    """The fastest (panel width, tile) for an n x n factorisation in prec
+    (64: complex128, 32: complex64): dict(jb, tile, ms, table)."""
+    _load()
+    key = _key(prec, n)
+    if key in _CACHE:
+        return _CACHE[key]
+    m = -(-n // 512) * 512
+    t0 = time.perf_counter()
+    table = []
+    for jb in panels:
+        for tile in tiles:
+            try:
+                ms = time_getrf(m, prec, jb, tile, device, reps)
+            except RuntimeError as e:  # e.g. a panel too large for the LDS at this width
+                if verbose:
+                    print(f"  autotune n={m} prec={prec} jb={jb} tile={tile}: {e}", flush=True)
+                continue
+            table.append((jb, list(tile), ms))
+            if verbose:
+                print(f"  autotune n={m} prec={prec} jb={jb} tile={tile}: {ms:.2f} ms", flush=True)
+    if not table:
+        raise RuntimeError(f"autotune: no panel width / tile runs for n={m}")
+    jb, tile, ms = min(table, key=lambda r: r[2])
+    best = dict(jb=jb, tile=tile, ms=ms, n=m, seconds=time.perf_counter() - t0, table=table)
+    _CACHE[key] = best
+    _save()
+    return best

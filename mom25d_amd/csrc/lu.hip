@@ -325,7 +325,7 @@ struct mfma_traits<float> {
 // form a 2 x 2 grid, each owning (BM/2) x (BN/2) as MFMA 16x16 tiles.  K' is
 // streamed in stages of 16 real columns through double-buffered LDS: the
 // global loads of stage s+1 are in flight while stage s runs on the matrix
-// cores.  Tile sizes are chosen per precision (gemm_tiles below).
+// cores.  The tile is chosen per precision at run time (g_tile below).
 constexpr int kGemmBK = 16;
 
 template <typename R, int BM, int BN>
@@ -420,38 +420,32 @@ __global__ __launch_bounds__(256) void cgemm_sub_kernel(int m, int n, int kk, co
       }
 }
 
-#ifndef MOM_GEMM64_BM
-#define MOM_GEMM64_BM 64
-#endif
-#ifndef MOM_GEMM64_BN
-#define MOM_GEMM64_BN 128
-#endif
-#ifndef MOM_GEMM32_BM
-#define MOM_GEMM32_BM 64
-#endif
-#ifndef MOM_GEMM32_BN
-#define MOM_GEMM32_BN 128
-#endif
+// Workgroup tiles of the trailing update, chosen per precision at run time
+// (mom_set_gemm_tile; the autotuner sets them).  Defaults: the fastest on
+// MI355X for the N ~ 12k factorisation.
+static int g_tile[2][2] = {{64, 128}, {64, 128}};  // [complex128, complex64][BM, BN]
 
-template <typename R>
-struct gemm_tiles;
-template <>
-struct gemm_tiles<double> {
-  static constexpr int BM = MOM_GEMM64_BM, BN = MOM_GEMM64_BN;
-};
-template <>
-struct gemm_tiles<float> {
-  static constexpr int BM = MOM_GEMM32_BM, BN = MOM_GEMM32_BN;
-};
-
-template <typename R>
-static int launch_cgemm(int m, int n, int k, const cplx<R>* A, int64_t lda, const cplx<R>* B, int64_t ldb,
-                        cplx<R>* C, int64_t ldc, hipStream_t s) {
-  constexpr int BM = gemm_tiles<R>::BM, BN = gemm_tiles<R>::BN;
+template <typename R, int BM, int BN>
+static int launch_cgemm_tile(int m, int n, int k, const cplx<R>* A, int64_t lda, const cplx<R>* B, int64_t ldb,
+                             cplx<R>* C, int64_t ldc, hipStream_t s) {
   if ((m + BM - 1) / BM > 65535) return fail("cgemm: too many row tiles");
   dim3 grid((2 * n + BN - 1) / BN, (m + BM - 1) / BM);
   hipLaunchKernelGGL((cgemm_sub_kernel<R, BM, BN>), grid, dim3(256), 0, s, m, n, k, A, lda, B, ldb, C, ldc);
   return check_launch("cgemm_sub_kernel");
+}
+
+template <typename R>
+static int launch_cgemm(int m, int n, int k, const cplx<R>* A, int64_t lda, const cplx<R>* B, int64_t ldb,
+                        cplx<R>* C, int64_t ldc, hipStream_t s) {
+  const int* t = g_tile[sizeof(R) == sizeof(double) ? 0 : 1];
+#define MOM_TILE(BM, BN) \
+  if (t[0] == BM && t[1] == BN) return launch_cgemm_tile<R, BM, BN>(m, n, k, A, lda, B, ldb, C, ldc, s);
+  MOM_TILE(64, 64)
+  MOM_TILE(64, 128)
+  MOM_TILE(128, 64)
+  MOM_TILE(128, 128)
+#undef MOM_TILE
+  return fail("cgemm: unsupported tile");
 }
 
 // ------------------------------------------------------------ solve phase
@@ -965,6 +959,22 @@ extern "C" int mom_getrs(int prec, int n, int nrhs, const void* LU, int64_t lda,
   if (prec == 64) return getrs<double>(n, nrhs, (const z64*)LU, lda, perm, (const z64*)B, (z64*)X, flags, s);
   if (prec == 32) return getrs<float>(n, nrhs, (const c32*)LU, lda, perm, (const c32*)B, (c32*)X, flags, s);
   return fail("mom_getrs: prec must be 64 or 32");
+}
+
+extern "C" int mom_set_gemm_tile(int prec, int bm, int bn) {
+  if (prec != 64 && prec != 32) return fail("mom_set_gemm_tile: prec must be 64 or 32");
+  if (!((bm == 64 || bm == 128) && (bn == 64 || bn == 128)))
+    return fail("mom_set_gemm_tile: tile must be 64x64, 64x128, 128x64 or 128x128");
+  g_tile[prec == 64 ? 0 : 1][0] = bm;
+  g_tile[prec == 64 ? 0 : 1][1] = bn;
+  return 0;
+}
+
+extern "C" int mom_get_gemm_tile(int prec, int* bm, int* bn) {
+  if (prec != 64 && prec != 32) return fail("mom_get_gemm_tile: prec must be 64 or 32");
+  *bm = g_tile[prec == 64 ? 0 : 1][0];
+  *bn = g_tile[prec == 64 ? 0 : 1][1];
+  return 0;
 }
 
 extern "C" int mom_cgemm_sub(int prec, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb,

@@ -324,6 +324,9 @@ class Solver:
                 judged per frequency by the Sommerfeld integrals at a few
                 nodes (relative tolerance surrogate_tol; the integrals are
                 used throughout where it misses).
+    autotune  : time the LU's panel widths and update tiles for this N and
+                precision on the device (mom25d_amd.autotune, cached per size
+                class) and use the fastest; panel then only sets the default.
     graphs    : in a sweep, replay each lane's factorisation (and the FP64
                 solve) from a HIP graph captured once per lane (off by
                 default: on this ROCm the replay of these multi-stream graphs
@@ -342,7 +345,7 @@ class Solver:
 
     def __init__(self, mesh, substrate, z0=None, verbose=True, method="auto", v_weight=None, z0_recip_min=None,
                  tand_floor=None, precision="fp64", device="cuda", panel=None, refine_tol=1e-15, refine_maxit=20,
-                 lanes=None, surrogate=None, surrogate_tol=2e-3, workers=None, graphs=False):
+                 lanes=None, surrogate=None, surrogate_tol=2e-3, workers=None, graphs=False, autotune=False):
         from .layered import Stackup
 
         if method not in ("auto", "dense"):
@@ -376,7 +379,14 @@ class Solver:
         self.v_weight = VOLTAGE_WEIGHT if v_weight is None else float(v_weight)
         self.z0_recip_min = z0_recip_min
         self.device = torch.device(device)
-        self.panel = panel or (64 if precision == "fp64" else 64)
+        self.panel = panel or 64
+        self.tile = None
+        if autotune:
+            from .autotune import tune_lu
+
+            best = tune_lu(mesh.n_unknowns, 64 if precision == "fp64" else 32, self.device)
+            self.panel, self.tile = best["jb"], tuple(best["tile"])
+        self.tuning = dict(jb=self.panel, tile=self.tile)
         self.refine_tol = refine_tol
         self.refine_maxit = refine_maxit
         self.gmres_restart = 30
@@ -595,6 +605,9 @@ class Solver:
         groups = self._panel_groups()
         sw = self._buf(f"swork{prec}", (int(_lib.lib().mom_getrs_work_bytes(prec, n)),), torch.uint8, lane)
         X = self._buf("X", (n, nrhs), torch.complex128, lane)
+
+        if self.tile is not None:
+            _lib.call("mom_set_gemm_tile", prec, self.tile[0], self.tile[1])
 
         def calls():
             _lib.call("mom_getrf", prec, n, P(A), A.stride(0), P(ipiv), P(info), P(work), wbytes, jb, groups, stream)

@@ -63,19 +63,50 @@ def test_j0_matches_mpmath():
     assert np.max(np.abs(gotr - refr) / np.sqrt(2 / (np.pi * xr))) < 1e-14
 
 
+@pytest.mark.parametrize("tile", [(64, 64), (64, 128), (128, 64), (128, 128)])
 @pytest.mark.parametrize("prec,tol", [(64, 1e-13), (32, 2e-5)])
 @pytest.mark.parametrize("m,n,k", [(64, 32, 64), (130, 77, 40), (257, 300, 64), (1000, 513, 128)])
-def test_cgemm_sub(prec, tol, m, n, k):
+def test_cgemm_sub(prec, tol, m, n, k, tile):
+    from mom25d_amd.autotune import get_tile, set_tile
+
     rng = np.random.default_rng(m + n + k)
     dt = torch.complex128 if prec == 64 else torch.complex64
     A = rng.standard_normal((m, k)) + 1j * rng.standard_normal((m, k))
     B = rng.standard_normal((k, n)) + 1j * rng.standard_normal((k, n))
     C = rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))
     Ad, Bd, Cd = (torch.from_numpy(a).to(DEV, dt) for a in (A, B, C))
-    _lib.call("mom_cgemm_sub", prec, m, n, k, _lib.ptr(Ad), k, _lib.ptr(Bd), n, _lib.ptr(Cd), n, _lib.stream_ptr())
+    old = get_tile(prec)
+    set_tile(prec, tile)
+    try:
+        _lib.call("mom_cgemm_sub", prec, m, n, k, _lib.ptr(Ad), k, _lib.ptr(Bd), n, _lib.ptr(Cd), n,
+                  _lib.stream_ptr())
+        got = Cd.cpu().numpy()
+    finally:
+        set_tile(prec, old)
     ref = C - A @ B
-    got = Cd.cpu().numpy()
     assert np.max(np.abs(got - ref)) < tol * np.max(np.abs(ref)) * np.sqrt(k)
+
+
+def test_autotuned_lu_solves_like_the_default():
+    """The autotuner times every panel width / tile and returns the fastest;
+    the factorisation with that choice solves a system like the default."""
+    from mom25d_amd.autotune import get_tile, set_tile, tune_lu
+
+    best = tune_lu(700, 64, reps=1)
+    print(f"autotune n={best['n']} complex128:", best["jb"], best["tile"], f"{best['ms']:.2f} ms", len(best["table"]), "runs")
+    assert len(best["table"]) == 12 and best["ms"] == min(r[2] for r in best["table"])
+    rng = np.random.default_rng(5)
+    n = 700
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    B = rng.standard_normal((n, 2)) + 1j * rng.standard_normal((n, 2))
+    old = get_tile(64)
+    set_tile(64, best["tile"])
+    try:
+        X, _, info = _lu_solve(A, B, 64, best["jb"])
+    finally:
+        set_tile(64, old)
+    assert info == 0
+    assert np.linalg.norm(A @ X - B) <= 1e-12 * np.linalg.norm(A) * np.linalg.norm(X)
 
 
 def _lu_solve(A, B, prec, jb):

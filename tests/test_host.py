@@ -115,6 +115,10 @@ def test_error_path_reports_message():
         _lib.call("mom_cgemm_sub", 16, 4, 4, 4, None, 4, None, 4, None, 4, None)
     with pytest.raises(RuntimeError, match="ncol must be"):
         _lib.call("mom_sommerfeld", 1, None, None, None, None, None, None, None, None, None, None, 5, None, None)
+    with pytest.raises(RuntimeError, match="tile must be"):
+        _lib.call("mom_set_gemm_tile", 64, 32, 64)
+    with pytest.raises(RuntimeError, match="nslots must be"):
+        _lib.call("mom_mlp_train_step", 100, 1, 1, 1, 1.0, 1, 1, 1, 1, None)
 
 
 def test_gmres_givens_solves_hessenberg_least_squares():
