@@ -43,10 +43,10 @@ def _cpu_model():
     return platform.processor() or "unknown"
 
 
-def time_path(mesh, sub, freqs, precision, warmup, steps, torch, lanes=None):
+def time_path(mesh, sub, freqs, precision, warmup, steps, torch, lanes=None, autotune=True):
     from mom25d_amd import Solver
 
-    s = Solver(mesh, sub, verbose=False, precision=precision, lanes=lanes)
+    s = Solver(mesh, sub, verbose=False, precision=precision, lanes=lanes, autotune=autotune)
     s.geometry().upload(s.device)
     nf = len(freqs)
     pick = lambda a, b: freqs[np.arange(a, b) % nf]  # noqa: E731
@@ -88,6 +88,24 @@ def fill_time(solver, freq, torch, reps=3):
     return ev[0].elapsed_time(ev[1]) * 1e-3 / reps
 
 
+def batch_random16(torch, precision):
+    """The 16 seeded random layouts (N 601-2965, 64 frequencies each) as one
+    batched sweep (sweep_batch); the first two problems warm it up."""
+    from mom25d_amd import sweep_batch
+    from mom25d_amd import workloads as W
+
+    problems = [(lay.mesh(**mk), st, f) for lay, st, mk, f in W.random16()]
+    sweep_batch(problems[:2], precision=precision)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rs = sweep_batch(problems, precision=precision)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    npts = sum(len(f) for _, _, f in problems)
+    return dict(points=npts, seconds=dt, points_per_s=npts / dt, precision=precision,
+                N=[m.n_unknowns for m, _, _ in problems], host_fit_seconds=sum(r.timing["sparams"] for r in rs))
+
+
 def cpu_baseline(mesh, sub, freqs, npts):
     from threadpoolctl import threadpool_limits
 
@@ -119,6 +137,8 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=("fp64", "mixed"), help="headline path")
     ap.add_argument("--no-alt", action="store_true", help="skip timing the other path")
     ap.add_argument("--lanes", type=int, default=None, help="frequency points in flight (default: automatic)")
+    ap.add_argument("--no-tune", action="store_true", help="default LU panel width / tiles instead of the autotuner's")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched sweep of the 16 random layouts")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -138,12 +158,12 @@ def main():
 
     prec = args.precision
     alt = "mixed" if prec == "fp64" else "fp64"
-    sm, rm, tm = time_path(mesh, sub, freqs, prec, args.warmup, args.steps, torch, args.lanes)
+    sm, rm, tm = time_path(mesh, sub, freqs, prec, args.warmup, args.steps, torch, args.lanes, not args.no_tune)
     lanes = sm._lane_count()
     print(f"{prec}: {args.steps} points in {tm:.3f} s ({sm.last_refine} refinement steps)", file=sys.stderr, flush=True)
     other = None
     if not args.no_alt:
-        sa, ra, ta = time_path(mesh, sub, freqs, alt, args.warmup, args.steps, torch, args.lanes)
+        sa, ra, ta = time_path(mesh, sub, freqs, alt, args.warmup, args.steps, torch, args.lanes, not args.no_tune)
         other = dict(precision=alt, value=args.steps / ta, ms_per_step=1e3 * ta / args.steps,
                      dS_max_vs_headline=float(np.max(np.abs(rm.S - ra.S))), refinement_steps_last=sa.last_refine)
         print(f"{alt}: {args.steps} points in {ta:.3f} s; |dS| {other['dS_max_vs_headline']:.2e}", file=sys.stderr,
@@ -154,7 +174,10 @@ def main():
     calls, secs, flops = gemm_roofline(sm, f_probe, torch)
     achieved = flops / secs if secs > 0 else 0.0
     peak = PEAK["fp64_mfma"] if prec == "fp64" else PEAK["fp32_mfma"]
-    kname = "cgemm_sub_kernel<double, 64, 128>" if prec == "fp64" else "cgemm_sub_kernel<float, 64, 128>"
+    from mom25d_amd.autotune import get_tile
+
+    bm, bn = sm.tile or get_tile(64 if prec == "fp64" else 32)
+    kname = f"cgemm_sub_kernel<{'double' if prec == 'fp64' else 'float'}, {bm}, {bn}>"
     roof = dict(bound="mfma", achieved=achieved / 1e12, peak=peak / 1e12, unit="TFLOP/s", frac=achieved / peak,
                 traffic=None, kernel=f"{kname} (LU trailing update)",
                 launches=calls, avg_launch_ms=1e3 * secs / max(calls, 1), flops_per_launch=flops / max(calls, 1))
@@ -162,6 +185,12 @@ def main():
     fill_flops = sm.geometry().algorithmic_flops()
     extra = dict(fill=dict(seconds=tf, algorithmic_flops=fill_flops, achieved_tflops=fill_flops / tf / 1e12,
                            frac_fp64_valu=fill_flops / tf / PEAK["fp64_valu"]))
+    extra["lu_config"] = dict(panel=sm.panel, tile=[bm, bn], autotuned=not args.no_tune)
+    if not args.no_batch:
+        extra["batch_random16"] = batch_random16(torch, prec)
+        b = extra["batch_random16"]
+        print(f"batch random16 ({prec}): {b['points']} points in {b['seconds']:.2f} s = {b['points_per_s']:.1f} points/s",
+              file=sys.stderr, flush=True)
     cpu = None
     if args.cpu_sample > 0:
         cpu = cpu_baseline(mesh, sub, freqs, args.cpu_sample)
