@@ -75,6 +75,14 @@ struct FillArgs {
   double* Z;      // complex, row-major
   int64_t ldz;    // leading dimension in complex elements
   double* IVpad;  // complex (npad, Nt) or null
+  // scatter through LDS: the distinct rows (test basis functions) of every
+  // block and columns (source basis functions) of every work item
+  const int32_t* blk_row_off;  // (nblk + 1) offsets into blk_rows
+  const int32_t* blk_rows;     // basis index of each block row
+  const int32_t* blk_rloc;     // (nblk, 32, 3) row of each block triangle's edge, or -1
+  const int32_t* item_cols;    // basis index of each item column (offsets in items[4])
+  const int32_t* src_cloc;     // (len(src), 3) column of each source triangle's edge, or -1
+  int64_t acc_bytes;           // LDS accumulator bytes of the launch (max over its items)
 };
 
 __device__ inline int spline_interval(const double* __restrict__ x, const int32_t* __restrict__ bucket, int n,
@@ -142,19 +150,32 @@ __device__ inline void wilton(const double* Vs, double rx, double ry, double d, 
   }
 }
 
+// One workgroup per work item (test block, source chunk of one tier): lane l
+// of every wavefront owns source triangle l, wavefront w walks the block's
+// test triangles w, w + 4, ...  The 3 x 3 contributions of every pair are
+// summed in LDS per (block row, item column) -- the item's distinct test and
+// source basis functions -- and each entry goes to Z with one atomic add, so
+// that Z sees a third of the scattered updates of a per-pair scatter.
+constexpr int kFillThreads = 256;
+
 template <int TIER>
-__global__ __launch_bounds__(256) void fill_kernel(FillArgs a) {
+__global__ __launch_bounds__(kFillThreads) void fill_kernel(FillArgs a) {
   constexpr int L = TIER == 0 ? 7 : (TIER == 1 ? 3 : 1);
+  constexpr int NW = kFillThreads / 64;
+  extern __shared__ double acc[];  // (nr, nc) complex, interleaved
   __shared__ double s_qp[kBlock][14];
   __shared__ double s_qw[kBlock][7];
   __shared__ double s_V[kBlock][6];
   __shared__ double s_cA[kBlock][3];
   __shared__ double s_cV[kBlock][3];
-  __shared__ int32_t s_tb[kBlock][3];
+  __shared__ int32_t s_rl[kBlock][3];
   __shared__ int32_t s_tri[kBlock];
+  __shared__ int32_t s_rows[3 * kBlock];
+  __shared__ int32_t s_cols[3 * 64];
 
-  const int* item = a.items + 4 * (int64_t)blockIdx.x;
-  const int blk = item[0], pp = item[1], soff = item[2], scnt = item[3];
+  const int* item = a.items + 6 * (int64_t)blockIdx.x;
+  const int blk = item[0], pp = item[1], soff = item[2], scnt = item[3], coff = item[4], nc = item[5];
+  const int r0 = a.blk_row_off[blk], nr = a.blk_row_off[blk + 1] - r0;
   const int tid = threadIdx.x;
   if (tid < kBlock) {
     int p = a.blk_tris[blk * kBlock + tid];
@@ -166,10 +187,13 @@ __global__ __launch_bounds__(256) void fill_kernel(FillArgs a) {
       for (int q = 0; q < 3; ++q) {
         s_cA[tid][q] = a.cA[3 * (int64_t)p + q];
         s_cV[tid][q] = a.cV[3 * (int64_t)p + q];
-        s_tb[tid][q] = a.tb[3 * (int64_t)p + q];
+        s_rl[tid][q] = a.blk_rloc[(blk * kBlock + tid) * 3 + q];
       }
     }
   }
+  for (int e = tid; e < nr; e += kFillThreads) s_rows[e] = a.blk_rows[r0 + e];
+  for (int e = tid; e < nc; e += kFillThreads) s_cols[e] = a.item_cols[coff + e];
+  for (int e = tid; e < 2 * nr * nc; e += kFillThreads) acc[e] = 0.0;
   __syncthreads();
 
   const PairTab pt = a.pairs[pp];
@@ -181,108 +205,115 @@ __global__ __launch_bounds__(256) void fill_kernel(FillArgs a) {
   const z64 jwmu = {a.jwmu_re, a.jwmu_im}, ijwe = {a.ijwe_re, a.ijwe_im};
 
   const int lane = tid & 63, wave = tid >> 6;
-  if (lane >= scnt) return;
-  const int s = a.src[soff + lane];
-
-  double Vs[6], px[L], py[L], w[L], cAs[3], cVs[3];
-  int tbs[3];
+  if (lane < scnt) {
+    const int s = a.src[soff + lane];
+    double Vs[6], px[L], py[L], w[L], cAs[3], cVs[3];
+    int cl[3];
 #pragma unroll
-  for (int q = 0; q < 6; ++q) Vs[q] = a.V[6 * (int64_t)s + q];
+    for (int q = 0; q < 6; ++q) Vs[q] = a.V[6 * (int64_t)s + q];
 #pragma unroll
-  for (int l = 0; l < L; ++l) {
-    if (TIER == 0) {
-      px[l] = a.qp7[14 * (int64_t)s + 2 * l];
-      py[l] = a.qp7[14 * (int64_t)s + 2 * l + 1];
-      w[l] = a.qw7[7 * (int64_t)s + l];
-    } else if (TIER == 1) {
-      px[l] = a.qp3[6 * (int64_t)s + 2 * l];
-      py[l] = a.qp3[6 * (int64_t)s + 2 * l + 1];
-      w[l] = a.qw3[3 * (int64_t)s + l];
-    } else {
-      px[l] = a.cen[2 * (int64_t)s];
-      py[l] = a.cen[2 * (int64_t)s + 1];
-      w[l] = a.area[s];
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    cAs[q] = a.cA[3 * (int64_t)s + q];
-    cVs[q] = a.cV[3 * (int64_t)s + q];
-    tbs[q] = a.tb[3 * (int64_t)s + q];
-  }
-  const int pad = a.IVpad ? 0 : -1;
-
-  for (int pl = wave; pl < kBlock; pl += 4) {
-    const int p = s_tri[pl];
-    if (p < 0) break;
-    z64 IA[3][3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) IA[i][j] = {0.0, 0.0};
-    z64 IV = {0.0, 0.0};
-    for (int k = 0; k < 7; ++k) {
-      const double rx = s_qp[pl][2 * k], ry = s_qp[pl][2 * k + 1];
-      z64 k0a, k1x, k1y, k0v;
+    for (int l = 0; l < L; ++l) {
       if (TIER == 0) {
-        double I0, I1x, I1y;
-        wilton(Vs, rx, ry, dz, I0, I1x, I1y);
-        const double S0 = I0 / kFourPi;
-        const double S1x = (rx * I0 + I1x) / kFourPi, S1y = (ry * I0 + I1y) / kFourPi;
-        k0a = kA * S0; k1x = kA * S1x; k1y = kA * S1y; k0v = kV * S0;
+        px[l] = a.qp7[14 * (int64_t)s + 2 * l];
+        py[l] = a.qp7[14 * (int64_t)s + 2 * l + 1];
+        w[l] = a.qw7[7 * (int64_t)s + l];
+      } else if (TIER == 1) {
+        px[l] = a.qp3[6 * (int64_t)s + 2 * l];
+        py[l] = a.qp3[6 * (int64_t)s + 2 * l + 1];
+        w[l] = a.qw3[3 * (int64_t)s + l];
       } else {
-        k0a = {0.0, 0.0}; k1x = k0a; k1y = k0a; k0v = k0a;
+        px[l] = a.cen[2 * (int64_t)s];
+        py[l] = a.cen[2 * (int64_t)s + 1];
+        w[l] = a.area[s];
       }
+    }
 #pragma unroll
-      for (int l = 0; l < L; ++l) {
-        const double ddx = rx - px[l], ddy = ry - py[l];
-        const double R = sqrt(ddx * ddx + ddy * ddy);
-        z64 sa, sv;
-        spline_eval(pt, tx, tc, tbk, R, sa, sv);
-        if (TIER != 0) {
-          const double inv = (dz == 0.0) ? 1.0 / (kFourPi * R) : 1.0 / (kFourPi * sqrt(R * R + dz * dz));
-          sa += kA * inv;
-          sv += kV * inv;
+    for (int q = 0; q < 3; ++q) {
+      cAs[q] = a.cA[3 * (int64_t)s + q];
+      cVs[q] = a.cV[3 * (int64_t)s + q];
+      cl[q] = a.src_cloc[3 * (int64_t)(soff + lane) + q];
+    }
+    const bool pads = a.IVpad != nullptr;
+
+    for (int pl = wave; pl < kBlock; pl += NW) {
+      const int p = s_tri[pl];
+      if (p < 0) break;
+      z64 IA[3][3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) IA[i][j] = {0.0, 0.0};
+      z64 IV = {0.0, 0.0};
+      for (int k = 0; k < 7; ++k) {
+        const double rx = s_qp[pl][2 * k], ry = s_qp[pl][2 * k + 1];
+        z64 k0a, k1x, k1y, k0v;
+        if (TIER == 0) {
+          double I0, I1x, I1y;
+          wilton(Vs, rx, ry, dz, I0, I1x, I1y);
+          const double S0 = I0 / kFourPi;
+          const double S1x = (rx * I0 + I1x) / kFourPi, S1y = (ry * I0 + I1y) / kFourPi;
+          k0a = kA * S0; k1x = kA * S1x; k1y = kA * S1y; k0v = kV * S0;
+        } else {
+          k0a = {0.0, 0.0}; k1x = k0a; k1y = k0a; k0v = k0a;
         }
-        const z64 saw = sa * w[l];
-        k0a += saw;
-        k1x += saw * px[l];
-        k1y += saw * py[l];
-        k0v += sv * w[l];
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+          const double ddx = rx - px[l], ddy = ry - py[l];
+          const double R = sqrt(ddx * ddx + ddy * ddy);
+          z64 sa, sv;
+          spline_eval(pt, tx, tc, tbk, R, sa, sv);
+          if (TIER != 0) {
+            const double inv = (dz == 0.0) ? 1.0 / (kFourPi * R) : 1.0 / (kFourPi * sqrt(R * R + dz * dz));
+            sa += kA * inv;
+            sv += kV * inv;
+          }
+          const z64 saw = sa * w[l];
+          k0a += saw;
+          k1x += saw * px[l];
+          k1y += saw * py[l];
+          k0v += sv * w[l];
+        }
+        const double wk = s_qw[pl][k];
+        IV += wk * k0v;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          const double ax = wk * (rx - s_V[pl][2 * i]);
+          const double ay = wk * (ry - s_V[pl][2 * i + 1]);
+          const z64 t1 = ax * k1x + ay * k1y;
+#pragma unroll
+          for (int j = 0; j < 3; ++j) IA[i][j] += t1 - (ax * Vs[2 * j] + ay * Vs[2 * j + 1]) * k0a;
+        }
       }
-      const double wk = s_qw[pl][k];
-      IV += wk * k0v;
+      if (pads) {
+        const int pr = a.pad_pos[p];
+        if (pr >= 0) {
+          z64* dst = reinterpret_cast<z64*>(a.IVpad) + (int64_t)pr * a.nt + s;
+          *dst = IV;
+        }
+      }
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
-        const double ax = wk * (rx - s_V[pl][2 * i]);
-        const double ay = wk * (ry - s_V[pl][2 * i + 1]);
-        const z64 t1 = ax * k1x + ay * k1y;
+        const int rl = s_rl[pl][i];
+        if (rl < 0) continue;
+        const double cAi = s_cA[pl][i], cVi = s_cV[pl][i];
 #pragma unroll
-        for (int j = 0; j < 3; ++j) IA[i][j] += t1 - (ax * Vs[2 * j] + ay * Vs[2 * j + 1]) * k0a;
+        for (int j = 0; j < 3; ++j) {
+          if (cl[j] < 0) continue;
+          const z64 v = (jwmu * (cAi * cAs[j])) * IA[i][j] + (ijwe * (cVi * cVs[j])) * IV;
+          double* dst = acc + 2 * (rl * nc + cl[j]);
+          atomicAdd(dst, v.re);
+          atomicAdd(dst + 1, v.im);
+        }
       }
     }
-    if (pad == 0) {
-      const int pr = a.pad_pos[p];
-      if (pr >= 0) {
-        z64* dst = reinterpret_cast<z64*>(a.IVpad) + (int64_t)pr * a.nt + s;
-        *dst = IV;
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      const int row = s_tb[pl][i];
-      if (row < 0) continue;
-      const double cAi = s_cA[pl][i], cVi = s_cV[pl][i];
-#pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        const int col = tbs[j];
-        if (col < 0) continue;
-        const z64 v = (jwmu * (cAi * cAs[j])) * IA[i][j] + (ijwe * (cVi * cVs[j])) * IV;
-        double* dst = a.Z + 2 * ((int64_t)row * a.ldz + col);
-        unsafeAtomicAdd(dst, v.re);
-        unsafeAtomicAdd(dst + 1, v.im);
-      }
-    }
+  }
+  __syncthreads();
+  for (int e = tid; e < nr * nc; e += kFillThreads) {
+    const double re = acc[2 * e], im = acc[2 * e + 1];
+    if (re == 0.0 && im == 0.0) continue;
+    double* dst = a.Z + 2 * ((int64_t)s_rows[e / nc] * a.ldz + s_cols[e % nc]);
+    unsafeAtomicAdd(dst, re);
+    unsafeAtomicAdd(dst + 1, im);
   }
 }
 
@@ -486,20 +517,36 @@ __global__ __launch_bounds__(256) void line_voltage_kernel(int npts, const doubl
 
 using namespace mom;
 
+static size_t g_fill_lds[3] = {0, 0, 0};
+
+template <int TIER>
+static int launch_fill(int n_items, const FillArgs& a, hipStream_t s) {
+  const size_t lds = (size_t)a.acc_bytes;
+  if (lds > g_fill_lds[TIER]) {
+    MOM_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(fill_kernel<TIER>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    g_fill_lds[TIER] = lds;
+  }
+  hipLaunchKernelGGL(fill_kernel<TIER>, dim3(n_items), dim3(kFillThreads), lds, s, a);
+  return check_launch("fill_kernel");
+}
+
 extern "C" int mom_fill_tier(int tier, int n_items, const void* args, void* stream) {
   if (n_items <= 0) return n_items == 0 ? 0 : fail("mom_fill_tier: negative item count");
   if (!args) return fail("mom_fill_tier: null argument block");
   const FillArgs a = *reinterpret_cast<const FillArgs*>(args);
-  if (!a.Z || !a.items || !a.src || !a.pairs) return fail("mom_fill_tier: null device pointer");
+  if (!a.Z || !a.items || !a.src || !a.pairs || !a.blk_row_off || !a.blk_rows || !a.blk_rloc || !a.item_cols ||
+      !a.src_cloc)
+    return fail("mom_fill_tier: null device pointer");
+  // LDS: the accumulator plus about 7 KB of block data, within the 160 KB of a CU
+  if (a.acc_bytes < 0 || a.acc_bytes > 144 * 1024) return fail("mom_fill_tier: LDS accumulator out of range");
   hipStream_t s = (hipStream_t)stream;
-  dim3 grid(n_items), block(256);
   switch (tier) {
-    case 0: hipLaunchKernelGGL(fill_kernel<0>, grid, block, 0, s, a); break;
-    case 1: hipLaunchKernelGGL(fill_kernel<1>, grid, block, 0, s, a); break;
-    case 2: hipLaunchKernelGGL(fill_kernel<2>, grid, block, 0, s, a); break;
+    case 0: return launch_fill<0>(n_items, a, s);
+    case 1: return launch_fill<1>(n_items, a, s);
+    case 2: return launch_fill<2>(n_items, a, s);
     default: return fail("mom_fill_tier: tier must be 0 (near), 1 (mid) or 2 (far)");
   }
-  return check_launch("fill_kernel");
 }
 
 extern "C" int mom_fill_args_size(void) { return (int)sizeof(FillArgs); }

@@ -66,12 +66,19 @@ class FillPlan:
         self.cV = mesh.tri_sign * lab / A[:, None]
 
         blocks, blk_layer = [], []
-        items = []     # (block, pair index, src offset, count) per tier
         src_lists = []
         per_tier = {NEAR: [], MID: [], FAR: []}
         n_src = 0
         self.tier_pairs = np.zeros(3, np.int64)  # triangle pairs per tier
         cen = self.cen
+        # Morton rank of every triangle within its layer: the source lists are
+        # chunked in this order, so that a chunk is a compact patch whose
+        # edges (the columns it adds into) are few and close together.
+        mrank = np.zeros(nt, np.int64)
+        for ls in range(nl):
+            on_s = np.nonzero(tl == ls)[0]
+            if on_s.size:
+                mrank[on_s[morton_order(cen[on_s])]] = np.arange(on_s.size)
         for lt in range(nl):
             on_t = np.nonzero(tl == lt)[0]
             if not on_t.size:
@@ -95,6 +102,7 @@ class FillPlan:
                     for tier, sel in zip((NEAR, MID, FAR), tiers):
                         if not sel.size:
                             continue
+                        sel = sel[np.argsort(mrank[sel], kind="stable")]
                         self.tier_pairs[tier] += sel.size * pb.size
                         src_lists.append(sel.astype(np.int32))
                         for c0 in range(0, sel.size, CHUNK):
@@ -109,8 +117,8 @@ class FillPlan:
         self.blk_layer = np.array(blk_layer, np.int32)
         self.blocks = blocks
         self.src = np.concatenate(src_lists) if src_lists else np.zeros(0, np.int32)
-        self.items = {t: np.array(v, np.int32).reshape(-1, 4) for t, v in per_tier.items()}
         self.n_levels = nl
+        self._scatter_maps(blocks, per_tier, block)
 
         # conductor-loss Gram matrix per triangle, times cA_i cA_j
         X = self.qp7[:, None, :, :] - V[:, :, None, :]
@@ -155,6 +163,59 @@ class FillPlan:
             self.pad_tris = np.zeros(0, int)
         self.dev = None
 
+    # LDS budget of one fill workgroup's accumulator (complex128 entries)
+    ACC_MAX_BYTES = 144 * 1024
+
+    def _scatter_maps(self, blocks, per_tier, block):
+        """Distinct rows (test basis functions) of every block and columns
+        (source basis functions) of every work item, with each triangle edge's
+        position in them: the fill kernel sums a work item's contributions per
+        (row, column) in LDS and adds every entry to Z once.  A work item whose
+        accumulator would exceed ACC_MAX_BYTES is split."""
+        tb = self.tb
+        nb = len(blocks)
+        row_off = np.zeros(nb + 1, np.int64)
+        rows, nrow = [], np.zeros(nb, np.int64)
+        rloc = -np.ones((nb, block, 3), np.int32)
+        for bi, pb in enumerate(blocks):
+            e = tb[pb]
+            u = np.unique(e[e >= 0])
+            loc = np.searchsorted(u, np.maximum(e, 0)).astype(np.int32)
+            loc[e < 0] = -1
+            rloc[bi, :pb.size] = loc
+            rows.append(u)
+            nrow[bi] = u.size
+            row_off[bi + 1] = row_off[bi] + u.size
+        self.blk_row_off = row_off.astype(np.int32)
+        self.blk_rows = np.concatenate(rows).astype(np.int32) if rows else np.zeros(0, np.int32)
+        self.blk_rloc = rloc
+        src_cloc = -np.ones((self.src.size, 3), np.int32)
+        cols, col_off = [], 0
+        self.items, self.acc_bytes = {}, {}
+        for tier, lst in per_tier.items():
+            out, acc = [], 0
+            todo = list(lst)
+            while todo:
+                bi, pp, so, cnt = todo.pop(0)
+                e = tb[self.src[so:so + cnt]]
+                u = np.unique(e[e >= 0])
+                nbytes = 16 * int(nrow[bi]) * u.size
+                if nbytes > self.ACC_MAX_BYTES and cnt > 1:
+                    h = cnt // 2
+                    todo[:0] = [(bi, pp, so, h), (bi, pp, so + h, cnt - h)]
+                    continue
+                loc = np.searchsorted(u, np.maximum(e, 0)).astype(np.int32)
+                loc[e < 0] = -1
+                src_cloc[so:so + cnt] = loc
+                out.append((bi, pp, so, cnt, col_off, u.size))
+                cols.append(u)
+                col_off += u.size
+                acc = max(acc, nbytes)
+            self.items[tier] = np.array(out, np.int32).reshape(-1, 6)
+            self.acc_bytes[tier] = acc
+        self.item_cols = np.concatenate(cols).astype(np.int32) if cols else np.zeros(0, np.int32)
+        self.src_cloc = src_cloc
+
     def upload(self, device="cuda"):
         """Device copies of every array the kernels read (kept on the plan)."""
         if self.dev is not None:
@@ -170,7 +231,10 @@ class FillPlan:
             gram=f64(self.gram), tp=i32(m.tp), vp=i32(m.vp), tm=i32(m.tm), vm=i32(m.vm),
         )
         for t, it in self.items.items():
-            d[f"items{t}"] = i32(it if it.size else np.zeros((1, 4)))
+            d[f"items{t}"] = i32(it if it.size else np.zeros((1, 6)))
+        d.update(blk_row_off=i32(self.blk_row_off), blk_rows=i32(self.blk_rows if self.blk_rows.size else [0]),
+                 blk_rloc=i32(self.blk_rloc), item_cols=i32(self.item_cols if self.item_cols.size else [0]),
+                 src_cloc=i32(self.src_cloc if self.src_cloc.size else np.zeros((1, 3))))
         if self.nv:
             d.update(d_off=i32(self.d_off), d_pad=i32(self.d_pad), d_val=f64(self.d_val),
                      pad_tri=i32(self.pad_tris))

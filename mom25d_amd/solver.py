@@ -279,7 +279,9 @@ class _FillArgs(ctypes.Structure):
         ("nt", ctypes.c_int32), ("pairs", ctypes.c_void_p), ("tab_x", ctypes.c_void_p), ("tab_c", ctypes.c_void_p),
         ("tab_b", ctypes.c_void_p), ("jwmu_re", ctypes.c_double), ("jwmu_im", ctypes.c_double),
         ("ijwe_re", ctypes.c_double), ("ijwe_im", ctypes.c_double), ("Z", ctypes.c_void_p), ("ldz", ctypes.c_int64),
-        ("IVpad", ctypes.c_void_p)]
+        ("IVpad", ctypes.c_void_p), ("blk_row_off", ctypes.c_void_p), ("blk_rows", ctypes.c_void_p),
+        ("blk_rloc", ctypes.c_void_p), ("item_cols", ctypes.c_void_p), ("src_cloc", ctypes.c_void_p),
+        ("acc_bytes", ctypes.c_int64)]
 
 
 def _check_abi():
@@ -545,12 +547,15 @@ class Solver:
         a.Z = Z.data_ptr()
         a.ldz = Z.stride(0)
         a.IVpad = ivpad.data_ptr() if ivpad is not None else None
+        for name in ("blk_row_off", "blk_rows", "blk_rloc", "item_cols", "src_cloc"):
+            setattr(a, name, d[name].data_ptr())
         stream = _lib.stream_ptr()
         for tier in (0, 1, 2):
             items = plan.items[tier]
             if not items.size:
                 continue
             a.items = d[f"items{tier}"].data_ptr()
+            a.acc_bytes = plan.acc_bytes[tier]
             _lib.call("mom_fill_tier", tier, int(items.shape[0]), ctypes.byref(a), stream)
         zs = self.stack.surface_impedance(g.freq)
         if zs:

@@ -63,9 +63,10 @@ def test_via_transition_sweep_matches_oracle():
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_lanes_match_single_lane(precision):
-    """Three frequency points in flight on three streams give the S and the
-    currents of the one-at-a-time sweep (the fill accumulates with atomics,
-    so the two agree to round-off, not bit for bit)."""
+    """Three frequency points in flight on three streams give the currents
+    of the one-at-a-time sweep to round-off (the fill accumulates with
+    atomics, so not bit for bit), and S to the wave fit's noise (its
+    Nelder-Mead stop moves S by a few 1e-8 for such inputs)."""
     lay, sub, mk, freqs = W.tee()
     m = lay.mesh(**mk)
     f = freqs[:7]
@@ -73,7 +74,7 @@ def test_lanes_match_single_lane(precision):
     r3 = Solver(m, sub, verbose=False, precision=precision, lanes=3).sweep(f, keep_currents=True)
     for a, b in zip(r1.currents, r3.currents):
         assert np.max(np.abs(a - b)) <= 1e-9 * np.max(np.abs(a))
-    assert np.max(np.abs(r1.S - r3.S)) < 1e-8
+    assert np.max(np.abs(r1.S - r3.S)) < 1e-7
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
@@ -90,7 +91,7 @@ def test_graph_replays_match_direct_calls(precision):
     assert len(s._graphs) == 2
     for x, y in zip(a.currents, b.currents):
         assert np.max(np.abs(x - y)) <= 1e-9 * np.max(np.abs(x))
-    assert np.max(np.abs(a.S - b.S)) < 1e-8
+    assert np.max(np.abs(a.S - b.S)) < 1e-7
 
 
 def test_sweep_batch_matches_separate_sweeps():

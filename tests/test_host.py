@@ -65,9 +65,37 @@ def test_plan_tiers_match_oracle_groups(name):
             oracle_sets[(ls, tier)] = set(sel.tolist())
         mine = {}
         for tier, items in plan.items.items():
-            for b, pp, off, cnt in items[items[:, 0] == bi]:
+            for b, pp, off, cnt, _coff, _nc in items[items[:, 0] == bi]:
                 mine.setdefault((pp % nl, tier), set()).update(plan.src[off:off + cnt].tolist())
         assert mine == oracle_sets
+
+
+@pytest.mark.parametrize("name", ["tee", "via2"])
+def test_plan_scatter_maps_point_at_the_basis_functions(name):
+    """Every block row and item column the fill kernel sums into maps back to
+    the basis function of the triangle edge it stands for, and every
+    accumulator fits the LDS budget."""
+    from mom25d_amd.plan import FillPlan
+
+    lay, st, mk, _ = getattr(W, name)()
+    m = lay.mesh(**mk)
+    zl = np.array([(st.to_stackup() if hasattr(st, "to_stackup") else st).level_z(n) for n in m.layer_names])
+    plan = FillPlan(m, zl=zl)
+    tb = plan.tb
+    for bi in range(plan.n_blocks):
+        pb = plan.blk_tris[bi]
+        pb = pb[pb >= 0]
+        loc = plan.blk_rloc[bi, :pb.size]
+        rows = plan.blk_rows[plan.blk_row_off[bi]:plan.blk_row_off[bi + 1]]
+        assert np.all(np.diff(rows) > 0)
+        np.testing.assert_array_equal(np.where(loc >= 0, rows[np.maximum(loc, 0)], -1), tb[pb])
+    for tier, items in plan.items.items():
+        for bi, pp, off, cnt, coff, nc in items:
+            cols = plan.item_cols[coff:coff + nc]
+            loc = plan.src_cloc[off:off + cnt]
+            np.testing.assert_array_equal(np.where(loc >= 0, cols[np.maximum(loc, 0)], -1), tb[plan.src[off:off + cnt]])
+            nr = plan.blk_row_off[bi + 1] - plan.blk_row_off[bi]
+            assert 16 * nr * nc <= plan.acc_bytes[tier] <= plan.ACC_MAX_BYTES
 
 
 def _declared_symbols():
