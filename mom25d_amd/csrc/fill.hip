@@ -152,15 +152,22 @@ __device__ inline void wilton(const double* Vs, double rx, double ry, double d, 
 
 // One workgroup per work item (test block, source chunk of one tier): lane l
 // of every wavefront owns source triangle l, wavefront w walks the block's
-// test triangles w, w + 4, ...  The 3 x 3 contributions of every pair are
+// test triangles w, w + NW, ...  The 3 x 3 contributions of every pair are
 // summed in LDS per (block row, item column) -- the item's distinct test and
 // source basis functions -- and each entry goes to Z with one atomic add, so
 // that Z sees a third of the scattered updates of a per-pair scatter.
-constexpr int kFillThreads = 256;
+// The LDS accumulator allows one workgroup per CU, so the mid and far tiers
+// run 8 wavefronts per workgroup (2 per SIMD); the near tier, whose closed
+// forms need more than 256 registers per lane, runs 4.
+template <int TIER>
+struct FillShape {
+  static constexpr int threads = TIER == 0 ? 256 : 512;
+};
 
 template <int TIER>
-__global__ __launch_bounds__(kFillThreads) void fill_kernel(FillArgs a) {
+__global__ __launch_bounds__(FillShape<TIER>::threads) void fill_kernel(FillArgs a) {
   constexpr int L = TIER == 0 ? 7 : (TIER == 1 ? 3 : 1);
+  constexpr int kFillThreads = FillShape<TIER>::threads;
   constexpr int NW = kFillThreads / 64;
   extern __shared__ double acc[];  // (nr, nc) complex, interleaved
   __shared__ double s_qp[kBlock][14];
@@ -244,6 +251,7 @@ __global__ __launch_bounds__(kFillThreads) void fill_kernel(FillArgs a) {
 #pragma unroll
         for (int j = 0; j < 3; ++j) IA[i][j] = {0.0, 0.0};
       z64 IV = {0.0, 0.0};
+#pragma unroll 1  // unrolled, the far tier needs 256 + 140 registers per lane
       for (int k = 0; k < 7; ++k) {
         const double rx = s_qp[pl][2 * k], ry = s_qp[pl][2 * k + 1];
         z64 k0a, k1x, k1y, k0v;
@@ -527,7 +535,7 @@ static int launch_fill(int n_items, const FillArgs& a, hipStream_t s) {
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     g_fill_lds[TIER] = lds;
   }
-  hipLaunchKernelGGL(fill_kernel<TIER>, dim3(n_items), dim3(kFillThreads), lds, s, a);
+  hipLaunchKernelGGL(fill_kernel<TIER>, dim3(n_items), dim3(FillShape<TIER>::threads), lds, s, a);
   return check_launch("fill_kernel");
 }
 
