@@ -43,10 +43,15 @@ def _cpu_model():
     return platform.processor() or "unknown"
 
 
-def time_path(mesh, sub, freqs, precision, warmup, steps, torch, lanes=None, autotune=True):
+def time_path(mesh, sub, freqs, precision, warmup, steps, torch, lanes=None, autotune=True, panel=None, tile=None,
+              block=None):
     from mom25d_amd import Solver
 
-    s = Solver(mesh, sub, verbose=False, precision=precision, lanes=lanes, autotune=autotune)
+    s = Solver(mesh, sub, verbose=False, precision=precision, lanes=lanes, autotune=autotune and tile is None,
+               panel=panel)
+    if tile is not None:
+        s.tile = tuple(tile)
+        s.block = block
     s.geometry().upload(s.device)
     nf = len(freqs)
     pick = lambda a, b: freqs[np.arange(a, b) % nf]  # noqa: E731
@@ -138,6 +143,9 @@ def main():
     ap.add_argument("--no-alt", action="store_true", help="skip timing the other path")
     ap.add_argument("--lanes", type=int, default=None, help="frequency points in flight (default: automatic)")
     ap.add_argument("--no-tune", action="store_true", help="default LU panel width / tiles instead of the autotuner's")
+    ap.add_argument("--panel", type=int, default=None, help="LU panel width (with --tile: instead of the autotuner)")
+    ap.add_argument("--tile", default=None, help="update-GEMM tile BM,BN of the headline path (skips the autotuner)")
+    ap.add_argument("--block", type=int, default=None, help="LU update depth (with --tile)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched sweep of the 16 random layouts")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -158,7 +166,9 @@ def main():
 
     prec = args.precision
     alt = "mixed" if prec == "fp64" else "fp64"
-    sm, rm, tm = time_path(mesh, sub, freqs, prec, args.warmup, args.steps, torch, args.lanes, not args.no_tune)
+    tile = tuple(int(x) for x in args.tile.split(",")) if args.tile else None
+    sm, rm, tm = time_path(mesh, sub, freqs, prec, args.warmup, args.steps, torch, args.lanes, not args.no_tune,
+                           args.panel, tile, args.block)
     lanes = sm._lane_count()
     print(f"{prec}: {args.steps} points in {tm:.3f} s ({sm.last_refine} refinement steps)", file=sys.stderr, flush=True)
     other = None
@@ -185,7 +195,8 @@ def main():
     fill_flops = sm.geometry().algorithmic_flops()
     extra = dict(fill=dict(seconds=tf, algorithmic_flops=fill_flops, achieved_tflops=fill_flops / tf / 1e12,
                            frac_fp64_valu=fill_flops / tf / PEAK["fp64_valu"]))
-    extra["lu_config"] = dict(panel=sm.panel, tile=[bm, bn], autotuned=not args.no_tune)
+    extra["lu_config"] = dict(panel=sm.panel, block=sm.block or sm.panel, tile=[bm, bn],
+                              autotuned=not args.no_tune and tile is None)
     if not args.no_batch:
         extra["batch_random16"] = batch_random16(torch, prec)
         b = extra["batch_random16"]

@@ -99,6 +99,15 @@ int mom_getrf(int prec, int n, void* A, int64_t lda, int32_t* ipiv, int32_t* inf
 int mom_set_gemm_tile(int prec, int bm, int bn);
 int mom_get_gemm_tile(int prec, int* bm, int* bn);
 
+/* Block width nb of mom_getrf's trailing updates, per precision (0: the panel
+ * width; otherwise a multiple of 8 up to 256, rounded down to a multiple of
+ * the panel width): a block of nb columns is factored as panels of jb
+ * columns, then the rest of the matrix takes one update of depth nb.
+ * Applies to the calls issued after it, process-wide (the autotuner's
+ * choice). */
+int mom_set_lu_block(int prec, int nb);
+int mom_get_lu_block(int prec);
+
 /* Row permutation of the factorisation: perm[i] = original row of row i. */
 int mom_build_perm(int n, const int32_t* ipiv, int32_t* perm, void* stream);
 

@@ -34,6 +34,8 @@ SIGNATURES = {
     "mom_getrf": (_i, [_i, _i, _p, _i64, _p, _p, _p, _i64, _i, _i, _p]),
     "mom_build_perm": (_i, [_i, _p, _p, _p]),
     "mom_set_gemm_tile": (_i, [_i, _i, _i]),
+    "mom_set_lu_block": (_i, [_i, _i]),
+    "mom_get_lu_block": (_i, [_i]),
     "mom_get_gemm_tile": (_i, [_i, _p, _p]),
     "mom_getrs_work_bytes": (_i64, [_i, _i]),
     "mom_getrs": (_i, [_i, _i, _i, _p, _i64, _p, _p, _p, _p, _p]),

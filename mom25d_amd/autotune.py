@@ -3,7 +3,8 @@
 For a matrix size and precision it times mom_getrf on a synthetic complex
 matrix of that size (random entries plus a dominant diagonal, so that the
 pivoting does the same work as on an impedance matrix) for every candidate
-panel width and trailing-update tile, and keeps the fastest.  Results are
+panel width, update depth (block width) and trailing-update tile, and keeps
+the fastest.  Results are
 cached per (precision, size rounded up to a multiple of 512) in memory and,
 when MOM25D_TUNE_CACHE names a file, in that JSON file.
 """
@@ -18,7 +19,8 @@ import torch
 from . import _lib
 
 PANELS = (32, 64, 128)
-TILES = ((64, 64), (64, 128), (128, 64), (128, 128))
+BLOCKS = (1, 2, 4)  # update depth nb = panel width x this (up to 256)
+TILES = ((64, 64), (64, 128))
 _CACHE = {}
 
 
@@ -45,6 +47,14 @@ def set_tile(prec, tile):
     _lib.call("mom_set_gemm_tile", prec, int(tile[0]), int(tile[1]))
 
 
+def set_block(prec, nb):
+    _lib.call("mom_set_lu_block", prec, int(nb))
+
+
+def get_block(prec):
+    return int(_lib.lib().mom_get_lu_block(prec))
+
+
 def get_tile(prec):
     import ctypes
 
@@ -53,8 +63,9 @@ def get_tile(prec):
     return bm.value, bn.value
 
 
-def time_getrf(n, prec, jb, tile, device="cuda", reps=2, seed=0):
-    """Median milliseconds of mom_getrf (n, prec, jb, tile) on a synthetic matrix."""
+def time_getrf(n, prec, jb, tile, device="cuda", reps=2, seed=0, nb=0):
+    """Median milliseconds of mom_getrf (n, prec, panel jb, block nb, tile)
+    on a synthetic matrix."""
     dev = torch.device(device)
     dt = torch.complex128 if prec == 64 else torch.complex64
     g = torch.Generator(device=dev)
@@ -67,8 +78,9 @@ def time_getrf(n, prec, jb, tile, device="cuda", reps=2, seed=0):
     info = torch.zeros(4, dtype=torch.int32, device=dev)
     wb = int(_lib.lib().mom_getrf_work_bytes(prec, n, jb))
     work = torch.empty(wb, dtype=torch.uint8, device=dev)
-    old = get_tile(prec)
+    old, old_nb = get_tile(prec), get_block(prec)
     set_tile(prec, tile)
+    set_block(prec, nb)
     P = _lib.ptr
     s = _lib.stream_ptr()
     times = []
@@ -83,12 +95,14 @@ def time_getrf(n, prec, jb, tile, device="cuda", reps=2, seed=0):
             times.append(e0.elapsed_time(e1))
     finally:
         set_tile(prec, old)
+        set_block(prec, old_nb)
     return float(np.median(times[1:]))
 
 
-def tune_lu(n, prec, device="cuda", panels=PANELS, tiles=TILES, reps=2, verbose=False):
-    """The fastest (panel width, tile) for an n x n factorisation in prec
-    (64: complex128, 32: complex64): dict(jb, tile, ms, table)."""
+def tune_lu(n, prec, device="cuda", panels=PANELS, blocks=BLOCKS, tiles=TILES, reps=2, verbose=False):
+    """The fastest (panel width, block width, tile) for an n x n
+    factorisation in prec (64: complex128, 32: complex64):
+    dict(jb, nb, tile, ms, table)."""
     _load()
     key = _key(prec, n)
     if key in _CACHE:
@@ -97,20 +111,24 @@ def tune_lu(n, prec, device="cuda", panels=PANELS, tiles=TILES, reps=2, verbose=
     t0 = time.perf_counter()
     table = []
     for jb in panels:
-        for tile in tiles:
-            try:
-                ms = time_getrf(m, prec, jb, tile, device, reps)
-            except RuntimeError as e:  # e.g. a panel too large for the LDS at this width
-                if verbose:
-                    print(f"  autotune n={m} prec={prec} jb={jb} tile={tile}: {e}", flush=True)
+        for mult in blocks:
+            nb = jb * mult
+            if nb > 256 or nb > m:
                 continue
-            table.append((jb, list(tile), ms))
-            if verbose:
-                print(f"  autotune n={m} prec={prec} jb={jb} tile={tile}: {ms:.2f} ms", flush=True)
+            for tile in tiles:
+                try:
+                    ms = time_getrf(m, prec, jb, tile, device, reps, nb=nb)
+                except RuntimeError as e:  # e.g. a panel too large for the LDS at this width
+                    if verbose:
+                        print(f"  autotune n={m} prec={prec} jb={jb} nb={nb} tile={tile}: {e}", flush=True)
+                    continue
+                table.append((jb, nb, list(tile), ms))
+                if verbose:
+                    print(f"  autotune n={m} prec={prec} jb={jb} nb={nb} tile={tile}: {ms:.2f} ms", flush=True)
     if not table:
-        raise RuntimeError(f"autotune: no panel width / tile runs for n={m}")
-    jb, tile, ms = min(table, key=lambda r: r[2])
-    best = dict(jb=jb, tile=tile, ms=ms, n=m, seconds=time.perf_counter() - t0, table=table)
+        raise RuntimeError(f"autotune: no panel width / block / tile runs for n={m}")
+    jb, nb, tile, ms = min(table, key=lambda r: r[3])
+    best = dict(jb=jb, nb=nb, tile=tile, ms=ms, n=m, seconds=time.perf_counter() - t0, table=table)
     _CACHE[key] = best
     _save()
     return best

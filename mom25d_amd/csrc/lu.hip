@@ -326,7 +326,10 @@ struct mfma_traits<float> {
 // streamed in stages of 16 real columns through double-buffered LDS: the
 // global loads of stage s+1 are in flight while stage s runs on the matrix
 // cores.  The tile is chosen per precision at run time (g_tile below).
-constexpr int kGemmBK = 16;
+#ifndef MOM_GEMM_BK
+#define MOM_GEMM_BK 16
+#endif
+constexpr int kGemmBK = MOM_GEMM_BK;
 
 template <typename R, int BM, int BN>
 __global__ __launch_bounds__(256) void cgemm_sub_kernel(int m, int n, int kk, const cplx<R>* __restrict__ Ap,
@@ -813,14 +816,53 @@ static int launch_panel(cplx<R>* A, int64_t lda, int n, int k0, int b, int32_t* 
   return check_launch("panel_kernel");
 }
 
+// Block width of the trailing updates, per precision (0: the panel width).
+// A block of nb columns is factored as panels of jb columns, each followed by
+// the update of the block's remaining columns; the rest of the matrix then
+// takes one update of depth nb.  Set by mom_set_lu_block (the autotuner).
+static int g_lu_nb[2] = {0, 0};
+
+// U12 = L11^-1 A12 for the block row [k0, k0 + b) of columns [c0, c1), with
+// b up to 256: 128-row triangular solves and the update between them.
 template <typename R>
-static int launch_update(cplx<R>* A, int64_t lda, int k0, int b, int r0, int r1, int c0, int c1, hipStream_t s) {
-  // trsm on the block row k0..k0+b of columns [c0, c1), then the GEMM on rows [r0, r1) x columns [c0, c1)
+static int trsm_block(cplx<R>* A, int64_t lda, int k0, int b, int c0, int c1, hipStream_t s) {
+  int rc;
+  for (int q0 = k0; q0 < k0 + b; q0 += 128) {
+    const int bq = std::min(128, k0 + b - q0);
+    hipLaunchKernelGGL(trsm_lower_unit_kernel<R>, dim3((c1 - c0 + 15) / 16), dim3(256), 0, s, A, lda, q0, bq, c0,
+                       c1);
+    if ((rc = check_launch("trsm_lower_unit_kernel"))) return rc;
+    const int q1 = q0 + bq;
+    if (q1 < k0 + b &&
+        (rc = launch_cgemm<R>(k0 + b - q1, c1 - c0, bq, A + (int64_t)q1 * lda + q0, lda, A + (int64_t)q0 * lda + c0,
+                              lda, A + (int64_t)q1 * lda + c0, lda, s)))
+      return rc;
+  }
+  return 0;
+}
+
+// The pivots [k0, k0 + b) applied to columns [c0, c1), in groups of <= 128
+// (the row interchanges are sequential, so groups in order are the same).
+template <typename R>
+static int laswp_block(cplx<R>* A, int64_t lda, int c0, int c1, int k0, int b, const int32_t* ipiv, hipStream_t s) {
+  if (c1 <= c0) return 0;
+  for (int q0 = k0; q0 < k0 + b; q0 += 128) {
+    hipLaunchKernelGGL(laswp_kernel<R>, dim3((c1 - c0 + 15) / 16), dim3(256), 0, s, A, lda, c0, c1, q0,
+                       std::min(128, k0 + b - q0), ipiv);
+    int rc = check_launch("laswp_kernel");
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+// Trailing update of rows [r0, n) x columns [c0, c1) by the factored block
+// column [k0, k0 + b): U12 = L11^-1 A12, then A22 -= L21 U12 (depth b).
+template <typename R>
+static int update_block(cplx<R>* A, int64_t lda, int n, int k0, int b, int r0, int c0, int c1, hipStream_t s) {
   if (c1 <= c0) return 0;
   int rc;
-  hipLaunchKernelGGL(trsm_lower_unit_kernel<R>, dim3((c1 - c0 + 15) / 16), dim3(256), 0, s, A, lda, k0, b, c0, c1);
-  if ((rc = check_launch("trsm_lower_unit_kernel"))) return rc;
-  const int mm = r1 - r0, nn = c1 - c0;
+  if ((rc = trsm_block<R>(A, lda, k0, b, c0, c1, s))) return rc;
+  const int mm = n - r0, nn = c1 - c0;
   if (mm <= 0) return 0;
   timing_mark_begin(s);
   if ((rc = launch_cgemm<R>(mm, nn, b, A + (int64_t)r0 * lda + k0, lda, A + (int64_t)k0 * lda + c0, lda,
@@ -830,8 +872,37 @@ static int launch_update(cplx<R>* A, int64_t lda, int k0, int b, int r0, int r1,
   return 0;
 }
 
-// Right-looking blocked LU with one panel of look-ahead: while the trailing
-// update of step p runs on the caller's stream, panel p+1 (whose columns
+struct PanelWork {
+  PivotSlot* slots;
+  void* slot_row;
+  void* diag_row;
+  int gmax;
+};
+
+// Factor the block column [k0, k0 + nb) (rows k0..n) with panels of width jb:
+// after each panel, its interchanges are applied to the block's other
+// columns and the block's remaining columns are updated.
+template <typename R>
+static int factor_block(cplx<R>* A, int64_t lda, int n, int k0, int nb, int jb, int32_t* ipiv, int32_t* info,
+                        const PanelWork& pw, hipStream_t s) {
+  int rc;
+  const int kend = k0 + nb;
+  for (int q0 = k0; q0 < kend; q0 += jb) {
+    const int bq = std::min(jb, kend - q0), q1 = q0 + bq;
+    if ((rc = launch_panel<R>(A, lda, n, q0, bq, ipiv, info, pw.slots, (cplx<R>*)pw.slot_row, (cplx<R>*)pw.diag_row,
+                              pw.gmax, s)))
+      return rc;
+    if ((rc = laswp_block<R>(A, lda, k0, q0, q0, bq, ipiv, s))) return rc;
+    if (q1 < kend) {
+      if ((rc = laswp_block<R>(A, lda, q1, kend, q0, bq, ipiv, s))) return rc;
+      if ((rc = update_block<R>(A, lda, n, q0, bq, q1, q1, kend, s))) return rc;
+    }
+  }
+  return 0;
+}
+
+// Right-looking blocked LU with one block of look-ahead: while the trailing
+// update of block p runs on the caller's stream, block p+1 (whose columns
 // were updated first) is factored on the side stream.
 template <typename R>
 static int getrf(int n, cplx<R>* A, int64_t lda, int32_t* ipiv, int32_t* info, void* work, size_t work_bytes, int jb,
@@ -841,6 +912,8 @@ static int getrf(int n, cplx<R>* A, int64_t lda, int32_t* ipiv, int32_t* info, v
   if (work_bytes < getrf_work_bytes<R>(n, jb)) return fail("mom_getrf: workspace too small");
   const int ncu = device_cus();
   if (ncu <= 0) return fail("mom_getrf: no device");
+  int nb = g_lu_nb[sizeof(R) == sizeof(double) ? 0 : 1];
+  nb = nb <= 0 ? jb : std::max(jb, nb - nb % jb);
   // slot arrays hold 256 workgroups
   const int gmax = std::min(std::min(ncu, 256), max_groups > 0 ? max_groups : 256);
   unsigned char* w = reinterpret_cast<unsigned char*>(work);
@@ -849,42 +922,39 @@ static int getrf(int n, cplx<R>* A, int64_t lda, int32_t* ipiv, int32_t* info, v
     w += (bytes + 255) & ~size_t(255);
     return p;
   };
-  const int npanel = (n + jb - 1) / jb;
-  PivotSlot* slots = reinterpret_cast<PivotSlot*>(take(sizeof(PivotSlot) * 2 * 256));
-  cplx<R>* slot_row = reinterpret_cast<cplx<R>*>(take(sizeof(cplx<R>) * 2 * 256 * (size_t)jb));
-  cplx<R>* diag_row = reinterpret_cast<cplx<R>*>(take(sizeof(cplx<R>) * 2 * (size_t)jb));
+  const int nblock = (n + nb - 1) / nb;
+  PanelWork pw;
+  pw.slots = reinterpret_cast<PivotSlot*>(take(sizeof(PivotSlot) * 2 * 256));
+  pw.slot_row = take(sizeof(cplx<R>) * 2 * 256 * (size_t)jb);
+  pw.diag_row = take(sizeof(cplx<R>) * 2 * (size_t)jb);
+  pw.gmax = gmax;
   int rc;
   hipLaunchKernelGGL(zero_u32_kernel, dim3((2 * 256 * 4 + 255) / 256), dim3(256), 0, s,
-                     reinterpret_cast<unsigned*>(slots), 2 * 256 * 4);
+                     reinterpret_cast<unsigned*>(pw.slots), 2 * 256 * 4);
   if ((rc = check_launch("zero_u32_kernel"))) return rc;
   hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, reinterpret_cast<unsigned*>(info), 1);
   if ((rc = check_launch("zero_u32_kernel"))) return rc;
   Lookahead* la = nullptr;
-  if ((rc = lookahead_for(s, 2 * (size_t)npanel + 2, &la))) return rc;
+  if ((rc = lookahead_for(s, 2 * (size_t)nblock + 2, &la))) return rc;
   hipStream_t side = la->side;
-  // panel 0 on the caller's stream
-  if ((rc = launch_panel<R>(A, lda, n, 0, std::min(jb, n), ipiv, info, slots, slot_row, diag_row, gmax, s)))
-    return rc;
-  for (int p = 0; p < npanel; ++p) {
-    const int k0 = p * jb, b = std::min(jb, n - k0);
+  // block 0 on the caller's stream
+  if ((rc = factor_block<R>(A, lda, n, 0, std::min(nb, n), jb, ipiv, info, pw, s))) return rc;
+  for (int p = 0; p < nblock; ++p) {
+    const int k0 = p * nb, b = std::min(nb, n - k0);
     const int c0 = k0 + b;
-    if (p > 0) MOM_TRY(hipStreamWaitEvent(s, la->ev[2 * p], 0));  // panel p done on the side stream
-    if (k0 > 0) {
-      hipLaunchKernelGGL(laswp_kernel<R>, dim3((k0 + 15) / 16), dim3(256), 0, s, A, lda, 0, k0, k0, b, ipiv);
-      if ((rc = check_launch("laswp_kernel"))) return rc;
-    }
+    if (p > 0) MOM_TRY(hipStreamWaitEvent(s, la->ev[2 * p], 0));  // block p done on the side stream
+    if ((rc = laswp_block<R>(A, lda, 0, k0, k0, b, ipiv, s))) return rc;
     if (c0 >= n) break;
-    hipLaunchKernelGGL(laswp_kernel<R>, dim3((n - c0 + 15) / 16), dim3(256), 0, s, A, lda, c0, n, k0, b, ipiv);
-    if ((rc = check_launch("laswp_kernel"))) return rc;
-    const int b1 = std::min(jb, n - c0);
-    // next panel's columns first, then hand the panel to the side stream
-    if ((rc = launch_update<R>(A, lda, k0, b, c0, n, c0, c0 + b1, s))) return rc;
+    if ((rc = laswp_block<R>(A, lda, c0, n, k0, b, ipiv, s))) return rc;
+    const int b1 = std::min(nb, n - c0);
+    // next block's columns first, then hand the block to the side stream
+    if ((rc = update_block<R>(A, lda, n, k0, b, c0, c0, c0 + b1, s))) return rc;
     MOM_TRY(hipEventRecord(la->ev[2 * p + 1], s));
     MOM_TRY(hipStreamWaitEvent(side, la->ev[2 * p + 1], 0));
-    if ((rc = launch_panel<R>(A, lda, n, c0, b1, ipiv, info, slots, slot_row, diag_row, gmax, side))) return rc;
+    if ((rc = factor_block<R>(A, lda, n, c0, b1, jb, ipiv, info, pw, side))) return rc;
     MOM_TRY(hipEventRecord(la->ev[2 * (p + 1)], side));
     // the rest of the trailing matrix
-    if ((rc = launch_update<R>(A, lda, k0, b, c0, n, c0 + b1, n, s))) return rc;
+    if ((rc = update_block<R>(A, lda, n, k0, b, c0, c0 + b1, n, s))) return rc;
   }
   return 0;
 }
@@ -960,6 +1030,15 @@ extern "C" int mom_getrs(int prec, int n, int nrhs, const void* LU, int64_t lda,
   if (prec == 32) return getrs<float>(n, nrhs, (const c32*)LU, lda, perm, (const c32*)B, (c32*)X, flags, s);
   return fail("mom_getrs: prec must be 64 or 32");
 }
+
+extern "C" int mom_set_lu_block(int prec, int nb) {
+  if (prec != 64 && prec != 32) return fail("mom_set_lu_block: prec must be 64 or 32");
+  if (nb < 0 || nb > 256 || nb % 8) return fail("mom_set_lu_block: block width must be 0 or a multiple of 8 up to 256");
+  g_lu_nb[prec == 64 ? 0 : 1] = nb;
+  return 0;
+}
+
+extern "C" int mom_get_lu_block(int prec) { return prec == 64 ? g_lu_nb[0] : g_lu_nb[1]; }
 
 extern "C" int mom_set_gemm_tile(int prec, int bm, int bn) {
   if (prec != 64 && prec != 32) return fail("mom_set_gemm_tile: prec must be 64 or 32");

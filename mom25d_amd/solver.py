@@ -18,6 +18,7 @@ import os
 import sys
 import time
 import types
+import warnings
 from concurrent.futures import ProcessPoolExecutor
 from dataclasses import dataclass
 
@@ -383,12 +384,13 @@ class Solver:
         self.device = torch.device(device)
         self.panel = panel or 64
         self.tile = None
+        self.block = None
         if autotune:
             from .autotune import tune_lu
 
             best = tune_lu(mesh.n_unknowns, 64 if precision == "fp64" else 32, self.device)
-            self.panel, self.tile = best["jb"], tuple(best["tile"])
-        self.tuning = dict(jb=self.panel, tile=self.tile)
+            self.panel, self.block, self.tile = best["jb"], best["nb"], tuple(best["tile"])
+        self.tuning = dict(jb=self.panel, nb=self.block, tile=self.tile)
         self.refine_tol = refine_tol
         self.refine_maxit = refine_maxit
         self.gmres_restart = 30
@@ -613,6 +615,8 @@ class Solver:
 
         if self.tile is not None:
             _lib.call("mom_set_gemm_tile", prec, self.tile[0], self.tile[1])
+        if self.block is not None:
+            _lib.call("mom_set_lu_block", prec, self.block)
 
         def calls():
             _lib.call("mom_getrf", prec, n, P(A), A.stride(0), P(ipiv), P(info), P(work), wbytes, jb, groups, stream)
@@ -741,6 +745,16 @@ class Solver:
             yd = torch.from_numpy(-y).to(self.device)
             _lib.call("mom_basis_sub", n, nrhs, k, P(Wb), P(yd), P(Y), stream)
             R = residual(Y)
+        else:
+            # restarts exhausted: report a result short of the FP64 floor
+            beta = torch.linalg.vector_norm(R, dim=0)
+            ynorm = torch.linalg.vector_norm(Y, dim=0)
+            yield
+            bh, target = beta.cpu().numpy(), self.refine_tol * znorm * ynorm.cpu().numpy()
+            if not np.all(bh < 1e3 * target):
+                warnings.warn(f"mixed-precision solve: GMRES stopped after {its} iterations at a normwise backward "
+                              f"error of {np.max(bh / (znorm * ynorm.cpu().numpy())):.1e} (target {self.refine_tol:.0e})",
+                              RuntimeWarning, stacklevel=2)
         self.last_refine = its
         return d * Y
 

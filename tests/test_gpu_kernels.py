@@ -93,18 +93,23 @@ def test_autotuned_lu_solves_like_the_default():
     from mom25d_amd.autotune import get_tile, set_tile, tune_lu
 
     best = tune_lu(700, 64, reps=1)
-    print(f"autotune n={best['n']} complex128:", best["jb"], best["tile"], f"{best['ms']:.2f} ms", len(best["table"]), "runs")
-    assert len(best["table"]) == 12 and best["ms"] == min(r[2] for r in best["table"])
+    print(f"autotune n={best['n']} complex128:", best["jb"], best["nb"], best["tile"], f"{best['ms']:.2f} ms",
+          len(best["table"]), "runs")
+    assert len(best["table"]) == 16 and best["ms"] == min(r[3] for r in best["table"])
     rng = np.random.default_rng(5)
     n = 700
     A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
     B = rng.standard_normal((n, 2)) + 1j * rng.standard_normal((n, 2))
-    old = get_tile(64)
+    from mom25d_amd.autotune import get_block, set_block
+
+    old, old_nb = get_tile(64), get_block(64)
     set_tile(64, best["tile"])
+    set_block(64, best["nb"])
     try:
         X, _, info = _lu_solve(A, B, 64, best["jb"])
     finally:
         set_tile(64, old)
+        set_block(64, old_nb)
     assert info == 0
     assert np.linalg.norm(A @ X - B) <= 1e-12 * np.linalg.norm(A) * np.linalg.norm(X)
 
@@ -145,6 +150,30 @@ def test_lu_fp64_matches_lapack(n, jb, nrhs):
     # the updated columns may split a near-tie differently on large matrices
     _, piv = sla.lu_factor(A)
     assert np.mean(ipiv == piv) > (0.999 if n > 200 else 1.0 - 1e-12)
+
+
+@pytest.mark.parametrize("n,jb,nb", [(777, 64, 256), (1500, 32, 128), (2049, 64, 128), (300, 128, 256)])
+def test_blocked_lu_matches_lapack(n, jb, nb):
+    """Blocks of nb columns factored as panels of jb (two-level blocking):
+    the solution and the pivot sequence of zgetrf."""
+    import scipy.linalg as sla
+
+    from mom25d_amd.autotune import get_block, set_block
+
+    rng = np.random.default_rng(n + nb)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    B = rng.standard_normal((n, 2)) + 1j * rng.standard_normal((n, 2))
+    old = get_block(64)
+    set_block(64, nb)
+    try:
+        X, ipiv, info = _lu_solve(A, B, 64, jb)
+    finally:
+        set_block(64, old)
+    assert info == 0
+    ref = np.linalg.solve(A, B)
+    assert np.max(np.abs(X - ref)) < 1e-10 * np.max(np.abs(ref))
+    _, piv = sla.lu_factor(A)
+    assert np.mean(ipiv == piv) > 0.999
 
 
 @pytest.mark.parametrize("n", [300, 1024])

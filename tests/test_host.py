@@ -145,6 +145,8 @@ def test_error_path_reports_message():
         _lib.call("mom_sommerfeld", 1, None, None, None, None, None, None, None, None, None, None, 5, None, None)
     with pytest.raises(RuntimeError, match="tile must be"):
         _lib.call("mom_set_gemm_tile", 64, 32, 64)
+    with pytest.raises(RuntimeError, match="block width must be"):
+        _lib.call("mom_set_lu_block", 64, 300)
     with pytest.raises(RuntimeError, match="nslots must be"):
         _lib.call("mom_mlp_train_step", 100, 1, 1, 1, 1.0, 1, 1, 1, 1, None)
 
