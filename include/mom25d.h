@@ -81,7 +81,9 @@ int mom_line_voltage(int npts, const double* pts, int li, int nl, int nt, int nr
 
 /* LU with partial pivoting of an n x n complex matrix in place (prec 64:
  * complex128, 32: complex64), panel width jb; ipiv[k] = row swapped with
- * row k (0-based); info[0] = first zero pivot + 1 or 0.  `work` holds
+ * row k (0-based); info[0] = first zero pivot + 1 or 0; info[1] = 1 when a
+ * panel's workgroups stopped waiting for each other (a bounded wait ran out;
+ * the factorisation is then invalid), else 0 (info holds 2 ints).  `work` holds
  * mom_getrf_work_bytes(prec, n, jb) bytes.  max_groups bounds the
  * workgroups of the panel kernel, whose workgroups must all be resident at
  * once (0: all compute units); a caller that factors L matrices concurrently
@@ -99,6 +101,11 @@ int mom_getrf(int prec, int n, void* A, int64_t lda, int32_t* ipiv, int32_t* inf
 int mom_set_gemm_tile(int prec, int bm, int bn);
 int mom_get_gemm_tile(int prec, int* bm, int* bn);
 
+/* Polls before a wait of the LU panel or of the triangular-solve chain gives
+ * up and flags the call as failed (info[1] / the solve workspace's fifth
+ * counter); default 2^21 (about a second).  Process-wide. */
+int mom_set_spin_limit(unsigned polls);
+
 /* Block width nb of mom_getrf's trailing updates, per precision (0: the panel
  * width; otherwise a multiple of 8 up to 256, rounded down to a multiple of
  * the panel width): a block of nb columns is factored as panels of jb
@@ -113,7 +120,8 @@ int mom_build_perm(int n, const int32_t* ipiv, int32_t* perm, void* stream);
 
 /* X = (LU)^-1 P B for nrhs <= 4 right-hand sides (n x nrhs, row-major).
  * `work` holds mom_getrs_work_bytes(prec, n) bytes of device memory (progress
- * counters, cleared by the call itself).  Replaces scipy.linalg.lu_solve
+ * counters, cleared by the call itself); its fifth uint32 is 1 after the call
+ * when a bounded wait of the block chain ran out (X is then invalid).  Replaces scipy.linalg.lu_solve
  * (LAPACK zgetrs), Engine/mom25d/solver.py:23. */
 int64_t mom_getrs_work_bytes(int prec, int n);
 int mom_getrs(int prec, int n, int nrhs, const void* LU, int64_t lda, const int32_t* perm, const void* B, void* X,

@@ -35,6 +35,7 @@ SIGNATURES = {
     "mom_build_perm": (_i, [_i, _p, _p, _p]),
     "mom_set_gemm_tile": (_i, [_i, _i, _i]),
     "mom_set_lu_block": (_i, [_i, _i]),
+    "mom_set_spin_limit": (_i, [ctypes.c_uint]),
     "mom_get_lu_block": (_i, [_i]),
     "mom_get_gemm_tile": (_i, [_i, _p, _p]),
     "mom_getrs_work_bytes": (_i64, [_i, _i]),

@@ -71,6 +71,13 @@ __device__ inline void wait_stores() { __builtin_amdgcn_s_waitcnt(0); }
 #endif
 __device__ inline void poll_pause() { __builtin_amdgcn_s_sleep(MOM_POLL_SLEEP); }
 
+// Upper bound on the polls of one wait (each at least a memory round trip,
+// so 2^21 polls take about a second or more): a wait that runs out ends the
+// kernel with an error flag instead of holding the GPU, and the workgroups
+// waiting on it run out in turn.  mom_set_spin_limit changes it (tests force
+// the error path with a tiny limit).
+static unsigned g_spin_limit = 1u << 21;
+
 // ------------------------------------------------------------------- panel
 // Per-workgroup publication slot, one per (column parity, workgroup): the
 // candidate pivot value / row index and a generation number, written with
@@ -89,6 +96,7 @@ struct PanelArgs {
   int64_t lda;
   int n, k0, jb, rows_per;  // rows_per: rows owned by one workgroup
   uint32_t gen0;            // generation of this panel's first column
+  uint32_t spin_limit;      // polls before a wait gives up
   int32_t* ipiv;
   int32_t* info;
   PivotSlot* slots;         // [2][256]
@@ -124,8 +132,10 @@ __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<R> a) {
   __shared__ R red_v[4];
   __shared__ int red_i[4];
   __shared__ int s_win;
+  __shared__ int s_abort;
 
   const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
+  if (tid == 0) s_abort = 0;
   const int jb = a.jb, rp = a.rows_per;
   const int r0 = a.k0 + g * rp;                 // first owned row
   const int r1 = min(a.n, r0 + rp);             // one past the last owned row
@@ -167,7 +177,14 @@ __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<R> a) {
     int i = 0x7fffffff;
     if (tid < G) {
       PivotSlot* sl = a.slots + par * 256 + tid;
-      while (load_coherent(&sl->gen) != gen) poll_pause();
+      unsigned spins = 0;
+      while (load_coherent(&sl->gen) != gen) {
+        poll_pause();
+        if (++spins > a.spin_limit) {
+          s_abort = 1;
+          break;
+        }
+      }
       v = (R)load_coherent(&sl->val);
       i = load_coherent(&sl->idx);
     }
@@ -176,6 +193,14 @@ __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<R> a) {
     const int piv = i;
     if (tid < G && mine_i == piv) s_win = tid;
     __syncthreads();
+    if (s_abort) {  // a workgroup never published: give up (reported in info[1])
+      // the remaining pivots of the panel become identities, so that the row
+      // interchanges that follow stay within the matrix
+      if (g == 0)
+        for (int q = tid; q < jb - j; q += 256) a.ipiv[c + q] = c + q;
+      if (tid == 0) a.info[1] = 1;
+      return;
+    }
     const int win = s_win;
     for (int q = tid; q < jb; q += 256) {
       prow[q] = load_coherent_c(a.slot_row + ((int64_t)par * 256 + win) * jb + q);
@@ -220,7 +245,7 @@ __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<R> a) {
 // of the (at most 2b) rows they touch; every workgroup then moves its 16
 // columns of those rows through LDS: all loads, then all stores.
 template <typename R>
-__global__ __launch_bounds__(256) void laswp_kernel(cplx<R>* A, int64_t lda, int c0, int c1, int k0, int b,
+__global__ __launch_bounds__(256) void laswp_kernel(cplx<R>* A, int64_t lda, int n, int c0, int c1, int k0, int b,
                                                     const int32_t* ipiv) {
   __shared__ int rows[256];
   __shared__ int src[256];
@@ -234,7 +259,8 @@ __global__ __launch_bounds__(256) void laswp_kernel(cplx<R>* A, int64_t lda, int
     }
     int nout = 0;
     for (int k = 0; k < b; ++k) {
-      const int p = ipiv[k0 + k];
+      int p = ipiv[k0 + k];
+      if (p < k0 + k || p >= n) p = k0 + k;  // a valid pivot row lies in [k0 + k, n)
       int pp;
       if (p < k0 + b) {
         pp = p - k0;
@@ -463,7 +489,10 @@ __global__ void build_perm_kernel(int n, const int32_t* ipiv, int32_t* perm) {
   if (threadIdx.x == 0)
     for (int k = 0; k < n; ++k) {
       int p = ipiv[k];
-      if (p != k) { int t = P[k]; P[k] = P[p]; P[p] = t; }
+      if (p <= k || p >= n) continue;  // a valid interchange has k < p < n
+      int t = P[k];
+      P[k] = P[p];
+      P[p] = t;
     }
   __syncthreads();
   if (lds)
@@ -510,7 +539,7 @@ __device__ __forceinline__ float lane_value(float v, int k) {
 
 template <typename R, bool LOWER, int NR>
 __global__ __launch_bounds__(256) void trsv_kernel(int n, const cplx<R>* __restrict__ A, int64_t lda, cplx<R>* X,
-                                                   unsigned* progress, unsigned* ticket) {
+                                                   unsigned* progress, unsigned* ticket, unsigned spin_limit) {
   constexpr int BS = kTrsvBS;
   constexpr int TRI = LOWER ? BS * (BS - 1) / 2 : BS * (BS + 1) / 2;
   __shared__ cplx<R> Dk[TRI];
@@ -553,12 +582,19 @@ __global__ __launch_bounds__(256) void trsv_kernel(int n, const cplx<R>* __restr
   unsigned done = 0;
   while (done < need) {
     if (tid == 0) {
-      unsigned v;
-      while ((v = load_coherent(progress)) <= done) poll_pause();
-      s_ready = v < need ? v : need;
+      unsigned v, spins = 0;
+      while ((v = load_coherent(progress)) <= done) {
+        poll_pause();
+        if (++spins > spin_limit) break;
+      }
+      s_ready = v <= done ? 0xffffffffu : (v < need ? v : need);
     }
     __syncthreads();
     const unsigned upto = s_ready;
+    if (upto == 0xffffffffu) {  // the block before never finished: give up (workspace counter 4)
+      if (tid == 0) progress[LOWER ? 4 : 3] = 1u;  // progress is counter 0 (lower) or 1 (upper)
+      return;
+    }
     for (unsigned d = done; d < upto; ++d) {
       const int j0 = (LOWER ? (int)d : nblk - 1 - (int)d) * BS;
       if (tid < BS * NR) {
@@ -802,7 +838,7 @@ static int launch_panel(cplx<R>* A, int64_t lda, int n, int k0, int b, int32_t* 
   rp = (m + G - 1) / G;
   G = (m + rp - 1) / rp;
   const uint32_t gen0 = 1u + (uint32_t)k0;  // unique per column of the factorisation
-  PanelArgs<R> pa{A, lda, n, k0, b, rp, gen0, ipiv, info, slots, slot_row, diag_row};
+  PanelArgs<R> pa{A, lda, n, k0, b, rp, gen0, g_spin_limit, ipiv, info, slots, slot_row, diag_row};
   // An ordinary launch, not a cooperative one: cooperative launches are
   // serialised across streams, which would stop the panels of concurrent
   // factorisations from overlapping.  Every workgroup of the panel is
@@ -844,10 +880,11 @@ static int trsm_block(cplx<R>* A, int64_t lda, int k0, int b, int c0, int c1, hi
 // The pivots [k0, k0 + b) applied to columns [c0, c1), in groups of <= 128
 // (the row interchanges are sequential, so groups in order are the same).
 template <typename R>
-static int laswp_block(cplx<R>* A, int64_t lda, int c0, int c1, int k0, int b, const int32_t* ipiv, hipStream_t s) {
+static int laswp_block(cplx<R>* A, int64_t lda, int n, int c0, int c1, int k0, int b, const int32_t* ipiv,
+                       hipStream_t s) {
   if (c1 <= c0) return 0;
   for (int q0 = k0; q0 < k0 + b; q0 += 128) {
-    hipLaunchKernelGGL(laswp_kernel<R>, dim3((c1 - c0 + 15) / 16), dim3(256), 0, s, A, lda, c0, c1, q0,
+    hipLaunchKernelGGL(laswp_kernel<R>, dim3((c1 - c0 + 15) / 16), dim3(256), 0, s, A, lda, n, c0, c1, q0,
                        std::min(128, k0 + b - q0), ipiv);
     int rc = check_launch("laswp_kernel");
     if (rc) return rc;
@@ -892,9 +929,9 @@ static int factor_block(cplx<R>* A, int64_t lda, int n, int k0, int nb, int jb, 
     if ((rc = launch_panel<R>(A, lda, n, q0, bq, ipiv, info, pw.slots, (cplx<R>*)pw.slot_row, (cplx<R>*)pw.diag_row,
                               pw.gmax, s)))
       return rc;
-    if ((rc = laswp_block<R>(A, lda, k0, q0, q0, bq, ipiv, s))) return rc;
+    if ((rc = laswp_block<R>(A, lda, n, k0, q0, q0, bq, ipiv, s))) return rc;
     if (q1 < kend) {
-      if ((rc = laswp_block<R>(A, lda, q1, kend, q0, bq, ipiv, s))) return rc;
+      if ((rc = laswp_block<R>(A, lda, n, q1, kend, q0, bq, ipiv, s))) return rc;
       if ((rc = update_block<R>(A, lda, n, q0, bq, q1, q1, kend, s))) return rc;
     }
   }
@@ -932,7 +969,7 @@ static int getrf(int n, cplx<R>* A, int64_t lda, int32_t* ipiv, int32_t* info, v
   hipLaunchKernelGGL(zero_u32_kernel, dim3((2 * 256 * 4 + 255) / 256), dim3(256), 0, s,
                      reinterpret_cast<unsigned*>(pw.slots), 2 * 256 * 4);
   if ((rc = check_launch("zero_u32_kernel"))) return rc;
-  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, reinterpret_cast<unsigned*>(info), 1);
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned*>(info), 2);
   if ((rc = check_launch("zero_u32_kernel"))) return rc;
   Lookahead* la = nullptr;
   if ((rc = lookahead_for(s, 2 * (size_t)nblock + 2, &la))) return rc;
@@ -943,9 +980,9 @@ static int getrf(int n, cplx<R>* A, int64_t lda, int32_t* ipiv, int32_t* info, v
     const int k0 = p * nb, b = std::min(nb, n - k0);
     const int c0 = k0 + b;
     if (p > 0) MOM_TRY(hipStreamWaitEvent(s, la->ev[2 * p], 0));  // block p done on the side stream
-    if ((rc = laswp_block<R>(A, lda, 0, k0, k0, b, ipiv, s))) return rc;
+    if ((rc = laswp_block<R>(A, lda, n, 0, k0, k0, b, ipiv, s))) return rc;
     if (c0 >= n) break;
-    if ((rc = laswp_block<R>(A, lda, c0, n, k0, b, ipiv, s))) return rc;
+    if ((rc = laswp_block<R>(A, lda, n, c0, n, k0, b, ipiv, s))) return rc;
     const int b1 = std::min(nb, n - c0);
     // next block's columns first, then hand the block to the side stream
     if ((rc = update_block<R>(A, lda, n, k0, b, c0, c0, c0 + b1, s))) return rc;
@@ -966,16 +1003,17 @@ static int getrs(int n, int nrhs, const cplx<R>* LU, int64_t lda, const int32_t*
   if (nrhs > 4) return fail("mom_getrs: at most 4 right-hand sides");
   const int nblk = (n + kTrsvBS - 1) / kTrsvBS;
   int rc;
-  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, flags, 4);
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, flags, 5);
   if ((rc = check_launch("zero_u32_kernel"))) return rc;
   hipLaunchKernelGGL(gather_rows_kernel<R>, dim3((n * nrhs + 255) / 256), dim3(256), 0, s, n, nrhs, perm, B, X);
   if ((rc = check_launch("gather_rows_kernel"))) return rc;
 #define MOM_TRSV(NR)                                                                                            \
   case NR:                                                                                                      \
-    hipLaunchKernelGGL((trsv_kernel<R, true, NR>), dim3(nblk), dim3(256), 0, s, n, LU, lda, X, flags, flags + 2); \
+    hipLaunchKernelGGL((trsv_kernel<R, true, NR>), dim3(nblk), dim3(256), 0, s, n, LU, lda, X, flags, flags + 2, \
+                       g_spin_limit);                                                                             \
     if ((rc = check_launch("trsv_kernel"))) return rc;                                                          \
     hipLaunchKernelGGL((trsv_kernel<R, false, NR>), dim3(nblk), dim3(256), 0, s, n, LU, lda, X, flags + 1,       \
-                       flags + 3);                                                                              \
+                       flags + 3, g_spin_limit);                                                                \
     break;
   switch (nrhs) {
     MOM_TRSV(1)
@@ -1029,6 +1067,12 @@ extern "C" int mom_getrs(int prec, int n, int nrhs, const void* LU, int64_t lda,
   if (prec == 64) return getrs<double>(n, nrhs, (const z64*)LU, lda, perm, (const z64*)B, (z64*)X, flags, s);
   if (prec == 32) return getrs<float>(n, nrhs, (const c32*)LU, lda, perm, (const c32*)B, (c32*)X, flags, s);
   return fail("mom_getrs: prec must be 64 or 32");
+}
+
+extern "C" int mom_set_spin_limit(unsigned polls) {
+  if (polls == 0) return fail("mom_set_spin_limit: the limit must be positive");
+  g_spin_limit = polls;
+  return 0;
 }
 
 extern "C" int mom_set_lu_block(int prec, int nb) {

@@ -652,6 +652,20 @@ class Solver:
             return X
         return (yield from self._gmres_ir(Z, A, perm, sw, B, lane))
 
+    def _check_solve(self, lane=0):
+        """Status words of the lane's last factorisation and solves (read
+        after the solve's result has been copied to the host): raise if a
+        bounded wait of the LU kernels ran out, warn on an exactly zero
+        pivot (as LAPACK's zgetrf reports; the solution is then not finite)."""
+        prec = 64 if self.precision == "fp64" else 32
+        info = self._bufs[("info", lane)].cpu().numpy()
+        sw = self._bufs[(f"swork{prec}", lane)][:20].cpu().numpy().view(np.uint32)
+        if info[1] or sw[4]:
+            raise RuntimeError("LU kernels: a bounded wait ran out (workgroups did not all run); result invalid")
+        if info[0]:
+            warnings.warn(f"LU: diagonal number {int(info[0])} is exactly zero (singular matrix)", RuntimeWarning,
+                          stacklevel=2)
+
     def _panel_groups(self):
         """Workgroup bound of the LU panel: every lane's panel resident at once."""
         lanes = getattr(self, "_active_lanes", 1)
@@ -766,7 +780,9 @@ class Solver:
 
     def _solve_dev(self, g, excitations):
         B = torch.from_numpy(np.ascontiguousarray(excitations, complex)).to(self.device)
-        return self._run(self._solve_task(g, B, 0)).clone()
+        X = self._run(self._solve_task(g, B, 0)).clone()
+        self._check_solve(0)
+        return X
 
     def _solve_task(self, g, B, lane, graph=False):
         """Generator: fill, LU and port solve of one frequency on the current
@@ -946,6 +962,7 @@ def _sweep_many(jobs, lanes, workers, keep_currents=False, graphs=False):
         x_dev = yield from s._solve_task(g, o["B"], lane, graphs)
         yield  # let the other lanes enqueue their work before waiting for this one
         x = x_dev.cpu().numpy()
+        s._check_solve(lane)
         if keep_currents:
             o["cur"][fi] = x
         h0 = time.perf_counter()

@@ -114,7 +114,7 @@ def test_autotuned_lu_solves_like_the_default():
     assert np.linalg.norm(A @ X - B) <= 1e-12 * np.linalg.norm(A) * np.linalg.norm(X)
 
 
-def _lu_solve(A, B, prec, jb):
+def _lu_solve(A, B, prec, jb, status=False):
     n = A.shape[0]
     dt = torch.complex128 if prec == 64 else torch.complex64
     Ad = torch.from_numpy(A).to(DEV, dt)
@@ -131,6 +131,8 @@ def _lu_solve(A, B, prec, jb):
     _lib.call("mom_getrf", prec, n, P(Ad), n, P(ipiv), P(info), P(work), wb, jb, 0, s)
     _lib.call("mom_build_perm", n, P(ipiv), P(perm), s)
     _lib.call("mom_getrs", prec, n, B.shape[1], P(Ad), n, P(perm), P(Bd), P(X), P(sw), s)
+    if status:
+        return X.cpu().numpy(), info.cpu().numpy(), sw[:20].cpu().numpy().view(np.uint32)
     return X.cpu().numpy(), ipiv.cpu().numpy(), int(info[0])
 
 
@@ -185,3 +187,23 @@ def test_lu_fp32_backward_error(n):
     assert info == 0
     r = np.linalg.norm(A @ X - B) / (np.linalg.norm(A, 2) * np.linalg.norm(X))
     assert r < 1e-5
+
+
+def test_bounded_waits_report_failure():
+    """With a wait limit of one poll the panel's workgroups and the solve
+    chain give up instead of waiting: every kernel still ends, the calls flag
+    the failure, and with the default limit back the next factorisation is
+    correct (the failure path forced from the host; no kernel hangs)."""
+    rng = np.random.default_rng(11)
+    n = 3000
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    B = rng.standard_normal((n, 1)) + 1j * rng.standard_normal((n, 1))
+    _lib.call("mom_set_spin_limit", 1)
+    try:
+        _, info, sw = _lu_solve(A, B, 64, 64, status=True)
+    finally:
+        _lib.call("mom_set_spin_limit", 1 << 21)
+    assert info[1] == 1 or sw[4] == 1
+    X, _, info0 = _lu_solve(A, B, 64, 64)
+    assert info0 == 0
+    assert np.linalg.norm(A @ X - B) <= 1e-12 * np.linalg.norm(A) * np.linalg.norm(X)
