@@ -93,6 +93,18 @@ def fill_time(solver, freq, torch, reps=3):
     return ev[0].elapsed_time(ev[1]) * 1e-3 / reps
 
 
+def measured_traffic(kernel, workload, panel, block):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (profiles/traffic.json: FETCH_SIZE + WRITE_SIZE per dispatch, scaled by
+    the same counters on a plain copy), or None for another configuration."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        table = json.load(fh)
+    return table.get(f"{kernel} {workload} panel {panel} block {block}", {}).get("bytes_per_launch")
+
+
 def batch_random16(torch, precision):
     """The 16 seeded random layouts (N 601-2965, 64 frequencies each) as one
     batched sweep (sweep_batch); the first two problems warm it up."""
@@ -146,6 +158,8 @@ def main():
     ap.add_argument("--panel", type=int, default=None, help="LU panel width (with --tile: instead of the autotuner)")
     ap.add_argument("--tile", default=None, help="update-GEMM tile BM,BN of the headline path (skips the autotuner)")
     ap.add_argument("--block", type=int, default=None, help="LU update depth (with --tile)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="end with a plain 1 GiB device copy (reference for the HBM byte counters)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched sweep of the 16 random layouts")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -188,8 +202,9 @@ def main():
 
     bm, bn = sm.tile or get_tile(64 if prec == "fp64" else 32)
     kname = f"cgemm_sub_kernel<{'double' if prec == 'fp64' else 'float'}, {bm}, {bn}>"
+    traffic = measured_traffic(kname, args.workload, sm.panel, sm.block or sm.panel)
     roof = dict(bound="mfma", achieved=achieved / 1e12, peak=peak / 1e12, unit="TFLOP/s", frac=achieved / peak,
-                traffic=None, kernel=f"{kname} (LU trailing update)",
+                traffic=traffic, kernel=f"{kname} (LU trailing update)",
                 launches=calls, avg_launch_ms=1e3 * secs / max(calls, 1), flops_per_launch=flops / max(calls, 1))
     tf = fill_time(sm, f_probe, torch)
     fill_flops = sm.geometry().algorithmic_flops()
@@ -229,6 +244,14 @@ def main():
         "extra": extra,
     }
     print(json.dumps(line), flush=True)
+    if args.calibrate_copy:
+        n = 1 << 28  # 2^28 float32 = 1 GiB read + 1 GiB written
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"calibration copy: {4 * n} bytes read, {4 * n} bytes written", file=sys.stderr, flush=True)
 
 
 if __name__ == "__main__":
