@@ -4,7 +4,8 @@
 Workload (default): the 40 x 20 mm via-stitched grounded-coplanar pour
 (`workloads.gcpw40`, 12 104 RWG + 324 via unknowns = 12 428, 51 frequencies
 1-20 GHz).  A step is one frequency point of the sweep: Green's tables, FP64
-fill, LU + port solve, de-embedding to S.  The headline path is chosen with
+fill, LU + port solve, de-embedding to S; by default the whole sweep is
+timed (--steps K: K points spread evenly over it).  The headline path is chosen with
 --precision (default: the plain FP64 path, complex128 LU); the other path
 (FP64 fill, complex64 LU on the matrix cores as the preconditioner of FP64
 GMRES refinement) is timed beside it on the same frequencies and |dS|
@@ -53,16 +54,24 @@ def time_path(mesh, sub, freqs, precision, warmup, steps, torch, lanes=None, aut
         s.tile = tuple(tile)
         s.block = block
     s.geometry().upload(s.device)
-    nf = len(freqs)
-    pick = lambda a, b: freqs[np.arange(a, b) % nf]  # noqa: E731
     if warmup:
-        s.sweep(pick(0, warmup))
+        s.sweep(freqs[len(freqs) // 2 - warmup // 2:][:warmup])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    r = s.sweep(pick(warmup, warmup + steps))
+    r = s.sweep(sweep_points(freqs, steps))
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     return s, r, t1 - t0
+
+
+def sweep_points(freqs, steps):
+    """The timed frequency points: the whole sweep when steps is its length,
+    else `steps` points spread evenly over it (low-frequency points cost more
+    refinement in the mixed path, so a contiguous slice would be biased)."""
+    nf = len(freqs)
+    if steps >= nf:
+        return freqs[np.arange(steps) % nf]
+    return freqs[np.round(np.linspace(0, nf - 1, steps)).astype(int)]
 
 
 def gemm_roofline(solver, freq, torch):
@@ -147,7 +156,8 @@ def cpu_baseline(mesh, sub, freqs, npts):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="frequency points timed, spread over the sweep (default: the whole sweep)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="gcpw40")
     ap.add_argument("--cpu-sample", type=int, default=1, help="frequency points timed on the CPU oracle (0: skip)")
@@ -173,6 +183,8 @@ def main():
     from mom25d_amd import workloads as W
 
     lay, sub, mk, freqs = W.ALL[args.workload]()
+    if args.steps is None:
+        args.steps = len(freqs)
     mesh = lay.mesh(**mk)
     N = mesh.n_unknowns
     print(f"workload {args.workload}: N={N} ({mesh.n_basis} RWG + {mesh.n_vias} vias), {len(freqs)} f", file=sys.stderr,

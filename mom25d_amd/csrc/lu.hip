@@ -510,14 +510,18 @@ __global__ void gather_rows_kernel(int n, int nrhs, const int32_t* perm, const c
 // Triangular solves as a chain of 64-row blocks: block k needs x_0 .. x_{k-1}
 // (lower; the mirror for upper).  Workgroup k, given block k by ticket, first
 // copies the triangle of its diagonal block into LDS (packed by columns, the
-// reciprocal of U's diagonal in place of the diagonal), then keeps catching
-// up with the blocks already finished (one coherent progress counter,
-// advanced in order) and accumulates A[k, j] x_j for all of them in
-// registers.  Once its last dependency is done, wavefront 0 runs the
-// substitution on the diagonal block (row i in lane i, x_k broadcast by lane
-// shuffle) -- the backward-stable form of LAPACK's ztrsv -- and publishes x_k.
-// Tickets are taken in launch order, so a workgroup only waits for
-// workgroups that are already running.
+// reciprocal of U's diagonal in place of the diagonal), then catches up with
+// the blocks before it in order, accumulating A[k, j] x_j in registers.  Once
+// its last dependency is in, wavefront 0 runs the substitution on the
+// diagonal block (row i in lane i, x_k broadcast from lane k) -- the
+// backward-stable form of LAPACK's ztrsv -- and stores x_k.
+//
+// The solution array itself is the flag: it is filled with a sentinel NaN
+// before the launch, and a reader polls each element it needs until it is no
+// longer the sentinel.  A finished block is therefore visible after one
+// memory round trip, with no completion wait and no counter.  Tickets are
+// taken in launch order, so a workgroup only waits for workgroups that are
+// already running.
 constexpr int kTrsvBS = 64;
 
 template <bool LOWER>
@@ -537,28 +541,62 @@ __device__ __forceinline__ float lane_value(float v, int k) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
 }
 
-template <typename R, bool LOWER, int NR>
-__global__ __launch_bounds__(256) void trsv_kernel(int n, const cplx<R>* __restrict__ A, int64_t lda, cplx<R>* X,
-                                                   unsigned* progress, unsigned* ticket, unsigned spin_limit) {
+template <typename R>
+struct Sentinel;
+template <>
+struct Sentinel<double> {
+  static constexpr unsigned long long bits = 0x7FF4D0D0D0D0D0D0ull;  // a signalling NaN no arithmetic produces
+  static __device__ inline bool is(double v) { return (unsigned long long)__double_as_longlong(v) == bits; }
+  static __device__ inline double value() { return __longlong_as_double((long long)bits); }
+};
+template <>
+struct Sentinel<float> {
+  static constexpr unsigned bits = 0x7FA0D0D0u;
+  static __device__ inline bool is(float v) { return __float_as_uint(v) == bits; }
+  static __device__ inline float value() { return __uint_as_float(bits); }
+};
+
+template <typename R>
+__global__ void fill_sentinel_kernel(cplx<R>* x, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = {Sentinel<R>::value(), Sentinel<R>::value()};
+}
+
+// Solves with the right-hand sides B (n x NR) into Xout (n x NR, filled with
+// the sentinel); *ticket hands out the blocks, *error is set to 1 when a
+// bounded wait ran out.  With INV the diagonal blocks are applied as the
+// precomputed inverses Dinv (64 x 64 row-major per block, trtri_blocks_kernel)
+// by a product spread over the workgroup instead of the substitution.
+template <typename R, bool LOWER, int NR, bool INV>
+__global__ __launch_bounds__(256) void trsv_kernel(int n, const cplx<R>* __restrict__ A, int64_t lda,
+                                                   const cplx<R>* __restrict__ B, cplx<R>* Xout,
+                                                   const cplx<R>* __restrict__ Dinv, unsigned* ticket,
+                                                   unsigned* error, unsigned spin_limit) {
   constexpr int BS = kTrsvBS;
   constexpr int TRI = LOWER ? BS * (BS - 1) / 2 : BS * (BS + 1) / 2;
-  __shared__ cplx<R> Dk[TRI];
+  __shared__ cplx<R> Dk[INV ? 1 : TRI];
   __shared__ cplx<R> xs[BS][NR];
   __shared__ cplx<R> part[4][BS][NR];
   __shared__ int s_blk;
-  __shared__ unsigned s_ready;
+  __shared__ int s_abort;
   const int tid = threadIdx.x;
   const int r = tid & 63, q = tid >> 6;  // row of the block, column quarter
   const int nblk = (n + BS - 1) / BS;
   if (tid == 0) {
     const unsigned t = atomicAdd(ticket, 1u);
     s_blk = LOWER ? (int)t : nblk - 1 - (int)t;
+    s_abort = 0;
   }
   __syncthreads();
   const int blk = s_blk;
   const int k0 = blk * BS, b = min(BS, n - k0);
   const int row = k0 + r;
-  for (int e = tid; e < BS * BS; e += 256) {
+  cplx<R> dv[INV ? 16 : 1];  // this thread's 16 entries of the inverse's row r
+  if (INV) {
+#pragma unroll
+    for (int e = 0; e < 16; ++e) dv[e] = Dinv[(int64_t)blk * BS * BS + r * BS + 16 * q + e];
+  }
+  for (int e = tid; !INV && e < BS * BS; e += 256) {
     const int i = e / BS, k = e % BS;
     if (LOWER ? i > k : i <= k) {
       cplx<R> v = (i < b && k < b) ? A[(int64_t)(k0 + i) * lda + k0 + k] : cplx<R>{(R)(i == k), 0};
@@ -569,51 +607,74 @@ __global__ __launch_bounds__(256) void trsv_kernel(int n, const cplx<R>* __restr
   cplx<R> acc[NR];
 #pragma unroll
   for (int c = 0; c < NR; ++c) acc[c] = {0, 0};
-  const unsigned need = LOWER ? (unsigned)blk : (unsigned)(nblk - 1 - blk);  // finished blocks needed
+  const int need = LOWER ? blk : nblk - 1 - blk;  // blocks before this one in solve order
   // This thread's 16 entries of A[row, block d] for the next dependency d,
-  // loaded while the workgroup waits for block d to be published.
+  // loaded while the workgroup waits for block d.
   cplx<R> la[16];
-  auto fetch = [&](unsigned d) {
-    const int j0 = (LOWER ? (int)d : nblk - 1 - (int)d) * BS + 16 * q;
+  auto fetch = [&](int d) {
+    const int j0 = (LOWER ? d : nblk - 1 - d) * BS + 16 * q;
 #pragma unroll
     for (int e = 0; e < 16; ++e) la[e] = (r < b && j0 + e < n) ? A[(int64_t)row * lda + j0 + e] : cplx<R>{0, 0};
   };
   if (need) fetch(0);
-  unsigned done = 0;
-  while (done < need) {
-    if (tid == 0) {
-      unsigned v, spins = 0;
-      while ((v = load_coherent(progress)) <= done) {
-        poll_pause();
-        if (++spins > spin_limit) break;
+  for (int d = 0; d < need; ++d) {
+    const int j0 = (LOWER ? d : nblk - 1 - d) * BS;
+    if (tid < BS * NR) {
+      const int i = tid / NR, c = tid % NR;
+      cplx<R> v = {0, 0};
+      if (j0 + i < n) {
+        unsigned spins = 0;
+        for (;;) {
+          v = load_coherent_c(Xout + (int64_t)(j0 + i) * NR + c);
+          if (!Sentinel<R>::is(v.re) && !Sentinel<R>::is(v.im)) break;
+          poll_pause();
+          if (++spins > spin_limit) {
+            s_abort = 1;
+            break;
+          }
+        }
       }
-      s_ready = v <= done ? 0xffffffffu : (v < need ? v : need);
+      xs[i][c] = v;
     }
     __syncthreads();
-    const unsigned upto = s_ready;
-    if (upto == 0xffffffffu) {  // the block before never finished: give up (workspace counter 4)
-      if (tid == 0) progress[LOWER ? 4 : 3] = 1u;  // progress is counter 0 (lower) or 1 (upper)
+    if (s_abort) {  // a block before this one never arrived: give up
+      if (tid == 0) *error = 1u;
       return;
     }
-    for (unsigned d = done; d < upto; ++d) {
-      const int j0 = (LOWER ? (int)d : nblk - 1 - (int)d) * BS;
-      if (tid < BS * NR) {
-        const int i = tid / NR, c = tid % NR;
-        xs[i][c] = (j0 + i < n) ? load_coherent_c(X + (int64_t)(j0 + i) * NR + c) : cplx<R>{0, 0};
-      }
-      __syncthreads();
 #pragma unroll
-      for (int e = 0; e < 16; ++e)
+    for (int e = 0; e < 16; ++e)
 #pragma unroll
-        for (int c = 0; c < NR; ++c) acc[c] += la[e] * xs[16 * q + e][c];
-      if (d + 1 < need) fetch(d + 1);
-      __syncthreads();
-    }
-    done = upto;
+      for (int c = 0; c < NR; ++c) acc[c] += la[e] * xs[16 * q + e][c];
+    if (d + 1 < need) fetch(d + 1);
+    __syncthreads();
   }
 #pragma unroll
   for (int c = 0; c < NR; ++c) part[q][r][c] = acc[c];
   __syncthreads();
+  if (INV) {
+    // x_k = Dinv_k (b_k - sum): v into LDS, then one product per (row, quarter)
+    if (tid < BS * NR) {
+      const int i = tid / NR, c = tid % NR;
+      xs[i][c] = i < b ? B[(int64_t)(k0 + i) * NR + c] - part[0][i][c] - part[1][i][c] - part[2][i][c] - part[3][i][c]
+                       : cplx<R>{0, 0};
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NR; ++c) acc[c] = {0, 0};
+#pragma unroll
+    for (int e = 0; e < 16; ++e)
+#pragma unroll
+      for (int c = 0; c < NR; ++c) acc[c] += dv[e] * xs[16 * q + e][c];
+#pragma unroll
+    for (int c = 0; c < NR; ++c) part[q][r][c] = acc[c];
+    __syncthreads();
+    if (tid < BS * NR) {
+      const int i = tid / NR, c = tid % NR;
+      if (i < b)
+        store_coherent_c(Xout + (int64_t)(k0 + i) * NR + c, part[0][i][c] + part[1][i][c] + part[2][i][c] + part[3][i][c]);
+    }
+    return;
+  }
   if (tid >= 64) return;
   // wavefront 0: v = b - sum, then substitution with the diagonal block; x_k
   // is read from lane k by every lane (uniform k: scalar broadcast)
@@ -621,7 +682,7 @@ __global__ __launch_bounds__(256) void trsv_kernel(int n, const cplx<R>* __restr
   cplx<R> v[NR];
 #pragma unroll
   for (int c = 0; c < NR; ++c)
-    v[c] = i < b ? X[(int64_t)(k0 + i) * NR + c] - part[0][i][c] - part[1][i][c] - part[2][i][c] - part[3][i][c]
+    v[c] = i < b ? B[(int64_t)(k0 + i) * NR + c] - part[0][i][c] - part[1][i][c] - part[2][i][c] - part[3][i][c]
                  : cplx<R>{0, 0};
   if (LOWER) {
 #pragma unroll
@@ -647,10 +708,8 @@ __global__ __launch_bounds__(256) void trsv_kernel(int n, const cplx<R>* __restr
   }
   if (i < b) {
 #pragma unroll
-    for (int c = 0; c < NR; ++c) store_coherent_c(X + (int64_t)(k0 + i) * NR + c, v[c]);
+    for (int c = 0; c < NR; ++c) store_coherent_c(Xout + (int64_t)(k0 + i) * NR + c, v[c]);
   }
-  wait_stores();
-  if (i == 0) store_coherent(progress, need + 1u);
 }
 
 // R = B - Z X (FP64), one wavefront per row; when rowsum is given it also
@@ -996,24 +1055,114 @@ static int getrf(int n, cplx<R>* A, int64_t lda, int32_t* ipiv, int32_t* info, v
   return 0;
 }
 
+// Inverses of the 64 x 64 diagonal blocks of the factors: blockIdx.y = 0 the
+// unit lower L_kk, 1 the upper U_kk (a partial last block is padded with the
+// identity).  Lane j solves T x = e_j by substitution in FP64 (T and the
+// columns in LDS) and the result is rounded to R: Dinv[(y * nblk + k) * 4096
+// + i * 64 + j] = (T_kk^-1)[i][j].
 template <typename R>
+__global__ __launch_bounds__(64) void trtri_blocks_kernel(int n, const cplx<R>* __restrict__ A, int64_t lda,
+                                                          cplx<R>* Dinv) {
+  constexpr int BS = kTrsvBS;
+  extern __shared__ unsigned char smem_raw[];
+  z64* T = reinterpret_cast<z64*>(smem_raw);  // [BS][BS]
+  z64* Xc = T + BS * BS;                      // [BS][BS], column j in lane j
+  const int blk = blockIdx.x, upper = blockIdx.y, j = threadIdx.x;
+  const int nblk = gridDim.x;
+  const int k0 = blk * BS, b = min(BS, n - k0);
+  for (int e = j; e < BS * BS; e += BS) {
+    const int i = e / BS, k = e % BS;
+    const bool in = upper ? i <= k : i > k;
+    z64 v = {(double)(i == k), 0.0};
+    if (in && i < b && k < b) {
+      const cplx<R> a = A[(int64_t)(k0 + i) * lda + k0 + k];
+      v = {(double)a.re, (double)a.im};
+    } else if (!in && i != k) {
+      v = {0.0, 0.0};
+    }
+    T[e] = v;
+  }
+  __syncthreads();
+  if (!upper) {
+    for (int i = 0; i < BS; ++i) {
+      z64 x = {(double)(i == j), 0.0};
+      for (int k = j; k < i; ++k) x -= T[i * BS + k] * Xc[k * BS + j];
+      Xc[i * BS + j] = i < j ? z64{0.0, 0.0} : x;
+    }
+  } else {
+    for (int i = BS - 1; i >= 0; --i) {
+      z64 x = {(double)(i == j), 0.0};
+      for (int k = i + 1; k <= j; ++k) x -= T[i * BS + k] * Xc[k * BS + j];
+      Xc[i * BS + j] = i > j ? z64{0.0, 0.0} : cdiv(x, T[i * BS + i]);
+    }
+  }
+  cplx<R>* out = Dinv + ((int64_t)upper * nblk + blk) * BS * BS;
+  for (int i = 0; i < BS; ++i) {
+    const z64 x = Xc[i * BS + j];
+    out[i * BS + j] = {(R)x.re, (R)x.im};
+  }
+}
+
+// Workspace of getrs: the block tickets of the two solves (words 0 and 2),
+// the failure flag (word 4), then the lower solve's output (n x 4 complex).
+template <typename R>
+static size_t getrs_work_bytes(int n) {
+  return 256 + sizeof(cplx<R>) * 4 * (size_t)std::max(n, 1);
+}
+
+// getrs_inv adds the inverted diagonal blocks of L and U after that.
+template <typename R>
+static size_t getrs_inv_work_bytes(int n) {
+  const size_t nblk = (size_t)(std::max(n, 1) + kTrsvBS - 1) / kTrsvBS;
+  return getrs_work_bytes<R>(n) + sizeof(cplx<R>) * 2 * nblk * kTrsvBS * kTrsvBS;
+}
+
+template <typename R>
+static int trtri_blocks(int n, const cplx<R>* LU, int64_t lda, void* work, hipStream_t s) {
+  if (n <= 0) return 0;
+  const int nblk = (n + kTrsvBS - 1) / kTrsvBS;
+  cplx<R>* D = reinterpret_cast<cplx<R>*>(reinterpret_cast<unsigned char*>(work) + getrs_work_bytes<R>(n));
+  const size_t lds = 2 * sizeof(z64) * kTrsvBS * kTrsvBS;
+  static bool attr_set = false;
+  if (!attr_set) {
+    if (hipFuncSetAttribute((const void*)trtri_blocks_kernel<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds) != hipSuccess)
+      return fail("trtri_blocks_kernel: cannot reserve LDS");
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(trtri_blocks_kernel<R>, dim3(nblk, 2), dim3(kTrsvBS), lds, s, n, LU, lda, D);
+  return check_launch("trtri_blocks_kernel");
+}
+
+template <typename R, bool INV>
 static int getrs(int n, int nrhs, const cplx<R>* LU, int64_t lda, const int32_t* perm, const cplx<R>* B, cplx<R>* X,
                  unsigned* flags, hipStream_t s) {
   if (n <= 0 || nrhs <= 0) return 0;
   if (nrhs > 4) return fail("mom_getrs: at most 4 right-hand sides");
   const int nblk = (n + kTrsvBS - 1) / kTrsvBS;
+  cplx<R>* Y = reinterpret_cast<cplx<R>*>(reinterpret_cast<unsigned char*>(flags) + 256);
+  const int64_t nx = (int64_t)n * nrhs;
+  const dim3 fg((unsigned)((nx + 255) / 256));
+  const cplx<R>* DL = INV ? reinterpret_cast<const cplx<R>*>(reinterpret_cast<const unsigned char*>(flags) +
+                                                             getrs_work_bytes<R>(n))
+                          : nullptr;
+  const cplx<R>* DU = INV ? DL + (int64_t)nblk * kTrsvBS * kTrsvBS : nullptr;
   int rc;
   hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, flags, 5);
   if ((rc = check_launch("zero_u32_kernel"))) return rc;
-  hipLaunchKernelGGL(gather_rows_kernel<R>, dim3((n * nrhs + 255) / 256), dim3(256), 0, s, n, nrhs, perm, B, X);
+  hipLaunchKernelGGL(gather_rows_kernel<R>, fg, dim3(256), 0, s, n, nrhs, perm, B, X);
   if ((rc = check_launch("gather_rows_kernel"))) return rc;
-#define MOM_TRSV(NR)                                                                                            \
-  case NR:                                                                                                      \
-    hipLaunchKernelGGL((trsv_kernel<R, true, NR>), dim3(nblk), dim3(256), 0, s, n, LU, lda, X, flags, flags + 2, \
-                       g_spin_limit);                                                                             \
-    if ((rc = check_launch("trsv_kernel"))) return rc;                                                          \
-    hipLaunchKernelGGL((trsv_kernel<R, false, NR>), dim3(nblk), dim3(256), 0, s, n, LU, lda, X, flags + 1,       \
-                       flags + 3, g_spin_limit);                                                                \
+  hipLaunchKernelGGL(fill_sentinel_kernel<R>, fg, dim3(256), 0, s, Y, nx);
+  if ((rc = check_launch("fill_sentinel_kernel"))) return rc;
+#define MOM_TRSV(NR)                                                                                                  \
+  case NR:                                                                                                            \
+    hipLaunchKernelGGL((trsv_kernel<R, true, NR, INV>), dim3(nblk), dim3(256), 0, s, n, LU, lda, X, Y, DL, flags,    \
+                       flags + 4, g_spin_limit);                                                                      \
+    if ((rc = check_launch("trsv_kernel"))) return rc;                                                                \
+    hipLaunchKernelGGL(fill_sentinel_kernel<R>, fg, dim3(256), 0, s, X, nx);                                          \
+    if ((rc = check_launch("fill_sentinel_kernel"))) return rc;                                                       \
+    hipLaunchKernelGGL((trsv_kernel<R, false, NR, INV>), dim3(nblk), dim3(256), 0, s, n, LU, lda, Y, X, DU,          \
+                       flags + 2, flags + 4, g_spin_limit);                                                           \
     break;
   switch (nrhs) {
     MOM_TRSV(1)
@@ -1050,12 +1199,10 @@ extern "C" int mom_build_perm(int n, const int32_t* ipiv, int32_t* perm, void* s
   return check_launch("build_perm_kernel");
 }
 
-// Workspace of mom_getrs: the four progress / ticket counters of the two
-// triangular solves (the same for every n and precision).
+// Workspace of mom_getrs: ticket and failure flag of each triangular solve
+// and the lower solve's output.
 extern "C" int64_t mom_getrs_work_bytes(int prec, int n) {
-  (void)prec;
-  (void)n;
-  return 256;
+  return prec == 64 ? (int64_t)getrs_work_bytes<double>(n) : (int64_t)getrs_work_bytes<float>(n);
 }
 
 extern "C" int mom_getrs(int prec, int n, int nrhs, const void* LU, int64_t lda, const int32_t* perm, const void* B,
@@ -1064,9 +1211,34 @@ extern "C" int mom_getrs(int prec, int n, int nrhs, const void* LU, int64_t lda,
   if (B == X) return fail("mom_getrs: B and X must not alias");
   hipStream_t s = (hipStream_t)stream;
   unsigned* flags = reinterpret_cast<unsigned*>(work);
-  if (prec == 64) return getrs<double>(n, nrhs, (const z64*)LU, lda, perm, (const z64*)B, (z64*)X, flags, s);
-  if (prec == 32) return getrs<float>(n, nrhs, (const c32*)LU, lda, perm, (const c32*)B, (c32*)X, flags, s);
+  if (prec == 64) return getrs<double, false>(n, nrhs, (const z64*)LU, lda, perm, (const z64*)B, (z64*)X, flags, s);
+  if (prec == 32) return getrs<float, false>(n, nrhs, (const c32*)LU, lda, perm, (const c32*)B, (c32*)X, flags, s);
   return fail("mom_getrs: prec must be 64 or 32");
+}
+
+// Workspace of mom_trtri_blocks / mom_getrs_inv: that of mom_getrs followed
+// by the inverted diagonal blocks of L and U.
+extern "C" int64_t mom_getrs_inv_work_bytes(int prec, int n) {
+  return prec == 64 ? (int64_t)getrs_inv_work_bytes<double>(n) : (int64_t)getrs_inv_work_bytes<float>(n);
+}
+
+extern "C" int mom_trtri_blocks(int prec, int n, const void* LU, int64_t lda, void* work, void* stream) {
+  if (!LU || !work) return fail("mom_trtri_blocks: null device pointer");
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == 64) return trtri_blocks<double>(n, (const z64*)LU, lda, work, s);
+  if (prec == 32) return trtri_blocks<float>(n, (const c32*)LU, lda, work, s);
+  return fail("mom_trtri_blocks: prec must be 64 or 32");
+}
+
+extern "C" int mom_getrs_inv(int prec, int n, int nrhs, const void* LU, int64_t lda, const int32_t* perm,
+                             const void* B, void* X, void* work, void* stream) {
+  if (!LU || !perm || !B || !X || !work) return fail("mom_getrs_inv: null device pointer");
+  if (B == X) return fail("mom_getrs_inv: B and X must not alias");
+  hipStream_t s = (hipStream_t)stream;
+  unsigned* flags = reinterpret_cast<unsigned*>(work);
+  if (prec == 64) return getrs<double, true>(n, nrhs, (const z64*)LU, lda, perm, (const z64*)B, (z64*)X, flags, s);
+  if (prec == 32) return getrs<float, true>(n, nrhs, (const c32*)LU, lda, perm, (const c32*)B, (c32*)X, flags, s);
+  return fail("mom_getrs_inv: prec must be 64 or 32");
 }
 
 extern "C" int mom_set_spin_limit(unsigned polls) {

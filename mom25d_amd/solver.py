@@ -97,11 +97,13 @@ def via_inductance(vias, h, spans=None):
     return 0.5 * filament_mutual(2 * h, d)
 
 
-def via_impedance(vias, spans, omega, sections):
+def via_impedance(vias, spans, omega, sections, cache=None):
     """Vector-potential impedance [ohm] of the via segments per stack section
     (Engine/mom25d/solver.py:125-199): open sections by quasi-static partial
     inductances of piece + image, PEC-closed sections by parallel-plate TEM
-    and TM_n modes."""
+    and TM_n modes.  cache: a dict kept by the caller for one set of vias
+    and spans; the partial inductances of open sections (independent of
+    frequency) are computed once into it."""
     from scipy.special import hankel2, kv
 
     xy = np.array([v.xy for v in vias], float).reshape(-1, 2)
@@ -122,9 +124,13 @@ def via_impedance(vias, spans, omega, sections):
         a1, a2, r = a1[on], a2[on], rho[np.ix_(on, on)]
         d = zt - zb
         if k is None:
-            p1, p2 = a1[:, None], a2[:, None]
-            q1, q2 = a1[None, :], a2[None, :]
-            L = filament_mutual_z(p1, p2, q1, q2, r) + filament_mutual_z(p1, p2, -q2, -q1, r)
+            L = None if cache is None else cache.get((zb, zt))
+            if L is None:
+                p1, p2 = a1[:, None], a2[:, None]
+                q1, q2 = a1[None, :], a2[None, :]
+                L = filament_mutual_z(p1, p2, q1, q2, r) + filament_mutual_z(p1, p2, -q2, -q1, r)
+                if cache is not None:
+                    cache[(zb, zt)] = L
             Zs = 1j * omega * L
         else:
             la = a2 - a1
@@ -320,7 +326,10 @@ class Solver:
 
     precision : "fp64" -- complex128 LU (the plain FP64 path);
                 "mixed" -- complex64 LU on the matrix cores with FP64
-                iterative refinement against the FP64 impedance matrix.
+                iterative refinement against the FP64 impedance matrix
+                (GMRES preconditioned by the LU, restarted every
+                gmres_restart steps, at most refine_maxit cycles, to a
+                normwise backward error of refine_tol).
     method    : "auto" / "dense" (the H-matrix solver is not part of this build).
     surrogate : a trained GreensSurrogate of this stack: the remainders of
                 the Green's tables come from its networks, corrected and
@@ -348,7 +357,8 @@ class Solver:
 
     def __init__(self, mesh, substrate, z0=None, verbose=True, method="auto", v_weight=None, z0_recip_min=None,
                  tand_floor=None, precision="fp64", device="cuda", panel=None, refine_tol=1e-15, refine_maxit=20,
-                 lanes=None, surrogate=None, surrogate_tol=2e-3, workers=None, graphs=False, autotune=False):
+                 lanes=None, surrogate=None, surrogate_tol=2e-3, workers=None, graphs=False, autotune=False,
+                 gmres_restart=120):
         from .layered import Stackup
 
         if method not in ("auto", "dense"):
@@ -393,7 +403,7 @@ class Solver:
         self.tuning = dict(jb=self.panel, nb=self.block, tile=self.tile)
         self.refine_tol = refine_tol
         self.refine_maxit = refine_maxit
-        self.gmres_restart = 30
+        self.gmres_restart = int(gmres_restart)
         self.lanes = lanes
         self.workers = workers
         self.graphs = graphs
@@ -405,6 +415,7 @@ class Solver:
         self._plan = None
         self._bufs = {}
         self._keep = {}
+        self._via_cache = {}
         _check_abi()
         self.groups = {}
         for p, port in enumerate(self.ports):
@@ -495,25 +506,27 @@ class Solver:
         omega = g.omega
         inv_jwe = 1 / (1j * omega * EPS0)
         nv = plan.nv
-        E = np.zeros((nv, nv), complex)
-        im0 = {}
+        # sum over the end levels of both vias of s_a s_b Im gV_ab(0):
+        # S^T M S with S[level, via] the summed end signs
+        levels = sorted({la for ek in plan.via_ends for la, _ in ek})
+        row = {la: i for i, la in enumerate(levels)}
+        S = np.zeros((len(levels), nv))
         for k, ek in enumerate(plan.via_ends):
-            for m, em in enumerate(plan.via_ends):
-                acc = 0.0
-                for la, sa in ek:
-                    for lb, sb in em:
-                        if (la, lb) not in im0:
-                            pr = g.pair(la, lb)
-                            rho = 0.0 if pr.dz else 1e-6 * g.h
-                            im0[la, lb] = pr.gV(np.array([rho]))[0].imag
-                        acc += sa * sb * im0[la, lb]
-                E[k, m] -= inv_jwe * 1j * acc
+            for la, sa in ek:
+                S[row[la], k] += sa
+        M = np.zeros((len(levels), len(levels)))
+        for la in levels:
+            for lb in levels:
+                pr = g.pair(la, lb)
+                rho = 0.0 if pr.dz else 1e-6 * g.h
+                M[row[la], row[lb]] = pr.gV(np.array([rho]))[0].imag
+        E = (-inv_jwe * 1j) * (S.T @ M @ S).astype(complex)
         st = self.stack
         segs = self.mesh.via_segments
         vz = {n: st.level_z(n) for n in self.mesh.layer_names}
         vz["gnd"] = 0.0
         spans = [(vz[s["bottom"]], vz[s["top"]]) for s in segs]
-        E += via_impedance(plan.vias, spans, omega, g.via_sections())
+        E += via_impedance(plan.vias, spans, omega, g.via_sections(), self._via_cache)
         if zs_via:
             lengths = np.array([z1 - z0 for z0, z1 in spans])
             E += np.diag([zs_via * l / (np.pi * v.drill) for l, v in zip(lengths, plan.vias)])
@@ -610,7 +623,10 @@ class Solver:
         wbytes = int(_lib.lib().mom_getrf_work_bytes(prec, n, jb))
         work = self._buf(f"work{prec}", (wbytes,), torch.uint8, lane)
         groups = self._panel_groups()
-        sw = self._buf(f"swork{prec}", (int(_lib.lib().mom_getrs_work_bytes(prec, n)),), torch.uint8, lane)
+        # the complex64 solves (GMRES preconditioner) use the inverted diagonal
+        # blocks of the factors, kept after the getrs workspace
+        swb = _lib.lib().mom_getrs_work_bytes(prec, n) if prec == 64 else _lib.lib().mom_getrs_inv_work_bytes(prec, n)
+        sw = self._buf(f"swork{prec}", (int(swb),), torch.uint8, lane)
         X = self._buf("X", (n, nrhs), torch.complex128, lane)
 
         if self.tile is not None:
@@ -623,6 +639,8 @@ class Solver:
             _lib.call("mom_build_perm", n, P(ipiv), P(perm), stream)
             if Z32 is None:
                 _lib.call("mom_getrs", 64, n, nrhs, P(A), A.stride(0), P(perm), P(B), P(X), P(sw), stream)
+            else:
+                _lib.call("mom_trtri_blocks", 32, n, P(A), A.stride(0), P(sw), stream)
 
         if graph:
             key = (lane, prec)
@@ -695,7 +713,7 @@ class Solver:
         def precond(V):
             W32 = V.to(torch.complex64)
             Y32 = torch.empty_like(W32)
-            _lib.call("mom_getrs", 32, n, nrhs, P(A32), A32.stride(0), P(perm), P(W32), P(Y32), P(sw), stream)
+            _lib.call("mom_getrs_inv", 32, n, nrhs, P(A32), A32.stride(0), P(perm), P(W32), P(Y32), P(sw), stream)
             return Y32.to(torch.complex128)
 
         def matvec(Y):  # (D Z D) Y

@@ -207,3 +207,42 @@ def test_bounded_waits_report_failure():
     X, _, info0 = _lu_solve(A, B, 64, 64)
     assert info0 == 0
     assert np.linalg.norm(A @ X - B) <= 1e-12 * np.linalg.norm(A) * np.linalg.norm(X)
+
+
+@pytest.mark.parametrize("prec,n,nrhs", [(32, 1, 1), (32, 300, 2), (32, 2049, 4), (64, 777, 3)])
+def test_getrs_with_inverted_blocks(prec, n, nrhs):
+    """The solve with explicitly inverted 64 x 64 diagonal blocks: the same
+    solution as the substitution solve up to cond(block) rounding, and in
+    complex128 a backward error near round-off (a diagonally dominant matrix,
+    so the blocks are well conditioned); the inverses are reused by a second
+    solve with new right-hand sides."""
+    rng = np.random.default_rng(n + nrhs)
+    dt = torch.complex128 if prec == 64 else torch.complex64
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + 4 * np.sqrt(n) * np.eye(n)
+    Ad = torch.from_numpy(A).to(DEV, dt)
+    ipiv = torch.empty(n, dtype=torch.int32, device=DEV)
+    perm = torch.empty(n, dtype=torch.int32, device=DEV)
+    info = torch.zeros(4, dtype=torch.int32, device=DEV)
+    wb = int(_lib.lib().mom_getrf_work_bytes(prec, n, 64))
+    work = torch.empty(wb, dtype=torch.uint8, device=DEV)
+    sw = torch.empty(int(_lib.lib().mom_getrs_inv_work_bytes(prec, n)), dtype=torch.uint8, device=DEV)
+    assert sw.numel() > int(_lib.lib().mom_getrs_work_bytes(prec, n))
+    s = _lib.stream_ptr()
+    P = _lib.ptr
+    _lib.call("mom_getrf", prec, n, P(Ad), n, P(ipiv), P(info), P(work), wb, 64, 0, s)
+    _lib.call("mom_build_perm", n, P(ipiv), P(perm), s)
+    _lib.call("mom_trtri_blocks", prec, n, P(Ad), n, P(sw), s)
+    for rep in range(2):
+        B = rng.standard_normal((n, nrhs)) + 1j * rng.standard_normal((n, nrhs))
+        Bd = torch.from_numpy(B).to(DEV, dt)
+        X = torch.empty_like(Bd)
+        _lib.call("mom_getrs_inv", prec, n, nrhs, P(Ad), n, P(perm), P(Bd), P(X), P(sw), s)
+        got = X.cpu().numpy()
+        assert int(sw[16:20].view(torch.int32).cpu()[0]) == 0
+        r = np.linalg.norm(A @ got - B) / (np.linalg.norm(A, 2) * np.linalg.norm(got))
+        assert r < (1e-6 if prec == 32 else 1e-14), (rep, r)
+        Xs = torch.empty_like(Bd)
+        sw2 = torch.empty(int(_lib.lib().mom_getrs_work_bytes(prec, n)), dtype=torch.uint8, device=DEV)
+        _lib.call("mom_getrs", prec, n, nrhs, P(Ad), n, P(perm), P(Bd), P(Xs), P(sw2), s)
+        ref = Xs.cpu().numpy()
+        assert np.max(np.abs(got - ref)) < (1e-5 if prec == 32 else 1e-12) * np.max(np.abs(ref))
