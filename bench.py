@@ -5,11 +5,12 @@ Workload (default): the 40 x 20 mm via-stitched grounded-coplanar pour
 (`workloads.gcpw40`, 12 104 RWG + 324 via unknowns = 12 428, 51 frequencies
 1-20 GHz).  A step is one frequency point of the sweep: Green's tables, FP64
 fill, LU + port solve, de-embedding to S; by default the whole sweep is
-timed (--steps K: K points spread evenly over it).  The headline path is chosen with
---precision (default: the plain FP64 path, complex128 LU); the other path
-(FP64 fill, complex64 LU on the matrix cores as the preconditioner of FP64
-GMRES refinement) is timed beside it on the same frequencies and |dS|
-between the two is reported.  Inputs (mesh, fill plan) are resident on the
+timed (--steps K: K points spread evenly over it).  The headline path is
+chosen with --precision: "mixed" (default: FP64 fill, complex64 LU on the
+matrix cores as the preconditioner of FP64 GMRES refinement, each lane's LU
+reused for its neighbouring frequencies while that pays) or "fp64" (the
+plain path, a complex128 LU per point); the other path is timed beside it on
+the same frequencies and |dS| between the two is reported.  Inputs (mesh, fill plan) are resident on the
 GPU before the timed region; the Green's tables of the timed frequencies are
 built inside it.
 
@@ -56,6 +57,7 @@ def time_path(mesh, sub, freqs, precision, warmup, steps, torch, lanes=None, aut
     s.geometry().upload(s.device)
     if warmup:
         s.sweep(freqs[len(freqs) // 2 - warmup // 2:][:warmup])
+    s.lu_stats = {k: 0 for k in s.lu_stats}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     r = s.sweep(sweep_points(freqs, steps))
@@ -161,7 +163,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="gcpw40")
     ap.add_argument("--cpu-sample", type=int, default=1, help="frequency points timed on the CPU oracle (0: skip)")
-    ap.add_argument("--precision", default="fp64", choices=("fp64", "mixed"), help="headline path")
+    ap.add_argument("--precision", default="mixed", choices=("fp64", "mixed"), help="headline path")
     ap.add_argument("--no-alt", action="store_true", help="skip timing the other path")
     ap.add_argument("--lanes", type=int, default=None, help="frequency points in flight (default: automatic)")
     ap.add_argument("--no-tune", action="store_true", help="default LU panel width / tiles instead of the autotuner's")
@@ -196,12 +198,12 @@ def main():
     sm, rm, tm = time_path(mesh, sub, freqs, prec, args.warmup, args.steps, torch, args.lanes, not args.no_tune,
                            args.panel, tile, args.block)
     lanes = sm._lane_count()
-    print(f"{prec}: {args.steps} points in {tm:.3f} s ({sm.last_refine} refinement steps)", file=sys.stderr, flush=True)
+    print(f"{prec}: {args.steps} points in {tm:.3f} s (LU: {sm.lu_stats})", file=sys.stderr, flush=True)
     other = None
     if not args.no_alt:
         sa, ra, ta = time_path(mesh, sub, freqs, alt, args.warmup, args.steps, torch, args.lanes, not args.no_tune)
         other = dict(precision=alt, value=args.steps / ta, ms_per_step=1e3 * ta / args.steps,
-                     dS_max_vs_headline=float(np.max(np.abs(rm.S - ra.S))), refinement_steps_last=sa.last_refine)
+                     dS_max_vs_headline=float(np.max(np.abs(rm.S - ra.S))), lu_stats=dict(sa.lu_stats))
         print(f"{alt}: {args.steps} points in {ta:.3f} s; |dS| {other['dS_max_vs_headline']:.2e}", file=sys.stderr,
               flush=True)
         del sa
@@ -224,6 +226,7 @@ def main():
                            frac_fp64_valu=fill_flops / tf / PEAK["fp64_valu"]))
     extra["lu_config"] = dict(panel=sm.panel, block=sm.block or sm.panel, tile=[bm, bn],
                               autotuned=not args.no_tune and tile is None)
+    extra["lu_stats"] = dict(sm.lu_stats)  # factorisations / reused LUs / GMRES steps in the timed sweep
     if not args.no_batch:
         extra["batch_random16"] = batch_random16(torch, prec)
         b = extra["batch_random16"]

@@ -45,12 +45,13 @@ __device__ inline cplx<R> load_coherent_c(const cplx<R>* p) {
   return {load_coherent(q), load_coherent(q + 1)};
 }
 
-// Cross-workgroup publication without cache maintenance: data and flags are
-// written and read with device-coherent (relaxed, agent-scope) accesses, and
-// a writer waits for its stores to complete (s_waitcnt 0) before raising the
-// flag.  Agent-scope release / acquire would instead write back / invalidate
-// the whole L2 of the XCD on every flag, which costs microseconds when other
-// kernels keep the L2 busy.
+// Cross-workgroup publication without cache maintenance: data is written
+// and read with device-coherent (relaxed, agent-scope) accesses, and every
+// published value is its own "ready" flag (a sentinel before it is written;
+// see the panel below), so no ordering between stores is needed.  Agent-scope
+// release / acquire would instead write back / invalidate the whole L2 of the
+// XCD on every flag, which costs microseconds when other kernels keep the L2
+// busy.
 template <typename T>
 __device__ inline void store_coherent(T* p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -62,8 +63,6 @@ __device__ inline void store_coherent_c(cplx<R>* p, cplx<R> v) {
   store_coherent(q, v.re);
   store_coherent(q + 1, v.im);
 }
-
-__device__ inline void wait_stores() { __builtin_amdgcn_s_waitcnt(0); }
 
 // Back-off between polls of a flag in memory (units of 64 clocks).
 #ifndef MOM_POLL_SLEEP
@@ -79,15 +78,40 @@ __device__ inline void poll_pause() { __builtin_amdgcn_s_sleep(MOM_POLL_SLEEP); 
 static unsigned g_spin_limit = 1u << 21;
 
 // ------------------------------------------------------------------- panel
-// Per-workgroup publication slot, one per (column parity, workgroup): the
-// candidate pivot value / row index and a generation number, written with
-// release order after the candidate row itself.  Every workgroup polls the G
-// slots of the other workgroups (one thread per slot) instead of a shared
-// counter, so no address sees more than one writer.
+// Values that are themselves the "ready" flags: the publication buffers of a
+// panel (and the solutions of the triangular solves below) are filled with a
+// sentinel NaN (a payload no arithmetic produces) before the launch, every
+// column of a panel has buffers of its own, and a reader polls each value
+// until it is no longer the sentinel.  A writer needs no completion wait and
+// no separate flag, so a column costs one memory round trip less.
+template <typename R>
+struct Sentinel;
+template <>
+struct Sentinel<double> {
+  static constexpr unsigned long long bits = 0x7FF4D0D0D0D0D0D0ull;
+  static __device__ inline bool is(double v) { return (unsigned long long)__double_as_longlong(v) == bits; }
+  static __device__ inline double value() { return __longlong_as_double((long long)bits); }
+};
+template <>
+struct Sentinel<float> {
+  static constexpr unsigned bits = 0x7FA0D0D0u;
+  static __device__ inline bool is(float v) { return __float_as_uint(v) == bits; }
+  static __device__ inline float value() { return __uint_as_float(bits); }
+};
+constexpr int32_t kIdxSentinel = -2;
+
+template <typename R>
+__device__ inline bool is_sentinel(cplx<R> v) {
+  return Sentinel<R>::is(v.re) || Sentinel<R>::is(v.im);
+}
+
+// Per-(column, workgroup) publication slot: the candidate pivot's |re|+|im|
+// and row (each checked against its own sentinel: a half-written slot reads
+// as not ready).
 struct alignas(16) PivotSlot {
   double val;
   int32_t idx;
-  uint32_t gen;
+  int32_t pad;
 };
 
 template <typename R>
@@ -95,14 +119,24 @@ struct PanelArgs {
   cplx<R>* A;
   int64_t lda;
   int n, k0, jb, rows_per;  // rows_per: rows owned by one workgroup
-  uint32_t gen0;            // generation of this panel's first column
   uint32_t spin_limit;      // polls before a wait gives up
   int32_t* ipiv;
   int32_t* info;
-  PivotSlot* slots;         // [2][256]
-  cplx<R>* slot_row;        // [2][256][jb]
-  cplx<R>* diag_row;        // [2][jb]
+  PivotSlot* slots;         // [jb][G]
+  cplx<R>* slot_row;        // [jb][G][jb]: each workgroup's candidate row
+  cplx<R>* diag_row;        // [jb][jb]: row c before the interchange
 };
+
+// The sentinel in a panel's publication buffers: nslot slots and nrow
+// complex entries (candidate rows followed by the diagonal rows).
+template <typename R>
+__global__ void panel_reset_kernel(PivotSlot* slots, int64_t nslot, cplx<R>* rowbuf, int64_t nrow) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nrow; i += stride)
+    rowbuf[i] = {Sentinel<R>::value(), Sentinel<R>::value()};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nslot; i += stride)
+    slots[i] = {Sentinel<double>::value(), kIdxSentinel, 0};
+}
 
 // (value, index) maximum over the workgroup, ties to the smaller index.
 template <typename R>
@@ -149,8 +183,8 @@ __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<R> a) {
 
   for (int j = 0; j < jb; ++j) {
     const int c = a.k0 + j;
-    const int par = j & 1;
-    const uint32_t gen = a.gen0 + (uint32_t)j;
+    cplx<R>* const crow = a.slot_row + (int64_t)j * G * jb;  // this column's candidate rows
+    cplx<R>* const drow_pub = a.diag_row + (int64_t)j * jb;
     // phase A: local candidate among owned rows >= c, published with its row
     R best = (R)-1;
     int bidx = 0x7fffffff;
@@ -160,33 +194,34 @@ __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<R> a) {
     }
     block_argmax(best, bidx, red_v, red_i);
     if (bidx != 0x7fffffff)
-      for (int q = tid; q < jb; q += 256)
-        store_coherent_c(a.slot_row + ((int64_t)par * 256 + g) * jb + q, rows[(bidx - r0) * jb + q]);
+      for (int q = tid; q < jb; q += 256) store_coherent_c(crow + (int64_t)g * jb + q, rows[(bidx - r0) * jb + q]);
     if (c >= r0 && c < r1)
-      for (int q = tid; q < jb; q += 256) store_coherent_c(a.diag_row + par * jb + q, rows[(c - r0) * jb + q]);
+      for (int q = tid; q < jb; q += 256) store_coherent_c(drow_pub + q, rows[(c - r0) * jb + q]);
     if (tid == 0) {
-      PivotSlot* sl = a.slots + par * 256 + g;
+      PivotSlot* sl = a.slots + (int64_t)j * G + g;
       store_coherent(&sl->val, (double)best);
       store_coherent(&sl->idx, bidx);
     }
-    wait_stores();
-    __syncthreads();
-    if (tid == 0) store_coherent(&a.slots[par * 256 + g].gen, gen);
     // phase B: wait for every workgroup's slot, global pivot
     R v = (R)-1;
     int i = 0x7fffffff;
     if (tid < G) {
-      PivotSlot* sl = a.slots + par * 256 + tid;
+      PivotSlot* sl = a.slots + (int64_t)j * G + tid;
       unsigned spins = 0;
-      while (load_coherent(&sl->gen) != gen) {
+      double sv;
+      int si;
+      for (;;) {
+        sv = load_coherent(&sl->val);
+        si = load_coherent(&sl->idx);
+        if (!Sentinel<double>::is(sv) && si != kIdxSentinel) break;
         poll_pause();
         if (++spins > a.spin_limit) {
           s_abort = 1;
           break;
         }
       }
-      v = (R)load_coherent(&sl->val);
-      i = load_coherent(&sl->idx);
+      v = (R)sv;
+      i = si;
     }
     const int mine_i = i;
     block_argmax(v, i, red_v, red_i);
@@ -202,11 +237,33 @@ __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<R> a) {
       return;
     }
     const int win = s_win;
-    for (int q = tid; q < jb; q += 256) {
-      prow[q] = load_coherent_c(a.slot_row + ((int64_t)par * 256 + win) * jb + q);
-      drow[q] = load_coherent_c(a.diag_row + par * jb + q);
+    // the winner's row and row c, polled element by element (published with
+    // the slots, so normally already there)
+    for (int q = tid; q < 2 * jb; q += 256) {
+      const cplx<R>* src = q < jb ? crow + (int64_t)win * jb + q : drow_pub + (q - jb);
+      unsigned spins = 0;
+      cplx<R> x;
+      for (;;) {
+        x = load_coherent_c(src);
+        if (!is_sentinel(x)) break;
+        poll_pause();
+        if (++spins > a.spin_limit) {
+          s_abort = 1;
+          break;
+        }
+      }
+      if (q < jb)
+        prow[q] = x;
+      else
+        drow[q - jb] = x;
     }
     __syncthreads();
+    if (s_abort) {  // as above: identity pivots for the rest of the panel
+      if (g == 0)
+        for (int q = tid; q < jb - j; q += 256) a.ipiv[c + q] = c + q;
+      if (tid == 0) a.info[1] = 1;
+      return;
+    }
     const bool nonzero = (prow[j].re != (R)0) || (prow[j].im != (R)0);
     const cplx<R> inv = nonzero ? cdiv(cplx<R>{(R)1, (R)0}, prow[j]) : cplx<R>{(R)0, (R)0};
     if (tid == 0) {
@@ -542,21 +599,6 @@ __device__ __forceinline__ float lane_value(float v, int k) {
 }
 
 template <typename R>
-struct Sentinel;
-template <>
-struct Sentinel<double> {
-  static constexpr unsigned long long bits = 0x7FF4D0D0D0D0D0D0ull;  // a signalling NaN no arithmetic produces
-  static __device__ inline bool is(double v) { return (unsigned long long)__double_as_longlong(v) == bits; }
-  static __device__ inline double value() { return __longlong_as_double((long long)bits); }
-};
-template <>
-struct Sentinel<float> {
-  static constexpr unsigned bits = 0x7FA0D0D0u;
-  static __device__ inline bool is(float v) { return __float_as_uint(v) == bits; }
-  static __device__ inline float value() { return __uint_as_float(bits); }
-};
-
-template <typename R>
 __global__ void fill_sentinel_kernel(cplx<R>* x, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x[i] = {Sentinel<R>::value(), Sentinel<R>::value()};
@@ -626,7 +668,7 @@ __global__ __launch_bounds__(256) void trsv_kernel(int n, const cplx<R>* __restr
         unsigned spins = 0;
         for (;;) {
           v = load_coherent_c(Xout + (int64_t)(j0 + i) * NR + c);
-          if (!Sentinel<R>::is(v.re) && !Sentinel<R>::is(v.im)) break;
+          if (!is_sentinel(v)) break;
           poll_pause();
           if (++spins > spin_limit) {
             s_abort = 1;
@@ -843,12 +885,13 @@ static int device_cus() {
   return g_ncu;
 }
 
+// Panel publication buffers for up to 256 workgroups: a slot per (column,
+// workgroup), a candidate row per (column, workgroup) and row c per column.
 template <typename R>
 static size_t getrf_work_bytes(int n, int jb) {
   size_t b = 0;
-  b += sizeof(PivotSlot) * 2 * 256 + 256;
-  b += sizeof(cplx<R>) * 2 * (size_t)256 * jb + 256;
-  b += sizeof(cplx<R>) * 2 * (size_t)jb + 256;
+  b += sizeof(PivotSlot) * (size_t)jb * 256 + 256;
+  b += sizeof(cplx<R>) * ((size_t)jb * 256 * jb + (size_t)jb * jb) + 256;
   return b + 1024;
 }
 
@@ -879,7 +922,7 @@ static size_t g_panel_lds_max = 0;
 
 template <typename R>
 static int launch_panel(cplx<R>* A, int64_t lda, int n, int k0, int b, int32_t* ipiv, int32_t* info,
-                        PivotSlot* slots, cplx<R>* slot_row, cplx<R>* diag_row, int gmax, hipStream_t s) {
+                        PivotSlot* slots, cplx<R>* rowbuf, int gmax, hipStream_t s) {
   const int m = n - k0;
   const size_t lds_max = 120 * 1024;
   if (g_panel_lds_max == 0) {
@@ -896,8 +939,14 @@ static int launch_panel(cplx<R>* A, int64_t lda, int n, int k0, int b, int32_t* 
   G = std::max(G, std::min(gmax, (m + 255) / 256));  // at least one row per thread block of work
   rp = (m + G - 1) / G;
   G = (m + rp - 1) / rp;
-  const uint32_t gen0 = 1u + (uint32_t)k0;  // unique per column of the factorisation
-  PanelArgs<R> pa{A, lda, n, k0, b, rp, gen0, g_spin_limit, ipiv, info, slots, slot_row, diag_row};
+  // publication buffers of this panel: [b][G] slots, [b][G][b] candidate
+  // rows, then [b][b] diagonal rows, all set to the sentinel first
+  const int64_t nslot = (int64_t)b * G, nrow = (int64_t)b * G * b + (int64_t)b * b;
+  const unsigned rgrid = (unsigned)std::min<int64_t>(2048, (nrow + 255) / 256);
+  hipLaunchKernelGGL(panel_reset_kernel<R>, dim3(rgrid), dim3(256), 0, s, slots, nslot, rowbuf, nrow);
+  int rc = check_launch("panel_reset_kernel");
+  if (rc) return rc;
+  PanelArgs<R> pa{A, lda, n, k0, b, rp, g_spin_limit, ipiv, info, slots, rowbuf, rowbuf + (int64_t)b * G * b};
   // An ordinary launch, not a cooperative one: cooperative launches are
   // serialised across streams, which would stop the panels of concurrent
   // factorisations from overlapping.  Every workgroup of the panel is
@@ -970,8 +1019,7 @@ static int update_block(cplx<R>* A, int64_t lda, int n, int k0, int b, int r0, i
 
 struct PanelWork {
   PivotSlot* slots;
-  void* slot_row;
-  void* diag_row;
+  void* rowbuf;
   int gmax;
 };
 
@@ -985,8 +1033,7 @@ static int factor_block(cplx<R>* A, int64_t lda, int n, int k0, int nb, int jb, 
   const int kend = k0 + nb;
   for (int q0 = k0; q0 < kend; q0 += jb) {
     const int bq = std::min(jb, kend - q0), q1 = q0 + bq;
-    if ((rc = launch_panel<R>(A, lda, n, q0, bq, ipiv, info, pw.slots, (cplx<R>*)pw.slot_row, (cplx<R>*)pw.diag_row,
-                              pw.gmax, s)))
+    if ((rc = launch_panel<R>(A, lda, n, q0, bq, ipiv, info, pw.slots, (cplx<R>*)pw.rowbuf, pw.gmax, s)))
       return rc;
     if ((rc = laswp_block<R>(A, lda, n, k0, q0, q0, bq, ipiv, s))) return rc;
     if (q1 < kend) {
@@ -1020,14 +1067,10 @@ static int getrf(int n, cplx<R>* A, int64_t lda, int32_t* ipiv, int32_t* info, v
   };
   const int nblock = (n + nb - 1) / nb;
   PanelWork pw;
-  pw.slots = reinterpret_cast<PivotSlot*>(take(sizeof(PivotSlot) * 2 * 256));
-  pw.slot_row = take(sizeof(cplx<R>) * 2 * 256 * (size_t)jb);
-  pw.diag_row = take(sizeof(cplx<R>) * 2 * (size_t)jb);
+  pw.slots = reinterpret_cast<PivotSlot*>(take(sizeof(PivotSlot) * (size_t)jb * 256));
+  pw.rowbuf = take(sizeof(cplx<R>) * ((size_t)jb * 256 * jb + (size_t)jb * jb));
   pw.gmax = gmax;
   int rc;
-  hipLaunchKernelGGL(zero_u32_kernel, dim3((2 * 256 * 4 + 255) / 256), dim3(256), 0, s,
-                     reinterpret_cast<unsigned*>(pw.slots), 2 * 256 * 4);
-  if ((rc = check_launch("zero_u32_kernel"))) return rc;
   hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned*>(info), 2);
   if ((rc = check_launch("zero_u32_kernel"))) return rc;
   Lookahead* la = nullptr;

@@ -330,6 +330,15 @@ class Solver:
                 (GMRES preconditioned by the LU, restarted every
                 gmres_restart steps, at most refine_maxit cycles, to a
                 normwise backward error of refine_tol).
+    reuse_lu  : mixed precision, in a sweep: a lane keeps its last complex64
+                LU as the preconditioner of its next frequency points (the
+                FP64 GMRES reaches the same tolerance with a few more steps)
+                and factors afresh once the GMRES steps of the last reused
+                point exceed the lane's average cost per point since that
+                factorisation (costs in GMRES steps; one factorisation counts
+                lu_equiv steps, by default a model of this implementation:
+                LU time / step time at this N).  lu_stats counts
+                factorisations, reuses and GMRES steps.
     method    : "auto" / "dense" (the H-matrix solver is not part of this build).
     surrogate : a trained GreensSurrogate of this stack: the remainders of
                 the Green's tables come from its networks, corrected and
@@ -358,7 +367,7 @@ class Solver:
     def __init__(self, mesh, substrate, z0=None, verbose=True, method="auto", v_weight=None, z0_recip_min=None,
                  tand_floor=None, precision="fp64", device="cuda", panel=None, refine_tol=1e-15, refine_maxit=20,
                  lanes=None, surrogate=None, surrogate_tol=2e-3, workers=None, graphs=False, autotune=False,
-                 gmres_restart=120):
+                 gmres_restart=120, reuse_lu=True, lu_equiv=None):
         from .layered import Stackup
 
         if method not in ("auto", "dense"):
@@ -404,6 +413,10 @@ class Solver:
         self.refine_tol = refine_tol
         self.refine_maxit = refine_maxit
         self.gmres_restart = int(gmres_restart)
+        self.reuse_lu = bool(reuse_lu)
+        self.lu_equiv = lu_equiv
+        self._anchor = {}
+        self.lu_stats = dict(factorisations=0, reused=0, fallbacks=0, gmres_steps=0)
         self.lanes = lanes
         self.workers = workers
         self.graphs = graphs
@@ -582,13 +595,22 @@ class Solver:
                       P(d["pad_tri"]), P(d["tp"]), P(d["vp"]), P(d["tm"]), P(d["vm"]), P(d["cV"]), P(E),
                       ijwe.real, ijwe.imag, P(Z), Z.stride(0), stream)
             self._keep[lane] = E
-        dsc = None
         if Z32 is not None:
-            dsc = self._buf("dscale", (n,), torch.float64, lane)
-            _lib.call("mom_diag_scale", n, P(Z), Z.stride(0), P(dsc), stream)
-        _lib.call("mom_symmetrize", n, P(Z), Z.stride(0), P(Z32), Z32.stride(0) if Z32 is not None else 0, P(dsc),
-                  stream)
+            self._scale_copy(Z, Z32, lane)
+        else:
+            _lib.call("mom_symmetrize", n, P(Z), Z.stride(0), None, 0, None, stream)
         return Z
+
+    def _scale_copy(self, Z, Z32, lane=0):
+        """Z <- (Z + Z^T) / 2 and Z32 <- D Z D in complex64 with
+        D = diag(|Z_ii|^-1/2) (kept in the lane's dscale buffer).  Idempotent
+        on Z, so it also serves a Z that is already symmetric."""
+        n = Z.shape[0]
+        P = _lib.ptr
+        stream = _lib.stream_ptr()
+        dsc = self._buf("dscale", (n,), torch.float64, lane)
+        _lib.call("mom_diag_scale", n, P(Z), Z.stride(0), P(dsc), stream)
+        _lib.call("mom_symmetrize", n, P(Z), Z.stride(0), P(Z32), Z32.stride(0), P(dsc), stream)
 
     def impedance(self, freq, greens=None):
         """The impedance matrix at freq (numpy complex128)."""
@@ -692,14 +714,16 @@ class Solver:
         cus = torch.cuda.get_device_properties(self.device).multi_processor_count
         return max(1, cus // lanes)
 
-    def _gmres_ir(self, Z, A32, perm, sw, B, lane=0):
+    def _gmres_ir(self, Z, A32, perm, sw, B, lane=0, max_its=None):
         """Restarted GMRES in FP64 on the diagonally scaled system
         (D Z D) y = D b, x = D y, right-preconditioned by the complex64 LU of
         D Z D.  All right-hand sides advance together (one Krylov basis per
         column).  Target: normwise backward error of the scaled system
         |D b - D Z D y| <= refine_tol |D Z D|_inf |y| (FP64 round-off); a
         restart cycle that does not halve the true residual ends the
-        iteration (the FP64 residual floor is reached)."""
+        iteration (the FP64 residual floor is reached).  With max_its it
+        returns None once that many steps have not reached the target (the
+        caller then factors this matrix itself)."""
         n, nrhs = B.shape
         P = _lib.ptr
         stream = _lib.stream_ptr()
@@ -771,6 +795,10 @@ class Solver:
                 k = j + 1
                 if np.all(np.abs(g[j + 1]) <= 0.25 * target):
                     break
+                if max_its is not None and its >= max_its:
+                    self.last_refine = its
+                    self.lu_stats["gmres_steps"] += its
+                    return None
             y = np.zeros((k, nrhs), complex)
             for c in range(nrhs):
                 y[:, c] = np.linalg.solve(np.triu(H[:k, :k, c]), g[:k, c])
@@ -788,6 +816,7 @@ class Solver:
                               f"error of {np.max(bh / (znorm * ynorm.cpu().numpy())):.1e} (target {self.refine_tol:.0e})",
                               RuntimeWarning, stacklevel=2)
         self.last_refine = its
+        self.lu_stats["gmres_steps"] += its
         return d * Y
 
     def solve_currents(self, freq, excitations=None, greens=None):
@@ -802,15 +831,61 @@ class Solver:
         self._check_solve(0)
         return X
 
-    def _solve_task(self, g, B, lane, graph=False):
+    def _solve_task(self, g, B, lane, graph=False, reuse=False):
         """Generator: fill, LU and port solve of one frequency on the current
         stream with the buffers of `lane`; returns the device currents (a
-        buffer of the lane: read it before the lane's next solve)."""
+        buffer of the lane: read it before the lane's next solve).  reuse
+        (mixed precision, within a sweep): precondition with the lane's last
+        LU when the reuse policy says so (reuse_lu)."""
         n = self.mesh.n_unknowns
         Z = self._buf("Z", (n, n), torch.complex128, lane)
-        Z32 = self._buf("Z32", (n, n), torch.complex64, lane) if self.precision == "mixed" else None
-        self._fill(g, Z, Z32, lane)
-        return (yield from self._lu_solve(Z, B, Z32, lane, graph))
+        mixed = self.precision == "mixed"
+        Z32 = self._buf("Z32", (n, n), torch.complex64, lane) if mixed else None
+        a = self._anchor.get(lane) if mixed and reuse and self.reuse_lu else None
+        if a is not None and self._reuse_next(a, g.freq):
+            self._fill(g, Z, None, lane)
+            cap = max(20, int(a["lu_equiv"] + a["its0"]))
+            X = yield from self._gmres_ir(Z, Z32, self._bufs[("perm", lane)], self._bufs[("swork32", lane)], B, lane,
+                                          max_its=cap)
+            if X is not None:
+                a["cost"] += self.last_refine
+                a["n"] += 1
+                a["last"] = self.last_refine
+                self.lu_stats["reused"] += 1
+                return X
+            self.lu_stats["fallbacks"] += 1
+            self._scale_copy(Z, Z32, lane)
+        else:
+            self._fill(g, Z, Z32, lane)
+        X = yield from self._lu_solve(Z, B, Z32, lane, graph)
+        if mixed:
+            self.lu_stats["factorisations"] += 1
+            eq = self._lu_equiv()
+            self._anchor[lane] = dict(freq=g.freq, lu_equiv=eq, its0=self.last_refine, cost=eq + self.last_refine,
+                                      n=1, last=None)
+        return X
+
+    def _lu_equiv(self):
+        """Cost of one complex64 factorisation in GMRES steps at this N: a
+        model of this implementation on MI355X (LU at ~32 TFLOP/s effective
+        or ~8 us per column of panel latency, whichever is larger; a step =
+        two triangular block chains of ~3.4 us per 64 rows + one FP64 Z read
+        + ~0.2 ms of host round trip)."""
+        if self.lu_equiv is not None:
+            return float(self.lu_equiv)
+        n = self.mesh.n_unknowns
+        t_lu = max(8 / 3 * n ** 3 / 32e12, 8e-6 * n)
+        t_step = 2 * (n / 64) * 3.4e-6 + 16 * n * n / 5e12 + 2e-4
+        return t_lu / t_step
+
+    @staticmethod
+    def _reuse_next(a, freq):
+        """Reuse the anchor's LU for freq: always once, then while the last
+        reuse cost fewer steps than the average per point since the
+        factorisation; never beyond half an octave from the anchor."""
+        if abs(freq - a["freq"]) > 0.5 * a["freq"]:
+            return False
+        return a["last"] is None or a["last"] < a["cost"] / a["n"]
 
     # ----------------------------------------------------------- de-embed
     def charges(self, x, omega):
@@ -902,14 +977,65 @@ class Solver:
         return reciprocal_z0(self._fit_context(), SI, zphi, prior)
 
 
+class LaneCursor:
+    """Hands the indices 0 .. count-1 to `lanes` lanes so that each lane
+    works through neighbouring indices (neighbouring frequencies, whose
+    factorisations can precondition each other): the range is cut into one
+    run per pair of lanes, and the two lanes of a run take its indices from
+    the two ends toward the middle (a lone lane takes its run upward).  A
+    lane whose run is used up splits the run with the most indices left and
+    takes the upper half upward from its middle (a lane that was taking that
+    run downward stays with the upper half).  Every index is handed out
+    exactly once."""
+
+    def __init__(self, count, lanes):
+        sizes = [2] * (lanes // 2) + [1] * (lanes % 2)
+        cuts = np.round(np.cumsum([0] + sizes) / lanes * count).astype(int)
+        self.lo = [int(c) for c in cuts[:-1]]
+        self.hi = [int(c) - 1 for c in cuts[1:]]  # inclusive
+        self.run, self.down = [], []
+        for r, k in enumerate(sizes):
+            self.run += [r] * k
+            self.down += [False, True][:k]
+
+    def _split(self, lane):
+        left = [h - lo + 1 for lo, h in zip(self.lo, self.hi)]
+        r = int(np.argmax(left))
+        if left[r] <= 0:
+            return False
+        mid = (self.lo[r] + self.hi[r] + 1) // 2
+        self.lo.append(mid)
+        self.hi.append(self.hi[r])
+        self.hi[r] = mid - 1
+        nr = len(self.lo) - 1
+        for other in range(len(self.run)):
+            if self.run[other] == r and self.down[other]:
+                self.run[other] = nr
+        self.run[lane], self.down[lane] = nr, False
+        return True
+
+    def take(self, lane):
+        r = self.run[lane]
+        if self.lo[r] > self.hi[r]:
+            if not self._split(lane):
+                return None
+            r = self.run[lane]
+        if self.down[lane]:
+            self.hi[r] -= 1
+            return self.hi[r] + 1
+        self.lo[r] += 1
+        return self.lo[r] - 1
+
+
 def run_lanes(task, count, lanes, device, solvers, graphs=False):
     """Run task(i, lane) for i < count, `lanes` at a time, round robin.
     Each task is a generator that enqueues GPU work on the current stream and
     yields before it waits for it; the lane's stream is made current whenever
     its task resumes, so the GPU work of all lanes overlaps.  Tasks complete
-    in order within a lane, and a lane takes the next index as soon as its
-    task ends.  The solvers bound their LU panels for `lanes` factorisations
-    in flight meanwhile."""
+    in order within a lane, and a lane takes its next index (LaneCursor:
+    neighbouring indices stay on one lane) as soon as its task ends.  The
+    solvers bound their LU panels for `lanes` factorisations in flight
+    meanwhile."""
     prev = torch.cuda.current_stream(device)
     own = lanes > 1 or graphs  # graphs are captured on a stream of their own
     streams = [torch.cuda.Stream(device) for _ in range(lanes)] if own else [prev]
@@ -919,14 +1045,15 @@ def run_lanes(task, count, lanes, device, solvers, graphs=False):
     for s in solvers:
         s._active_lanes = lanes
     active = [None] * lanes
-    nxt = 0
+    cursor = LaneCursor(count, lanes)
     try:
         while True:
             busy = False
             for lane in range(lanes):
-                if active[lane] is None and nxt < count:
-                    active[lane] = task(nxt, lane)
-                    nxt += 1
+                if active[lane] is None:
+                    i = cursor.take(lane)
+                    if i is not None:
+                        active[lane] = task(i, lane)
                 if active[lane] is None:
                     continue
                 busy = True
@@ -964,6 +1091,8 @@ def _sweep_many(jobs, lanes, workers, keep_currents=False, graphs=False):
         out.append(o)
         points += [(ji, fi) for fi in range(nf)]
     device = jobs[0][0].device
+    for s, _ in jobs:
+        s._anchor = {}  # no factorisation carries over from an earlier call
     torch.cuda.synchronize(device)
     t1 = time.time()
     pool = fit_pool(workers)
@@ -977,7 +1106,7 @@ def _sweep_many(jobs, lanes, workers, keep_currents=False, graphs=False):
         o = out[ji]
         s = o["solver"]
         g = o["tabs"][fi]
-        x_dev = yield from s._solve_task(g, o["B"], lane, graphs)
+        x_dev = yield from s._solve_task(g, o["B"], lane, graphs, reuse=True)
         yield  # let the other lanes enqueue their work before waiting for this one
         x = x_dev.cpu().numpy()
         s._check_solve(lane)

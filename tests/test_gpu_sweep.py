@@ -77,6 +77,29 @@ def test_lanes_match_single_lane(precision):
     assert np.max(np.abs(r1.S - r3.S)) < 1e-7
 
 
+def test_lu_reuse_matches_fresh_factorisations():
+    """Mixed precision with each lane's complex64 LU reused as the
+    preconditioner of its next frequency points: the FP64 GMRES reaches the
+    same tolerance, so the currents agree with a factorisation per point to
+    round-off of the refinement and S to the wave fit's noise; the policy did
+    reuse factorisations (and took fewer than one per point)."""
+    lay, sub, mk, freqs = W.tee()
+    m = lay.mesh(**mk)
+    f = freqs[:12]
+    fresh = Solver(m, sub, verbose=False, precision="mixed", lanes=2, reuse_lu=False)
+    r0 = fresh.sweep(f, keep_currents=True)
+    s = Solver(m, sub, verbose=False, precision="mixed", lanes=2)
+    r1 = s.sweep(f, keep_currents=True)
+    st = s.lu_stats
+    print("reuse stats", st, "fresh", fresh.lu_stats)
+    assert fresh.lu_stats["factorisations"] == len(f) and fresh.lu_stats["reused"] == 0
+    assert st["reused"] > 0 and st["factorisations"] + st["reused"] - st["fallbacks"] == len(f)
+    assert st["factorisations"] < len(f)
+    for a, b in zip(r0.currents, r1.currents):
+        assert np.max(np.abs(a - b)) <= 1e-9 * np.max(np.abs(a))
+    assert np.max(np.abs(r0.S - r1.S)) < 1e-7
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_graph_replays_match_direct_calls(precision):
     """Each lane's factorisation replayed from its HIP graph (captured on the
@@ -85,8 +108,10 @@ def test_graph_replays_match_direct_calls(precision):
     lay, sub, mk, freqs = W.tee()
     m = lay.mesh(**mk)
     f = freqs[:9]
-    a = Solver(m, sub, verbose=False, precision=precision, lanes=2, graphs=False).sweep(f, keep_currents=True)
-    s = Solver(m, sub, verbose=False, precision=precision, lanes=2, graphs=True)
+    # a factorisation per point (no LU reuse), so that every lane replays graphs
+    a = Solver(m, sub, verbose=False, precision=precision, lanes=2, graphs=False,
+               reuse_lu=False).sweep(f, keep_currents=True)
+    s = Solver(m, sub, verbose=False, precision=precision, lanes=2, graphs=True, reuse_lu=False)
     b = s.sweep(f, keep_currents=True)
     assert len(s._graphs) == 2
     for x, y in zip(a.currents, b.currents):

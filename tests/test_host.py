@@ -199,3 +199,31 @@ def test_fit_pool_does_not_rerun_unguarded_scripts(tmp_path):
         "print('sum', p.submit(np.add, 1, 2).result() + p.submit(np.add, 3, 4).result(), flush=True)\n")
     out = subprocess.run([sys.executable, str(script)], capture_output=True, text=True, timeout=300).stdout
     assert out.count("MAIN RUN") == 1 and "sum 10" in out
+
+
+@pytest.mark.parametrize("count,lanes", [(0, 2), (1, 3), (10, 1), (51, 2), (64, 3), (7, 4), (100, 5)])
+def test_lane_cursor_hands_out_runs_of_neighbours(count, lanes):
+    """Every index once; a lane's indices are consecutive (up or down)
+    except where it splits another lane's run, so neighbouring frequencies
+    share a lane."""
+    import random
+
+    from mom25d_amd.solver import LaneCursor
+
+    c = LaneCursor(count, lanes)
+    rng = random.Random(count * 31 + lanes)
+    seen, per_lane, active = [], {k: [] for k in range(lanes)}, list(range(lanes))
+    while active:
+        lane = rng.choice(active)
+        i = c.take(lane)
+        if i is None:
+            active.remove(lane)
+            continue
+        seen.append(i)
+        per_lane[lane].append(i)
+    assert sorted(seen) == list(range(count))
+    # a lane jumps only when it splits another run (each split halves one)
+    jumps = sum(int(abs(b - a) != 1) for idx in per_lane.values() for a, b in zip(idx, idx[1:]))
+    assert jumps <= lanes * max(1, int(np.ceil(np.log2(count + 2))))
+    if lanes <= 2:
+        assert jumps <= 1
