@@ -754,34 +754,64 @@ __global__ __launch_bounds__(256) void trsv_kernel(int n, const cplx<R>* __restr
   }
 }
 
-// R = B - Z X (FP64), one wavefront per row; when rowsum is given it also
-// receives d_r sum_k (|Re z_rk| + |Im z_rk|) d_k (d = 1 when dsc is null),
-// the row sums of the diagonally scaled matrix (for its norm).
-__global__ __launch_bounds__(256) void residual_kernel(int n, int nrhs, const z64* __restrict__ Z, int64_t ldz,
+// R = B - Z X (FP64); when rowsum is given it also receives
+// d_r sum_k (|Re z_rk| + |Im z_rk|) d_k (d = 1 when dsc is null), the row
+// sums of the diagonally scaled matrix (for its norm).  HBM-bound on the one
+// pass over Z: each wavefront takes 4 rows, so one load of x_k serves 4 rows,
+// and keeps 4 independent 16-byte streams of Z in flight (non-temporal: Z
+// is read once, x stays in the caches).
+template <int NR, bool ROWSUM>
+__global__ __launch_bounds__(256) void residual_kernel(int n, const z64* __restrict__ Z, int64_t ldz,
                                                        const z64* __restrict__ X, const z64* __restrict__ B,
                                                        z64* __restrict__ Rout, double* __restrict__ rowsum,
                                                        const double* __restrict__ dsc) {
+  constexpr int RW = 4;
   const int lane = threadIdx.x & 63;
-  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (r >= n) return;
-  z64 s[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-  double a = 0.0;
+  const int r0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * RW;
+  if (r0 >= n) return;
+  typedef double dv2 __attribute__((ext_vector_type(2)));
+  const dv2* zr[RW];
+#pragma unroll
+  for (int q = 0; q < RW; ++q) zr[q] = reinterpret_cast<const dv2*>(Z + (int64_t)min(r0 + q, n - 1) * ldz);
+  z64 s[RW][NR];
+  double a[RW];
+#pragma unroll
+  for (int q = 0; q < RW; ++q) {
+    a[q] = 0.0;
+#pragma unroll
+    for (int c = 0; c < NR; ++c) s[q][c] = {0, 0};
+  }
+#pragma unroll 2
   for (int k = lane; k < n; k += 64) {
-    z64 zz = Z[(int64_t)r * ldz + k];
-    if (rowsum) a += (fabs(zz.re) + fabs(zz.im)) * (dsc ? dsc[k] : 1.0);
-    for (int c = 0; c < nrhs && c < 4; ++c) s[c] += zz * X[(int64_t)k * nrhs + c];
-  }
-  for (int c = 0; c < nrhs && c < 4; ++c) {
-    double re = s[c].re, im = s[c].im;
-    for (int off = 32; off > 0; off >>= 1) {
-      re += __shfl_down(re, off, 64);
-      im += __shfl_down(im, off, 64);
+    z64 x[NR];
+#pragma unroll
+    for (int c = 0; c < NR; ++c) x[c] = X[(int64_t)k * NR + c];
+    const double dk = (ROWSUM && dsc) ? dsc[k] : 1.0;
+#pragma unroll
+    for (int q = 0; q < RW; ++q) {
+      const dv2 v = __builtin_nontemporal_load(zr[q] + k);
+      const z64 zz = {v.x, v.y};
+      if (ROWSUM) a[q] += (fabs(zz.re) + fabs(zz.im)) * dk;
+#pragma unroll
+      for (int c = 0; c < NR; ++c) s[q][c] += zz * x[c];
     }
-    if (lane == 0) Rout[(int64_t)r * nrhs + c] = B[(int64_t)r * nrhs + c] - z64{re, im};
   }
-  if (rowsum) {
-    for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
-    if (lane == 0) rowsum[r] = a * (dsc ? dsc[r] : 1.0);
+#pragma unroll
+  for (int q = 0; q < RW; ++q) {
+#pragma unroll
+    for (int c = 0; c < NR; ++c) {
+      double re = s[q][c].re, im = s[q][c].im;
+      for (int off = 32; off > 0; off >>= 1) {
+        re += __shfl_down(re, off, 64);
+        im += __shfl_down(im, off, 64);
+      }
+      if (lane == 0 && r0 + q < n) Rout[(int64_t)(r0 + q) * NR + c] = B[(int64_t)(r0 + q) * NR + c] - z64{re, im};
+    }
+    if (ROWSUM) {
+      double t = a[q];
+      for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+      if (lane == 0 && r0 + q < n) rowsum[r0 + q] = t * (dsc ? dsc[r0 + q] : 1.0);
+    }
   }
 }
 
@@ -1327,9 +1357,27 @@ extern "C" int mom_cgemm_sub(int prec, int m, int n, int k, const void* A, int64
 extern "C" int mom_residual(int n, int nrhs, const void* Z, int64_t ldz, const void* X, const void* B, void* R,
                             double* rowsum, const double* dsc, void* stream) {
   if (n <= 0) return 0;
-  if (nrhs > 4) return fail("mom_residual: at most 4 right-hand sides");
-  hipLaunchKernelGGL(residual_kernel, dim3((n + 3) / 4), dim3(256), 0, (hipStream_t)stream, n, nrhs, (const z64*)Z,
-                     ldz, (const z64*)X, (const z64*)B, (z64*)R, rowsum, dsc);
+  if (nrhs < 1 || nrhs > 4) return fail("mom_residual: 1 to 4 right-hand sides");
+  if ((reinterpret_cast<uintptr_t>(Z) | ((uintptr_t)ldz * sizeof(z64))) % 16)
+    return fail("mom_residual: Z rows must be 16-byte aligned");
+  const dim3 grid((unsigned)((n + 15) / 16)), block(256);  // 4 rows per wavefront
+  hipStream_t s = (hipStream_t)stream;
+  const z64 *z = (const z64*)Z, *x = (const z64*)X, *b = (const z64*)B;
+  z64* r = (z64*)R;
+#define MOM_RES(NR)                                                                                     \
+  case NR:                                                                                              \
+    if (rowsum)                                                                                         \
+      hipLaunchKernelGGL((residual_kernel<NR, true>), grid, block, 0, s, n, z, ldz, x, b, r, rowsum, dsc); \
+    else                                                                                                \
+      hipLaunchKernelGGL((residual_kernel<NR, false>), grid, block, 0, s, n, z, ldz, x, b, r, rowsum, dsc); \
+    break;
+  switch (nrhs) {
+    MOM_RES(1)
+    MOM_RES(2)
+    MOM_RES(3)
+    MOM_RES(4)
+  }
+#undef MOM_RES
   return check_launch("residual_kernel");
 }
 

@@ -246,3 +246,26 @@ def test_getrs_with_inverted_blocks(prec, n, nrhs):
         _lib.call("mom_getrs", prec, n, nrhs, P(Ad), n, P(perm), P(Bd), P(Xs), P(sw2), s)
         ref = Xs.cpu().numpy()
         assert np.max(np.abs(got - ref)) < (1e-5 if prec == 32 else 1e-12) * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("n,nrhs,rowsum", [(1, 1, False), (301, 3, True), (1000, 2, True), (517, 4, False),
+                                            (64, 1, True)])
+def test_residual_matches_numpy(n, nrhs, rowsum):
+    """R = B - Z X in FP64 (the refinement residual), and the row sums
+    d_r sum_k |z_rk|_1 d_k of the diagonally scaled matrix."""
+    rng = np.random.default_rng(n * 7 + nrhs)
+    Z = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    X = rng.standard_normal((n, nrhs)) + 1j * rng.standard_normal((n, nrhs))
+    B = rng.standard_normal((n, nrhs)) + 1j * rng.standard_normal((n, nrhs))
+    d = rng.uniform(0.5, 2.0, n)
+    Zd, Xd, Bd = (torch.from_numpy(a).to(DEV) for a in (Z, X, B))
+    R = torch.empty_like(Bd)
+    rs = torch.empty(n, dtype=torch.float64, device=DEV) if rowsum else None
+    dd = torch.from_numpy(d).to(DEV) if rowsum else None
+    _lib.call("mom_residual", n, nrhs, _lib.ptr(Zd), n, _lib.ptr(Xd), _lib.ptr(Bd), _lib.ptr(R), _lib.ptr(rs),
+              _lib.ptr(dd), _lib.stream_ptr())
+    ref = B - Z @ X
+    assert np.max(np.abs(R.cpu().numpy() - ref)) < 1e-13 * np.sqrt(n) * np.max(np.abs(Z)) * np.max(np.abs(X))
+    if rowsum:
+        rref = d * ((np.abs(Z.real) + np.abs(Z.imag)) @ d)
+        np.testing.assert_allclose(rs.cpu().numpy(), rref, rtol=1e-13)
