@@ -142,6 +142,18 @@ int64_t mom_getrs_inv_work_bytes(int prec, int n);
 int mom_trtri_blocks(int prec, int n, const void* LU, int64_t lda, void* work, void* stream);
 int mom_getrs_inv(int prec, int n, int nrhs, const void* LU, int64_t lda, const int32_t* perm, const void* B,
                   void* X, void* work, void* stream);
+/* mom_getrs_inv with complex128 B and X (the LU in prec): B is rounded to
+ * prec on the way in and the solution widened on the way out, inside the
+ * solve's own kernels (the GMRES preconditioner step). */
+int mom_getrs_inv_z(int prec, int n, int nrhs, const void* LU, int64_t lda, const int32_t* perm, const void* B,
+                    void* X, void* work, void* stream);
+
+/* W = D Z D X (complex128, D = diag(dsc), nrhs <= 4): the product of the
+ * diagonally scaled system in the GMRES refinement, one HBM pass over Z.  No
+ * counterpart in the reference, which solves in FP64 directly
+ * (Engine/mom25d/solver.py:11-31). */
+int mom_scaled_matvec(int n, int nrhs, const void* Z, int64_t ldz, const double* dsc, const void* X, void* W,
+                      void* stream);
 
 /* C (m x n) -= A (m x k) B (k x n), complex, on the matrix cores (the LU's
  * trailing update, exposed for testing and tuning). */

@@ -43,6 +43,8 @@ SIGNATURES = {
     "mom_getrs_inv_work_bytes": (_i64, [_i, _i]),
     "mom_trtri_blocks": (_i, [_i, _i, _p, _i64, _p, _p]),
     "mom_getrs_inv": (_i, [_i, _i, _i, _p, _i64, _p, _p, _p, _p, _p]),
+    "mom_getrs_inv_z": (_i, [_i, _i, _i, _p, _i64, _p, _p, _p, _p, _p]),
+    "mom_scaled_matvec": (_i, [_i, _i, _p, _i64, _p, _p, _p, _p]),
     "mom_cgemm_sub": (_i, [_i, _i, _i, _i, _p, _i64, _p, _i64, _p, _i64, _p]),
     "mom_residual": (_i, [_i, _i, _p, _i64, _p, _p, _p, _p, _p, _p]),
     "mom_cvt_z_to_c": (_i, [_i64, _p, _p, _p]),

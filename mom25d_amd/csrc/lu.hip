@@ -556,14 +556,6 @@ __global__ void build_perm_kernel(int n, const int32_t* ipiv, int32_t* perm) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) perm[i] = P[i];
 }
 
-template <typename R>
-__global__ void gather_rows_kernel(int n, int nrhs, const int32_t* perm, const cplx<R>* src, cplx<R>* dst) {
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n * nrhs) return;
-  int i = q / nrhs, c = q % nrhs;
-  dst[q] = src[(int64_t)perm[i] * nrhs + c];
-}
-
 // Triangular solves as a chain of 64-row blocks: block k needs x_0 .. x_{k-1}
 // (lower; the mirror for upper).  Workgroup k, given block k by ticket, first
 // copies the triangle of its diagonal block into LDS (packed by columns, the
@@ -598,21 +590,33 @@ __device__ __forceinline__ float lane_value(float v, int k) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
 }
 
-template <typename R>
-__global__ void fill_sentinel_kernel(cplx<R>* x, int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) x[i] = {Sentinel<R>::value(), Sentinel<R>::value()};
+// Set-up of a solve in one launch: the 5 status words cleared, X = P B
+// (rows gathered by perm, n x nrhs, rounded to R) and Y (the lower solve's
+// output) filled with the sentinel.
+template <typename R, typename TB>
+__global__ void getrs_prep_kernel(int n, int nrhs, const int32_t* perm, const cplx<TB>* B, cplx<R>* X, cplx<R>* Y,
+                                  unsigned* flags) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < 5) flags[q] = 0u;
+  if (q >= n * nrhs) return;
+  const int i = q / nrhs, c = q % nrhs;
+  const cplx<TB> b = B[(int64_t)perm[i] * nrhs + c];
+  X[q] = {(R)b.re, (R)b.im};
+  Y[q] = {Sentinel<R>::value(), Sentinel<R>::value()};
 }
 
 // Solves with the right-hand sides B (n x NR) into Xout (n x NR, filled with
 // the sentinel); *ticket hands out the blocks, *error is set to 1 when a
 // bounded wait ran out.  With INV the diagonal blocks are applied as the
 // precomputed inverses Dinv (64 x 64 row-major per block, trtri_blocks_kernel)
-// by a product spread over the workgroup instead of the substitution.
+// by a product spread over the workgroup instead of the substitution.  When
+// B_reset is B, each block sets the rows of B it has read to the sentinel
+// (B then serves as the next solve's output; no other block reads them).
+// With out64 the solution is also written there in complex128.
 template <typename R, bool LOWER, int NR, bool INV>
 __global__ __launch_bounds__(256) void trsv_kernel(int n, const cplx<R>* __restrict__ A, int64_t lda,
-                                                   const cplx<R>* __restrict__ B, cplx<R>* Xout,
-                                                   const cplx<R>* __restrict__ Dinv, unsigned* ticket,
+                                                   const cplx<R>* B, cplx<R>* Xout, const cplx<R>* __restrict__ Dinv,
+                                                   cplx<R>* B_reset, z64* __restrict__ out64, unsigned* ticket,
                                                    unsigned* error, unsigned spin_limit) {
   constexpr int BS = kTrsvBS;
   constexpr int TRI = LOWER ? BS * (BS - 1) / 2 : BS * (BS + 1) / 2;
@@ -701,6 +705,8 @@ __global__ __launch_bounds__(256) void trsv_kernel(int n, const cplx<R>* __restr
                        : cplx<R>{0, 0};
     }
     __syncthreads();
+    if (B_reset && tid < BS * NR && tid / NR < b)
+      B_reset[(int64_t)(k0 + tid / NR) * NR + tid % NR] = {Sentinel<R>::value(), Sentinel<R>::value()};
 #pragma unroll
     for (int c = 0; c < NR; ++c) acc[c] = {0, 0};
 #pragma unroll
@@ -712,8 +718,11 @@ __global__ __launch_bounds__(256) void trsv_kernel(int n, const cplx<R>* __restr
     __syncthreads();
     if (tid < BS * NR) {
       const int i = tid / NR, c = tid % NR;
-      if (i < b)
-        store_coherent_c(Xout + (int64_t)(k0 + i) * NR + c, part[0][i][c] + part[1][i][c] + part[2][i][c] + part[3][i][c]);
+      if (i < b) {
+        const cplx<R> x = part[0][i][c] + part[1][i][c] + part[2][i][c] + part[3][i][c];
+        store_coherent_c(Xout + (int64_t)(k0 + i) * NR + c, x);
+        if (out64) out64[(int64_t)(k0 + i) * NR + c] = {(double)x.re, (double)x.im};
+      }
     }
     return;
   }
@@ -726,6 +735,11 @@ __global__ __launch_bounds__(256) void trsv_kernel(int n, const cplx<R>* __restr
   for (int c = 0; c < NR; ++c)
     v[c] = i < b ? B[(int64_t)(k0 + i) * NR + c] - part[0][i][c] - part[1][i][c] - part[2][i][c] - part[3][i][c]
                  : cplx<R>{0, 0};
+  if (B_reset && i < b) {
+#pragma unroll
+    for (int c = 0; c < NR; ++c)
+      B_reset[(int64_t)(k0 + i) * NR + c] = {Sentinel<R>::value(), Sentinel<R>::value()};
+  }
   if (LOWER) {
 #pragma unroll
     for (int k = 0; k < BS - 1; ++k) {
@@ -750,7 +764,10 @@ __global__ __launch_bounds__(256) void trsv_kernel(int n, const cplx<R>* __restr
   }
   if (i < b) {
 #pragma unroll
-    for (int c = 0; c < NR; ++c) store_coherent_c(Xout + (int64_t)(k0 + i) * NR + c, v[c]);
+    for (int c = 0; c < NR; ++c) {
+      store_coherent_c(Xout + (int64_t)(k0 + i) * NR + c, v[c]);
+      if (out64) out64[(int64_t)(k0 + i) * NR + c] = {(double)v[c].re, (double)v[c].im};
+    }
   }
 }
 
@@ -759,8 +776,9 @@ __global__ __launch_bounds__(256) void trsv_kernel(int n, const cplx<R>* __restr
 // sums of the diagonally scaled matrix (for its norm).  HBM-bound on the one
 // pass over Z: each wavefront takes 4 rows, so one load of x_k serves 4 rows,
 // and keeps 4 independent 16-byte streams of Z in flight (non-temporal: Z
-// is read once, x stays in the caches).
-template <int NR, bool ROWSUM>
+// is read once, x stays in the caches).  SCALED: Rout = D Z D X instead
+// (D = diag(dsc); B and rowsum unused), the GMRES product of the scaled system.
+template <int NR, bool ROWSUM, bool SCALED = false>
 __global__ __launch_bounds__(256) void residual_kernel(int n, const z64* __restrict__ Z, int64_t ldz,
                                                        const z64* __restrict__ X, const z64* __restrict__ B,
                                                        z64* __restrict__ Rout, double* __restrict__ rowsum,
@@ -784,9 +802,12 @@ __global__ __launch_bounds__(256) void residual_kernel(int n, const z64* __restr
 #pragma unroll 2
   for (int k = lane; k < n; k += 64) {
     z64 x[NR];
+    const double dk = ((ROWSUM || SCALED) && dsc) ? dsc[k] : 1.0;
 #pragma unroll
-    for (int c = 0; c < NR; ++c) x[c] = X[(int64_t)k * NR + c];
-    const double dk = (ROWSUM && dsc) ? dsc[k] : 1.0;
+    for (int c = 0; c < NR; ++c) {
+      x[c] = X[(int64_t)k * NR + c];
+      if (SCALED) x[c] = {x[c].re * dk, x[c].im * dk};
+    }
 #pragma unroll
     for (int q = 0; q < RW; ++q) {
       const dv2 v = __builtin_nontemporal_load(zr[q] + k);
@@ -805,7 +826,12 @@ __global__ __launch_bounds__(256) void residual_kernel(int n, const z64* __restr
         re += __shfl_down(re, off, 64);
         im += __shfl_down(im, off, 64);
       }
-      if (lane == 0 && r0 + q < n) Rout[(int64_t)(r0 + q) * NR + c] = B[(int64_t)(r0 + q) * NR + c] - z64{re, im};
+      if (lane == 0 && r0 + q < n) {
+        if (SCALED)
+          Rout[(int64_t)(r0 + q) * NR + c] = {dsc[r0 + q] * re, dsc[r0 + q] * im};
+        else
+          Rout[(int64_t)(r0 + q) * NR + c] = B[(int64_t)(r0 + q) * NR + c] - z64{re, im};
+      }
     }
     if (ROWSUM) {
       double t = a[q];
@@ -1177,10 +1203,11 @@ __global__ __launch_bounds__(64) void trtri_blocks_kernel(int n, const cplx<R>* 
 }
 
 // Workspace of getrs: the block tickets of the two solves (words 0 and 2),
-// the failure flag (word 4), then the lower solve's output (n x 4 complex).
+// the failure flag (word 4), then the lower solve's output and the
+// precision-R copy of the solution (n x 4 complex each).
 template <typename R>
 static size_t getrs_work_bytes(int n) {
-  return 256 + sizeof(cplx<R>) * 4 * (size_t)std::max(n, 1);
+  return 256 + 2 * sizeof(cplx<R>) * 4 * (size_t)std::max(n, 1);
 }
 
 // getrs_inv adds the inverted diagonal blocks of L and U after that.
@@ -1207,13 +1234,17 @@ static int trtri_blocks(int n, const cplx<R>* LU, int64_t lda, void* work, hipSt
   return check_launch("trtri_blocks_kernel");
 }
 
-template <typename R, bool INV>
-static int getrs(int n, int nrhs, const cplx<R>* LU, int64_t lda, const int32_t* perm, const cplx<R>* B, cplx<R>* X,
-                 unsigned* flags, hipStream_t s) {
+// X = (LU)^-1 P B.  B in precision TB; the solution goes to X (precision R)
+// or, when X64 is given, to X64 in complex128 (X is then the workspace's
+// copy).
+template <typename R, bool INV, typename TB>
+static int getrs(int n, int nrhs, const cplx<R>* LU, int64_t lda, const int32_t* perm, const cplx<TB>* B, cplx<R>* X,
+                 z64* X64, unsigned* flags, hipStream_t s) {
   if (n <= 0 || nrhs <= 0) return 0;
   if (nrhs > 4) return fail("mom_getrs: at most 4 right-hand sides");
   const int nblk = (n + kTrsvBS - 1) / kTrsvBS;
   cplx<R>* Y = reinterpret_cast<cplx<R>*>(reinterpret_cast<unsigned char*>(flags) + 256);
+  if (X64) X = Y + 4 * (int64_t)std::max(n, 1);
   const int64_t nx = (int64_t)n * nrhs;
   const dim3 fg((unsigned)((nx + 255) / 256));
   const cplx<R>* DL = INV ? reinterpret_cast<const cplx<R>*>(reinterpret_cast<const unsigned char*>(flags) +
@@ -1221,21 +1252,16 @@ static int getrs(int n, int nrhs, const cplx<R>* LU, int64_t lda, const int32_t*
                           : nullptr;
   const cplx<R>* DU = INV ? DL + (int64_t)nblk * kTrsvBS * kTrsvBS : nullptr;
   int rc;
-  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, flags, 5);
-  if ((rc = check_launch("zero_u32_kernel"))) return rc;
-  hipLaunchKernelGGL(gather_rows_kernel<R>, fg, dim3(256), 0, s, n, nrhs, perm, B, X);
-  if ((rc = check_launch("gather_rows_kernel"))) return rc;
-  hipLaunchKernelGGL(fill_sentinel_kernel<R>, fg, dim3(256), 0, s, Y, nx);
-  if ((rc = check_launch("fill_sentinel_kernel"))) return rc;
+  hipLaunchKernelGGL((getrs_prep_kernel<R, TB>), fg, dim3(256), 0, s, n, nrhs, perm, B, X, Y, flags);
+  if ((rc = check_launch("getrs_prep_kernel"))) return rc;
+  // lower solve P B -> Y (clearing X for the upper solve), upper Y -> X
 #define MOM_TRSV(NR)                                                                                                  \
   case NR:                                                                                                            \
-    hipLaunchKernelGGL((trsv_kernel<R, true, NR, INV>), dim3(nblk), dim3(256), 0, s, n, LU, lda, X, Y, DL, flags,    \
-                       flags + 4, g_spin_limit);                                                                      \
+    hipLaunchKernelGGL((trsv_kernel<R, true, NR, INV>), dim3(nblk), dim3(256), 0, s, n, LU, lda, X, Y, DL, X,        \
+                       (z64*)nullptr, flags, flags + 4, g_spin_limit);                                                \
     if ((rc = check_launch("trsv_kernel"))) return rc;                                                                \
-    hipLaunchKernelGGL(fill_sentinel_kernel<R>, fg, dim3(256), 0, s, X, nx);                                          \
-    if ((rc = check_launch("fill_sentinel_kernel"))) return rc;                                                       \
     hipLaunchKernelGGL((trsv_kernel<R, false, NR, INV>), dim3(nblk), dim3(256), 0, s, n, LU, lda, Y, X, DU,          \
-                       flags + 2, flags + 4, g_spin_limit);                                                           \
+                       (cplx<R>*)nullptr, X64, flags + 2, flags + 4, g_spin_limit);                                   \
     break;
   switch (nrhs) {
     MOM_TRSV(1)
@@ -1284,8 +1310,10 @@ extern "C" int mom_getrs(int prec, int n, int nrhs, const void* LU, int64_t lda,
   if (B == X) return fail("mom_getrs: B and X must not alias");
   hipStream_t s = (hipStream_t)stream;
   unsigned* flags = reinterpret_cast<unsigned*>(work);
-  if (prec == 64) return getrs<double, false>(n, nrhs, (const z64*)LU, lda, perm, (const z64*)B, (z64*)X, flags, s);
-  if (prec == 32) return getrs<float, false>(n, nrhs, (const c32*)LU, lda, perm, (const c32*)B, (c32*)X, flags, s);
+  if (prec == 64)
+    return getrs<double, false, double>(n, nrhs, (const z64*)LU, lda, perm, (const z64*)B, (z64*)X, nullptr, flags, s);
+  if (prec == 32)
+    return getrs<float, false, float>(n, nrhs, (const c32*)LU, lda, perm, (const c32*)B, (c32*)X, nullptr, flags, s);
   return fail("mom_getrs: prec must be 64 or 32");
 }
 
@@ -1309,9 +1337,24 @@ extern "C" int mom_getrs_inv(int prec, int n, int nrhs, const void* LU, int64_t 
   if (B == X) return fail("mom_getrs_inv: B and X must not alias");
   hipStream_t s = (hipStream_t)stream;
   unsigned* flags = reinterpret_cast<unsigned*>(work);
-  if (prec == 64) return getrs<double, true>(n, nrhs, (const z64*)LU, lda, perm, (const z64*)B, (z64*)X, flags, s);
-  if (prec == 32) return getrs<float, true>(n, nrhs, (const c32*)LU, lda, perm, (const c32*)B, (c32*)X, flags, s);
+  if (prec == 64)
+    return getrs<double, true, double>(n, nrhs, (const z64*)LU, lda, perm, (const z64*)B, (z64*)X, nullptr, flags, s);
+  if (prec == 32)
+    return getrs<float, true, float>(n, nrhs, (const c32*)LU, lda, perm, (const c32*)B, (c32*)X, nullptr, flags, s);
   return fail("mom_getrs_inv: prec must be 64 or 32");
+}
+
+extern "C" int mom_getrs_inv_z(int prec, int n, int nrhs, const void* LU, int64_t lda, const int32_t* perm,
+                               const void* B, void* X, void* work, void* stream) {
+  if (!LU || !perm || !B || !X || !work) return fail("mom_getrs_inv_z: null device pointer");
+  if (B == X) return fail("mom_getrs_inv_z: B and X must not alias");
+  hipStream_t s = (hipStream_t)stream;
+  unsigned* flags = reinterpret_cast<unsigned*>(work);
+  if (prec == 64)
+    return getrs<double, true, double>(n, nrhs, (const z64*)LU, lda, perm, (const z64*)B, nullptr, (z64*)X, flags, s);
+  if (prec == 32)
+    return getrs<float, true, double>(n, nrhs, (const c32*)LU, lda, perm, (const z64*)B, nullptr, (z64*)X, flags, s);
+  return fail("mom_getrs_inv_z: prec must be 64 or 32");
 }
 
 extern "C" int mom_set_spin_limit(unsigned polls) {
@@ -1378,6 +1421,32 @@ extern "C" int mom_residual(int n, int nrhs, const void* Z, int64_t ldz, const v
     MOM_RES(4)
   }
 #undef MOM_RES
+  return check_launch("residual_kernel");
+}
+
+extern "C" int mom_scaled_matvec(int n, int nrhs, const void* Z, int64_t ldz, const double* dsc, const void* X,
+                                 void* W, void* stream) {
+  if (n <= 0) return 0;
+  if (nrhs < 1 || nrhs > 4) return fail("mom_scaled_matvec: 1 to 4 right-hand sides");
+  if (!Z || !dsc || !X || !W) return fail("mom_scaled_matvec: null device pointer");
+  if ((reinterpret_cast<uintptr_t>(Z) | ((uintptr_t)ldz * sizeof(z64))) % 16)
+    return fail("mom_scaled_matvec: Z rows must be 16-byte aligned");
+  const dim3 grid((unsigned)((n + 15) / 16)), block(256);
+  hipStream_t s = (hipStream_t)stream;
+  const z64 *z = (const z64*)Z, *x = (const z64*)X;
+  z64* w = (z64*)W;
+#define MOM_SMV(NR)                                                                                      \
+  case NR:                                                                                               \
+    hipLaunchKernelGGL((residual_kernel<NR, false, true>), grid, block, 0, s, n, z, ldz, x, nullptr, w, nullptr, \
+                       dsc);                                                                             \
+    break;
+  switch (nrhs) {
+    MOM_SMV(1)
+    MOM_SMV(2)
+    MOM_SMV(3)
+    MOM_SMV(4)
+  }
+#undef MOM_SMV
   return check_launch("residual_kernel");
 }
 

@@ -730,20 +730,18 @@ class Solver:
         dv = self._bufs[("dscale", lane)]
         d = dv[:, None]
         m = self.gmres_restart
-        zeros = torch.zeros_like(B)
         rows = self._buf("rowsum", (n,), torch.float64, lane)
         Bs = d * B
 
-        def precond(V):
-            W32 = V.to(torch.complex64)
-            Y32 = torch.empty_like(W32)
-            _lib.call("mom_getrs_inv", 32, n, nrhs, P(A32), A32.stride(0), P(perm), P(W32), P(Y32), P(sw), stream)
-            return Y32.to(torch.complex128)
+        def precond(V, out=None):  # complex64 LU solve of complex128 vectors
+            out = torch.empty_like(V) if out is None else out
+            _lib.call("mom_getrs_inv_z", 32, n, nrhs, P(A32), A32.stride(0), P(perm), P(V), P(out), P(sw), stream)
+            return out
 
         def matvec(Y):  # (D Z D) Y
-            R = torch.empty_like(B)
-            _lib.call("mom_residual", n, nrhs, P(Z), Z.stride(0), P(d * Y), P(zeros), P(R), None, None, stream)
-            return -(d * R)
+            W = torch.empty_like(B)
+            _lib.call("mom_scaled_matvec", n, nrhs, P(Z), Z.stride(0), P(dv), P(Y), P(W), stream)
+            return W
 
         def residual(Y, rowsum=None):  # D (b - Z D Y)
             R = torch.empty_like(B)
@@ -776,7 +774,7 @@ class Solver:
             sn = np.zeros((m, nrhs), complex)
             k = 0
             for j in range(m):
-                Wb[j] = precond(Vb[j])
+                precond(Vb[j], Wb[j])
                 w = matvec(Wb[j])
                 hj = torch.zeros((j + 2, nrhs), dtype=torch.complex128, device=self.device)
                 h = torch.empty((j + 1, nrhs), dtype=torch.complex128, device=self.device)

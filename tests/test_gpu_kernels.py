@@ -269,3 +269,38 @@ def test_residual_matches_numpy(n, nrhs, rowsum):
     if rowsum:
         rref = d * ((np.abs(Z.real) + np.abs(Z.imag)) @ d)
         np.testing.assert_allclose(rs.cpu().numpy(), rref, rtol=1e-13)
+
+
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (333, 2), (1500, 4)])
+def test_getrs_inv_z_and_scaled_matvec(n, nrhs):
+    """The GMRES building blocks: the complex64 LU solve taking and giving
+    complex128 vectors equals the complex64 solve of the rounded vector,
+    widened; and W = D Z D X equals the numpy product."""
+    rng = np.random.default_rng(n + 17)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + 4 * np.sqrt(n) * np.eye(n)
+    Ad = torch.from_numpy(A).to(DEV, torch.complex64)
+    ipiv = torch.empty(n, dtype=torch.int32, device=DEV)
+    perm = torch.empty(n, dtype=torch.int32, device=DEV)
+    info = torch.zeros(4, dtype=torch.int32, device=DEV)
+    wb = int(_lib.lib().mom_getrf_work_bytes(32, n, 64))
+    work = torch.empty(wb, dtype=torch.uint8, device=DEV)
+    sw = torch.empty(int(_lib.lib().mom_getrs_inv_work_bytes(32, n)), dtype=torch.uint8, device=DEV)
+    s = _lib.stream_ptr()
+    P = _lib.ptr
+    _lib.call("mom_getrf", 32, n, P(Ad), n, P(ipiv), P(info), P(work), wb, 64, 0, s)
+    _lib.call("mom_build_perm", n, P(ipiv), P(perm), s)
+    _lib.call("mom_trtri_blocks", 32, n, P(Ad), n, P(sw), s)
+    B = rng.standard_normal((n, nrhs)) + 1j * rng.standard_normal((n, nrhs))
+    B64 = torch.from_numpy(B).to(DEV)
+    X64 = torch.empty_like(B64)
+    _lib.call("mom_getrs_inv_z", 32, n, nrhs, P(Ad), n, P(perm), P(B64), P(X64), P(sw), s)
+    B32 = B64.to(torch.complex64)
+    X32 = torch.empty_like(B32)
+    _lib.call("mom_getrs_inv", 32, n, nrhs, P(Ad), n, P(perm), P(B32), P(X32), P(sw), s)
+    np.testing.assert_array_equal(X64.cpu().numpy(), X32.cpu().numpy().astype(complex))
+    d = rng.uniform(0.5, 2.0, n)
+    Zd = torch.from_numpy(A).to(DEV)
+    W = torch.empty_like(B64)
+    _lib.call("mom_scaled_matvec", n, nrhs, P(Zd), n, P(torch.from_numpy(d).to(DEV)), P(B64), P(W), s)
+    ref = d[:, None] * (A @ (d[:, None] * B))
+    assert np.max(np.abs(W.cpu().numpy() - ref)) < 1e-13 * np.sqrt(n) * np.max(np.abs(ref))
