@@ -198,6 +198,7 @@ def main():
     sm, rm, tm = time_path(mesh, sub, freqs, prec, args.warmup, args.steps, torch, args.lanes, not args.no_tune,
                            args.panel, tile, args.block)
     lanes = sm._lane_count()
+    sweep_lu_stats = dict(sm.lu_stats)  # the timed sweep's (later solves add to sm.lu_stats)
     print(f"{prec}: {args.steps} points in {tm:.3f} s (LU: {sm.lu_stats})", file=sys.stderr, flush=True)
     other = None
     if not args.no_alt:
@@ -226,7 +227,7 @@ def main():
                            frac_fp64_valu=fill_flops / tf / PEAK["fp64_valu"]))
     extra["lu_config"] = dict(panel=sm.panel, block=sm.block or sm.panel, tile=[bm, bn],
                               autotuned=not args.no_tune and tile is None)
-    extra["lu_stats"] = dict(sm.lu_stats)  # factorisations / reused LUs / GMRES steps in the timed sweep
+    extra["lu_stats"] = sweep_lu_stats  # factorisations / reused LUs / GMRES steps in the timed sweep
     if not args.no_batch:
         extra["batch_random16"] = batch_random16(torch, prec)
         b = extra["batch_random16"]
