@@ -56,11 +56,11 @@ def time_path(mesh, sub, freqs, precision, warmup, steps, torch, lanes=None, aut
         s.block = block
     s.geometry().upload(s.device)
     if warmup:
-        s.sweep(freqs[len(freqs) // 2 - warmup // 2:][:warmup])
+        s.sweep(freqs[len(freqs) // 2 - warmup // 2:][:warmup], interp=None)
     s.lu_stats = {k: 0 for k in s.lu_stats}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    r = s.sweep(sweep_points(freqs, steps))
+    r = s.sweep(sweep_points(freqs, steps), interp=None)  # exact fill at every point
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     return s, r, t1 - t0

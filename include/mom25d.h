@@ -155,6 +155,17 @@ int mom_getrs_inv_z(int prec, int n, int nrhs, const void* LU, int64_t lda, cons
 int mom_scaled_matvec(int n, int nrhs, const void* Z, int64_t ldz, const double* dsc, const void* X, void* W,
                       void* stream);
 
+/* Frequency interpolation of Z (Engine/mom25d/freqinterp.py:86-139).
+ * mom_interp_anchor: W = j w Z exp(+j ke R) (N x N, W stored in prec_w = 64 or 32
+ * bits per real part), R the distance of the basis-function centres pos
+ * (N x 3, FP64), 0 on the via-via block (indices >= nb).
+ * mom_interp_combine: Z = (sum_k L[k] W[k]) exp(-j ke R) / (j w) from K <= 8
+ * anchors; W and L are host arrays of device pointers and weights. */
+int mom_interp_anchor(int prec_w, int N, int nb, const void* Z, int64_t ldz, const double* pos, double omega,
+                      double ke, void* W, int64_t ldw, void* stream);
+int mom_interp_combine(int prec_w, int N, int nb, int K, const void* const* W, const double* L, int64_t ldw,
+                       const double* pos, double omega, double ke, void* Z, int64_t ldz, void* stream);
+
 /* C (m x n) -= A (m x k) B (k x n), complex, on the matrix cores (the LU's
  * trailing update, exposed for testing and tuning). */
 int mom_cgemm_sub(int prec, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb, void* C,

@@ -817,31 +817,44 @@ class Solver:
         self.lu_stats["gmres_steps"] += its
         return d * Y
 
-    def solve_currents(self, freq, excitations=None, greens=None):
+    def solve_currents(self, freq, excitations=None, greens=None, interp=None):
+        """RWG (+ via segment) coefficients for the given excitations (default:
+        one column per port); interp: a FreqInterp (set up) supplying Z
+        (Engine/mom25d/solver.py:984-1031)."""
         g = greens or self.greens(freq)
         if excitations is None:
             excitations = self.port_excitations()
-        return self._solve_dev(g, excitations).cpu().numpy()
+        return self._solve_dev(g, excitations, interp).cpu().numpy()
 
-    def _solve_dev(self, g, excitations):
+    def _solve_dev(self, g, excitations, interp=None):
         B = torch.from_numpy(np.ascontiguousarray(excitations, complex)).to(self.device)
-        X = self._run(self._solve_task(g, B, 0)).clone()
+        X = self._run(self._solve_task(g, B, 0, interp=interp)).clone()
         self._check_solve(0)
         return X
 
-    def _solve_task(self, g, B, lane, graph=False, reuse=False):
+    def _solve_task(self, g, B, lane, graph=False, reuse=False, interp=None):
         """Generator: fill, LU and port solve of one frequency on the current
         stream with the buffers of `lane`; returns the device currents (a
         buffer of the lane: read it before the lane's next solve).  reuse
         (mixed precision, within a sweep): precondition with the lane's last
-        LU when the reuse policy says so (reuse_lu)."""
+        LU when the reuse policy says so (reuse_lu).  interp: a FreqInterp
+        (set up for this solver) supplies Z instead of the fill."""
         n = self.mesh.n_unknowns
         Z = self._buf("Z", (n, n), torch.complex128, lane)
         mixed = self.precision == "mixed"
         Z32 = self._buf("Z32", (n, n), torch.complex64, lane) if mixed else None
+
+        def fill(Z32x):
+            if interp is None:
+                self._fill(g, Z, Z32x, lane)
+                return
+            interp.fill(g.freq, Z, lane)
+            if Z32x is not None:
+                self._scale_copy(Z, Z32x, lane)
+
         a = self._anchor.get(lane) if mixed and reuse and self.reuse_lu else None
         if a is not None and self._reuse_next(a, g.freq):
-            self._fill(g, Z, None, lane)
+            fill(None)
             cap = max(20, int(a["lu_equiv"] + a["its0"]))
             X = yield from self._gmres_ir(Z, Z32, self._bufs[("perm", lane)], self._bufs[("swork32", lane)], B, lane,
                                           max_its=cap)
@@ -854,7 +867,7 @@ class Solver:
             self.lu_stats["fallbacks"] += 1
             self._scale_copy(Z, Z32, lane)
         else:
-            self._fill(g, Z, Z32, lane)
+            fill(Z32)
         X = yield from self._lu_solve(Z, B, Z32, lane, graph)
         if mixed:
             self.lu_stats["factorisations"] += 1
@@ -922,6 +935,19 @@ class Solver:
                   P(tabs.tab_b), P(qd), 1.0 / EPS0, P(out), _lib.stream_ptr())
         return out.cpu().numpy()
 
+    def _basis_positions(self):
+        """Centres of the basis functions (N x 3): RWG edge midpoints on their
+        level, via segments at mid span (Engine/mom25d/freqinterp.py:53-60)."""
+        m = self.mesh
+        pos = np.c_[m.edge_mid[:m.n_basis], self.zl[m.edge_layer[:m.n_basis]]]
+        if m.n_vias:
+            vz = {n: self.stack.level_z(n) for n in m.layer_names}
+            vz["gnd"] = 0.0
+            zc = [0.5 * (vz[g["bottom"]] + vz[g["top"]]) for g in m.via_segments]
+            xy = [m.vias[g["via"]].xy for g in m.via_segments]
+            pos = np.r_[pos, np.c_[np.asarray(xy, float).reshape(-1, 2), zc]]
+        return np.ascontiguousarray(pos, float)
+
     def _fit_context(self):
         """The port description the wave fits need (picklable)."""
         groups = list(self.groups.values())
@@ -959,15 +985,36 @@ class Solver:
     def _schedule(self, task, count, lanes):
         run_lanes(task, count, lanes, self.device, [self], self.graphs)
 
-    def sweep(self, freqs, keep_currents=False, interp=None, values=None, checkpoint=None, wait_memory=None):
-        """S-parameters at freqs (every frequency filled and factored; the
-        reference's interpolated-Z mode is not part of this build, so
-        interp="auto" behaves like None)."""
-        if interp not in (None, "auto"):
-            raise ValueError("interp must be None or 'auto' (no frequency interpolation in this build)")
+    def sweep(self, freqs, keep_currents=False, interp="auto", values=None, checkpoint=None, wait_memory=None):
+        """S-parameters at freqs (Engine/mom25d/solver.py:1189-1335).
+
+        interp : "auto" (default) -> interpolate Z between anchor fills
+                 (FreqInterp with its defaults) when the sweep has at least
+                 twice as many frequencies as anchors, else fill every point;
+                 a FreqInterp -> use it; None -> exact fill at every point.
+        values, checkpoint, wait_memory : accepted for the reference's
+                 signature; lumped values and checkpoints are not part of
+                 this build (values must be empty)."""
+        if values:
+            raise ValueError("lumped component values are not supported by this build")
         freqs = np.atleast_1d(np.asarray(freqs, float))
+        if isinstance(interp, str):
+            if interp != "auto":
+                raise ValueError("interp must be 'auto', None or a FreqInterp")
+            from .freqinterp import FreqInterp
+
+            interp = FreqInterp().setup(self, freqs)
+            if len(freqs) < 2 * len(interp.anchors):
+                interp = None
+        elif interp is not None:
+            interp.setup(self, freqs)
+        if interp is not None and self.verbose:
+            print(f"  FreqInterp: {len(interp.anchors)} anchor fills for {len(freqs)} frequencies "
+                  f"(order {interp.order}, k_e = {interp.k_ratio:.3f} k0)")
+        self.last_interp = interp
         lanes = min(self._lane_count(), len(freqs))
-        return _sweep_many([(self, freqs)], lanes, self._worker_count(len(freqs)), keep_currents, self.graphs)[0]
+        return _sweep_many([(self, freqs, interp)], lanes, self._worker_count(len(freqs)), keep_currents,
+                           self.graphs)[0]
 
     # ---------------------------------------------------------- reference Z0
     def _reciprocal_z0(self, SI, zphi, prior=1e-4):
@@ -1077,7 +1124,7 @@ def _sweep_many(jobs, lanes, workers, keep_currents=False, graphs=False):
     out = []
     points = []
     t0 = time.time()
-    for ji, (s, freqs) in enumerate(jobs):
+    for ji, (s, freqs, *rest) in enumerate(jobs):
         nf, n = len(freqs), len(s.ports)
         o = dict(solver=s, freqs=freqs, S=np.zeros((nf, n, n), complex), gam=np.zeros((nf, n), complex),
                  zl=np.zeros((nf, n), complex), zr=np.zeros((nf, n)), res=np.zeros((nf, n)),
@@ -1086,10 +1133,11 @@ def _sweep_many(jobs, lanes, workers, keep_currents=False, graphs=False):
         s.geometry().upload(s.device)
         o["B"] = torch.from_numpy(np.ascontiguousarray(s.port_excitations(), complex)).to(s.device)
         o["ctx"] = s._fit_context()
+        o["interp"] = rest[0] if rest else None
         out.append(o)
         points += [(ji, fi) for fi in range(nf)]
     device = jobs[0][0].device
-    for s, _ in jobs:
+    for s, *_ in jobs:
         s._anchor = {}  # no factorisation carries over from an earlier call
     torch.cuda.synchronize(device)
     t1 = time.time()
@@ -1104,7 +1152,7 @@ def _sweep_many(jobs, lanes, workers, keep_currents=False, graphs=False):
         o = out[ji]
         s = o["solver"]
         g = o["tabs"][fi]
-        x_dev = yield from s._solve_task(g, o["B"], lane, graphs, reuse=True)
+        x_dev = yield from s._solve_task(g, o["B"], lane, graphs, reuse=True, interp=o["interp"])
         yield  # let the other lanes enqueue their work before waiting for this one
         x = x_dev.cpu().numpy()
         s._check_solve(lane)

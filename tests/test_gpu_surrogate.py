@@ -67,12 +67,12 @@ def test_surrogate_tables_gain_and_cost(via2):
           f"{rep['seconds']}; check error median {np.median(rep['check_err']):.1e} max {rep['check_err'].max():.1e}")
     assert not rep["fallback"] and ratio < 0.1
     f = freqs[::10]
-    ref = Solver(mesh, st, verbose=False).sweep(f)
-    net = Solver(mesh, st, verbose=False, surrogate=sur, surrogate_tol=np.inf).sweep(f)
+    ref = Solver(mesh, st, verbose=False).sweep(f, interp=None)
+    net = Solver(mesh, st, verbose=False, surrogate=sur, surrogate_tol=np.inf).sweep(f, interp=None)
     dS = np.abs(ref.S - net.S).max(axis=(1, 2))
     print("|dS| networks vs integrals per frequency:", " ".join(f"{x:.1e}" for x in dS))
     assert dS.max() < 1e-2
     judged = Solver(mesh, st, verbose=False, surrogate=sur, surrogate_tol=1e-5)
-    r = judged.sweep(f)
+    r = judged.sweep(f, interp=None)
     assert len(judged.last_tables.surrogate_report["fallback"]) == len(f)
     assert np.abs(r.S - ref.S).max() < 1e-8

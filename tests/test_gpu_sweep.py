@@ -23,7 +23,7 @@ pytestmark = pytest.mark.gpu
 def test_sweep_matches_reference(name, precision, tol, golden):
     g = golden("sweeps.npz")
     lay, sub, mk, _ = getattr(W, name)()
-    r = Solver(lay.mesh(**mk), sub, verbose=False, precision=precision).sweep(g[f"{name}_freqs"])
+    r = Solver(lay.mesh(**mk), sub, verbose=False, precision=precision).sweep(g[f"{name}_freqs"], interp=None)
     dS = np.max(np.abs(r.S - g[f"{name}_S"]))
     print(f"{name} {precision} |dS| = {dS:.2e}")
     assert dS < tol
@@ -56,7 +56,7 @@ def test_via_transition_sweep_matches_oracle():
     lay, st, mk, _ = W.via2()
     m = lay.mesh(**mk)
     f = [3e9, 11e9]
-    r = Solver(m, st, verbose=False).sweep(f)
+    r = Solver(m, st, verbose=False).sweep(f, interp=None)
     o = OracleSolver(m, st, use_c=True).sweep(f)
     assert np.max(np.abs(r.S - o["S"])) < 1e-6
 
@@ -70,8 +70,8 @@ def test_lanes_match_single_lane(precision):
     lay, sub, mk, freqs = W.tee()
     m = lay.mesh(**mk)
     f = freqs[:7]
-    r1 = Solver(m, sub, verbose=False, precision=precision, lanes=1).sweep(f, keep_currents=True)
-    r3 = Solver(m, sub, verbose=False, precision=precision, lanes=3).sweep(f, keep_currents=True)
+    r1 = Solver(m, sub, verbose=False, precision=precision, lanes=1).sweep(f, keep_currents=True, interp=None)
+    r3 = Solver(m, sub, verbose=False, precision=precision, lanes=3).sweep(f, keep_currents=True, interp=None)
     for a, b in zip(r1.currents, r3.currents):
         assert np.max(np.abs(a - b)) <= 1e-9 * np.max(np.abs(a))
     assert np.max(np.abs(r1.S - r3.S)) < 1e-7
@@ -87,9 +87,9 @@ def test_lu_reuse_matches_fresh_factorisations():
     m = lay.mesh(**mk)
     f = freqs[:12]
     fresh = Solver(m, sub, verbose=False, precision="mixed", lanes=2, reuse_lu=False)
-    r0 = fresh.sweep(f, keep_currents=True)
+    r0 = fresh.sweep(f, keep_currents=True, interp=None)
     s = Solver(m, sub, verbose=False, precision="mixed", lanes=2)
-    r1 = s.sweep(f, keep_currents=True)
+    r1 = s.sweep(f, keep_currents=True, interp=None)
     st = s.lu_stats
     print("reuse stats", st, "fresh", fresh.lu_stats)
     assert fresh.lu_stats["factorisations"] == len(f) and fresh.lu_stats["reused"] == 0
@@ -110,9 +110,9 @@ def test_graph_replays_match_direct_calls(precision):
     f = freqs[:9]
     # a factorisation per point (no LU reuse), so that every lane replays graphs
     a = Solver(m, sub, verbose=False, precision=precision, lanes=2, graphs=False,
-               reuse_lu=False).sweep(f, keep_currents=True)
+               reuse_lu=False).sweep(f, keep_currents=True, interp=None)
     s = Solver(m, sub, verbose=False, precision=precision, lanes=2, graphs=True, reuse_lu=False)
-    b = s.sweep(f, keep_currents=True)
+    b = s.sweep(f, keep_currents=True, interp=None)
     assert len(s._graphs) == 2
     for x, y in zip(a.currents, b.currents):
         assert np.max(np.abs(x - y)) <= 1e-9 * np.max(np.abs(x))
@@ -130,6 +130,6 @@ def test_sweep_batch_matches_separate_sweeps():
     problems = [(lay.mesh(**mk), st, freqs[::16]) for lay, st, mk, freqs in probs]
     batch = sweep_batch(problems, precision="mixed", lanes=3, workers=2)
     for (m, st, f), r in zip(problems, batch):
-        ref = Solver(m, st, verbose=False, precision="mixed", lanes=1, workers=0, graphs=False).sweep(f)
+        ref = Solver(m, st, verbose=False, precision="mixed", lanes=1, workers=0, graphs=False).sweep(f, interp=None)
         print(f"N={m.n_unknowns}: |dS| batch vs separate {np.max(np.abs(r.S - ref.S)):.1e}")
         assert np.max(np.abs(r.S - ref.S)) < 1e-7
