@@ -46,7 +46,7 @@ def _cpu_model():
 
 
 def time_path(mesh, sub, freqs, precision, warmup, steps, torch, lanes=None, autotune=True, panel=None, tile=None,
-              block=None):
+              block=None, interp=None):
     from mom25d_amd import Solver
 
     s = Solver(mesh, sub, verbose=False, precision=precision, lanes=lanes, autotune=autotune and tile is None,
@@ -60,7 +60,7 @@ def time_path(mesh, sub, freqs, precision, warmup, steps, torch, lanes=None, aut
     s.lu_stats = {k: 0 for k in s.lu_stats}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    r = s.sweep(sweep_points(freqs, steps), interp=None)  # exact fill at every point
+    r = s.sweep(sweep_points(freqs, steps), interp=interp)  # None: exact fill at every point
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     return s, r, t1 - t0
@@ -173,6 +173,8 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="end with a plain 1 GiB device copy (reference for the HBM byte counters)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched sweep of the 16 random layouts")
+    ap.add_argument("--no-interp", action="store_true",
+                    help="skip timing the headline path with the interpolated impedance (interp='auto')")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -209,6 +211,19 @@ def main():
               flush=True)
         del sa
 
+    interp_line = None
+    if not args.no_interp and args.steps == len(freqs):
+        si, rI, tI = time_path(mesh, sub, freqs, prec, args.warmup, args.steps, torch, args.lanes, not args.no_tune,
+                               args.panel, tile, args.block, interp="auto")
+        fi = si.last_interp
+        interp_line = dict(value=args.steps / tI, ms_per_step=1e3 * tI / args.steps,
+                           anchors=None if fi is None else len(fi.anchors),
+                           dS_max_vs_headline=float(np.max(np.abs(rI.S - rm.S))),
+                           dS_max_vs_fp64=None if other is None else float(np.max(np.abs(rI.S - ra.S))))
+        print(f"{prec} with interp='auto': {args.steps} points in {tI:.3f} s; {interp_line}", file=sys.stderr,
+              flush=True)
+        del si
+
     f_probe = freqs[len(freqs) // 2]
     calls, secs, flops = gemm_roofline(sm, f_probe, torch)
     achieved = flops / secs if secs > 0 else 0.0
@@ -228,6 +243,8 @@ def main():
     extra["lu_config"] = dict(panel=sm.panel, block=sm.block or sm.panel, tile=[bm, bn],
                               autotuned=not args.no_tune and tile is None)
     extra["lu_stats"] = sweep_lu_stats  # factorisations / reused LUs / GMRES steps in the timed sweep
+    if interp_line is not None:
+        extra["interp_auto"] = interp_line  # the reference's default sweep mode (interpolated Z between anchors)
     if not args.no_batch:
         extra["batch_random16"] = batch_random16(torch, prec)
         b = extra["batch_random16"]
