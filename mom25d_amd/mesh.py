@@ -4,12 +4,15 @@ This is the input side of the per-frequency solve: it must produce exactly
 the mesh the reference builds from the same `Layout` calls, because the
 impedance matrix is compared entry by entry.  The behaviour follows
 Engine/mom25d/mesh.py (Port :62-188, Via :191-241, Layout :309-693,
-_grid_lines :696-703, Mesh :706-1116); pours, polygons, thick metal,
-apertures, lumped gaps and triangular meshing are not part of this build.
+_grid_lines :696-703, Mesh :706-1116); lumped gaps (add_lumped,
+add_component) are axis-aligned bridges as in the reference's grid meshes;
+pours, polygons, thick metal, apertures, internal S-parameter ports and
+triangular meshing are not part of this build.
 
 Conventions kept from the reference because the numbers depend on them:
-  * grid lines: every rectangle edge and every port's reference / gap
-    coordinate, optional refinement lines +-edge_cell, then uniform
+  * grid lines: every rectangle edge, every port's reference / gap
+    coordinate and every lumped bridge's two ends and gap, optional
+    refinement lines +-edge_cell, then uniform
     subdivision of each interval into ceil(len / max_cell) parts;
   * a cell is copper if its centre lies strictly inside a rectangle or a
     via pad of the layer; it is split into two triangles along a diagonal
@@ -140,6 +143,7 @@ class Layout:
     rect_layers: list = field(default_factory=list)
     planes: list = field(default_factory=list)
     lumped: list = field(default_factory=list)
+    components: list = field(default_factory=list)
     internal_ports: list = field(default_factory=list)
 
     def add_rect(self, x0, y0, x1, y1, layer="top"):
@@ -152,6 +156,52 @@ class Layout:
         self.ports.append(p)
         self.add_rect(*p.rect(), layer=p.layer)
         return p
+
+    def add_lumped(self, name, a, b, width, value=None, layer="top"):
+        """Lumped component from pad point a to pad point b (x, y) [m]: a
+        copper bridge of `width` with a delta gap in its middle, loaded with
+        `value` (lumped.R / L / C / series / parallel, a number [ohm], a
+        callable f -> Z, a string such as 'C 1p' or 'R 10 + L 0.5n'; None: an
+        open gap, e.g. one port of a multi-port component).  The voltage is
+        taken a -> b.  Returns the bridge's Port geometry
+        (Engine/mom25d/mesh.py:353-378; bridges along x or y only)."""
+        from .lumped import as_value, parse
+
+        a, b = np.asarray(a, float), np.asarray(b, float)
+        d = b - a
+        length = float(np.hypot(*d))
+        if length <= 0:
+            raise ValueError("lumped component: a and b coincide")
+        if any(q.name == name for q in self.lumped) or any(q.name == name for q in self.ports):
+            raise ValueError(f"name {name!r} already used")
+        if abs(d[1]) <= 1e-12 * length:
+            direction = "+x" if d[0] > 0 else "-x"
+        elif abs(d[0]) <= 1e-12 * length:
+            direction = "+y" if d[1] > 0 else "-y"
+        else:
+            raise ValueError(f"lumped {name}: oblique bridges need a triangular mesh (not part of this build)")
+        g = Port(name, tuple(a), direction, width, length, gap_from_end=0.5 * length, layer=layer)
+        g.value = None if value is None else (parse(value) if isinstance(value, str) else as_value(value))
+        self.lumped.append(g)
+        self.add_rect(*g.rect(), layer=layer)
+        return g
+
+    def add_component(self, model, gaps):
+        """Multi-port component (lumped.Touchstone or a .sNp path) across the
+        named lumped gaps, in the model's port order (add the gaps with
+        value=None) (Engine/mom25d/mesh.py:433-446)."""
+        from .lumped import Touchstone
+
+        if isinstance(model, str):
+            model = Touchstone(model)
+        names = [q.name for q in self.lumped]
+        for n in gaps:
+            if n not in names:
+                raise ValueError(f"no lumped gap {n!r}")
+        if getattr(model, "n", len(gaps)) != len(gaps):
+            raise ValueError(f"{model} has {model.n} ports, {len(gaps)} gaps given")
+        self.components.append((model, list(gaps)))
+        return self
 
     def add_via(self, x, y, drill=0.3e-3, pad=None, name=None, top="top", bottom="gnd", connect=(),
                 antipad=None):
@@ -238,6 +288,8 @@ class Mesh:
         fy = [c for r in layout.rects for c in (r[1], r[3])]
         for p in layout.ports:
             (fx if p.axis == 0 else fy).extend([p.coord(0), p.coord(p.s_gap)])
+        for p in getattr(layout, "lumped", []):  # a lumped gap is a grid line, and so are the bridge's ends
+            (fx if p.axis == 0 else fy).extend([p.coord(0), p.coord(p.s_gap), p.coord(p.length)])
         rx, ry = [], []
         if edge_cell:
             ax, ay = np.array(fx), np.array(fy)
@@ -401,6 +453,19 @@ class Mesh:
         idx = np.nonzero(on)[0]
         sign = np.sign(self.flow_direction()[idx] @ -port.u)
         return idx, sign
+
+    def lumped_vectors(self):
+        """(N_unknowns x n_lumped) signed edge lengths l of each lumped gap's
+        cut (the delta-gap excitation; the gap current a -> b is l . x)
+        (Engine/mom25d/mesh.py:1077-1087)."""
+        lps = list(getattr(self.layout, "lumped", []))
+        Lm = np.zeros((self.n_unknowns, len(lps)))
+        for k, p in enumerate(lps):
+            idx, sgn = self.cut_edges(p, p.s_gap)
+            if idx.size == 0:
+                raise ValueError(f"lumped {p.name}: no mesh edges on its gap")
+            Lm[idx, k] = -sgn * self.edge_len[idx]
+        return Lm
 
     def cut_positions(self, port):
         g = self.xg if port.axis == 0 else self.yg

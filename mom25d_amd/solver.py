@@ -389,8 +389,13 @@ class Solver:
         self.ports = mesh.layout.ports
         if not self.ports:
             raise ValueError("layout has no ports")
-        if getattr(mesh.layout, "lumped", None) or getattr(mesh.layout, "internal_ports", None):
-            raise ValueError("lumped components / internal ports are not supported by this build")
+        if getattr(mesh.layout, "internal_ports", None):
+            raise ValueError("internal S-parameter ports are not supported by this build")
+        # lumped components: delta gaps loaded after the solve (Woodbury), Engine/mom25d/solver.py:853-858
+        self.lumped = list(getattr(mesh.layout, "lumped", []))
+        self.components = list(getattr(mesh.layout, "components", []))
+        self.Lm = mesh.lumped_vectors() if self.lumped else None
+        self.values = {}
         self.pm = [PortModel(mesh, p, st.level_h(p.layer)) for p in self.ports]
         self.method = "dense"
         self.precision = precision
@@ -660,11 +665,19 @@ class Solver:
             _lib.call("mom_getrf", prec, n, P(A), A.stride(0), P(ipiv), P(info), P(work), wbytes, jb, groups, stream)
             _lib.call("mom_build_perm", n, P(ipiv), P(perm), stream)
             if Z32 is None:
-                _lib.call("mom_getrs", 64, n, nrhs, P(A), A.stride(0), P(perm), P(B), P(X), P(sw), stream)
+                for c0, c1 in _column_chunks(nrhs):  # the solves take up to 4 right-hand sides
+                    if nrhs <= 4:
+                        _lib.call("mom_getrs", 64, n, nrhs, P(A), A.stride(0), P(perm), P(B), P(X), P(sw), stream)
+                    else:
+                        Bc = B[:, c0:c1].contiguous()
+                        Xc = torch.empty_like(Bc)
+                        _lib.call("mom_getrs", 64, n, c1 - c0, P(A), A.stride(0), P(perm), P(Bc), P(Xc), P(sw),
+                                  stream)
+                        X[:, c0:c1] = Xc
             else:
                 _lib.call("mom_trtri_blocks", 32, n, P(A), A.stride(0), P(sw), stream)
 
-        if graph:
+        if graph and nrhs <= 4:
             key = (lane, prec)
             sig = (stream, A.data_ptr(), B.data_ptr(), X.data_ptr(), work.data_ptr(), sw.data_ptr(), ipiv.data_ptr(),
                    groups, jb, n, nrhs)
@@ -690,7 +703,25 @@ class Solver:
         if Z32 is None:
             self.last_refine = 0
             return X
-        return (yield from self._gmres_ir(Z, A, perm, sw, B, lane))
+        return (yield from self._gmres_chunks(Z, A, perm, sw, B, lane))
+
+    def _gmres_chunks(self, Z, A32, perm, sw, B, lane=0, max_its=None):
+        """_gmres_ir over groups of at most 4 right-hand sides (the solve
+        kernels' limit); None if a group does not converge within max_its."""
+        nrhs = B.shape[1]
+        if nrhs <= 4:
+            return (yield from self._gmres_ir(Z, A32, perm, sw, B, lane, max_its))
+        X = torch.empty_like(B)
+        its = 0
+        for c0, c1 in _column_chunks(nrhs):
+            Xc = yield from self._gmres_ir(Z, A32, perm, sw, B[:, c0:c1].contiguous(), lane, max_its)
+            its += self.last_refine
+            if Xc is None:
+                self.last_refine = its
+                return None
+            X[:, c0:c1] = Xc
+        self.last_refine = its
+        return X
 
     def _check_solve(self, lane=0):
         """Status words of the lane's last factorisation and solves (read
@@ -817,14 +848,65 @@ class Solver:
         self.lu_stats["gmres_steps"] += its
         return d * Y
 
-    def solve_currents(self, freq, excitations=None, greens=None, interp=None):
+    def solve_currents(self, freq, excitations=None, greens=None, Z=None, interp=None):
         """RWG (+ via segment) coefficients for the given excitations (default:
-        one column per port); interp: a FreqInterp (set up) supplying Z
+        one column per port), with the lumped loads applied; Z: a precomputed
+        impedance matrix (numpy); interp: a FreqInterp (set up) supplying Z
         (Engine/mom25d/solver.py:984-1031)."""
         g = greens or self.greens(freq)
         if excitations is None:
             excitations = self.port_excitations()
-        return self._solve_dev(g, excitations, interp).cpu().numpy()
+        ne = excitations.shape[1]
+        if Z is not None:
+            interp = _GivenZ(Z)
+        X = self._solve_dev(g, self._with_gaps(excitations), interp).cpu().numpy()
+        return self._apply_lumped(X, ne, freq)
+
+    # ------------------------------------------------------------- lumped
+    def _with_gaps(self, excitations):
+        """The excitations followed by one delta-gap column per lumped gap."""
+        E = np.asarray(excitations)
+        return E if self.Lm is None else np.hstack([E, self.Lm])
+
+    def lumped_impedance(self, freq):
+        """D: (n_lumped x n_lumped) impedance of the components across the
+        lumped gaps at freq (open gaps: OPEN) (Engine/mom25d/solver.py:1039-1061)."""
+        from .lumped import as_value, parse
+
+        nl = 0 if self.Lm is None else self.Lm.shape[1]
+        D = np.zeros((nl, nl), complex)
+        names = [q.name for q in self.lumped]
+        used = set()
+        for model, gaps in self.components:
+            ix = [names.index(n) for n in gaps]
+            D[np.ix_(ix, ix)] = model.z(freq)
+            used.update(ix)
+        for k, q in enumerate(self.lumped):
+            v = self.values.get(q.name, q.value)
+            if k in used and v is None:
+                continue
+            if isinstance(v, str):
+                v = parse(v)
+            D[k, k] = self.OPEN if v is None else as_value(v).z(freq)
+        return D
+
+    def _load_lumped(self, Xe, Xi, freq):
+        """Solution with the components across the lumped gaps:
+        (Z + L D L^T)^-1 e = Xe - Xi D (I + L^T Xi D)^-1 L^T Xe (Woodbury), with
+        Xi = Z^-1 L (Engine/mom25d/solver.py:1063-1069)."""
+        D = self.lumped_impedance(freq)
+        M = self.Lm.T @ Xi
+        y = self.Lm.T @ Xe
+        return Xe - Xi @ (D @ np.linalg.solve(np.eye(len(D)) + M @ D, y))
+
+    def _apply_lumped(self, X, ne, freq):
+        """Loaded solution of the first ne columns from the solve of
+        [excitations, gap columns]."""
+        return X[:, :ne] if self.Lm is None else self._load_lumped(X[:, :ne], X[:, ne:], freq)
+
+    def lumped_currents(self, x):
+        """Current a -> b through each lumped gap, (n_lumped, Nexc)."""
+        return self.Lm.T @ x
 
     def _solve_dev(self, g, excitations, interp=None):
         B = torch.from_numpy(np.ascontiguousarray(excitations, complex)).to(self.device)
@@ -856,8 +938,8 @@ class Solver:
         if a is not None and self._reuse_next(a, g.freq):
             fill(None)
             cap = max(20, int(a["lu_equiv"] + a["its0"]))
-            X = yield from self._gmres_ir(Z, Z32, self._bufs[("perm", lane)], self._bufs[("swork32", lane)], B, lane,
-                                          max_its=cap)
+            X = yield from self._gmres_chunks(Z, Z32, self._bufs[("perm", lane)], self._bufs[("swork32", lane)], B,
+                                              lane, max_its=cap)
             if X is not None:
                 a["cost"] += self.last_refine
                 a["n"] += 1
@@ -992,11 +1074,10 @@ class Solver:
                  (FreqInterp with its defaults) when the sweep has at least
                  twice as many frequencies as anchors, else fill every point;
                  a FreqInterp -> use it; None -> exact fill at every point.
-        values, checkpoint, wait_memory : accepted for the reference's
-                 signature; lumped values and checkpoints are not part of
-                 this build (values must be empty)."""
-        if values:
-            raise ValueError("lumped component values are not supported by this build")
+        values : {lumped name: value} overriding the layout's components.
+        checkpoint, wait_memory : accepted for the reference's signature;
+                 checkpoints and memory waits are not part of this build."""
+        self.values = dict(values or {})
         freqs = np.atleast_1d(np.asarray(freqs, float))
         if isinstance(interp, str):
             if interp != "auto":
@@ -1020,6 +1101,22 @@ class Solver:
     def _reciprocal_z0(self, SI, zphi, prior=1e-4):
         """Port reference impedances (Engine/mom25d/solver.py:1357-1397)."""
         return reciprocal_z0(self._fit_context(), SI, zphi, prior)
+
+
+def _column_chunks(n, width=4):
+    """Column ranges of at most `width` columns covering 0 .. n-1."""
+    return [(c, min(c + width, n)) for c in range(0, n, width)]
+
+
+class _GivenZ:
+    """Stands in for a FreqInterp: supplies a given impedance matrix (numpy)."""
+
+    def __init__(self, Z):
+        self.Z = np.ascontiguousarray(Z, complex)
+
+    def fill(self, f, Zdev, lane=0):
+        Zdev.copy_(torch.from_numpy(self.Z))
+        return Zdev
 
 
 class LaneCursor:
@@ -1131,12 +1228,13 @@ def _sweep_many(jobs, lanes, workers, keep_currents=False, graphs=False):
                  waves=np.zeros((nf, n, n), complex), cur=[None] * nf, futures={}, host=0.0)
         o["tabs"] = s.tables(freqs)
         s.geometry().upload(s.device)
-        o["B"] = torch.from_numpy(np.ascontiguousarray(s.port_excitations(), complex)).to(s.device)
+        o["B"] = torch.from_numpy(np.ascontiguousarray(s._with_gaps(s.port_excitations()), complex)).to(s.device)
         o["ctx"] = s._fit_context()
         o["interp"] = rest[0] if rest else None
         out.append(o)
         points += [(ji, fi) for fi in range(nf)]
     device = jobs[0][0].device
+    n_exc = [len(o["solver"].ports) for o in out]
     for s, *_ in jobs:
         s._anchor = {}  # no factorisation carries over from an earlier call
     torch.cuda.synchronize(device)
@@ -1154,13 +1252,13 @@ def _sweep_many(jobs, lanes, workers, keep_currents=False, graphs=False):
         g = o["tabs"][fi]
         x_dev = yield from s._solve_task(g, o["B"], lane, graphs, reuse=True, interp=o["interp"])
         yield  # let the other lanes enqueue their work before waiting for this one
-        x = x_dev.cpu().numpy()
+        f = o["freqs"][fi]
+        x = s._apply_lumped(x_dev.cpu().numpy(), n_exc[ji], f)
         s._check_solve(lane)
         if keep_currents:
             o["cur"][fi] = x
         h0 = time.perf_counter()
         I, V = s._port_iv(g, x)
-        f = o["freqs"][fi]
         if pool is None:
             store(o, fi, sparams_point(o["ctx"], f, I, V))
         else:
