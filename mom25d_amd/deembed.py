@@ -75,14 +75,17 @@ class FitContext:
     z0_fixed: np.ndarray = None
     z0_recip_min: float = None
     extra: dict = field(default_factory=dict)
+    zi: list = field(default_factory=list)  # reference impedance of each internal port
 
 
-def sparams_point(ctx, f, I, V):
-    """S and the fit results of one frequency from the port currents and
-    voltages at the sample cuts (I[p], V[p]: (Ns_p, Nexc)), as in
-    Engine/mom25d/solver.py:1284-1325.  Returns (S, gamma, z0_line,
-    residual, z0_ref, A) per port."""
-    n = len(ctx.z0_from)
+def sparams_point(ctx, f, I, V, Ii=None):
+    """S and the fit results of one frequency from the feed currents and
+    voltages at the sample cuts (I[p], V[p]: (Ns_p, Nexc)) and the currents
+    through the internal ports' sources (Ii: (n_internal, Nexc)), as in
+    Engine/mom25d/solver.py:1284-1325.  Feeds come first, then internal
+    ports.  Returns (S, gamma, z0_line, residual, z0_ref, A) per port."""
+    npt, ni = len(ctx.z0_from), len(ctx.zi)
+    n = npt + ni
     gam = np.zeros(n, complex)
     zl = np.zeros(n, complex)
     res = np.zeros(n)
@@ -104,10 +107,45 @@ def sparams_point(ctx, f, I, V):
             Acoef[p, e], Bcoef[p, e] = a, bb
         for p in members:
             gam[p], zl[p], res[p] = gm, z0, r
+    zr = np.zeros(n)
+    if ni:  # internal ports: V = Vs - z_ref I; a = Vs / 2 sqrt(z), b = (V - z I) / 2 sqrt(z), current-normalised
+        zi = np.asarray(ctx.zi, float)
+        Vs = np.zeros((ni, n))
+        Vs[:, npt:] = np.eye(ni)
+        Acoef[npt:] = Vs / (2 * zi[:, None])
+        Bcoef[npt:] = -(Vs - 2 * zi[:, None] * np.asarray(Ii)) / (2 * zi[:, None])
+        zl[npt:] = zi
+        zr[npt:] = zi
     SI = -Bcoef @ np.linalg.inv(Acoef)
-    zr = reciprocal_z0(ctx, SI, zl.real) if ctx.z0_fixed is None else np.asarray(ctx.z0_fixed, float)
+    if npt and ctx.z0_fixed is None:
+        zr[:npt] = reciprocal_z0(ctx, SI[:npt, :npt], zl[:npt].real)
+        if ni:
+            zr[:npt] = anchor_z0(ctx, SI, zr[:npt], ctx.zi)
+    elif npt:
+        zr[:npt] = np.asarray(ctx.z0_fixed, float)
     dd = np.sqrt(zr)
     return dd[:, None] * SI / dd[None, :], gam, zl, res, zr, Acoef
+
+
+def anchor_z0(ctx, SI, zfeed, zi, prior=1e-4):
+    """Feed reference impedances refined by reciprocity with the internal
+    ports, whose z_ref is exact: per feed group the coupling-weighted mean of
+    log(z_i |SI_ij / SI_ji|) over internal ports i and members j, with the
+    feed-only estimate as a weak prior (Engine/mom25d/solver.py:1337-1355)."""
+    npt = len(zfeed)
+    z = np.array(zfeed, float)
+    cmin = ctx.z0_recip_min
+    for members in ctx.groups:
+        num, den = prior * np.log(np.mean(z[members])), prior
+        for j in members:
+            for k, zk in enumerate(zi):
+                i = npt + k
+                w = min(abs(SI[i, j]), abs(SI[j, i])) ** 2
+                if w > 0 and (cmin is None or w >= cmin ** 2):
+                    num += w * np.log(zk * abs(SI[i, j] / SI[j, i]))
+                    den += w
+        z[members] = np.exp(num / den)
+    return z
 
 
 def reciprocal_z0(ctx, SI, zphi, prior=1e-4):

@@ -166,7 +166,12 @@ class FreqInterp:
         block (RWG, RWG-via, via-via); returns the largest
         (freqinterp.py:141-162)."""
         Zi = self.impedance(f) if Zi is None else Zi
-        Ze = self.solver.impedance(f)
+        s = self.solver
+        n = s.mesh.n_unknowns
+        Zd = torch.empty((n, n), dtype=torch.complex128, device=s.device)
+        s._fill(s.greens(f), Zd, None, lane="interp-check")  # buffers of its own: lanes may be mid-solve
+        Ze = Zd.cpu().numpy()
+        del Zd
         nb = self.nb
         blocks = {"RWG": (slice(0, nb), slice(0, nb)), "RWG-via": (slice(0, nb), slice(nb, None)),
                   "via-via": (slice(nb, None), slice(nb, None))}
@@ -187,6 +192,10 @@ def eps_eff(solver):
     """Quasi-static eps_eff under the first port's level, for the phase
     extraction only (freqinterp.py:199-209)."""
     st = solver.stack
-    layer = solver.ports[0].layer
+    if solver.ports:
+        layer = solver.ports[0].layer
+    else:  # internal ports only: the level of the first gap port, else the first mesh layer
+        gaps = {q.name: q.layer for q in solver.lumped}
+        layer = next((gaps[ip["name"]] for ip in solver.iports if ip["kind"] == "gap"), solver.mesh.layer_names[0])
     e = st.eps_below(layer)
     return e if st.covered(layer) else 0.5 * (e + 1)

@@ -6,8 +6,9 @@ impedance matrix is compared entry by entry.  The behaviour follows
 Engine/mom25d/mesh.py (Port :62-188, Via :191-241, Layout :309-693,
 _grid_lines :696-703, Mesh :706-1116); lumped gaps (add_lumped,
 add_component) are axis-aligned bridges as in the reference's grid meshes;
-pours, polygons, thick metal, apertures, internal S-parameter ports and
-triangular meshing are not part of this build.
+internal S-parameter ports (add_gap_port, add_via_port) as well;
+pours, polygons, thick metal, apertures and triangular meshing are not part
+of this build.
 
 Conventions kept from the reference because the numbers depend on them:
   * grid lines: every rectangle edge, every port's reference / gap
@@ -185,6 +186,28 @@ class Layout:
         self.lumped.append(g)
         self.add_rect(*g.rect(), layer=layer)
         return g
+
+    def add_gap_port(self, name, a, b, width, layer="top", z_ref=50.0):
+        """Internal S-parameter port (reference impedance z_ref [ohm]) across
+        a delta gap: a copper bridge from pad point a to pad point b like
+        add_lumped, voltage a -> b (Engine/mom25d/mesh.py:406-413)."""
+        g = self.add_lumped(name, a, b, width, None, layer)
+        self.internal_ports.append(dict(kind="gap", name=name, z_ref=float(z_ref)))
+        return g
+
+    def add_via_port(self, name, via, z_ref=50.0):
+        """Internal S-parameter port at the bottom of a via that ends on the
+        ground plane: a delta gap between the barrel and the plane, voltage
+        barrel -> ground, port current flowing down the via.  `via`: index into
+        Layout.vias or the Via (Engine/mom25d/mesh.py:415-431)."""
+        k = via if isinstance(via, (int, np.integer)) else next(i for i, v in enumerate(self.vias) if v is via)
+        if self.vias[k].bottom != "gnd":
+            raise ValueError(f"via port {name!r}: the via must end on the ground plane (bottom='gnd')")
+        used = [q.name for q in self.lumped] + [q.name for q in self.ports] + [q["name"] for q in self.internal_ports]
+        if name in used:
+            raise ValueError(f"name {name!r} already used")
+        self.internal_ports.append(dict(kind="via", name=name, via=int(k), z_ref=float(z_ref)))
+        return k
 
     def add_component(self, model, gaps):
         """Multi-port component (lumped.Touchstone or a .sNp path) across the
@@ -466,6 +489,25 @@ class Mesh:
                 raise ValueError(f"lumped {p.name}: no mesh edges on its gap")
             Lm[idx, k] = -sgn * self.edge_len[idx]
         return Lm
+
+    def internal_port_vectors(self):
+        """(N_unknowns x n_internal) delta-gap vectors of the internal ports:
+        a gap port's gap cut (as lumped_vectors, current a -> b), a via port's
+        bottom segment (current flowing down into the ground plane)
+        (Engine/mom25d/mesh.py:1058-1075)."""
+        ips = list(getattr(self.layout, "internal_ports", []))
+        names = [q.name for q in getattr(self.layout, "lumped", [])]
+        Lm = self.lumped_vectors() if names else None
+        out = np.zeros((self.n_unknowns, len(ips)))
+        for k, ip in enumerate(ips):
+            if ip["kind"] == "gap":
+                out[:, k] = Lm[:, names.index(ip["name"])]
+            else:
+                seg = [i for i, g in enumerate(self.via_segments) if g["via"] == ip["via"] and g["bottom"] == "gnd"]
+                if len(seg) != 1:
+                    raise ValueError(f"via port {ip['name']}: no via segment ending on the ground plane")
+                out[self.n_basis + seg[0], k] = 1.0
+        return out
 
     def cut_positions(self, port):
         g = self.xg if port.axis == 0 else self.yg

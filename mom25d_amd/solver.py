@@ -387,14 +387,25 @@ class Solver:
         if getattr(mesh, "level_order", None) != order:
             mesh.assign_levels(order)
         self.ports = mesh.layout.ports
-        if not self.ports:
+        self.iports = list(getattr(mesh.layout, "internal_ports", []))
+        if not self.ports and not self.iports:
             raise ValueError("layout has no ports")
-        if getattr(mesh.layout, "internal_ports", None):
-            raise ValueError("internal S-parameter ports are not supported by this build")
-        # lumped components: delta gaps loaded after the solve (Woodbury), Engine/mom25d/solver.py:853-858
+        # lumped components: delta gaps loaded after the solve (Woodbury); internal ports: S-parameter
+        # ports with a fixed reference impedance across a delta gap, via ports adding their own columns
+        # after the lumped gaps (Engine/mom25d/solver.py:848-868)
         self.lumped = list(getattr(mesh.layout, "lumped", []))
         self.components = list(getattr(mesh.layout, "components", []))
         self.Lm = mesh.lumped_vectors() if self.lumped else None
+        self.iport_index = []
+        if self.iports:
+            names = [q.name for q in self.lumped]
+            V = mesh.internal_port_vectors()
+            extra = [k for k, ip in enumerate(self.iports) if ip["kind"] == "via"]
+            if extra:
+                self.Lm = V[:, extra] if self.Lm is None else np.hstack([self.Lm, V[:, extra]])
+            col = {k: len(names) + i for i, k in enumerate(extra)}
+            self.iport_index = [names.index(ip["name"]) if ip["kind"] == "gap" else col[k]
+                                for k, ip in enumerate(self.iports)]
         self.values = {}
         self.pm = [PortModel(mesh, p, st.level_h(p.layer)) for p in self.ports]
         self.method = "dense"
@@ -627,7 +638,11 @@ class Solver:
 
     # -------------------------------------------------------------- solve
     def port_excitations(self):
-        return np.stack([pm.excitation for pm in self.pm], 1)
+        """One column per port: 1 V delta gap at each feed, then a 1 V source
+        (in series with z_ref) at each internal port (Engine/mom25d/solver.py:1029-1034)."""
+        cols = [pm.excitation for pm in self.pm]
+        cols += [self.Lm[:, c] for c in self.iport_index]
+        return np.stack(cols, 1)
 
     def _lu_solve(self, Z, B, Z32=None, lane=0, graph=False):
         """Generator: X = Z^-1 B on the device (the value it returns).  Z
@@ -877,13 +892,16 @@ class Solver:
         D = np.zeros((nl, nl), complex)
         names = [q.name for q in self.lumped]
         used = set()
+        for ip, c in zip(self.iports, self.iport_index):  # internal ports: terminated in z_ref
+            D[c, c] = ip["z_ref"]
+            used.add(c)
         for model, gaps in self.components:
             ix = [names.index(n) for n in gaps]
             D[np.ix_(ix, ix)] = model.z(freq)
             used.update(ix)
         for k, q in enumerate(self.lumped):
             v = self.values.get(q.name, q.value)
-            if k in used and v is None:
+            if k in used and (v is None or k in self.iport_index):
                 continue
             if isinstance(v, str):
                 v = parse(v)
@@ -1010,12 +1028,16 @@ class Solver:
         pd = torch.from_numpy(pts).to(self.device)
         tl = self._buf("tri_layer", (plan.nt,), torch.int32)
         tl.copy_(torch.from_numpy(m.tri_layer.astype(np.int32)))
-        out = torch.empty((pts.shape[0], q.shape[1]), dtype=torch.complex128, device=self.device)
         P = _lib.ptr
-        _lib.call("mom_line_voltage", pts.shape[0], P(pd), li, len(m.layer_names), plan.nt, q.shape[1], P(tl),
-                  P(d["V"]), P(d["qp7"]), P(d["qw7"]), P(greens.pairs_dev), P(tabs.tab_x), P(tabs.tab_cw),
-                  P(tabs.tab_b), P(qd), 1.0 / EPS0, P(out), _lib.stream_ptr())
-        return out.cpu().numpy()
+        cols = []
+        for c0, c1 in _column_chunks(q.shape[1]):  # the kernel takes up to 4 columns
+            qc = qd if q.shape[1] <= 4 else qd[:, c0:c1].contiguous()
+            out = torch.empty((pts.shape[0], c1 - c0), dtype=torch.complex128, device=self.device)
+            _lib.call("mom_line_voltage", pts.shape[0], P(pd), li, len(m.layer_names), plan.nt, c1 - c0, P(tl),
+                      P(d["V"]), P(d["qp7"]), P(d["qw7"]), P(greens.pairs_dev), P(tabs.tab_x), P(tabs.tab_cw),
+                      P(tabs.tab_b), P(qc), 1.0 / EPS0, P(out), _lib.stream_ptr())
+            cols.append(out.cpu().numpy())
+        return np.hstack(cols)
 
     def _basis_positions(self):
         """Centres of the basis functions (N x 3): RWG edge midpoints on their
@@ -1039,7 +1061,15 @@ class Solver:
                           eps=[st.eps_below(p.layer) for p in first], covered=[st.covered(p.layer) for p in first],
                           z0_from=[p.z0_from for p in self.ports], s=[pm.s for pm in self.pm],
                           lossless=bool(self.lossless), v_weight=self.v_weight, z0_fixed=self.z0_fixed,
-                          z0_recip_min=self.z0_recip_min)
+                          z0_recip_min=self.z0_recip_min, zi=[ip["z_ref"] for ip in self.iports])
+
+    def _internal_currents(self, x):
+        """Current through each internal port's source, (n_internal, Nexc)."""
+        return self.Lm[:, self.iport_index].T @ x if self.iports else None
+
+    @property
+    def port_names(self):
+        return [p.name for p in self.ports] + [ip["name"] for ip in self.iports]
 
     def _port_iv(self, g, x):
         """Currents and voltages at every port's sample cuts (the voltages
@@ -1062,7 +1092,8 @@ class Solver:
 
     def _sparams(self, f, g, x, S, gam, zl, zr, res, waves, fi):
         I, V = self._port_iv(g, x)
-        self._store_point(sparams_point(self._fit_context(), f, I, V), fi, S, gam, zl, zr, res, waves)
+        self._store_point(sparams_point(self._fit_context(), f, I, V, self._internal_currents(x)), fi, S, gam, zl, zr,
+                          res, waves)
 
     def _schedule(self, task, count, lanes):
         run_lanes(task, count, lanes, self.device, [self], self.graphs)
@@ -1222,7 +1253,7 @@ def _sweep_many(jobs, lanes, workers, keep_currents=False, graphs=False):
     points = []
     t0 = time.time()
     for ji, (s, freqs, *rest) in enumerate(jobs):
-        nf, n = len(freqs), len(s.ports)
+        nf, n = len(freqs), len(s.ports) + len(s.iports)
         o = dict(solver=s, freqs=freqs, S=np.zeros((nf, n, n), complex), gam=np.zeros((nf, n), complex),
                  zl=np.zeros((nf, n), complex), zr=np.zeros((nf, n)), res=np.zeros((nf, n)),
                  waves=np.zeros((nf, n, n), complex), cur=[None] * nf, futures={}, host=0.0)
@@ -1234,7 +1265,7 @@ def _sweep_many(jobs, lanes, workers, keep_currents=False, graphs=False):
         out.append(o)
         points += [(ji, fi) for fi in range(nf)]
     device = jobs[0][0].device
-    n_exc = [len(o["solver"].ports) for o in out]
+    n_exc = [len(o["solver"].ports) + len(o["solver"].iports) for o in out]
     for s, *_ in jobs:
         s._anchor = {}  # no factorisation carries over from an earlier call
     torch.cuda.synchronize(device)
@@ -1259,10 +1290,11 @@ def _sweep_many(jobs, lanes, workers, keep_currents=False, graphs=False):
             o["cur"][fi] = x
         h0 = time.perf_counter()
         I, V = s._port_iv(g, x)
+        Ii = s._internal_currents(x)
         if pool is None:
-            store(o, fi, sparams_point(o["ctx"], f, I, V))
+            store(o, fi, sparams_point(o["ctx"], f, I, V, Ii))
         else:
-            o["futures"][fi] = pool.submit(sparams_point, o["ctx"], f, I, V)
+            o["futures"][fi] = pool.submit(sparams_point, o["ctx"], f, I, V, Ii)
         o["host"] += time.perf_counter() - h0
         done[0] += 1
         if s.verbose:
@@ -1273,7 +1305,7 @@ def _sweep_many(jobs, lanes, workers, keep_currents=False, graphs=False):
         for fi, fut in o["futures"].items():
             store(o, fi, fut.result())
     t2 = time.time()
-    return [SweepResult(o["freqs"], o["S"], o["zr"], o["gam"], o["zl"], o["res"], [p.name for p in o["solver"].ports],
+    return [SweepResult(o["freqs"], o["S"], o["zr"], o["gam"], o["zl"], o["res"], o["solver"].port_names,
                         o["cur"] if keep_currents else None, o["waves"],
                         timing=dict(tables=t1 - t0, solve=t2 - t1, sparams=o["host"])) for o in out]
 
