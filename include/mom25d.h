@@ -139,6 +139,11 @@ int mom_getrs(int prec, int n, int nrhs, const void* LU, int64_t lda, const int3
  * Engine/mom25d/solver.py:23, in the mixed-precision path that the reference
  * does not have. */
 int64_t mom_getrs_inv_work_bytes(int prec, int n);
+/* Row blocks of the complex64 mom_getrs_inv(_z) chain: 64 (default) or 128
+ * (the 128-row inverses composed by mom_trtri_blocks from the 64-row ones;
+ * half as many blocks in the chain).  Process-wide. */
+int mom_set_solve_rows(int rows);
+int mom_get_solve_rows(void);
 int mom_trtri_blocks(int prec, int n, const void* LU, int64_t lda, void* work, void* stream);
 int mom_getrs_inv(int prec, int n, int nrhs, const void* LU, int64_t lda, const int32_t* perm, const void* B,
                   void* X, void* work, void* stream);

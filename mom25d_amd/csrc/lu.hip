@@ -612,21 +612,25 @@ __global__ void getrs_prep_kernel(int n, int nrhs, const int32_t* perm, const cp
 // by a product spread over the workgroup instead of the substitution.  When
 // B_reset is B, each block sets the rows of B it has read to the sentinel
 // (B then serves as the next solve's output; no other block reads them).
-// With out64 the solution is also written there in complex128.
-template <typename R, bool LOWER, int NR, bool INV>
-__global__ __launch_bounds__(256) void trsv_kernel(int n, const cplx<R>* __restrict__ A, int64_t lda,
-                                                   const cplx<R>* B, cplx<R>* Xout, const cplx<R>* __restrict__ Dinv,
-                                                   cplx<R>* B_reset, z64* __restrict__ out64, unsigned* ticket,
-                                                   unsigned* error, unsigned spin_limit) {
-  constexpr int BS = kTrsvBS;
-  constexpr int TRI = LOWER ? BS * (BS - 1) / 2 : BS * (BS + 1) / 2;
-  __shared__ cplx<R> Dk[INV ? 1 : TRI];
+// With out64 the solution is also written there in complex128.  Blocks of
+// BS rows (64, or 128 with inverted blocks) and BS / 16 column segments of
+// 16 per row: BS^2 / 16 threads.
+template <typename R, bool LOWER, int NR, bool INV, int BS = kTrsvBS>
+__global__ __launch_bounds__(BS * BS / 16) void trsv_kernel(int n, const cplx<R>* __restrict__ A, int64_t lda,
+                                                            const cplx<R>* B, cplx<R>* Xout,
+                                                            const cplx<R>* __restrict__ Dinv, cplx<R>* B_reset,
+                                                            z64* __restrict__ out64, unsigned* ticket,
+                                                            unsigned* error, unsigned spin_limit) {
+  static_assert(INV || BS == kTrsvBS, "the substitution works on 64-row blocks");
+  constexpr int Q = BS / 16, NT = BS * Q;
+  constexpr int TRI = INV ? 1 : (LOWER ? BS * (BS - 1) / 2 : BS * (BS + 1) / 2);
+  __shared__ cplx<R> Dk[TRI];
   __shared__ cplx<R> xs[BS][NR];
-  __shared__ cplx<R> part[4][BS][NR];
+  __shared__ cplx<R> part[Q][BS][NR];
   __shared__ int s_blk;
   __shared__ int s_abort;
   const int tid = threadIdx.x;
-  const int r = tid & 63, q = tid >> 6;  // row of the block, column quarter
+  const int r = tid % BS, q = tid / BS;  // row of the block, column segment
   const int nblk = (n + BS - 1) / BS;
   if (tid == 0) {
     const unsigned t = atomicAdd(ticket, 1u);
@@ -637,12 +641,25 @@ __global__ __launch_bounds__(256) void trsv_kernel(int n, const cplx<R>* __restr
   const int blk = s_blk;
   const int k0 = blk * BS, b = min(BS, n - k0);
   const int row = k0 + r;
-  cplx<R> dv[INV ? 16 : 1];  // this thread's 16 entries of the inverse's row r
-  if (INV) {
+  // this thread's 16 entries of the inverse's row r: loaded up front in the
+  // 64-row form; in the 128-row form (128 VGPRs per thread at 1024 threads)
+  // only after the dependency loop, so that they are not live beside the
+  // prefetched tile of A
+  constexpr bool EARLY_DV = BS == kTrsvBS;
+  cplx<R> dv[INV ? 16 : 1];
+  auto load_dv = [&]() {
+    const cplx<R>* drow = Dinv + (int64_t)blk * BS * BS + r * BS + 16 * q;
 #pragma unroll
-    for (int e = 0; e < 16; ++e) dv[e] = Dinv[(int64_t)blk * BS * BS + r * BS + 16 * q + e];
-  }
-  for (int e = tid; !INV && e < BS * BS; e += 256) {
+    for (int e = 0; e < 16; ++e) dv[e] = drow[e];
+  };
+  auto partsum = [&](int i, int c) {
+    cplx<R> t = part[0][i][c];
+#pragma unroll
+    for (int k = 1; k < Q; ++k) t += part[k][i][c];
+    return t;
+  };
+  if (INV && EARLY_DV) load_dv();
+  for (int e = tid; !INV && e < BS * BS; e += NT) {
     const int i = e / BS, k = e % BS;
     if (LOWER ? i > k : i <= k) {
       cplx<R> v = (i < b && k < b) ? A[(int64_t)(k0 + i) * lda + k0 + k] : cplx<R>{(R)(i == k), 0};
@@ -654,17 +671,23 @@ __global__ __launch_bounds__(256) void trsv_kernel(int n, const cplx<R>* __restr
 #pragma unroll
   for (int c = 0; c < NR; ++c) acc[c] = {0, 0};
   const int need = LOWER ? blk : nblk - 1 - blk;  // blocks before this one in solve order
-  // This thread's 16 entries of A[row, block d] for the next dependency d,
-  // loaded while the workgroup waits for block d.
+  // This thread's 16 entries of A[row, block d] for dependency d, loaded
+  // before the workgroup waits for block d (the wait covers their latency).
   cplx<R> la[16];
   auto fetch = [&](int d) {
     const int j0 = (LOWER ? d : nblk - 1 - d) * BS + 16 * q;
+    const cplx<R>* arow = A + (int64_t)row * lda + j0;  // one base, immediate offsets
+    if (r < b && j0 + 16 <= n) {
 #pragma unroll
-    for (int e = 0; e < 16; ++e) la[e] = (r < b && j0 + e < n) ? A[(int64_t)row * lda + j0 + e] : cplx<R>{0, 0};
+      for (int e = 0; e < 16; ++e) la[e] = arow[e];
+    } else {
+#pragma unroll
+      for (int e = 0; e < 16; ++e) la[e] = (r < b && j0 + e < n) ? arow[e] : cplx<R>{0, 0};
+    }
   };
-  if (need) fetch(0);
   for (int d = 0; d < need; ++d) {
     const int j0 = (LOWER ? d : nblk - 1 - d) * BS;
+    fetch(d);
     if (tid < BS * NR) {
       const int i = tid / NR, c = tid % NR;
       cplx<R> v = {0, 0};
@@ -688,12 +711,16 @@ __global__ __launch_bounds__(256) void trsv_kernel(int n, const cplx<R>* __restr
       return;
     }
 #pragma unroll
-    for (int e = 0; e < 16; ++e)
+    for (int e = 0; e < 16; ++e) {
 #pragma unroll
       for (int c = 0; c < NR; ++c) acc[c] += la[e] * xs[16 * q + e][c];
-    if (d + 1 < need) fetch(d + 1);
+      // in the 128-row form keep the LDS reads to 4 columns in flight
+      // (hoisting all 16 exceeds 128 VGPRs at 1024 threads and spills)
+      if (!EARLY_DV && (e & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+    }
     __syncthreads();
   }
+  if (INV && !EARLY_DV) load_dv();
 #pragma unroll
   for (int c = 0; c < NR; ++c) part[q][r][c] = acc[c];
   __syncthreads();
@@ -701,8 +728,7 @@ __global__ __launch_bounds__(256) void trsv_kernel(int n, const cplx<R>* __restr
     // x_k = Dinv_k (b_k - sum): v into LDS, then one product per (row, quarter)
     if (tid < BS * NR) {
       const int i = tid / NR, c = tid % NR;
-      xs[i][c] = i < b ? B[(int64_t)(k0 + i) * NR + c] - part[0][i][c] - part[1][i][c] - part[2][i][c] - part[3][i][c]
-                       : cplx<R>{0, 0};
+      xs[i][c] = i < b ? B[(int64_t)(k0 + i) * NR + c] - partsum(i, c) : cplx<R>{0, 0};
     }
     __syncthreads();
     if (B_reset && tid < BS * NR && tid / NR < b)
@@ -710,16 +736,18 @@ __global__ __launch_bounds__(256) void trsv_kernel(int n, const cplx<R>* __restr
 #pragma unroll
     for (int c = 0; c < NR; ++c) acc[c] = {0, 0};
 #pragma unroll
-    for (int e = 0; e < 16; ++e)
+    for (int e = 0; e < 16; ++e) {
 #pragma unroll
       for (int c = 0; c < NR; ++c) acc[c] += dv[e] * xs[16 * q + e][c];
+      if (!EARLY_DV && (e & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+    }
 #pragma unroll
     for (int c = 0; c < NR; ++c) part[q][r][c] = acc[c];
     __syncthreads();
     if (tid < BS * NR) {
       const int i = tid / NR, c = tid % NR;
       if (i < b) {
-        const cplx<R> x = part[0][i][c] + part[1][i][c] + part[2][i][c] + part[3][i][c];
+        const cplx<R> x = partsum(i, c);
         store_coherent_c(Xout + (int64_t)(k0 + i) * NR + c, x);
         if (out64) out64[(int64_t)(k0 + i) * NR + c] = {(double)x.re, (double)x.im};
       }
@@ -733,8 +761,7 @@ __global__ __launch_bounds__(256) void trsv_kernel(int n, const cplx<R>* __restr
   cplx<R> v[NR];
 #pragma unroll
   for (int c = 0; c < NR; ++c)
-    v[c] = i < b ? B[(int64_t)(k0 + i) * NR + c] - part[0][i][c] - part[1][i][c] - part[2][i][c] - part[3][i][c]
-                 : cplx<R>{0, 0};
+    v[c] = i < b ? B[(int64_t)(k0 + i) * NR + c] - partsum(i, c) : cplx<R>{0, 0};
   if (B_reset && i < b) {
 #pragma unroll
     for (int c = 0; c < NR; ++c)
@@ -1022,6 +1049,10 @@ static int launch_panel(cplx<R>* A, int64_t lda, int n, int k0, int b, int32_t* 
 // takes one update of depth nb.  Set by mom_set_lu_block (the autotuner).
 static int g_lu_nb[2] = {0, 0};
 
+// Row blocks of the complex64 solves with inverted diagonal blocks (64 or
+// 128), set by mom_set_solve_rows.
+static int g_solve_rows = 64;
+
 // U12 = L11^-1 A12 for the block row [k0, k0 + b) of columns [c0, c1), with
 // b up to 256: 128-row triangular solves and the update between them.
 template <typename R>
@@ -1202,6 +1233,73 @@ __global__ __launch_bounds__(64) void trtri_blocks_kernel(int n, const cplx<R>* 
   }
 }
 
+// Inverse of a 128-row diagonal block of L or U from the inverses of its two
+// 64-row blocks (trtri_blocks_kernel): [[A, 0], [C, D]]^-1 =
+// [[A^-1, 0], [-D^-1 C A^-1, D^-1]] and [[A, B], [0, D]]^-1 =
+// [[A^-1, -A^-1 B D^-1], [0, D^-1]].  blockIdx.x: the 128-block, blockIdx.y:
+// 0 lower, 1 upper.  A second 64-block past n is the identity (as the padding
+// of the 64-block inverses).  The two 64 x 64 products run through LDS.
+constexpr int kTrsvBS2 = 128;
+
+template <typename R>
+__global__ __launch_bounds__(256) void compose_inverse_kernel(int n, const cplx<R>* __restrict__ A, int64_t lda,
+                                                              const cplx<R>* __restrict__ D64, int nblk64,
+                                                              cplx<R>* __restrict__ D128) {
+  constexpr int H = kTrsvBS;  // 64
+  extern __shared__ unsigned char smem_raw[];
+  cplx<R>* S1 = reinterpret_cast<cplx<R>*>(smem_raw);  // [H][H]
+  cplx<R>* S2 = S1 + H * H;                             // [H][H]
+  const int K = blockIdx.x, upper = blockIdx.y, tid = threadIdx.x;
+  const int nblk128 = gridDim.x;
+  const int a = 2 * K, b = 2 * K + 1;
+  const bool has_b = b < nblk64;
+  const cplx<R>* Ia = D64 + ((int64_t)upper * nblk64 + a) * H * H;
+  const cplx<R>* Ib = has_b ? D64 + ((int64_t)upper * nblk64 + b) * H * H : nullptr;
+  auto inv_b = [&](int i, int k) { return has_b ? Ib[i * H + k] : cplx<R>{(R)(i == k), 0}; };
+  // off-diagonal block: lower C = L[rows b, cols a]; upper B = U[rows a, cols b]
+  const int r0 = (upper ? a : b) * H, c0 = (upper ? b : a) * H;
+  for (int e = tid; e < H * H; e += 256) {
+    const int i = e / H, k = e % H;
+    S1[e] = (r0 + i < n && c0 + k < n) ? A[(int64_t)(r0 + i) * lda + c0 + k] : cplx<R>{0, 0};
+    S2[e] = upper ? inv_b(i, k) : Ia[e];
+  }
+  __syncthreads();
+  // thread: row i, 16 columns j0 .. j0 + 15 of a 64 x 64 product
+  const int i = tid / 4, j0 = (tid % 4) * 16;
+  z64 acc[16];
+  auto product = [&]() {  // acc = S1 (left) x S2 (right) in FP64
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[e] = {0, 0};
+    for (int k = 0; k < H; ++k) {
+      const cplx<R> x = S1[i * H + k];
+      const z64 xd = {(double)x.re, (double)x.im};
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const cplx<R> y = S2[k * H + j0 + e];
+        acc[e] += xd * z64{(double)y.re, (double)y.im};
+      }
+    }
+  };
+  product();  // T = C A^-1 (lower) or B D^-1 (upper)
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < 16; ++e) S2[i * H + j0 + e] = {(R)acc[e].re, (R)acc[e].im};  // T into S2 (right operand)
+  for (int e = tid; e < H * H; e += 256) S1[e] = upper ? Ia[e] : inv_b(e / H, e % H);  // left: A^-1 / D^-1
+  __syncthreads();
+  product();  // D^-1 T (lower) or A^-1 T (upper)
+  cplx<R>* out = D128 + ((int64_t)upper * nblk128 + K) * kTrsvBS2 * kTrsvBS2;
+  const int mr = upper ? 0 : H, mc = upper ? H : 0;  // quadrant of the product
+#pragma unroll
+  for (int e = 0; e < 16; ++e)
+    out[(int64_t)(mr + i) * kTrsvBS2 + mc + j0 + e] = {(R)(-acc[e].re), (R)(-acc[e].im)};
+  for (int e = tid; e < H * H; e += 256) {
+    const int ii = e / H, k = e % H;
+    out[(int64_t)ii * kTrsvBS2 + k] = Ia[e];                                  // top-left: A^-1
+    out[(int64_t)(H + ii) * kTrsvBS2 + H + k] = inv_b(ii, k);                // bottom-right: D^-1
+    out[(int64_t)(upper ? H + ii : ii) * kTrsvBS2 + (upper ? k : H + k)] = {0, 0};  // the zero quadrant
+  }
+}
+
 // Workspace of getrs: the block tickets of the two solves (words 0 and 2),
 // the failure flag (word 4), then the lower solve's output and the
 // precision-R copy of the solution (n x 4 complex each).
@@ -1210,11 +1308,19 @@ static size_t getrs_work_bytes(int n) {
   return 256 + 2 * sizeof(cplx<R>) * 4 * (size_t)std::max(n, 1);
 }
 
-// getrs_inv adds the inverted diagonal blocks of L and U after that.
+// getrs_inv adds the inverted 64-row diagonal blocks of L and U after that,
+// and in complex64 the inverted 128-row blocks composed from them.
+template <typename R>
+static constexpr bool inv128() {
+  return sizeof(R) == sizeof(float);
+}
 template <typename R>
 static size_t getrs_inv_work_bytes(int n) {
   const size_t nblk = (size_t)(std::max(n, 1) + kTrsvBS - 1) / kTrsvBS;
-  return getrs_work_bytes<R>(n) + sizeof(cplx<R>) * 2 * nblk * kTrsvBS * kTrsvBS;
+  const size_t nblk2 = (size_t)(std::max(n, 1) + kTrsvBS2 - 1) / kTrsvBS2;
+  size_t b = getrs_work_bytes<R>(n) + sizeof(cplx<R>) * 2 * nblk * kTrsvBS * kTrsvBS;
+  if (inv128<R>()) b += sizeof(cplx<R>) * 2 * nblk2 * kTrsvBS2 * kTrsvBS2;
+  return b;
 }
 
 template <typename R>
@@ -1231,7 +1337,49 @@ static int trtri_blocks(int n, const cplx<R>* LU, int64_t lda, void* work, hipSt
     attr_set = true;
   }
   hipLaunchKernelGGL(trtri_blocks_kernel<R>, dim3(nblk, 2), dim3(kTrsvBS), lds, s, n, LU, lda, D);
-  return check_launch("trtri_blocks_kernel");
+  int rc = check_launch("trtri_blocks_kernel");
+  if (rc || !inv128<R>()) return rc;
+  const int nblk2 = (n + kTrsvBS2 - 1) / kTrsvBS2;
+  cplx<R>* D2 = D + (int64_t)2 * nblk * kTrsvBS * kTrsvBS;
+  const size_t lds2 = 2 * sizeof(cplx<R>) * kTrsvBS * kTrsvBS;
+  static bool attr2_set = false;
+  if (!attr2_set) {
+    if (hipFuncSetAttribute((const void*)compose_inverse_kernel<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds2) != hipSuccess)
+      return fail("compose_inverse_kernel: cannot reserve LDS");
+    attr2_set = true;
+  }
+  hipLaunchKernelGGL(compose_inverse_kernel<R>, dim3(nblk2, 2), dim3(256), lds2, s, n, LU, lda, D, nblk, D2);
+  return check_launch("compose_inverse_kernel");
+}
+
+// X = (LU)^-1 P B.  B in precision TB; the solution goes to X (precision R)
+// or, when X64 is given, to X64 in complex128 (X is then the workspace's
+// copy).
+// The lower solve P B (in X) -> Y, clearing X for the upper solve, then the
+// upper solve Y -> X (and X64), with blocks of BSX rows.
+template <typename R, bool INV, int BSX>
+static int trsv_pair(int n, int nrhs, const cplx<R>* LU, int64_t lda, cplx<R>* X, cplx<R>* Y, const cplx<R>* DL,
+                     const cplx<R>* DU, z64* X64, unsigned* flags, hipStream_t s) {
+  const int nblkx = (n + BSX - 1) / BSX;
+  const dim3 block(BSX * BSX / 16);
+  int rc;
+#define MOM_TRSV(NR)                                                                                                  \
+  case NR:                                                                                                            \
+    hipLaunchKernelGGL((trsv_kernel<R, true, NR, INV, BSX>), dim3(nblkx), block, 0, s, n, LU, lda, X, Y, DL, X,      \
+                       (z64*)nullptr, flags, flags + 4, g_spin_limit);                                                \
+    if ((rc = check_launch("trsv_kernel"))) return rc;                                                                \
+    hipLaunchKernelGGL((trsv_kernel<R, false, NR, INV, BSX>), dim3(nblkx), block, 0, s, n, LU, lda, Y, X, DU,        \
+                       (cplx<R>*)nullptr, X64, flags + 2, flags + 4, g_spin_limit);                                   \
+    break;
+  switch (nrhs) {
+    MOM_TRSV(1)
+    MOM_TRSV(2)
+    MOM_TRSV(3)
+    MOM_TRSV(4)
+  }
+#undef MOM_TRSV
+  return check_launch("trsv_kernel");
 }
 
 // X = (LU)^-1 P B.  B in precision TB; the solution goes to X (precision R)
@@ -1246,31 +1394,25 @@ static int getrs(int n, int nrhs, const cplx<R>* LU, int64_t lda, const int32_t*
   cplx<R>* Y = reinterpret_cast<cplx<R>*>(reinterpret_cast<unsigned char*>(flags) + 256);
   if (X64) X = Y + 4 * (int64_t)std::max(n, 1);
   const int64_t nx = (int64_t)n * nrhs;
-  const dim3 fg((unsigned)((nx + 255) / 256));
+  hipLaunchKernelGGL((getrs_prep_kernel<R, TB>), dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, s, n, nrhs, perm,
+                     B, X, Y, flags);
+  int rc = check_launch("getrs_prep_kernel");
+  if (rc) return rc;
   const cplx<R>* DL = INV ? reinterpret_cast<const cplx<R>*>(reinterpret_cast<const unsigned char*>(flags) +
                                                              getrs_work_bytes<R>(n))
                           : nullptr;
   const cplx<R>* DU = INV ? DL + (int64_t)nblk * kTrsvBS * kTrsvBS : nullptr;
-  int rc;
-  hipLaunchKernelGGL((getrs_prep_kernel<R, TB>), fg, dim3(256), 0, s, n, nrhs, perm, B, X, Y, flags);
-  if ((rc = check_launch("getrs_prep_kernel"))) return rc;
-  // lower solve P B -> Y (clearing X for the upper solve), upper Y -> X
-#define MOM_TRSV(NR)                                                                                                  \
-  case NR:                                                                                                            \
-    hipLaunchKernelGGL((trsv_kernel<R, true, NR, INV>), dim3(nblk), dim3(256), 0, s, n, LU, lda, X, Y, DL, X,        \
-                       (z64*)nullptr, flags, flags + 4, g_spin_limit);                                                \
-    if ((rc = check_launch("trsv_kernel"))) return rc;                                                                \
-    hipLaunchKernelGGL((trsv_kernel<R, false, NR, INV>), dim3(nblk), dim3(256), 0, s, n, LU, lda, Y, X, DU,          \
-                       (cplx<R>*)nullptr, X64, flags + 2, flags + 4, g_spin_limit);                                   \
-    break;
-  switch (nrhs) {
-    MOM_TRSV(1)
-    MOM_TRSV(2)
-    MOM_TRSV(3)
-    MOM_TRSV(4)
+  // complex64 with inverted blocks and g_solve_rows = 128: the composed
+  // 128-row inverses (half the chain's blocks)
+  if constexpr (INV && inv128<R>()) {
+    if (g_solve_rows == kTrsvBS2) {
+      const int nblk2 = (n + kTrsvBS2 - 1) / kTrsvBS2;
+      const cplx<R>* D2L = DU + (int64_t)nblk * kTrsvBS * kTrsvBS;
+      const cplx<R>* D2U = D2L + (int64_t)nblk2 * kTrsvBS2 * kTrsvBS2;
+      return trsv_pair<R, INV, kTrsvBS2>(n, nrhs, LU, lda, X, Y, D2L, D2U, X64, flags, s);
+    }
   }
-#undef MOM_TRSV
-  return check_launch("trsv_kernel");
+  return trsv_pair<R, INV, kTrsvBS>(n, nrhs, LU, lda, X, Y, DL, DU, X64, flags, s);
 }
 
 }  // namespace mom
@@ -1356,6 +1498,14 @@ extern "C" int mom_getrs_inv_z(int prec, int n, int nrhs, const void* LU, int64_
     return getrs<float, true, double>(n, nrhs, (const c32*)LU, lda, perm, (const z64*)B, nullptr, (z64*)X, flags, s);
   return fail("mom_getrs_inv_z: prec must be 64 or 32");
 }
+
+extern "C" int mom_set_solve_rows(int rows) {
+  if (rows != 64 && rows != 128) return fail("mom_set_solve_rows: rows must be 64 or 128");
+  g_solve_rows = rows;
+  return 0;
+}
+
+extern "C" int mom_get_solve_rows(void) { return g_solve_rows; }
 
 extern "C" int mom_set_spin_limit(unsigned polls) {
   if (polls == 0) return fail("mom_set_spin_limit: the limit must be positive");

@@ -209,13 +209,23 @@ def test_bounded_waits_report_failure():
     assert np.linalg.norm(A @ X - B) <= 1e-12 * np.linalg.norm(A) * np.linalg.norm(X)
 
 
-@pytest.mark.parametrize("prec,n,nrhs", [(32, 1, 1), (32, 300, 2), (32, 2049, 4), (64, 777, 3)])
-def test_getrs_with_inverted_blocks(prec, n, nrhs):
-    """The solve with explicitly inverted 64 x 64 diagonal blocks: the same
-    solution as the substitution solve up to cond(block) rounding, and in
-    complex128 a backward error near round-off (a diagonally dominant matrix,
-    so the blocks are well conditioned); the inverses are reused by a second
-    solve with new right-hand sides."""
+@pytest.mark.parametrize("prec,n,nrhs,rows", [(32, 1, 1, 64), (32, 300, 2, 64), (32, 2049, 4, 64), (64, 777, 3, 64),
+                                            (32, 1, 1, 128), (32, 300, 2, 128), (32, 2049, 4, 128), (32, 130, 3, 128)])
+def test_getrs_with_inverted_blocks(prec, n, nrhs, rows):
+    """The solve with explicitly inverted diagonal blocks (64 rows, or in
+    complex64 128 rows composed from them): the same solution as the
+    substitution solve up to cond(block) rounding, and in complex128 a
+    backward error near round-off (a diagonally dominant matrix, so the
+    blocks are well conditioned); the inverses are reused by a second solve
+    with new right-hand sides."""
+    _lib.call("mom_set_solve_rows", rows)
+    try:
+        _getrs_inv_case(prec, n, nrhs)
+    finally:
+        _lib.call("mom_set_solve_rows", 64)
+
+
+def _getrs_inv_case(prec, n, nrhs):
     rng = np.random.default_rng(n + nrhs)
     dt = torch.complex128 if prec == 64 else torch.complex64
     A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + 4 * np.sqrt(n) * np.eye(n)
