@@ -1007,7 +1007,7 @@ template <typename R>
 static int launch_panel(cplx<R>* A, int64_t lda, int n, int k0, int b, int32_t* ipiv, int32_t* info,
                         PivotSlot* slots, cplx<R>* rowbuf, int gmax, hipStream_t s) {
   const int m = n - k0;
-  const size_t lds_max = 120 * 1024;
+  const size_t lds_max = 152 * 1024;  // of the 160 KB per workgroup, beside the static rows
   if (g_panel_lds_max == 0) {
     MOM_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(panel_kernel<double>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max));

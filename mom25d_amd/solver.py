@@ -355,8 +355,9 @@ class Solver:
     workers   : processes that run the wave fits of a sweep beside the GPU
                 work (None: up to 8 for sweeps of 16 points or more; 0: none).
     lanes     : frequency points in flight during a sweep, each on its own
-                stream with its own matrices (None: 4 up to N = 4000, else 2,
-                as many as fit in a third of the free device memory).  The latency-bound
+                stream with its own matrices (None: 4 up to N = 4000, else 3
+                in mixed precision and 2 in FP64, as many as fit in a third of
+                the free device memory).  The latency-bound
                 steps of one point (LU panels, triangular solves, host
                 de-embedding) then overlap the matrix-core work of another.
     """
@@ -492,7 +493,7 @@ class Solver:
             self._bufs[key] = b
         return b
 
-    PANEL_LDS = 120 * 1024  # LDS of one LU panel workgroup (the library's bound)
+    PANEL_LDS = 152 * 1024  # LDS of one LU panel workgroup (the library's bound)
 
     def _worker_count(self, nf):
         """Processes for the wave fits of a sweep (0: in this process)."""
@@ -501,16 +502,19 @@ class Solver:
         return min(8, max(1, (os.cpu_count() or 2) // 2)) if nf >= 16 else 0
 
     def _lane_count(self):
-        """Lanes of a sweep: as asked (default 4 up to N = 4000, else 2, as
-        many as fit in a third of the free device memory), and no more than
-        allow every lane's LU panel to be resident at once."""
+        """Lanes of a sweep: as asked (default 4 up to N = 4000, else 3 in
+        mixed precision and 2 in FP64, as many as fit in a third of the free
+        device memory), and no more than allow every lane's LU panel to be
+        resident at once."""
         n = self.mesh.n_unknowns
         if self.lanes is not None:
             lanes = max(1, int(self.lanes))
         else:
             per = n * n * (16 + (8 if self.precision == "mixed" else 0))
             free, _total = torch.cuda.mem_get_info(self.device)
-            want = 4 if n <= 4000 else 2
+            # mixed precision spends most of a sweep in latency-bound GMRES steps: a third lane
+            # overlaps them (measured at N = 12 428: 4.33 s vs 4.46-4.66 s for 51 points)
+            want = 4 if n <= 4000 else (3 if self.precision == "mixed" else 2)
             lanes = max(1, min(want, int(free / 3 // per)))
         elem = 16 if self.precision == "fp64" else 8
         rows = max(1, self.PANEL_LDS // (self.panel * elem))
@@ -978,16 +982,19 @@ class Solver:
 
     def _lu_equiv(self):
         """Cost of one complex64 factorisation in GMRES steps at this N: a
-        model of this implementation on MI355X (LU at ~32 TFLOP/s effective
+        model of this implementation on MI355X -- LU at ~32 TFLOP/s effective
         or ~8 us per column of panel latency, whichever is larger; a step =
-        two triangular block chains of ~3.4 us per 64 rows + one FP64 Z read
-        + ~0.2 ms of host round trip)."""
+        two triangular block chains of ~2.9 us per 64 rows + one FP64 Z read
+        + ~0.2 ms of host round trip -- times 1.4, because the steps of
+        concurrent lanes overlap better than factorisations do (the sweep of
+        the 12.4 k-unknown pour was fastest at that factor: 4.11 s for 51
+        points against 4.35 s at 1.0 and 4.41 s at 4.6)."""
         if self.lu_equiv is not None:
             return float(self.lu_equiv)
         n = self.mesh.n_unknowns
         t_lu = max(8 / 3 * n ** 3 / 32e12, 8e-6 * n)
-        t_step = 2 * (n / 64) * 3.4e-6 + 16 * n * n / 5e12 + 2e-4
-        return t_lu / t_step
+        t_step = 2 * (n / 64) * 2.9e-6 + 16 * n * n / 5e12 + 2e-4
+        return 1.4 * t_lu / t_step
 
     @staticmethod
     def _reuse_next(a, freq):
