@@ -433,6 +433,7 @@ class Solver:
         self.reuse_lu = bool(reuse_lu)
         self.lu_equiv = lu_equiv
         self._anchor = {}
+        self._history = {}
         self.lu_stats = dict(factorisations=0, reused=0, fallbacks=0, gmres_steps=0)
         self.lanes = lanes
         self.workers = workers
@@ -724,16 +725,18 @@ class Solver:
             return X
         return (yield from self._gmres_chunks(Z, A, perm, sw, B, lane))
 
-    def _gmres_chunks(self, Z, A32, perm, sw, B, lane=0, max_its=None):
+    def _gmres_chunks(self, Z, A32, perm, sw, B, lane=0, max_its=None, X0=None):
         """_gmres_ir over groups of at most 4 right-hand sides (the solve
-        kernels' limit); None if a group does not converge within max_its."""
+        kernels' limit); None if a group does not converge within max_its.
+        X0: an initial guess of the solution (or None)."""
         nrhs = B.shape[1]
         if nrhs <= 4:
-            return (yield from self._gmres_ir(Z, A32, perm, sw, B, lane, max_its))
+            return (yield from self._gmres_ir(Z, A32, perm, sw, B, lane, max_its, X0))
         X = torch.empty_like(B)
         its = 0
         for c0, c1 in _column_chunks(nrhs):
-            Xc = yield from self._gmres_ir(Z, A32, perm, sw, B[:, c0:c1].contiguous(), lane, max_its)
+            Xc = yield from self._gmres_ir(Z, A32, perm, sw, B[:, c0:c1].contiguous(), lane, max_its,
+                                           None if X0 is None else X0[:, c0:c1].contiguous())
             its += self.last_refine
             if Xc is None:
                 self.last_refine = its
@@ -764,7 +767,7 @@ class Solver:
         cus = torch.cuda.get_device_properties(self.device).multi_processor_count
         return max(1, cus // lanes)
 
-    def _gmres_ir(self, Z, A32, perm, sw, B, lane=0, max_its=None):
+    def _gmres_ir(self, Z, A32, perm, sw, B, lane=0, max_its=None, X0=None):
         """Restarted GMRES in FP64 on the diagonally scaled system
         (D Z D) y = D b, x = D y, right-preconditioned by the complex64 LU of
         D Z D.  All right-hand sides advance together (one Krylov basis per
@@ -773,7 +776,9 @@ class Solver:
         restart cycle that does not halve the true residual ends the
         iteration (the FP64 residual floor is reached).  With max_its it
         returns None once that many steps have not reached the target (the
-        caller then factors this matrix itself)."""
+        caller then factors this matrix itself).  X0: an initial guess of x;
+        the iteration then starts from y0 + M (D b - D Z D y0), y0 = D^-1 X0,
+        instead of M D b."""
         n, nrhs = B.shape
         P = _lib.ptr
         stream = _lib.stream_ptr()
@@ -798,7 +803,11 @@ class Solver:
             _lib.call("mom_residual", n, nrhs, P(Z), Z.stride(0), P(d * Y), P(B), P(R), P(rowsum), P(dv), stream)
             return d * R
 
-        Y = precond(Bs)
+        if X0 is None:
+            Y = precond(Bs)
+        else:
+            Y0 = X0 / d
+            Y = Y0 + precond(residual(Y0))
         R = residual(Y, rows)
         yield
         znorm = float(rows.max())
@@ -961,8 +970,9 @@ class Solver:
             fill(None)
             cap = max(20, int(a["lu_equiv"] + a["its0"]))
             X = yield from self._gmres_chunks(Z, Z32, self._bufs[("perm", lane)], self._bufs[("swork32", lane)], B,
-                                              lane, max_its=cap)
+                                              lane, max_its=cap, X0=self._extrapolate(lane, g.freq, B))
             if X is not None:
+                self._remember(lane, g.freq, X)
                 a["cost"] += self.last_refine
                 a["n"] += 1
                 a["last"] = self.last_refine
@@ -974,11 +984,42 @@ class Solver:
             fill(Z32)
         X = yield from self._lu_solve(Z, B, Z32, lane, graph)
         if mixed:
+            if reuse:
+                self._remember(lane, g.freq, X)
             self.lu_stats["factorisations"] += 1
             eq = self._lu_equiv()
             self._anchor[lane] = dict(freq=g.freq, lu_equiv=eq, its0=self.last_refine, cost=eq + self.last_refine,
                                       n=1, last=None)
         return X
+
+    def _remember(self, lane, freq, X):
+        """Keep the lane's last three solutions (device) for _extrapolate."""
+        h = self._history.setdefault(lane, [])
+        h.append((float(freq), X))
+        del h[:-3]
+
+    def _extrapolate(self, lane, freq, B):
+        """Initial guess at freq from the lane's last solutions: the Lagrange
+        polynomial through up to three of them, evaluated at freq, when freq
+        lies within one spacing of the last one (None otherwise).  The
+        solution varies smoothly with frequency away from resonances; GMRES
+        corrects whatever the extrapolation misses."""
+        h = self._history.get(lane, [])
+        if not h or h[-1][1].shape != B.shape:
+            return None
+        fs = [f for f, _ in h]
+        step = abs(fs[-1] - fs[-2]) if len(fs) > 1 else None
+        if step is None or step == 0 or abs(freq - fs[-1]) > 1.5 * step:
+            return None
+        pts = [(f, X) for f, X in h if abs(freq - f) <= 3.5 * step]
+        X0 = torch.zeros_like(B)
+        for i, (fi, Xi) in enumerate(pts):
+            w = 1.0
+            for j, (fj, _) in enumerate(pts):
+                if j != i:
+                    w *= (freq - fj) / (fi - fj)
+            X0 += w * Xi
+        return X0
 
     def _lu_equiv(self):
         """Cost of one complex64 factorisation in GMRES steps at this N: a
@@ -1274,7 +1315,8 @@ def _sweep_many(jobs, lanes, workers, keep_currents=False, graphs=False):
     device = jobs[0][0].device
     n_exc = [len(o["solver"].ports) + len(o["solver"].iports) for o in out]
     for s, *_ in jobs:
-        s._anchor = {}  # no factorisation carries over from an earlier call
+        s._anchor = {}  # no factorisation or solution carries over from an earlier call
+        s._history = {}
     torch.cuda.synchronize(device)
     t1 = time.time()
     pool = fit_pool(workers)
