@@ -232,6 +232,17 @@ class Layout:
         self.vias.append(v)
         return v
 
+    def add_polygon(self, geom, layer="top"):
+        """Polygon copper (Engine/mom25d/mesh.py:543-560) is not part of this
+        build: it needs shapely geometry, which this image lacks."""
+        raise ValueError("polygon copper is not supported by this build (needs shapely); use add_rect")
+
+    def add_pour(self, region, clearance, **kw):
+        """Copper pours (Engine/mom25d/mesh.py:488-506) are not part of this
+        build: they need shapely geometry, which this image lacks."""
+        raise ValueError("copper pours are not supported by this build (needs shapely); build the copper from "
+                         "rectangles and add the stitching vias with add_via")
+
     def add_plane(self, name):
         if name not in self.planes:
             self.planes.append(name)

@@ -56,6 +56,15 @@ def test_touchstone_formats_and_interpolation(tmp_path):
         Touchstone(str(ri)).z(5e9)
 
 
+def test_layout_rejects_what_this_build_lacks():
+    from mom25d_amd.mesh import Layout
+
+    with pytest.raises(ValueError, match="shapely"):
+        Layout().add_pour((0, 0, 1e-3, 1e-3), 0.2e-3)
+    with pytest.raises(ValueError, match="shapely"):
+        Layout().add_polygon(None)
+
+
 def test_layout_rejects_oblique_and_duplicate_gaps():
     from mom25d_amd.mesh import Layout
 
