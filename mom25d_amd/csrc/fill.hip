@@ -245,11 +245,12 @@ __global__ __launch_bounds__(FillShape<TIER>::threads) void fill_kernel(FillArgs
     for (int pl = wave; pl < kBlock; pl += NW) {
       const int p = s_tri[pl];
       if (p < 0) break;
-      z64 IA[3][3];
+      // IA[i][j] = sum_k w_k (r_k - v_i) . (K1_k - v_j K0_k) summed over the
+      // test points first: T_i = sum_k a_ik . K1_k and M_i = sum_k a_ik K0_k
+      // (a_ik = w_k (r_k - v_i)), then IA[i][j] = T_i - v_j . M_i
+      z64 T[3], Mx[3], My[3];
 #pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) IA[i][j] = {0.0, 0.0};
+      for (int i = 0; i < 3; ++i) T[i] = Mx[i] = My[i] = {0.0, 0.0};
       z64 IV = {0.0, 0.0};
 #pragma unroll 1  // unrolled, the far tier needs 256 + 140 registers per lane
       for (int k = 0; k < 7; ++k) {
@@ -287,9 +288,9 @@ __global__ __launch_bounds__(FillShape<TIER>::threads) void fill_kernel(FillArgs
         for (int i = 0; i < 3; ++i) {
           const double ax = wk * (rx - s_V[pl][2 * i]);
           const double ay = wk * (ry - s_V[pl][2 * i + 1]);
-          const z64 t1 = ax * k1x + ay * k1y;
-#pragma unroll
-          for (int j = 0; j < 3; ++j) IA[i][j] += t1 - (ax * Vs[2 * j] + ay * Vs[2 * j + 1]) * k0a;
+          T[i] += ax * k1x + ay * k1y;
+          Mx[i] += ax * k0a;
+          My[i] += ay * k0a;
         }
       }
       if (pads) {
@@ -307,7 +308,8 @@ __global__ __launch_bounds__(FillShape<TIER>::threads) void fill_kernel(FillArgs
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
           if (cl[j] < 0) continue;
-          const z64 v = (jwmu * (cAi * cAs[j])) * IA[i][j] + (ijwe * (cVi * cVs[j])) * IV;
+          const z64 ia = T[i] - (Vs[2 * j] * Mx[i] + Vs[2 * j + 1] * My[i]);
+          const z64 v = (jwmu * (cAi * cAs[j])) * ia + (ijwe * (cVi * cVs[j])) * IV;
           double* dst = acc + 2 * (rl * nc + cl[j]);
           atomicAdd(dst, v.re);
           atomicAdd(dst + 1, v.im);
