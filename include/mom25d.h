@@ -73,10 +73,11 @@ int mom_symmetrize(int n, double* Z, int64_t ldz, float* Z32, int64_t ld32, cons
 int mom_diag_scale(int n, const double* Z, int64_t ldz, double* d, void* stream);
 
 /* Line voltage at points of level li from the triangle charges q (nt x nrhs),
- * Engine/mom25d/solver.py:1101-1125 (Solver.line_voltage). */
+ * Engine/mom25d/solver.py:1101-1125 (Solver.line_voltage).  tab_b: the bucket
+ * entries of the tables, 16 bytes each ({double split; int32 i0; int32 pad}). */
 int mom_line_voltage(int npts, const double* pts, int li, int nl, int nt, int nrhs, const int32_t* tri_layer,
                      const double* V, const double* qp7, const double* qw7, const void* pairs, const double* tab_x,
-                     const double* tab_cw, const int32_t* tab_b, const double* q, double inv_eps0, double* out,
+                     const double* tab_cw, const void* tab_b, const double* q, double inv_eps0, double* out,
                      void* stream);
 
 /* LU with partial pivoting of an n x n complex matrix in place (prec 64:
@@ -118,7 +119,7 @@ int mom_get_lu_block(int prec);
 /* Row permutation of the factorisation: perm[i] = original row of row i. */
 int mom_build_perm(int n, const int32_t* ipiv, int32_t* perm, void* stream);
 
-/* X = (LU)^-1 P B for nrhs <= 4 right-hand sides (n x nrhs, row-major).
+/* X = (LU)^-1 P B for nrhs <= 8 right-hand sides (n x nrhs, row-major).
  * `work` holds mom_getrs_work_bytes(prec, n) bytes of device memory (block
  * tickets, cleared by the call itself, and the intermediate L^-1 P B); its
  * fifth uint32 is 1 after the call when a bounded wait of the block chain ran
@@ -144,6 +145,14 @@ int64_t mom_getrs_inv_work_bytes(int prec, int n);
  * half as many blocks in the chain).  Process-wide. */
 int mom_set_solve_rows(int rows);
 int mom_get_solve_rows(void);
+/* Chain form of the 64-row solves with inverted blocks (default on):
+ * mom_trtri_blocks also forms M_k = D_k^-1 C_k, C_k the coupling of block k to
+ * the block solved just before it, so that a block computes
+ * y_k = D_k^-1 (b_k - all other couplings) while it waits and then only
+ * x_k = y_k - M_k x_prev.  Off: x_k = D_k^-1 (b_k - all couplings) after the
+ * wait.  Process-wide. */
+int mom_set_solve_chain(int on);
+int mom_get_solve_chain(void);
 int mom_trtri_blocks(int prec, int n, const void* LU, int64_t lda, void* work, void* stream);
 int mom_getrs_inv(int prec, int n, int nrhs, const void* LU, int64_t lda, const int32_t* perm, const void* B,
                   void* X, void* work, void* stream);

@@ -47,6 +47,8 @@ SIGNATURES = {
     "mom_scaled_matvec": (_i, [_i, _i, _p, _i64, _p, _p, _p, _p]),
     "mom_set_solve_rows": (_i, [_i]),
     "mom_get_solve_rows": (_i, []),
+    "mom_set_solve_chain": (_i, [_i]),
+    "mom_get_solve_chain": (_i, []),
     "mom_interp_anchor": (_i, [_i, _i, _i, _p, _i64, _p, _d, _d, _p, _i64, _p]),
     "mom_interp_combine": (_i, [_i, _i, _i, _i, _p, _p, _i64, _p, _d, _d, _p, _i64, _p]),
     "mom_cgemm_sub": (_i, [_i, _i, _i, _i, _p, _i64, _p, _i64, _p, _i64, _p]),

@@ -20,8 +20,10 @@
 // wavefront owns source triangle l of the chunk; wavefront w walks test
 // triangles w, w+4, ... of the block, whose data is staged in LDS.  The
 // smooth kernel is a cubic spline read from global memory (L1/L2 resident);
-// its interval comes from a uniform bucket table plus a one-step fix-up,
-// giving exactly scipy's find_interval result (clamped end intervals).
+// its interval comes from a uniform bucket table whose entries hold the
+// interval at the start of the bucket and the one node inside it, so the
+// lookup is one load and one comparison (no data-dependent loop), giving
+// exactly scipy's find_interval result (clamped end intervals).
 // Contributions are summed into Z with hardware FP64 atomics; the matrix is
 // symmetrised by a later pass, as the reference does.
 
@@ -33,10 +35,16 @@ constexpr int kBlock = 32;     // test triangles per block
 constexpr int kChunk = 64;     // source triangles per work item (one per lane)
 constexpr double kFourPi = 12.566370614359172954;
 
+struct Bucket {        // interval lookup entry (16 bytes; see spline_interval)
+  double split;        // the node inside the bucket's range (+inf: none)
+  int32_t i0;          // interval at the start of the range
+  int32_t pad_;
+};
+
 struct PairTab {       // spline of one (observer level, source level) pair at one frequency
   int64_t x_off;       // offset of the rho nodes
   int64_t c_off;       // offset (in doubles) of the coefficients, 16 per interval
-  int64_t b_off;       // offset of the bucket table
+  int64_t b_off;       // offset (in entries) of the bucket table
   int32_t n;           // number of nodes
   int32_t nbucket;
   double inv_dx;       // 1 / bucket width
@@ -69,7 +77,7 @@ struct FillArgs {
   const PairTab* pairs;
   const double* tab_x;
   const double* tab_c;
-  const int32_t* tab_b;
+  const Bucket* tab_b;
   double jwmu_re, jwmu_im;
   double ijwe_re, ijwe_im;
   double* Z;      // complex, row-major
@@ -85,20 +93,22 @@ struct FillArgs {
   int64_t acc_bytes;           // LDS accumulator bytes of the launch (max over its items)
 };
 
-__device__ inline int spline_interval(const double* __restrict__ x, const int32_t* __restrict__ bucket, int n,
-                                      int nbucket, double inv_dx, double R) {
+// Entry b of a bucket table covers the distances R with (int)(R * inv_dx)
+// == b (the last one every larger R too): i0 is the spline interval at the
+// start of that range and split the node inside it (+inf if none), the host
+// having placed at most one node per range (width <= half the smallest
+// spacing) and computed the ranges for this exact double arithmetic.
+__device__ inline int spline_interval(const Bucket* __restrict__ bucket, int nbucket, double inv_dx, double R) {
   int b = (int)(R * inv_dx);
   b = b < 0 ? 0 : (b >= nbucket ? nbucket - 1 : b);
-  int i = bucket[b];
-  while (i > 0 && R < x[i]) --i;
-  while (i < n - 2 && R >= x[i + 1]) ++i;
-  return i;
+  const Bucket e = bucket[b];
+  return e.i0 + (R >= e.split ? 1 : 0);
 }
 
 // S_A(R), S_V(R) of the tabulated smooth part (scipy PPoly evaluation).
 __device__ inline void spline_eval(const PairTab& pt, const double* __restrict__ x, const double* __restrict__ c,
-                                   const int32_t* __restrict__ bucket, double R, z64& sa, z64& sv) {
-  int i = spline_interval(x, bucket, pt.n, pt.nbucket, pt.inv_dx, R);
+                                   const Bucket* __restrict__ bucket, double R, z64& sa, z64& sv) {
+  const int i = spline_interval(bucket, pt.nbucket, pt.inv_dx, R);
   double d = R - x[i];
   const double* q = c + (int64_t)i * 16;  // [k][kernel][re,im], k = 0..3 (cubic .. constant)
   double z = d;
@@ -206,7 +216,7 @@ __global__ __launch_bounds__(FillShape<TIER>::threads) void fill_kernel(FillArgs
   const PairTab pt = a.pairs[pp];
   const double* __restrict__ tx = a.tab_x + pt.x_off;
   const double* __restrict__ tc = a.tab_c + pt.c_off;
-  const int32_t* __restrict__ tbk = a.tab_b + pt.b_off;
+  const Bucket* __restrict__ tbk = a.tab_b + pt.b_off;
   const z64 kA = {pt.cA_re, pt.cA_im}, kV = {pt.cV_re, pt.cV_im};
   const double dz = pt.dz;
   const z64 jwmu = {a.jwmu_re, a.jwmu_im}, ijwe = {a.ijwe_re, a.ijwe_im};
@@ -252,7 +262,10 @@ __global__ __launch_bounds__(FillShape<TIER>::threads) void fill_kernel(FillArgs
 #pragma unroll
       for (int i = 0; i < 3; ++i) T[i] = Mx[i] = My[i] = {0.0, 0.0};
       z64 IV = {0.0, 0.0};
-#pragma unroll 1  // unrolled, the far tier needs 256 + 140 registers per lane
+      // fully unrolled, the far tier would need 256 + 140 registers per lane;
+      // two test points at a time keep two spline lookups in flight there
+      constexpr int KU = TIER == 2 ? 2 : 1;
+#pragma unroll KU
       for (int k = 0; k < 7; ++k) {
         const double rx = s_qp[pl][2 * k], ry = s_qp[pl][2 * k + 1];
         z64 k0a, k1x, k1y, k0v;
@@ -465,7 +478,7 @@ __global__ __launch_bounds__(256) void line_voltage_kernel(int npts, const doubl
                                                            const PairTab* __restrict__ pairs,
                                                            const double* __restrict__ tab_x,
                                                            const double* __restrict__ tab_cw,
-                                                           const int32_t* __restrict__ tab_b,
+                                                           const Bucket* __restrict__ tab_b,
                                                            const double* __restrict__ q, double inv_eps0,
                                                            double* __restrict__ out) {
   const int m = blockIdx.x, tid = threadIdx.x;
@@ -479,12 +492,12 @@ __global__ __launch_bounds__(256) void line_voltage_kernel(int npts, const doubl
     if (pt.zero) continue;
     const double* x = tab_x + pt.x_off;
     const double* cw = tab_cw + pt.c_off / 2;
-    const int32_t* bk = tab_b + pt.b_off;
+    const Bucket* bk = tab_b + pt.b_off;
     z64 k = {0.0, 0.0};
     for (int l = 0; l < 7; ++l) {
       const double dx = rx - qp7[14 * (int64_t)t + 2 * l], dy = ry - qp7[14 * (int64_t)t + 2 * l + 1];
       const double R = sqrt(dx * dx + dy * dy);
-      const int i = spline_interval(x, bk, pt.n, pt.nbucket, pt.inv_dx, R);
+      const int i = spline_interval(bk, pt.nbucket, pt.inv_dx, R);
       const double d = R - x[i];
       const double* c = cw + (int64_t)i * 8;
       double z = d;
@@ -604,11 +617,12 @@ extern "C" int mom_diag_scale(int n, const double* Z, int64_t ldz, double* d, vo
 
 extern "C" int mom_line_voltage(int npts, const double* pts, int li, int nl, int nt, int nrhs,
                                 const int32_t* tri_layer, const double* V, const double* qp7, const double* qw7,
-                                const void* pairs, const double* tab_x, const double* tab_cw, const int32_t* tab_b,
+                                const void* pairs, const double* tab_x, const double* tab_cw, const void* tab_b,
                                 const double* q, double inv_eps0, double* out, void* stream) {
   if (npts <= 0) return 0;
   if (nrhs < 1 || nrhs > kMaxRhs) return fail("mom_line_voltage: 1..4 right-hand sides");
   hipLaunchKernelGGL(line_voltage_kernel, dim3(npts), dim3(256), 0, (hipStream_t)stream, npts, pts, li, nl, nt, nrhs,
-                     tri_layer, V, qp7, qw7, (const PairTab*)pairs, tab_x, tab_cw, tab_b, q, inv_eps0, out);
+                     tri_layer, V, qp7, qw7, (const PairTab*)pairs, tab_x, tab_cw, (const Bucket*)tab_b, q, inv_eps0,
+                     out);
   return check_launch("line_voltage_kernel");
 }

@@ -572,6 +572,7 @@ __global__ void build_perm_kernel(int n, const int32_t* ipiv, int32_t* perm) {
 // taken in launch order, so a workgroup only waits for workgroups that are
 // already running.
 constexpr int kTrsvBS = 64;
+constexpr int kMaxRhs = 8;  // right-hand sides per triangular solve
 
 template <bool LOWER>
 __device__ __forceinline__ int packed_index(int i, int k) {
@@ -614,11 +615,17 @@ __global__ void getrs_prep_kernel(int n, int nrhs, const int32_t* perm, const cp
 // (B then serves as the next solve's output; no other block reads them).
 // With out64 the solution is also written there in complex128.  Blocks of
 // BS rows (64, or 128 with inverted blocks) and BS / 16 column segments of
-// 16 per row: BS^2 / 16 threads.
+// 16 per row: BS^2 / 16 threads.  With Mchain (64-row inverted blocks: the
+// products M_k = Dinv_k A_k,prev of each block's inverse and its coupling
+// to the block solved just before it, chain_matrix_kernel) a block forms
+// y_k = Dinv_k (b_k - sum of all earlier couplings but that one) while it
+// waits for the previous block, and then only x_k = y_k - M_k x_prev: one
+// product on the chain's critical path instead of two.
 template <typename R, bool LOWER, int NR, bool INV, int BS = kTrsvBS>
 __global__ __launch_bounds__(BS * BS / 16) void trsv_kernel(int n, const cplx<R>* __restrict__ A, int64_t lda,
                                                             const cplx<R>* B, cplx<R>* Xout,
-                                                            const cplx<R>* __restrict__ Dinv, cplx<R>* B_reset,
+                                                            const cplx<R>* __restrict__ Dinv,
+                                                            const cplx<R>* __restrict__ Mchain, cplx<R>* B_reset,
                                                             z64* __restrict__ out64, unsigned* ticket,
                                                             unsigned* error, unsigned spin_limit) {
   static_assert(INV || BS == kTrsvBS, "the substitution works on 64-row blocks");
@@ -627,6 +634,7 @@ __global__ __launch_bounds__(BS * BS / 16) void trsv_kernel(int n, const cplx<R>
   __shared__ cplx<R> Dk[TRI];
   __shared__ cplx<R> xs[BS][NR];
   __shared__ cplx<R> part[Q][BS][NR];
+  __shared__ cplx<R> ys[INV ? BS : 1][NR];
   __shared__ int s_blk;
   __shared__ int s_abort;
   const int tid = threadIdx.x;
@@ -685,11 +693,12 @@ __global__ __launch_bounds__(BS * BS / 16) void trsv_kernel(int n, const cplx<R>
       for (int e = 0; e < 16; ++e) la[e] = (r < b && j0 + e < n) ? arow[e] : cplx<R>{0, 0};
     }
   };
-  for (int d = 0; d < need; ++d) {
+  // x of dependency d into xs; false when a bounded wait ran out (the block
+  // before this one never arrived: the kernel gives up)
+  auto wait_dep = [&](int d) {
     const int j0 = (LOWER ? d : nblk - 1 - d) * BS;
-    fetch(d);
-    if (tid < BS * NR) {
-      const int i = tid / NR, c = tid % NR;
+    for (int t = tid; t < BS * NR; t += NT) {
+      const int i = t / NR, c = t % NR;
       cplx<R> v = {0, 0};
       if (j0 + i < n) {
         unsigned spins = 0;
@@ -706,10 +715,17 @@ __global__ __launch_bounds__(BS * BS / 16) void trsv_kernel(int n, const cplx<R>
       xs[i][c] = v;
     }
     __syncthreads();
-    if (s_abort) {  // a block before this one never arrived: give up
+    if (s_abort) {
       if (tid == 0) *error = 1u;
-      return;
+      return false;
     }
+    return true;
+  };
+  // the chain form leaves the last dependency (the previous block) to M_k
+  const bool chain = INV && BS == kTrsvBS && Mchain != nullptr && need > 0;
+  for (int d = 0; d < (chain ? need - 1 : need); ++d) {
+    fetch(d);
+    if (!wait_dep(d)) return;
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
 #pragma unroll
@@ -726,13 +742,13 @@ __global__ __launch_bounds__(BS * BS / 16) void trsv_kernel(int n, const cplx<R>
   __syncthreads();
   if (INV) {
     // x_k = Dinv_k (b_k - sum): v into LDS, then one product per (row, quarter)
-    if (tid < BS * NR) {
-      const int i = tid / NR, c = tid % NR;
+    for (int t = tid; t < BS * NR; t += NT) {
+      const int i = t / NR, c = t % NR;
       xs[i][c] = i < b ? B[(int64_t)(k0 + i) * NR + c] - partsum(i, c) : cplx<R>{0, 0};
     }
     __syncthreads();
-    if (B_reset && tid < BS * NR && tid / NR < b)
-      B_reset[(int64_t)(k0 + tid / NR) * NR + tid % NR] = {Sentinel<R>::value(), Sentinel<R>::value()};
+    for (int t = tid; B_reset && t < BS * NR; t += NT)
+      if (t / NR < b) B_reset[(int64_t)(k0 + t / NR) * NR + t % NR] = {Sentinel<R>::value(), Sentinel<R>::value()};
 #pragma unroll
     for (int c = 0; c < NR; ++c) acc[c] = {0, 0};
 #pragma unroll
@@ -744,10 +760,29 @@ __global__ __launch_bounds__(BS * BS / 16) void trsv_kernel(int n, const cplx<R>
 #pragma unroll
     for (int c = 0; c < NR; ++c) part[q][r][c] = acc[c];
     __syncthreads();
-    if (tid < BS * NR) {
-      const int i = tid / NR, c = tid % NR;
+    if (chain) {
+      // y = Dinv (b - earlier couplings) kept in LDS; then x = y - M x_prev
+      for (int t = tid; t < BS * NR; t += NT) ys[t / NR][t % NR] = partsum(t / NR, t % NR);
+      cplx<R> mv[16];
+      const cplx<R>* mrow = Mchain + (int64_t)blk * BS * BS + r * BS + 16 * q;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) mv[e] = mrow[e];
+      if (!wait_dep(need - 1)) return;
+#pragma unroll
+      for (int c = 0; c < NR; ++c) acc[c] = {0, 0};
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+#pragma unroll
+        for (int c = 0; c < NR; ++c) acc[c] += mv[e] * xs[16 * q + e][c];
+      }
+#pragma unroll
+      for (int c = 0; c < NR; ++c) part[q][r][c] = acc[c];
+      __syncthreads();
+    }
+    for (int t = tid; t < BS * NR; t += NT) {
+      const int i = t / NR, c = t % NR;
       if (i < b) {
-        const cplx<R> x = partsum(i, c);
+        const cplx<R> x = chain ? ys[i][c] - partsum(i, c) : partsum(i, c);
         store_coherent_c(Xout + (int64_t)(k0 + i) * NR + c, x);
         if (out64) out64[(int64_t)(k0 + i) * NR + c] = {(double)x.re, (double)x.im};
       }
@@ -1052,6 +1087,7 @@ static int g_lu_nb[2] = {0, 0};
 // Row blocks of the complex64 solves with inverted diagonal blocks (64 or
 // 128), set by mom_set_solve_rows.
 static int g_solve_rows = 64;
+static int g_chain = 1;  // chain form of the 64-row inverted-block solves (mom_set_solve_chain)
 
 // U12 = L11^-1 A12 for the block row [k0, k0 + b) of columns [c0, c1), with
 // b up to 256: 128-row triangular solves and the update between them.
@@ -1300,12 +1336,58 @@ __global__ __launch_bounds__(256) void compose_inverse_kernel(int n, const cplx<
   }
 }
 
+// M_k = Dinv_k C_k for the chain form of the inverted-block solves, with C_k
+// the coupling of block k to the block solved just before it: lower
+// (blockIdx.y = 0) C_k = L[rows of block k, columns of block k - 1], upper
+// (1) C_k = U[rows of block k, columns of block k + 1]; zero for the first
+// block of each solve.  Formed in FP64 from the factors and inverses in
+// precision R, rounded to R; rows and columns beyond n are zero.
+template <typename R>
+__global__ __launch_bounds__(256) void chain_matrix_kernel(int n, const cplx<R>* __restrict__ A, int64_t lda,
+                                                           const cplx<R>* __restrict__ D, int nblk,
+                                                           cplx<R>* __restrict__ M) {
+  constexpr int H = kTrsvBS;
+  extern __shared__ unsigned char smem_raw[];
+  cplx<R>* SD = reinterpret_cast<cplx<R>*>(smem_raw);  // Dinv_k [H][H]
+  cplx<R>* SC = SD + H * H;                             // C_k [H][H]
+  const int k = blockIdx.x, upper = blockIdx.y, tid = threadIdx.x;
+  const int prev = upper ? k + 1 : k - 1;
+  cplx<R>* out = M + ((int64_t)upper * nblk + k) * H * H;
+  if (prev < 0 || prev >= nblk) {
+    for (int e = tid; e < H * H; e += 256) out[e] = {0, 0};
+    return;
+  }
+  const cplx<R>* Dk = D + ((int64_t)upper * nblk + k) * H * H;
+  const int r0 = k * H, c0 = prev * H;
+  for (int e = tid; e < H * H; e += 256) {
+    const int i = e / H, j = e % H;
+    SD[e] = Dk[e];
+    SC[e] = (r0 + i < n && c0 + j < n) ? A[(int64_t)(r0 + i) * lda + c0 + j] : cplx<R>{0, 0};
+  }
+  __syncthreads();
+  const int i = tid / 4, j0 = (tid % 4) * 16;  // row i, columns j0 .. j0 + 15
+  z64 acc[16];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) acc[e] = {0, 0};
+  for (int kk = 0; kk < H; ++kk) {
+    const cplx<R> x = SD[i * H + kk];
+    const z64 xd = {(double)x.re, (double)x.im};
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const cplx<R> y = SC[kk * H + j0 + e];
+      acc[e] += xd * z64{(double)y.re, (double)y.im};
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < 16; ++e) out[i * H + j0 + e] = {(R)acc[e].re, (R)acc[e].im};
+}
+
 // Workspace of getrs: the block tickets of the two solves (words 0 and 2),
 // the failure flag (word 4), then the lower solve's output and the
 // precision-R copy of the solution (n x 4 complex each).
 template <typename R>
 static size_t getrs_work_bytes(int n) {
-  return 256 + 2 * sizeof(cplx<R>) * 4 * (size_t)std::max(n, 1);
+  return 256 + 2 * sizeof(cplx<R>) * kMaxRhs * (size_t)std::max(n, 1);
 }
 
 // getrs_inv adds the inverted 64-row diagonal blocks of L and U after that,
@@ -1314,13 +1396,24 @@ template <typename R>
 static constexpr bool inv128() {
   return sizeof(R) == sizeof(float);
 }
+// After those, the chain matrices M_k of the lower and upper solves (64 x 64
+// per block, chain_matrix_kernel).
 template <typename R>
-static size_t getrs_inv_work_bytes(int n) {
+static size_t getrs_inv_chain_offset(int n) {
   const size_t nblk = (size_t)(std::max(n, 1) + kTrsvBS - 1) / kTrsvBS;
   const size_t nblk2 = (size_t)(std::max(n, 1) + kTrsvBS2 - 1) / kTrsvBS2;
   size_t b = getrs_work_bytes<R>(n) + sizeof(cplx<R>) * 2 * nblk * kTrsvBS * kTrsvBS;
   if (inv128<R>()) b += sizeof(cplx<R>) * 2 * nblk2 * kTrsvBS2 * kTrsvBS2;
   return b;
+}
+template <typename R>
+static size_t getrs_inv_work_bytes(int n) {
+  const size_t nblk = (size_t)(std::max(n, 1) + kTrsvBS - 1) / kTrsvBS;
+  return getrs_inv_chain_offset<R>(n) + sizeof(cplx<R>) * 2 * nblk * kTrsvBS * kTrsvBS;
+}
+template <typename R>
+static cplx<R>* chain_matrices(void* work, int n) {
+  return reinterpret_cast<cplx<R>*>(reinterpret_cast<unsigned char*>(work) + getrs_inv_chain_offset<R>(n));
 }
 
 template <typename R>
@@ -1338,7 +1431,18 @@ static int trtri_blocks(int n, const cplx<R>* LU, int64_t lda, void* work, hipSt
   }
   hipLaunchKernelGGL(trtri_blocks_kernel<R>, dim3(nblk, 2), dim3(kTrsvBS), lds, s, n, LU, lda, D);
   int rc = check_launch("trtri_blocks_kernel");
-  if (rc || !inv128<R>()) return rc;
+  if (rc) return rc;
+  const size_t ldsm = 2 * sizeof(cplx<R>) * kTrsvBS * kTrsvBS;
+  static bool attrm_set = false;
+  if (!attrm_set) {
+    if (hipFuncSetAttribute((const void*)chain_matrix_kernel<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)ldsm) != hipSuccess)
+      return fail("chain_matrix_kernel: cannot reserve LDS");
+    attrm_set = true;
+  }
+  hipLaunchKernelGGL(chain_matrix_kernel<R>, dim3(nblk, 2), dim3(256), ldsm, s, n, LU, lda, D, nblk,
+                     chain_matrices<R>(work, n));
+  if ((rc = check_launch("chain_matrix_kernel")) || !inv128<R>()) return rc;
   const int nblk2 = (n + kTrsvBS2 - 1) / kTrsvBS2;
   cplx<R>* D2 = D + (int64_t)2 * nblk * kTrsvBS * kTrsvBS;
   const size_t lds2 = 2 * sizeof(cplx<R>) * kTrsvBS * kTrsvBS;
@@ -1360,16 +1464,17 @@ static int trtri_blocks(int n, const cplx<R>* LU, int64_t lda, void* work, hipSt
 // upper solve Y -> X (and X64), with blocks of BSX rows.
 template <typename R, bool INV, int BSX>
 static int trsv_pair(int n, int nrhs, const cplx<R>* LU, int64_t lda, cplx<R>* X, cplx<R>* Y, const cplx<R>* DL,
-                     const cplx<R>* DU, z64* X64, unsigned* flags, hipStream_t s) {
+                     const cplx<R>* DU, const cplx<R>* ML, const cplx<R>* MU, z64* X64, unsigned* flags,
+                     hipStream_t s) {
   const int nblkx = (n + BSX - 1) / BSX;
   const dim3 block(BSX * BSX / 16);
   int rc;
 #define MOM_TRSV(NR)                                                                                                  \
   case NR:                                                                                                            \
-    hipLaunchKernelGGL((trsv_kernel<R, true, NR, INV, BSX>), dim3(nblkx), block, 0, s, n, LU, lda, X, Y, DL, X,      \
+    hipLaunchKernelGGL((trsv_kernel<R, true, NR, INV, BSX>), dim3(nblkx), block, 0, s, n, LU, lda, X, Y, DL, ML, X,  \
                        (z64*)nullptr, flags, flags + 4, g_spin_limit);                                                \
     if ((rc = check_launch("trsv_kernel"))) return rc;                                                                \
-    hipLaunchKernelGGL((trsv_kernel<R, false, NR, INV, BSX>), dim3(nblkx), block, 0, s, n, LU, lda, Y, X, DU,        \
+    hipLaunchKernelGGL((trsv_kernel<R, false, NR, INV, BSX>), dim3(nblkx), block, 0, s, n, LU, lda, Y, X, DU, MU,    \
                        (cplx<R>*)nullptr, X64, flags + 2, flags + 4, g_spin_limit);                                   \
     break;
   switch (nrhs) {
@@ -1377,6 +1482,14 @@ static int trsv_pair(int n, int nrhs, const cplx<R>* LU, int64_t lda, cplx<R>* X
     MOM_TRSV(2)
     MOM_TRSV(3)
     MOM_TRSV(4)
+  }
+  if constexpr (BSX == kTrsvBS) {  // 5 to 8 columns on 64-row blocks only
+    switch (nrhs) {
+      MOM_TRSV(5)
+      MOM_TRSV(6)
+      MOM_TRSV(7)
+      MOM_TRSV(8)
+    }
   }
 #undef MOM_TRSV
   return check_launch("trsv_kernel");
@@ -1389,10 +1502,10 @@ template <typename R, bool INV, typename TB>
 static int getrs(int n, int nrhs, const cplx<R>* LU, int64_t lda, const int32_t* perm, const cplx<TB>* B, cplx<R>* X,
                  z64* X64, unsigned* flags, hipStream_t s) {
   if (n <= 0 || nrhs <= 0) return 0;
-  if (nrhs > 4) return fail("mom_getrs: at most 4 right-hand sides");
+  if (nrhs > kMaxRhs) return fail("mom_getrs: at most 8 right-hand sides");
   const int nblk = (n + kTrsvBS - 1) / kTrsvBS;
   cplx<R>* Y = reinterpret_cast<cplx<R>*>(reinterpret_cast<unsigned char*>(flags) + 256);
-  if (X64) X = Y + 4 * (int64_t)std::max(n, 1);
+  if (X64) X = Y + kMaxRhs * (int64_t)std::max(n, 1);
   const int64_t nx = (int64_t)n * nrhs;
   hipLaunchKernelGGL((getrs_prep_kernel<R, TB>), dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, s, n, nrhs, perm,
                      B, X, Y, flags);
@@ -1405,14 +1518,16 @@ static int getrs(int n, int nrhs, const cplx<R>* LU, int64_t lda, const int32_t*
   // complex64 with inverted blocks and g_solve_rows = 128: the composed
   // 128-row inverses (half the chain's blocks)
   if constexpr (INV && inv128<R>()) {
-    if (g_solve_rows == kTrsvBS2) {
+    if (g_solve_rows == kTrsvBS2 && nrhs <= 4) {  // (more columns would spill at 1024 threads)
       const int nblk2 = (n + kTrsvBS2 - 1) / kTrsvBS2;
       const cplx<R>* D2L = DU + (int64_t)nblk * kTrsvBS * kTrsvBS;
       const cplx<R>* D2U = D2L + (int64_t)nblk2 * kTrsvBS2 * kTrsvBS2;
-      return trsv_pair<R, INV, kTrsvBS2>(n, nrhs, LU, lda, X, Y, D2L, D2U, X64, flags, s);
+      return trsv_pair<R, INV, kTrsvBS2>(n, nrhs, LU, lda, X, Y, D2L, D2U, nullptr, nullptr, X64, flags, s);
     }
   }
-  return trsv_pair<R, INV, kTrsvBS>(n, nrhs, LU, lda, X, Y, DL, DU, X64, flags, s);
+  const cplx<R>* ML = INV && g_chain ? chain_matrices<R>(flags, n) : nullptr;
+  const cplx<R>* MU = ML ? ML + (int64_t)nblk * kTrsvBS * kTrsvBS : nullptr;
+  return trsv_pair<R, INV, kTrsvBS>(n, nrhs, LU, lda, X, Y, DL, DU, ML, MU, X64, flags, s);
 }
 
 }  // namespace mom
@@ -1506,6 +1621,13 @@ extern "C" int mom_set_solve_rows(int rows) {
 }
 
 extern "C" int mom_get_solve_rows(void) { return g_solve_rows; }
+
+extern "C" int mom_set_solve_chain(int on) {
+  g_chain = on ? 1 : 0;
+  return 0;
+}
+
+extern "C" int mom_get_solve_chain(void) { return g_chain; }
 
 extern "C" int mom_set_spin_limit(unsigned polls) {
   if (polls == 0) return fail("mom_set_spin_limit: the limit must be positive");

@@ -20,7 +20,7 @@ into a closed-form leading part and a tabulated smooth remainder
 
 The splines are uploaded in the layout the fill kernel reads (16 doubles per
 interval: c0..c3 of S_A, S_V, re / im) together with a bucket table that
-turns the interval search into one lookup and a one-step fix-up.
+turns the interval search into one lookup and one comparison (bucket_table).
 """
 
 import numpy as np
@@ -50,6 +50,48 @@ def rho_grid(lam, h, rmax):
     while r[-1] < rmax:
         r.append(r[-1] + min(lam / 80, max(h / 25, r[-1] / 25)))
     return np.array(r)
+
+
+BUCKET_DTYPE = np.dtype([("split", "<f8"), ("i0", "<i4"), ("pad", "<i4")])
+
+
+def bucket_table(rho, dxb):
+    """Interval lookup of the fill kernels for the nodes rho: entry b serves
+    every distance R >= 0 with int(R * (1 / dxb)) == b in double arithmetic
+    (the last entry also every larger R), so its range [lo_b, lo_b+1) is
+    exact for the kernels' own rounding.  i0 is the spline interval of lo_b
+    and split the one node inside the range (+inf if none, or if crossing it
+    does not change the clamped interval); the interval of R is then
+    i0 + (R >= split), equal to scipy's clamped
+    searchsorted(rho, R, 'right') - 1.  dxb must be at most half the
+    smallest node spacing, which leaves at most one node per range."""
+    n = rho.size
+    inv = 1.0 / dxb
+    nb = int(rho[-1] * inv) + 2
+    b = np.arange(nb, dtype=np.float64)
+    lo = b * dxb  # then the smallest double with floor(lo * inv) >= b
+    for _ in range(64):
+        dn = np.nextafter(lo, -np.inf)
+        m = (np.floor(dn * inv) >= b) & (dn >= 0)
+        if not m.any():
+            break
+        lo = np.where(m, dn, lo)
+    for _ in range(64):
+        m = np.floor(lo * inv) < b
+        if not m.any():
+            break
+        lo = np.where(m, np.nextafter(lo, np.inf), lo)
+    hi = np.append(lo[1:], np.inf)
+    raw = np.searchsorted(rho, lo, side="right") - 1
+    j = raw + 1  # first node above lo
+    inside = np.searchsorted(rho, hi, side="left") - j
+    if np.any(inside > 1):
+        raise RuntimeError("bucket_table: a bucket holds more than one node (width above half the node spacing)")
+    t = np.zeros(nb, BUCKET_DTYPE)
+    t["i0"] = np.clip(raw, 0, n - 2)
+    moves = (inside >= 1) & (raw >= 0) & (j <= n - 2)
+    t["split"] = np.where(moves, rho[np.minimum(j, n - 1)], np.inf)
+    return t
 
 
 def spline_coeffs(x, Y):
@@ -410,9 +452,8 @@ class SweepTables:
             pairs = {}
             recs = np.zeros(L * L, PAIR_DTYPE)
             dxb = 0.5 * np.diff(rho).min()
-            nbucket = int(rho[-1] / dxb) + 2
-            edges = np.arange(nbucket) * dxb
-            bucket = np.clip(np.searchsorted(rho, edges, side="right") - 1, 0, rho.size - 2).astype(np.int32)
+            bucket = bucket_table(rho, dxb)
+            nbucket = bucket.size
             xs.append(rho)
             bks.append(bucket)
             for i in range(L):
@@ -449,7 +490,7 @@ class SweepTables:
         self.tab_x = torch.from_numpy(np.concatenate(xs)).to(dev)
         self.tab_c = torch.from_numpy(np.concatenate(cs)).to(dev)
         self.tab_cw = torch.from_numpy(np.concatenate(cws)).to(dev)
-        self.tab_b = torch.from_numpy(np.concatenate(bks)).to(dev)
+        self.tab_b = torch.from_numpy(np.concatenate(bks).view(np.uint8).copy()).to(dev)  # BUCKET_DTYPE entries
         allrec = np.concatenate(pair_rows)
         self._pairs_blob = torch.from_numpy(allrec.view(np.uint8).copy()).to(dev)
         rec_bytes = PAIR_DTYPE.itemsize * L * L

@@ -338,8 +338,18 @@ class Solver:
                 factorisation (costs in GMRES steps; one factorisation counts
                 lu_equiv steps, by default a model of this implementation:
                 LU time / step time at this N).  lu_stats counts
-                factorisations, reuses and GMRES steps.
-    method    : "auto" / "dense" (the H-matrix solver is not part of this build).
+                factorisations, reuses, GMRES steps and complex128 LUs.
+    lu64_steps: mixed precision, in a sweep: when the solve of a point with
+                its own fresh complex64 LU needs more than lu64_steps GMRES
+                steps, the complex64 factors have lost too much (the
+                low-frequency end of a sweep, where the charge term of the
+                MPIE outweighs the current-loop term by many orders and
+                complex64 keeps too few digits of the latter): that point is
+                factored again in complex128 to precondition its neighbours,
+                and new anchors up to twice its frequency are complex128
+                LUs (GMRES then needs ~10-20 steps per reused point instead of
+                50-120).  None: complex64 anchors only.
+    method    :"auto" / "dense" (the H-matrix solver is not part of this build).
     surrogate : a trained GreensSurrogate of this stack: the remainders of
                 the Green's tables come from its networks, corrected and
                 judged per frequency by the Sommerfeld integrals at a few
@@ -368,7 +378,7 @@ class Solver:
     def __init__(self, mesh, substrate, z0=None, verbose=True, method="auto", v_weight=None, z0_recip_min=None,
                  tand_floor=None, precision="fp64", device="cuda", panel=None, refine_tol=1e-15, refine_maxit=20,
                  lanes=None, surrogate=None, surrogate_tol=2e-3, workers=None, graphs=False, autotune=False,
-                 gmres_restart=120, reuse_lu=True, lu_equiv=None):
+                 gmres_restart=120, reuse_lu=True, lu_equiv=None, lu64_steps=20):
         from .layered import Stackup
 
         if method not in ("auto", "dense"):
@@ -434,7 +444,9 @@ class Solver:
         self.lu_equiv = lu_equiv
         self._anchor = {}
         self._history = {}
-        self.lu_stats = dict(factorisations=0, reused=0, fallbacks=0, gmres_steps=0)
+        self.lu64_steps = lu64_steps
+        self._f_poor = 0.0
+        self.lu_stats = dict(factorisations=0, reused=0, fallbacks=0, gmres_steps=0, fp64_lu=0)
         self.lanes = lanes
         self.workers = workers
         self.graphs = graphs
@@ -725,18 +737,18 @@ class Solver:
             return X
         return (yield from self._gmres_chunks(Z, A, perm, sw, B, lane))
 
-    def _gmres_chunks(self, Z, A32, perm, sw, B, lane=0, max_its=None, X0=None):
+    def _gmres_chunks(self, Z, A32, perm, sw, B, lane=0, max_its=None, X0=None, pc_prec=32):
         """_gmres_ir over groups of at most 4 right-hand sides (the solve
         kernels' limit); None if a group does not converge within max_its.
         X0: an initial guess of the solution (or None)."""
         nrhs = B.shape[1]
         if nrhs <= 4:
-            return (yield from self._gmres_ir(Z, A32, perm, sw, B, lane, max_its, X0))
+            return (yield from self._gmres_ir(Z, A32, perm, sw, B, lane, max_its, X0, pc_prec))
         X = torch.empty_like(B)
         its = 0
         for c0, c1 in _column_chunks(nrhs):
             Xc = yield from self._gmres_ir(Z, A32, perm, sw, B[:, c0:c1].contiguous(), lane, max_its,
-                                           None if X0 is None else X0[:, c0:c1].contiguous())
+                                           None if X0 is None else X0[:, c0:c1].contiguous(), pc_prec)
             its += self.last_refine
             if Xc is None:
                 self.last_refine = its
@@ -750,10 +762,10 @@ class Solver:
         after the solve's result has been copied to the host): raise if a
         bounded wait of the LU kernels ran out, warn on an exactly zero
         pivot (as LAPACK's zgetrf reports; the solution is then not finite)."""
-        prec = 64 if self.precision == "fp64" else 32
         info = self._bufs[("info", lane)].cpu().numpy()
-        sw = self._bufs[(f"swork{prec}", lane)][:20].cpu().numpy().view(np.uint32)
-        if info[1] or sw[4]:
+        waits = [self._bufs[(k, lane)][:20].cpu().numpy().view(np.uint32)[4]
+                 for k in ("swork32", "swork64", "swork64i") if (k, lane) in self._bufs]
+        if info[1] or any(waits):
             raise RuntimeError("LU kernels: a bounded wait ran out (workgroups did not all run); result invalid")
         if info[0]:
             warnings.warn(f"LU: diagonal number {int(info[0])} is exactly zero (singular matrix)", RuntimeWarning,
@@ -767,7 +779,7 @@ class Solver:
         cus = torch.cuda.get_device_properties(self.device).multi_processor_count
         return max(1, cus // lanes)
 
-    def _gmres_ir(self, Z, A32, perm, sw, B, lane=0, max_its=None, X0=None):
+    def _gmres_ir(self, Z, A32, perm, sw, B, lane=0, max_its=None, X0=None, pc_prec=32):
         """Restarted GMRES in FP64 on the diagonally scaled system
         (D Z D) y = D b, x = D y, right-preconditioned by the complex64 LU of
         D Z D.  All right-hand sides advance together (one Krylov basis per
@@ -778,7 +790,8 @@ class Solver:
         returns None once that many steps have not reached the target (the
         caller then factors this matrix itself).  X0: an initial guess of x;
         the iteration then starts from y0 + M (D b - D Z D y0), y0 = D^-1 X0,
-        instead of M D b."""
+        instead of M D b.  pc_prec: 64 when A32 holds a complex128 LU of
+        D Z D instead (with its inverted diagonal blocks in sw)."""
         n, nrhs = B.shape
         P = _lib.ptr
         stream = _lib.stream_ptr()
@@ -790,7 +803,8 @@ class Solver:
 
         def precond(V, out=None):  # complex64 LU solve of complex128 vectors
             out = torch.empty_like(V) if out is None else out
-            _lib.call("mom_getrs_inv_z", 32, n, nrhs, P(A32), A32.stride(0), P(perm), P(V), P(out), P(sw), stream)
+            _lib.call("mom_getrs_inv_z", pc_prec, n, nrhs, P(A32), A32.stride(0), P(perm), P(V), P(out), P(sw),
+                      stream)
             return out
 
         def matvec(Y):  # (D Z D) Y
@@ -965,12 +979,15 @@ class Solver:
             if Z32x is not None:
                 self._scale_copy(Z, Z32x, lane)
 
-        a = self._anchor.get(lane) if mixed and reuse and self.reuse_lu else None
+        sweep = mixed and reuse and self.reuse_lu
+        a = self._anchor.get(lane) if sweep else None
+        filled = False
         if a is not None and self._reuse_next(a, g.freq):
             fill(None)
+            filled = True
             cap = max(20, int(a["lu_equiv"] + a["its0"]))
-            X = yield from self._gmres_chunks(Z, Z32, self._bufs[("perm", lane)], self._bufs[("swork32", lane)], B,
-                                              lane, max_its=cap, X0=self._extrapolate(lane, g.freq, B))
+            X = yield from self._gmres_chunks(Z, a["A"], a["perm"], a["sw"], B, lane, max_its=cap,
+                                              X0=self._extrapolate(lane, g.freq, B), pc_prec=a["prec"])
             if X is not None:
                 self._remember(lane, g.freq, X)
                 a["cost"] += self.last_refine
@@ -979,18 +996,72 @@ class Solver:
                 self.lu_stats["reused"] += 1
                 return X
             self.lu_stats["fallbacks"] += 1
+        if sweep and g.freq <= 2 * self._f_poor:
+            # a band where complex64 factors precondition poorly: complex128 anchor
+            if not filled:
+                fill(None)
+            A, perm, sw = self._lu64_factor(Z, lane)
+            X = yield from self._gmres_chunks(Z, A, perm, sw, B, lane, pc_prec=64)
+            self._remember(lane, g.freq, X)
+            self.lu_stats["factorisations"] += 1
+            self._set_anchor(lane, g.freq, 64, A, perm, sw)
+            return X
+        if filled:
             self._scale_copy(Z, Z32, lane)
         else:
             fill(Z32)
         X = yield from self._lu_solve(Z, B, Z32, lane, graph)
         if mixed:
-            if reuse:
-                self._remember(lane, g.freq, X)
             self.lu_stats["factorisations"] += 1
-            eq = self._lu_equiv()
-            self._anchor[lane] = dict(freq=g.freq, lu_equiv=eq, its0=self.last_refine, cost=eq + self.last_refine,
-                                      n=1, last=None)
+            if sweep:
+                self._remember(lane, g.freq, X)
+                if self.lu64_steps is not None and self.last_refine > self.lu64_steps:
+                    # the complex64 factors lost too much here (its own solve took many
+                    # steps): factor this point again in complex128 for its neighbours
+                    self._f_poor = max(self._f_poor, g.freq)
+                    self._set_anchor(lane, g.freq, 64, *self._lu64_factor(Z, lane))
+                else:
+                    self._set_anchor(lane, g.freq, 32, Z32, self._bufs[("perm", lane)],
+                                     self._bufs[("swork32", lane)], self.last_refine)
         return X
+
+    def _set_anchor(self, lane, freq, prec, A, perm, sw, its0=0):
+        """The lane's factorisation that preconditions its next points."""
+        eq = self._lu_equiv()
+        self._anchor[lane] = dict(freq=freq, prec=prec, A=A, perm=perm, sw=sw, lu_equiv=eq, its0=its0,
+                                  cost=eq + its0, n=1, last=None)
+
+    def _lu64_factor(self, Z, lane):
+        """complex128 LU of D Z D (D = diag(|Z_ii|^-1/2), into the lane's
+        dscale) with the inverted diagonal blocks of its factors, enqueued on
+        the current stream; returns (LU, perm, solve workspace) for the
+        preconditioner of the mixed path's GMRES (pc_prec = 64).  Z must be
+        symmetric (as the fill leaves it)."""
+        n = Z.shape[0]
+        P = _lib.ptr
+        stream = _lib.stream_ptr()
+        L = _lib.lib()
+        dsc = self._buf("dscale", (n,), torch.float64, lane)
+        _lib.call("mom_diag_scale", n, P(Z), Z.stride(0), P(dsc), stream)
+        A = self._buf("A64", (n, n), torch.complex128, lane)
+        torch.mul(Z, dsc[:, None], out=A)
+        A.mul_(dsc[None, :])
+        ipiv = self._buf("ipiv64", (n,), torch.int32, lane)
+        perm = self._buf("perm64", (n,), torch.int32, lane)
+        info = self._buf("info", (4,), torch.int32, lane)
+        groups = self._panel_groups()
+        jb = self.panel  # (tuned for complex64) halved until the panel's rows fit the LDS of `groups` workgroups
+        while groups and jb > 16 and -(-n // max(1, self.PANEL_LDS // (16 * jb))) > groups:
+            jb //= 2
+        wbytes = int(L.mom_getrf_work_bytes(64, n, jb))
+        work = self._buf("work64", (wbytes,), torch.uint8, lane)
+        sw = self._buf("swork64i", (int(L.mom_getrs_inv_work_bytes(64, n)),), torch.uint8, lane)
+        sw[:256].zero_()  # status words (a solve clears them too; _check_solve may read them before one runs)
+        _lib.call("mom_getrf", 64, n, P(A), A.stride(0), P(ipiv), P(info), P(work), wbytes, jb, groups, stream)
+        _lib.call("mom_build_perm", n, P(ipiv), P(perm), stream)
+        _lib.call("mom_trtri_blocks", 64, n, P(A), A.stride(0), P(sw), stream)
+        self.lu_stats["fp64_lu"] += 1
+        return A, perm, sw
 
     def _remember(self, lane, freq, X):
         """Keep the lane's last three solutions (device) for _extrapolate."""
@@ -1041,8 +1112,10 @@ class Solver:
     def _reuse_next(a, freq):
         """Reuse the anchor's LU for freq: always once, then while the last
         reuse cost fewer steps than the average per point since the
-        factorisation; never beyond half an octave from the anchor."""
-        if abs(freq - a["freq"]) > 0.5 * a["freq"]:
+        factorisation; a complex64 LU never beyond half an octave from the
+        anchor (a complex128 one preconditions several octaves well: its
+        step count grows slowly, and the cost rule ends its reuse)."""
+        if a["prec"] == 32 and abs(freq - a["freq"]) > 0.5 * a["freq"]:
             return False
         return a["last"] is None or a["last"] < a["cost"] / a["n"]
 
@@ -1317,6 +1390,7 @@ def _sweep_many(jobs, lanes, workers, keep_currents=False, graphs=False):
     for s, *_ in jobs:
         s._anchor = {}  # no factorisation or solution carries over from an earlier call
         s._history = {}
+        s._f_poor = 0.0
     torch.cuda.synchronize(device)
     t1 = time.time()
     pool = fit_pool(workers)

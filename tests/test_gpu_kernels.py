@@ -137,7 +137,7 @@ def _lu_solve(A, B, prec, jb, status=False):
 
 
 @pytest.mark.parametrize("n,jb,nrhs", [(1, 8, 1), (37, 8, 3), (200, 32, 2), (777, 64, 4), (1500, 64, 2),
-                                       (2049, 64, 1)])
+                                       (2049, 64, 1), (129, 32, 6), (300, 32, 8)])
 def test_lu_fp64_matches_lapack(n, jb, nrhs):
     import scipy.linalg as sla
 
@@ -210,19 +210,36 @@ def test_bounded_waits_report_failure():
 
 
 @pytest.mark.parametrize("prec,n,nrhs,rows", [(32, 1, 1, 64), (32, 300, 2, 64), (32, 2049, 4, 64), (64, 777, 3, 64),
-                                            (32, 1, 1, 128), (32, 300, 2, 128), (32, 2049, 4, 128), (32, 130, 3, 128)])
+                                            (32, 1, 1, 128), (32, 300, 2, 128), (32, 2049, 4, 128), (32, 130, 3, 128),
+                                            (32, 1000, 8, 64), (64, 500, 5, 64), (32, 700, 7, 128)])
 def test_getrs_with_inverted_blocks(prec, n, nrhs, rows):
     """The solve with explicitly inverted diagonal blocks (64 rows, or in
     complex64 128 rows composed from them): the same solution as the
     substitution solve up to cond(block) rounding, and in complex128 a
     backward error near round-off (a diagonally dominant matrix, so the
     blocks are well conditioned); the inverses are reused by a second solve
-    with new right-hand sides."""
+    with new right-hand sides.  Up to 8 right-hand sides; with more than 4
+    the 128-row setting solves on 64-row blocks."""
     _lib.call("mom_set_solve_rows", rows)
     try:
         _getrs_inv_case(prec, n, nrhs)
     finally:
         _lib.call("mom_set_solve_rows", 64)
+
+
+@pytest.mark.parametrize("prec,n,nrhs", [(32, 2049, 2), (64, 777, 3), (32, 1000, 8), (32, 65, 1), (64, 130, 6)])
+def test_getrs_inv_chain_and_plain_forms(prec, n, nrhs):
+    """Both forms of the 64-row inverted-block solve -- the chain form
+    x_k = y_k - M_k x_prev (y_k formed before the previous block arrives)
+    and the plain x_k = D_k^-1 (b_k - all couplings) -- meet the bars of the
+    substitution solve, with partial last blocks (65 = 64 + 1 rows)."""
+    try:
+        for chain in (0, 1):
+            _lib.call("mom_set_solve_chain", chain)
+            assert _lib.lib().mom_get_solve_chain() == chain
+            _getrs_inv_case(prec, n, nrhs)
+    finally:
+        _lib.call("mom_set_solve_chain", 1)
 
 
 def _getrs_inv_case(prec, n, nrhs):

@@ -77,24 +77,29 @@ def test_lanes_match_single_lane(precision):
     assert np.max(np.abs(r1.S - r3.S)) < 1e-7
 
 
-def test_lu_reuse_matches_fresh_factorisations():
-    """Mixed precision with each lane's complex64 LU reused as the
-    preconditioner of its next frequency points: the FP64 GMRES reaches the
-    same tolerance, so the currents agree with a factorisation per point to
-    round-off of the refinement and S to the wave fit's noise; the policy did
-    reuse factorisations (and took fewer than one per point)."""
+@pytest.mark.parametrize("lu64_steps", [20, 0])
+def test_lu_reuse_matches_fresh_factorisations(lu64_steps):
+    """Mixed precision with each lane's LU reused as the preconditioner of
+    its next frequency points: the FP64 GMRES reaches the same tolerance, so
+    the currents agree with a factorisation per point to round-off of the
+    refinement and S to the wave fit's noise; the policy did reuse
+    factorisations (and took fewer than one per point).  lu64_steps=0 makes
+    every anchor after the first complex64 solve a complex128 LU (the path of
+    the low-frequency end of large sweeps)."""
     lay, sub, mk, freqs = W.tee()
     m = lay.mesh(**mk)
     f = freqs[:12]
     fresh = Solver(m, sub, verbose=False, precision="mixed", lanes=2, reuse_lu=False)
     r0 = fresh.sweep(f, keep_currents=True, interp=None)
-    s = Solver(m, sub, verbose=False, precision="mixed", lanes=2)
+    s = Solver(m, sub, verbose=False, precision="mixed", lanes=2, lu64_steps=lu64_steps)
     r1 = s.sweep(f, keep_currents=True, interp=None)
     st = s.lu_stats
     print("reuse stats", st, "fresh", fresh.lu_stats)
     assert fresh.lu_stats["factorisations"] == len(f) and fresh.lu_stats["reused"] == 0
+    assert fresh.lu_stats["fp64_lu"] == 0
     assert st["reused"] > 0 and st["factorisations"] + st["reused"] - st["fallbacks"] == len(f)
     assert st["factorisations"] < len(f)
+    assert (st["fp64_lu"] > 0) == (lu64_steps == 0)
     for a, b in zip(r0.currents, r1.currents):
         assert np.max(np.abs(a - b)) <= 1e-9 * np.max(np.abs(a))
     assert np.max(np.abs(r0.S - r1.S)) < 1e-7

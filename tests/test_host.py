@@ -10,7 +10,7 @@ import pytest
 from scipy.interpolate import CubicSpline
 
 from mom25d_amd import workloads as W
-from mom25d_amd.greens import PAIR_DTYPE, ppoly_eval, rho_grid, spline_coeffs
+from mom25d_amd.greens import BUCKET_DTYPE, PAIR_DTYPE, bucket_table, ppoly_eval, rho_grid, spline_coeffs
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -43,6 +43,35 @@ def test_rho_grid_matches_reference(golden):
     for f in (1e9, 7.3e9, 20e9):
         g = SlabGreens(f, 3.55, 0.508e-3, 0.0027, rho_max=0.05)
         np.testing.assert_array_equal(rho_grid(2 * np.pi / abs(g.k1), 0.508e-3, 0.05), t[f"slab_{f / 1e9:g}_rho"])
+
+
+@pytest.mark.parametrize("grid", ["slab1", "slab20", "offset", "aligned"])
+def test_bucket_table_reproduces_scipy_intervals(grid):
+    """The fill kernels' interval lookup -- b = (int)(R * inv_dx) clamped,
+    then i0 + (R >= split) of entry b, emulated here in the same double
+    arithmetic -- equals scipy's clamped searchsorted(rho, R, 'right') - 1 at
+    random distances, at every node and every bucket edge and their
+    neighbouring doubles, and beyond both ends of the grid."""
+    rng = np.random.default_rng(3)
+    if grid == "offset":  # uneven grid that starts above 0
+        rho = np.cumsum(rng.uniform(0.5, 3.0, 400)) * 1e-5 + 3e-6
+    elif grid == "aligned":  # nodes within rounding of bucket edges
+        rho = np.arange(400) * (1e-4 / 3)
+    else:
+        f = 1e9 if grid == "slab1" else 20e9
+        rho = rho_grid(2 * np.pi / (2 * np.pi * f / 3e8 * np.sqrt(3.55)), 0.508e-3, 0.05)
+    dxb = 0.5 * np.diff(rho).min()
+    t = bucket_table(rho, dxb)
+    assert t.dtype == BUCKET_DTYPE and BUCKET_DTYPE.itemsize == 16
+    inv = 1.0 / dxb
+    edges = np.arange(t.size) * dxb
+    pts = [rng.uniform(0, 1.3 * rho[-1], 20000), rho, edges, [0.0, 2 * rho[-1]]]
+    R = np.concatenate(pts)
+    R = np.concatenate([R, np.nextafter(R, np.inf), np.nextafter(R, 0)])
+    b = np.clip((R * inv).astype(np.int64), 0, t.size - 1)
+    got = t["i0"][b] + (R >= t["split"][b])
+    want = np.clip(np.searchsorted(rho, R, side="right") - 1, 0, rho.size - 2)
+    np.testing.assert_array_equal(got, want)
 
 
 @pytest.mark.parametrize("name", ["tee", "via2"])
