@@ -94,6 +94,19 @@ int mom_line_voltage(int npts, const double* pts, int li, int nl, int nt, int nr
 int64_t mom_getrf_work_bytes(int prec, int n, int jb);
 int mom_getrf(int prec, int n, void* A, int64_t lda, int32_t* ipiv, int32_t* info, void* work, int64_t work_bytes,
               int jb, int max_groups, void* stream);
+/* The same factorisation without row interchanges (ipiv[k] = k), for the
+ * preconditioner of the mixed-precision GMRES (no counterpart in the
+ * reference, which factors with LAPACK's partial pivoting only).  A panel is
+ * then its diagonal block factored by one workgroup plus an independent solve
+ * per row below it, with no cross-workgroup exchange per column, and the
+ * factorisation is bounded by its update GEMM.  On the diagonally scaled MoM
+ * matrices the element growth stays near 2.  An exactly zero pivot is
+ * replaced by 1 and reported in info[0] (the factors are then finite but not
+ * those of A).  Panel width min(jb, 128) in complex64, min(jb, 64) in
+ * complex128 inside update blocks of width jb.  No max_groups: every launch
+ * runs to completion by itself. */
+int mom_getrf_nopiv(int prec, int n, void* A, int64_t lda, int32_t* ipiv, int32_t* info, void* work,
+                    int64_t work_bytes, int jb, void* stream);
 
 /* Workgroup tile (rows x real columns) of the trailing-update GEMM of
  * mom_getrf and mom_cgemm_sub, per precision: 64x64, 64x128, 128x64 or

@@ -296,6 +296,120 @@ __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<R> a) {
   }
 }
 
+// ------------------------------------------------- unpivoted panel
+template <typename R>
+static constexpr int kNoPivPanelMax() {  // widest panel whose diagonal block fits one workgroup's LDS
+  return sizeof(R) == sizeof(double) ? 64 : 128;
+}
+
+// The panel of the LU without row interchanges (mom_getrf_nopiv: the
+// preconditioner factorisation of the mixed path, where GMRES absorbs the
+// difference): no column needs a cross-workgroup pivot search, so a panel is
+// two ordinary launches -- the b x b diagonal block factored by one workgroup
+// in LDS, then the rows below it, L21 = A21 U11^-1, by independent
+// workgroups of 64 rows -- instead of one exchange per column.
+//
+// Both use one barrier per column: thread (c, q) updates column c of the
+// rows of its class q (rows q, q + 8, ...); in step j the threads of each
+// column c > j subtract a_ij (u_jc / u_jj), with the unscaled a_ij, from
+// their rows below j, and column j is scaled by 1 / u_jj when no one reads
+// it any more (the diagonal block, held in LDS, in step j + 1; the 64-row
+// tiles below, held in registers, publish column j to LDS first).  An exactly
+// zero pivot is replaced by 1 and reported in info[0] (first one, 1-based),
+// which leaves finite factors that the caller discards.
+template <typename R>
+__device__ inline cplx<R> nonzero_pivot(cplx<R> p) {
+  return (p.re == (R)0 && p.im == (R)0) ? cplx<R>{(R)1, (R)0} : p;
+}
+
+template <typename R>
+__global__ __launch_bounds__(1024) void diag_lu_kernel(cplx<R>* A, int64_t lda, int k0, int b, int32_t* ipiv,
+                                                       int32_t* info) {
+  // the block in LDS; thread (c, q) updates column c of the rows i = q mod 8,
+  // 8 rows at a time (their loads issued together)
+  extern __shared__ unsigned char smem_raw[];
+  cplx<R>* T = reinterpret_cast<cplx<R>*>(smem_raw);  // [b][b]
+  const int tid = threadIdx.x, c = tid % 128;
+  const int q = __builtin_amdgcn_readfirstlane(tid / 128);  // uniform per wavefront
+  for (int e = tid; e < b * b; e += 1024) T[e] = A[(int64_t)(k0 + e / b) * lda + k0 + e % b];
+  for (int e = tid; e < b; e += 1024) ipiv[k0 + e] = k0 + e;
+  __syncthreads();
+  for (int j = 0; j < b; ++j) {
+    const cplx<R> p = T[j * b + j];  // final since step j - 1
+    if (j > 0) {  // finish column j - 1: its rows below the diagonal scaled, one row per thread
+      const cplx<R> pp = T[(j - 1) * b + j - 1];
+      if (j + tid < b) T[(j + tid) * b + j - 1] = T[(j + tid) * b + j - 1] * cdiv(cplx<R>{(R)1, (R)0},
+                                                                                     nonzero_pivot(pp));
+      if (tid == 0) T[(j - 1) * b + j - 1] = nonzero_pivot(pp);
+    }
+    if (tid == 0 && p.re == (R)0 && p.im == (R)0 && *info == 0) *info = k0 + j + 1;
+    if (c > j && c < b) {
+      const cplx<R> m = cdiv(T[j * b + c], nonzero_pivot(p));
+      const int first = j + 1 + ((q - (j + 1)) & 7);  // first row > j of this class
+      for (int i0 = first; i0 < b; i0 += 64) {
+        cplx<R> a[8], t[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const int i = min(i0 + 8 * k, b - 1);
+          a[k] = T[i * b + j];
+          t[k] = T[i * b + c];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          if (i0 + 8 * k < b) T[(i0 + 8 * k) * b + c] = t[k] - a[k] * m;
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) T[(b - 1) * b + b - 1] = nonzero_pivot(T[(b - 1) * b + b - 1]);
+  __syncthreads();
+  for (int e = tid; e < b * b; e += 1024) A[(int64_t)(k0 + e / b) * lda + k0 + e % b] = T[e];
+}
+
+// L21 = A21 U11^-1 for the rows [k0 + b, n) below a factored diagonal block,
+// 64 rows per workgroup (thread (c, q): column c of the rows i = q mod 8, in
+// registers), U11 staged in LDS.
+template <typename R>
+__global__ __launch_bounds__(1024) void panel_below_kernel(cplx<R>* A, int64_t lda, int n, int k0, int b) {
+  constexpr int RT = 64, HR = RT / 8;
+  extern __shared__ unsigned char smem_raw[];
+  cplx<R>* U = reinterpret_cast<cplx<R>*>(smem_raw);  // [b][b]: U11 (its lower part unused)
+  __shared__ cplx<R> colbuf[2][RT];
+  const int tid = threadIdx.x, c = tid % 128;
+  const int q = __builtin_amdgcn_readfirstlane(tid / 128);
+  const bool mine = c < b;
+  const int r0 = k0 + b + blockIdx.x * RT, nr = min(RT, n - r0);
+  for (int e = tid; e < b * b; e += 1024) U[e] = A[(int64_t)(k0 + e / b) * lda + k0 + e % b];
+  cplx<R> v[HR];
+#pragma unroll
+  for (int t = 0; t < HR; ++t) {
+    const int i = 8 * t + q;
+    v[t] = (mine && i < nr) ? A[(int64_t)(r0 + i) * lda + k0 + c] : cplx<R>{0, 0};
+  }
+  for (int j = 0; j < b; ++j) {
+    const int buf = j & 1;
+    if (c == j) {
+#pragma unroll
+      for (int t = 0; t < HR; ++t) colbuf[buf][8 * t + q] = v[t];
+    }
+    __syncthreads();
+    const cplx<R> inv = cdiv(cplx<R>{(R)1, (R)0}, nonzero_pivot(U[j * b + j]));
+    if (mine && c > j) {
+      const cplx<R> m = U[j * b + c] * inv;
+#pragma unroll
+      for (int t = 0; t < HR; ++t) v[t] -= colbuf[buf][8 * t + q] * m;
+    } else if (c == j) {
+#pragma unroll
+      for (int t = 0; t < HR; ++t) v[t] = v[t] * inv;
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < HR; ++t) {
+    const int i = 8 * t + q;
+    if (mine && i < nr) A[(int64_t)(r0 + i) * lda + k0 + c] = v[t];
+  }
+}
+
 // ----------------------------------------------------------------- laswp
 // Row interchanges ipiv[k0 .. k0+b) applied, in order, to columns [c0, c1)
 // of A.  The first wavefront folds the b interchanges into one permutation
@@ -1140,6 +1254,38 @@ static int update_block(cplx<R>* A, int64_t lda, int n, int k0, int b, int r0, i
   return 0;
 }
 
+// The unpivoted panel of columns [k0, k0 + b) (b <= kNoPivPanelMax): its
+// diagonal block, then the rows below.
+template <typename R>
+static int launch_panel_nopiv(cplx<R>* A, int64_t lda, int n, int k0, int b, int32_t* ipiv, int32_t* info,
+                              hipStream_t s) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    const size_t most = sizeof(cplx<R>) * (size_t)kNoPivPanelMax<R>() * kNoPivPanelMax<R>();
+    if (hipFuncSetAttribute((const void*)diag_lu_kernel<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)most) !=
+        hipSuccess)
+      return fail("diag_lu_kernel: cannot reserve LDS");
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(diag_lu_kernel<R>, dim3(1), dim3(1024), sizeof(cplx<R>) * (size_t)b * b, s, A, lda, k0, b, ipiv,
+                     info);
+  int rc = check_launch("diag_lu_kernel");
+  if (rc) return rc;
+  const int below = n - k0 - b;
+  if (below <= 0) return 0;
+  static bool attr2_set = false;
+  if (!attr2_set) {
+    const size_t most = sizeof(cplx<R>) * (size_t)kNoPivPanelMax<R>() * kNoPivPanelMax<R>();
+    if (hipFuncSetAttribute((const void*)panel_below_kernel<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)most) != hipSuccess)
+      return fail("panel_below_kernel: cannot reserve LDS");
+    attr2_set = true;
+  }
+  hipLaunchKernelGGL(panel_below_kernel<R>, dim3((below + 63) / 64), dim3(1024), sizeof(cplx<R>) * (size_t)b * b, s,
+                     A, lda, n, k0, b);
+  return check_launch("panel_below_kernel");
+}
+
 struct PanelWork {
   PivotSlot* slots;
   void* rowbuf;
@@ -1151,16 +1297,20 @@ struct PanelWork {
 // columns and the block's remaining columns are updated.
 template <typename R>
 static int factor_block(cplx<R>* A, int64_t lda, int n, int k0, int nb, int jb, int32_t* ipiv, int32_t* info,
-                        const PanelWork& pw, hipStream_t s) {
+                        const PanelWork& pw, bool pivot, hipStream_t s) {
   int rc;
   const int kend = k0 + nb;
   for (int q0 = k0; q0 < kend; q0 += jb) {
     const int bq = std::min(jb, kend - q0), q1 = q0 + bq;
-    if ((rc = launch_panel<R>(A, lda, n, q0, bq, ipiv, info, pw.slots, (cplx<R>*)pw.rowbuf, pw.gmax, s)))
+    if (pivot) {
+      if ((rc = launch_panel<R>(A, lda, n, q0, bq, ipiv, info, pw.slots, (cplx<R>*)pw.rowbuf, pw.gmax, s)))
+        return rc;
+      if ((rc = laswp_block<R>(A, lda, n, k0, q0, q0, bq, ipiv, s))) return rc;
+    } else if ((rc = launch_panel_nopiv<R>(A, lda, n, q0, bq, ipiv, info, s))) {
       return rc;
-    if ((rc = laswp_block<R>(A, lda, n, k0, q0, q0, bq, ipiv, s))) return rc;
+    }
     if (q1 < kend) {
-      if ((rc = laswp_block<R>(A, lda, n, q1, kend, q0, bq, ipiv, s))) return rc;
+      if (pivot && (rc = laswp_block<R>(A, lda, n, q1, kend, q0, bq, ipiv, s))) return rc;
       if ((rc = update_block<R>(A, lda, n, q0, bq, q1, q1, kend, s))) return rc;
     }
   }
@@ -1172,7 +1322,7 @@ static int factor_block(cplx<R>* A, int64_t lda, int n, int k0, int nb, int jb, 
 // were updated first) is factored on the side stream.
 template <typename R>
 static int getrf(int n, cplx<R>* A, int64_t lda, int32_t* ipiv, int32_t* info, void* work, size_t work_bytes, int jb,
-                 int max_groups, hipStream_t s) {
+                 int max_groups, hipStream_t s, bool pivot = true) {
   if (n <= 0) return 0;
   if (jb <= 0 || jb > 128 || (jb % 8) != 0) return fail("mom_getrf: panel width must be a multiple of 8 in [8, 128]");
   if (work_bytes < getrf_work_bytes<R>(n, jb)) return fail("mom_getrf: workspace too small");
@@ -1180,6 +1330,9 @@ static int getrf(int n, cplx<R>* A, int64_t lda, int32_t* ipiv, int32_t* info, v
   if (ncu <= 0) return fail("mom_getrf: no device");
   int nb = g_lu_nb[sizeof(R) == sizeof(double) ? 0 : 1];
   nb = nb <= 0 ? jb : std::max(jb, nb - nb % jb);
+  // unpivoted: the diagonal block of a panel lives in one workgroup's LDS
+  // (64 columns in complex128); the update block keeps its width
+  const int jp = pivot ? jb : std::min(jb, kNoPivPanelMax<R>());
   // slot arrays hold 256 workgroups
   const int gmax = std::min(std::min(ncu, 256), max_groups > 0 ? max_groups : 256);
   unsigned char* w = reinterpret_cast<unsigned char*>(work);
@@ -1200,20 +1353,20 @@ static int getrf(int n, cplx<R>* A, int64_t lda, int32_t* ipiv, int32_t* info, v
   if ((rc = lookahead_for(s, 2 * (size_t)nblock + 2, &la))) return rc;
   hipStream_t side = la->side;
   // block 0 on the caller's stream
-  if ((rc = factor_block<R>(A, lda, n, 0, std::min(nb, n), jb, ipiv, info, pw, s))) return rc;
+  if ((rc = factor_block<R>(A, lda, n, 0, std::min(nb, n), jp, ipiv, info, pw, pivot, s))) return rc;
   for (int p = 0; p < nblock; ++p) {
     const int k0 = p * nb, b = std::min(nb, n - k0);
     const int c0 = k0 + b;
     if (p > 0) MOM_TRY(hipStreamWaitEvent(s, la->ev[2 * p], 0));  // block p done on the side stream
-    if ((rc = laswp_block<R>(A, lda, n, 0, k0, k0, b, ipiv, s))) return rc;
+    if (pivot && (rc = laswp_block<R>(A, lda, n, 0, k0, k0, b, ipiv, s))) return rc;
     if (c0 >= n) break;
-    if ((rc = laswp_block<R>(A, lda, n, c0, n, k0, b, ipiv, s))) return rc;
+    if (pivot && (rc = laswp_block<R>(A, lda, n, c0, n, k0, b, ipiv, s))) return rc;
     const int b1 = std::min(nb, n - c0);
     // next block's columns first, then hand the block to the side stream
     if ((rc = update_block<R>(A, lda, n, k0, b, c0, c0, c0 + b1, s))) return rc;
     MOM_TRY(hipEventRecord(la->ev[2 * p + 1], s));
     MOM_TRY(hipStreamWaitEvent(side, la->ev[2 * p + 1], 0));
-    if ((rc = factor_block<R>(A, lda, n, c0, b1, jb, ipiv, info, pw, side))) return rc;
+    if ((rc = factor_block<R>(A, lda, n, c0, b1, jp, ipiv, info, pw, pivot, side))) return rc;
     MOM_TRY(hipEventRecord(la->ev[2 * (p + 1)], side));
     // the rest of the trailing matrix
     if ((rc = update_block<R>(A, lda, n, k0, b, c0, c0 + b1, n, s))) return rc;
@@ -1546,6 +1699,17 @@ extern "C" int mom_getrf(int prec, int n, void* A, int64_t lda, int32_t* ipiv, i
   if (prec == 64) return getrf<double>(n, (z64*)A, lda, ipiv, info, work, (size_t)work_bytes, jb, max_groups, s);
   if (prec == 32) return getrf<float>(n, (c32*)A, lda, ipiv, info, work, (size_t)work_bytes, jb, max_groups, s);
   return fail("mom_getrf: prec must be 64 (complex128) or 32 (complex64)");
+}
+
+extern "C" int mom_getrf_nopiv(int prec, int n, void* A, int64_t lda, int32_t* ipiv, int32_t* info, void* work,
+                               int64_t work_bytes, int jb, void* stream) {
+  if (!A || !ipiv || !info || !work) return fail("mom_getrf_nopiv: null device pointer");
+  if (lda < n) return fail("mom_getrf_nopiv: lda < n");
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == 64)
+    return getrf<double>(n, (z64*)A, lda, ipiv, info, work, (size_t)work_bytes, jb, 0, s, false);
+  if (prec == 32) return getrf<float>(n, (c32*)A, lda, ipiv, info, work, (size_t)work_bytes, jb, 0, s, false);
+  return fail("mom_getrf_nopiv: prec must be 64 (complex128) or 32 (complex64)");
 }
 
 extern "C" int mom_build_perm(int n, const int32_t* ipiv, int32_t* perm, void* stream) {

@@ -340,16 +340,25 @@ class Solver:
                 LU time / step time at this N).  lu_stats counts
                 factorisations, reuses, GMRES steps and complex128 LUs.
     lu64_steps: mixed precision, in a sweep: when the solve of a point with
-                its own fresh complex64 LU needs more than lu64_steps GMRES
-                steps, the complex64 factors have lost too much (the
+                its own fresh complex64 LU has not converged after lu64_steps
+                GMRES steps, the complex64 factors have lost too much (the
                 low-frequency end of a sweep, where the charge term of the
                 MPIE outweighs the current-loop term by many orders and
-                complex64 keeps too few digits of the latter): that point is
-                factored again in complex128 to precondition its neighbours,
-                and new anchors up to twice its frequency are complex128
-                LUs (GMRES then needs ~10-20 steps per reused point instead of
-                50-120).  None: complex64 anchors only.
-    method    :"auto" / "dense" (the H-matrix solver is not part of this build).
+                complex64 keeps too few digits of the latter): that solve is
+                abandoned, the point is factored in complex128 (which also
+                preconditions its neighbours), and new anchors up to twice
+                its frequency are complex128 LUs (GMRES then needs ~10-20
+                steps per reused point instead of 50-120).  None: complex64
+                anchors only.
+    nopiv     : mixed precision: factor the preconditioner LUs (complex64
+                and complex128 anchors) without row interchanges
+                (mom_getrf_nopiv: no cross-workgroup pivot search per
+                column, so the factorisation is bounded by its update GEMM).
+                On the diagonally scaled MoM matrices the unpivoted factors
+                precondition as well as pivoted ones (element growth ~2); a
+                solve whose GMRES has not converged after NOPIV_CAP steps
+                is factored again with pivoting.
+    method    : "auto" / "dense" (the H-matrix solver is not part of this build).
     surrogate : a trained GreensSurrogate of this stack: the remainders of
                 the Green's tables come from its networks, corrected and
                 judged per frequency by the Sommerfeld integrals at a few
@@ -378,7 +387,7 @@ class Solver:
     def __init__(self, mesh, substrate, z0=None, verbose=True, method="auto", v_weight=None, z0_recip_min=None,
                  tand_floor=None, precision="fp64", device="cuda", panel=None, refine_tol=1e-15, refine_maxit=20,
                  lanes=None, surrogate=None, surrogate_tol=2e-3, workers=None, graphs=False, autotune=False,
-                 gmres_restart=120, reuse_lu=True, lu_equiv=None, lu64_steps=20):
+                 gmres_restart=120, reuse_lu=True, lu_equiv=None, lu64_steps=20, nopiv=True):
         from .layered import Stackup
 
         if method not in ("auto", "dense"):
@@ -445,8 +454,10 @@ class Solver:
         self._anchor = {}
         self._history = {}
         self.lu64_steps = lu64_steps
+        self.nopiv = bool(nopiv)
         self._f_poor = 0.0
-        self.lu_stats = dict(factorisations=0, reused=0, fallbacks=0, gmres_steps=0, fp64_lu=0)
+        self.lu_stats = dict(factorisations=0, reused=0, fallbacks=0, gmres_steps=0, fp64_lu=0, pivoted_retries=0,
+                             abandoned=0)
         self.lanes = lanes
         self.workers = workers
         self.graphs = graphs
@@ -661,7 +672,9 @@ class Solver:
         cols += [self.Lm[:, c] for c in self.iport_index]
         return np.stack(cols, 1)
 
-    def _lu_solve(self, Z, B, Z32=None, lane=0, graph=False):
+    NOPIV_CAP = 200  # GMRES steps after which an unpivoted preconditioner is replaced by a pivoted one
+
+    def _lu_solve(self, Z, B, Z32=None, lane=0, graph=False, pivot=False, cap=None):
         """Generator: X = Z^-1 B on the device (the value it returns).  Z
         (complex128) is overwritten by its LU in the fp64 path; in the mixed
         path Z32 holds the complex64 LU and Z stays intact for the refinement
@@ -669,7 +682,11 @@ class Solver:
         graph: replay the factorisation (and the FP64 solve) from a HIP graph
         of this lane's buffers, captured on the second call (the first runs
         the same calls directly); the current stream must not be the default
-        stream."""
+        stream.  The mixed path's complex64 LU has no row interchanges when
+        nopiv is set (pivot=False); should its GMRES not converge within
+        NOPIV_CAP steps, the point is factored again with pivoting.  cap: the
+        mixed path returns None once its GMRES has taken that many steps
+        without converging (the caller decides what follows)."""
         n, nrhs = B.shape
         stream = _lib.stream_ptr()
         P = _lib.ptr
@@ -693,8 +710,14 @@ class Solver:
         if self.block is not None:
             _lib.call("mom_set_lu_block", prec, self.block)
 
+        nopiv = Z32 is not None and self.nopiv and not pivot
+
         def calls():
-            _lib.call("mom_getrf", prec, n, P(A), A.stride(0), P(ipiv), P(info), P(work), wbytes, jb, groups, stream)
+            if nopiv:
+                _lib.call("mom_getrf_nopiv", prec, n, P(A), A.stride(0), P(ipiv), P(info), P(work), wbytes, jb, stream)
+            else:
+                _lib.call("mom_getrf", prec, n, P(A), A.stride(0), P(ipiv), P(info), P(work), wbytes, jb, groups,
+                          stream)
             _lib.call("mom_build_perm", n, P(ipiv), P(perm), stream)
             if Z32 is None:
                 for c0, c1 in _column_chunks(nrhs):  # the solves take up to 4 right-hand sides
@@ -712,7 +735,7 @@ class Solver:
         if graph and nrhs <= 4:
             key = (lane, prec)
             sig = (stream, A.data_ptr(), B.data_ptr(), X.data_ptr(), work.data_ptr(), sw.data_ptr(), ipiv.data_ptr(),
-                   groups, jb, n, nrhs)
+                   groups, jb, n, nrhs, nopiv)
             g = self._graphs.get(key)
             if g is not None and g[0] != sig:
                 _lib.call("mom_graph_destroy", g[1])
@@ -735,7 +758,14 @@ class Solver:
         if Z32 is None:
             self.last_refine = 0
             return X
-        return (yield from self._gmres_chunks(Z, A, perm, sw, B, lane))
+        if cap is not None:  # the caller handles a solve that has not converged within cap steps
+            return (yield from self._gmres_chunks(Z, A, perm, sw, B, lane, max_its=cap))
+        X = yield from self._gmres_chunks(Z, A, perm, sw, B, lane, max_its=self.NOPIV_CAP if nopiv else None)
+        if X is None:  # the unpivoted factors precondition too poorly here: factor again with pivoting
+            self.lu_stats["pivoted_retries"] += 1
+            self._scale_copy(Z, Z32, lane)
+            X = yield from self._lu_solve(Z, B, Z32, lane, graph, pivot=True)
+        return X
 
     def _gmres_chunks(self, Z, A32, perm, sw, B, lane=0, max_its=None, X0=None, pc_prec=32):
         """_gmres_ir over groups of at most 4 right-hand sides (the solve
@@ -996,33 +1026,39 @@ class Solver:
                 self.lu_stats["reused"] += 1
                 return X
             self.lu_stats["fallbacks"] += 1
-        if sweep and g.freq <= 2 * self._f_poor:
-            # a band where complex64 factors precondition poorly: complex128 anchor
-            if not filled:
-                fill(None)
-            A, perm, sw = self._lu64_factor(Z, lane)
+        wide = self.lu64_steps is not None and sweep
+        if not (wide and g.freq <= 2 * self._f_poor):
+            if filled:
+                self._scale_copy(Z, Z32, lane)
+            else:
+                fill(Z32)
+                filled = True
+            # in a sweep, a complex64 solve still short of the target after
+            # lu64_steps steps is abandoned for a complex128 anchor
+            X = yield from self._lu_solve(Z, B, Z32, lane, graph, cap=self.lu64_steps if wide else None)
+            if X is not None:
+                if mixed:
+                    self.lu_stats["factorisations"] += 1
+                    if sweep:
+                        self._remember(lane, g.freq, X)
+                        self._set_anchor(lane, g.freq, 32, Z32, self._bufs[("perm", lane)],
+                                         self._bufs[("swork32", lane)], self.last_refine)
+                return X
+            self.lu_stats["abandoned"] += 1
+            self._f_poor = max(self._f_poor, g.freq)
+        # a band where complex64 factors precondition poorly: complex128 anchor
+        if not filled:
+            fill(None)
+        A, perm, sw = self._lu64_factor(Z, lane)
+        X = yield from self._gmres_chunks(Z, A, perm, sw, B, lane, pc_prec=64,
+                                          max_its=self.NOPIV_CAP if self.nopiv else None)
+        if X is None:  # unpivoted factors too poor here: factor again with pivoting
+            self.lu_stats["pivoted_retries"] += 1
+            A, perm, sw = self._lu64_factor(Z, lane, pivot=True)
             X = yield from self._gmres_chunks(Z, A, perm, sw, B, lane, pc_prec=64)
-            self._remember(lane, g.freq, X)
-            self.lu_stats["factorisations"] += 1
-            self._set_anchor(lane, g.freq, 64, A, perm, sw)
-            return X
-        if filled:
-            self._scale_copy(Z, Z32, lane)
-        else:
-            fill(Z32)
-        X = yield from self._lu_solve(Z, B, Z32, lane, graph)
-        if mixed:
-            self.lu_stats["factorisations"] += 1
-            if sweep:
-                self._remember(lane, g.freq, X)
-                if self.lu64_steps is not None and self.last_refine > self.lu64_steps:
-                    # the complex64 factors lost too much here (its own solve took many
-                    # steps): factor this point again in complex128 for its neighbours
-                    self._f_poor = max(self._f_poor, g.freq)
-                    self._set_anchor(lane, g.freq, 64, *self._lu64_factor(Z, lane))
-                else:
-                    self._set_anchor(lane, g.freq, 32, Z32, self._bufs[("perm", lane)],
-                                     self._bufs[("swork32", lane)], self.last_refine)
+        self._remember(lane, g.freq, X)
+        self.lu_stats["factorisations"] += 1
+        self._set_anchor(lane, g.freq, 64, A, perm, sw)
         return X
 
     def _set_anchor(self, lane, freq, prec, A, perm, sw, its0=0):
@@ -1031,12 +1067,13 @@ class Solver:
         self._anchor[lane] = dict(freq=freq, prec=prec, A=A, perm=perm, sw=sw, lu_equiv=eq, its0=its0,
                                   cost=eq + its0, n=1, last=None)
 
-    def _lu64_factor(self, Z, lane):
+    def _lu64_factor(self, Z, lane, pivot=False):
         """complex128 LU of D Z D (D = diag(|Z_ii|^-1/2), into the lane's
         dscale) with the inverted diagonal blocks of its factors, enqueued on
         the current stream; returns (LU, perm, solve workspace) for the
         preconditioner of the mixed path's GMRES (pc_prec = 64).  Z must be
-        symmetric (as the fill leaves it)."""
+        symmetric (as the fill leaves it).  Without row interchanges when
+        nopiv is set and pivot is False."""
         n = Z.shape[0]
         P = _lib.ptr
         stream = _lib.stream_ptr()
@@ -1049,7 +1086,8 @@ class Solver:
         ipiv = self._buf("ipiv64", (n,), torch.int32, lane)
         perm = self._buf("perm64", (n,), torch.int32, lane)
         info = self._buf("info", (4,), torch.int32, lane)
-        groups = self._panel_groups()
+        nopiv = self.nopiv and not pivot
+        groups = 0 if nopiv else self._panel_groups()
         jb = self.panel  # (tuned for complex64) halved until the panel's rows fit the LDS of `groups` workgroups
         while groups and jb > 16 and -(-n // max(1, self.PANEL_LDS // (16 * jb))) > groups:
             jb //= 2
@@ -1057,7 +1095,10 @@ class Solver:
         work = self._buf("work64", (wbytes,), torch.uint8, lane)
         sw = self._buf("swork64i", (int(L.mom_getrs_inv_work_bytes(64, n)),), torch.uint8, lane)
         sw[:256].zero_()  # status words (a solve clears them too; _check_solve may read them before one runs)
-        _lib.call("mom_getrf", 64, n, P(A), A.stride(0), P(ipiv), P(info), P(work), wbytes, jb, groups, stream)
+        if nopiv:
+            _lib.call("mom_getrf_nopiv", 64, n, P(A), A.stride(0), P(ipiv), P(info), P(work), wbytes, jb, stream)
+        else:
+            _lib.call("mom_getrf", 64, n, P(A), A.stride(0), P(ipiv), P(info), P(work), wbytes, jb, groups, stream)
         _lib.call("mom_build_perm", n, P(ipiv), P(perm), stream)
         _lib.call("mom_trtri_blocks", 64, n, P(A), A.stride(0), P(sw), stream)
         self.lu_stats["fp64_lu"] += 1
@@ -1094,17 +1135,19 @@ class Solver:
 
     def _lu_equiv(self):
         """Cost of one complex64 factorisation in GMRES steps at this N: a
-        model of this implementation on MI355X -- LU at ~32 TFLOP/s effective
-        or ~8 us per column of panel latency, whichever is larger; a step =
+        model of this implementation on MI355X -- the LU at ~50 TFLOP/s
+        effective or ~2.5 us per column of panel latency, whichever is larger
+        (unpivoted, nopiv; pivoted: ~32 TFLOP/s or ~8 us per column); a step =
         two triangular block chains of ~2.9 us per 64 rows + one FP64 Z read
         + ~0.2 ms of host round trip -- times 1.4, because the steps of
-        concurrent lanes overlap better than factorisations do (the sweep of
-        the 12.4 k-unknown pour was fastest at that factor: 4.11 s for 51
-        points against 4.35 s at 1.0 and 4.41 s at 4.6)."""
+        concurrent lanes overlap better than factorisations do (measured on
+        the 12.4 k-unknown pour, unpivoted: 17.3 points/s at the model's 78,
+        16.9 at 123, 14.6 at 50)."""
         if self.lu_equiv is not None:
             return float(self.lu_equiv)
         n = self.mesh.n_unknowns
-        t_lu = max(8 / 3 * n ** 3 / 32e12, 8e-6 * n)
+        rate, per_col = (50e12, 2.5e-6) if self.nopiv else (32e12, 8e-6)
+        t_lu = max(8 / 3 * n ** 3 / rate, per_col * n)
         t_step = 2 * (n / 64) * 2.9e-6 + 16 * n * n / 5e12 + 2e-4
         return 1.4 * t_lu / t_step
 

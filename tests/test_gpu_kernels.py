@@ -136,6 +136,49 @@ def _lu_solve(A, B, prec, jb, status=False):
     return X.cpu().numpy(), ipiv.cpu().numpy(), int(info[0])
 
 
+@pytest.mark.parametrize("prec,n,jb", [(32, 1, 8), (32, 200, 32), (32, 777, 64), (32, 2049, 128), (64, 300, 128),
+                                       (64, 1029, 64), (64, 130, 8)])
+def test_lu_nopiv_reconstructs_the_matrix(prec, n, jb):
+    """The LU without row interchanges: identity ipiv, no zero pivot, and
+    L U = A to the precision's round-off (a diagonally dominant matrix, whose
+    unpivoted factors exist and are well conditioned), over panels that
+    leave partial blocks (n not a multiple of jb, complex128 panels of 64
+    inside 128-wide blocks)."""
+    rng = np.random.default_rng(n + jb)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + 4 * np.sqrt(n) * np.eye(n)
+    dt = torch.complex128 if prec == 64 else torch.complex64
+    Ad = torch.from_numpy(A).to(DEV, dt)
+    ipiv = torch.empty(n, dtype=torch.int32, device=DEV)
+    info = torch.full((4,), 7, dtype=torch.int32, device=DEV)
+    wb = int(_lib.lib().mom_getrf_work_bytes(prec, n, jb))
+    work = torch.empty(wb, dtype=torch.uint8, device=DEV)
+    _lib.call("mom_getrf_nopiv", prec, n, _lib.ptr(Ad), n, _lib.ptr(ipiv), _lib.ptr(info), _lib.ptr(work), wb, jb,
+              _lib.stream_ptr())
+    LU = Ad.cpu().numpy().astype(np.complex128)
+    assert int(info[0]) == 0 and int(info[1]) == 0
+    np.testing.assert_array_equal(ipiv.cpu().numpy(), np.arange(n))
+    L = np.tril(LU, -1) + np.eye(n)
+    U = np.triu(LU)
+    Ar = Ad.new_tensor(A).cpu().numpy().astype(np.complex128)  # A in the factor's precision
+    err = np.linalg.norm(L @ U - Ar) / np.linalg.norm(Ar)
+    assert err < (3e-6 if prec == 32 else 1e-14), err
+
+
+def test_lu_nopiv_reports_zero_pivot():
+    """A zero leading entry: the unpivoted LU reports pivot 1 in info[0] and
+    leaves finite factors (the zero pivot is replaced by 1), no fault."""
+    A = np.array([[0, 1, 2], [1, 0, 1], [2, 1, 5]], complex)
+    Ad = torch.from_numpy(A).to(DEV, torch.complex64)
+    ipiv = torch.empty(3, dtype=torch.int32, device=DEV)
+    info = torch.zeros(4, dtype=torch.int32, device=DEV)
+    wb = int(_lib.lib().mom_getrf_work_bytes(32, 3, 8))
+    work = torch.empty(wb, dtype=torch.uint8, device=DEV)
+    _lib.call("mom_getrf_nopiv", 32, 3, _lib.ptr(Ad), 3, _lib.ptr(ipiv), _lib.ptr(info), _lib.ptr(work), wb, 8,
+              _lib.stream_ptr())
+    assert int(info[0]) == 1
+    assert np.all(np.isfinite(Ad.cpu().numpy()))
+
+
 @pytest.mark.parametrize("n,jb,nrhs", [(1, 8, 1), (37, 8, 3), (200, 32, 2), (777, 64, 4), (1500, 64, 2),
                                        (2049, 64, 1), (129, 32, 6), (300, 32, 8)])
 def test_lu_fp64_matches_lapack(n, jb, nrhs):
