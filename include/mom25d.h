@@ -182,6 +182,17 @@ int mom_getrs_inv_z(int prec, int n, int nrhs, const void* LU, int64_t lda, cons
 int mom_scaled_matvec(int n, int nrhs, const void* Z, int64_t ldz, const double* dsc, const void* X, void* W,
                       void* stream);
 
+/* The same product for a symmetric Z (Z = Z^T, as the fill and the frequency
+ * interpolation leave it), reading only the 64 x 64 tiles on and above the
+ * diagonal: half the HBM traffic of mom_scaled_matvec.  work: device
+ * workspace of mom_scaled_symv_work_bytes(n, nrhs) bytes (the tiles' partial
+ * products, summed in a fixed order: the result does not depend on
+ * scheduling).  No counterpart in the reference, which solves in FP64
+ * directly (Engine/mom25d/solver.py:11-31). */
+int64_t mom_scaled_symv_work_bytes(int n, int nrhs);
+int mom_scaled_symv(int n, int nrhs, const void* Z, int64_t ldz, const double* dsc, const void* X, void* W,
+                    void* work, int64_t work_bytes, void* stream);
+
 /* Frequency interpolation of Z (Engine/mom25d/freqinterp.py:86-139).
  * mom_interp_anchor: W = j w Z exp(+j ke R) (N x N, W stored in prec_w = 64 or 32
  * bits per real part), R the distance of the basis-function centres pos

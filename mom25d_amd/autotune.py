@@ -63,9 +63,10 @@ def get_tile(prec):
     return bm.value, bn.value
 
 
-def time_getrf(n, prec, jb, tile, device="cuda", reps=2, seed=0, nb=0):
+def time_getrf(n, prec, jb, tile, device="cuda", reps=2, seed=0, nb=0, pivot=True):
     """Median milliseconds of mom_getrf (n, prec, panel jb, block nb, tile)
-    on a synthetic matrix."""
+    on a synthetic matrix; pivot=False: mom_getrf_nopiv (the mixed path's
+    preconditioner factorisation)."""
     dev = torch.device(device)
     dt = torch.complex128 if prec == 64 else torch.complex64
     g = torch.Generator(device=dev)
@@ -89,7 +90,10 @@ def time_getrf(n, prec, jb, tile, device="cuda", reps=2, seed=0, nb=0):
             A.copy_(A0)
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record()
-            _lib.call("mom_getrf", prec, n, P(A), n, P(ipiv), P(info), P(work), wb, jb, 0, s)
+            if pivot:
+                _lib.call("mom_getrf", prec, n, P(A), n, P(ipiv), P(info), P(work), wb, jb, 0, s)
+            else:
+                _lib.call("mom_getrf_nopiv", prec, n, P(A), n, P(ipiv), P(info), P(work), wb, jb, s)
             e1.record()
             torch.cuda.synchronize(dev)
             times.append(e0.elapsed_time(e1))
@@ -99,12 +103,13 @@ def time_getrf(n, prec, jb, tile, device="cuda", reps=2, seed=0, nb=0):
     return float(np.median(times[1:]))
 
 
-def tune_lu(n, prec, device="cuda", panels=PANELS, blocks=BLOCKS, tiles=TILES, reps=2, verbose=False):
+def tune_lu(n, prec, device="cuda", panels=PANELS, blocks=BLOCKS, tiles=TILES, reps=2, verbose=False, pivot=True):
     """The fastest (panel width, block width, tile) for an n x n
-    factorisation in prec (64: complex128, 32: complex64):
+    factorisation in prec (64: complex128, 32: complex64), with row
+    interchanges (mom_getrf) or without (pivot=False, mom_getrf_nopiv):
     dict(jb, nb, tile, ms, table)."""
     _load()
-    key = _key(prec, n)
+    key = _key(prec, n) + ("" if pivot else ":nopiv")
     if key in _CACHE:
         return _CACHE[key]
     m = -(-n // 512) * 512
@@ -117,7 +122,7 @@ def tune_lu(n, prec, device="cuda", panels=PANELS, blocks=BLOCKS, tiles=TILES, r
                 continue
             for tile in tiles:
                 try:
-                    ms = time_getrf(m, prec, jb, tile, device, reps, nb=nb)
+                    ms = time_getrf(m, prec, jb, tile, device, reps, nb=nb, pivot=pivot)
                 except RuntimeError as e:  # e.g. a panel too large for the LDS at this width
                     if verbose:
                         print(f"  autotune n={m} prec={prec} jb={jb} nb={nb} tile={tile}: {e}", flush=True)

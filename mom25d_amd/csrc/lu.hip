@@ -298,114 +298,153 @@ __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<R> a) {
 
 // ------------------------------------------------- unpivoted panel
 template <typename R>
-static constexpr int kNoPivPanelMax() {  // widest panel whose diagonal block fits one workgroup's LDS
+static constexpr int kNoPivPanelMax() {  // widest unpivoted panel (columns held by the register layout of panel_below_kernel)
   return sizeof(R) == sizeof(double) ? 64 : 128;
 }
 
 // The panel of the LU without row interchanges (mom_getrf_nopiv: the
 // preconditioner factorisation of the mixed path, where GMRES absorbs the
 // difference): no column needs a cross-workgroup pivot search, so a panel is
-// two ordinary launches -- the b x b diagonal block factored by one workgroup
-// in LDS, then the rows below it, L21 = A21 U11^-1, by independent
-// workgroups of 64 rows -- instead of one exchange per column.
-//
-// Both use one barrier per column: thread (c, q) updates column c of the
-// rows of its class q (rows q, q + 8, ...); in step j the threads of each
-// column c > j subtract a_ij (u_jc / u_jj), with the unscaled a_ij, from
-// their rows below j, and column j is scaled by 1 / u_jj when no one reads
-// it any more (the diagonal block, held in LDS, in step j + 1; the 64-row
-// tiles below, held in registers, publish column j to LDS first).  An exactly
-// zero pivot is replaced by 1 and reported in info[0] (first one, 1-based),
-// which leaves finite factors that the caller discards.
+// two ordinary launches -- the b x b diagonal block factored by one workgroup,
+// then the rows below it, L21 = A21 U11^-1, by independent workgroups of 16
+// rows -- instead of one exchange per column.  Both keep their part of the
+// matrix in registers (thread (c, q): column c of the rows i = q mod Q) and
+// exchange only one row or column per step through LDS, so neither holds a
+// large LDS allocation: the rows-below workgroups (256 threads, 1 KB of LDS)
+// find room on CUs that the concurrent trailing-update GEMM of the look-ahead
+// still occupies.  An exactly zero pivot is replaced by 1 and reported in
+// info[0] (first one, 1-based), which leaves finite factors that the caller
+// discards.
 template <typename R>
 __device__ inline cplx<R> nonzero_pivot(cplx<R> p) {
   return (p.re == (R)0 && p.im == (R)0) ? cplx<R>{(R)1, (R)0} : p;
 }
 
+// The diagonal block: 1024 threads, thread (c, q) = (tid % NC, tid / NC)
+// holds column c of the rows q, q + Q, ... (NC = 128 columns and 16 rows per
+// thread in complex64, 64 columns and 4 rows in complex128).  Step j: the
+// owners of row j publish its columns >= j; the owners of column j scale
+// their rows below j by 1 / u_jj and publish them; every column c > j then
+// subtracts l_ij u_jc from its rows below j.  Two barriers per step; the row
+// and column buffers alternate between steps, so step j + 1 publishes while a
+// slower wavefront may still read step j's.
 template <typename R>
 __global__ __launch_bounds__(1024) void diag_lu_kernel(cplx<R>* A, int64_t lda, int k0, int b, int32_t* ipiv,
                                                        int32_t* info) {
-  // the block in LDS; thread (c, q) updates column c of the rows i = q mod 8,
-  // 8 rows at a time (their loads issued together)
-  extern __shared__ unsigned char smem_raw[];
-  cplx<R>* T = reinterpret_cast<cplx<R>*>(smem_raw);  // [b][b]
-  const int tid = threadIdx.x, c = tid % 128;
-  const int q = __builtin_amdgcn_readfirstlane(tid / 128);  // uniform per wavefront
-  for (int e = tid; e < b * b; e += 1024) T[e] = A[(int64_t)(k0 + e / b) * lda + k0 + e % b];
-  for (int e = tid; e < b; e += 1024) ipiv[k0 + e] = k0 + e;
-  __syncthreads();
-  for (int j = 0; j < b; ++j) {
-    const cplx<R> p = T[j * b + j];  // final since step j - 1
-    if (j > 0) {  // finish column j - 1: its rows below the diagonal scaled, one row per thread
-      const cplx<R> pp = T[(j - 1) * b + j - 1];
-      if (j + tid < b) T[(j + tid) * b + j - 1] = T[(j + tid) * b + j - 1] * cdiv(cplx<R>{(R)1, (R)0},
-                                                                                     nonzero_pivot(pp));
-      if (tid == 0) T[(j - 1) * b + j - 1] = nonzero_pivot(pp);
-    }
-    if (tid == 0 && p.re == (R)0 && p.im == (R)0 && *info == 0) *info = k0 + j + 1;
-    if (c > j && c < b) {
-      const cplx<R> m = cdiv(T[j * b + c], nonzero_pivot(p));
-      const int first = j + 1 + ((q - (j + 1)) & 7);  // first row > j of this class
-      for (int i0 = first; i0 < b; i0 += 64) {
-        cplx<R> a[8], t[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-          const int i = min(i0 + 8 * k, b - 1);
-          a[k] = T[i * b + j];
-          t[k] = T[i * b + c];
-        }
-#pragma unroll
-        for (int k = 0; k < 8; ++k)
-          if (i0 + 8 * k < b) T[(i0 + 8 * k) * b + c] = t[k] - a[k] * m;
-      }
-    }
-    __syncthreads();
-  }
-  if (tid == 0) T[(b - 1) * b + b - 1] = nonzero_pivot(T[(b - 1) * b + b - 1]);
-  __syncthreads();
-  for (int e = tid; e < b * b; e += 1024) A[(int64_t)(k0 + e / b) * lda + k0 + e % b] = T[e];
-}
-
-// L21 = A21 U11^-1 for the rows [k0 + b, n) below a factored diagonal block,
-// 64 rows per workgroup (thread (c, q): column c of the rows i = q mod 8, in
-// registers), U11 staged in LDS.
-template <typename R>
-__global__ __launch_bounds__(1024) void panel_below_kernel(cplx<R>* A, int64_t lda, int n, int k0, int b) {
-  constexpr int RT = 64, HR = RT / 8;
-  extern __shared__ unsigned char smem_raw[];
-  cplx<R>* U = reinterpret_cast<cplx<R>*>(smem_raw);  // [b][b]: U11 (its lower part unused)
-  __shared__ cplx<R> colbuf[2][RT];
-  const int tid = threadIdx.x, c = tid % 128;
-  const int q = __builtin_amdgcn_readfirstlane(tid / 128);
+  constexpr int NC = kNoPivPanelMax<R>(), Q = 1024 / NC, HR = NC / Q;
+  __shared__ cplx<R> rowbuf[2][NC];
+  __shared__ cplx<R> colbuf[2][NC];
+  const int tid = threadIdx.x, c = tid % NC;
+  const int q = __builtin_amdgcn_readfirstlane(tid / NC);  // uniform per wavefront
   const bool mine = c < b;
-  const int r0 = k0 + b + blockIdx.x * RT, nr = min(RT, n - r0);
-  for (int e = tid; e < b * b; e += 1024) U[e] = A[(int64_t)(k0 + e / b) * lda + k0 + e % b];
   cplx<R> v[HR];
 #pragma unroll
   for (int t = 0; t < HR; ++t) {
-    const int i = 8 * t + q;
-    v[t] = (mine && i < nr) ? A[(int64_t)(r0 + i) * lda + k0 + c] : cplx<R>{0, 0};
+    const int i = q + Q * t;
+    v[t] = cplx<R>{(R)0, (R)0};
+    if (mine && i < b) v[t] = A[(int64_t)(k0 + i) * lda + k0 + c];
+  }
+  for (int e = tid; e < b; e += 1024) ipiv[k0 + e] = k0 + e;
+  for (int j = 0; j < b; ++j) {
+    const int buf = j & 1;
+    if (q == (j % Q) && c >= j && mine) {  // row j (register j / Q, selected without dynamic indexing)
+      cplx<R> x = v[0];
+#pragma unroll
+      for (int t = 1; t < HR; ++t)
+        if (t == j / Q) x = v[t];
+      rowbuf[buf][c] = x;
+    }
+    __syncthreads();
+    const cplx<R> p = rowbuf[buf][j];
+    if (tid == 0 && p.re == (R)0 && p.im == (R)0 && *info == 0) *info = k0 + j + 1;
+    if (c == j) {
+      const cplx<R> inv = cdiv(cplx<R>{(R)1, (R)0}, nonzero_pivot(p));
+#pragma unroll
+      for (int t = 0; t < HR; ++t) {
+        const int i = q + Q * t;
+        if (i > j && i < b) {
+          v[t] = v[t] * inv;
+          colbuf[buf][i] = v[t];
+        } else if (i == j) {
+          v[t] = nonzero_pivot(p);
+        }
+      }
+    }
+    __syncthreads();
+    if (c > j && mine) {
+      const cplx<R> u = rowbuf[buf][c];
+#pragma unroll
+      for (int t = 0; t < HR; ++t) {
+        const int i = q + Q * t;
+        if (i > j && i < b) v[t] -= colbuf[buf][i] * u;
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < HR; ++t) {
+    const int i = q + Q * t;
+    if (mine && i < b) A[(int64_t)(k0 + i) * lda + k0 + c] = v[t];
+  }
+}
+
+// L21 = A21 U11^-1 for the rows [k0 + b, n) below a factored diagonal block:
+// 256 threads per workgroup of 16 rows, thread (c, q) = (tid % NC, tid / NC)
+// holds column c of the rows q, q + Q, ... in registers.  Row j of U11 is
+// read from memory (L2: every workgroup reads the same block) two steps
+// ahead of its use; the column being finished is exchanged through LDS.
+template <typename R>
+__global__ __launch_bounds__(256) void panel_below_kernel(cplx<R>* A, int64_t lda, int n, int k0, int b) {
+  constexpr int NC = kNoPivPanelMax<R>(), Q = 256 / NC, RT = 16, HR = RT / Q;
+  __shared__ cplx<R> colbuf[2][RT];
+  const int tid = threadIdx.x, c = tid % NC;
+  const int q = __builtin_amdgcn_readfirstlane(tid / NC);
+  const bool mine = c < b;
+  const int r0 = k0 + b + blockIdx.x * RT, nr = min(RT, n - r0);
+  const cplx<R>* U = A + (int64_t)k0 * lda + k0;
+  constexpr cplx<R> zero = {(R)0, (R)0};
+  cplx<R> v[HR];
+#pragma unroll
+  for (int t = 0; t < HR; ++t) {
+    const int i = q + Q * t;
+    v[t] = zero;
+    if (mine && i < nr) v[t] = A[(int64_t)(r0 + i) * lda + k0 + c];
+  }
+  // u_jc and u_jj of rows j and j + 1 (rows past b, and columns past b, read as 0)
+  cplx<R> uc0 = zero, ud0 = U[0], uc1 = zero, ud1 = zero;
+  if (mine) uc0 = U[c];
+  if (b > 1) {
+    ud1 = U[lda + 1];
+    if (mine) uc1 = U[lda + c];
   }
   for (int j = 0; j < b; ++j) {
     const int buf = j & 1;
+    cplx<R> uc2 = zero, ud2 = zero;
+    if (j + 2 < b) {
+      ud2 = U[(int64_t)(j + 2) * lda + j + 2];
+      if (mine) uc2 = U[(int64_t)(j + 2) * lda + c];
+    }
     if (c == j) {
 #pragma unroll
-      for (int t = 0; t < HR; ++t) colbuf[buf][8 * t + q] = v[t];
+      for (int t = 0; t < HR; ++t) colbuf[buf][q + Q * t] = v[t];
     }
     __syncthreads();
-    const cplx<R> inv = cdiv(cplx<R>{(R)1, (R)0}, nonzero_pivot(U[j * b + j]));
+    const cplx<R> inv = cdiv(cplx<R>{(R)1, (R)0}, nonzero_pivot(ud0));
     if (mine && c > j) {
-      const cplx<R> m = U[j * b + c] * inv;
+      const cplx<R> m = uc0 * inv;
 #pragma unroll
-      for (int t = 0; t < HR; ++t) v[t] -= colbuf[buf][8 * t + q] * m;
+      for (int t = 0; t < HR; ++t) v[t] -= colbuf[buf][q + Q * t] * m;
     } else if (c == j) {
 #pragma unroll
       for (int t = 0; t < HR; ++t) v[t] = v[t] * inv;
     }
+    uc0 = uc1;
+    ud0 = ud1;
+    uc1 = uc2;
+    ud1 = ud2;
   }
 #pragma unroll
   for (int t = 0; t < HR; ++t) {
-    const int i = 8 * t + q;
+    const int i = q + Q * t;
     if (mine && i < nr) A[(int64_t)(r0 + i) * lda + k0 + c] = v[t];
   }
 }
@@ -1259,30 +1298,13 @@ static int update_block(cplx<R>* A, int64_t lda, int n, int k0, int b, int r0, i
 template <typename R>
 static int launch_panel_nopiv(cplx<R>* A, int64_t lda, int n, int k0, int b, int32_t* ipiv, int32_t* info,
                               hipStream_t s) {
-  static bool attr_set = false;
-  if (!attr_set) {
-    const size_t most = sizeof(cplx<R>) * (size_t)kNoPivPanelMax<R>() * kNoPivPanelMax<R>();
-    if (hipFuncSetAttribute((const void*)diag_lu_kernel<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)most) !=
-        hipSuccess)
-      return fail("diag_lu_kernel: cannot reserve LDS");
-    attr_set = true;
-  }
-  hipLaunchKernelGGL(diag_lu_kernel<R>, dim3(1), dim3(1024), sizeof(cplx<R>) * (size_t)b * b, s, A, lda, k0, b, ipiv,
-                     info);
+  if (b > kNoPivPanelMax<R>()) return fail("mom_getrf_nopiv: panel wider than the diagonal-block kernel");
+  hipLaunchKernelGGL(diag_lu_kernel<R>, dim3(1), dim3(1024), 0, s, A, lda, k0, b, ipiv, info);
   int rc = check_launch("diag_lu_kernel");
   if (rc) return rc;
   const int below = n - k0 - b;
   if (below <= 0) return 0;
-  static bool attr2_set = false;
-  if (!attr2_set) {
-    const size_t most = sizeof(cplx<R>) * (size_t)kNoPivPanelMax<R>() * kNoPivPanelMax<R>();
-    if (hipFuncSetAttribute((const void*)panel_below_kernel<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)most) != hipSuccess)
-      return fail("panel_below_kernel: cannot reserve LDS");
-    attr2_set = true;
-  }
-  hipLaunchKernelGGL(panel_below_kernel<R>, dim3((below + 63) / 64), dim3(1024), sizeof(cplx<R>) * (size_t)b * b, s,
-                     A, lda, n, k0, b);
+  hipLaunchKernelGGL(panel_below_kernel<R>, dim3((below + 15) / 16), dim3(256), 0, s, A, lda, n, k0, b);
   return check_launch("panel_below_kernel");
 }
 
@@ -1884,6 +1906,168 @@ extern "C" int mom_scaled_matvec(int n, int nrhs, const void* Z, int64_t ldz, co
   }
 #undef MOM_SMV
   return check_launch("residual_kernel");
+}
+
+// W = D Z D X for a symmetric Z (Z = Z^T, as the fill leaves it) from the
+// tiles on and above the diagonal only: half the HBM traffic of the
+// row-wise product.  symv_tile_kernel: workgroup (I, J), I <= J, reads the
+// 64 x 64 tile Z_IJ once and forms both its row product Z_IJ (D X)_J and,
+// off the diagonal, its column product Z_IJ^T (D X)_I, which stands for the
+// tile Z_JI below the diagonal.  Thread (a, b) = (tid / 16, tid % 16) holds
+// the 4 x 4 entries of rows 4a.., columns 4b..; row sums reduce over the 16
+// lanes of equal a, column sums over the lanes of equal b and the 4
+// wavefronts.  The partial products go to Q[K][S] (64 x NR each): the row
+// product of tile (I, J) to Q[I][J], the column product to Q[J][I], so that
+// block K's partials are Q[K][0 .. nb) -- summed in a fixed order by
+// symv_reduce_kernel, which also applies D (deterministic: no atomics).
+constexpr int kSymvT = 64;
+
+template <int NR>
+__global__ __launch_bounds__(256) void symv_tile_kernel(int n, const z64* __restrict__ Z, int64_t ldz,
+                                                        const z64* __restrict__ X, const double* __restrict__ dsc,
+                                                        z64* __restrict__ Q) {
+  const int J = blockIdx.x, I = blockIdx.y;
+  if (J < I) return;
+  const int nb = gridDim.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int a = tid >> 4, b = tid & 15;
+  const int r0 = I * kSymvT + 4 * a, c0 = J * kSymvT + 4 * b;
+  typedef double dv2 __attribute__((ext_vector_type(2)));
+  z64 yi[4][NR], yj[4][NR];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int r = r0 + u, c = c0 + u;
+    const double dr = r < n ? dsc[r] : 0.0, dc = c < n ? dsc[c] : 0.0;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      yi[u][k] = {0.0, 0.0};
+      yj[u][k] = {0.0, 0.0};
+      if (r < n) yi[u][k] = X[(int64_t)r * NR + k] * dr;
+      if (c < n) yj[u][k] = X[(int64_t)c * NR + k] * dc;
+    }
+  }
+  z64 z[4][4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int r = r0 + u;
+    const dv2* zr = reinterpret_cast<const dv2*>(Z + (int64_t)min(r, n - 1) * ldz + c0);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      z[u][v] = {0.0, 0.0};
+      if (r < n && c0 + v < n) {
+        const dv2 t = __builtin_nontemporal_load(zr + v);
+        z[u][v] = {t.x, t.y};
+      }
+    }
+  }
+  z64 rp[4][NR], cp[4][NR];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      rp[u][k] = {0.0, 0.0};
+      cp[u][k] = {0.0, 0.0};
+    }
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int k = 0; k < NR; ++k) {
+        rp[u][k] += z[u][v] * yj[v][k];
+        cp[v][k] += z[u][v] * yi[u][k];
+      }
+  // row products: sum over the 16 lanes of this row group
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      double re = rp[u][k].re, im = rp[u][k].im;
+#pragma unroll
+      for (int off = 1; off < 16; off <<= 1) {
+        re += __shfl_xor(re, off, 64);
+        im += __shfl_xor(im, off, 64);
+      }
+      rp[u][k] = {re, im};
+    }
+  z64* qrow = Q + ((int64_t)I * nb + J) * kSymvT * NR;
+  if (b == 0) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int k = 0; k < NR; ++k) qrow[(4 * a + u) * NR + k] = rp[u][k];
+  }
+  if (I == J) return;  // uniform per workgroup
+  // column products: the 4 row groups of this wavefront, then the 4 wavefronts
+  __shared__ z64 cs[4][kSymvT][NR];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      double re = cp[v][k].re, im = cp[v][k].im;
+      re += __shfl_xor(re, 16, 64);
+      im += __shfl_xor(im, 16, 64);
+      re += __shfl_xor(re, 32, 64);
+      im += __shfl_xor(im, 32, 64);
+      if (lane < 16) cs[wave][4 * b + v][k] = {re, im};
+    }
+  __syncthreads();
+  z64* qcol = Q + ((int64_t)J * nb + I) * kSymvT * NR;
+  for (int e = tid; e < kSymvT * NR; e += 256) {
+    const int c = e / NR, k = e % NR;
+    z64 t = cs[0][c][k];
+    t += cs[1][c][k];
+    t += cs[2][c][k];
+    t += cs[3][c][k];
+    qcol[e] = t;
+  }
+}
+
+template <int NR>
+__global__ __launch_bounds__(256) void symv_reduce_kernel(int n, int nb, const z64* __restrict__ Q,
+                                                          const double* __restrict__ dsc, z64* __restrict__ W) {
+  const int K = blockIdx.x, e = threadIdx.x;
+  if (e >= kSymvT * NR) return;
+  const int row = K * kSymvT + e / NR;
+  const z64* q = Q + (int64_t)K * nb * kSymvT * NR + e;
+  z64 t = {0.0, 0.0};
+  for (int S = 0; S < nb; ++S) t += q[(int64_t)S * kSymvT * NR];
+  if (row < n) W[(int64_t)K * kSymvT * NR + e] = t * dsc[row];
+}
+
+extern "C" int64_t mom_scaled_symv_work_bytes(int n, int nrhs) {
+  const int64_t nb = (n + kSymvT - 1) / kSymvT;
+  return nb * nb * kSymvT * (int64_t)nrhs * (int64_t)sizeof(z64);
+}
+
+extern "C" int mom_scaled_symv(int n, int nrhs, const void* Z, int64_t ldz, const double* dsc, const void* X,
+                               void* W, void* work, int64_t work_bytes, void* stream) {
+  if (n <= 0) return 0;
+  if (nrhs < 1 || nrhs > 4) return fail("mom_scaled_symv: 1 to 4 right-hand sides");
+  if (!Z || !dsc || !X || !W || !work) return fail("mom_scaled_symv: null device pointer");
+  if (ldz < n) return fail("mom_scaled_symv: ldz < n");
+  if ((reinterpret_cast<uintptr_t>(Z) | ((uintptr_t)ldz * sizeof(z64))) % 16)
+    return fail("mom_scaled_symv: Z rows must be 16-byte aligned");
+  if (work_bytes < mom_scaled_symv_work_bytes(n, nrhs)) return fail("mom_scaled_symv: workspace too small");
+  const int nb = (n + kSymvT - 1) / kSymvT;
+  if (nb > 65535) return fail("mom_scaled_symv: n too large");
+  hipStream_t s = (hipStream_t)stream;
+  const z64 *z = (const z64*)Z, *x = (const z64*)X;
+  z64 *w = (z64*)W, *q = (z64*)work;
+#define MOM_SYMV(NR)                                                                                          \
+  case NR:                                                                                                    \
+    hipLaunchKernelGGL((symv_tile_kernel<NR>), dim3(nb, nb), dim3(256), 0, s, n, z, ldz, x, dsc, q);          \
+    if (int rc = check_launch("symv_tile_kernel")) return rc;                                                 \
+    hipLaunchKernelGGL((symv_reduce_kernel<NR>), dim3(nb), dim3(256), 0, s, n, nb, (const z64*)q, dsc, w);     \
+    break;
+  switch (nrhs) {
+    MOM_SYMV(1)
+    MOM_SYMV(2)
+    MOM_SYMV(3)
+    MOM_SYMV(4)
+  }
+#undef MOM_SYMV
+  return check_launch("symv_reduce_kernel");
 }
 
 extern "C" int mom_cvt_z_to_c(int64_t n_reals, const double* src, float* dst, void* stream) {

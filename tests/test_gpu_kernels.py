@@ -374,3 +374,46 @@ def test_getrs_inv_z_and_scaled_matvec(n, nrhs):
     _lib.call("mom_scaled_matvec", n, nrhs, P(Zd), n, P(torch.from_numpy(d).to(DEV)), P(B64), P(W), s)
     ref = d[:, None] * (A @ (d[:, None] * B))
     assert np.max(np.abs(W.cpu().numpy() - ref)) < 1e-13 * np.sqrt(n) * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("n,nrhs,ld", [(1, 1, 1), (63, 2, 63), (64, 3, 70), (65, 2, 66), (333, 4, 333),
+                                       (1500, 2, 1500), (2049, 1, 2052)])
+def test_scaled_symv_matches_full_product(n, nrhs, ld):
+    """W = D Z D X from the tiles on and above the diagonal of a symmetric Z
+    equals the numpy product and the row-wise kernel over every row (partial
+    last tiles, leading dimensions past n, 1 to 4 right-hand sides), and a
+    second call on a smaller size reuses the workspace."""
+    rng = np.random.default_rng(n + 7 * nrhs)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    Z = A + A.T  # complex symmetric
+    Zp = np.zeros((n, ld), complex)
+    Zp[:, :n] = Z
+    Zp[:, n:] = np.nan  # columns past n are never read
+    Zd = torch.from_numpy(Zp).to(DEV)
+    d = rng.uniform(0.5, 2.0, n)
+    dd = torch.from_numpy(d).to(DEV)
+    X = rng.standard_normal((n, nrhs)) + 1j * rng.standard_normal((n, nrhs))
+    Xd = torch.from_numpy(X).to(DEV)
+    wb = int(_lib.lib().mom_scaled_symv_work_bytes(n, nrhs))
+    work = torch.empty(wb, dtype=torch.uint8, device=DEV)
+    W = torch.empty_like(Xd)
+    P = _lib.ptr
+    s = _lib.stream_ptr()
+    _lib.call("mom_scaled_symv", n, nrhs, P(Zd), ld, P(dd), P(Xd), P(W), P(work), wb, s)
+    ref = d[:, None] * (Z @ (d[:, None] * X))
+    got = W.cpu().numpy()
+    assert np.all(np.isfinite(got))
+    assert np.max(np.abs(got - ref)) < 1e-13 * np.sqrt(n) * np.max(np.abs(ref))
+    Wf = torch.empty_like(Xd)
+    _lib.call("mom_scaled_matvec", n, nrhs, P(Zd), ld, P(dd), P(Xd), P(Wf), s)
+    assert np.max(np.abs(got - Wf.cpu().numpy())) < 1e-13 * np.sqrt(n) * np.max(np.abs(ref))
+    if n > 64:  # a smaller problem through the same workspace
+        m = n // 2
+        Zm = torch.from_numpy(np.ascontiguousarray(Z[:m, :m])).to(DEV)
+        Xm = Xd[:m].contiguous()
+        Wm = torch.empty_like(Xm)
+        _lib.call("mom_scaled_symv", m, nrhs, P(Zm), m, P(dd), P(Xm), P(Wm), P(work), wb, s)
+        refm = d[:m, None] * (Z[:m, :m] @ (d[:m, None] * X[:m]))
+        assert np.max(np.abs(Wm.cpu().numpy() - refm)) < 1e-13 * np.sqrt(m) * np.max(np.abs(refm))
+    with pytest.raises(RuntimeError):
+        _lib.call("mom_scaled_symv", n, nrhs, P(Zd), ld, P(dd), P(Xd), P(W), P(work), wb - 16, s)
