@@ -193,6 +193,20 @@ int64_t mom_scaled_symv_work_bytes(int n, int nrhs);
 int mom_scaled_symv(int n, int nrhs, const void* Z, int64_t ldz, const double* dsc, const void* X, void* W,
                     void* work, int64_t work_bytes, void* stream);
 
+/* One Arnoldi step of the mixed path's restarted GMRES in one call: W_j =
+ * M^-1 V_j (mom_getrs_inv_z with the factors LU, perm and their solve
+ * workspace), w = D Z D W_j into V_{j+1} (mom_scaled_symv when symv_work is
+ * non-null, else mom_scaled_matvec), classical Gram-Schmidt twice against
+ * V_0 .. V_j with the summed coefficients in H[0 .. j], H[j+1] = |w| and
+ * V_{j+1} = w / |w| per column, then H[0 .. j+1] ((j + 2) x nrhs) copied to
+ * the pinned host buffer Hh on the stream.  V: (m + 1) x n x nrhs and W:
+ * m x n x nrhs complex128; H: device workspace of 2 (m + 1) nrhs complex128;
+ * 0 <= j < m, nrhs <= 4.  No counterpart in the reference, which solves in
+ * FP64 directly (Engine/mom25d/solver.py:11-31). */
+int mom_gmres_arnoldi(int prec, int n, int nrhs, int j, int m, const void* LU, int64_t lda, const int32_t* perm,
+                      void* solve_work, const void* Z, int64_t ldz, const double* dsc, void* symv_work,
+                      int64_t symv_bytes, void* V, void* W, void* H, void* Hh, void* stream);
+
 /* Frequency interpolation of Z (Engine/mom25d/freqinterp.py:86-139).
  * mom_interp_anchor: W = j w Z exp(+j ke R) (N x N, W stored in prec_w = 64 or 32
  * bits per real part), R the distance of the basis-function centres pos

@@ -2129,3 +2129,79 @@ extern "C" int mom_basis_sub(int n, int nrhs, int j1, const void* V, const void*
                      (const z64*)V, (const z64*)h, (z64*)w);
   return check_launch("basis_sub_kernel");
 }
+
+// ----------------------------------------------------------- GMRES step
+// One Arnoldi step of the mixed path's right-preconditioned GMRES in a
+// single call (the host then only reads the new Hessenberg column): W_j =
+// M^-1 V_j, w = (D Z D) W_j into V_{j+1}, classical Gram-Schmidt twice
+// against V_0 .. V_j with the coefficients summed into H[0 .. j], H[j+1] =
+// |w| per column, V_{j+1} = w / |w|, and H[0 .. j+1] copied to the host
+// buffer Hh asynchronously on the stream.
+namespace {
+
+__global__ void h_add_kernel(int cnt, z64* __restrict__ H, const z64* __restrict__ T) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < cnt) H[q] += T[q];
+}
+
+// hn[c] = (|w[:, c]|, 0): one workgroup per column
+__global__ __launch_bounds__(256) void col_norm_kernel(int n, int nrhs, const z64* __restrict__ w, z64* hn) {
+  const int c = blockIdx.x, tid = threadIdx.x;
+  double t = 0.0;
+  for (int r = tid; r < n; r += 256) {
+    const z64 v = w[(int64_t)r * nrhs + c];
+    t += v.re * v.re + v.im * v.im;
+  }
+  for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+  __shared__ double part[4];
+  if ((tid & 63) == 0) part[tid >> 6] = t;
+  __syncthreads();
+  if (tid == 0) hn[c] = {sqrt(part[0] + part[1] + part[2] + part[3]), 0.0};
+}
+
+__global__ void col_scale_kernel(int n, int nrhs, z64* __restrict__ w, const z64* __restrict__ hn) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n * nrhs) return;
+  const double inv = 1.0 / hn[q % nrhs].re;
+  w[q] = w[q] * inv;
+}
+
+}  // namespace
+
+extern "C" int mom_gmres_arnoldi(int prec, int n, int nrhs, int j, int m, const void* LU, int64_t lda,
+                                 const int32_t* perm, void* solve_work, const void* Z, int64_t ldz, const double* dsc,
+                                 void* symv_work, int64_t symv_bytes, void* V, void* W, void* H, void* Hh,
+                                 void* stream) {
+  if (n <= 0) return 0;
+  if (nrhs < 1 || nrhs > 4) return fail("mom_gmres_arnoldi: 1 to 4 right-hand sides");
+  if (j < 0 || j >= m) return fail("mom_gmres_arnoldi: step outside the restart length");
+  if (!V || !W || !H || !Hh) return fail("mom_gmres_arnoldi: null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t vec = (int64_t)n * nrhs;
+  z64* Vj = (z64*)V + (int64_t)j * vec;
+  z64* Vn = Vj + vec;  // w, then V_{j+1}
+  z64* Wj = (z64*)W + (int64_t)j * vec;
+  z64* Hc = (z64*)H;              // (m + 1) x nrhs: this column
+  z64* T = Hc + (int64_t)(m + 1) * nrhs;  // (m + 1) x nrhs: second Gram-Schmidt pass
+  int rc;
+  if ((rc = mom_getrs_inv_z(prec, n, nrhs, LU, lda, perm, Vj, Wj, solve_work, stream))) return rc;
+  if (symv_work)
+    rc = mom_scaled_symv(n, nrhs, Z, ldz, dsc, Wj, Vn, symv_work, symv_bytes, stream);
+  else
+    rc = mom_scaled_matvec(n, nrhs, Z, ldz, dsc, Wj, Vn, stream);
+  if (rc) return rc;
+  if ((rc = mom_basis_dot(n, nrhs, j + 1, V, Vn, Hc, stream))) return rc;
+  if ((rc = mom_basis_sub(n, nrhs, j + 1, V, Hc, Vn, stream))) return rc;
+  if ((rc = mom_basis_dot(n, nrhs, j + 1, V, Vn, T, stream))) return rc;
+  if ((rc = mom_basis_sub(n, nrhs, j + 1, V, T, Vn, stream))) return rc;
+  const int cnt = (j + 1) * nrhs;
+  hipLaunchKernelGGL(h_add_kernel, dim3((cnt + 255) / 256), dim3(256), 0, s, cnt, Hc, (const z64*)T);
+  if ((rc = check_launch("h_add_kernel"))) return rc;
+  hipLaunchKernelGGL(col_norm_kernel, dim3(nrhs), dim3(256), 0, s, n, nrhs, (const z64*)Vn, Hc + (int64_t)(j + 1) * nrhs);
+  if ((rc = check_launch("col_norm_kernel"))) return rc;
+  hipLaunchKernelGGL(col_scale_kernel, dim3((unsigned)((vec + 255) / 256)), dim3(256), 0, s, n, nrhs, Vn,
+                     (const z64*)(Hc + (int64_t)(j + 1) * nrhs));
+  if ((rc = check_launch("col_scale_kernel"))) return rc;
+  MOM_TRY(hipMemcpyAsync(Hh, Hc, sizeof(z64) * (size_t)(j + 2) * nrhs, hipMemcpyDeviceToHost, s));
+  return 0;
+}

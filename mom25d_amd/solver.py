@@ -13,6 +13,7 @@ Per frequency of a sweep:
 """
 
 import ctypes
+import math
 import multiprocessing
 import os
 import sys
@@ -306,19 +307,27 @@ def givens_column(h, cs, sn, g, j):
     triangular form in place (GMRES; one problem per right-hand side along the
     last axis): apply the earlier rotations G_i = [[c_i*, s_i*], [-s_i, c_i]]
     to h (length j + 2), make the rotation that zeroes h[j + 1], and apply it
-    to the right-hand side g.  |g[j + 1]| is then the least-squares residual."""
-    for i in range(j):
-        t = np.conj(cs[i]) * h[i] + np.conj(sn[i]) * h[i + 1]
-        h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1]
-        h[i] = t
-    den = np.sqrt(np.abs(h[j]) ** 2 + np.abs(h[j + 1]) ** 2)
-    den = np.where(den == 0, 1.0, den)
-    cs[j] = h[j] / den
-    sn[j] = h[j + 1] / den
-    h[j] = np.conj(cs[j]) * h[j] + np.conj(sn[j]) * h[j + 1]
-    h[j + 1] = 0
-    g[j + 1] = -sn[j] * g[j]
-    g[j] = np.conj(cs[j]) * g[j]
+    to the right-hand side g.  |g[j + 1]| is then the least-squares residual.
+    Scalar Python complex arithmetic: the arrays are short (a few right-hand
+    sides), where per-element numpy calls would cost more than the work."""
+    for c in range(h.shape[1]):
+        col = h[:, c].tolist()
+        csc, snc = cs[:j, c].tolist(), sn[:j, c].tolist()
+        for i in range(j):
+            ci, si = csc[i], snc[i]
+            a, b = col[i], col[i + 1]
+            col[i] = ci.conjugate() * a + si.conjugate() * b
+            col[i + 1] = -si * a + ci * b
+        a, b = col[j], col[j + 1]
+        den = math.sqrt(abs(a) ** 2 + abs(b) ** 2) or 1.0
+        cj, sj = a / den, b / den
+        col[j] = cj.conjugate() * a + sj.conjugate() * b
+        col[j + 1] = 0j
+        cs[j, c], sn[j, c] = cj, sj
+        h[:, c] = col
+        gj = complex(g[j, c])
+        g[j + 1, c] = -sj * gj
+        g[j, c] = cj.conjugate() * gj
 
 
 class Solver:
@@ -443,7 +452,9 @@ class Solver:
         if autotune:
             from .autotune import tune_lu
 
-            best = tune_lu(mesh.n_unknowns, 64 if precision == "fp64" else 32, self.device)
+            # the mixed path factors its preconditioners without row interchanges when nopiv is set
+            best = tune_lu(mesh.n_unknowns, 64 if precision == "fp64" else 32, self.device,
+                           pivot=not (precision == "mixed" and nopiv))
             self.panel, self.block, self.tile = best["jb"], best["nb"], tuple(best["tile"])
         self.tuning = dict(jb=self.panel, nb=self.block, tile=self.tile)
         self.refine_tol = refine_tol
@@ -517,7 +528,21 @@ class Solver:
             self._bufs[key] = b
         return b
 
+    def _pinned(self, name, shape, dtype, lane=0):
+        """Pinned host buffer kept between calls (one per lane)."""
+        key = ("pinned", name, lane)
+        b = self._bufs.get(key)
+        if b is None or tuple(b.shape) != tuple(shape) or b.dtype != dtype:
+            b = torch.empty(shape, dtype=dtype, pin_memory=True)
+            self._bufs[key] = b
+        return b
+
     PANEL_LDS = 152 * 1024  # LDS of one LU panel workgroup (the library's bound)
+    # GMRES products from the tiles on and above the diagonal of Z (mom_scaled_symv):
+    # valid because every Z the solver builds is symmetric (the fill ends with
+    # Z <- (Z + Z^T) / 2 as Engine/mom25d/solver.py:522-524 does; the frequency
+    # interpolation preserves it); False: the row-wise product over all of Z
+    symmetric_products = True
 
     def _worker_count(self, nf):
         """Processes for the wave fits of a sweep (0: in this process)."""
@@ -540,8 +565,14 @@ class Solver:
             # overlaps them (measured at N = 12 428: 4.33 s vs 4.46-4.66 s for 51 points)
             want = 4 if n <= 4000 else (3 if self.precision == "mixed" else 2)
             lanes = max(1, min(want, int(free / 3 // per)))
-        elem = 16 if self.precision == "fp64" else 8
-        rows = max(1, self.PANEL_LDS // (self.panel * elem))
+        if self.precision == "mixed" and self.nopiv:
+            # unpivoted factorisations wait for no other workgroup; a pivoted
+            # one is only a fallback, whose panel narrows (down to 16 columns
+            # of complex128) until its rows fit its share of the CUs
+            elem, jb = 16, 16
+        else:
+            elem, jb = (16 if self.precision == "fp64" else 8), self.panel
+        rows = max(1, self.PANEL_LDS // (jb * elem))
         groups = -(-n // rows)
         cus = torch.cuda.get_device_properties(self.device).multi_processor_count
         return max(1, min(lanes, min(cus, 256) // groups))
@@ -696,9 +727,14 @@ class Solver:
         ipiv = self._buf("ipiv", (n,), torch.int32, lane)
         perm = self._buf("perm", (n,), torch.int32, lane)
         info = self._buf("info", (4,), torch.int32, lane)
+        nopiv = Z32 is not None and self.nopiv and not pivot
+        groups = self._panel_groups()
+        elem = 16 if prec == 64 else 8
+        # a pivoted panel's rows must fit the LDS of `groups` workgroups
+        while not nopiv and groups and jb > 16 and -(-n // max(1, self.PANEL_LDS // (elem * jb))) > groups:
+            jb //= 2
         wbytes = int(_lib.lib().mom_getrf_work_bytes(prec, n, jb))
         work = self._buf(f"work{prec}", (wbytes,), torch.uint8, lane)
-        groups = self._panel_groups()
         # the complex64 solves (GMRES preconditioner) use the inverted diagonal
         # blocks of the factors, kept after the getrs workspace
         swb = _lib.lib().mom_getrs_work_bytes(prec, n) if prec == 64 else _lib.lib().mom_getrs_inv_work_bytes(prec, n)
@@ -709,8 +745,6 @@ class Solver:
             _lib.call("mom_set_gemm_tile", prec, self.tile[0], self.tile[1])
         if self.block is not None:
             _lib.call("mom_set_lu_block", prec, self.block)
-
-        nopiv = Z32 is not None and self.nopiv and not pivot
 
         def calls():
             if nopiv:
@@ -787,14 +821,21 @@ class Solver:
         self.last_refine = its
         return X
 
-    def _check_solve(self, lane=0):
+    def _status_tensors(self, lane=0):
+        """The device status words of the lane's factorisation and solves."""
+        return [self._bufs[("info", lane)]] + [self._bufs[(k, lane)][:20] for k in ("swork32", "swork64", "swork64i")
+                                               if (k, lane) in self._bufs]
+
+    def _check_solve(self, lane=0, status=None):
         """Status words of the lane's last factorisation and solves (read
-        after the solve's result has been copied to the host): raise if a
-        bounded wait of the LU kernels ran out, warn on an exactly zero
+        after the solve's result has been copied to the host; status: their
+        host copies, as _status_tensors lists them, when already read): raise
+        if a bounded wait of the LU kernels ran out, warn on an exactly zero
         pivot (as LAPACK's zgetrf reports; the solution is then not finite)."""
-        info = self._bufs[("info", lane)].cpu().numpy()
-        waits = [self._bufs[(k, lane)][:20].cpu().numpy().view(np.uint32)[4]
-                 for k in ("swork32", "swork64", "swork64i") if (k, lane) in self._bufs]
+        if status is None:
+            status = [t.cpu().numpy() for t in self._status_tensors(lane)]
+        info = status[0]
+        waits = [w.view(np.uint32)[4] for w in status[1:]]
         if info[1] or any(waits):
             raise RuntimeError("LU kernels: a bounded wait ran out (workgroups did not all run); result invalid")
         if info[0]:
@@ -837,9 +878,16 @@ class Solver:
                       stream)
             return out
 
+        if self.symmetric_products:  # Z = Z^T (the fill symmetrises it): half the HBM traffic
+            wbs = int(_lib.lib().mom_scaled_symv_work_bytes(n, nrhs))
+            swk = self._buf("symv_work", (wbs,), torch.uint8, lane)
+
         def matvec(Y):  # (D Z D) Y
             W = torch.empty_like(B)
-            _lib.call("mom_scaled_matvec", n, nrhs, P(Z), Z.stride(0), P(dv), P(Y), P(W), stream)
+            if self.symmetric_products:
+                _lib.call("mom_scaled_symv", n, nrhs, P(Z), Z.stride(0), P(dv), P(Y), P(W), P(swk), wbs, stream)
+            else:
+                _lib.call("mom_scaled_matvec", n, nrhs, P(Z), Z.stride(0), P(dv), P(Y), P(W), stream)
             return W
 
         def residual(Y, rowsum=None):  # D (b - Z D Y)
@@ -853,18 +901,20 @@ class Solver:
             Y0 = X0 / d
             Y = Y0 + precond(residual(Y0))
         R = residual(Y, rows)
-        yield
-        znorm = float(rows.max())
+        (zmax,) = yield from _host(rows.max())
+        znorm = float(zmax)
         its = 0
         prev = None
         Vb = torch.empty((m + 1, n, nrhs), dtype=torch.complex128, device=self.device)
         Wb = torch.empty((m, n, nrhs), dtype=torch.complex128, device=self.device)
+        Hd = self._buf("gmres_h", (2 * (m + 1) * nrhs,), torch.complex128, lane)
+        Hh = self._pinned("gmres_hh", ((m + 1) * nrhs,), torch.complex128, lane)
+        ev = torch.cuda.Event()
         for _restart in range(self.refine_maxit):
             beta = torch.linalg.vector_norm(R, dim=0)
             ynorm = torch.linalg.vector_norm(Y, dim=0)
-            yield
-            bh = beta.cpu().numpy()
-            target = self.refine_tol * znorm * ynorm.cpu().numpy()
+            bh, yh = yield from _host(beta, ynorm)
+            target = self.refine_tol * znorm * yh
             # converged, or stalled at the FP64 residual floor (within 1e3 of the target)
             if np.all(bh <= target) or (prev is not None and np.all((bh > 0.5 * prev) & (bh < 1e3 * target))):
                 break
@@ -877,19 +927,15 @@ class Solver:
             sn = np.zeros((m, nrhs), complex)
             k = 0
             for j in range(m):
-                precond(Vb[j], Wb[j])
-                w = matvec(Wb[j])
-                hj = torch.zeros((j + 2, nrhs), dtype=torch.complex128, device=self.device)
-                h = torch.empty((j + 1, nrhs), dtype=torch.complex128, device=self.device)
-                for _ in range(2):  # classical Gram-Schmidt, twice
-                    _lib.call("mom_basis_dot", n, nrhs, j + 1, P(Vb), P(w), P(h), stream)
-                    _lib.call("mom_basis_sub", n, nrhs, j + 1, P(Vb), P(h), P(w), stream)
-                    hj[:j + 1] += h
-                nw = torch.linalg.vector_norm(w, dim=0)
-                hj[j + 1] = nw
-                Vb[j + 1] = w / nw
-                yield
-                hc = hj.cpu().numpy()
+                # one call: preconditioner solve, product, Gram-Schmidt twice, normalisation, and
+                # the new Hessenberg column copied to pinned host memory
+                _lib.call("mom_gmres_arnoldi", pc_prec, n, nrhs, j, m, P(A32), A32.stride(0), P(perm), P(sw), P(Z),
+                          Z.stride(0), P(dv), P(swk) if self.symmetric_products else None,
+                          wbs if self.symmetric_products else 0, P(Vb), P(Wb), P(Hd), P(Hh), stream)
+                ev.record()
+                while not ev.query():
+                    yield
+                hc = Hh.numpy()[:(j + 2) * nrhs].reshape(j + 2, nrhs).copy()
                 its += 1
                 givens_column(hc, cs, sn, g, j)
                 H[:j + 2, j] = hc
@@ -910,11 +956,11 @@ class Solver:
             # restarts exhausted: report a result short of the FP64 floor
             beta = torch.linalg.vector_norm(R, dim=0)
             ynorm = torch.linalg.vector_norm(Y, dim=0)
-            yield
-            bh, target = beta.cpu().numpy(), self.refine_tol * znorm * ynorm.cpu().numpy()
+            bh, yh = yield from _host(beta, ynorm)
+            target = self.refine_tol * znorm * yh
             if not np.all(bh < 1e3 * target):
                 warnings.warn(f"mixed-precision solve: GMRES stopped after {its} iterations at a normwise backward "
-                              f"error of {np.max(bh / (znorm * ynorm.cpu().numpy())):.1e} (target {self.refine_tol:.0e})",
+                              f"error of {np.max(bh / (znorm * yh)):.1e} (target {self.refine_tol:.0e})",
                               RuntimeWarning, stacklevel=2)
         self.last_refine = its
         self.lu_stats["gmres_steps"] += its
@@ -1298,6 +1344,23 @@ class Solver:
         return reciprocal_z0(self._fit_context(), SI, zphi, prior)
 
 
+def _host(*tensors):
+    """Generator: numpy copies of device tensors, read without holding up the
+    other lanes: asynchronous copies into pinned host memory on the current
+    stream, then it yields until they have landed (a blocking read would stop
+    the host from feeding the other lanes' streams meanwhile)."""
+    bufs = []
+    for t in tensors:
+        h = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+        h.copy_(t, non_blocking=True)
+        bufs.append(h)
+    ev = torch.cuda.Event()
+    ev.record()
+    while not ev.query():
+        yield
+    return [h.numpy().copy() for h in bufs]
+
+
 def _column_chunks(n, width=4):
     """Column ranges of at most `width` columns covering 0 .. n-1."""
     return [(c, min(c + width, n)) for c in range(0, n, width)]
@@ -1448,10 +1511,11 @@ def _sweep_many(jobs, lanes, workers, keep_currents=False, graphs=False):
         s = o["solver"]
         g = o["tabs"][fi]
         x_dev = yield from s._solve_task(g, o["B"], lane, graphs, reuse=True, interp=o["interp"])
-        yield  # let the other lanes enqueue their work before waiting for this one
+        # read without blocking: the other lanes enqueue their work meanwhile
+        xh, *status = yield from _host(x_dev, *s._status_tensors(lane))
         f = o["freqs"][fi]
-        x = s._apply_lumped(x_dev.cpu().numpy(), n_exc[ji], f)
-        s._check_solve(lane)
+        x = s._apply_lumped(xh, n_exc[ji], f)
+        s._check_solve(lane, status)
         if keep_currents:
             o["cur"][fi] = x
         h0 = time.perf_counter()
