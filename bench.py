@@ -175,7 +175,14 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched sweep of the 16 random layouts")
     ap.add_argument("--no-interp", action="store_true",
                     help="skip timing the headline path with the interpolated impedance (interp='auto')")
+    ap.add_argument("--hw-queues", type=int, default=16,
+                    help="GPU_MAX_HW_QUEUES of this process (HIP's default is 4)")
     args = ap.parse_args()
+    # One hardware queue per stream: each lane of a sweep has a stream, and its factorisations a look-ahead
+    # stream.  With HIP's 4 queues per process, streams share queues, and a kernel waits behind the kernels
+    # (and event waits) of another lane queued on the same hardware queue.  Read by HIP when it initialises,
+    # so it is set before torch touches the GPU.
+    os.environ["GPU_MAX_HW_QUEUES"] = str(max(1, min(32, args.hw_queues)))
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py: this build runs on one GPU only (frequency sharding across GPUs is a design note, "

@@ -307,12 +307,14 @@ static constexpr int kNoPivPanelMax() {  // widest unpivoted panel (columns held
 // difference): no column needs a cross-workgroup pivot search, so a panel is
 // two ordinary launches -- the b x b diagonal block factored by one workgroup,
 // then the rows below it, L21 = A21 U11^-1, by independent workgroups of 16
-// rows -- instead of one exchange per column.  Both keep their part of the
-// matrix in registers (thread (c, q): column c of the rows i = q mod Q) and
-// exchange only one row or column per step through LDS, so neither holds a
-// large LDS allocation: the rows-below workgroups (256 threads, 1 KB of LDS)
-// find room on CUs that the concurrent trailing-update GEMM of the look-ahead
-// still occupies.  An exactly zero pivot is replaced by 1 and reported in
+// rows -- instead of one exchange per column.  The rows below keep their part
+// of the matrix in registers (thread (c, q): column c of the rows i = q mod Q)
+// and exchange only the column being finished through LDS (1 KB), so that
+// these 256-thread workgroups find room on CUs that the concurrent
+// trailing-update GEMM of the look-ahead still occupies.  (A register form of
+// the diagonal block, two barriers per column, measured slower than the LDS
+// form below: 280 against 250 us per 128-column block.)  An exactly zero
+// pivot is replaced by 1 and reported in
 // info[0] (first one, 1-based), which leaves finite factors that the caller
 // discards.
 template <typename R>
@@ -320,71 +322,53 @@ __device__ inline cplx<R> nonzero_pivot(cplx<R> p) {
   return (p.re == (R)0 && p.im == (R)0) ? cplx<R>{(R)1, (R)0} : p;
 }
 
-// The diagonal block: 1024 threads, thread (c, q) = (tid % NC, tid / NC)
-// holds column c of the rows q, q + Q, ... (NC = 128 columns and 16 rows per
-// thread in complex64, 64 columns and 4 rows in complex128).  Step j: the
-// owners of row j publish its columns >= j; the owners of column j scale
-// their rows below j by 1 / u_jj and publish them; every column c > j then
-// subtracts l_ij u_jc from its rows below j.  Two barriers per step; the row
-// and column buffers alternate between steps, so step j + 1 publishes while a
-// slower wavefront may still read step j's.
+// The diagonal block: one workgroup of 1024 threads with the block in LDS
+// (b x b, up to 128 KB, which also keeps the look-ahead's update GEMM off
+// its CU); thread (c, q) updates column c of the rows i = q mod 8, 8 rows at
+// a time (their loads issued together), one barrier per column.  Column j is
+// scaled by 1 / u_jj in step j + 1, when no one reads it any more.
 template <typename R>
 __global__ __launch_bounds__(1024) void diag_lu_kernel(cplx<R>* A, int64_t lda, int k0, int b, int32_t* ipiv,
                                                        int32_t* info) {
-  constexpr int NC = kNoPivPanelMax<R>(), Q = 1024 / NC, HR = NC / Q;
-  __shared__ cplx<R> rowbuf[2][NC];
-  __shared__ cplx<R> colbuf[2][NC];
-  const int tid = threadIdx.x, c = tid % NC;
-  const int q = __builtin_amdgcn_readfirstlane(tid / NC);  // uniform per wavefront
-  const bool mine = c < b;
-  cplx<R> v[HR];
-#pragma unroll
-  for (int t = 0; t < HR; ++t) {
-    const int i = q + Q * t;
-    v[t] = cplx<R>{(R)0, (R)0};
-    if (mine && i < b) v[t] = A[(int64_t)(k0 + i) * lda + k0 + c];
-  }
+  // the block in LDS; thread (c, q) updates column c of the rows i = q mod 8,
+  // 8 rows at a time (their loads issued together)
+  extern __shared__ unsigned char smem_raw[];
+  cplx<R>* T = reinterpret_cast<cplx<R>*>(smem_raw);  // [b][b]
+  const int tid = threadIdx.x, c = tid % 128;
+  const int q = __builtin_amdgcn_readfirstlane(tid / 128);  // uniform per wavefront
+  for (int e = tid; e < b * b; e += 1024) T[e] = A[(int64_t)(k0 + e / b) * lda + k0 + e % b];
   for (int e = tid; e < b; e += 1024) ipiv[k0 + e] = k0 + e;
+  __syncthreads();
   for (int j = 0; j < b; ++j) {
-    const int buf = j & 1;
-    if (q == (j % Q) && c >= j && mine) {  // row j (register j / Q, selected without dynamic indexing)
-      cplx<R> x = v[0];
-#pragma unroll
-      for (int t = 1; t < HR; ++t)
-        if (t == j / Q) x = v[t];
-      rowbuf[buf][c] = x;
+    const cplx<R> p = T[j * b + j];  // final since step j - 1
+    if (j > 0) {  // finish column j - 1: its rows below the diagonal scaled, one row per thread
+      const cplx<R> pp = T[(j - 1) * b + j - 1];
+      if (j + tid < b) T[(j + tid) * b + j - 1] = T[(j + tid) * b + j - 1] * cdiv(cplx<R>{(R)1, (R)0},
+                                                                                     nonzero_pivot(pp));
+      if (tid == 0) T[(j - 1) * b + j - 1] = nonzero_pivot(pp);
     }
-    __syncthreads();
-    const cplx<R> p = rowbuf[buf][j];
     if (tid == 0 && p.re == (R)0 && p.im == (R)0 && *info == 0) *info = k0 + j + 1;
-    if (c == j) {
-      const cplx<R> inv = cdiv(cplx<R>{(R)1, (R)0}, nonzero_pivot(p));
+    if (c > j && c < b) {
+      const cplx<R> m = cdiv(T[j * b + c], nonzero_pivot(p));
+      const int first = j + 1 + ((q - (j + 1)) & 7);  // first row > j of this class
+      for (int i0 = first; i0 < b; i0 += 64) {
+        cplx<R> a[8], t[8];
 #pragma unroll
-      for (int t = 0; t < HR; ++t) {
-        const int i = q + Q * t;
-        if (i > j && i < b) {
-          v[t] = v[t] * inv;
-          colbuf[buf][i] = v[t];
-        } else if (i == j) {
-          v[t] = nonzero_pivot(p);
+        for (int k = 0; k < 8; ++k) {
+          const int i = min(i0 + 8 * k, b - 1);
+          a[k] = T[i * b + j];
+          t[k] = T[i * b + c];
         }
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          if (i0 + 8 * k < b) T[(i0 + 8 * k) * b + c] = t[k] - a[k] * m;
       }
     }
     __syncthreads();
-    if (c > j && mine) {
-      const cplx<R> u = rowbuf[buf][c];
-#pragma unroll
-      for (int t = 0; t < HR; ++t) {
-        const int i = q + Q * t;
-        if (i > j && i < b) v[t] -= colbuf[buf][i] * u;
-      }
-    }
   }
-#pragma unroll
-  for (int t = 0; t < HR; ++t) {
-    const int i = q + Q * t;
-    if (mine && i < b) A[(int64_t)(k0 + i) * lda + k0 + c] = v[t];
-  }
+  if (tid == 0) T[(b - 1) * b + b - 1] = nonzero_pivot(T[(b - 1) * b + b - 1]);
+  __syncthreads();
+  for (int e = tid; e < b * b; e += 1024) A[(int64_t)(k0 + e / b) * lda + k0 + e % b] = T[e];
 }
 
 // L21 = A21 U11^-1 for the rows [k0 + b, n) below a factored diagonal block:
@@ -1299,7 +1283,16 @@ template <typename R>
 static int launch_panel_nopiv(cplx<R>* A, int64_t lda, int n, int k0, int b, int32_t* ipiv, int32_t* info,
                               hipStream_t s) {
   if (b > kNoPivPanelMax<R>()) return fail("mom_getrf_nopiv: panel wider than the diagonal-block kernel");
-  hipLaunchKernelGGL(diag_lu_kernel<R>, dim3(1), dim3(1024), 0, s, A, lda, k0, b, ipiv, info);
+  static bool attr_set = false;
+  if (!attr_set) {
+    const size_t most = sizeof(cplx<R>) * (size_t)kNoPivPanelMax<R>() * kNoPivPanelMax<R>();
+    if (hipFuncSetAttribute((const void*)diag_lu_kernel<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)most) !=
+        hipSuccess)
+      return fail("diag_lu_kernel: cannot reserve LDS");
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(diag_lu_kernel<R>, dim3(1), dim3(1024), sizeof(cplx<R>) * (size_t)b * b, s, A, lda, k0, b, ipiv,
+                     info);
   int rc = check_launch("diag_lu_kernel");
   if (rc) return rc;
   const int below = n - k0 - b;
@@ -1914,9 +1907,10 @@ extern "C" int mom_scaled_matvec(int n, int nrhs, const void* Z, int64_t ldz, co
 // 64 x 64 tile Z_IJ once and forms both its row product Z_IJ (D X)_J and,
 // off the diagonal, its column product Z_IJ^T (D X)_I, which stands for the
 // tile Z_JI below the diagonal.  Thread (a, b) = (tid / 16, tid % 16) holds
-// the 4 x 4 entries of rows 4a.., columns 4b..; row sums reduce over the 16
-// lanes of equal a, column sums over the lanes of equal b and the 4
-// wavefronts.  The partial products go to Q[K][S] (64 x NR each): the row
+// the 4 x 4 entries of rows 4a + u and columns b + 16 v (u, v < 4), so that
+// each load instruction reads 16 consecutive entries of a row per row group;
+// row sums reduce over the 16 lanes of equal a, column sums over the lanes
+// of equal b and the 4 wavefronts.  The partial products go to Q[K][S] (64 x NR each): the row
 // product of tile (I, J) to Q[I][J], the column product to Q[J][I], so that
 // block K's partials are Q[K][0 .. nb) -- summed in a fixed order by
 // symv_reduce_kernel, which also applies D (deterministic: no atomics).
@@ -1931,12 +1925,12 @@ __global__ __launch_bounds__(256) void symv_tile_kernel(int n, const z64* __rest
   const int nb = gridDim.x;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int a = tid >> 4, b = tid & 15;
-  const int r0 = I * kSymvT + 4 * a, c0 = J * kSymvT + 4 * b;
+  const int r0 = I * kSymvT + 4 * a, c0 = J * kSymvT + b;
   typedef double dv2 __attribute__((ext_vector_type(2)));
   z64 yi[4][NR], yj[4][NR];
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
-    const int r = r0 + u, c = c0 + u;
+    const int r = r0 + u, c = c0 + 16 * u;
     const double dr = r < n ? dsc[r] : 0.0, dc = c < n ? dsc[c] : 0.0;
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
@@ -1954,8 +1948,8 @@ __global__ __launch_bounds__(256) void symv_tile_kernel(int n, const z64* __rest
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       z[u][v] = {0.0, 0.0};
-      if (r < n && c0 + v < n) {
-        const dv2 t = __builtin_nontemporal_load(zr + v);
+      if (r < n && c0 + 16 * v < n) {
+        const dv2 t = __builtin_nontemporal_load(zr + 16 * v);
         z[u][v] = {t.x, t.y};
       }
     }
@@ -2009,7 +2003,7 @@ __global__ __launch_bounds__(256) void symv_tile_kernel(int n, const z64* __rest
       im += __shfl_xor(im, 16, 64);
       re += __shfl_xor(re, 32, 64);
       im += __shfl_xor(im, 32, 64);
-      if (lane < 16) cs[wave][4 * b + v][k] = {re, im};
+      if (lane < 16) cs[wave][b + 16 * v][k] = {re, im};
     }
   __syncthreads();
   z64* qcol = Q + ((int64_t)J * nb + I) * kSymvT * NR;
@@ -2023,16 +2017,25 @@ __global__ __launch_bounds__(256) void symv_tile_kernel(int n, const z64* __rest
   }
 }
 
+// block K of W: thread (p, e) sums the partials S = p, p + 4, ... of entry e
+// (4 independent chains of loads per entry), then the 4 sums in order.
 template <int NR>
-__global__ __launch_bounds__(256) void symv_reduce_kernel(int n, int nb, const z64* __restrict__ Q,
-                                                          const double* __restrict__ dsc, z64* __restrict__ W) {
-  const int K = blockIdx.x, e = threadIdx.x;
-  if (e >= kSymvT * NR) return;
-  const int row = K * kSymvT + e / NR;
-  const z64* q = Q + (int64_t)K * nb * kSymvT * NR + e;
+__global__ __launch_bounds__(1024) void symv_reduce_kernel(int n, int nb, const z64* __restrict__ Q,
+                                                           const double* __restrict__ dsc, z64* __restrict__ W) {
+  constexpr int E = kSymvT * NR, PS = 1024 / E;  // entries per block, partitions of S
+  __shared__ z64 part[PS][E];
+  const int K = blockIdx.x, e = threadIdx.x % E, p = threadIdx.x / E;  // p == PS: idle threads (NR = 3)
+  const z64* q = Q + (int64_t)K * nb * E + e;
   z64 t = {0.0, 0.0};
-  for (int S = 0; S < nb; ++S) t += q[(int64_t)S * kSymvT * NR];
-  if (row < n) W[(int64_t)K * kSymvT * NR + e] = t * dsc[row];
+  if (p < PS)
+    for (int S = p; S < nb; S += PS) t += q[(int64_t)S * E];
+  if (p < PS) part[p][e] = t;
+  __syncthreads();
+  if (p != 0) return;
+  z64 sum = part[0][e];
+  for (int k = 1; k < PS; ++k) sum += part[k][e];
+  const int row = K * kSymvT + e / NR;
+  if (row < n) W[(int64_t)K * E + e] = sum * dsc[row];
 }
 
 extern "C" int64_t mom_scaled_symv_work_bytes(int n, int nrhs) {
@@ -2058,7 +2061,7 @@ extern "C" int mom_scaled_symv(int n, int nrhs, const void* Z, int64_t ldz, cons
   case NR:                                                                                                    \
     hipLaunchKernelGGL((symv_tile_kernel<NR>), dim3(nb, nb), dim3(256), 0, s, n, z, ldz, x, dsc, q);          \
     if (int rc = check_launch("symv_tile_kernel")) return rc;                                                 \
-    hipLaunchKernelGGL((symv_reduce_kernel<NR>), dim3(nb), dim3(256), 0, s, n, nb, (const z64*)q, dsc, w);     \
+    hipLaunchKernelGGL((symv_reduce_kernel<NR>), dim3(nb), dim3(1024), 0, s, n, nb, (const z64*)q, dsc, w);    \
     break;
   switch (nrhs) {
     MOM_SYMV(1)

@@ -1184,8 +1184,9 @@ class Solver:
         model of this implementation on MI355X -- the LU at ~50 TFLOP/s
         effective or ~2.5 us per column of panel latency, whichever is larger
         (unpivoted, nopiv; pivoted: ~32 TFLOP/s or ~8 us per column); a step =
-        two triangular block chains of ~2.9 us per 64 rows + one FP64 Z read
-        + ~0.2 ms of host round trip -- times 1.4, because the steps of
+        two triangular block chains of ~2.9 us per 64 rows + one FP64 read
+        of Z (of its upper triangle with symmetric_products) + ~0.2 ms of
+        host round trip -- times 1.4, because the steps of
         concurrent lanes overlap better than factorisations do (measured on
         the 12.4 k-unknown pour, unpivoted: 17.3 points/s at the model's 78,
         16.9 at 123, 14.6 at 50)."""
@@ -1194,7 +1195,7 @@ class Solver:
         n = self.mesh.n_unknowns
         rate, per_col = (50e12, 2.5e-6) if self.nopiv else (32e12, 8e-6)
         t_lu = max(8 / 3 * n ** 3 / rate, per_col * n)
-        t_step = 2 * (n / 64) * 2.9e-6 + 16 * n * n / 5e12 + 2e-4
+        t_step = 2 * (n / 64) * 2.9e-6 + (8 if self.symmetric_products else 16) * n * n / 5e12 + 2e-4
         return 1.4 * t_lu / t_step
 
     @staticmethod
