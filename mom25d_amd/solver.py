@@ -1044,7 +1044,9 @@ class Solver:
         (set up for this solver) supplies Z instead of the fill."""
         n = self.mesh.n_unknowns
         Z = self._buf("Z", (n, n), torch.complex128, lane)
-        mixed = self.precision == "mixed"
+        # mixed precision relies on a symmetric Z (the fill's and the interpolation's are); a given
+        # non-symmetric matrix is solved by the complex128 LU instead
+        mixed = self.precision == "mixed" and getattr(interp, "symmetric", True)
         Z32 = self._buf("Z32", (n, n), torch.complex64, lane) if mixed else None
 
         def fill(Z32x):
@@ -1372,6 +1374,8 @@ class _GivenZ:
 
     def __init__(self, Z):
         self.Z = np.ascontiguousarray(Z, complex)
+        # the mixed path's GMRES reads only one triangle of Z; another Z takes the complex128 LU
+        self.symmetric = bool(np.array_equal(self.Z, self.Z.T))
 
     def fill(self, f, Zdev, lane=0):
         Zdev.copy_(torch.from_numpy(self.Z))

@@ -52,6 +52,32 @@ def test_currents_backward_stable(name, f, precision):
     assert fw < 1e-7
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_given_matrix_symmetric_or_not(precision):
+    """solve_currents(Z=...) solves the matrix it is given: a symmetric one
+    through the path's own solver (the mixed path's GMRES reads one triangle
+    of Z), a non-symmetric one (which the mixed path may not symmetrise or
+    read by one triangle) through the complex128 LU -- both to LAPACK's
+    answer on that matrix."""
+    import scipy.linalg as sla
+
+    lay, st, mk, _ = W.tee()
+    m = lay.mesh(**mk)
+    s = Solver(m, st, verbose=False, precision=precision)
+    f = 4e9
+    Z = s.impedance(f)
+    b = s.port_excitations()
+    rng = np.random.default_rng(5)
+    K = rng.standard_normal(Z.shape)
+    # a skew part of 1e-8 relative: symmetrising it away would move x by ~cond(Z) 1e-8, far above the bars
+    Zn = Z + 1e-8 * np.abs(Z).max() * (K - K.T)
+    for M in (Z, Zn):
+        x = s.solve_currents(f, Z=M.copy())
+        ref = sla.lu_solve(sla.lu_factor(M), b)
+        assert np.max(np.abs(x - ref)) < 1e-7 * np.max(np.abs(ref))
+        assert np.linalg.norm(M @ x - b) < 1e-13 * np.linalg.norm(M, 2) * np.linalg.norm(x)
+
+
 def test_via_transition_sweep_matches_oracle():
     lay, st, mk, _ = W.via2()
     m = lay.mesh(**mk)

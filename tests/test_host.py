@@ -256,3 +256,18 @@ def test_lane_cursor_hands_out_runs_of_neighbours(count, lanes):
     assert jumps <= lanes * max(1, int(np.ceil(np.log2(count + 2))))
     if lanes <= 2:
         assert jumps <= 1
+
+
+def test_given_matrix_symmetry_flag():
+    """A given impedance matrix is marked symmetric only when it equals its
+    transpose exactly (the mixed path then keeps its one-triangle GMRES
+    product; otherwise the solve takes the complex128 LU)."""
+    from mom25d_amd.solver import _GivenZ
+
+    rng = np.random.default_rng(0)
+    A = rng.standard_normal((5, 5)) + 1j * rng.standard_normal((5, 5))
+    assert _GivenZ(A + A.T).symmetric
+    assert not _GivenZ(A).symmetric
+    B = A + A.T
+    B[1, 3] += 1e-15
+    assert not _GivenZ(B).symmetric
