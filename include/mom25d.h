@@ -200,9 +200,12 @@ int mom_scaled_symv(int n, int nrhs, const void* Z, int64_t ldz, const double* d
  * V_0 .. V_j with the summed coefficients in H[0 .. j], H[j+1] = |w| and
  * V_{j+1} = w / |w| per column, then H[0 .. j+1] ((j + 2) x nrhs) copied to
  * the pinned host buffer Hh on the stream.  V: (m + 1) x n x nrhs and W:
- * m x n x nrhs complex128; H: device workspace of 2 (m + 1) nrhs complex128;
+ * m x n x nrhs complex128; H: device workspace of mom_gmres_work_elems(m,
+ * nrhs) complex128 (the column, the second Gram-Schmidt pass, and the
+ * partial dot products and norms of 8 row ranges, summed in a fixed order);
  * 0 <= j < m, nrhs <= 4.  No counterpart in the reference, which solves in
  * FP64 directly (Engine/mom25d/solver.py:11-31). */
+int64_t mom_gmres_work_elems(int m, int nrhs);
 int mom_gmres_arnoldi(int prec, int n, int nrhs, int j, int m, const void* LU, int64_t lda, const int32_t* perm,
                       void* solve_work, const void* Z, int64_t ldz, const double* dsc, void* symv_work,
                       int64_t symv_bytes, void* V, void* W, void* H, void* Hh, void* stream);

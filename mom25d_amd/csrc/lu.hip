@@ -2142,34 +2142,87 @@ extern "C" int mom_basis_sub(int n, int nrhs, int j1, const void* V, const void*
 // buffer Hh asynchronously on the stream.
 namespace {
 
-__global__ void h_add_kernel(int cnt, z64* __restrict__ H, const z64* __restrict__ T) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < cnt) H[q] += T[q];
+constexpr int kOrthSplit = 8;  // row ranges of the step's dot products and norms
+
+// part[s][i][c] = sum over the rows of range s of conj(V[i, r, c]) w[r, c]:
+// workgroup (i, s), the ranges cutting n into kOrthSplit pieces.
+__global__ __launch_bounds__(256) void dot_part_kernel(int n, int nrhs, int j1, const z64* __restrict__ V,
+                                                       const z64* __restrict__ w, z64* __restrict__ part) {
+  const int i = blockIdx.x, sp = blockIdx.y, tid = threadIdx.x;
+  const int chunk = (n + kOrthSplit - 1) / kOrthSplit, r0 = sp * chunk, r1 = min(n, r0 + chunk);
+  z64 acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  const z64* Vi = V + (int64_t)i * n * nrhs;
+  for (int r = r0 + tid; r < r1; r += 256)
+    for (int c = 0; c < nrhs; ++c) {
+      const z64 v = Vi[(int64_t)r * nrhs + c], x = w[(int64_t)r * nrhs + c];
+      acc[c] += z64{v.re * x.re + v.im * x.im, v.re * x.im - v.im * x.re};
+    }
+  __shared__ z64 red[4][4];
+  const int lane = tid & 63, wave = tid >> 6;
+  for (int c = 0; c < nrhs; ++c) {
+    double re = acc[c].re, im = acc[c].im;
+    for (int off = 32; off > 0; off >>= 1) {
+      re += __shfl_down(re, off, 64);
+      im += __shfl_down(im, off, 64);
+    }
+    if (lane == 0) red[wave][c] = {re, im};
+  }
+  __syncthreads();
+  if (tid < nrhs) {
+    z64 t = red[0][tid];
+    t += red[1][tid];
+    t += red[2][tid];
+    t += red[3][tid];
+    part[((int64_t)sp * j1 + i) * nrhs + tid] = t;
+  }
 }
 
-// hn[c] = (|w[:, c]|, 0): one workgroup per column
-__global__ __launch_bounds__(256) void col_norm_kernel(int n, int nrhs, const z64* __restrict__ w, z64* hn) {
-  const int c = blockIdx.x, tid = threadIdx.x;
+// h[i][c] = sum_s part[s][i][c] in order; with H also H[i][c] += h[i][c]
+__global__ void dot_reduce_kernel(int cnt, const z64* __restrict__ part, z64* __restrict__ h, z64* __restrict__ H) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= cnt) return;
+  z64 t = part[q];
+  for (int sp = 1; sp < kOrthSplit; ++sp) t += part[(int64_t)sp * cnt + q];
+  h[q] = t;
+  if (H) H[q] += t;
+}
+
+// npart[s][c] = sum over the rows of range s of |w[r, c]|^2: workgroup (c, s)
+__global__ __launch_bounds__(256) void norm_part_kernel(int n, int nrhs, const z64* __restrict__ w,
+                                                        double* __restrict__ npart) {
+  const int c = blockIdx.x, sp = blockIdx.y, tid = threadIdx.x;
+  const int chunk = (n + kOrthSplit - 1) / kOrthSplit, r0 = sp * chunk, r1 = min(n, r0 + chunk);
   double t = 0.0;
-  for (int r = tid; r < n; r += 256) {
+  for (int r = r0 + tid; r < r1; r += 256) {
     const z64 v = w[(int64_t)r * nrhs + c];
     t += v.re * v.re + v.im * v.im;
   }
   for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
-  __shared__ double part[4];
-  if ((tid & 63) == 0) part[tid >> 6] = t;
+  __shared__ double red[4];
+  if ((tid & 63) == 0) red[tid >> 6] = t;
   __syncthreads();
-  if (tid == 0) hn[c] = {sqrt(part[0] + part[1] + part[2] + part[3]), 0.0};
+  if (tid == 0) npart[sp * nrhs + c] = red[0] + red[1] + red[2] + red[3];
 }
 
-__global__ void col_scale_kernel(int n, int nrhs, z64* __restrict__ w, const z64* __restrict__ hn) {
+// |w[:, c]| from the partials (summed in order by every thread), w[:, c] /= it,
+// and hn[c] = (|w[:, c]|, 0) written once
+__global__ void col_scale_kernel(int n, int nrhs, z64* __restrict__ w, const double* __restrict__ npart,
+                                 z64* __restrict__ hn) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n * nrhs) return;
-  const double inv = 1.0 / hn[q % nrhs].re;
-  w[q] = w[q] * inv;
+  const int c = q % nrhs;
+  double t = npart[c];
+  for (int sp = 1; sp < kOrthSplit; ++sp) t += npart[sp * nrhs + c];
+  const double nrm = sqrt(t);
+  if (q < nrhs) hn[q] = {nrm, 0.0};
+  w[q] = w[q] * (1.0 / nrm);
 }
 
 }  // namespace
+
+extern "C" int64_t mom_gmres_work_elems(int m, int nrhs) {
+  return (int64_t)(kOrthSplit + 2) * (m + 1) * nrhs + (kOrthSplit * nrhs + 1) / 2 + 1;
+}
 
 extern "C" int mom_gmres_arnoldi(int prec, int n, int nrhs, int j, int m, const void* LU, int64_t lda,
                                  const int32_t* perm, void* solve_work, const void* Z, int64_t ldz, const double* dsc,
@@ -2184,8 +2237,10 @@ extern "C" int mom_gmres_arnoldi(int prec, int n, int nrhs, int j, int m, const 
   z64* Vj = (z64*)V + (int64_t)j * vec;
   z64* Vn = Vj + vec;  // w, then V_{j+1}
   z64* Wj = (z64*)W + (int64_t)j * vec;
-  z64* Hc = (z64*)H;              // (m + 1) x nrhs: this column
-  z64* T = Hc + (int64_t)(m + 1) * nrhs;  // (m + 1) x nrhs: second Gram-Schmidt pass
+  z64* Hc = (z64*)H;                            // (m + 1) x nrhs: this column
+  z64* T = Hc + (int64_t)(m + 1) * nrhs;        // (m + 1) x nrhs: second Gram-Schmidt pass
+  z64* part = T + (int64_t)(m + 1) * nrhs;      // kOrthSplit x (m + 1) x nrhs: partial dot products
+  double* npart = reinterpret_cast<double*>(part + (int64_t)kOrthSplit * (m + 1) * nrhs);  // kOrthSplit x nrhs
   int rc;
   if ((rc = mom_getrs_inv_z(prec, n, nrhs, LU, lda, perm, Vj, Wj, solve_work, stream))) return rc;
   if (symv_work)
@@ -2193,17 +2248,21 @@ extern "C" int mom_gmres_arnoldi(int prec, int n, int nrhs, int j, int m, const 
   else
     rc = mom_scaled_matvec(n, nrhs, Z, ldz, dsc, Wj, Vn, stream);
   if (rc) return rc;
-  if ((rc = mom_basis_dot(n, nrhs, j + 1, V, Vn, Hc, stream))) return rc;
-  if ((rc = mom_basis_sub(n, nrhs, j + 1, V, Hc, Vn, stream))) return rc;
-  if ((rc = mom_basis_dot(n, nrhs, j + 1, V, Vn, T, stream))) return rc;
-  if ((rc = mom_basis_sub(n, nrhs, j + 1, V, T, Vn, stream))) return rc;
-  const int cnt = (j + 1) * nrhs;
-  hipLaunchKernelGGL(h_add_kernel, dim3((cnt + 255) / 256), dim3(256), 0, s, cnt, Hc, (const z64*)T);
-  if ((rc = check_launch("h_add_kernel"))) return rc;
-  hipLaunchKernelGGL(col_norm_kernel, dim3(nrhs), dim3(256), 0, s, n, nrhs, (const z64*)Vn, Hc + (int64_t)(j + 1) * nrhs);
-  if ((rc = check_launch("col_norm_kernel"))) return rc;
+  const int j1 = j + 1, cnt = j1 * nrhs;
+  const z64* Vc = (const z64*)V;
+  for (int pass = 0; pass < 2; ++pass) {  // classical Gram-Schmidt, twice
+    z64* h = pass == 0 ? Hc : T;
+    hipLaunchKernelGGL(dot_part_kernel, dim3(j1, kOrthSplit), dim3(256), 0, s, n, nrhs, j1, Vc, (const z64*)Vn, part);
+    if ((rc = check_launch("dot_part_kernel"))) return rc;
+    hipLaunchKernelGGL(dot_reduce_kernel, dim3((cnt + 255) / 256), dim3(256), 0, s, cnt, (const z64*)part, h,
+                       pass == 0 ? (z64*)nullptr : Hc);
+    if ((rc = check_launch("dot_reduce_kernel"))) return rc;
+    if ((rc = mom_basis_sub(n, nrhs, j1, V, h, Vn, stream))) return rc;
+  }
+  hipLaunchKernelGGL(norm_part_kernel, dim3(nrhs, kOrthSplit), dim3(256), 0, s, n, nrhs, (const z64*)Vn, npart);
+  if ((rc = check_launch("norm_part_kernel"))) return rc;
   hipLaunchKernelGGL(col_scale_kernel, dim3((unsigned)((vec + 255) / 256)), dim3(256), 0, s, n, nrhs, Vn,
-                     (const z64*)(Hc + (int64_t)(j + 1) * nrhs));
+                     (const double*)npart, Hc + (int64_t)j1 * nrhs);
   if ((rc = check_launch("col_scale_kernel"))) return rc;
   MOM_TRY(hipMemcpyAsync(Hh, Hc, sizeof(z64) * (size_t)(j + 2) * nrhs, hipMemcpyDeviceToHost, s));
   return 0;

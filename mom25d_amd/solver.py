@@ -907,7 +907,7 @@ class Solver:
         prev = None
         Vb = torch.empty((m + 1, n, nrhs), dtype=torch.complex128, device=self.device)
         Wb = torch.empty((m, n, nrhs), dtype=torch.complex128, device=self.device)
-        Hd = self._buf("gmres_h", (2 * (m + 1) * nrhs,), torch.complex128, lane)
+        Hd = self._buf("gmres_h", (int(_lib.lib().mom_gmres_work_elems(m, nrhs)),), torch.complex128, lane)
         Hh = self._pinned("gmres_hh", ((m + 1) * nrhs,), torch.complex128, lane)
         ev = torch.cuda.Event()
         for _restart in range(self.refine_maxit):

@@ -417,3 +417,62 @@ def test_scaled_symv_matches_full_product(n, nrhs, ld):
         assert np.max(np.abs(Wm.cpu().numpy() - refm)) < 1e-13 * np.sqrt(m) * np.max(np.abs(refm))
     with pytest.raises(RuntimeError):
         _lib.call("mom_scaled_symv", n, nrhs, P(Zd), ld, P(dd), P(Xd), P(W), P(work), wb - 16, s)
+
+
+@pytest.mark.parametrize("n,nrhs,symv", [(300, 2, True), (700, 1, False), (129, 3, True)])
+def test_gmres_arnoldi_step(n, nrhs, symv):
+    """The fused GMRES step: after k steps from a unit start block, the basis
+    is orthonormal per column and each step satisfies the Arnoldi relation
+    D Z D W_j = sum_i V_i H[i, j] with W_j = M^-1 V_j (the complex64 LU solve
+    of V_j), column by column; H[j+1, j] is real and positive.  The host
+    buffer receives the same column as the device workspace."""
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    Z = A + A.T + 4 * np.sqrt(n) * np.eye(n)  # symmetric, well conditioned
+    d = rng.uniform(0.5, 2.0, n)
+    DZD = d[:, None] * Z * d[None, :]
+    P = _lib.ptr
+    s = _lib.stream_ptr()
+    LUd = torch.from_numpy(DZD).to(DEV, torch.complex64)
+    ipiv = torch.empty(n, dtype=torch.int32, device=DEV)
+    perm = torch.empty(n, dtype=torch.int32, device=DEV)
+    info = torch.zeros(4, dtype=torch.int32, device=DEV)
+    wb = int(_lib.lib().mom_getrf_work_bytes(32, n, 64))
+    work = torch.empty(wb, dtype=torch.uint8, device=DEV)
+    sw = torch.empty(int(_lib.lib().mom_getrs_inv_work_bytes(32, n)), dtype=torch.uint8, device=DEV)
+    _lib.call("mom_getrf", 32, n, P(LUd), n, P(ipiv), P(info), P(work), wb, 64, 0, s)
+    _lib.call("mom_build_perm", n, P(ipiv), P(perm), s)
+    _lib.call("mom_trtri_blocks", 32, n, P(LUd), n, P(sw), s)
+    m, k = 8, 6
+    Zd = torch.from_numpy(Z).to(DEV)
+    dd = torch.from_numpy(d).to(DEV)
+    V = torch.zeros((m + 1, n, nrhs), dtype=torch.complex128, device=DEV)
+    W = torch.zeros((m, n, nrhs), dtype=torch.complex128, device=DEV)
+    v0 = rng.standard_normal((n, nrhs)) + 1j * rng.standard_normal((n, nrhs))
+    V[0] = torch.from_numpy(v0 / np.linalg.norm(v0, axis=0)).to(DEV)
+    H = torch.empty(int(_lib.lib().mom_gmres_work_elems(m, nrhs)), dtype=torch.complex128, device=DEV)
+    Hh = torch.empty((m + 1) * nrhs, dtype=torch.complex128, pin_memory=True)
+    swb = int(_lib.lib().mom_scaled_symv_work_bytes(n, nrhs))
+    swk = torch.empty(swb, dtype=torch.uint8, device=DEV)
+    cols = []
+    for j in range(k):
+        _lib.call("mom_gmres_arnoldi", 32, n, nrhs, j, m, P(LUd), n, P(perm), P(sw), P(Zd), n, P(dd),
+                  P(swk) if symv else None, swb if symv else 0, P(V), P(W), P(H), P(Hh), s)
+        torch.cuda.synchronize()
+        hc = Hh.numpy()[:(j + 2) * nrhs].reshape(j + 2, nrhs).copy()
+        np.testing.assert_array_equal(hc, H[:(j + 2) * nrhs].cpu().numpy().reshape(j + 2, nrhs))
+        cols.append(hc)
+    Vh = V.cpu().numpy()
+    Wh = W.cpu().numpy()
+    for c in range(nrhs):
+        Q = Vh[:k + 1, :, c].T
+        assert np.abs(Q.conj().T @ Q - np.eye(k + 1)).max() < 1e-12
+        for j in range(k):
+            h = cols[j][:, c]
+            assert h[j + 1].imag == 0 and h[j + 1].real > 0
+            lhs = DZD @ Wh[j, :, c]
+            assert np.linalg.norm(lhs - Q[:, :j + 2] @ h) < 1e-12 * np.linalg.norm(lhs)
+        # W_j is the complex64 LU solve of V_j: its residual against D Z D is at complex64 level
+        for j in range(k):
+            r = DZD @ Wh[j, :, c] - Vh[j, :, c]
+            assert np.linalg.norm(r) < 1e-4 * np.linalg.norm(Vh[j, :, c])
