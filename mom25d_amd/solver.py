@@ -891,6 +891,8 @@ class Solver:
             return W
 
         def residual(Y, rowsum=None):  # D (b - Z D Y)
+            if rowsum is None and self.symmetric_products:  # D b - (D Z D) Y from the upper tiles of Z
+                return Bs - matvec(Y)
             R = torch.empty_like(B)
             _lib.call("mom_residual", n, nrhs, P(Z), Z.stride(0), P(d * Y), P(B), P(R), P(rowsum), P(dv), stream)
             return d * R
