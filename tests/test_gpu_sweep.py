@@ -52,6 +52,24 @@ def test_currents_backward_stable(name, f, precision):
     assert fw < 1e-7
 
 
+def test_mixed_sweep_symmetric_and_full_products_agree():
+    """The mixed path's GMRES with the product from the upper tiles of Z
+    (the default) and with the row-wise product over all of Z reach the same
+    S and currents: Z is exactly symmetric after the fill."""
+    lay, st, mk, _ = W.via2()
+    m = lay.mesh(**mk)
+    f = [2e9, 9e9, 14e9]
+    out = []
+    for sym in (True, False):
+        s = Solver(m, st, verbose=False, precision="mixed")
+        s.symmetric_products = sym
+        r = s.sweep(f, interp=None, keep_currents=True)
+        out.append(r)
+    assert np.abs(out[0].S - out[1].S).max() < 1e-8
+    for a, b in zip(out[0].currents, out[1].currents):
+        assert np.abs(a - b).max() < 1e-9 * np.abs(b).max()
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_given_matrix_symmetric_or_not(precision):
     """solve_currents(Z=...) solves the matrix it is given: a symmetric one
