@@ -134,7 +134,11 @@ def batch_random16(torch, precision):
                 N=[m.n_unknowns for m, _, _ in problems], host_fit_seconds=sum(r.timing["sparams"] for r in rs))
 
 
-def cpu_baseline(mesh, sub, freqs, npts):
+def cpu_baseline(mesh, sub, freqs, npts, reference_defaults=False):
+    """The oracle on the host: npts points with all host threads; with
+    reference_defaults also the same points at the reference's own thread
+    defaults (8 OpenMP threads for the fill, Engine/mom25d/_fast.py:49-50;
+    4 BLAS threads, Engine/mom25d/__init__.py:5-17)."""
     from threadpoolctl import threadpool_limits
 
     from oracle.solve_ref import OracleSolver
@@ -147,12 +151,20 @@ def cpu_baseline(mesh, sub, freqs, npts):
         t0 = time.perf_counter()
         o.sweep(freqs[:npts])
         dt = time.perf_counter() - t0
-    return dict(value=npts / dt, unit="freq points/s", cores=cores, kind="port",
+    split = {k: round(v, 3) for k, v in o.timing.items()}
+    extra = {}
+    if reference_defaults:
+        with threadpool_limits(limits=8, user_api="openmp"), threadpool_limits(limits=4, user_api="blas"):
+            t1 = time.perf_counter()
+            o.sweep(freqs[:npts])
+            extra["reference_defaults"] = dict(value=npts / (time.perf_counter() - t1), fill_threads=8, blas_threads=4,
+                                               split={k: round(v, 3) for k, v in o.timing.items()})
+    return dict(value=npts / dt, unit="freq points/s", cores=cores, kind="port", **extra,
                 sample=f"{npts} frequency point(s) ({', '.join(f'{f / 1e9:g}' for f in freqs[:npts])} GHz) of the same "
                        f"workload: oracle tables (scipy J0), C/OpenMP pair kernel + serial scatter, "
                        f"LAPACK zgetrf/zgetrs (OpenBLAS), numpy de-embedding; per-mesh set-up untimed; "
                        f"CPU {_cpu_model()}",
-                split={k: round(v, 3) for k, v in o.timing.items()})
+                split=split)
 
 
 def main():
@@ -175,6 +187,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched sweep of the 16 random layouts")
     ap.add_argument("--no-interp", action="store_true",
                     help="skip timing the headline path with the interpolated impedance (interp='auto')")
+    ap.add_argument("--cpu-reference-defaults", action="store_true",
+                    help="also time the CPU sample at the reference's thread defaults (8 fill, 4 BLAS threads)")
     ap.add_argument("--hw-queues", type=int, default=16,
                     help="GPU_MAX_HW_QUEUES of this process (HIP's default is 4)")
     args = ap.parse_args()
@@ -259,7 +273,7 @@ def main():
               file=sys.stderr, flush=True)
     cpu = None
     if args.cpu_sample > 0:
-        cpu = cpu_baseline(mesh, sub, freqs, args.cpu_sample)
+        cpu = cpu_baseline(mesh, sub, freqs, args.cpu_sample, args.cpu_reference_defaults)
         print(f"cpu baseline: {cpu['value']:.4f} points/s {cpu['split']}", file=sys.stderr, flush=True)
 
     line = {
