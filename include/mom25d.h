@@ -135,8 +135,10 @@ int mom_build_perm(int n, const int32_t* ipiv, int32_t* perm, void* stream);
 /* X = (LU)^-1 P B for nrhs <= 8 right-hand sides (n x nrhs, row-major).
  * `work` holds mom_getrs_work_bytes(prec, n) bytes of device memory (block
  * tickets, cleared by the call itself, and the intermediate L^-1 P B); its
- * fifth uint32 is 1 after the call when a bounded wait of the block chain ran
- * out (X is then invalid).  Replaces scipy.linalg.lu_solve (LAPACK zgetrs),
+ * fifth uint32 is the failure word: a solve whose bounded wait ran out sets
+ * it to 1 (X is then invalid) and no solve clears it, so the caller zeroes it
+ * (with a new workspace, and before a run of solves it wants to check) and
+ * reads it once after the run.  Replaces scipy.linalg.lu_solve (LAPACK zgetrs),
  * Engine/mom25d/solver.py:23. */
 int64_t mom_getrs_work_bytes(int prec, int n);
 int mom_getrs(int prec, int n, int nrhs, const void* LU, int64_t lda, const int32_t* perm, const void* B, void* X,

@@ -728,14 +728,17 @@ __device__ __forceinline__ float lane_value(float v, int k) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
 }
 
-// Set-up of a solve in one launch: the 5 status words cleared, X = P B
-// (rows gathered by perm, n x nrhs, rounded to R) and Y (the lower solve's
-// output) filled with the sentinel.
+// Set-up of a solve in one launch: the two block tickets cleared (words 0
+// and 2), X = P B (rows gathered by perm, n x nrhs, rounded to R) and Y (the
+// lower solve's output) filled with the sentinel.  The failure word (word 4)
+// is left alone: it is sticky, set by any solve whose wait ran out and
+// cleared only by the caller, so that a check after a run of solves (a GMRES
+// solve calls this hundreds of times) still sees a failure in any of them.
 template <typename R, typename TB>
 __global__ void getrs_prep_kernel(int n, int nrhs, const int32_t* perm, const cplx<TB>* B, cplx<R>* X, cplx<R>* Y,
                                   unsigned* flags) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < 5) flags[q] = 0u;
+  if (q == 0 || q == 2) flags[q] = 0u;
   if (q >= n * nrhs) return;
   const int i = q / nrhs, c = q % nrhs;
   const cplx<TB> b = B[(int64_t)perm[i] * nrhs + c];
@@ -853,7 +856,7 @@ __global__ __launch_bounds__(BS * BS / 16) void trsv_kernel(int n, const cplx<R>
     }
     __syncthreads();
     if (s_abort) {
-      if (tid == 0) *error = 1u;
+      if (tid == 0) atomicOr(error, 1u);
       return false;
     }
     return true;
@@ -1633,17 +1636,17 @@ static int trtri_blocks(int n, const cplx<R>* LU, int64_t lda, void* work, hipSt
 template <typename R, bool INV, int BSX>
 static int trsv_pair(int n, int nrhs, const cplx<R>* LU, int64_t lda, cplx<R>* X, cplx<R>* Y, const cplx<R>* DL,
                      const cplx<R>* DU, const cplx<R>* ML, const cplx<R>* MU, z64* X64, unsigned* flags,
-                     hipStream_t s) {
+                     unsigned* error, hipStream_t s) {
   const int nblkx = (n + BSX - 1) / BSX;
   const dim3 block(BSX * BSX / 16);
   int rc;
 #define MOM_TRSV(NR)                                                                                                  \
   case NR:                                                                                                            \
     hipLaunchKernelGGL((trsv_kernel<R, true, NR, INV, BSX>), dim3(nblkx), block, 0, s, n, LU, lda, X, Y, DL, ML, X,  \
-                       (z64*)nullptr, flags, flags + 4, g_spin_limit);                                                \
+                       (z64*)nullptr, flags, error, g_spin_limit);                                                    \
     if ((rc = check_launch("trsv_kernel"))) return rc;                                                                \
     hipLaunchKernelGGL((trsv_kernel<R, false, NR, INV, BSX>), dim3(nblkx), block, 0, s, n, LU, lda, Y, X, DU, MU,    \
-                       (cplx<R>*)nullptr, X64, flags + 2, flags + 4, g_spin_limit);                                   \
+                       (cplx<R>*)nullptr, X64, flags + 2, error, g_spin_limit);                                       \
     break;
   switch (nrhs) {
     MOM_TRSV(1)
@@ -1666,10 +1669,13 @@ static int trsv_pair(int n, int nrhs, const cplx<R>* LU, int64_t lda, cplx<R>* X
 // X = (LU)^-1 P B.  B in precision TB; the solution goes to X (precision R)
 // or, when X64 is given, to X64 in complex128 (X is then the workspace's
 // copy).
+// The failure word is word 4 of the workspace unless `error` points
+// elsewhere.
 template <typename R, bool INV, typename TB>
 static int getrs(int n, int nrhs, const cplx<R>* LU, int64_t lda, const int32_t* perm, const cplx<TB>* B, cplx<R>* X,
-                 z64* X64, unsigned* flags, hipStream_t s) {
+                 z64* X64, unsigned* flags, hipStream_t s, unsigned* error = nullptr) {
   if (n <= 0 || nrhs <= 0) return 0;
+  if (!error) error = flags + 4;
   if (nrhs > kMaxRhs) return fail("mom_getrs: at most 8 right-hand sides");
   const int nblk = (n + kTrsvBS - 1) / kTrsvBS;
   cplx<R>* Y = reinterpret_cast<cplx<R>*>(reinterpret_cast<unsigned char*>(flags) + 256);
@@ -1690,12 +1696,12 @@ static int getrs(int n, int nrhs, const cplx<R>* LU, int64_t lda, const int32_t*
       const int nblk2 = (n + kTrsvBS2 - 1) / kTrsvBS2;
       const cplx<R>* D2L = DU + (int64_t)nblk * kTrsvBS * kTrsvBS;
       const cplx<R>* D2U = D2L + (int64_t)nblk2 * kTrsvBS2 * kTrsvBS2;
-      return trsv_pair<R, INV, kTrsvBS2>(n, nrhs, LU, lda, X, Y, D2L, D2U, nullptr, nullptr, X64, flags, s);
+      return trsv_pair<R, INV, kTrsvBS2>(n, nrhs, LU, lda, X, Y, D2L, D2U, nullptr, nullptr, X64, flags, error, s);
     }
   }
   const cplx<R>* ML = INV && g_chain ? chain_matrices<R>(flags, n) : nullptr;
   const cplx<R>* MU = ML ? ML + (int64_t)nblk * kTrsvBS * kTrsvBS : nullptr;
-  return trsv_pair<R, INV, kTrsvBS>(n, nrhs, LU, lda, X, Y, DL, DU, ML, MU, X64, flags, s);
+  return trsv_pair<R, INV, kTrsvBS>(n, nrhs, LU, lda, X, Y, DL, DU, ML, MU, X64, flags, error, s);
 }
 
 }  // namespace mom

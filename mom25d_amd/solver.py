@@ -44,13 +44,6 @@ class Substrate:
     t: float = 35e-6
     ground_loss: bool = True
 
-    def __del__(self):
-        for _sig, h in getattr(self, "_graphs", {}).values():
-            try:
-                _lib.call("mom_graph_destroy", h)
-            except Exception:  # interpreter shutdown: the library may be gone
-                pass
-
     @property
     def lossless(self):
         return not self.tand and not self.sigma
@@ -518,15 +511,28 @@ class Solver:
     def greens(self, freq):
         return self.tables([freq])[0]
 
-    def _buf(self, name, shape, dtype, lane=0):
-        """Device buffer kept between frequencies (one set per lane)."""
+    def _buf(self, name, shape, dtype, lane=0, zero=False):
+        """Device buffer kept between frequencies (one set per lane); zero:
+        a new buffer starts zeroed (the solve workspaces, whose status words
+        the library reads before it writes them)."""
         key = (name, lane)
         b = self._bufs.get(key)
         if b is None or tuple(b.shape) != tuple(shape) or b.dtype != dtype:
             self._bufs.pop(key, None)
-            b = torch.empty(shape, dtype=dtype, device=self.device)
+            b = (torch.zeros if zero else torch.empty)(shape, dtype=dtype, device=self.device)
             self._bufs[key] = b
         return b
+
+    SOLVE_WORKSPACES = ("swork32", "swork64", "swork64i")
+
+    def _clear_status(self, lane=0):
+        """Zero the failure words of the lane's solve workspaces (on the
+        current stream): the solves only ever set them, so a check after a
+        frequency point sees a failure of any solve the point ran."""
+        for k in self.SOLVE_WORKSPACES:
+            b = self._bufs.get((k, lane))
+            if b is not None:
+                b[16:20].zero_()
 
     def _pinned(self, name, shape, dtype, lane=0):
         """Pinned host buffer kept between calls (one per lane)."""
@@ -738,7 +744,7 @@ class Solver:
         # the complex64 solves (GMRES preconditioner) use the inverted diagonal
         # blocks of the factors, kept after the getrs workspace
         swb = _lib.lib().mom_getrs_work_bytes(prec, n) if prec == 64 else _lib.lib().mom_getrs_inv_work_bytes(prec, n)
-        sw = self._buf(f"swork{prec}", (int(swb),), torch.uint8, lane)
+        sw = self._buf(f"swork{prec}", (int(swb),), torch.uint8, lane, zero=True)
         X = self._buf("X", (n, nrhs), torch.complex128, lane)
 
         if self.tile is not None:
@@ -823,7 +829,7 @@ class Solver:
 
     def _status_tensors(self, lane=0):
         """The device status words of the lane's factorisation and solves."""
-        return [self._bufs[("info", lane)]] + [self._bufs[(k, lane)][:20] for k in ("swork32", "swork64", "swork64i")
+        return [self._bufs[("info", lane)]] + [self._bufs[(k, lane)][:20] for k in self.SOLVE_WORKSPACES
                                                if (k, lane) in self._bufs]
 
     def _check_solve(self, lane=0, status=None):
@@ -1045,6 +1051,7 @@ class Solver:
         LU when the reuse policy says so (reuse_lu).  interp: a FreqInterp
         (set up for this solver) supplies Z instead of the fill."""
         n = self.mesh.n_unknowns
+        self._clear_status(lane)
         Z = self._buf("Z", (n, n), torch.complex128, lane)
         # mixed precision relies on a symmetric Z (the fill's and the interpolation's are); a given
         # non-symmetric matrix is solved by the complex128 LU instead
@@ -1143,8 +1150,7 @@ class Solver:
             jb //= 2
         wbytes = int(L.mom_getrf_work_bytes(64, n, jb))
         work = self._buf("work64", (wbytes,), torch.uint8, lane)
-        sw = self._buf("swork64i", (int(L.mom_getrs_inv_work_bytes(64, n)),), torch.uint8, lane)
-        sw[:256].zero_()  # status words (a solve clears them too; _check_solve may read them before one runs)
+        sw = self._buf("swork64i", (int(L.mom_getrs_inv_work_bytes(64, n)),), torch.uint8, lane, zero=True)
         if nopiv:
             _lib.call("mom_getrf_nopiv", 64, n, P(A), A.stride(0), P(ipiv), P(info), P(work), wbytes, jb, stream)
         else:
@@ -1321,8 +1327,14 @@ class Solver:
                  twice as many frequencies as anchors, else fill every point;
                  a FreqInterp -> use it; None -> exact fill at every point.
         values : {lumped name: value} overriding the layout's components.
-        checkpoint, wait_memory : accepted for the reference's signature;
-                 checkpoints and memory waits are not part of this build."""
+        checkpoint : path of an .npz rewritten (atomically) after every
+                 frequency point; a sweep started again with the same path,
+                 frequencies, mesh and values solves only the points the file
+                 does not hold yet (with keep_currents every point is solved).
+        wait_memory : seconds to wait for enough free device memory for one
+                 lane's buffers (memory_estimate) before the first fill, then
+                 MemoryError; None reads $MOM25D_WAIT_MEMORY (default 0: warn
+                 only).  Engine/mom25d/solver.py:1137-1181, 1224-1256."""
         self.values = dict(values or {})
         freqs = np.atleast_1d(np.asarray(freqs, float))
         if isinstance(interp, str):
@@ -1339,9 +1351,51 @@ class Solver:
             print(f"  FreqInterp: {len(interp.anchors)} anchor fills for {len(freqs)} frequencies "
                   f"(order {interp.order}, k_e = {interp.k_ratio:.3f} k0)")
         self.last_interp = interp
-        lanes = min(self._lane_count(), len(freqs))
-        return _sweep_many([(self, freqs, interp)], lanes, self._worker_count(len(freqs)), keep_currents,
-                           self.graphs)[0]
+        if wait_memory is None:
+            wait_memory = float(os.environ.get("MOM25D_WAIT_MEMORY", 0) or 0)
+        ck = SweepCheckpoint(checkpoint, self, freqs) if checkpoint else None
+        todo = np.arange(len(freqs)) if ck is None or keep_currents else np.flatnonzero(~ck.done)
+        if len(todo) == 0:
+            return ck.result(self.port_names)
+        self._wait_memory(self.memory_estimate(), wait_memory)
+        lanes = min(self._lane_count(), len(todo))
+        # with a checkpoint the wave fits run in this process, so that each point's S is final when it is written
+        workers = 0 if ck is not None else self._worker_count(len(todo))
+        r = _sweep_many([(self, freqs[todo], interp)], lanes, workers, keep_currents, self.graphs,
+                        on_point=None if ck is None else (lambda fi, row: ck.store(todo[fi], row)))[0]
+        if ck is None or len(todo) == len(freqs):
+            return r
+        return ck.result(self.port_names, timing=r.timing)
+
+    def memory_estimate(self):
+        """Device memory [bytes] of one lane of a sweep: Z in complex128,
+        in mixed precision its complex64 LU copy, and the LU and solve
+        workspaces and GMRES basis (~5 % more); the lane count adapts to
+        what is free beyond that (Engine/mom25d/solver.py:1137-1151 counts
+        3 N x N complex128 on the host)."""
+        n = int(self.mesh.n_unknowns)
+        per = 16 * n * n + (8 * n * n if self.precision == "mixed" else 0)
+        return int(1.05 * per)
+
+    def _wait_memory(self, need, wait):
+        """Free device memory against need before the first fill: wait <= 0
+        warns, wait > 0 polls for up to `wait` seconds and then raises
+        MemoryError (another process may hold the device for a while)."""
+        free, _total = torch.cuda.mem_get_info(self.device)
+        if need <= free:
+            return
+        msg = f"estimated {need / 1e9:.1f} GB > free device memory {free / 1e9:.1f} GB (N = {self.mesh.n_unknowns})"
+        if not wait or wait <= 0:
+            warnings.warn(f"{msg}; continuing (sweep(wait_memory=seconds) waits instead)", RuntimeWarning,
+                          stacklevel=3)
+            return
+        t_end = time.time() + wait
+        while time.time() < t_end:
+            time.sleep(min(5.0, max(0.2, wait / 20)))
+            free, _total = torch.cuda.mem_get_info(self.device)
+            if need <= free:
+                return
+        raise MemoryError(f"{msg} after waiting {wait:.0f} s")
 
     # ---------------------------------------------------------- reference Z0
     def _reciprocal_z0(self, SI, zphi, prior=1e-4):
@@ -1480,9 +1534,66 @@ def run_lanes(task, count, lanes, device, solvers, graphs=False):
             s._active_lanes = 1
 
 
-def _sweep_many(jobs, lanes, workers, keep_currents=False, graphs=False):
+class SweepCheckpoint:
+    """The results of a sweep so far in an .npz file, keyed by a digest of
+    the frequencies, the mesh and the lumped values: rewritten through a
+    temporary file and os.replace after every point, so that a sweep killed
+    at any moment leaves the previous complete file (the resume semantics of
+    Engine/mom25d/solver.py:1238-1256, 1330-1333)."""
+
+    FIELDS = ("S", "gam", "zl", "zr", "res", "waves")
+
+    def __init__(self, path, solver, freqs):
+        import hashlib
+
+        self.path = os.fspath(path)
+        self.freqs = np.asarray(freqs, float)
+        m = solver.mesh
+        h = hashlib.sha256()
+        for a in (self.freqs, m.nodes, m.tris, m.tri_layer):
+            h.update(np.ascontiguousarray(a).tobytes())
+        h.update(repr(sorted((str(k), repr(v)) for k, v in solver.values.items())).encode())
+        self.key = h.hexdigest()
+        nf, n = len(freqs), len(solver.ports) + len(solver.iports)
+        self.arrays = dict(S=np.zeros((nf, n, n), complex), gam=np.zeros((nf, n), complex),
+                           zl=np.zeros((nf, n), complex), zr=np.zeros((nf, n)), res=np.zeros((nf, n)),
+                           waves=np.zeros((nf, n, n), complex))
+        self.done = np.zeros(nf, bool)
+        if os.path.exists(self.path):
+            try:
+                with np.load(self.path) as f:
+                    if str(f["key"]) == self.key and f["S"].shape == self.arrays["S"].shape:
+                        self.done = f["done"].astype(bool)
+                        for k in self.FIELDS:
+                            self.arrays[k] = f[k].copy()
+                    elif solver.verbose:
+                        print("  checkpoint: the file belongs to another sweep; starting afresh", flush=True)
+            except (OSError, KeyError, ValueError) as exc:  # unreadable: start afresh
+                if solver.verbose:
+                    print(f"  checkpoint: unreadable ({exc}); starting afresh", flush=True)
+                self.done[:] = False
+
+    def store(self, fi, row):
+        """Point fi's (S, gamma, z0_line, residual, z0, waves) as
+        sparams_point returns them, then the file."""
+        for k, v in zip(("S", "gam", "zl", "res", "zr", "waves"), row):
+            self.arrays[k][fi] = v
+        self.done[fi] = True
+        tmp = self.path + ".tmp.npz"
+        np.savez(tmp, key=np.array(self.key), done=self.done, **self.arrays)
+        os.replace(tmp, self.path)
+
+    def result(self, names, timing=None):
+        a = self.arrays
+        return SweepResult(self.freqs, a["S"], a["zr"], a["gam"], a["zl"], a["res"], names, None, a["waves"],
+                           timing=timing or {})
+
+
+def _sweep_many(jobs, lanes, workers, keep_currents=False, graphs=False, on_point=None):
     """Sweeps of (solver, freqs) jobs as one schedule of (job, frequency)
-    points on `lanes` streams; wave fits in `workers` processes (0: here)."""
+    points on `lanes` streams; wave fits in `workers` processes (0: here).
+    on_point(fi, result row) is called for each point of the first job as
+    soon as its S is known (only with workers = 0)."""
     out = []
     points = []
     t0 = time.time()
@@ -1529,7 +1640,10 @@ def _sweep_many(jobs, lanes, workers, keep_currents=False, graphs=False):
         I, V = s._port_iv(g, x)
         Ii = s._internal_currents(x)
         if pool is None:
-            store(o, fi, sparams_point(o["ctx"], f, I, V, Ii))
+            row = sparams_point(o["ctx"], f, I, V, Ii)
+            store(o, fi, row)
+            if on_point is not None and ji == 0:
+                on_point(fi, row)
         else:
             o["futures"][fi] = pool.submit(sparams_point, o["ctx"], f, I, V, Ii)
         o["host"] += time.perf_counter() - h0

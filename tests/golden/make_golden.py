@@ -148,8 +148,67 @@ def do_sweep():
     np.savez_compressed(os.path.join(HERE, "sweeps.npz"), **out)
 
 
+def _dense_sweep(name, fn, fs, rows_at=None, rows=None, currents_at=None):
+    """The reference's plain per-frequency path (method="dense", interp=None)
+    on one workload: S / z0 / gamma / z0_line / fit_residual at fs, with
+    optionally the impedance rows `rows` at rows_at and the port currents
+    at currents_at (one of fs)."""
+    lay, sub, mk, _ = fn(NS)
+    m = lay.mesh(**mk)
+    out = {f"{name}_digest": mesh_digest(m), f"{name}_n_unknowns": m.n_unknowns, f"{name}_freqs": np.asarray(fs)}
+    s = ref.Solver(m, sub, verbose=False, method="dense")
+    t = time.time()
+    r = s.sweep(fs, interp=None, keep_currents=currents_at is not None)
+    print(f"{name} sweep N={m.n_unknowns}: {time.time() - t:.0f}s", flush=True)
+    for k in ("S", "z0", "gamma", "z0_line", "fit_residual"):
+        out[f"{name}_{k}"] = getattr(r, k)
+    if currents_at is not None:
+        k = int(np.argmin(np.abs(np.asarray(fs) - currents_at)))
+        out[f"{name}_currents"] = np.asarray(r.currents[k])
+        out[f"{name}_currents_freq"] = fs[k]
+    if rows_at is not None:
+        t = time.time()
+        out[f"{name}_rows"] = np.asarray(rows)
+        out[f"{name}_Zrows"] = s.impedance(rows_at)[rows]
+        out[f"{name}_Zrows_freq"] = rows_at
+        print(f"{name} Z rows: {time.time() - t:.0f}s", flush=True)
+    return out
+
+
+def do_gcpw40():
+    """The 12 428-unknown via-stitched pour: S at 1, 10 and 20 GHz, currents
+    at 10 GHz, three RWG rows and two via rows of Z at 10 GHz."""
+    lay, sub, mk, _ = W.gcpw40(NS)
+    m = lay.mesh(**mk)
+    nb, n = m.n_basis, m.n_unknowns
+    out = _dense_sweep("gcpw40", W.gcpw40, np.array([1e9, 10e9, 20e9]), rows_at=10e9,
+                       rows=[0, nb // 2, nb - 1, nb, n - 1], currents_at=10e9)
+    np.savez_compressed(os.path.join(HERE, "config_gcpw40.npz"), **out)
+
+
+def do_lpf7():
+    """The stepped-impedance lowpass (8 837 unknowns): S at the sweep's first
+    point and its middle, three rows of Z at the middle."""
+    lay, sub, mk, f = W.lpf7(NS)
+    n = lay.mesh(**mk).n_unknowns
+    out = _dense_sweep("lpf7", W.lpf7, f[[0, 50]], rows_at=f[50], rows=[0, n // 2, n - 1])
+    np.savez_compressed(os.path.join(HERE, "config_lpf7.npz"), **out)
+
+
+def do_random():
+    """Two of the 16 random chains (seeds 0 and 15) at two of their 64
+    frequencies each, with two rows of Z at the second."""
+    out = {}
+    for seed in (0, 15):
+        lay, sub, mk, f = W.random_chain(seed, NS)
+        n = lay.mesh(**mk).n_unknowns
+        out.update(_dense_sweep(f"random{seed}", lambda ns, s=seed: W.random_chain(s, ns), f[[0, 40]],
+                                rows_at=f[40], rows=[0, n - 1]))
+    np.savez_compressed(os.path.join(HERE, "config_random.npz"), **out)
+
+
 if __name__ == "__main__":
-    sets = sys.argv[1:] or ["mesh", "tables", "z", "sweep"]
+    sets = sys.argv[1:] or ["mesh", "tables", "z", "sweep", "random", "lpf7", "gcpw40"]
     for s in sets:
         t = time.time()
         globals()[f"do_{s}"]()

@@ -125,7 +125,7 @@ def _lu_solve(A, B, prec, jb, status=False):
     info = torch.zeros(4, dtype=torch.int32, device=DEV)
     wb = int(_lib.lib().mom_getrf_work_bytes(prec, n, jb))
     work = torch.empty(wb, dtype=torch.uint8, device=DEV)
-    sw = torch.empty(int(_lib.lib().mom_getrs_work_bytes(prec, n)), dtype=torch.uint8, device=DEV)
+    sw = torch.zeros(int(_lib.lib().mom_getrs_work_bytes(prec, n)), dtype=torch.uint8, device=DEV)
     s = _lib.stream_ptr()
     P = _lib.ptr
     _lib.call("mom_getrf", prec, n, P(Ad), n, P(ipiv), P(info), P(work), wb, jb, 0, s)
@@ -252,6 +252,49 @@ def test_bounded_waits_report_failure():
     assert np.linalg.norm(A @ X - B) <= 1e-12 * np.linalg.norm(A) * np.linalg.norm(X)
 
 
+def test_solve_failure_word_is_sticky():
+    """A solve whose wait ran out sets the failure word of its workspace; a
+    later solve with the same workspace that succeeds does not clear it (a
+    GMRES solve runs hundreds of preconditioner solves before its one check),
+    and the caller's own clear does."""
+    rng = np.random.default_rng(12)
+    n = 3000
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + 4 * np.sqrt(n) * np.eye(n)
+    B = rng.standard_normal((n, 1)) + 1j * rng.standard_normal((n, 1))
+    Ad = torch.from_numpy(A).to(DEV)
+    Bd = torch.from_numpy(B).to(DEV)
+    X = torch.empty_like(Bd)
+    ipiv = torch.empty(n, dtype=torch.int32, device=DEV)
+    perm = torch.empty(n, dtype=torch.int32, device=DEV)
+    info = torch.zeros(4, dtype=torch.int32, device=DEV)
+    wb = int(_lib.lib().mom_getrf_work_bytes(64, n, 64))
+    work = torch.empty(wb, dtype=torch.uint8, device=DEV)
+    sw = torch.zeros(int(_lib.lib().mom_getrs_work_bytes(64, n)), dtype=torch.uint8, device=DEV)
+    s = _lib.stream_ptr()
+    P = _lib.ptr
+    _lib.call("mom_getrf", 64, n, P(Ad), n, P(ipiv), P(info), P(work), wb, 64, 0, s)
+    _lib.call("mom_build_perm", n, P(ipiv), P(perm), s)
+    torch.cuda.synchronize()
+    assert int(info[1]) == 0
+    _lib.call("mom_set_spin_limit", 1)
+    try:
+        _lib.call("mom_getrs", 64, n, 1, P(Ad), n, P(perm), P(Bd), P(X), P(sw), s)
+        torch.cuda.synchronize()
+    finally:
+        _lib.call("mom_set_spin_limit", 1 << 21)
+    if sw[:20].cpu().numpy().view(np.uint32)[4] != 1:
+        pytest.skip("every block of the one-poll solve found its dependency at once (nothing to check)")
+    _lib.call("mom_getrs", 64, n, 1, P(Ad), n, P(perm), P(Bd), P(X), P(sw), s)
+    torch.cuda.synchronize()
+    assert sw[:20].cpu().numpy().view(np.uint32)[4] == 1  # the successful solve left the failure standing
+    Xh = X.cpu().numpy()
+    assert np.linalg.norm(A @ Xh - B) <= 1e-12 * np.linalg.norm(A) * np.linalg.norm(Xh)
+    sw[16:20].zero_()
+    _lib.call("mom_getrs", 64, n, 1, P(Ad), n, P(perm), P(Bd), P(X), P(sw), s)
+    torch.cuda.synchronize()
+    assert sw[:20].cpu().numpy().view(np.uint32)[4] == 0
+
+
 @pytest.mark.parametrize("prec,n,nrhs,rows", [(32, 1, 1, 64), (32, 300, 2, 64), (32, 2049, 4, 64), (64, 777, 3, 64),
                                             (32, 1, 1, 128), (32, 300, 2, 128), (32, 2049, 4, 128), (32, 130, 3, 128),
                                             (32, 1000, 8, 64), (64, 500, 5, 64), (32, 700, 7, 128)])
@@ -295,7 +338,7 @@ def _getrs_inv_case(prec, n, nrhs):
     info = torch.zeros(4, dtype=torch.int32, device=DEV)
     wb = int(_lib.lib().mom_getrf_work_bytes(prec, n, 64))
     work = torch.empty(wb, dtype=torch.uint8, device=DEV)
-    sw = torch.empty(int(_lib.lib().mom_getrs_inv_work_bytes(prec, n)), dtype=torch.uint8, device=DEV)
+    sw = torch.zeros(int(_lib.lib().mom_getrs_inv_work_bytes(prec, n)), dtype=torch.uint8, device=DEV)
     assert sw.numel() > int(_lib.lib().mom_getrs_work_bytes(prec, n))
     s = _lib.stream_ptr()
     P = _lib.ptr
@@ -312,7 +355,7 @@ def _getrs_inv_case(prec, n, nrhs):
         r = np.linalg.norm(A @ got - B) / (np.linalg.norm(A, 2) * np.linalg.norm(got))
         assert r < (1e-6 if prec == 32 else 1e-14), (rep, r)
         Xs = torch.empty_like(Bd)
-        sw2 = torch.empty(int(_lib.lib().mom_getrs_work_bytes(prec, n)), dtype=torch.uint8, device=DEV)
+        sw2 = torch.zeros(int(_lib.lib().mom_getrs_work_bytes(prec, n)), dtype=torch.uint8, device=DEV)
         _lib.call("mom_getrs", prec, n, nrhs, P(Ad), n, P(perm), P(Bd), P(Xs), P(sw2), s)
         ref = Xs.cpu().numpy()
         assert np.max(np.abs(got - ref)) < (1e-5 if prec == 32 else 1e-12) * np.max(np.abs(ref))
@@ -354,7 +397,7 @@ def test_getrs_inv_z_and_scaled_matvec(n, nrhs):
     info = torch.zeros(4, dtype=torch.int32, device=DEV)
     wb = int(_lib.lib().mom_getrf_work_bytes(32, n, 64))
     work = torch.empty(wb, dtype=torch.uint8, device=DEV)
-    sw = torch.empty(int(_lib.lib().mom_getrs_inv_work_bytes(32, n)), dtype=torch.uint8, device=DEV)
+    sw = torch.zeros(int(_lib.lib().mom_getrs_inv_work_bytes(32, n)), dtype=torch.uint8, device=DEV)
     s = _lib.stream_ptr()
     P = _lib.ptr
     _lib.call("mom_getrf", 32, n, P(Ad), n, P(ipiv), P(info), P(work), wb, 64, 0, s)
@@ -439,7 +482,7 @@ def test_gmres_arnoldi_step(n, nrhs, symv):
     info = torch.zeros(4, dtype=torch.int32, device=DEV)
     wb = int(_lib.lib().mom_getrf_work_bytes(32, n, 64))
     work = torch.empty(wb, dtype=torch.uint8, device=DEV)
-    sw = torch.empty(int(_lib.lib().mom_getrs_inv_work_bytes(32, n)), dtype=torch.uint8, device=DEV)
+    sw = torch.zeros(int(_lib.lib().mom_getrs_inv_work_bytes(32, n)), dtype=torch.uint8, device=DEV)
     _lib.call("mom_getrf", 32, n, P(LUd), n, P(ipiv), P(info), P(work), wb, 64, 0, s)
     _lib.call("mom_build_perm", n, P(ipiv), P(perm), s)
     _lib.call("mom_trtri_blocks", 32, n, P(LUd), n, P(sw), s)
