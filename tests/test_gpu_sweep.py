@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-6), ("mixed", 1e-5)])
-@pytest.mark.parametrize("name", ["tiny", "tee"])
+@pytest.mark.parametrize("name", ["tiny", "tee", "via2"])
 def test_sweep_matches_reference(name, precision, tol, golden):
     g = golden("sweeps.npz")
     lay, sub, mk, _ = getattr(W, name)()
