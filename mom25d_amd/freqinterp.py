@@ -149,9 +149,9 @@ class FreqInterp:
         Lc = (ctypes.c_double * len(L))(*L)
         _lib.call("mom_interp_combine", self._prec, n, self.nb, len(Ws), ptrs, Lc, Ws[0].stride(0),
                   _lib.ptr(self._pos), w, ke, _lib.ptr(Z), Z.stride(0), _lib.stream_ptr())
-        if not self._checked and np.min(np.abs(self.anchors - f)) > 0.25 * self.step:
+        if self.check and not self._checked and np.min(np.abs(self.anchors - f)) > 0.25 * self.step:
             self._checked = True
-            self.report_error(f, Z.cpu().numpy())
+            self.report_error(f, Z)
         return Z
 
     # ---------------------------------------------------------- public
@@ -162,40 +162,46 @@ class FreqInterp:
         return self.fill(f, Z).cpu().numpy()
 
     def report_error(self, f, Zi=None):
-        """Relative interpolation error of Z at f against one exact fill, per
-        block (RWG, RWG-via, via-via); returns the largest
-        (freqinterp.py:141-162)."""
-        Zi = self.impedance(f) if Zi is None else Zi
+        """Relative Frobenius error of the interpolated Z at f against an
+        exact fill, per block (RWG, RWG-via, via-via), the largest returned
+        (the check of Engine/mom25d/freqinterp.py:141-162).  Both matrices
+        stay on the device; Zi: an interpolated matrix already at hand."""
         s = self.solver
         n = s.mesh.n_unknowns
-        Zd = torch.empty((n, n), dtype=torch.complex128, device=s.device)
-        s._fill(s.greens(f), Zd, None, lane="interp-check")  # buffers of its own: lanes may be mid-solve
-        Ze = Zd.cpu().numpy()
-        del Zd
+        dev = s.device
+        Zint = torch.empty((n, n), dtype=torch.complex128, device=dev)
+        if Zi is None:
+            self.fill(f, Zint)
+        else:
+            Zint.copy_(torch.as_tensor(Zi))
+        Zex = torch.empty((n, n), dtype=torch.complex128, device=dev)
+        s._fill(s.greens(f), Zex, None, lane="interp-check")  # buffers of its own: lanes may be mid-solve
         nb = self.nb
-        blocks = {"RWG": (slice(0, nb), slice(0, nb)), "RWG-via": (slice(0, nb), slice(nb, None)),
-                  "via-via": (slice(nb, None), slice(nb, None))}
         errs = {}
-        for k, (a, b) in blocks.items():
-            ne = np.linalg.norm(Ze[a, b])
-            if ne > 0:
-                errs[k] = np.linalg.norm(Zi[a, b] - Ze[a, b]) / ne
-        err = max(errs.values())
-        print(f"  FreqInterp check at {f / 1e9:.4f} GHz: relative error of Z "
+        for name, rows, cols in (("RWG", slice(0, nb), slice(0, nb)), ("RWG-via", slice(0, nb), slice(nb, n)),
+                                 ("via-via", slice(nb, n), slice(nb, n))):
+            ref = float(torch.linalg.vector_norm(Zex[rows, cols]))
+            if ref > 0:
+                errs[name] = float(torch.linalg.vector_norm(Zint[rows, cols] - Zex[rows, cols])) / ref
+        del Zint, Zex
+        self.last_error = max(errs.values())
+        print(f"  FreqInterp check at {f / 1e9:.4f} GHz, Z error per block: "
               + ", ".join(f"{k} {v:.1e}" for k, v in errs.items())
-              + f" (anchors every {np.diff(self.anchors).mean() / 1e9:.3g} GHz, order {self.order})")
-        self.last_error = err
-        return err
+              + f" (anchor spacing {self.step / 1e9:.3g} GHz, order {self.order})", flush=True)
+        return self.last_error
 
 
 def eps_eff(solver):
-    """Quasi-static eps_eff under the first port's level, for the phase
-    extraction only (freqinterp.py:199-209)."""
-    st = solver.stack
-    if solver.ports:
-        layer = solver.ports[0].layer
-    else:  # internal ports only: the level of the first gap port, else the first mesh layer
-        gaps = {q.name: q.layer for q in solver.lumped}
-        layer = next((gaps[ip["name"]] for ip in solver.iports if ip["kind"] == "gap"), solver.mesh.layer_names[0])
-    e = st.eps_below(layer)
-    return e if st.covered(layer) else 0.5 * (e + 1)
+    """Quasi-static effective permittivity that sets the phase reference
+    k_e = sqrt(eps_eff) k0 of the interpolation
+    (Engine/mom25d/freqinterp.py:199-209): at the metal level of the first
+    feed (or of the first gap port, or the first mesh level), the mean of
+    the dielectric below and the air above for an uncovered level, the
+    dielectric's own value under a cover."""
+    levels = [p.layer for p in solver.ports]
+    if not levels:
+        gap_level = {q.name: q.layer for q in solver.lumped}
+        levels = [gap_level[ip["name"]] for ip in solver.iports if ip["kind"] == "gap"]
+    level = levels[0] if levels else solver.mesh.layer_names[0]
+    below = solver.stack.eps_below(level)
+    return below if solver.stack.covered(level) else 0.5 * (below + 1.0)

@@ -62,121 +62,170 @@ class Substrate:
 
 
 # --------------------------------------------------------------------- vias
+def _grover(a1, a2, b1, b2, d):
+    """Partial mutual inductance [H] of two parallel z-directed filaments
+    occupying [a1, a2] and [b1, b2] at lateral distance d: the double
+    integral of mu0 / (4 pi R) over both lengths, whose antiderivative in the
+    offset u = z - z' is F(u) = u asinh(u / d) - sqrt(u^2 + d^2) (Grover);
+    broadcasts over array arguments."""
+    def F(u):
+        return u * np.arcsinh(u / d) - np.hypot(u, d)
+
+    return MU0 / (4 * np.pi) * (F(a2 - b1) + F(a1 - b2) - F(a2 - b2) - F(a1 - b1))
+
+
 def filament_mutual(l, d):
-    """Partial mutual inductance of two parallel aligned filaments of length l
-    at distance d (d = radius: partial self inductance of a tube)."""
-    return MU0 * l / (2 * np.pi) * (np.arcsinh(l / d) - np.sqrt(1 + (d / l) ** 2) + d / l)
+    """Partial mutual inductance of two parallel, aligned filaments of length
+    l at distance d (d = the radius: the partial self inductance of a tube)."""
+    return _grover(0.0, l, 0.0, l, d)
 
 
 def filament_mutual_z(a1, a2, b1, b2, d):
-    """Partial mutual inductance (Grover) of parallel z-filaments spanning
-    [a1, a2] and [b1, b2] at lateral distance d."""
-    def f(s):
-        return s * np.arcsinh(s / d) - np.sqrt(s * s + d * d)
+    """Partial mutual inductance of parallel z-filaments spanning [a1, a2] and
+    [b1, b2] at lateral distance d."""
+    return _grover(a1, a2, b1, b2, d)
 
-    return MU0 / (4 * np.pi) * (f(a2 - b1) - f(a2 - b2) - f(a1 - b1) + f(a1 - b2))
+
+def _via_geometry(vias):
+    """Lateral distances between via axes, with each via's radius on the
+    diagonal and the smaller radius for coaxial pairs."""
+    xy = np.array([v.xy for v in vias], float).reshape(-1, 2)
+    rad = np.array([0.5 * v.drill for v in vias], float)
+    dist = np.hypot(*(xy[:, None, :] - xy[None, :, :]).transpose(2, 0, 1))
+    dist = np.where(dist < 1e-12, np.minimum.outer(rad, rad), dist)
+    dist[np.diag_indices_from(dist)] = rad
+    return dist
+
+
+def _with_image(a1, a2, b1, b2, d):
+    """Filament [a1, a2] against filament [b1, b2] plus the image of the
+    latter in the ground plane z = 0 (the span [-b2, -b1], same current
+    direction for a vertical current)."""
+    return _grover(a1, a2, b1, b2, d) + _grover(a1, a2, -b2, -b1, d)
 
 
 def via_inductance(vias, h, spans=None):
     """Quasi-static partial inductance matrix [H] of vertical via currents
-    with their ground-plane images (Engine/mom25d/solver.py:93-122, k = 0)."""
-    xy = np.array([v.xy for v in vias], float).reshape(-1, 2)
-    d = np.linalg.norm(xy[:, None] - xy[None], axis=-1)
-    np.fill_diagonal(d, [0.5 * v.drill for v in vias])
-    if spans is not None:
-        z = np.asarray(spans, float).reshape(-1, 2)
-        a1, a2 = z[:, None, 0], z[:, None, 1]
-        b1, b2 = z[None, :, 0], z[None, :, 1]
-        return filament_mutual_z(a1, a2, b1, b2, d) + filament_mutual_z(a1, a2, -b2, -b1, d)
-    return 0.5 * filament_mutual(2 * h, d)
+    with their ground-plane images (the k = 0 case of
+    Engine/mom25d/solver.py:93-122).  Without spans every via runs from the
+    plane to height h: the via and its image form one filament of 2 h, of
+    which each half holds half the flux."""
+    d = _via_geometry(vias)
+    if spans is None:
+        return 0.5 * _grover(0.0, 2 * h, 0.0, 2 * h, d)
+    z = np.asarray(spans, float).reshape(-1, 2)
+    return _with_image(z[:, None, 0], z[:, None, 1], z[None, :, 0], z[None, :, 1], d)
+
+
+def _parallel_plate_z(a1, a2, r, height, k, omega):
+    """Impedance between z-directed via pieces [a1_i, a2_i] inside a
+    parallel-plate section of the given height (PEC on both sides): the TEM
+    mode, omega mu0 / (4 height) l_i l_j H0^(2)(k r), plus the evanescent
+    TM_n modes, j omega mu0 / (pi height) K0(q_n r) c_ni c_nj with
+    k_n = n pi / height, q_n = sqrt(k_n^2 - k^2) and c_n = int cos(k_n z) dz
+    over the piece; the sum runs until k_n r_min passes 45 (K0 < 1e-20)."""
+    from scipy.special import hankel2, kv
+
+    length = a2 - a1
+    Z = omega * MU0 / (4 * height) * np.outer(length, length) * hankel2(0, k * r)
+    n_modes = int(np.ceil(45 * height / (np.pi * r.min()))) + 1
+    kn = np.pi / height * np.arange(1, n_modes + 1)
+    c = (np.sin(np.outer(a2, kn)) - np.sin(np.outer(a1, kn))) / kn  # (pieces, modes)
+    if np.abs(c).max() > 1e-12 * height:
+        qn = np.sqrt(kn ** 2 - k ** 2 + 0j)
+        Z = Z + 1j * omega * MU0 / (np.pi * height) * np.einsum("ijn,in,jn->ij", kv(0, r[..., None] * qn), c, c)
+    return Z
 
 
 def via_impedance(vias, spans, omega, sections, cache=None):
-    """Vector-potential impedance [ohm] of the via segments per stack section
-    (Engine/mom25d/solver.py:125-199): open sections by quasi-static partial
-    inductances of piece + image, PEC-closed sections by parallel-plate TEM
-    and TM_n modes.  cache: a dict kept by the caller for one set of vias
-    and spans; the partial inductances of open sections (independent of
-    frequency) are computed once into it."""
-    from scipy.special import hankel2, kv
-
-    xy = np.array([v.xy for v in vias], float).reshape(-1, 2)
-    rho = np.linalg.norm(xy[:, None] - xy[None], axis=-1)
-    rad = np.array([0.5 * v.drill for v in vias])
-    same = rho < 1e-12
-    rho = np.where(same, np.minimum(rad[:, None], rad[None, :]), rho)
-    np.fill_diagonal(rho, rad)
+    """Vector-potential impedance [ohm] between the via segments, summed over
+    the sections of the stack (Engine/mom25d/solver.py:125-199): in a section
+    open at the top, j omega times the partial inductance of the pieces with
+    their ground images; in a section closed by PEC, the parallel-plate mode
+    sum.  sections: [(z_bottom, z_top, k_tem or None)].  cache: a dict the
+    caller keeps for one set of vias and spans, holding the
+    frequency-independent inductances of the open sections."""
+    dist = _via_geometry(vias)
     spans = np.asarray(spans, float).reshape(-1, 2)
-    nv = len(spans)
-    Z = np.zeros((nv, nv), complex)
+    Z = np.zeros((len(spans), len(spans)), complex)
     for zb, zt, k in sections:
-        a1 = np.clip(spans[:, 0], zb, zt) - zb
-        a2 = np.clip(spans[:, 1], zb, zt) - zb
-        on = np.nonzero(a2 - a1 > 1e-12 * (zt - zb))[0]
-        if not on.size:
+        height = zt - zb
+        lo = np.clip(spans[:, 0], zb, zt) - zb
+        hi = np.clip(spans[:, 1], zb, zt) - zb
+        inside = np.flatnonzero(hi - lo > 1e-12 * height)
+        if inside.size == 0:
             continue
-        a1, a2, r = a1[on], a2[on], rho[np.ix_(on, on)]
-        d = zt - zb
+        r = dist[np.ix_(inside, inside)]
+        a1, a2 = lo[inside], hi[inside]
         if k is None:
-            L = None if cache is None else cache.get((zb, zt))
+            key = (zb, zt)
+            L = None if cache is None else cache.get(key)
             if L is None:
-                p1, p2 = a1[:, None], a2[:, None]
-                q1, q2 = a1[None, :], a2[None, :]
-                L = filament_mutual_z(p1, p2, q1, q2, r) + filament_mutual_z(p1, p2, -q2, -q1, r)
+                L = _with_image(a1[:, None], a2[:, None], a1[None, :], a2[None, :], r)
                 if cache is not None:
-                    cache[(zb, zt)] = L
-            Zs = 1j * omega * L
+                    cache[key] = L
+            Z[np.ix_(inside, inside)] += 1j * omega * L
         else:
-            la = a2 - a1
-            Zs = omega * MU0 / (4 * d) * np.outer(la, la) * hankel2(0, k * r)
-            nmax = int(np.ceil(40 * d / (np.pi * r.min()))) + 1
-            kn = np.arange(1, nmax + 1) * np.pi / d
-            c = (np.sin(kn * a2[:, None]) - np.sin(kn * a1[:, None])) / kn
-            if np.abs(c).max() > 1e-12 * d:
-                qn = np.sqrt(kn ** 2 - k ** 2 + 0j)
-                K = kv(0, np.multiply.outer(r, qn))
-                Zs = Zs + 1j * omega * MU0 / (np.pi * d) * np.einsum("ijn,in,jn->ij", K, c, c)
-        Z[np.ix_(on, on)] += Zs
+            Z[np.ix_(inside, inside)] += _parallel_plate_z(a1, a2, r, height, k, omega)
     return Z
 
 
 # -------------------------------------------------------------------- ports
 class PortModel:
-    """Delta-gap excitation and current / voltage sample cuts of one port
-    (Engine/mom25d/solver.py:570-610)."""
+    """Excitation and sample cuts of one feed (the delta-gap model of
+    Engine/mom25d/solver.py:570-610).
+
+    The gap column puts 1 V across the RWG edges of the cut at s_gap (each
+    coefficient is the signed edge length).  The wave fit samples the line
+    at the mesh's own cuts inside [margin, s_gap - margin] (margin, unless
+    the port sets one, max(1.5 W, 4 h): clear of the gap's and the
+    circuit's near fields): the current through each cut, and the scalar
+    potential averaged over 8 points around it, at +-d along the line (d half
+    the distance to the nearest other cut) and (-0.3, -0.1, 0.1, 0.3) W
+    across it."""
+
+    ACROSS = (-0.3, -0.1, 0.1, 0.3)
 
     def __init__(self, mesh, port, h):
         self.port = port
-        margin = port.margin if port.margin is not None else max(1.5 * port.width, 4 * h)
-        s = mesh.cut_positions(port)
-        s = s[(s >= margin - 1e-12) & (s <= port.s_gap - margin + 1e-12)]
+        margin = max(1.5 * port.width, 4 * h) if port.margin is None else port.margin
+        all_cuts = np.asarray(mesh.cut_positions(port), float)
+        window = (all_cuts >= margin - 1e-12) & (all_cuts <= port.s_gap - margin + 1e-12)
+        s = all_cuts[window]
         if s.size < 4:
-            raise ValueError(f"port {port.name}: feed line too short or mesh too coarse "
-                             f"({s.size} sample cuts in the fitting window "
-                             f"[{margin * 1e3:.3g}, {(port.s_gap - margin) * 1e3:.3g}] mm)")
+            raise ValueError(f"port {port.name}: feed line too short or mesh too coarse ({s.size} sample cuts in "
+                             f"the fitting window [{margin * 1e3:.3g}, {(port.s_gap - margin) * 1e3:.3g}] mm)")
         self.s = s
-        self.cuts = [mesh.cut_edges(port, si) for si in s]
-        idx, sgn = mesh.cut_edges(port, port.s_gap)
-        if idx.size == 0:
+        gap_idx, gap_sgn = mesh.cut_edges(port, port.s_gap)
+        if gap_idx.size == 0:
             raise ValueError(f"port {port.name}: no mesh edges on the gap cut")
         self.excitation = np.zeros(mesh.n_unknowns)
-        self.excitation[idx] = sgn * mesh.edge_len[idx]
-        self.len = mesh.edge_len
-        allcuts = mesh.cut_positions(port)
-        k = np.searchsorted(allcuts, s)
-        lo = s - allcuts[np.maximum(k - 1, 0)]
-        hi = allcuts[np.minimum(k + 1, allcuts.size - 1)] - s
-        d = 0.5 * np.minimum(np.where(lo > 0, lo, np.inf), np.where(hi > 0, hi, np.inf))
-        frac = np.array([-0.3, -0.1, 0.1, 0.3])
-        along = np.stack([s - d, s + d], 1)
-        self.vpts = port.point(along[:, :, None], frac * port.width).reshape(-1, 2)
-        self.nper = 2 * frac.size
+        self.excitation[gap_idx] = gap_sgn * mesh.edge_len[gap_idx]
+        # every sample cut's edges as one flat list: current(cut) = sum of w * x[edge] over its entries
+        per_cut = [mesh.cut_edges(port, c) for c in s]
+        self._edge = np.concatenate([i for i, _ in per_cut]).astype(np.int64)
+        self._w = np.concatenate([sg * mesh.edge_len[i] for i, sg in per_cut])
+        self._cut = np.repeat(np.arange(s.size), [len(i) for i, _ in per_cut])
+        # voltage points: the two neighbours of each cut at half the gap to its nearest other cut
+        pos = np.flatnonzero(window)
+        gaps = np.diff(all_cuts)
+        left = np.where(pos > 0, gaps[np.maximum(pos - 1, 0)], np.inf)
+        right = np.where(pos < all_cuts.size - 1, gaps[np.minimum(pos, gaps.size - 1)], np.inf)
+        left = np.where(left > 0, left, np.inf)
+        right = np.where(right > 0, right, np.inf)
+        half = 0.5 * np.minimum(left, right)
+        along = np.stack([s - half, s + half], 1)
+        self.vpts = port.point(along[:, :, None], np.asarray(self.ACROSS) * port.width).reshape(-1, 2)
+        self.nper = 2 * len(self.ACROSS)
 
     def currents(self, x):
-        """Total current toward the circuit at each sample cut, (Ns, Nexc)."""
-        return np.array([(sg * self.len[i]) @ x[i] for i, sg in self.cuts])
+        """Total current toward the circuit through each sample cut, (Ns, Nexc)."""
+        out = np.zeros((self.s.size,) + x.shape[1:], complex)
+        np.add.at(out, self._cut, self._w.reshape((-1,) + (1,) * (x.ndim - 1)) * x[self._edge])
+        return out
 
     def voltages(self, phi):
+        """Mean potential of each cut's 8 points, (Ns, Nexc)."""
         return phi.reshape(self.s.size, self.nper, -1).mean(1)
 
 
@@ -995,38 +1044,45 @@ class Solver:
         return E if self.Lm is None else np.hstack([E, self.Lm])
 
     def lumped_impedance(self, freq):
-        """D: (n_lumped x n_lumped) impedance of the components across the
-        lumped gaps at freq (open gaps: OPEN) (Engine/mom25d/solver.py:1039-1061)."""
+        """D (n_gaps x n_gaps): the impedances across the lumped gaps at freq
+        (the loads of Engine/mom25d/solver.py:1039-1061).  An internal port's
+        gap is terminated in its z_ref; an N-port component fills the block
+        of its gaps; every other gap takes its value (a sweep's `values`
+        override the layout's), and a gap without one stays open (OPEN)."""
         from .lumped import as_value, parse
 
-        nl = 0 if self.Lm is None else self.Lm.shape[1]
-        D = np.zeros((nl, nl), complex)
-        names = [q.name for q in self.lumped]
-        used = set()
-        for ip, c in zip(self.iports, self.iport_index):  # internal ports: terminated in z_ref
+        n_gaps = 0 if self.Lm is None else self.Lm.shape[1]
+        D = np.zeros((n_gaps, n_gaps), complex)
+        position = {q.name: k for k, q in enumerate(self.lumped)}
+        taken = np.zeros(n_gaps, bool)  # filled by an internal port or a component
+        internal = np.zeros(n_gaps, bool)
+        for ip, c in zip(self.iports, self.iport_index):
             D[c, c] = ip["z_ref"]
-            used.add(c)
+            taken[c] = internal[c] = True
         for model, gaps in self.components:
-            ix = [names.index(n) for n in gaps]
-            D[np.ix_(ix, ix)] = model.z(freq)
-            used.update(ix)
+            block = [position[g] for g in gaps]
+            D[np.ix_(block, block)] = model.z(freq)
+            taken[block] = True
         for k, q in enumerate(self.lumped):
-            v = self.values.get(q.name, q.value)
-            if k in used and (v is None or k in self.iport_index):
-                continue
-            if isinstance(v, str):
-                v = parse(v)
-            D[k, k] = self.OPEN if v is None else as_value(v).z(freq)
+            value = self.values.get(q.name, q.value)
+            if taken[k] and (value is None or internal[k]):
+                continue  # a component's own entry stays unless a value is given for the gap itself
+            if value is None:
+                D[k, k] = self.OPEN
+            else:
+                D[k, k] = as_value(parse(value) if isinstance(value, str) else value).z(freq)
         return D
 
     def _load_lumped(self, Xe, Xi, freq):
-        """Solution with the components across the lumped gaps:
-        (Z + L D L^T)^-1 e = Xe - Xi D (I + L^T Xi D)^-1 L^T Xe (Woodbury), with
-        Xi = Z^-1 L (Engine/mom25d/solver.py:1063-1069)."""
+        """Currents with the loads D across the gaps, from the unloaded
+        solutions Xe = Z^-1 e and Xi = Z^-1 L (L: the gaps' delta-gap
+        columns).  The loaded system (Z + L D L^T) x = e gives Z x = e - L D y
+        with y = L^T x the gap currents, so x = Xe - Xi D y, and applying L^T
+        gives (I + L^T Xi D) y = L^T Xe: one small solve per frequency
+        (Engine/mom25d/solver.py:1063-1069)."""
         D = self.lumped_impedance(freq)
-        M = self.Lm.T @ Xi
-        y = self.Lm.T @ Xe
-        return Xe - Xi @ (D @ np.linalg.solve(np.eye(len(D)) + M @ D, y))
+        y = np.linalg.solve(np.eye(len(D)) + (self.Lm.T @ Xi) @ D, self.Lm.T @ Xe)
+        return Xe - Xi @ (D @ y)
 
     def _apply_lumped(self, X, ne, freq):
         """Loaded solution of the first ne columns from the solve of
@@ -1034,8 +1090,9 @@ class Solver:
         return X[:, :ne] if self.Lm is None else self._load_lumped(X[:, :ne], X[:, ne:], freq)
 
     def lumped_currents(self, x):
-        """Current a -> b through each lumped gap, (n_lumped, Nexc)."""
-        return self.Lm.T @ x
+        """Current a -> b through each lumped gap, (n_gaps, Nexc): the gap
+        columns' signed edge lengths applied to x."""
+        return np.einsum("ng,ne->ge", self.Lm, x)
 
     def _solve_dev(self, g, excitations, interp=None):
         B = torch.from_numpy(np.ascontiguousarray(excitations, complex)).to(self.device)
@@ -1220,21 +1277,37 @@ class Solver:
         return a["last"] is None or a["last"] < a["cost"] / a["n"]
 
     # ----------------------------------------------------------- de-embed
+    def _divergence(self):
+        """Sparse map from the unknowns to the surface divergence of the
+        current on each triangle, (Nt x N): an RWG basis contributes
+        +-l / A on its two triangles, a via segment +-1 / (pad area) on the
+        triangles of its top and bottom pads (its current leaves the top pad
+        and enters the bottom one).  Built once per mesh."""
+        if getattr(self, "_div_op", None) is None:
+            import scipy.sparse as sp
+
+            m = self.mesh
+            b = m.tri_basis
+            t_idx, slot = np.nonzero(b >= 0)
+            cols = b[t_idx, slot]
+            vals = m.tri_sign[t_idx, slot] * m.edge_len[cols] / m.area[t_idx]
+            rows, cs, vs = [t_idx], [cols], [vals]
+            for k, seg in enumerate(m.via_segments):
+                for key, sign in (("top_tris", 1.0), ("bottom_tris", -1.0)):
+                    pads = seg[key]
+                    if pads is not None and len(pads):
+                        pads = np.asarray(pads)
+                        rows.append(pads)
+                        cs.append(np.full(pads.size, m.n_basis + k))
+                        vs.append(np.full(pads.size, sign / m.area[pads].sum()))
+            self._div_op = sp.csr_matrix((np.concatenate(vs), (np.concatenate(rows), np.concatenate(cs))),
+                                         shape=(len(m.tris), m.n_unknowns))
+        return self._div_op
+
     def charges(self, x, omega):
-        """Surface charge per triangle (Nt, Nexc) (Engine/mom25d/solver.py:1076-1091)."""
-        m = self.mesh
-        b = m.tri_basis
-        lab = np.where(b >= 0, m.edge_len[np.maximum(b, 0)], 0.0)
-        cV = m.tri_sign * lab / m.area[:, None]
-        div = np.einsum("ti,tie->te", cV, np.where((b >= 0)[..., None], x[np.maximum(b, 0)], 0))
-        for k, g in enumerate(m.via_segments):
-            t = g["top_tris"]
-            if t is not None:
-                div[t] += x[m.n_basis + k] / m.area[t].sum()
-            t = g["bottom_tris"]
-            if t is not None:
-                div[t] -= x[m.n_basis + k] / m.area[t].sum()
-        return -div / (1j * omega)
+        """Surface charge per triangle, (Nt, Nexc), from continuity:
+        q = -div(J) / (j omega) (Engine/mom25d/solver.py:1076-1091)."""
+        return -(self._divergence() @ x) / (1j * omega)
 
     def line_voltage(self, greens, q, pts, layer="top"):
         """Voltage to the ground plane at points of a metal layer from the

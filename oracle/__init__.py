@@ -18,9 +18,12 @@ Functions and what pins them (tests named in parentheses):
                                                                    mesh at 2 frequencies, symmetry,
                                                                    numpy path == C path (test_oracle_assemble)
   fill.c (oracle_fill_block)   Engine/mom25d/_fast.py:82-158        == numpy path to 1e-12 (test_oracle_assemble)
+  deembed_ref                  Engine/mom25d/solver.py:570-670, 1337-1427  wave fit with the reference's
+                                                                   optimisers, reference impedances, port
+                                                                   samples (through solve_ref.sweep)
   solve_ref.sweep              Engine/mom25d/solver.py:11-31, 569-1427  golden S / gamma / z0_line of the
-                                                                   reference on the tee (test_oracle_sweep),
-                                                                   Kirschning-Jansen through line
+                                                                   reference on tiny, tee and the via
+                                                                   transition (test_oracle_sweep)
 Third-party arithmetic used as in the reference: scipy.special.jv / j0
 (AMOS / Cephes), scipy.interpolate.CubicSpline, scipy.linalg.lu_factor,
 scipy.optimize.minimize(_scalar) -- the reference's own dependencies

@@ -1,8 +1,10 @@
 """Oracle: the per-frequency sweep on the CPU (Engine/mom25d/solver.py:11-31,
 984-1335): tables, assemble, LAPACK LU (scipy), charges, line voltages, wave
-fit and S.  The port models, the wave fit and the reference-impedance fit
-are the product's host FP64 code (mom25d_amd.solver), which is pinned here
-by the reference's golden S / gamma / z0_line vectors.  Test infrastructure only."""
+fit and S.  The port samples, the wave fit (with the reference's optimisers)
+and the reference-impedance fit are the oracle's own (deembed_ref), pinned
+by the reference's golden S / gamma / z0_line vectors.  Only the input data
+model (Stackup: levels, heights, surface impedance) and the Kirschning-Jansen
+start value (analytic.beta) come from the package.  Test infrastructure only."""
 
 import time
 
@@ -27,10 +29,9 @@ class OracleSolver:
     """CPU restatement of Solver(mesh, stack) with method='dense', interp=None."""
 
     def __init__(self, mesh, substrate, use_c=True, lapack_threads=None):
-        # host-side port / grouping set-up shared with the product (pure numpy)
-        from mom25d_amd.deembed import VOLTAGE_WEIGHT, FitContext
         from mom25d_amd.layered import Stackup
-        from mom25d_amd.solver import PortModel
+
+        from .deembed_ref import PortSamplesRef
 
         self.mesh = mesh
         self.stack = substrate if isinstance(substrate, Stackup) else substrate.to_stackup()
@@ -39,7 +40,7 @@ class OracleSolver:
         if getattr(mesh, "level_order", None) != order:
             mesh.assign_levels(order)
         self.ports = mesh.layout.ports
-        self.pm = [PortModel(mesh, p, st.level_h(p.layer)) for p in self.ports]
+        self.pm = [PortSamplesRef(mesh, p, st.level_h(p.layer)) for p in self.ports]
         v = mesh.nodes
         self.rho_max = 1.02 * np.hypot(*(v.max(0) - v.min(0)))
         self.zl = np.array([st.level_z(n) for n in mesh.layer_names])
@@ -52,9 +53,7 @@ class OracleSolver:
         for p, port in enumerate(self.ports):
             key = (round(port.width / st.level_h(port.layer), 9), port.layer, None, port.z0_from)
             self.groups.setdefault(key, []).append(p)
-        self.z0_ctx = FitContext(groups=list(self.groups.values()), width=None, h=None, eps=None, covered=None,
-                                 z0_from=[p.z0_from for p in self.ports], s=None, lossless=None, v_weight=None)
-        self.v_weight = VOLTAGE_WEIGHT
+        self.v_weight = 1.0  # VOLTAGE_WEIGHT, Engine/mom25d/solver.py:615
         self.timing = {}
 
     def geometry(self):
@@ -112,7 +111,8 @@ class OracleSolver:
 
     def sweep(self, freqs):
         from mom25d_amd import analytic
-        from mom25d_amd.deembed import fit_waves, reciprocal_z0
+
+        from .deembed_ref import fit_waves_ref, z0_reciprocal_ref
 
         freqs = np.atleast_1d(np.asarray(freqs, float))
         nf, n = len(freqs), len(self.ports)
@@ -152,14 +152,14 @@ class OracleSolver:
                         I_l.append(Ip[:, e])
                         V_l.append(Vp[:, e])
                         keys.append((p, e))
-                gm, z0, coefs, r = fit_waves(s_l, I_l, V_l, 1e-3 * b0 + 1j * b0, lossless=self.stack.lossless,
-                                             bounds=bnd, v_weight=self.v_weight)
+                gm, z0, coefs, r = fit_waves_ref(s_l, I_l, V_l, 1e-3 * b0 + 1j * b0, lossless=self.stack.lossless,
+                                                 bounds=bnd, v_weight=self.v_weight)
                 for (p, e), (a, bb) in zip(keys, coefs):
                     A[p, e], B[p, e] = a, bb
                 for p in members:
                     gam[fi, p], zl[fi, p], res[fi, p] = gm, z0, r
             SI = -B @ np.linalg.inv(A)
-            zr[fi] = reciprocal_z0(self.z0_ctx, SI, zl[fi].real)
+            zr[fi] = z0_reciprocal_ref(SI, zl[fi].real, list(self.groups.values()), [p.z0_from for p in self.ports])
             d = np.sqrt(zr[fi])
             S[fi] = d[:, None] * SI / d[None, :]
             t4 = time.time()

@@ -56,6 +56,44 @@ def test_touchstone_formats_and_interpolation(tmp_path):
         Touchstone(str(ri)).z(5e9)
 
 
+def test_series_part_from_s2p(tmp_path):
+    """A series R + L + C written as a 2-port over 1-10 GHz: inside the band
+    Z is the file's (interpolated), the model fitted for outside it recovers
+    R, L and C and continues Z above and below the band; without
+    extrapolation a frequency outside the band raises."""
+    from mom25d_amd.lumped import series_from_s2p
+
+    r, l, c = 0.5, 0.15e-9, 0.2e-12
+    f = np.linspace(1e9, 10e9, 40)
+    z = r + 1j * (2 * np.pi * f * l - 1 / (2 * np.pi * f * c))
+    s21 = 2 * 50 / (2 * 50 + z)
+    s11 = z / (2 * 50 + z)
+    p = tmp_path / "part.s2p"
+    p.write_text("# Hz S RI R 50\n" + "".join(f"{fi} {a.real} {a.imag} {b.real} {b.imag} {b.real} {b.imag} "
+                                              f"{a.real} {a.imag}\n" for fi, a, b in zip(f, s11, s21)))
+    v = parse(str(p))
+    r0, r1, ind, cap = v.fit
+    assert abs(ind / l - 1) < 1e-9 and abs(cap / c - 1) < 1e-9 and abs(r0 - r) < 1e-9 and abs(r1) < 1e-18
+    for fx in (20e9, 0.5e9):
+        w = 2 * np.pi * fx
+        assert abs(v.z(fx) - (r + 1j * (w * l - 1 / (w * c)))) < 1e-6
+    z5 = r + 1j * (2 * np.pi * 5e9 * l - 1 / (2 * np.pi * 5e9 * c))
+    assert abs(v.z(5e9) - z5) < 1e-3 * abs(z5)
+    with pytest.raises(ValueError, match="outside"):
+        series_from_s2p(str(p), extrapolate=False).z(20e9)
+
+
+def test_parse_grammar_errors():
+    """'||' binds weaker than '+', and malformed text raises ValueError."""
+    f = 2e9
+    v = parse("R 10 + L 1n || C 2p")
+    ref = parallel(series(R(10), L(1e-9)), C(2e-12))
+    assert v.z(f) == pytest.approx(ref.z(f))
+    for bad in ("X 5", "R", "R 5 +", "R 5 C 1p"):
+        with pytest.raises(ValueError):
+            parse(bad)
+
+
 def test_layout_rejects_what_this_build_lacks():
     from mom25d_amd.mesh import Layout
 

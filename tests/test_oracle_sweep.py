@@ -1,4 +1,6 @@
-"""The oracle's sweep against the reference's S / gamma / z0_line (golden)."""
+"""The oracle's sweep (its own restatement of the reference's de-embedding,
+oracle/deembed_ref.py) against the reference's S / gamma / z0_line (golden):
+the one-slab tee and tiny meshes and the two-level via transition."""
 
 import numpy as np
 import pytest
@@ -7,7 +9,7 @@ from mom25d_amd import workloads as W
 from oracle.solve_ref import OracleSolver
 
 
-@pytest.mark.parametrize("name", ["tiny", "tee"])
+@pytest.mark.parametrize("name", ["tiny", "tee", "via2"])
 def test_sweep_matches_reference(name, golden):
     g = golden("sweeps.npz")
     lay, sub, mk, _ = getattr(W, name)()
