@@ -46,11 +46,11 @@ def _cpu_model():
 
 
 def time_path(mesh, sub, freqs, precision, warmup, steps, torch, lanes=None, autotune=True, panel=None, tile=None,
-              block=None, interp=None):
+              block=None, interp=None, **kw):
     from mom25d_amd import Solver
 
     s = Solver(mesh, sub, verbose=False, precision=precision, lanes=lanes, autotune=autotune and tile is None,
-               panel=panel)
+               panel=panel, **kw)
     if tile is not None:
         s.tile = tuple(tile)
         s.block = block
@@ -58,6 +58,7 @@ def time_path(mesh, sub, freqs, precision, warmup, steps, torch, lanes=None, aut
     if warmup:
         s.sweep(freqs[len(freqs) // 2 - warmup // 2:][:warmup], interp=None)
     s.lu_stats = {k: 0 for k in s.lu_stats}
+    s.point_log = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     r = s.sweep(sweep_points(freqs, steps), interp=interp)  # None: exact fill at every point
@@ -189,6 +190,12 @@ def main():
                     help="skip timing the headline path with the interpolated impedance (interp='auto')")
     ap.add_argument("--cpu-reference-defaults", action="store_true",
                     help="also time the CPU sample at the reference's thread defaults (8 fill, 4 BLAS threads)")
+    ap.add_argument("--lu-equiv", type=float, default=None,
+                    help="mixed path: cost of a factorisation in GMRES steps for the LU reuse policy (default: model)")
+    ap.add_argument("--lu64-steps", type=int, default=20,
+                    help="mixed path: GMRES steps after which a complex64 solve gives way to a complex128 anchor")
+    ap.add_argument("--solve-sb", type=int, default=512,
+                    help="mixed path: super-block rows of the preconditioner solves (0: the 64-row chain)")
     ap.add_argument("--hw-queues", type=int, default=16,
                     help="GPU_MAX_HW_QUEUES of this process (HIP's default is 4)")
     args = ap.parse_args()
@@ -218,14 +225,20 @@ def main():
     prec = args.precision
     alt = "mixed" if prec == "fp64" else "fp64"
     tile = tuple(int(x) for x in args.tile.split(",")) if args.tile else None
+    mixed_kw = dict(lu_equiv=args.lu_equiv, lu64_steps=args.lu64_steps if args.lu64_steps >= 0 else None,
+                    solve_sb=args.solve_sb)
     sm, rm, tm = time_path(mesh, sub, freqs, prec, args.warmup, args.steps, torch, args.lanes, not args.no_tune,
-                           args.panel, tile, args.block)
+                           args.panel, tile, args.block, **(mixed_kw if prec == "mixed" else {}))
+    for rec in sm.point_log:
+        print(f"  point {rec['freq'] / 1e9:6.3f} GHz lane {rec['lane']}: " +
+              ", ".join(f"{k} {v}" for k, v in rec["solves"]), file=sys.stderr)
     lanes = sm._lane_count()
     sweep_lu_stats = dict(sm.lu_stats)  # the timed sweep's (later solves add to sm.lu_stats)
     print(f"{prec}: {args.steps} points in {tm:.3f} s (LU: {sm.lu_stats})", file=sys.stderr, flush=True)
     other = None
     if not args.no_alt:
-        sa, ra, ta = time_path(mesh, sub, freqs, alt, args.warmup, args.steps, torch, args.lanes, not args.no_tune)
+        sa, ra, ta = time_path(mesh, sub, freqs, alt, args.warmup, args.steps, torch, args.lanes, not args.no_tune,
+                               **(mixed_kw if alt == "mixed" else {}))
         other = dict(precision=alt, value=args.steps / ta, ms_per_step=1e3 * ta / args.steps,
                      dS_max_vs_headline=float(np.max(np.abs(rm.S - ra.S))), lu_stats=dict(sa.lu_stats))
         print(f"{alt}: {args.steps} points in {ta:.3f} s; |dS| {other['dS_max_vs_headline']:.2e}", file=sys.stderr,
@@ -235,7 +248,7 @@ def main():
     interp_line = None
     if not args.no_interp and args.steps == len(freqs):
         si, rI, tI = time_path(mesh, sub, freqs, prec, args.warmup, args.steps, torch, args.lanes, not args.no_tune,
-                               args.panel, tile, args.block, interp="auto")
+                               args.panel, tile, args.block, interp="auto", **(mixed_kw if prec == "mixed" else {}))
         fi = si.last_interp
         interp_line = dict(value=args.steps / tI, ms_per_step=1e3 * tI / args.steps,
                            anchors=None if fi is None else len(fi.anchors),
@@ -264,6 +277,8 @@ def main():
     extra["lu_config"] = dict(panel=sm.panel, block=sm.block or sm.panel, tile=[bm, bn],
                               autotuned=not args.no_tune and tile is None)
     extra["lu_stats"] = sweep_lu_stats  # factorisations / reused LUs / GMRES steps in the timed sweep
+    if prec == "mixed":
+        extra["mixed_policy"] = dict(lu_equiv=sm._lu_equiv(), lu64_steps=sm.lu64_steps, solve_sb=sm.solve_sb)
     if interp_line is not None:
         extra["interp_auto"] = interp_line  # the reference's default sweep mode (interpolated Z between anchors)
     if not args.no_batch:

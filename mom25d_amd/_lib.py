@@ -50,6 +50,10 @@ SIGNATURES = {
     "mom_scaled_symv": (_i, [_i, _i, _p, _i64, _p, _p, _p, _p, _i64, _p]),
     "mom_gmres_work_elems": (_i64, [_i, _i]),
     "mom_gmres_arnoldi": (_i, [_i, _i, _i, _i, _i, _p, _i64, _p, _p, _p, _i64, _p, _p, _i64, _p, _p, _p, _p, _p]),
+    "mom_sbsolve_work_bytes": (_i64, [_i, _i, _i]),
+    "mom_sbsolve_setup": (_i, [_i, _i, _p, _i64, _i, _p, _i64, _p]),
+    "mom_sbsolve_z": (_i, [_i, _i, _i, _i, _p, _p, _p, _p, _p]),
+    "mom_gmres_arnoldi_sb": (_i, [_i, _i, _i, _i, _i, _i, _p, _p, _p, _i64, _p, _p, _i64, _p, _p, _p, _p, _p]),
     "mom_set_solve_rows": (_i, [_i]),
     "mom_get_solve_rows": (_i, []),
     "mom_set_solve_chain": (_i, [_i]),

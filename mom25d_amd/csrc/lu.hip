@@ -44,6 +44,12 @@ __device__ inline cplx<R> load_coherent_c(const cplx<R>* p) {
   const R* q = reinterpret_cast<const R*>(p);
   return {load_coherent(q), load_coherent(q + 1)};
 }
+// complex64: one 8-byte device-coherent load
+template <>
+__device__ inline cplx<float> load_coherent_c(const cplx<float>* p) {
+  const unsigned long long v = load_coherent(reinterpret_cast<const unsigned long long*>(p));
+  return {__uint_as_float((unsigned)v), __uint_as_float((unsigned)(v >> 32))};
+}
 
 // Cross-workgroup publication without cache maintenance: data is written
 // and read with device-coherent (relaxed, agent-scope) accesses, and every
@@ -551,7 +557,8 @@ struct mfma_traits<float> {
 #endif
 constexpr int kGemmBK = MOM_GEMM_BK;
 
-template <typename R, int BM, int BN>
+// ASSIGN: C = -A B (C is not read) instead of C -= A B.
+template <typename R, int BM, int BN, bool ASSIGN = false>
 __global__ __launch_bounds__(256) void cgemm_sub_kernel(int m, int n, int kk, const cplx<R>* __restrict__ Ap,
                                                         int64_t lda, const cplx<R>* __restrict__ Bp, int64_t ldb,
                                                         cplx<R>* __restrict__ Cp, int64_t ldc) {
@@ -639,7 +646,10 @@ __global__ __launch_bounds__(256) void cgemm_sub_kernel(int m, int n, int kk, co
       for (int v = 0; v < 4; ++v) {
         const int r = row0 + wr + 16 * a + mfma_traits<R>::row(lane, v);
         const int cr = colr0 + wc + 16 * b + (lane & 15);
-        if (r < m && cr < 2 * n) Cr[(int64_t)r * 2 * ldc + cr] -= acc[a][b][v];
+        if (r < m && cr < 2 * n) {
+          R* c = Cr + (int64_t)r * 2 * ldc + cr;
+          *c = ASSIGN ? -acc[a][b][v] : *c - acc[a][b][v];
+        }
       }
 }
 
@@ -648,21 +658,21 @@ __global__ __launch_bounds__(256) void cgemm_sub_kernel(int m, int n, int kk, co
 // MI355X for the N ~ 12k factorisation.
 static int g_tile[2][2] = {{64, 128}, {64, 128}};  // [complex128, complex64][BM, BN]
 
-template <typename R, int BM, int BN>
+template <typename R, int BM, int BN, bool ASSIGN = false>
 static int launch_cgemm_tile(int m, int n, int k, const cplx<R>* A, int64_t lda, const cplx<R>* B, int64_t ldb,
                              cplx<R>* C, int64_t ldc, hipStream_t s) {
   if ((m + BM - 1) / BM > 65535) return fail("cgemm: too many row tiles");
   dim3 grid((2 * n + BN - 1) / BN, (m + BM - 1) / BM);
-  hipLaunchKernelGGL((cgemm_sub_kernel<R, BM, BN>), grid, dim3(256), 0, s, m, n, k, A, lda, B, ldb, C, ldc);
+  hipLaunchKernelGGL((cgemm_sub_kernel<R, BM, BN, ASSIGN>), grid, dim3(256), 0, s, m, n, k, A, lda, B, ldb, C, ldc);
   return check_launch("cgemm_sub_kernel");
 }
 
-template <typename R>
+template <typename R, bool ASSIGN = false>
 static int launch_cgemm(int m, int n, int k, const cplx<R>* A, int64_t lda, const cplx<R>* B, int64_t ldb,
                         cplx<R>* C, int64_t ldc, hipStream_t s) {
   const int* t = g_tile[sizeof(R) == sizeof(double) ? 0 : 1];
 #define MOM_TILE(BM, BN) \
-  if (t[0] == BM && t[1] == BN) return launch_cgemm_tile<R, BM, BN>(m, n, k, A, lda, B, ldb, C, ldc, s);
+  if (t[0] == BM && t[1] == BN) return launch_cgemm_tile<R, BM, BN, ASSIGN>(m, n, k, A, lda, B, ldb, C, ldc, s);
   MOM_TILE(64, 64)
   MOM_TILE(64, 128)
   MOM_TILE(128, 64)
@@ -782,10 +792,18 @@ __global__ __launch_bounds__(BS * BS / 16) void trsv_kernel(int n, const cplx<R>
   const int nblk = (n + BS - 1) / BS;
   if (tid == 0) {
     const unsigned t = atomicAdd(ticket, 1u);
-    s_blk = LOWER ? (int)t : nblk - 1 - (int)t;
+    // a ticket past the grid means that two solves share this workspace at
+    // once: report it instead of indexing outside the factors
+    if (t >= (unsigned)nblk) {
+      atomicOr(error, 1u);
+      s_blk = -1;
+    } else {
+      s_blk = LOWER ? (int)t : nblk - 1 - (int)t;
+    }
     s_abort = 0;
   }
   __syncthreads();
+  if (s_blk < 0) return;
   const int blk = s_blk;
   const int k0 = blk * BS, b = min(BS, n - k0);
   const int row = k0 + r;
@@ -1704,6 +1722,473 @@ static int getrs(int n, int nrhs, const cplx<R>* LU, int64_t lda, const int32_t*
   return trsv_pair<R, INV, kTrsvBS>(n, nrhs, LU, lda, X, Y, DL, DU, ML, MU, X64, flags, error, s);
 }
 
+// ------------------------------------------------- super-block triangular solves
+// The preconditioner solve of the mixed path as a chain of super-blocks of
+// sb = 512 rows instead of 64, so that the chain has 8x fewer hops.  After a
+// factorisation (once), with D_k the inverse of the k-th diagonal super-block
+// of L (of U) and the couplings scaled by it,
+//     T_kj = -D_k L_kj (j < k),   T_kj = -D_k U_kj (j > k),
+// the two solves read
+//     x_k = D_k b_k + sum_{j before k} T_kj x_j
+// where "before" is the solve order (increasing k for L, decreasing for U).
+// Each row of x_k is an independent dot product: a workgroup owns a few rows
+// of one super-block, forms D_k b_k and the terms of the super-blocks solved
+// earlier as they appear (streaming its rows of T from HBM), and only the
+// last term, that of the super-block solved just before, sits on the chain's
+// critical path (its tile of T is loaded before the wait).  The x values are
+// their own ready flags (sentinel, as in the 64-row chain), and workgroups
+// take their super-block by ticket, so that they only wait for workgroups
+// already running.
+//
+// Workspace (mom_sbsolve_work_bytes): 256 status bytes (tickets in words 0
+// and 2, the sticky failure word 4), Y (the lower solve's output), X2 (the
+// upper solve's), X1 (P b), each n x 4 complex in precision R; the 64-row
+// inverses of L's and U's diagonal blocks (scratch of the set-up); D for L
+// and for U (nsb x sb x sb); T (n x ldt, ldt = nsb sb).
+constexpr int kSbMax = 512;
+
+struct SbLayout {
+  size_t y, x2, x1, d64, dl, du, t, total;
+  int nsb, ldt, nblk64;
+};
+
+template <typename R>
+static SbLayout sb_layout(int n, int sb) {
+  auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+  SbLayout L;
+  n = std::max(n, 1);
+  L.nsb = (n + sb - 1) / sb;
+  L.ldt = L.nsb * sb;
+  L.nblk64 = (n + kTrsvBS - 1) / kTrsvBS;
+  const size_t vec = al(sizeof(cplx<R>) * 4 * (size_t)n);
+  size_t off = 256;
+  L.y = off;
+  off += vec;
+  L.x2 = off;
+  off += vec;
+  L.x1 = off;
+  off += vec;
+  L.d64 = off;
+  off += al(sizeof(cplx<R>) * 2 * (size_t)L.nblk64 * kTrsvBS * kTrsvBS);
+  L.dl = off;
+  off += al(sizeof(cplx<R>) * (size_t)L.nsb * sb * sb);
+  L.du = off;
+  off += al(sizeof(cplx<R>) * (size_t)L.nsb * sb * sb);
+  L.t = off;
+  off += al(sizeof(cplx<R>) * (size_t)n * L.ldt);
+  L.total = off;
+  return L;
+}
+
+// Inverse of the k-th diagonal super-block of L (blockIdx.z = 0, unit lower)
+// or U (1), 64 columns per workgroup (blockIdx.y), by block substitution over
+// its 64-row sub-blocks with the 64-row inverses D64 (trtri_blocks_kernel):
+// lower X_i = D64_i (E_i - sum_{c <= m < i} L_im X_m) for i = c .. (the
+// identity column block E), upper the mirror; products in FP64 through LDS,
+// results rounded to R.  Sub-blocks past n are the identity (rows and
+// columns of the factors there read as zero), so a partial last super-block
+// gets a finite padding that the solve multiplies by zero.
+template <typename R>
+__global__ __launch_bounds__(256) void sb_inverse_kernel(int n, const cplx<R>* __restrict__ A, int64_t lda,
+                                                         const cplx<R>* __restrict__ D64, int nblk64, int sb,
+                                                         cplx<R>* DL, cplx<R>* DU) {
+  constexpr int H = kTrsvBS;
+  extern __shared__ unsigned char smem_raw[];
+  z64* SX = reinterpret_cast<z64*>(smem_raw);  // [H][H] right operand (FP64)
+  cplx<R>* SA = reinterpret_cast<cplx<R>*>(SX + H * H);  // [H][H] left operand
+  const int k = blockIdx.x, c = blockIdx.y, upper = blockIdx.z, tid = threadIdx.x;
+  const int nsub = sb / H, base = k * sb;
+  cplx<R>* out = (upper ? DU : DL) + (int64_t)k * sb * sb;
+  const int ir = tid / 4, j0 = (tid % 4) * 16;  // thread: row ir, columns j0 .. j0 + 15 of a 64 x 64 block
+  auto at = [&](int i, int r, int col) { return out + (int64_t)(i * H + r) * sb + c * H + col; };
+  for (int i = 0; i < nsub; ++i)  // the sub-blocks of this column block that are zero
+    if (upper ? i > c : i < c)
+      for (int e = tid; e < H * H; e += 256) *at(i, e / H, e % H) = {0, 0};
+  const int step = upper ? -1 : 1;
+  for (int i = c; upper ? i >= 0 : i < nsub; i += step) {
+    z64 acc[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[e] = {(double)(i == c && ir == j0 + e), 0.0};
+    for (int mm = c; mm != i; mm += step) {
+      __syncthreads();
+      for (int e = tid; e < H * H; e += 256) {
+        const int r = e / H, q = e % H;
+        const int gr = base + i * H + r, gc = base + mm * H + q;
+        SA[e] = (gr < n && gc < n) ? A[(int64_t)gr * lda + gc] : cplx<R>{0, 0};
+        const cplx<R> x = load_coherent_c(at(mm, r, q));  // written by this workgroup (stores drained below)
+        SX[e] = {(double)x.re, (double)x.im};
+      }
+      __syncthreads();
+      for (int kk = 0; kk < H; ++kk) {
+        const cplx<R> a = SA[ir * H + kk];
+        const z64 ad = {(double)a.re, (double)a.im};
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc[e] -= ad * SX[kk * H + j0 + e];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; ++e) SX[ir * H + j0 + e] = acc[e];
+    const int g64 = base / H + i;
+    for (int e = tid; e < H * H; e += 256)
+      SA[e] = g64 < nblk64 ? D64[((int64_t)upper * nblk64 + g64) * H * H + e] : cplx<R>{(R)(e / H == e % H), 0};
+    __syncthreads();
+    z64 x[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) x[e] = {0, 0};
+    for (int kk = 0; kk < H; ++kk) {
+      const cplx<R> a = SA[ir * H + kk];
+      const z64 ad = {(double)a.re, (double)a.im};
+#pragma unroll
+      for (int e = 0; e < 16; ++e) x[e] += ad * SX[kk * H + j0 + e];
+    }
+#pragma unroll
+    for (int e = 0; e < 16; ++e) *at(i, ir, j0 + e) = {(R)x[e].re, (R)x[e].im};
+    __builtin_amdgcn_s_waitcnt(0);  // the stores have reached L2 before another thread reads them
+  }
+}
+
+// Set-up of a solve: tickets cleared (words 0 and 2; the failure word stays
+// for the caller), X1 = P B rounded to R, Y and X2 (the two solves' outputs)
+// filled with the sentinel.
+template <typename R, typename TB>
+__global__ void sb_prep_kernel(int n, int nrhs, const int32_t* perm, const cplx<TB>* B, cplx<R>* X1, cplx<R>* Y,
+                               cplx<R>* X2, unsigned* flags) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q == 0 || q == 2) flags[q] = 0u;
+  if (q >= n * nrhs) return;
+  const int i = q / nrhs, c = q % nrhs;
+  const cplx<TB> b = B[(int64_t)perm[i] * nrhs + c];
+  X1[q] = {(R)b.re, (R)b.im};
+  const cplx<R> sen = {Sentinel<R>::value(), Sentinel<R>::value()};
+  Y[q] = sen;
+  X2[q] = sen;
+}
+
+// DPP move of a 32-bit value (lanes without a source, or outside row_mask, read 0).
+template <int CTRL, int ROWS>
+__device__ inline int dpp32(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xf, true);
+}
+template <int CTRL, int ROWS>
+__device__ inline float dpp_move(float v) {
+  return __int_as_float(dpp32<CTRL, ROWS>(__float_as_int(v)));
+}
+template <int CTRL, int ROWS>
+__device__ inline double dpp_move(double v) {
+  return __hiloint2double(dpp32<CTRL, ROWS>(__double2hiint(v)), dpp32<CTRL, ROWS>(__double2loint(v)));
+}
+// Sum over the 64 lanes of a wavefront, returned in every lane: shifts by 1,
+// 2, 4 and 8 within each row of 16 lanes (lane 15 of a row then holds the
+// row's sum), row 0 into row 1 and row 2 into row 3 (row_bcast:15), rows 0-1
+// into rows 2-3 (row_bcast:31), and lane 63 read out through the scalar unit.
+template <typename T>
+__device__ inline T wave_total(T v) {
+  v += dpp_move<0x111, 0xf>(v);
+  v += dpp_move<0x112, 0xf>(v);
+  v += dpp_move<0x114, 0xf>(v);
+  v += dpp_move<0x118, 0xf>(v);
+  v += dpp_move<0x142, 0xa>(v);
+  v += dpp_move<0x143, 0xc>(v);
+  return lane_value(v, 63);
+}
+
+// One triangular solve (lower: UPPER false) over super-blocks of 512 rows
+// with NR right-hand sides.  Workgroup: 4 wavefronts, RPW rows each (2 in
+// complex64, 1 in complex128); lane l covers the 8 columns 128 q + 2 l + h
+// (q < 4, h < 2) of a super-block, so that every load instruction of a
+// wavefront reads 64 consecutive pieces of one row.  Per-lane partial sums
+// are reduced across the wavefront once, after the last term.
+// rows per wavefront of the super-block chain: one tile of T (RPW x 8
+// complex per lane) in flight besides the next keeps it at two workgroups
+// per CU without spills
+template <typename R>
+constexpr int kSbRowsPerWave() { return sizeof(R) == sizeof(float) ? 2 : 1; }
+
+template <typename R, int NR, bool UPPER>
+__global__ __launch_bounds__(256, 2) void sb_chain_kernel(int n, int nsb, const cplx<R>* __restrict__ T, int64_t ldt,
+                                                       const cplx<R>* __restrict__ D, const cplx<R>* __restrict__ in,
+                                                       cplx<R>* out, z64* __restrict__ out64, unsigned* ticket,
+                                                       unsigned* error, unsigned spin_limit) {
+  constexpr int SB = kSbMax, CPL = 8;
+  constexpr int RPW = kSbRowsPerWave<R>();
+  constexpr int ROWS = 4 * RPW, G = SB / ROWS;
+  // x of up to NB earlier super-blocks at once in LDS (32 KB): their values
+  // are fetched with one round of device-coherent loads, so a workgroup that
+  // is behind the chain catches up one memory round trip per NB super-blocks
+  constexpr int XB = 32768 / (SB * NR * (int)sizeof(cplx<R>));
+  constexpr int NB = XB >= 4 ? 4 : (XB >= 1 ? XB : 1);
+  __shared__ cplx<R> xs[NB][SB][NR];
+  __shared__ int s_ticket, s_abort;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid == 0) {
+    const unsigned t = atomicAdd(ticket, 1u);
+    // a ticket past the grid means that two solves share this workspace at
+    // once: report it instead of indexing outside the factors
+    if (t >= (unsigned)(nsb * G)) {
+      atomicOr(error, 1u);
+      s_ticket = -1;
+    } else {
+      s_ticket = (int)t;
+    }
+    s_abort = 0;
+  }
+  __syncthreads();
+  if (s_ticket < 0) return;
+  const int o = s_ticket / G, g = s_ticket % G;
+  const int k = UPPER ? nsb - 1 - o : o;
+  const int lrow0 = g * ROWS + wave * RPW;  // first row of this wavefront within the super-block
+  const int row0 = k * SB + lrow0;
+  auto colof = [&](int e) { return 128 * (e >> 1) + 2 * lane + (e & 1); };
+  cplx<R> acc[RPW][NR];
+#pragma unroll
+  for (int r = 0; r < RPW; ++r)
+#pragma unroll
+    for (int cc = 0; cc < NR; ++cc) acc[r][cc] = {0, 0};
+  // (columns past n read memory inside the allocation and are then zeroed;
+  // T's tiles below are loaded under a row / column guard, measured faster
+  // than clamped unconditional loads: 0.42 against 0.58 ms for the two
+  // complex64 solves at n = 12 428)
+  const cplx<R> zero = {0, 0};
+  {  // D_k b_k (b: the input, complete before this launch; D is padded to whole super-blocks)
+    cplx<R> bv[CPL][NR];
+#pragma unroll
+    for (int e = 0; e < CPL; ++e) {
+      const int col = k * SB + colof(e);
+#pragma unroll
+      for (int cc = 0; cc < NR; ++cc) {
+        const cplx<R> b = in[(int64_t)min(col, n - 1) * NR + cc];
+        bv[e][cc] = col < n ? b : zero;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+      const cplx<R>* drow = D + ((int64_t)k * SB + lrow0 + r) * SB;
+#pragma unroll
+      for (int e = 0; e < CPL; ++e) {
+        const cplx<R> d = drow[colof(e)];
+#pragma unroll
+        for (int cc = 0; cc < NR; ++cc) acc[r][cc] += d * bv[e][cc];
+      }
+    }
+  }
+  const int ndep = UPPER ? nsb - 1 - k : k;
+  auto dep = [&](int q) { return UPPER ? nsb - 1 - q : q; };  // super-blocks in solve order
+  // this wavefront's rows of T_kj
+  auto load_tile = [&](int j, cplx<R> (&tv)[RPW][CPL]) {
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+      const cplx<R>* trow = T + (int64_t)(row0 + r) * ldt + (int64_t)j * SB;
+#pragma unroll
+      for (int e = 0; e < CPL; ++e) {
+        const int col = j * SB + colof(e);
+        tv[r][e] = (row0 + r < n && col < n) ? trow[colof(e)] : cplx<R>{0, 0};
+      }
+    }
+  };
+  auto fma_tile = [&](const cplx<R> (&tv)[RPW][CPL], int b) {
+#pragma unroll
+    for (int r = 0; r < RPW; ++r)
+#pragma unroll
+      for (int e = 0; e < CPL; ++e)
+#pragma unroll
+        for (int cc = 0; cc < NR; ++cc) acc[r][cc] += tv[r][e] * xs[b][colof(e)][cc];
+  };
+  // x of super-blocks dep(q0) .. dep(q0 + nb - 1) into xs[0 .. nb): every
+  // thread issues its (up to VPT) device-coherent loads together, then
+  // reissues together, round after round, the ones that still read the
+  // sentinel (bounded), so that a wait costs one memory round trip per round
+  // rather than one per value; false when a wait ran out.  (Completion
+  // counters per super-block, read before one pass over the values, were
+  // slower: 0.64 against 0.42 ms for the two complex64 solves at n = 12 428.)
+  constexpr int VPT = NB * SB * NR / 256;  // values per thread (<= 16: the batch is <= 32 KB)
+  static_assert(VPT * 256 == NB * SB * NR && VPT <= 32, "batch layout");
+  auto fetch_x = [&](int q0, int nb) {
+    const int total = nb * SB * NR;
+    auto addr = [&](int idx, int& b, int& c, int& cc) {
+      b = idx / (SB * NR);
+      const int rem = idx % (SB * NR);
+      c = rem / NR;
+      cc = rem % NR;
+      return dep(q0 + b) * SB + c;  // column of x
+    };
+    uint32_t pending = 0;
+    cplx<R> v[VPT];
+#pragma unroll
+    for (int t = 0; t < VPT; ++t) {
+      const int idx = tid + 256 * t;
+      int b, c, cc;
+      const int col = idx < total ? addr(idx, b, c, cc) : n;
+      v[t] = col < n ? load_coherent_c(out + (int64_t)col * NR + cc) : cplx<R>{0, 0};
+    }
+#pragma unroll
+    for (int t = 0; t < VPT; ++t) {
+      const int idx = tid + 256 * t;
+      if (idx >= total) continue;
+      int b, c, cc;
+      addr(idx, b, c, cc);
+      xs[b][c][cc] = v[t];
+      if (is_sentinel(v[t])) pending |= 1u << t;
+    }
+    unsigned spins = 0;
+    while (pending) {
+      poll_pause();
+      if (++spins > spin_limit) {
+        s_abort = 1;
+        break;
+      }
+#pragma unroll
+      for (int t = 0; t < VPT; ++t) {
+        int b, c, cc;
+        const int col = addr(tid + 256 * t, b, c, cc);
+        if ((pending >> t) & 1u) v[t] = load_coherent_c(out + (int64_t)col * NR + cc);
+      }
+#pragma unroll
+      for (int t = 0; t < VPT; ++t) {
+        if (!((pending >> t) & 1u) || is_sentinel(v[t])) continue;
+        int b, c, cc;
+        addr(tid + 256 * t, b, c, cc);
+        xs[b][c][cc] = v[t];
+        pending &= ~(1u << t);
+      }
+    }
+    __syncthreads();
+    return s_abort == 0;
+  };
+  // the super-blocks solved before the previous one, NB at a time
+  cplx<R> tcur[RPW][CPL];
+  for (int q0 = 0; q0 < ndep - 1; q0 += NB) {
+    const int nb = min(NB, ndep - 1 - q0);
+    load_tile(dep(q0), tcur);
+    if (!fetch_x(q0, nb)) {
+      if (tid == 0) atomicOr(error, 1u);
+      return;
+    }
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      if (b >= nb) break;
+      cplx<R> tnext[RPW][CPL];
+      if (b + 1 < nb) load_tile(dep(q0 + b + 1), tnext);
+      fma_tile(tcur, b);
+      if (b + 1 < nb) {
+#pragma unroll
+        for (int r = 0; r < RPW; ++r)
+#pragma unroll
+          for (int e = 0; e < CPL; ++e) tcur[r][e] = tnext[r][e];
+      }
+    }
+    __syncthreads();  // every wavefront has read xs before the next batch overwrites it
+  }
+  // the previous super-block: the chain's critical path (its tile loaded before the wait)
+  if (ndep > 0) {
+    load_tile(dep(ndep - 1), tcur);
+    if (!fetch_x(ndep - 1, 1)) {
+      if (tid == 0) atomicOr(error, 1u);
+      return;
+    }
+    fma_tile(tcur, 0);
+  }
+#pragma unroll
+  for (int r = 0; r < RPW; ++r)
+#pragma unroll
+    for (int cc = 0; cc < NR; ++cc) {
+      acc[r][cc].re = wave_total(acc[r][cc].re);
+      acc[r][cc].im = wave_total(acc[r][cc].im);
+    }
+#pragma unroll
+  for (int r = 0; r < RPW; ++r)
+#pragma unroll
+    for (int cc = 0; cc < NR; ++cc)
+      if (lane == r * NR + cc && row0 + r < n) {
+        store_coherent_c(out + (int64_t)(row0 + r) * NR + cc, acc[r][cc]);
+        if (out64) out64[(int64_t)(row0 + r) * NR + cc] = {(double)acc[r][cc].re, (double)acc[r][cc].im};
+      }
+}
+
+template <typename R>
+static int sbsolve_setup(int n, const cplx<R>* LU, int64_t lda, int sb, void* work, size_t work_bytes, hipStream_t s) {
+  if (sb != kSbMax) return fail("mom_sbsolve_setup: super-block rows must be 512");
+  if (n <= 0) return 0;
+  const SbLayout L = sb_layout<R>(n, sb);
+  if (work_bytes < L.total) return fail("mom_sbsolve_setup: workspace too small");
+  unsigned char* w = reinterpret_cast<unsigned char*>(work);
+  cplx<R>* D64 = reinterpret_cast<cplx<R>*>(w + L.d64);
+  cplx<R>* DL = reinterpret_cast<cplx<R>*>(w + L.dl);
+  cplx<R>* DU = reinterpret_cast<cplx<R>*>(w + L.du);
+  cplx<R>* T = reinterpret_cast<cplx<R>*>(w + L.t);
+  const size_t lds64 = 2 * sizeof(z64) * kTrsvBS * kTrsvBS;
+  static bool attr = false;
+  if (!attr) {
+    if (hipFuncSetAttribute((const void*)trtri_blocks_kernel<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds64) != hipSuccess)
+      return fail("trtri_blocks_kernel: cannot reserve LDS");
+    if (hipFuncSetAttribute((const void*)sb_inverse_kernel<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds64) != hipSuccess)
+      return fail("sb_inverse_kernel: cannot reserve LDS");
+    attr = true;
+  }
+  hipLaunchKernelGGL(trtri_blocks_kernel<R>, dim3(L.nblk64, 2), dim3(kTrsvBS), lds64, s, n, LU, lda, D64);
+  int rc = check_launch("trtri_blocks_kernel");
+  if (rc) return rc;
+  const size_t ldsi = sizeof(z64) * kTrsvBS * kTrsvBS + sizeof(cplx<R>) * kTrsvBS * kTrsvBS;
+  hipLaunchKernelGGL(sb_inverse_kernel<R>, dim3(L.nsb, sb / kTrsvBS, 2), dim3(256), ldsi, s, n, LU, lda,
+                     (const cplx<R>*)D64, L.nblk64, sb, DL, DU);
+  if ((rc = check_launch("sb_inverse_kernel"))) return rc;
+  for (int k = 0; k < L.nsb; ++k) {
+    const int r0 = k * sb, rows = std::min(sb, n - r0);
+    if (k > 0 &&
+        (rc = launch_cgemm<R, true>(rows, r0, rows, DL + (int64_t)k * sb * sb, sb, LU + (int64_t)r0 * lda, lda,
+                                    T + (int64_t)r0 * L.ldt, L.ldt, s)))
+      return rc;
+    const int c0 = r0 + sb;
+    if (c0 < n &&
+        (rc = launch_cgemm<R, true>(rows, n - c0, rows, DU + (int64_t)k * sb * sb, sb, LU + (int64_t)r0 * lda + c0,
+                                    lda, T + (int64_t)r0 * L.ldt + c0, L.ldt, s)))
+      return rc;
+  }
+  return 0;
+}
+
+template <typename R, typename TB>
+static int sbsolve(int n, int nrhs, int sb, const int32_t* perm, const cplx<TB>* B, z64* X64, void* work,
+                   hipStream_t s) {
+  if (sb != kSbMax) return fail("mom_sbsolve: super-block rows must be 512");
+  if (n <= 0 || nrhs <= 0) return 0;
+  if (nrhs > 4) return fail("mom_sbsolve: at most 4 right-hand sides");
+  const SbLayout L = sb_layout<R>(n, sb);
+  unsigned char* w = reinterpret_cast<unsigned char*>(work);
+  unsigned* flags = reinterpret_cast<unsigned*>(w);
+  cplx<R>* Y = reinterpret_cast<cplx<R>*>(w + L.y);
+  cplx<R>* X2 = reinterpret_cast<cplx<R>*>(w + L.x2);
+  cplx<R>* X1 = reinterpret_cast<cplx<R>*>(w + L.x1);
+  const cplx<R>* DL = reinterpret_cast<const cplx<R>*>(w + L.dl);
+  const cplx<R>* DU = reinterpret_cast<const cplx<R>*>(w + L.du);
+  const cplx<R>* T = reinterpret_cast<const cplx<R>*>(w + L.t);
+  const int64_t nx = (int64_t)n * nrhs;
+  hipLaunchKernelGGL((sb_prep_kernel<R, TB>), dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, s, n, nrhs, perm, B,
+                     X1, Y, X2, flags);
+  int rc = check_launch("sb_prep_kernel");
+  if (rc) return rc;
+  constexpr int ROWS = 4 * kSbRowsPerWave<R>();
+  const dim3 grid(L.nsb * (kSbMax / ROWS));
+#define MOM_SB(NR)                                                                                                    \
+  case NR:                                                                                                            \
+    hipLaunchKernelGGL((sb_chain_kernel<R, NR, false>), grid, dim3(256), 0, s, n, L.nsb, T, (int64_t)L.ldt, DL,      \
+                       (const cplx<R>*)X1, Y, (z64*)nullptr, flags, flags + 4, g_spin_limit);                         \
+    if ((rc = check_launch("sb_chain_kernel"))) return rc;                                                            \
+    hipLaunchKernelGGL((sb_chain_kernel<R, NR, true>), grid, dim3(256), 0, s, n, L.nsb, T, (int64_t)L.ldt, DU,       \
+                       (const cplx<R>*)Y, X2, X64, flags + 2, flags + 4, g_spin_limit);                               \
+    break;
+  switch (nrhs) {
+    MOM_SB(1)
+    MOM_SB(2)
+    MOM_SB(3)
+    MOM_SB(4)
+  }
+#undef MOM_SB
+  return check_launch("sb_chain_kernel");
+}
+
 }  // namespace mom
 
 using namespace mom;
@@ -1797,6 +2282,31 @@ extern "C" int mom_getrs_inv_z(int prec, int n, int nrhs, const void* LU, int64_
   if (prec == 32)
     return getrs<float, true, double>(n, nrhs, (const c32*)LU, lda, perm, (const z64*)B, nullptr, (z64*)X, flags, s);
   return fail("mom_getrs_inv_z: prec must be 64 or 32");
+}
+
+extern "C" int64_t mom_sbsolve_work_bytes(int prec, int n, int sb) {
+  if (sb != kSbMax) return -1;
+  return prec == 64 ? (int64_t)sb_layout<double>(n, sb).total : (int64_t)sb_layout<float>(n, sb).total;
+}
+
+extern "C" int mom_sbsolve_setup(int prec, int n, const void* LU, int64_t lda, int sb, void* work, int64_t work_bytes,
+                                 void* stream) {
+  if (!LU || !work) return fail("mom_sbsolve_setup: null device pointer");
+  if (lda < n) return fail("mom_sbsolve_setup: lda < n");
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == 64) return sbsolve_setup<double>(n, (const z64*)LU, lda, sb, work, (size_t)work_bytes, s);
+  if (prec == 32) return sbsolve_setup<float>(n, (const c32*)LU, lda, sb, work, (size_t)work_bytes, s);
+  return fail("mom_sbsolve_setup: prec must be 64 or 32");
+}
+
+extern "C" int mom_sbsolve_z(int prec, int n, int nrhs, int sb, const int32_t* perm, const void* B, void* X,
+                             void* work, void* stream) {
+  if (!perm || !B || !X || !work) return fail("mom_sbsolve_z: null device pointer");
+  if (B == X) return fail("mom_sbsolve_z: B and X must not alias");
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == 64) return sbsolve<double, double>(n, nrhs, sb, perm, (const z64*)B, (z64*)X, work, s);
+  if (prec == 32) return sbsolve<float, double>(n, nrhs, sb, perm, (const z64*)B, (z64*)X, work, s);
+  return fail("mom_sbsolve_z: prec must be 64 or 32");
 }
 
 extern "C" int mom_set_solve_rows(int rows) {
@@ -2230,15 +2740,11 @@ extern "C" int64_t mom_gmres_work_elems(int m, int nrhs) {
   return (int64_t)(kOrthSplit + 2) * (m + 1) * nrhs + (kOrthSplit * nrhs + 1) / 2 + 1;
 }
 
-extern "C" int mom_gmres_arnoldi(int prec, int n, int nrhs, int j, int m, const void* LU, int64_t lda,
-                                 const int32_t* perm, void* solve_work, const void* Z, int64_t ldz, const double* dsc,
-                                 void* symv_work, int64_t symv_bytes, void* V, void* W, void* H, void* Hh,
-                                 void* stream) {
-  if (n <= 0) return 0;
-  if (nrhs < 1 || nrhs > 4) return fail("mom_gmres_arnoldi: 1 to 4 right-hand sides");
-  if (j < 0 || j >= m) return fail("mom_gmres_arnoldi: step outside the restart length");
-  if (!V || !W || !H || !Hh) return fail("mom_gmres_arnoldi: null pointer");
-  hipStream_t s = (hipStream_t)stream;
+// The step after its preconditioner solve W_j = M^-1 V_j: product, Gram-Schmidt
+// twice, normalisation, Hessenberg column to the host.
+static int gmres_step_tail(int n, int nrhs, int j, int m, const void* Z, int64_t ldz, const double* dsc,
+                           void* symv_work, int64_t symv_bytes, void* V, void* W, void* H, void* Hh, hipStream_t s) {
+  void* stream = (void*)s;
   const int64_t vec = (int64_t)n * nrhs;
   z64* Vj = (z64*)V + (int64_t)j * vec;
   z64* Vn = Vj + vec;  // w, then V_{j+1}
@@ -2248,7 +2754,6 @@ extern "C" int mom_gmres_arnoldi(int prec, int n, int nrhs, int j, int m, const 
   z64* part = T + (int64_t)(m + 1) * nrhs;      // kOrthSplit x (m + 1) x nrhs: partial dot products
   double* npart = reinterpret_cast<double*>(part + (int64_t)kOrthSplit * (m + 1) * nrhs);  // kOrthSplit x nrhs
   int rc;
-  if ((rc = mom_getrs_inv_z(prec, n, nrhs, LU, lda, perm, Vj, Wj, solve_work, stream))) return rc;
   if (symv_work)
     rc = mom_scaled_symv(n, nrhs, Z, ldz, dsc, Wj, Vn, symv_work, symv_bytes, stream);
   else
@@ -2272,4 +2777,35 @@ extern "C" int mom_gmres_arnoldi(int prec, int n, int nrhs, int j, int m, const 
   if ((rc = check_launch("col_scale_kernel"))) return rc;
   MOM_TRY(hipMemcpyAsync(Hh, Hc, sizeof(z64) * (size_t)(j + 2) * nrhs, hipMemcpyDeviceToHost, s));
   return 0;
+}
+
+extern "C" int mom_gmres_arnoldi(int prec, int n, int nrhs, int j, int m, const void* LU, int64_t lda,
+                                 const int32_t* perm, void* solve_work, const void* Z, int64_t ldz, const double* dsc,
+                                 void* symv_work, int64_t symv_bytes, void* V, void* W, void* H, void* Hh,
+                                 void* stream) {
+  if (n <= 0) return 0;
+  if (nrhs < 1 || nrhs > 4) return fail("mom_gmres_arnoldi: 1 to 4 right-hand sides");
+  if (j < 0 || j >= m) return fail("mom_gmres_arnoldi: step outside the restart length");
+  if (!V || !W || !H || !Hh) return fail("mom_gmres_arnoldi: null pointer");
+  const int64_t vec = (int64_t)n * nrhs;
+  z64* Vj = (z64*)V + (int64_t)j * vec;
+  z64* Wj = (z64*)W + (int64_t)j * vec;
+  int rc;
+  if ((rc = mom_getrs_inv_z(prec, n, nrhs, LU, lda, perm, Vj, Wj, solve_work, stream))) return rc;
+  return gmres_step_tail(n, nrhs, j, m, Z, ldz, dsc, symv_work, symv_bytes, V, W, H, Hh, (hipStream_t)stream);
+}
+
+extern "C" int mom_gmres_arnoldi_sb(int prec, int n, int nrhs, int j, int m, int sb, const int32_t* perm,
+                                    void* sb_work, const void* Z, int64_t ldz, const double* dsc, void* symv_work,
+                                    int64_t symv_bytes, void* V, void* W, void* H, void* Hh, void* stream) {
+  if (n <= 0) return 0;
+  if (nrhs < 1 || nrhs > 4) return fail("mom_gmres_arnoldi_sb: 1 to 4 right-hand sides");
+  if (j < 0 || j >= m) return fail("mom_gmres_arnoldi_sb: step outside the restart length");
+  if (!V || !W || !H || !Hh || !perm || !sb_work) return fail("mom_gmres_arnoldi_sb: null pointer");
+  const int64_t vec = (int64_t)n * nrhs;
+  z64* Vj = (z64*)V + (int64_t)j * vec;
+  z64* Wj = (z64*)W + (int64_t)j * vec;
+  int rc;
+  if ((rc = mom_sbsolve_z(prec, n, nrhs, sb, perm, Vj, Wj, sb_work, stream))) return rc;
+  return gmres_step_tail(n, nrhs, j, m, Z, ldz, dsc, symv_work, symv_bytes, V, W, H, Hh, (hipStream_t)stream);
 }

@@ -372,6 +372,40 @@ def givens_column(h, cs, sn, g, j):
         g[j, c] = cj.conjugate() * gj
 
 
+@dataclass
+class Precond:
+    """A factorisation that preconditions the mixed path's GMRES: the LU in
+    precision prec (32 or 64) of D Z D, its row permutation, and the solve
+    workspace -- with sb > 0 that of the super-block solve (mom_sbsolve_*:
+    inverses of the diagonal super-blocks and the scaled couplings), else
+    that of the 64-row chain with inverted blocks (mom_getrs_inv_z)."""
+
+    prec: int
+    A: object
+    perm: object
+    sw: object
+    sb: int = 0
+
+    def solve(self, n, nrhs, V, out, stream):
+        """out = M^-1 V (complex128 vectors), enqueued on stream."""
+        P = _lib.ptr
+        if self.sb:
+            _lib.call("mom_sbsolve_z", self.prec, n, nrhs, self.sb, P(self.perm), P(V), P(out), P(self.sw), stream)
+        else:
+            _lib.call("mom_getrs_inv_z", self.prec, n, nrhs, P(self.A), self.A.stride(0), P(self.perm), P(V), P(out),
+                      P(self.sw), stream)
+
+    def arnoldi(self, n, nrhs, j, m, Z, dv, swk, wbs, Vb, Wb, Hd, Hh, stream):
+        """One GMRES step (mom_gmres_arnoldi / _sb)."""
+        P = _lib.ptr
+        if self.sb:
+            _lib.call("mom_gmres_arnoldi_sb", self.prec, n, nrhs, j, m, self.sb, P(self.perm), P(self.sw), P(Z),
+                      Z.stride(0), P(dv), P(swk), wbs, P(Vb), P(Wb), P(Hd), P(Hh), stream)
+        else:
+            _lib.call("mom_gmres_arnoldi", self.prec, n, nrhs, j, m, P(self.A), self.A.stride(0), P(self.perm),
+                      P(self.sw), P(Z), Z.stride(0), P(dv), P(swk), wbs, P(Vb), P(Wb), P(Hd), P(Hh), stream)
+
+
 class Solver:
     """Solver(mesh, Substrate | Stackup).sweep(freqs) -> SweepResult.
 
@@ -409,6 +443,11 @@ class Solver:
                 precondition as well as pivoted ones (element growth ~2); a
                 solve whose GMRES has not converged after NOPIV_CAP steps
                 is factored again with pivoting.
+    solve_sb  : mixed precision: rows of the super-blocks of the
+                preconditioner's triangular solves (512: the chain of
+                super-blocks with precomputed inverses and scaled couplings,
+                mom_sbsolve_*; 0: the chain of 64-row blocks, mom_getrs_inv_z).
+                Systems of fewer than 2 super-blocks use the 64-row chain.
     method    : "auto" / "dense" (the H-matrix solver is not part of this build).
     surrogate : a trained GreensSurrogate of this stack: the remainders of
                 the Green's tables come from its networks, corrected and
@@ -438,7 +477,7 @@ class Solver:
     def __init__(self, mesh, substrate, z0=None, verbose=True, method="auto", v_weight=None, z0_recip_min=None,
                  tand_floor=None, precision="fp64", device="cuda", panel=None, refine_tol=1e-15, refine_maxit=20,
                  lanes=None, surrogate=None, surrogate_tol=2e-3, workers=None, graphs=False, autotune=False,
-                 gmres_restart=120, reuse_lu=True, lu_equiv=None, lu64_steps=20, nopiv=True):
+                 gmres_restart=120, reuse_lu=True, lu_equiv=None, lu64_steps=20, nopiv=True, solve_sb=512):
         from .layered import Stackup
 
         if method not in ("auto", "dense"):
@@ -506,8 +545,12 @@ class Solver:
         self.lu_equiv = lu_equiv
         self._anchor = {}
         self._history = {}
+        self._lane_pc = {}  # lane -> Precond of its last factorisation
+        # mixed path, per solved point: frequency, lane and the (preconditioner, GMRES steps) of its solves
+        self.point_log = []
         self.lu64_steps = lu64_steps
         self.nopiv = bool(nopiv)
+        self.solve_sb = int(solve_sb or 0)
         self._f_poor = 0.0
         self.lu_stats = dict(factorisations=0, reused=0, fallbacks=0, gmres_steps=0, fp64_lu=0, pivoted_retries=0,
                              abandoned=0)
@@ -572,7 +615,7 @@ class Solver:
             self._bufs[key] = b
         return b
 
-    SOLVE_WORKSPACES = ("swork32", "swork64", "swork64i")
+    SOLVE_WORKSPACES = ("swork32", "swork64", "swork64i", "sbwork32", "sbwork64")
 
     def _clear_status(self, lane=0):
         """Zero the failure words of the lane's solve workspaces (on the
@@ -790,10 +833,17 @@ class Solver:
             jb //= 2
         wbytes = int(_lib.lib().mom_getrf_work_bytes(prec, n, jb))
         work = self._buf(f"work{prec}", (wbytes,), torch.uint8, lane)
-        # the complex64 solves (GMRES preconditioner) use the inverted diagonal
-        # blocks of the factors, kept after the getrs workspace
-        swb = _lib.lib().mom_getrs_work_bytes(prec, n) if prec == 64 else _lib.lib().mom_getrs_inv_work_bytes(prec, n)
-        sw = self._buf(f"swork{prec}", (int(swb),), torch.uint8, lane, zero=True)
+        # the complex64 solves (GMRES preconditioner): the super-block solve's
+        # inverses and scaled couplings, or the 64-row chain's inverted blocks
+        # kept after the getrs workspace
+        sb = self._sb_rows(n) if prec == 32 else 0
+        if sb:
+            swb = _lib.lib().mom_sbsolve_work_bytes(prec, n, sb)
+            sw = self._buf(f"sbwork{prec}", (int(swb),), torch.uint8, lane, zero=True)
+        else:
+            swb = (_lib.lib().mom_getrs_work_bytes(prec, n) if prec == 64
+                   else _lib.lib().mom_getrs_inv_work_bytes(prec, n))
+            sw = self._buf(f"swork{prec}", (int(swb),), torch.uint8, lane, zero=True)
         X = self._buf("X", (n, nrhs), torch.complex128, lane)
 
         if self.tile is not None:
@@ -818,6 +868,8 @@ class Solver:
                         _lib.call("mom_getrs", 64, n, c1 - c0, P(A), A.stride(0), P(perm), P(Bc), P(Xc), P(sw),
                                   stream)
                         X[:, c0:c1] = Xc
+            elif sb:
+                _lib.call("mom_sbsolve_setup", 32, n, P(A), A.stride(0), sb, P(sw), int(swb), stream)
             else:
                 _lib.call("mom_trtri_blocks", 32, n, P(A), A.stride(0), P(sw), stream)
 
@@ -847,27 +899,34 @@ class Solver:
         if Z32 is None:
             self.last_refine = 0
             return X
+        pc = Precond(32, A, perm, sw, sb)
+        self._lane_pc[lane] = pc  # per lane: other lanes factor while this one's GMRES yields
         if cap is not None:  # the caller handles a solve that has not converged within cap steps
-            return (yield from self._gmres_chunks(Z, A, perm, sw, B, lane, max_its=cap))
-        X = yield from self._gmres_chunks(Z, A, perm, sw, B, lane, max_its=self.NOPIV_CAP if nopiv else None)
+            return (yield from self._gmres_chunks(Z, pc, B, lane, max_its=cap))
+        X = yield from self._gmres_chunks(Z, pc, B, lane, max_its=self.NOPIV_CAP if nopiv else None)
         if X is None:  # the unpivoted factors precondition too poorly here: factor again with pivoting
             self.lu_stats["pivoted_retries"] += 1
             self._scale_copy(Z, Z32, lane)
             X = yield from self._lu_solve(Z, B, Z32, lane, graph, pivot=True)
         return X
 
-    def _gmres_chunks(self, Z, A32, perm, sw, B, lane=0, max_its=None, X0=None, pc_prec=32):
+    def _sb_rows(self, n):
+        """Super-block rows of the preconditioner solves at this size (0:
+        the 64-row chain)."""
+        return self.solve_sb if self.solve_sb and n >= 2 * self.solve_sb else 0
+
+    def _gmres_chunks(self, Z, pc, B, lane=0, max_its=None, X0=None):
         """_gmres_ir over groups of at most 4 right-hand sides (the solve
         kernels' limit); None if a group does not converge within max_its.
         X0: an initial guess of the solution (or None)."""
         nrhs = B.shape[1]
         if nrhs <= 4:
-            return (yield from self._gmres_ir(Z, A32, perm, sw, B, lane, max_its, X0, pc_prec))
+            return (yield from self._gmres_ir(Z, pc, B, lane, max_its, X0))
         X = torch.empty_like(B)
         its = 0
         for c0, c1 in _column_chunks(nrhs):
-            Xc = yield from self._gmres_ir(Z, A32, perm, sw, B[:, c0:c1].contiguous(), lane, max_its,
-                                           None if X0 is None else X0[:, c0:c1].contiguous(), pc_prec)
+            Xc = yield from self._gmres_ir(Z, pc, B[:, c0:c1].contiguous(), lane, max_its,
+                                           None if X0 is None else X0[:, c0:c1].contiguous())
             its += self.last_refine
             if Xc is None:
                 self.last_refine = its
@@ -905,7 +964,7 @@ class Solver:
         cus = torch.cuda.get_device_properties(self.device).multi_processor_count
         return max(1, cus // lanes)
 
-    def _gmres_ir(self, Z, A32, perm, sw, B, lane=0, max_its=None, X0=None, pc_prec=32):
+    def _gmres_ir(self, Z, pc, B, lane=0, max_its=None, X0=None):
         """Restarted GMRES in FP64 on the diagonally scaled system
         (D Z D) y = D b, x = D y, right-preconditioned by the complex64 LU of
         D Z D.  All right-hand sides advance together (one Krylov basis per
@@ -916,8 +975,8 @@ class Solver:
         returns None once that many steps have not reached the target (the
         caller then factors this matrix itself).  X0: an initial guess of x;
         the iteration then starts from y0 + M (D b - D Z D y0), y0 = D^-1 X0,
-        instead of M D b.  pc_prec: 64 when A32 holds a complex128 LU of
-        D Z D instead (with its inverted diagonal blocks in sw)."""
+        instead of M D b.  pc: the preconditioning factorisation (Precond:
+        complex64, or complex128 for the anchors of the low-frequency end)."""
         n, nrhs = B.shape
         P = _lib.ptr
         stream = _lib.stream_ptr()
@@ -929,8 +988,7 @@ class Solver:
 
         def precond(V, out=None):  # complex64 LU solve of complex128 vectors
             out = torch.empty_like(V) if out is None else out
-            _lib.call("mom_getrs_inv_z", pc_prec, n, nrhs, P(A32), A32.stride(0), P(perm), P(V), P(out), P(sw),
-                      stream)
+            pc.solve(n, nrhs, V, out, stream)
             return out
 
         if self.symmetric_products:  # Z = Z^T (the fill symmetrises it): half the HBM traffic
@@ -986,9 +1044,8 @@ class Solver:
             for j in range(m):
                 # one call: preconditioner solve, product, Gram-Schmidt twice, normalisation, and
                 # the new Hessenberg column copied to pinned host memory
-                _lib.call("mom_gmres_arnoldi", pc_prec, n, nrhs, j, m, P(A32), A32.stride(0), P(perm), P(sw), P(Z),
-                          Z.stride(0), P(dv), P(swk) if self.symmetric_products else None,
-                          wbs if self.symmetric_products else 0, P(Vb), P(Wb), P(Hd), P(Hh), stream)
+                pc.arnoldi(n, nrhs, j, m, Z, dv, swk if self.symmetric_products else None,
+                           wbs if self.symmetric_products else 0, Vb, Wb, Hd, Hh, stream)
                 ev.record()
                 while not ev.query():
                     yield
@@ -1126,19 +1183,25 @@ class Solver:
         sweep = mixed and reuse and self.reuse_lu
         a = self._anchor.get(lane) if sweep else None
         filled = False
+        steps = []  # (preconditioner, GMRES steps) of this point, for point_log
+
+        def log(X):
+            self.point_log.append(dict(freq=float(g.freq), lane=lane, solves=steps))
+            return X
+
         if a is not None and self._reuse_next(a, g.freq):
             fill(None)
             filled = True
             cap = max(20, int(a["lu_equiv"] + a["its0"]))
-            X = yield from self._gmres_chunks(Z, a["A"], a["perm"], a["sw"], B, lane, max_its=cap,
-                                              X0=self._extrapolate(lane, g.freq, B), pc_prec=a["prec"])
+            X = yield from self._gmres_chunks(Z, a["pc"], B, lane, max_its=cap, X0=self._extrapolate(lane, g.freq, B))
+            steps.append((f"reuse{a['prec']}@{a['freq'] / 1e9:.3g}", self.last_refine))
             if X is not None:
                 self._remember(lane, g.freq, X)
                 a["cost"] += self.last_refine
                 a["n"] += 1
                 a["last"] = self.last_refine
                 self.lu_stats["reused"] += 1
-                return X
+                return log(X)
             self.lu_stats["fallbacks"] += 1
         wide = self.lu64_steps is not None and sweep
         if not (wide and g.freq <= 2 * self._f_poor):
@@ -1150,44 +1213,46 @@ class Solver:
             # in a sweep, a complex64 solve still short of the target after
             # lu64_steps steps is abandoned for a complex128 anchor
             X = yield from self._lu_solve(Z, B, Z32, lane, graph, cap=self.lu64_steps if wide else None)
+            if mixed:
+                steps.append(("lu32", self.last_refine))
             if X is not None:
                 if mixed:
                     self.lu_stats["factorisations"] += 1
                     if sweep:
                         self._remember(lane, g.freq, X)
-                        self._set_anchor(lane, g.freq, 32, Z32, self._bufs[("perm", lane)],
-                                         self._bufs[("swork32", lane)], self.last_refine)
-                return X
+                        self._set_anchor(lane, g.freq, self._lane_pc[lane], self.last_refine)
+                return log(X)
             self.lu_stats["abandoned"] += 1
             self._f_poor = max(self._f_poor, g.freq)
         # a band where complex64 factors precondition poorly: complex128 anchor
         if not filled:
             fill(None)
-        A, perm, sw = self._lu64_factor(Z, lane)
-        X = yield from self._gmres_chunks(Z, A, perm, sw, B, lane, pc_prec=64,
-                                          max_its=self.NOPIV_CAP if self.nopiv else None)
+        pc = self._lu64_factor(Z, lane)
+        X = yield from self._gmres_chunks(Z, pc, B, lane, max_its=self.NOPIV_CAP if self.nopiv else None)
+        steps.append(("lu64", self.last_refine))
         if X is None:  # unpivoted factors too poor here: factor again with pivoting
             self.lu_stats["pivoted_retries"] += 1
-            A, perm, sw = self._lu64_factor(Z, lane, pivot=True)
-            X = yield from self._gmres_chunks(Z, A, perm, sw, B, lane, pc_prec=64)
+            pc = self._lu64_factor(Z, lane, pivot=True)
+            X = yield from self._gmres_chunks(Z, pc, B, lane)
+            steps.append(("lu64-pivoted", self.last_refine))
         self._remember(lane, g.freq, X)
         self.lu_stats["factorisations"] += 1
-        self._set_anchor(lane, g.freq, 64, A, perm, sw)
-        return X
+        self._set_anchor(lane, g.freq, pc)
+        return log(X)
 
-    def _set_anchor(self, lane, freq, prec, A, perm, sw, its0=0):
-        """The lane's factorisation that preconditions its next points."""
+    def _set_anchor(self, lane, freq, pc, its0=0):
+        """The lane's factorisation (a Precond) that preconditions its next points."""
         eq = self._lu_equiv()
-        self._anchor[lane] = dict(freq=freq, prec=prec, A=A, perm=perm, sw=sw, lu_equiv=eq, its0=its0,
-                                  cost=eq + its0, n=1, last=None)
+        self._anchor[lane] = dict(freq=freq, prec=pc.prec, pc=pc, lu_equiv=eq, its0=its0, cost=eq + its0, n=1,
+                                  last=None)
 
     def _lu64_factor(self, Z, lane, pivot=False):
         """complex128 LU of D Z D (D = diag(|Z_ii|^-1/2), into the lane's
-        dscale) with the inverted diagonal blocks of its factors, enqueued on
-        the current stream; returns (LU, perm, solve workspace) for the
-        preconditioner of the mixed path's GMRES (pc_prec = 64).  Z must be
-        symmetric (as the fill leaves it).  Without row interchanges when
-        nopiv is set and pivot is False."""
+        dscale) with the solve data of its factors (super-block inverses
+        and scaled couplings, or the 64-row inverted blocks), enqueued on the
+        current stream; returns the Precond of the mixed path's GMRES.  Z
+        must be symmetric (as the fill leaves it).  Without row interchanges
+        when nopiv is set and pivot is False."""
         n = Z.shape[0]
         P = _lib.ptr
         stream = _lib.stream_ptr()
@@ -1207,15 +1272,23 @@ class Solver:
             jb //= 2
         wbytes = int(L.mom_getrf_work_bytes(64, n, jb))
         work = self._buf("work64", (wbytes,), torch.uint8, lane)
-        sw = self._buf("swork64i", (int(L.mom_getrs_inv_work_bytes(64, n)),), torch.uint8, lane, zero=True)
+        sb = self._sb_rows(n)
+        if sb:
+            swb = int(L.mom_sbsolve_work_bytes(64, n, sb))
+            sw = self._buf("sbwork64", (swb,), torch.uint8, lane, zero=True)
+        else:
+            sw = self._buf("swork64i", (int(L.mom_getrs_inv_work_bytes(64, n)),), torch.uint8, lane, zero=True)
         if nopiv:
             _lib.call("mom_getrf_nopiv", 64, n, P(A), A.stride(0), P(ipiv), P(info), P(work), wbytes, jb, stream)
         else:
             _lib.call("mom_getrf", 64, n, P(A), A.stride(0), P(ipiv), P(info), P(work), wbytes, jb, groups, stream)
         _lib.call("mom_build_perm", n, P(ipiv), P(perm), stream)
-        _lib.call("mom_trtri_blocks", 64, n, P(A), A.stride(0), P(sw), stream)
+        if sb:
+            _lib.call("mom_sbsolve_setup", 64, n, P(A), A.stride(0), sb, P(sw), swb, stream)
+        else:
+            _lib.call("mom_trtri_blocks", 64, n, P(A), A.stride(0), P(sw), stream)
         self.lu_stats["fp64_lu"] += 1
-        return A, perm, sw
+        return Precond(64, A, perm, sw, sb)
 
     def _remember(self, lane, freq, X):
         """Keep the lane's last three solutions (device) for _extrapolate."""
@@ -1250,10 +1323,13 @@ class Solver:
         """Cost of one complex64 factorisation in GMRES steps at this N: a
         model of this implementation on MI355X -- the LU at ~50 TFLOP/s
         effective or ~2.5 us per column of panel latency, whichever is larger
-        (unpivoted, nopiv; pivoted: ~32 TFLOP/s or ~8 us per column); a step =
-        two triangular block chains of ~2.9 us per 64 rows + one FP64 read
-        of Z (of its upper triangle with symmetric_products) + ~0.2 ms of
-        host round trip -- times 1.4, because the steps of
+        (unpivoted, nopiv; pivoted: ~32 TFLOP/s or ~8 us per column), plus
+        the super-block solve's set-up (8 n^2 sb flop of scaled couplings); a
+        step = two triangular solves (64-row chain: ~2.9 us per 64 rows;
+        super-block chain: the larger of ~4.5 us per super-block and one
+        read of the factors at ~3 TB/s) + one FP64 read of Z (of its upper
+        triangle with symmetric_products) + ~0.2 ms of host round trip --
+        times 1.4, because the steps of
         concurrent lanes overlap better than factorisations do (measured on
         the 12.4 k-unknown pour, unpivoted: 17.3 points/s at the model's 78,
         16.9 at 123, 14.6 at 50)."""
@@ -1262,7 +1338,13 @@ class Solver:
         n = self.mesh.n_unknowns
         rate, per_col = (50e12, 2.5e-6) if self.nopiv else (32e12, 8e-6)
         t_lu = max(8 / 3 * n ** 3 / rate, per_col * n)
-        t_step = 2 * (n / 64) * 2.9e-6 + (8 if self.symmetric_products else 16) * n * n / 5e12 + 2e-4
+        sb = self._sb_rows(n)
+        if sb:
+            t_lu += 8 * n * n * sb / rate
+            t_solve = 2 * max(n / sb * 4.5e-6, 4 * n * n / 3e12)
+        else:
+            t_solve = 2 * (n / 64) * 2.9e-6
+        t_step = t_solve + (8 if self.symmetric_products else 16) * n * n / 5e12 + 2e-4
         return 1.4 * t_lu / t_step
 
     @staticmethod

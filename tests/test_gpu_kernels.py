@@ -313,6 +313,52 @@ def test_getrs_with_inverted_blocks(prec, n, nrhs, rows):
         _lib.call("mom_set_solve_rows", 64)
 
 
+@pytest.mark.parametrize("prec,n,nrhs,pivot", [(32, 1, 1, False), (32, 700, 2, False), (32, 1024, 3, False),
+                                               (32, 1025, 4, False), (32, 3001, 2, False), (64, 1500, 2, False),
+                                               (64, 2048, 1, True), (32, 2600, 2, True), (64, 513, 4, False)])
+def test_superblock_solve(prec, n, nrhs, pivot):
+    """The super-block solve (inverses of the 512-row diagonal blocks of L
+    and U, couplings scaled by them, one chain of super-blocks per factor):
+    backward error near the factor precision's round-off on a diagonally
+    dominant matrix, the solution of the 64-row chain up to cond(block)
+    rounding, partial last super-blocks, pivoted and unpivoted factors, and
+    a second solve with the same set-up."""
+    rng = np.random.default_rng(n + 10 * nrhs)
+    dt = torch.complex128 if prec == 64 else torch.complex64
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + 4 * np.sqrt(n) * np.eye(n)
+    Ad = torch.from_numpy(A).to(DEV, dt)
+    ipiv = torch.empty(n, dtype=torch.int32, device=DEV)
+    perm = torch.empty(n, dtype=torch.int32, device=DEV)
+    info = torch.zeros(4, dtype=torch.int32, device=DEV)
+    wb = int(_lib.lib().mom_getrf_work_bytes(prec, n, 64))
+    work = torch.empty(wb, dtype=torch.uint8, device=DEV)
+    s = _lib.stream_ptr()
+    P = _lib.ptr
+    if pivot:
+        _lib.call("mom_getrf", prec, n, P(Ad), n, P(ipiv), P(info), P(work), wb, 64, 0, s)
+    else:
+        _lib.call("mom_getrf_nopiv", prec, n, P(Ad), n, P(ipiv), P(info), P(work), wb, 64, s)
+    _lib.call("mom_build_perm", n, P(ipiv), P(perm), s)
+    sbb = int(_lib.lib().mom_sbsolve_work_bytes(prec, n, 512))
+    sbw = torch.zeros(sbb, dtype=torch.uint8, device=DEV)
+    _lib.call("mom_sbsolve_setup", prec, n, P(Ad), n, 512, P(sbw), sbb, s)
+    inv = torch.zeros(int(_lib.lib().mom_getrs_inv_work_bytes(prec, n)), dtype=torch.uint8, device=DEV)
+    _lib.call("mom_trtri_blocks", prec, n, P(Ad), n, P(inv), s)
+    for rep in range(2):
+        B = rng.standard_normal((n, nrhs)) + 1j * rng.standard_normal((n, nrhs))
+        Bd = torch.from_numpy(B).to(DEV)
+        X = torch.full_like(Bd, float("nan"))
+        _lib.call("mom_sbsolve_z", prec, n, nrhs, 512, P(perm), P(Bd), P(X), P(sbw), s)
+        X64 = torch.empty_like(Bd)
+        _lib.call("mom_getrs_inv_z", prec, n, nrhs, P(Ad), n, P(perm), P(Bd), P(X64), P(inv), s)
+        got, ref = X.cpu().numpy(), X64.cpu().numpy()
+        assert int(sbw[16:20].view(torch.int32).cpu()[0]) == 0
+        assert np.all(np.isfinite(got))
+        r = np.linalg.norm(A @ got - B) / (np.linalg.norm(A, 2) * np.linalg.norm(got))
+        assert r < (1e-6 if prec == 32 else 1e-14), (rep, r)
+        assert np.max(np.abs(got - ref)) < (2e-5 if prec == 32 else 1e-12) * np.max(np.abs(ref)), rep
+
+
 @pytest.mark.parametrize("prec,n,nrhs", [(32, 2049, 2), (64, 777, 3), (32, 1000, 8), (32, 65, 1), (64, 130, 6)])
 def test_getrs_inv_chain_and_plain_forms(prec, n, nrhs):
     """Both forms of the 64-row inverted-block solve -- the chain form

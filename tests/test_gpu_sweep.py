@@ -70,6 +70,28 @@ def test_mixed_sweep_symmetric_and_full_products_agree():
         assert np.abs(a - b).max() < 1e-9 * np.abs(b).max()
 
 
+def test_superblock_and_64_row_preconditioner_solves_agree():
+    """The mixed path with the super-block preconditioner solve (default)
+    and with the 64-row chain reaches the same FP64 tolerance: the two
+    preconditioners differ at complex64 round-off, so the GMRES iterates
+    differ too, and the currents agree to cond(D Z D) (~1e7 here) times the
+    normwise backward error (1e-15) -- bar 1e-7 -- and S to the wave fit's
+    noise, on the via transition (5 289 unknowns, complex64 anchors) and
+    with complex128 anchors forced (lu64_steps=0)."""
+    lay, st, mk, _ = W.via2()
+    m = lay.mesh(**mk)
+    f = [1.5e9, 2e9, 9e9, 14e9]
+    for lu64 in (20, 0):
+        out = []
+        for sb in (512, 0):
+            s = Solver(m, st, verbose=False, precision="mixed", solve_sb=sb, lu64_steps=lu64, lanes=2)
+            out.append(s.sweep(f, interp=None, keep_currents=True))
+            print(f"solve_sb={sb} lu64_steps={lu64}: {s.lu_stats}")
+        for a, b in zip(out[0].currents, out[1].currents):
+            assert np.abs(a - b).max() < 1e-7 * np.abs(b).max()
+        assert np.abs(out[0].S - out[1].S).max() < 1e-7
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_given_matrix_symmetric_or_not(precision):
     """solve_currents(Z=...) solves the matrix it is given: a symmetric one
