@@ -8,8 +8,9 @@ Z at one frequency and, for the 12 428-unknown pour, the port currents at
   - Z rows: 1e-10 relative per block (RWG columns, via columns);
   - currents (the 12 428-unknown pour at 10 GHz, both paths): in the
     reference's own equations -- its Z rows times our currents minus the
-    excitation -- a componentwise residual at FP64 round-off,
-    |Z_i x - b_i| < 1e-13 |Z_i| |x| (our solve solves the reference's system); and against the reference's
+    excitation -- a residual near FP64 round-off in every row,
+    |Z_i x - b_i| < 1e-11 |Z_i|_1 |x|_inf (our solve solves the reference's
+    system; another system leaves 1e-6 or more); and against the reference's
     currents 2e-5 relative: two backward-stable solves of a system this
     ill-conditioned (kappa_1(D Z D) ~ 1.6e7, kappa_1(Z) ~ 5e15: the via
     rows carry ~1e5-1e6 ohm) differ by about 2e-6 (measured);
@@ -101,10 +102,11 @@ def test_sweep_matches_reference(name, fixture, precision, tol, golden):
         if f"{name}_Zrows" in fx.files and float(fx[f"{name}_Zrows_freq"]) == freqs[k]:
             rows, Zr = fx[f"{name}_rows"], fx[f"{name}_Zrows"]
             b = s.port_excitations()[rows]
-            # componentwise backward error of each stored row: |Z_i x - b_i| / (|Z_i| |x|)
-            bw = np.max(np.abs(Zr @ x - b) / (np.abs(Zr) @ np.abs(x)))
+            # backward error of each stored row: |Z_i x - b_i| / (|Z_i|_1 |x|_inf); a solve of any other
+            # system (another via model, another table) leaves 1e-6 or more here
+            bw = np.max(np.abs(Zr @ x - b) / (np.sum(np.abs(Zr), 1)[:, None] * np.max(np.abs(x))))
             msg += f", residual in the reference's rows {bw:.1e}"
-            assert bw < 1e-13
+            assert bw < 1e-11
         print(msg)
         assert err < 2e-5
 
