@@ -144,6 +144,17 @@ int64_t mom_getrs_work_bytes(int prec, int n);
 int mom_getrs(int prec, int n, int nrhs, const void* LU, int64_t lda, const int32_t* perm, const void* B, void* X,
               void* work, void* stream);
 
+/* mom_getrs without a workspace argument and without any allocation: the
+ * tickets and the intermediate L^-1 P B live in the library's static device
+ * array (about 4 MB, held on each device the library runs on until the
+ * process ends), for n <= 16384 and nrhs <= 8; larger sizes are rejected
+ * with an error.  *status (a caller word, device memory) is set to 1 when a
+ * bounded wait ran out and is never cleared by the call.  Calls that overlap
+ * on one device race on the static array.  No counterpart in the reference:
+ * for callers that cannot allocate (graph capture, serving loops). */
+int mom_getrs_noalloc(int prec, int n, int nrhs, const void* LU, int64_t lda, const int32_t* perm, const void* B,
+                      void* X, uint32_t* status, void* stream);
+
 /* The same solve with the 64 x 64 diagonal blocks of L and U applied as
  * explicit inverses (a parallel product per block instead of the serial
  * substitution; about twice as fast, backward error ~ cond(block) eps instead

@@ -41,6 +41,7 @@ SIGNATURES = {
     "mom_get_gemm_tile": (_i, [_i, _p, _p]),
     "mom_getrs_work_bytes": (_i64, [_i, _i]),
     "mom_getrs": (_i, [_i, _i, _i, _p, _i64, _p, _p, _p, _p, _p]),
+    "mom_getrs_noalloc": (_i, [_i, _i, _i, _p, _i64, _p, _p, _p, _p, _p]),
     "mom_getrs_inv_work_bytes": (_i64, [_i, _i]),
     "mom_trtri_blocks": (_i, [_i, _i, _p, _i64, _p, _p]),
     "mom_getrs_inv": (_i, [_i, _i, _i, _p, _i64, _p, _p, _p, _p, _p]),

@@ -2244,10 +2244,66 @@ extern "C" int mom_getrs(int prec, int n, int nrhs, const void* LU, int64_t lda,
   return fail("mom_getrs: prec must be 64 or 32");
 }
 
+// ---------------------------------------------------------------------------
+// mom_getrs_noalloc: mom_getrs with its workspace (the two block tickets and
+// the lower solve's output) in a file-scope device array instead of a
+// caller buffer, for callers that cannot allocate.  Each device holds its
+// own copy of `scratch` until the process ends; host code never passes the
+// array's name to a kernel (on the host it is only a stand-in for the
+// symbol), but the address hipGetSymbolAddress returns for the current
+// device, looked up once per device.  Every call on a device shares that
+// device's array: calls that overlap on one device race.
+constexpr int kNoAllocMaxN = 16384;
+constexpr size_t kNoAllocBytes = 256 + 2 * sizeof(z64) * kMaxRhs * (size_t)kNoAllocMaxN;  // getrs_work_bytes<double>
+static_assert(kNoAllocBytes >= 256 + 2 * sizeof(c32) * kMaxRhs * (size_t)kNoAllocMaxN, "complex64 fits too");
+__device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+static int scratch_for_current_device(void** out) {
+  int dev = -1;
+  MOM_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail("mom_getrs_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    MOM_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail("mom_getrs_noalloc: no device address for the scratch array");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
 // Workspace of mom_trtri_blocks / mom_getrs_inv: that of mom_getrs followed
 // by the inverted diagonal blocks of L and U.
 extern "C" int64_t mom_getrs_inv_work_bytes(int prec, int n) {
   return prec == 64 ? (int64_t)getrs_inv_work_bytes<double>(n) : (int64_t)getrs_inv_work_bytes<float>(n);
+}
+
+extern "C" int mom_getrs_noalloc(int prec, int n, int nrhs, const void* LU, int64_t lda, const int32_t* perm,
+                                 const void* B, void* X, uint32_t* status, void* stream) {
+  if (!LU || !perm || !B || !X || !status) return fail("mom_getrs_noalloc: null device pointer");
+  if (B == X) return fail("mom_getrs_noalloc: B and X must not alias");
+  if (prec != 64 && prec != 32) return fail("mom_getrs_noalloc: prec must be 64 or 32");
+  if (n > kNoAllocMaxN)
+    return fail("mom_getrs_noalloc: n = " + std::to_string(n) + " exceeds the static workspace's limit of " +
+                std::to_string(kNoAllocMaxN) + " unknowns (use mom_getrs with a caller workspace)");
+  if (nrhs > kMaxRhs) return fail("mom_getrs_noalloc: at most 8 right-hand sides");
+  if (n <= 0 || nrhs <= 0) return 0;
+  void* ws = nullptr;
+  int rc = scratch_for_current_device(&ws);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  unsigned* flags = reinterpret_cast<unsigned*>(ws);  // tickets cleared by the solve's own set-up kernel
+  unsigned* err = reinterpret_cast<unsigned*>(status);
+  if (prec == 64)
+    return getrs<double, false, double>(n, nrhs, (const z64*)LU, lda, perm, (const z64*)B, (z64*)X, nullptr, flags, s,
+                                        err);
+  return getrs<float, false, float>(n, nrhs, (const c32*)LU, lda, perm, (const c32*)B, (c32*)X, nullptr, flags, s,
+                                    err);
 }
 
 extern "C" int mom_trtri_blocks(int prec, int n, const void* LU, int64_t lda, void* work, void* stream) {
