@@ -107,6 +107,19 @@ int mom_getrf(int prec, int n, void* A, int64_t lda, int32_t* ipiv, int32_t* inf
  * runs to completion by itself. */
 int mom_getrf_nopiv(int prec, int n, void* A, int64_t lda, int32_t* ipiv, int32_t* info, void* work,
                     int64_t work_bytes, int jb, void* stream);
+/* The same factorisation of a complex-symmetric matrix (A = A^T, as the
+ * diagonally scaled MoM matrix D Z D is: Galerkin testing with the basis
+ * itself), A = L D L^T without pivoting.  Only the lower triangle of A is
+ * read; on return the strict lower triangle holds L (unit diagonal) and the
+ * upper triangle U = D L^T, so that the factors are laid out exactly as
+ * mom_getrf_nopiv's and every solve entry point takes them unchanged.  Each
+ * update block's row of U is the transposed, pivot-scaled column of L
+ * already computed (no triangular solve), and the trailing updates skip the
+ * tiles strictly above the diagonal (which the later blocks re-derive from
+ * L): about half the update flops of mom_getrf_nopiv.  ipiv[k] = k; zero
+ * pivots as mom_getrf_nopiv.  No counterpart in the reference. */
+int mom_sytrf_nopiv(int prec, int n, void* A, int64_t lda, int32_t* ipiv, int32_t* info, void* work,
+                    int64_t work_bytes, int jb, void* stream);
 
 /* Workgroup tile (rows x real columns) of the trailing-update GEMM of
  * mom_getrf and mom_cgemm_sub, per precision: 64x64, 64x128, 128x64 or
@@ -222,6 +235,34 @@ int64_t mom_gmres_work_elems(int m, int nrhs);
 int mom_gmres_arnoldi(int prec, int n, int nrhs, int j, int m, const void* LU, int64_t lda, const int32_t* perm,
                       void* solve_work, const void* Z, int64_t ldz, const double* dsc, void* symv_work,
                       int64_t symv_bytes, void* V, void* W, void* H, void* Hh, void* stream);
+
+/* Super-block triangular solves of unpivoted factors (the mixed path's
+ * preconditioner M = L U of D Z D).  With the factors cut into super-blocks
+ * of sb = 512 rows, D_k the inverse of the k-th diagonal super-block of L
+ * (of U) and T_kj = -D_k L_kj (j < k), -D_k U_kj (j > k), the solves read
+ * x_k = D_k b_k + sum over the super-blocks solved before k of T_kj x_j:
+ * each row is an independent dot product, and only the term of the
+ * super-block solved just before sits on the chain's critical path (n / sb
+ * hops instead of mom_getrs's n / 64).
+ * mom_sbsolve_work_bytes(prec, n, sb): workspace bytes (-1 unless sb = 512;
+ * about 2 n^2 elements of prec for D and T plus the vectors).
+ * mom_sbsolve_setup: D and T from the factors LU (lda) into work, once per
+ * factorisation (inverting the diagonal super-blocks in FP64).
+ * mom_sbsolve_z: X = M^-1 P B for complex128 B, X (n x nrhs, nrhs <= 4,
+ * B != X), perm as mom_getrs's; the workspace's status words hold the
+ * workgroup tickets and a sticky failure word (word 4, set by a bounded wait
+ * that gave up or a ticket outside the grid, cleared only by the caller).
+ * mom_gmres_arnoldi_sb: mom_gmres_arnoldi with this solve in place of
+ * mom_getrs_inv_z.  No counterpart in the reference, which solves in FP64
+ * directly (Engine/mom25d/solver.py:11-31). */
+int64_t mom_sbsolve_work_bytes(int prec, int n, int sb);
+int mom_sbsolve_setup(int prec, int n, const void* LU, int64_t lda, int sb, void* work, int64_t work_bytes,
+                      void* stream);
+int mom_sbsolve_z(int prec, int n, int nrhs, int sb, const int32_t* perm, const void* B, void* X, void* work,
+                  void* stream);
+int mom_gmres_arnoldi_sb(int prec, int n, int nrhs, int j, int m, int sb, const int32_t* perm, void* sb_work,
+                         const void* Z, int64_t ldz, const double* dsc, void* symv_work, int64_t symv_bytes, void* V,
+                         void* W, void* H, void* Hh, void* stream);
 
 /* Frequency interpolation of Z (Engine/mom25d/freqinterp.py:86-139).
  * mom_interp_anchor: W = j w Z exp(+j ke R) (N x N, W stored in prec_w = 64 or 32

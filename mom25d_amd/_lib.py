@@ -33,6 +33,7 @@ SIGNATURES = {
     "mom_getrf_work_bytes": (_i64, [_i, _i, _i]),
     "mom_getrf": (_i, [_i, _i, _p, _i64, _p, _p, _p, _i64, _i, _i, _p]),
     "mom_getrf_nopiv": (_i, [_i, _i, _p, _i64, _p, _p, _p, _i64, _i, _p]),
+    "mom_sytrf_nopiv": (_i, [_i, _i, _p, _i64, _p, _p, _p, _i64, _i, _p]),
     "mom_build_perm": (_i, [_i, _p, _p, _p]),
     "mom_set_gemm_tile": (_i, [_i, _i, _i]),
     "mom_set_lu_block": (_i, [_i, _i]),

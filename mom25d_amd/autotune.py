@@ -63,16 +63,19 @@ def get_tile(prec):
     return bm.value, bn.value
 
 
-def time_getrf(n, prec, jb, tile, device="cuda", reps=2, seed=0, nb=0, pivot=True):
+def time_getrf(n, prec, jb, tile, device="cuda", reps=2, seed=0, nb=0, pivot=True, symmetric=False):
     """Median milliseconds of mom_getrf (n, prec, panel jb, block nb, tile)
     on a synthetic matrix; pivot=False: mom_getrf_nopiv (the mixed path's
-    preconditioner factorisation)."""
+    preconditioner factorisation); symmetric: mom_sytrf_nopiv on a
+    complex-symmetric matrix."""
     dev = torch.device(device)
     dt = torch.complex128 if prec == 64 else torch.complex64
     g = torch.Generator(device=dev)
     g.manual_seed(seed)
     A0 = torch.complex(torch.randn(n, n, device=dev, generator=g, dtype=torch.float64),
                        torch.randn(n, n, device=dev, generator=g, dtype=torch.float64)).to(dt)
+    if symmetric:
+        A0 = A0 + A0.T
     A0.diagonal().add_(float(np.sqrt(n)) * 4)
     A = torch.empty_like(A0)
     ipiv = torch.empty(n, dtype=torch.int32, device=dev)
@@ -93,7 +96,8 @@ def time_getrf(n, prec, jb, tile, device="cuda", reps=2, seed=0, nb=0, pivot=Tru
             if pivot:
                 _lib.call("mom_getrf", prec, n, P(A), n, P(ipiv), P(info), P(work), wb, jb, 0, s)
             else:
-                _lib.call("mom_getrf_nopiv", prec, n, P(A), n, P(ipiv), P(info), P(work), wb, jb, s)
+                _lib.call("mom_sytrf_nopiv" if symmetric else "mom_getrf_nopiv", prec, n, P(A), n, P(ipiv), P(info),
+                          P(work), wb, jb, s)
             e1.record()
             torch.cuda.synchronize(dev)
             times.append(e0.elapsed_time(e1))
@@ -103,13 +107,14 @@ def time_getrf(n, prec, jb, tile, device="cuda", reps=2, seed=0, nb=0, pivot=Tru
     return float(np.median(times[1:]))
 
 
-def tune_lu(n, prec, device="cuda", panels=PANELS, blocks=BLOCKS, tiles=TILES, reps=2, verbose=False, pivot=True):
+def tune_lu(n, prec, device="cuda", panels=PANELS, blocks=BLOCKS, tiles=TILES, reps=2, verbose=False, pivot=True,
+            symmetric=False):
     """The fastest (panel width, block width, tile) for an n x n
     factorisation in prec (64: complex128, 32: complex64), with row
-    interchanges (mom_getrf) or without (pivot=False, mom_getrf_nopiv):
-    dict(jb, nb, tile, ms, table)."""
+    interchanges (mom_getrf) or without (pivot=False, mom_getrf_nopiv; and
+    symmetric: mom_sytrf_nopiv): dict(jb, nb, tile, ms, table)."""
     _load()
-    key = _key(prec, n) + ("" if pivot else ":nopiv")
+    key = _key(prec, n) + ("" if pivot else ":sym" if symmetric else ":nopiv")
     if key in _CACHE:
         return _CACHE[key]
     m = -(-n // 512) * 512
@@ -122,7 +127,7 @@ def tune_lu(n, prec, device="cuda", panels=PANELS, blocks=BLOCKS, tiles=TILES, r
                 continue
             for tile in tiles:
                 try:
-                    ms = time_getrf(m, prec, jb, tile, device, reps, nb=nb, pivot=pivot)
+                    ms = time_getrf(m, prec, jb, tile, device, reps, nb=nb, pivot=pivot, symmetric=symmetric)
                 except RuntimeError as e:  # e.g. a panel too large for the LDS at this width
                     if verbose:
                         print(f"  autotune n={m} prec={prec} jb={jb} nb={nb} tile={tile}: {e}", flush=True)

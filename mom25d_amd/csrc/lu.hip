@@ -335,14 +335,24 @@ __device__ inline cplx<R> nonzero_pivot(cplx<R> p) {
 // scaled by 1 / u_jj in step j + 1, when no one reads it any more.
 template <typename R>
 __global__ __launch_bounds__(1024) void diag_lu_kernel(cplx<R>* A, int64_t lda, int k0, int b, int32_t* ipiv,
-                                                       int32_t* info) {
+                                                       int32_t* info, int sym) {
   // the block in LDS; thread (c, q) updates column c of the rows i = q mod 8,
-  // 8 rows at a time (their loads issued together)
+  // 8 rows at a time (their loads issued together).  sym: the block of a
+  // complex-symmetric matrix whose updates reached only its lower triangle;
+  // the upper one is taken from it.
   extern __shared__ unsigned char smem_raw[];
   cplx<R>* T = reinterpret_cast<cplx<R>*>(smem_raw);  // [b][b]
   const int tid = threadIdx.x, c = tid % 128;
   const int q = __builtin_amdgcn_readfirstlane(tid / 128);  // uniform per wavefront
-  for (int e = tid; e < b * b; e += 1024) T[e] = A[(int64_t)(k0 + e / b) * lda + k0 + e % b];
+  for (int e = tid; e < b * b; e += 1024) {
+    int i = e / b, j = e % b;
+    if (sym && j > i) {
+      const int t = i;
+      i = j;
+      j = t;
+    }
+    T[e] = A[(int64_t)(k0 + i) * lda + k0 + j];
+  }
   for (int e = tid; e < b; e += 1024) ipiv[k0 + e] = k0 + e;
   __syncthreads();
   for (int j = 0; j < b; ++j) {
@@ -520,6 +530,29 @@ __global__ __launch_bounds__(256) void trsm_lower_unit_kernel(cplx<R>* A, int64_
     if (col < c1) A[(int64_t)(k0 + r) * lda + col] = Bs[r][tx];
 }
 
+// The block row U12 of a complex-symmetric matrix factored without
+// interchanges (A = L D L^T, stored as L and U = D L^T): U[i][j] = U[i][i]
+// L[j][i] for rows i in [k0, k0 + b) and columns j in [c0, c1), from the
+// already final L21 column block -- instead of the triangular solve
+// U12 = L11^-1 A12 of the general factorisation.  A 32 x 32 tile per
+// workgroup, transposed through LDS so that both reads and writes run along
+// rows.
+template <typename R>
+__global__ __launch_bounds__(256) void u12_from_l21_kernel(cplx<R>* A, int64_t lda, int k0, int b, int c0, int c1) {
+  __shared__ cplx<R> t[32][33];
+  const int i0 = k0 + blockIdx.y * 32, j0 = c0 + blockIdx.x * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 8 rows of 32 per pass
+  for (int r = ty; r < 32; r += 8) {  // t[jj][ii] = L[j0 + jj][i0 + ii]
+    const int j = j0 + r, i = i0 + tx;
+    t[r][tx] = (j < c1 && i < k0 + b) ? A[(int64_t)j * lda + i] : cplx<R>{0, 0};
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int i = i0 + r, j = j0 + tx;
+    if (i < k0 + b && j < c1) A[(int64_t)i * lda + j] = A[(int64_t)i * lda + i] * t[tx][r];
+  }
+}
+
 // ------------------------------------------------------------------ update
 // C (m x n complex, at Cp, ldc) -= A (m x kk, at Ap, lda) * B (kk x n, at Bp, ldb)
 // Real view: C' (m x 2n) -= A' (m x 2kk) B'' (2kk x 2n).  Workgroup tile: 64
@@ -558,10 +591,15 @@ struct mfma_traits<float> {
 constexpr int kGemmBK = MOM_GEMM_BK;
 
 // ASSIGN: C = -A B (C is not read) instead of C -= A B.
+// skip_above >= 0: C's column j lies on the global diagonal of row
+// j + skip_above (the trailing block of a symmetric factorisation), and tiles
+// whose rows all lie above the diagonal are not computed (they are never
+// read again).
 template <typename R, int BM, int BN, bool ASSIGN = false>
 __global__ __launch_bounds__(256) void cgemm_sub_kernel(int m, int n, int kk, const cplx<R>* __restrict__ Ap,
                                                         int64_t lda, const cplx<R>* __restrict__ Bp, int64_t ldb,
-                                                        cplx<R>* __restrict__ Cp, int64_t ldc) {
+                                                        cplx<R>* __restrict__ Cp, int64_t ldc, int skip_above = -1) {
+  if (skip_above >= 0 && (int)(blockIdx.y * BM + BM - 1) < (int)(blockIdx.x * (BN / 2)) + skip_above) return;
   typedef typename mfma_traits<R>::acc_t acc_t;
   constexpr int BK = kGemmBK;
   constexpr int TM = BM / 32, TN = BN / 32;          // MFMA tiles per wavefront
@@ -658,21 +696,39 @@ __global__ __launch_bounds__(256) void cgemm_sub_kernel(int m, int n, int kk, co
 // MI355X for the N ~ 12k factorisation.
 static int g_tile[2][2] = {{64, 128}, {64, 128}};  // [complex128, complex64][BM, BN]
 
+// flops of the tiles a launch computes (all, or with skip_above those on or
+// below the diagonal), for the roofline report
+template <int BM, int BN>
+static double cgemm_flops(int m, int n, int k, int skip_above) {
+  if (skip_above < 0) return 8.0 * m * (double)n * k;
+  double f = 0;
+  for (int ty = 0; ty * BM < m; ++ty)
+    for (int tx = 0; tx * (BN / 2) < n; ++tx)
+      if (ty * BM + BM - 1 >= tx * (BN / 2) + skip_above)
+        f += 8.0 * std::min(BM, m - ty * BM) * (double)std::min(BN / 2, n - tx * (BN / 2)) * k;
+  return f;
+}
+
+static double g_last_gemm_flops = 0;  // of the last launch_cgemm (timing report)
+
 template <typename R, int BM, int BN, bool ASSIGN = false>
 static int launch_cgemm_tile(int m, int n, int k, const cplx<R>* A, int64_t lda, const cplx<R>* B, int64_t ldb,
-                             cplx<R>* C, int64_t ldc, hipStream_t s) {
+                             cplx<R>* C, int64_t ldc, hipStream_t s, int skip_above) {
   if ((m + BM - 1) / BM > 65535) return fail("cgemm: too many row tiles");
   dim3 grid((2 * n + BN - 1) / BN, (m + BM - 1) / BM);
-  hipLaunchKernelGGL((cgemm_sub_kernel<R, BM, BN, ASSIGN>), grid, dim3(256), 0, s, m, n, k, A, lda, B, ldb, C, ldc);
+  hipLaunchKernelGGL((cgemm_sub_kernel<R, BM, BN, ASSIGN>), grid, dim3(256), 0, s, m, n, k, A, lda, B, ldb, C, ldc,
+                     skip_above);
+  g_last_gemm_flops = cgemm_flops<BM, BN>(m, n, k, skip_above);
   return check_launch("cgemm_sub_kernel");
 }
 
 template <typename R, bool ASSIGN = false>
 static int launch_cgemm(int m, int n, int k, const cplx<R>* A, int64_t lda, const cplx<R>* B, int64_t ldb,
-                        cplx<R>* C, int64_t ldc, hipStream_t s) {
+                        cplx<R>* C, int64_t ldc, hipStream_t s, int skip_above = -1) {
   const int* t = g_tile[sizeof(R) == sizeof(double) ? 0 : 1];
-#define MOM_TILE(BM, BN) \
-  if (t[0] == BM && t[1] == BN) return launch_cgemm_tile<R, BM, BN, ASSIGN>(m, n, k, A, lda, B, ldb, C, ldc, s);
+#define MOM_TILE(BM, BN)                                                                                      \
+  if (t[0] == BM && t[1] == BN)                                                                               \
+    return launch_cgemm_tile<R, BM, BN, ASSIGN>(m, n, k, A, lda, B, ldb, C, ldc, s, skip_above);
   MOM_TILE(64, 64)
   MOM_TILE(64, 128)
   MOM_TILE(128, 64)
@@ -1283,18 +1339,28 @@ static int laswp_block(cplx<R>* A, int64_t lda, int n, int c0, int c1, int k0, i
 
 // Trailing update of rows [r0, n) x columns [c0, c1) by the factored block
 // column [k0, k0 + b): U12 = L11^-1 A12, then A22 -= L21 U12 (depth b).
+// sym (a complex-symmetric matrix without interchanges): U12 = D L21^T from
+// the final L21 instead, and only the tiles on or below the diagonal updated
+// (the upper triangle is never read: every later U12 comes from L21 too).
 template <typename R>
-static int update_block(cplx<R>* A, int64_t lda, int n, int k0, int b, int r0, int c0, int c1, hipStream_t s) {
+static int update_block(cplx<R>* A, int64_t lda, int n, int k0, int b, int r0, int c0, int c1, hipStream_t s,
+                        bool sym = false) {
   if (c1 <= c0) return 0;
   int rc;
-  if ((rc = trsm_block<R>(A, lda, k0, b, c0, c1, s))) return rc;
+  if (sym) {
+    hipLaunchKernelGGL(u12_from_l21_kernel<R>, dim3((c1 - c0 + 31) / 32, (b + 31) / 32), dim3(256), 0, s, A, lda, k0,
+                       b, c0, c1);
+    if ((rc = check_launch("u12_from_l21_kernel"))) return rc;
+  } else if ((rc = trsm_block<R>(A, lda, k0, b, c0, c1, s))) {
+    return rc;
+  }
   const int mm = n - r0, nn = c1 - c0;
   if (mm <= 0) return 0;
   timing_mark_begin(s);
   if ((rc = launch_cgemm<R>(mm, nn, b, A + (int64_t)r0 * lda + k0, lda, A + (int64_t)k0 * lda + c0, lda,
-                            A + (int64_t)r0 * lda + c0, lda, s)))
+                            A + (int64_t)r0 * lda + c0, lda, s, sym ? c0 - r0 : -1)))
     return rc;
-  timing_mark_end(s, 8.0 * (double)mm * (double)nn * (double)b);
+  timing_mark_end(s, g_last_gemm_flops);
   return 0;
 }
 
@@ -1302,7 +1368,7 @@ static int update_block(cplx<R>* A, int64_t lda, int n, int k0, int b, int r0, i
 // diagonal block, then the rows below.
 template <typename R>
 static int launch_panel_nopiv(cplx<R>* A, int64_t lda, int n, int k0, int b, int32_t* ipiv, int32_t* info,
-                              hipStream_t s) {
+                              hipStream_t s, bool sym = false) {
   if (b > kNoPivPanelMax<R>()) return fail("mom_getrf_nopiv: panel wider than the diagonal-block kernel");
   static bool attr_set = false;
   if (!attr_set) {
@@ -1313,7 +1379,7 @@ static int launch_panel_nopiv(cplx<R>* A, int64_t lda, int n, int k0, int b, int
     attr_set = true;
   }
   hipLaunchKernelGGL(diag_lu_kernel<R>, dim3(1), dim3(1024), sizeof(cplx<R>) * (size_t)b * b, s, A, lda, k0, b, ipiv,
-                     info);
+                     info, (int)sym);
   int rc = check_launch("diag_lu_kernel");
   if (rc) return rc;
   const int below = n - k0 - b;
@@ -1333,7 +1399,7 @@ struct PanelWork {
 // columns and the block's remaining columns are updated.
 template <typename R>
 static int factor_block(cplx<R>* A, int64_t lda, int n, int k0, int nb, int jb, int32_t* ipiv, int32_t* info,
-                        const PanelWork& pw, bool pivot, hipStream_t s) {
+                        const PanelWork& pw, bool pivot, hipStream_t s, bool sym = false) {
   int rc;
   const int kend = k0 + nb;
   for (int q0 = k0; q0 < kend; q0 += jb) {
@@ -1342,12 +1408,12 @@ static int factor_block(cplx<R>* A, int64_t lda, int n, int k0, int nb, int jb, 
       if ((rc = launch_panel<R>(A, lda, n, q0, bq, ipiv, info, pw.slots, (cplx<R>*)pw.rowbuf, pw.gmax, s)))
         return rc;
       if ((rc = laswp_block<R>(A, lda, n, k0, q0, q0, bq, ipiv, s))) return rc;
-    } else if ((rc = launch_panel_nopiv<R>(A, lda, n, q0, bq, ipiv, info, s))) {
+    } else if ((rc = launch_panel_nopiv<R>(A, lda, n, q0, bq, ipiv, info, s, sym))) {
       return rc;
     }
     if (q1 < kend) {
       if (pivot && (rc = laswp_block<R>(A, lda, n, q1, kend, q0, bq, ipiv, s))) return rc;
-      if ((rc = update_block<R>(A, lda, n, q0, bq, q1, q1, kend, s))) return rc;
+      if ((rc = update_block<R>(A, lda, n, q0, bq, q1, q1, kend, s, sym))) return rc;
     }
   }
   return 0;
@@ -1358,7 +1424,8 @@ static int factor_block(cplx<R>* A, int64_t lda, int n, int k0, int nb, int jb, 
 // were updated first) is factored on the side stream.
 template <typename R>
 static int getrf(int n, cplx<R>* A, int64_t lda, int32_t* ipiv, int32_t* info, void* work, size_t work_bytes, int jb,
-                 int max_groups, hipStream_t s, bool pivot = true) {
+                 int max_groups, hipStream_t s, bool pivot = true, bool sym = false) {
+  if (sym && pivot) return fail("getrf: the symmetric form has no row interchanges");
   if (n <= 0) return 0;
   if (jb <= 0 || jb > 128 || (jb % 8) != 0) return fail("mom_getrf: panel width must be a multiple of 8 in [8, 128]");
   if (work_bytes < getrf_work_bytes<R>(n, jb)) return fail("mom_getrf: workspace too small");
@@ -1389,7 +1456,7 @@ static int getrf(int n, cplx<R>* A, int64_t lda, int32_t* ipiv, int32_t* info, v
   if ((rc = lookahead_for(s, 2 * (size_t)nblock + 2, &la))) return rc;
   hipStream_t side = la->side;
   // block 0 on the caller's stream
-  if ((rc = factor_block<R>(A, lda, n, 0, std::min(nb, n), jp, ipiv, info, pw, pivot, s))) return rc;
+  if ((rc = factor_block<R>(A, lda, n, 0, std::min(nb, n), jp, ipiv, info, pw, pivot, s, sym))) return rc;
   for (int p = 0; p < nblock; ++p) {
     const int k0 = p * nb, b = std::min(nb, n - k0);
     const int c0 = k0 + b;
@@ -1399,13 +1466,13 @@ static int getrf(int n, cplx<R>* A, int64_t lda, int32_t* ipiv, int32_t* info, v
     if (pivot && (rc = laswp_block<R>(A, lda, n, c0, n, k0, b, ipiv, s))) return rc;
     const int b1 = std::min(nb, n - c0);
     // next block's columns first, then hand the block to the side stream
-    if ((rc = update_block<R>(A, lda, n, k0, b, c0, c0, c0 + b1, s))) return rc;
+    if ((rc = update_block<R>(A, lda, n, k0, b, c0, c0, c0 + b1, s, sym))) return rc;
     MOM_TRY(hipEventRecord(la->ev[2 * p + 1], s));
     MOM_TRY(hipStreamWaitEvent(side, la->ev[2 * p + 1], 0));
-    if ((rc = factor_block<R>(A, lda, n, c0, b1, jp, ipiv, info, pw, pivot, side))) return rc;
+    if ((rc = factor_block<R>(A, lda, n, c0, b1, jp, ipiv, info, pw, pivot, side, sym))) return rc;
     MOM_TRY(hipEventRecord(la->ev[2 * (p + 1)], side));
     // the rest of the trailing matrix
-    if ((rc = update_block<R>(A, lda, n, k0, b, c0, c0 + b1, n, s))) return rc;
+    if ((rc = update_block<R>(A, lda, n, k0, b, c0, c0 + b1, n, s, sym))) return rc;
   }
   return 0;
 }
@@ -2218,6 +2285,17 @@ extern "C" int mom_getrf_nopiv(int prec, int n, void* A, int64_t lda, int32_t* i
   return fail("mom_getrf_nopiv: prec must be 64 (complex128) or 32 (complex64)");
 }
 
+extern "C" int mom_sytrf_nopiv(int prec, int n, void* A, int64_t lda, int32_t* ipiv, int32_t* info, void* work,
+                               int64_t work_bytes, int jb, void* stream) {
+  if (!A || !ipiv || !info || !work) return fail("mom_sytrf_nopiv: null device pointer");
+  if (lda < n) return fail("mom_sytrf_nopiv: lda < n");
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == 64)
+    return getrf<double>(n, (z64*)A, lda, ipiv, info, work, (size_t)work_bytes, jb, 0, s, false, true);
+  if (prec == 32) return getrf<float>(n, (c32*)A, lda, ipiv, info, work, (size_t)work_bytes, jb, 0, s, false, true);
+  return fail("mom_sytrf_nopiv: prec must be 64 (complex128) or 32 (complex64)");
+}
+
 extern "C" int mom_build_perm(int n, const int32_t* ipiv, int32_t* perm, void* stream) {
   if (n <= 0) return 0;
   size_t lds = n <= 16000 ? sizeof(int32_t) * (size_t)n : 0;
@@ -2474,145 +2552,183 @@ extern "C" int mom_scaled_matvec(int n, int nrhs, const void* Z, int64_t ldz, co
 }
 
 // W = D Z D X for a symmetric Z (Z = Z^T, as the fill leaves it) from the
-// tiles on and above the diagonal only: half the HBM traffic of the
-// row-wise product.  symv_tile_kernel: workgroup (I, J), I <= J, reads the
-// 64 x 64 tile Z_IJ once and forms both its row product Z_IJ (D X)_J and,
-// off the diagonal, its column product Z_IJ^T (D X)_I, which stands for the
-// tile Z_JI below the diagonal.  Thread (a, b) = (tid / 16, tid % 16) holds
-// the 4 x 4 entries of rows 4a + u and columns b + 16 v (u, v < 4), so that
-// each load instruction reads 16 consecutive entries of a row per row group;
-// row sums reduce over the 16 lanes of equal a, column sums over the lanes
-// of equal b and the 4 wavefronts.  The partial products go to Q[K][S] (64 x NR each): the row
-// product of tile (I, J) to Q[I][J], the column product to Q[J][I], so that
-// block K's partials are Q[K][0 .. nb) -- summed in a fixed order by
+// 64 x 64 tiles on and above the diagonal only: half the HBM traffic of the
+// row-wise product.  symv_strip_kernel: workgroup (c, I) streams the tiles
+// Z_IJ, J = I + 8c .. I + 8c + 7, of block row I.  Each tile gives its row
+// product Z_IJ (D X)_J, summed over the strip in registers, and, off the
+// diagonal, its column product Z_IJ^T (D X)_I, which stands for the tile
+// Z_JI below the diagonal.  Wavefront w takes the tile's columns 8w .. 8w + 7
+// and then 32 + 8w .. 32 + 8w + 7 (two half-tiles) over all 64 rows: lane
+// (a, b) = (lane / 8, lane % 8) holds the entries of rows a + 8u (u < 8) in
+// one column, so that a column product reduces over the 8 lanes of equal b
+// inside the wavefront (no LDS, no barrier per tile) and each load
+// instruction reads 8 rows of 8 consecutive entries.  The half-tiles pass
+// through a ring of registers with the next ones' loads in flight; two
+// workgroups of 256 threads share a CU, one streaming while the other starts
+// or ends its strip.  The partial products go to Q[K][S] (64 x NR each):
+// the column product of tile (I, J) to Q[J][I] (S = I < J), the row product
+// of strip c of block row I to Q[I][I + c], so that block K's partials are
+// Q[K][0 .. K + nstrips(K)) -- summed in a fixed order by
 // symv_reduce_kernel, which also applies D (deterministic: no atomics).
 constexpr int kSymvT = 64;
-
+constexpr int kSymvChunk = 8;         // tiles per workgroup along a block row
+constexpr int kSymvLanesPerRow = 8;   // lanes that read one row's segment per load
+// steps (column groups of 4 LPR) in the register ring of
+// symv_strip_kernel<NR>: as many as fit in 256 VGPRs without spilling
 template <int NR>
-__global__ __launch_bounds__(256) void symv_tile_kernel(int n, const z64* __restrict__ Z, int64_t ldz,
-                                                        const z64* __restrict__ X, const double* __restrict__ dsc,
-                                                        z64* __restrict__ Q) {
-  const int J = blockIdx.x, I = blockIdx.y;
-  if (J < I) return;
-  const int nb = gridDim.x;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int a = tid >> 4, b = tid & 15;
-  const int r0 = I * kSymvT + 4 * a, c0 = J * kSymvT + b;
+constexpr int kSymvBufs = NR == 1 ? 4 : 2;
+
+template <int NR, int NBUF, int LPR>
+__global__ __launch_bounds__(256) void symv_strip_kernel(int n, const z64* __restrict__ Z, int64_t ldz,
+                                                         const z64* __restrict__ X, int ldx,
+                                                         const double* __restrict__ dsc, z64* __restrict__ Q) {
+  constexpr int RPI = 64 / LPR;              // rows per load instruction
+  constexpr int U = kSymvT / RPI;            // rows per lane
+  constexpr int SPT = kSymvT / (4 * LPR);    // steps per tile
+  constexpr int CH = kSymvChunk, T = SPT * CH;
+  const int I = blockIdx.y, nb = gridDim.y;
+  const int J0 = I + (int)blockIdx.x * CH;
+  if (J0 >= nb) return;  // uniform per workgroup
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int a = lane / LPR, b = lane % LPR;
+  __shared__ z64 ys[kSymvT * NR];      // (D X) of block row I, 0 beyond n
+  __shared__ z64 part[4][kSymvT][NR];  // the wavefronts' row products
+  // The loop body has no branches (a branch around a load makes the
+  // compiler wait for every load in flight at the join): indices beyond n
+  // are clamped to n - 1, and such an entry (finite) meets a zero of (D X)
+  // in both products or lands in a row of W that is never written; tiles
+  // past the end of the block row (J >= nb) are read at column n - 1 and
+  // contribute zeros; a column product that is not to be kept goes to the
+  // workspace's spare slot.
   typedef double dv2 __attribute__((ext_vector_type(2)));
-  z64 yi[4][NR], yj[4][NR];
+  const z64* zrow[U];
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int r = r0 + u, c = c0 + 16 * u;
-    const double dr = r < n ? dsc[r] : 0.0, dc = c < n ? dsc[c] : 0.0;
+  for (int u = 0; u < U; ++u) zrow[u] = Z + (int64_t)min(I * kSymvT + a + RPI * u, n - 1) * ldz;
+  z64 zb[NBUF][U], xb[NBUF][NR];
+  double db[NBUF];
+  auto column = [&](int t) { return (J0 + t / SPT) * kSymvT + (t % SPT) * 4 * LPR + LPR * w + b; };
+  auto load = [&](int t, z64 (&z)[U], z64 (&xj)[NR], double& dc) {
+    const int c = column(t), cc = min(c, n - 1);
+    dc = dsc[cc] * (c < n ? 1.0 : 0.0);
+#pragma unroll
+    for (int k = 0; k < NR; ++k) xj[k] = X[(int64_t)cc * ldx + k];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      // plain loads: measured faster here than non-temporal ones
+      const dv2 v = *reinterpret_cast<const dv2*>(zrow[u] + cc);
+      z[u] = {v.x, v.y};
+    }
+  };
+  // (D X)_I: one entry per thread (threads past 64 NR repeat the last one)
+  const int ye = min(tid, kSymvT * NR - 1), yr = I * kSymvT + ye / NR, yrc = min(yr, n - 1);
+  const z64 yx = X[(int64_t)yrc * ldx + ye % NR];
+  const double yd = dsc[yrc] * (yr < n ? 1.0 : 0.0);
+#pragma unroll
+  for (int t = 0; t < NBUF - 1; ++t) load(t, zb[t], xb[t], db[t]);
+  __builtin_amdgcn_sched_barrier(0);
+  ys[ye] = yx * yd;
+  __syncthreads();
+  z64 rp[U][NR];
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+#pragma unroll
+    for (int k = 0; k < NR; ++k) rp[u][k] = {0.0, 0.0};
+  z64* const spare = Q + (int64_t)nb * nb * kSymvT * NR;
+#pragma unroll
+  for (int t = 0; t < T; ++t) {
+    if (t + NBUF - 1 < T)
+      load(t + NBUF - 1, zb[(t + NBUF - 1) % NBUF], xb[(t + NBUF - 1) % NBUF], db[(t + NBUF - 1) % NBUF]);
+    // keeps the loads of later steps and the reads of ys below in this
+    // iteration (hoisted, they would take the registers of the ring)
+    __builtin_amdgcn_sched_barrier(0);
+    asm volatile("" ::: "memory");
+    const z64(&z)[U] = zb[t % NBUF];
+    z64 xj[NR], cp[NR];
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
-      yi[u][k] = {0.0, 0.0};
-      yj[u][k] = {0.0, 0.0};
-      if (r < n) yi[u][k] = X[(int64_t)r * NR + k] * dr;
-      if (c < n) yj[u][k] = X[(int64_t)c * NR + k] * dc;
-    }
-  }
-  z64 z[4][4];
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int r = r0 + u;
-    const dv2* zr = reinterpret_cast<const dv2*>(Z + (int64_t)min(r, n - 1) * ldz + c0);
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      z[u][v] = {0.0, 0.0};
-      if (r < n && c0 + 16 * v < n) {
-        const dv2 t = __builtin_nontemporal_load(zr + 16 * v);
-        z[u][v] = {t.x, t.y};
-      }
-    }
-  }
-  z64 rp[4][NR], cp[4][NR];
-#pragma unroll
-  for (int u = 0; u < 4; ++u)
-#pragma unroll
-    for (int k = 0; k < NR; ++k) {
-      rp[u][k] = {0.0, 0.0};
-      cp[u][k] = {0.0, 0.0};
+      xj[k] = xb[t % NBUF][k] * db[t % NBUF];
+      cp[k] = {0.0, 0.0};
     }
 #pragma unroll
-  for (int u = 0; u < 4; ++u)
-#pragma unroll
-    for (int v = 0; v < 4; ++v)
+    for (int u = 0; u < U; ++u)
 #pragma unroll
       for (int k = 0; k < NR; ++k) {
-        rp[u][k] += z[u][v] * yj[v][k];
-        cp[v][k] += z[u][v] * yi[u][k];
+        rp[u][k] += z[u] * xj[k];
+        cp[k] += z[u] * ys[(a + RPI * u) * NR + k];
+        // the row sums are due here (left to float to the end of the strip,
+        // their updates would keep every step's registers live)
+        asm volatile("" : "+v"(rp[u][k].re), "+v"(rp[u][k].im));
       }
-  // row products: sum over the 16 lanes of this row group
 #pragma unroll
-  for (int u = 0; u < 4; ++u)
+    for (int k = 0; k < NR; ++k) {
+      double re = cp[k].re, im = cp[k].im;
+#pragma unroll
+      for (int off = LPR; off < 64; off <<= 1) {
+        re += __shfl_xor(re, off, 64);
+        im += __shfl_xor(im, off, 64);
+      }
+      cp[k] = {re, im};
+    }
+    // every lane of the column holds the sum; all write it (same value).
+    // The diagonal tile is whole: its row product covers it.
+    const int J = J0 + t / SPT, cl = column(t) - J * kSymvT;
+    const bool keep = J < nb && J != I;
+    z64* q = (keep ? Q + ((int64_t)J * nb + I) * kSymvT * NR : spare) + cl * NR;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) q[k] = cp[k];
+  }
+  // row products: the LPR lanes of equal a, then the 4 wavefronts in order
+#pragma unroll
+  for (int u = 0; u < U; ++u)
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
       double re = rp[u][k].re, im = rp[u][k].im;
 #pragma unroll
-      for (int off = 1; off < 16; off <<= 1) {
+      for (int off = 1; off < LPR; off <<= 1) {
         re += __shfl_xor(re, off, 64);
         im += __shfl_xor(im, off, 64);
       }
-      rp[u][k] = {re, im};
-    }
-  z64* qrow = Q + ((int64_t)I * nb + J) * kSymvT * NR;
-  if (b == 0) {
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-      for (int k = 0; k < NR; ++k) qrow[(4 * a + u) * NR + k] = rp[u][k];
-  }
-  if (I == J) return;  // uniform per workgroup
-  // column products: the 4 row groups of this wavefront, then the 4 wavefronts
-  __shared__ z64 cs[4][kSymvT][NR];
-#pragma unroll
-  for (int v = 0; v < 4; ++v)
-#pragma unroll
-    for (int k = 0; k < NR; ++k) {
-      double re = cp[v][k].re, im = cp[v][k].im;
-      re += __shfl_xor(re, 16, 64);
-      im += __shfl_xor(im, 16, 64);
-      re += __shfl_xor(re, 32, 64);
-      im += __shfl_xor(im, 32, 64);
-      if (lane < 16) cs[wave][b + 16 * v][k] = {re, im};
+      if (b == 0) part[w][a + RPI * u][k] = {re, im};
     }
   __syncthreads();
-  z64* qcol = Q + ((int64_t)J * nb + I) * kSymvT * NR;
+  z64* q = Q + ((int64_t)I * nb + I + (int)blockIdx.x) * kSymvT * NR;  // slot S = I + c
   for (int e = tid; e < kSymvT * NR; e += 256) {
-    const int c = e / NR, k = e % NR;
-    z64 t = cs[0][c][k];
-    t += cs[1][c][k];
-    t += cs[2][c][k];
-    t += cs[3][c][k];
-    qcol[e] = t;
+    const int r = e / NR, k = e % NR;
+    z64 t = part[0][r][k];
+#pragma unroll
+    for (int v = 1; v < 4; ++v) t += part[v][r][k];
+    q[e] = t;
   }
 }
 
-// block K of W: thread (p, e) sums the partials S = p, p + 4, ... of entry e
-// (4 independent chains of loads per entry), then the 4 sums in order.
+// block K of W: thread (p, e) sums the partials S = p, p + PS, ... < K +
+// nchunks(K) of entry e (independent chains of loads per entry), then the
+// PS sums in order.
 template <int NR>
-__global__ __launch_bounds__(1024) void symv_reduce_kernel(int n, int nb, const z64* __restrict__ Q,
-                                                           const double* __restrict__ dsc, z64* __restrict__ W) {
+__global__ __launch_bounds__(1024) void symv_reduce_kernel(int n, int nb, int chunk, const z64* __restrict__ Q,
+                                                           const double* __restrict__ dsc, z64* __restrict__ W,
+                                                           int ldw) {
   constexpr int E = kSymvT * NR, PS = 1024 / E;  // entries per block, partitions of S
   __shared__ z64 part[PS][E];
-  const int K = blockIdx.x, e = threadIdx.x % E, p = threadIdx.x / E;  // p == PS: idle threads (NR = 3)
+  const int K = blockIdx.x, e = threadIdx.x % E, p = threadIdx.x / E;
+  const int ns = K + (nb - K + chunk - 1) / chunk;
   const z64* q = Q + (int64_t)K * nb * E + e;
   z64 t = {0.0, 0.0};
-  if (p < PS)
-    for (int S = p; S < nb; S += PS) t += q[(int64_t)S * E];
-  if (p < PS) part[p][e] = t;
+  for (int S = p; S < ns; S += PS) t += q[(int64_t)S * E];
+  part[p][e] = t;
   __syncthreads();
   if (p != 0) return;
   z64 sum = part[0][e];
   for (int k = 1; k < PS; ++k) sum += part[k][e];
   const int row = K * kSymvT + e / NR;
-  if (row < n) W[(int64_t)K * E + e] = sum * dsc[row];
+  if (row < n) W[(int64_t)row * ldw + e % NR] = sum * dsc[row];
 }
 
 extern "C" int64_t mom_scaled_symv_work_bytes(int n, int nrhs) {
+  // the partials of the passes (of at most 2 right-hand sides each) and a
+  // spare slot for the column products that are not kept
   const int64_t nb = (n + kSymvT - 1) / kSymvT;
-  return nb * nb * kSymvT * (int64_t)nrhs * (int64_t)sizeof(z64);
+  return (nb * nb + 1) * kSymvT * (int64_t)std::max(1, std::min(nrhs, 2)) * (int64_t)sizeof(z64);
 }
 
 extern "C" int mom_scaled_symv(int n, int nrhs, const void* Z, int64_t ldz, const double* dsc, const void* X,
@@ -2629,20 +2745,26 @@ extern "C" int mom_scaled_symv(int n, int nrhs, const void* Z, int64_t ldz, cons
   hipStream_t s = (hipStream_t)stream;
   const z64 *z = (const z64*)Z, *x = (const z64*)X;
   z64 *w = (z64*)W, *q = (z64*)work;
-#define MOM_SYMV(NR)                                                                                          \
-  case NR:                                                                                                    \
-    hipLaunchKernelGGL((symv_tile_kernel<NR>), dim3(nb, nb), dim3(256), 0, s, n, z, ldz, x, dsc, q);          \
-    if (int rc = check_launch("symv_tile_kernel")) return rc;                                                 \
-    hipLaunchKernelGGL((symv_reduce_kernel<NR>), dim3(nb), dim3(1024), 0, s, n, nb, (const z64*)q, dsc, w);    \
-    break;
-  switch (nrhs) {
-    MOM_SYMV(1)
-    MOM_SYMV(2)
-    MOM_SYMV(3)
-    MOM_SYMV(4)
+  const int ch = kSymvChunk, nc = (nb + ch - 1) / ch;
+  // right-hand sides in passes of at most 2 (the strip kernel's registers)
+  for (int k0 = 0; k0 < nrhs; k0 += 2) {
+    const int nr = std::min(2, nrhs - k0);
+    if (nr == 2) {
+      hipLaunchKernelGGL((symv_strip_kernel<2, kSymvBufs<2>, kSymvLanesPerRow>), dim3(nc, nb), dim3(256), 0, s, n, z, ldz, x + k0, nrhs,
+                         dsc, q);
+      if (int rc = check_launch("symv_strip_kernel")) return rc;
+      hipLaunchKernelGGL((symv_reduce_kernel<2>), dim3(nb), dim3(1024), 0, s, n, nb, ch, (const z64*)q, dsc, w + k0,
+                         nrhs);
+    } else {
+      hipLaunchKernelGGL((symv_strip_kernel<1, kSymvBufs<1>, kSymvLanesPerRow>), dim3(nc, nb), dim3(256), 0, s, n, z, ldz, x + k0, nrhs,
+                         dsc, q);
+      if (int rc = check_launch("symv_strip_kernel")) return rc;
+      hipLaunchKernelGGL((symv_reduce_kernel<1>), dim3(nb), dim3(1024), 0, s, n, nb, ch, (const z64*)q, dsc, w + k0,
+                         nrhs);
+    }
+    if (int rc = check_launch("symv_reduce_kernel")) return rc;
   }
-#undef MOM_SYMV
-  return check_launch("symv_reduce_kernel");
+  return 0;
 }
 
 extern "C" int mom_cvt_z_to_c(int64_t n_reals, const double* src, float* dst, void* stream) {

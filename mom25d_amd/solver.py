@@ -443,6 +443,13 @@ class Solver:
                 precondition as well as pivoted ones (element growth ~2); a
                 solve whose GMRES has not converged after NOPIV_CAP steps
                 is factored again with pivoting.
+    ldlt      : mixed precision, with nopiv: D Z D is complex symmetric
+                (Galerkin testing with the basis itself), so factor it as
+                L D L^T (mom_sytrf_nopiv: the rows of U come from the columns
+                of L and the trailing updates skip the tiles above the
+                diagonal, about half the update flops of the LU).  The
+                factors are stored as L and U = D L^T, so every solve takes
+                them unchanged.
     solve_sb  : mixed precision: rows of the super-blocks of the
                 preconditioner's triangular solves (512: the chain of
                 super-blocks with precomputed inverses and scaled couplings,
@@ -477,7 +484,8 @@ class Solver:
     def __init__(self, mesh, substrate, z0=None, verbose=True, method="auto", v_weight=None, z0_recip_min=None,
                  tand_floor=None, precision="fp64", device="cuda", panel=None, refine_tol=1e-15, refine_maxit=20,
                  lanes=None, surrogate=None, surrogate_tol=2e-3, workers=None, graphs=False, autotune=False,
-                 gmres_restart=120, reuse_lu=True, lu_equiv=None, lu64_steps=20, nopiv=True, solve_sb=512):
+                 gmres_restart=120, reuse_lu=True, lu_equiv=None, lu64_steps=20, nopiv=True, solve_sb=512,
+                 ldlt=True):
         from .layered import Stackup
 
         if method not in ("auto", "dense"):
@@ -534,8 +542,9 @@ class Solver:
             from .autotune import tune_lu
 
             # the mixed path factors its preconditioners without row interchanges when nopiv is set
+            nopiv_mixed = precision == "mixed" and nopiv
             best = tune_lu(mesh.n_unknowns, 64 if precision == "fp64" else 32, self.device,
-                           pivot=not (precision == "mixed" and nopiv))
+                           pivot=not nopiv_mixed, symmetric=nopiv_mixed and ldlt)
             self.panel, self.block, self.tile = best["jb"], best["nb"], tuple(best["tile"])
         self.tuning = dict(jb=self.panel, nb=self.block, tile=self.tile)
         self.refine_tol = refine_tol
@@ -550,6 +559,7 @@ class Solver:
         self.point_log = []
         self.lu64_steps = lu64_steps
         self.nopiv = bool(nopiv)
+        self.ldlt = bool(ldlt)
         self.solve_sb = int(solve_sb or 0)
         self._f_poor = 0.0
         self.lu_stats = dict(factorisations=0, reused=0, fallbacks=0, gmres_steps=0, fp64_lu=0, pivoted_retries=0,
@@ -853,7 +863,8 @@ class Solver:
 
         def calls():
             if nopiv:
-                _lib.call("mom_getrf_nopiv", prec, n, P(A), A.stride(0), P(ipiv), P(info), P(work), wbytes, jb, stream)
+                _lib.call(self._nopiv_entry, prec, n, P(A), A.stride(0), P(ipiv), P(info), P(work), wbytes, jb,
+                          stream)
             else:
                 _lib.call("mom_getrf", prec, n, P(A), A.stride(0), P(ipiv), P(info), P(work), wbytes, jb, groups,
                           stream)
@@ -909,6 +920,12 @@ class Solver:
             self._scale_copy(Z, Z32, lane)
             X = yield from self._lu_solve(Z, B, Z32, lane, graph, pivot=True)
         return X
+
+    @property
+    def _nopiv_entry(self):
+        """The unpivoted factorisation of the preconditioners: L D L^T of the
+        complex-symmetric D Z D, or the general LU."""
+        return "mom_sytrf_nopiv" if self.ldlt else "mom_getrf_nopiv"
 
     def _sb_rows(self, n):
         """Super-block rows of the preconditioner solves at this size (0:
@@ -1279,7 +1296,7 @@ class Solver:
         else:
             sw = self._buf("swork64i", (int(L.mom_getrs_inv_work_bytes(64, n)),), torch.uint8, lane, zero=True)
         if nopiv:
-            _lib.call("mom_getrf_nopiv", 64, n, P(A), A.stride(0), P(ipiv), P(info), P(work), wbytes, jb, stream)
+            _lib.call(self._nopiv_entry, 64, n, P(A), A.stride(0), P(ipiv), P(info), P(work), wbytes, jb, stream)
         else:
             _lib.call("mom_getrf", 64, n, P(A), A.stride(0), P(ipiv), P(info), P(work), wbytes, jb, groups, stream)
         _lib.call("mom_build_perm", n, P(ipiv), P(perm), stream)
@@ -1323,7 +1340,9 @@ class Solver:
         """Cost of one complex64 factorisation in GMRES steps at this N: a
         model of this implementation on MI355X -- the LU at ~50 TFLOP/s
         effective or ~2.5 us per column of panel latency, whichever is larger
-        (unpivoted, nopiv; pivoted: ~32 TFLOP/s or ~8 us per column), plus
+        (unpivoted, nopiv; pivoted: ~32 TFLOP/s or ~8 us per column; L D L^T,
+        ldlt: half the LU's flops at the same rate: 54 ms against 93 ms
+        measured at n = 12 428), plus
         the super-block solve's set-up (8 n^2 sb flop of scaled couplings); a
         step = two triangular solves (64-row chain: ~2.9 us per 64 rows;
         super-block chain: the larger of ~4.5 us per super-block and one
@@ -1337,7 +1356,8 @@ class Solver:
             return float(self.lu_equiv)
         n = self.mesh.n_unknowns
         rate, per_col = (50e12, 2.5e-6) if self.nopiv else (32e12, 8e-6)
-        t_lu = max(8 / 3 * n ** 3 / rate, per_col * n)
+        flops = (4 / 3 if self.nopiv and self.ldlt else 8 / 3) * n ** 3
+        t_lu = max(flops / rate, per_col * n)
         sb = self._sb_rows(n)
         if sb:
             t_lu += 8 * n * n * sb / rate

@@ -164,6 +164,48 @@ def test_lu_nopiv_reconstructs_the_matrix(prec, n, jb):
     assert err < (3e-6 if prec == 32 else 1e-14), err
 
 
+@pytest.mark.parametrize("prec,n,jb", [(32, 1, 8), (32, 200, 32), (32, 777, 64), (32, 2049, 128), (32, 3000, 64),
+                                       (64, 300, 128), (64, 1029, 64), (64, 130, 8)])
+def test_ldlt_nopiv_factors(prec, n, jb):
+    """The complex-symmetric factorisation A = L D L^T stored as L and
+    U = D L^T: only the lower triangle is read (the upper one holds large
+    garbage that must not reach the factors), identity ipiv, U equal to the
+    pivots times L^T, L U = A to the precision's round-off, and the same
+    factors as the general unpivoted LU of the symmetric matrix up to
+    rounding (the two compute the same quantities in different orders)."""
+    rng = np.random.default_rng(3 * n + jb)
+    G = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    A = G + G.T + 4 * np.sqrt(n) * np.eye(n)
+    dt = torch.complex128 if prec == 64 else torch.complex64
+    Ar = torch.from_numpy(A).to(dt).numpy().astype(np.complex128)  # A in the factor's precision
+    junk = 1e6 * (rng.standard_normal((n, n)) + 1j)
+    Ain = np.tril(A) + np.triu(junk, 1)
+    P = _lib.ptr
+    wb = int(_lib.lib().mom_getrf_work_bytes(prec, n, jb))
+    out = {}
+    for entry, M in (("mom_sytrf_nopiv", Ain), ("mom_getrf_nopiv", A)):
+        Ad = torch.from_numpy(M).to(DEV, dt)
+        ipiv = torch.empty(n, dtype=torch.int32, device=DEV)
+        info = torch.full((4,), 7, dtype=torch.int32, device=DEV)
+        work = torch.empty(wb, dtype=torch.uint8, device=DEV)
+        _lib.call(entry, prec, n, P(Ad), n, P(ipiv), P(info), P(work), wb, jb, _lib.stream_ptr())
+        assert int(info[0]) == 0 and int(info[1]) == 0, entry
+        np.testing.assert_array_equal(ipiv.cpu().numpy(), np.arange(n))
+        out[entry] = Ad.cpu().numpy().astype(np.complex128)
+    LU = out["mom_sytrf_nopiv"]
+    L = np.tril(LU, -1) + np.eye(n)
+    U = np.triu(LU)
+    eps = 6e-8 if prec == 32 else 1.2e-16
+    # outside the diagonal blocks U is d_i L_ji itself; inside them it comes from the block's elimination,
+    # which agrees to a few units of round-off of U's off-diagonal size
+    dU = np.abs(np.triu(U, 1) - np.triu(np.diag(U)[:, None] * L.T, 1)).max()
+    assert dU <= 32 * eps * max(np.abs(np.triu(U, 1)).max(), 1e-300), dU
+    err = np.linalg.norm(L @ U - Ar) / np.linalg.norm(Ar)
+    assert err < (3e-6 if prec == 32 else 1e-14), err
+    gen = out["mom_getrf_nopiv"]
+    assert np.max(np.abs(LU - gen)) < 1e3 * eps * np.max(np.abs(gen))
+
+
 def test_lu_nopiv_reports_zero_pivot():
     """A zero leading entry: the unpivoted LU reports pivot 1 in info[0] and
     leaves finite factors (the zero pivot is replaced by 1), no fault."""
@@ -315,17 +357,21 @@ def test_getrs_with_inverted_blocks(prec, n, nrhs, rows):
 
 @pytest.mark.parametrize("prec,n,nrhs,pivot", [(32, 1, 1, False), (32, 700, 2, False), (32, 1024, 3, False),
                                                (32, 1025, 4, False), (32, 3001, 2, False), (64, 1500, 2, False),
-                                               (64, 2048, 1, True), (32, 2600, 2, True), (64, 513, 4, False)])
+                                               (64, 2048, 1, True), (32, 2600, 2, True), (64, 513, 4, False),
+                                               (32, 2600, 3, "ldlt"), (64, 1100, 2, "ldlt")])
 def test_superblock_solve(prec, n, nrhs, pivot):
     """The super-block solve (inverses of the 512-row diagonal blocks of L
     and U, couplings scaled by them, one chain of super-blocks per factor):
     backward error near the factor precision's round-off on a diagonally
     dominant matrix, the solution of the 64-row chain up to cond(block)
     rounding, partial last super-blocks, pivoted and unpivoted factors, and
-    a second solve with the same set-up."""
+    a second solve with the same set-up; pivot="ldlt": the L D L^T factors
+    of a complex-symmetric matrix."""
     rng = np.random.default_rng(n + 10 * nrhs)
     dt = torch.complex128 if prec == 64 else torch.complex64
     A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + 4 * np.sqrt(n) * np.eye(n)
+    if pivot == "ldlt":
+        A = A + A.T
     Ad = torch.from_numpy(A).to(DEV, dt)
     ipiv = torch.empty(n, dtype=torch.int32, device=DEV)
     perm = torch.empty(n, dtype=torch.int32, device=DEV)
@@ -334,7 +380,9 @@ def test_superblock_solve(prec, n, nrhs, pivot):
     work = torch.empty(wb, dtype=torch.uint8, device=DEV)
     s = _lib.stream_ptr()
     P = _lib.ptr
-    if pivot:
+    if pivot == "ldlt":
+        _lib.call("mom_sytrf_nopiv", prec, n, P(Ad), n, P(ipiv), P(info), P(work), wb, 64, s)
+    elif pivot:
         _lib.call("mom_getrf", prec, n, P(Ad), n, P(ipiv), P(info), P(work), wb, 64, 0, s)
     else:
         _lib.call("mom_getrf_nopiv", prec, n, P(Ad), n, P(ipiv), P(info), P(work), wb, 64, s)
@@ -470,8 +518,9 @@ def test_getrs_inv_z_and_scaled_matvec(n, nrhs):
 def test_scaled_symv_matches_full_product(n, nrhs, ld):
     """W = D Z D X from the tiles on and above the diagonal of a symmetric Z
     equals the numpy product and the row-wise kernel over every row (partial
-    last tiles, leading dimensions past n, 1 to 4 right-hand sides), and a
-    second call on a smaller size reuses the workspace."""
+    last tiles, leading dimensions past n, 1 to 4 right-hand sides in passes
+    of at most 2, block rows shorter and longer than a workgroup's strip of 8
+    tiles), and a second call on a smaller size reuses the workspace."""
     rng = np.random.default_rng(n + 7 * nrhs)
     A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
     Z = A + A.T  # complex symmetric

@@ -92,6 +92,25 @@ def test_superblock_and_64_row_preconditioner_solves_agree():
         assert np.abs(out[0].S - out[1].S).max() < 1e-7
 
 
+def test_ldlt_and_lu_preconditioners_agree():
+    """The mixed path with L D L^T preconditioners (default) and with the
+    general unpivoted LU reaches the same FP64 tolerance (the factors agree
+    to complex64 round-off: same bar as the two solve forms above), with
+    complex64 and with complex128 anchors."""
+    lay, st, mk, _ = W.via2()
+    m = lay.mesh(**mk)
+    f = [1.5e9, 2e9, 9e9, 14e9]
+    for lu64 in (20, 0):
+        out = []
+        for ldlt in (True, False):
+            s = Solver(m, st, verbose=False, precision="mixed", ldlt=ldlt, lu64_steps=lu64, lanes=2)
+            out.append(s.sweep(f, interp=None, keep_currents=True))
+            print(f"ldlt={ldlt} lu64_steps={lu64}: {s.lu_stats}")
+        for a, b in zip(out[0].currents, out[1].currents):
+            assert np.abs(a - b).max() < 1e-7 * np.abs(b).max()
+        assert np.abs(out[0].S - out[1].S).max() < 1e-7
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_given_matrix_symmetric_or_not(precision):
     """solve_currents(Z=...) solves the matrix it is given: a symmetric one
