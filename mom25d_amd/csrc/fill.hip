@@ -29,6 +29,8 @@
 
 #include "common.h"
 
+#include <type_traits>
+
 namespace mom {
 
 constexpr int kBlock = 32;     // test triangles per block
@@ -91,7 +93,17 @@ struct FillArgs {
   const int32_t* item_cols;    // basis index of each item column (offsets in items[4])
   const int32_t* src_cloc;     // (len(src), 3) column of each source triangle's edge, or -1
   int64_t acc_bytes;           // LDS accumulator bytes of the launch (max over its items)
+  // per work item, bounds [lo, hi] on its source-test distances R (padded
+  // for rounding): the spline intervals between them are staged in LDS when
+  // there are at most stage_max (<= kStageMax) of them
+  const double* item_r;
+  int32_t stage_max;
+  int32_t pad2_;
 };
+
+constexpr int kStageMax = 64;        // spline intervals one workgroup can stage
+constexpr int kStageStride = 18;     // doubles per staged interval (16 + pad: spreads LDS banks)
+constexpr int kStageBuckets = 2048;  // lookup buckets one workgroup can stage (one byte each)
 
 // Entry b of a bucket table covers the distances R with (int)(R * inv_dx)
 // == b (the last one every larger R too): i0 is the spline interval at the
@@ -111,6 +123,45 @@ __device__ inline void spline_eval(const PairTab& pt, const double* __restrict__
   const int i = spline_interval(bucket, pt.nbucket, pt.inv_dx, R);
   double d = R - x[i];
   const double* q = c + (int64_t)i * 16;  // [k][kernel][re,im], k = 0..3 (cubic .. constant)
+  double z = d;
+  double ar = q[12] + q[8] * z, ai = q[13] + q[9] * z;
+  double vr = q[14] + q[10] * z, vi = q[15] + q[11] * z;
+  z *= d;
+  ar += q[4] * z; ai += q[5] * z; vr += q[6] * z; vi += q[7] * z;
+  z *= d;
+  ar += q[0] * z; ai += q[1] * z; vr += q[2] * z; vi += q[3] * z;
+  sa = {ar, ai};
+  sv = {vr, vi};
+}
+
+// 1 / sqrt(x) for x > 0: the hardware estimate refined by one Newton step
+// (y (3 - x y^2) / 2, written as y + y e / 2 with e = 1 - x y^2 by FMA).
+__device__ inline double rsqrt_refined(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(-x * y, y, 1.0);
+  return fma(0.5 * y, e, y);
+}
+
+// The same from the intervals [base, base + nst) staged in LDS: nodes
+// sx[0 .. nst] (x[base ..]), coefficients sc (kStageStride doubles per
+// interval) and, for the buckets [b0, b0 + nbk), the interval at each
+// bucket's start relative to base (sb).  The interval of R is then
+// sb[b] + (R >= sx[sb[b] + 1]), clamped to the last staged interval: the
+// global table's i0 + (R >= split), since a bucket holds at most one node
+// and its split is that node, or +inf when there is none or crossing it
+// leaves the clamped interval unchanged.  Identical arithmetic, so identical
+// values; R lies in the item's bounds, so every index is in the staged range.
+__device__ inline void spline_eval_staged(const PairTab& pt, const double* sx, const double* sc,
+                                          const unsigned char* sb, int b0, int nbk, int nst, double R, z64& sa,
+                                          z64& sv) {
+  int b = (int)(R * pt.inv_dx);
+  b = (b < 0 ? 0 : (b >= pt.nbucket ? pt.nbucket - 1 : b)) - b0;
+  b = b < 0 ? 0 : (b >= nbk ? nbk - 1 : b);
+  int i = sb[b];
+  i += R >= sx[i + 1] ? 1 : 0;
+  i = i >= nst ? nst - 1 : i;
+  double d = R - sx[i];
+  const double* q = sc + i * kStageStride;
   double z = d;
   double ar = q[12] + q[8] * z, ai = q[13] + q[9] * z;
   double vr = q[14] + q[10] * z, vi = q[15] + q[11] * z;
@@ -166,12 +217,13 @@ __device__ inline void wilton(const double* Vs, double rx, double ry, double d, 
 // summed in LDS per (block row, item column) -- the item's distinct test and
 // source basis functions -- and each entry goes to Z with one atomic add, so
 // that Z sees a third of the scattered updates of a per-pair scatter.
-// The LDS accumulator allows one workgroup per CU, so the mid and far tiers
-// run 8 wavefronts per workgroup (2 per SIMD); the near tier, whose closed
-// forms need more than 256 registers per lane, runs 4.
+// The LDS accumulator allows one workgroup per CU, so a workgroup holds as
+// many wavefronts as the registers allow: 16 on the far tier (4 per SIMD,
+// 128 registers), 12 on the mid tier, 8 on the near tier, whose closed
+// forms need about 250 registers per lane.
 template <int TIER>
 struct FillShape {
-  static constexpr int threads = TIER == 0 ? 256 : 512;
+  static constexpr int threads = TIER == 0 ? 512 : (TIER == 1 ? 768 : 1024);
 };
 
 template <int TIER>
@@ -189,11 +241,34 @@ __global__ __launch_bounds__(FillShape<TIER>::threads) void fill_kernel(FillArgs
   __shared__ int32_t s_tri[kBlock];
   __shared__ int32_t s_rows[3 * kBlock];
   __shared__ int32_t s_cols[3 * 64];
+  __shared__ double s_sx[kStageMax + 1];             // staged spline nodes
+  __shared__ double s_sc[kStageMax * kStageStride];  // and coefficients
+  __shared__ unsigned char s_sb[kStageBuckets];      // and buckets
 
   const int* item = a.items + 6 * (int64_t)blockIdx.x;
   const int blk = item[0], pp = item[1], soff = item[2], scnt = item[3], coff = item[4], nc = item[5];
   const int r0 = a.blk_row_off[blk], nr = a.blk_row_off[blk + 1] - r0;
   const int tid = threadIdx.x;
+  const PairTab pt = a.pairs[pp];
+  const double* __restrict__ tx = a.tab_x + pt.x_off;
+  const double* __restrict__ tc = a.tab_c + pt.c_off;
+  const Bucket* __restrict__ tbk = a.tab_b + pt.b_off;
+  // the spline intervals and lookup buckets of this item's distances
+  // (uniform per workgroup); base is one interval early, so that the
+  // buckets' start intervals (at most one node before the item's smallest
+  // distance) are staged too
+  const double rlo = a.item_r[2 * (int64_t)blockIdx.x], rhi = a.item_r[2 * (int64_t)blockIdx.x + 1];
+  const int base = max(spline_interval(tbk, pt.nbucket, pt.inv_dx, rlo) - 1, 0);
+  const int nst = spline_interval(tbk, pt.nbucket, pt.inv_dx, rhi) - base + 1;
+  const int b0 = min((int)(rlo * pt.inv_dx), pt.nbucket - 1);
+  const int nbk = min((int)(rhi * pt.inv_dx), pt.nbucket - 1) - b0 + 1;
+  const bool staged = nst <= min(a.stage_max, kStageMax) && nbk <= kStageBuckets;
+  if (staged) {
+    for (int e = tid; e < nst * 16; e += kFillThreads)
+      s_sc[(e / 16) * kStageStride + e % 16] = tc[(int64_t)base * 16 + e];
+    for (int e = tid; e <= nst; e += kFillThreads) s_sx[e] = tx[base + e];
+    for (int e = tid; e < nbk; e += kFillThreads) s_sb[e] = (unsigned char)(tbk[b0 + e].i0 - base);
+  }
   if (tid < kBlock) {
     int p = a.blk_tris[blk * kBlock + tid];
     s_tri[tid] = p;
@@ -213,16 +288,15 @@ __global__ __launch_bounds__(FillShape<TIER>::threads) void fill_kernel(FillArgs
   for (int e = tid; e < 2 * nr * nc; e += kFillThreads) acc[e] = 0.0;
   __syncthreads();
 
-  const PairTab pt = a.pairs[pp];
-  const double* __restrict__ tx = a.tab_x + pt.x_off;
-  const double* __restrict__ tc = a.tab_c + pt.c_off;
-  const Bucket* __restrict__ tbk = a.tab_b + pt.b_off;
   const z64 kA = {pt.cA_re, pt.cA_im}, kV = {pt.cV_re, pt.cV_im};
   const double dz = pt.dz;
   const z64 jwmu = {a.jwmu_re, a.jwmu_im}, ijwe = {a.ijwe_re, a.ijwe_im};
 
   const int lane = tid & 63, wave = tid >> 6;
-  if (lane < scnt) {
+  // the pair loop, instantiated for staged and global spline reads
+  auto pairs = [&](auto staged_tag) {
+    constexpr bool STAGED = decltype(staged_tag)::value;
+    if (lane >= scnt) return;
     const int s = a.src[soff + lane];
     double Vs[6], px[L], py[L], w[L], cAs[3], cVs[3];
     int cl[3];
@@ -255,15 +329,22 @@ __global__ __launch_bounds__(FillShape<TIER>::threads) void fill_kernel(FillArgs
     for (int pl = wave; pl < kBlock; pl += NW) {
       const int p = s_tri[pl];
       if (p < 0) break;
-      // IA[i][j] = sum_k w_k (r_k - v_i) . (K1_k - v_j K0_k) summed over the
-      // test points first: T_i = sum_k a_ik . K1_k and M_i = sum_k a_ik K0_k
-      // (a_ik = w_k (r_k - v_i)), then IA[i][j] = T_i - v_j . M_i
-      z64 T[3], Mx[3], My[3];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) T[i] = Mx[i] = My[i] = {0.0, 0.0};
-      z64 IV = {0.0, 0.0};
-      // fully unrolled, the far tier would need 256 + 140 registers per lane;
-      // two test points at a time keep two spline lookups in flight there
+      // Positions relative to the test triangle's first vertex o, so that
+      // the moments below stay of the size of the triangles and their
+      // distance (no cancellation of absolute coordinates).  With a_ik =
+      // w_k (r_k - v_i), IA[i][j] = sum_k a_ik . (K1_k - v_j K0_k) is formed
+      // from six sums over the test points:
+      //   A0 = sum w_k K0A_k, A = sum w_k r_k K0A_k, C = sum w_k K1_k,
+      //   B = sum w_k r_k . K1_k
+      // as M_i = A - v_i A0 (= sum_k a_ik K0A_k) and T_i = B - v_i . C
+      // (= sum_k a_ik . K1_k), IA[i][j] = T_i - v_j . M_i.  On the far tier
+      // (one source point p') K1 = p' K0A, so T_i = p' . M_i and
+      // IA[i][j] = (p' - v_j) . M_i.
+      const double ox = s_V[pl][0], oy = s_V[pl][1];
+      z64 A0 = {0.0, 0.0}, Ax = A0, Ay = A0, Bs = A0, Cx = A0, Cy = A0, IV = A0;
+      // fully unrolled, the far tier would need more registers than
+      // occupancy allows; two test points at a time keep two spline
+      // lookups in flight there
       constexpr int KU = TIER == 2 ? 2 : 1;
 #pragma unroll KU
       for (int k = 0; k < 7; ++k) {
@@ -273,7 +354,7 @@ __global__ __launch_bounds__(FillShape<TIER>::threads) void fill_kernel(FillArgs
           double I0, I1x, I1y;
           wilton(Vs, rx, ry, dz, I0, I1x, I1y);
           const double S0 = I0 / kFourPi;
-          const double S1x = (rx * I0 + I1x) / kFourPi, S1y = (ry * I0 + I1y) / kFourPi;
+          const double S1x = ((rx - ox) * I0 + I1x) / kFourPi, S1y = ((ry - oy) * I0 + I1y) / kFourPi;
           k0a = kA * S0; k1x = kA * S1x; k1y = kA * S1y; k0v = kV * S0;
         } else {
           k0a = {0.0, 0.0}; k1x = k0a; k1y = k0a; k0v = k0a;
@@ -281,29 +362,48 @@ __global__ __launch_bounds__(FillShape<TIER>::threads) void fill_kernel(FillArgs
 #pragma unroll
         for (int l = 0; l < L; ++l) {
           const double ddx = rx - px[l], ddy = ry - py[l];
-          const double R = sqrt(ddx * ddx + ddy * ddy);
+          const double r2 = ddx * ddx + ddy * ddy;
+          double R;
           z64 sa, sv;
-          spline_eval(pt, tx, tc, tbk, R, sa, sv);
           if (TIER != 0) {
-            const double inv = (dz == 0.0) ? 1.0 / (kFourPi * R) : 1.0 / (kFourPi * sqrt(R * R + dz * dz));
+            // 1 / R and R from one reciprocal square root refined by a
+            // Newton step (within an ulp or two of the rounded sqrt and
+            // divide; R > 0 off the near tier)
+            const double y = rsqrt_refined(r2);
+            R = r2 * y;
+            const double inv = (dz == 0.0 ? y : rsqrt_refined(r2 + dz * dz)) * (1.0 / kFourPi);
+            if (STAGED)
+              spline_eval_staged(pt, s_sx, s_sc, s_sb, b0, nbk, nst, R, sa, sv);
+            else
+              spline_eval(pt, tx, tc, tbk, R, sa, sv);
             sa += kA * inv;
             sv += kV * inv;
+          } else {
+            R = sqrt(r2);
+            if (STAGED)
+              spline_eval_staged(pt, s_sx, s_sc, s_sb, b0, nbk, nst, R, sa, sv);
+            else
+              spline_eval(pt, tx, tc, tbk, R, sa, sv);
           }
           const z64 saw = sa * w[l];
           k0a += saw;
-          k1x += saw * px[l];
-          k1y += saw * py[l];
+          if (TIER != 2) {
+            k1x += saw * (px[l] - ox);
+            k1y += saw * (py[l] - oy);
+          }
           k0v += sv * w[l];
         }
-        const double wk = s_qw[pl][k];
+        const double wk = s_qw[pl][k], lx = rx - ox, ly = ry - oy;
         IV += wk * k0v;
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          const double ax = wk * (rx - s_V[pl][2 * i]);
-          const double ay = wk * (ry - s_V[pl][2 * i + 1]);
-          T[i] += ax * k1x + ay * k1y;
-          Mx[i] += ax * k0a;
-          My[i] += ay * k0a;
+        const z64 a0 = wk * k0a;
+        A0 += a0;
+        Ax += lx * a0;
+        Ay += ly * a0;
+        if (TIER != 2) {
+          const z64 cx = wk * k1x, cy = wk * k1y;
+          Cx += cx;
+          Cy += cy;
+          Bs += lx * cx + ly * cy;
         }
       }
       if (pads) {
@@ -317,11 +417,16 @@ __global__ __launch_bounds__(FillShape<TIER>::threads) void fill_kernel(FillArgs
       for (int i = 0; i < 3; ++i) {
         const int rl = s_rl[pl][i];
         if (rl < 0) continue;
+        const double vx = s_V[pl][2 * i] - ox, vy = s_V[pl][2 * i + 1] - oy;
+        const z64 Mx = Ax - vx * A0, My = Ay - vy * A0;
+        const z64 Ti = TIER == 2 ? z64{0.0, 0.0} : Bs - (vx * Cx + vy * Cy);
         const double cAi = s_cA[pl][i], cVi = s_cV[pl][i];
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
           if (cl[j] < 0) continue;
-          const z64 ia = T[i] - (Vs[2 * j] * Mx[i] + Vs[2 * j + 1] * My[i]);
+          const double ux = Vs[2 * j] - ox, uy = Vs[2 * j + 1] - oy;
+          const z64 ia = TIER == 2 ? (px[0] - Vs[2 * j]) * Mx + (py[0] - Vs[2 * j + 1]) * My
+                                   : Ti - (ux * Mx + uy * My);
           const z64 v = (jwmu * (cAi * cAs[j])) * ia + (ijwe * (cVi * cVs[j])) * IV;
           double* dst = acc + 2 * (rl * nc + cl[j]);
           atomicAdd(dst, v.re);
@@ -329,7 +434,11 @@ __global__ __launch_bounds__(FillShape<TIER>::threads) void fill_kernel(FillArgs
         }
       }
     }
-  }
+  };
+  if (staged)
+    pairs(std::true_type{});
+  else
+    pairs(std::false_type{});
   __syncthreads();
   for (int e = tid; e < nr * nc; e += kFillThreads) {
     const double re = acc[2 * e], im = acc[2 * e + 1];
@@ -561,8 +670,10 @@ extern "C" int mom_fill_tier(int tier, int n_items, const void* args, void* stre
   if (!a.Z || !a.items || !a.src || !a.pairs || !a.blk_row_off || !a.blk_rows || !a.blk_rloc || !a.item_cols ||
       !a.src_cloc)
     return fail("mom_fill_tier: null device pointer");
-  // LDS: the accumulator plus about 7 KB of block data, within the 160 KB of a CU
-  if (a.acc_bytes < 0 || a.acc_bytes > 144 * 1024) return fail("mom_fill_tier: LDS accumulator out of range");
+  // LDS: the accumulator plus about 18 KB of block data and staged spline,
+  // within the 160 KB of a CU
+  if (a.acc_bytes < 0 || a.acc_bytes > 136 * 1024) return fail("mom_fill_tier: LDS accumulator out of range");
+  if (!a.item_r) return fail("mom_fill_tier: null item bounds");
   hipStream_t s = (hipStream_t)stream;
   switch (tier) {
     case 0: return launch_fill<0>(n_items, a, s);

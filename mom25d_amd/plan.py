@@ -119,6 +119,7 @@ class FillPlan:
         self.src = np.concatenate(src_lists) if src_lists else np.zeros(0, np.int32)
         self.n_levels = nl
         self._scatter_maps(blocks, per_tier, block)
+        self._item_bounds(blocks)
 
         # conductor-loss Gram matrix per triangle, times cA_i cA_j
         X = self.qp7[:, None, :, :] - V[:, :, None, :]
@@ -164,7 +165,7 @@ class FillPlan:
         self.dev = None
 
     # LDS budget of one fill workgroup's accumulator (complex128 entries)
-    ACC_MAX_BYTES = 144 * 1024
+    ACC_MAX_BYTES = 136 * 1024
 
     def _scatter_maps(self, blocks, per_tier, block):
         """Distinct rows (test basis functions) of every block and columns
@@ -216,6 +217,37 @@ class FillPlan:
         self.item_cols = np.concatenate(cols).astype(np.int32) if cols else np.zeros(0, np.int32)
         self.src_cloc = src_cloc
 
+    def _item_bounds(self, blocks):
+        """Per work item, bounds [lo, hi] on the in-plane distance R between
+        any test point of its block (7-point rule) and any source point of
+        its tier (7-point rule near, 3-point mid, centroid far): from the
+        bounding boxes' centres and half-diagonals, padded by 1e-9 relative so
+        that the kernel's own rounding of R stays inside.  The fill kernel
+        stages the spline intervals of [lo, hi] in LDS."""
+        def circles(pts, counts):
+            # pts: (sum(counts), 2) grouped by item; centre and radius of each group's bounding box
+            starts = np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(np.int64)
+            lo = np.minimum.reduceat(pts, starts, axis=0)
+            hi = np.maximum.reduceat(pts, starts, axis=0)
+            return 0.5 * (lo + hi), 0.5 * np.hypot(*(hi - lo).T)
+
+        tp = [self.qp7[pb].reshape(-1, 2) for pb in blocks]
+        bc, br = circles(np.concatenate(tp), [t.shape[0] for t in tp]) if tp else (np.zeros((0, 2)), np.zeros(0))
+        src_pts = {NEAR: self.qp7, MID: self.qp3, FAR: self.cen[:, None, :]}
+        self.item_r = {}
+        for tier, it in self.items.items():
+            if not it.size:
+                self.item_r[tier] = np.zeros((0, 2))
+                continue
+            so, cnt = it[:, 2].astype(np.int64), it[:, 3].astype(np.int64)
+            idx = np.repeat(so - np.concatenate([[0], np.cumsum(cnt)[:-1]]), cnt) + np.arange(cnt.sum())
+            pts = src_pts[tier][self.src[idx]].reshape(-1, 2)
+            sc, sr = circles(pts, cnt * src_pts[tier].shape[1])
+            D = np.hypot(*(sc - bc[it[:, 0]]).T)
+            r = br[it[:, 0]] + sr
+            lo = np.maximum(0.0, (D - r) * (1 - 1e-9) - 1e-12)
+            self.item_r[tier] = np.stack([np.where(D - r > 0, lo, 0.0), (D + r) * (1 + 1e-9) + 1e-12], 1)
+
     def upload(self, device="cuda"):
         """Device copies of every array the kernels read (kept on the plan)."""
         if self.dev is not None:
@@ -232,6 +264,7 @@ class FillPlan:
         )
         for t, it in self.items.items():
             d[f"items{t}"] = i32(it if it.size else np.zeros((1, 6)))
+            d[f"item_r{t}"] = f64(self.item_r[t] if self.item_r[t].size else np.zeros((1, 2)))
         d.update(blk_row_off=i32(self.blk_row_off), blk_rows=i32(self.blk_rows if self.blk_rows.size else [0]),
                  blk_rloc=i32(self.blk_rloc), item_cols=i32(self.item_cols if self.item_cols.size else [0]),
                  src_cloc=i32(self.src_cloc if self.src_cloc.size else np.zeros((1, 3))))

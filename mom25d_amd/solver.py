@@ -331,7 +331,8 @@ class _FillArgs(ctypes.Structure):
         ("ijwe_re", ctypes.c_double), ("ijwe_im", ctypes.c_double), ("Z", ctypes.c_void_p), ("ldz", ctypes.c_int64),
         ("IVpad", ctypes.c_void_p), ("blk_row_off", ctypes.c_void_p), ("blk_rows", ctypes.c_void_p),
         ("blk_rloc", ctypes.c_void_p), ("item_cols", ctypes.c_void_p), ("src_cloc", ctypes.c_void_p),
-        ("acc_bytes", ctypes.c_int64)]
+        ("acc_bytes", ctypes.c_int64), ("item_r", ctypes.c_void_p), ("stage_max", ctypes.c_int32),
+        ("pad2_", ctypes.c_int32)]
 
 
 def _check_abi():
@@ -651,6 +652,11 @@ class Solver:
     # Z <- (Z + Z^T) / 2 as Engine/mom25d/solver.py:522-524 does; the frequency
     # interpolation preserves it); False: the row-wise product over all of Z
     symmetric_products = True
+    # spline intervals a fill work item may stage in LDS (those between its
+    # smallest and largest source-test distance; an item that needs more
+    # reads the table from global memory; 0: never stage).  The values are
+    # the same either way.
+    fill_stage = 64
 
     def _worker_count(self, nf):
         """Processes for the wave fits of a sweep (0: in this process)."""
@@ -766,6 +772,8 @@ class Solver:
             if not items.size:
                 continue
             a.items = d[f"items{tier}"].data_ptr()
+            a.item_r = d[f"item_r{tier}"].data_ptr()
+            a.stage_max = self.fill_stage
             a.acc_bytes = plan.acc_bytes[tier]
             _lib.call("mom_fill_tier", tier, int(items.shape[0]), ctypes.byref(a), stream)
         zs = self.stack.surface_impedance(g.freq)

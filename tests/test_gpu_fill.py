@@ -85,3 +85,24 @@ def test_fill_matches_oracle(name, f):
     err = blocks_close(Z, R, m.n_basis, 1e-10)
     print(f"{name} {f / 1e9:g} GHz N={m.n_unknowns} max blockwise rel err {err:.2e}")
     assert err < 1e-10
+
+
+@pytest.mark.parametrize("name,f", [("tee", 10e9), ("via2", 5e9)])
+def test_staged_and_global_spline_reads_agree(name, f):
+    """The fill with each work item's spline intervals staged in LDS (the
+    default) and with every spline read from global memory (fill_stage = 0)
+    computes the same arithmetic: the matrices agree to the rounding of the
+    accumulation order (the LDS and global atomics sum in arrival order),
+    and a cap of 4 intervals mixes both paths within one launch."""
+    lay, st, mk, _ = getattr(W, name)()
+    m = lay.mesh(**mk)
+    s = Solver(m, st, verbose=False)
+    Zs = s.impedance(f)
+    out = {}
+    for cap in (0, 4):
+        s.fill_stage = cap
+        out[cap] = s.impedance(f)
+    for cap, Z in out.items():
+        err = np.max(np.abs(Z - Zs)) / np.max(np.abs(Zs))
+        print(f"{name} stage cap {cap}: max rel diff {err:.1e}")
+        assert err < 1e-14, cap

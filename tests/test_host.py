@@ -127,6 +127,30 @@ def test_plan_scatter_maps_point_at_the_basis_functions(name):
             assert 16 * nr * nc <= plan.acc_bytes[tier] <= plan.ACC_MAX_BYTES
 
 
+@pytest.mark.parametrize("name", ["tee", "via2"])
+def test_plan_item_bounds_hold_every_distance(name):
+    """The distance bounds of every work item (from which the fill kernel
+    stages spline intervals) contain the in-plane distance of every (test
+    point, source point) pair of the item, by brute force over the pairs: a
+    bound too tight would read the wrong interval."""
+    from mom25d_amd.plan import FAR, MID, NEAR, FillPlan
+
+    lay, st, mk, _ = getattr(W, name)()
+    m = lay.mesh(**mk)
+    zl = np.array([(st.to_stackup() if hasattr(st, "to_stackup") else st).level_z(n) for n in m.layer_names])
+    plan = FillPlan(m, zl=zl)
+    src_pts = {NEAR: plan.qp7, MID: plan.qp3, FAR: plan.cen[:, None, :]}
+    for tier, items in plan.items.items():
+        bounds = plan.item_r[tier]
+        assert bounds.shape == (len(items), 2)
+        for (bi, pp, off, cnt, coff, nc), (lo, hi) in zip(items, bounds):
+            pb = plan.blk_tris[bi]
+            test = plan.qp7[pb[pb >= 0]].reshape(-1, 2)
+            srcp = src_pts[tier][plan.src[off:off + cnt]].reshape(-1, 2)
+            R = np.hypot(*(test[:, None, :] - srcp[None, :, :]).transpose(2, 0, 1))
+            assert 0.0 <= lo <= R.min() and R.max() <= hi, (tier, bi, off)
+
+
 def _declared_symbols():
     text = open(os.path.join(ROOT, "include", "mom25d.h")).read()
     return sorted(set(re.findall(r"^\s*(?:const char\*|int64_t|int)\s+(mom_\w+)\(", text, re.M)))
