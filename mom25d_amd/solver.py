@@ -563,8 +563,11 @@ class Solver:
         self.ldlt = bool(ldlt)
         self.solve_sb = int(solve_sb or 0)
         self._f_poor = 0.0
+        self._c64_refs = []  # (frequency, GMRES steps) of converged fresh complex64 factorisations
+        self._c64_pending = {}  # lane -> frequency of its complex64 factorisation in progress
+        self._active_lanes = 1
         self.lu_stats = dict(factorisations=0, reused=0, fallbacks=0, gmres_steps=0, fp64_lu=0, pivoted_retries=0,
-                             abandoned=0)
+                             abandoned=0, predicted_poor=0)
         self.lanes = lanes
         self.workers = workers
         self.graphs = graphs
@@ -1207,6 +1210,8 @@ class Solver:
 
         sweep = mixed and reuse and self.reuse_lu
         a = self._anchor.get(lane) if sweep else None
+        if a is not None and self._jumped(lane, g.freq):
+            a = None  # the lane moved to another part of the sweep: its factors are of a distant frequency
         filled = False
         steps = []  # (preconditioner, GMRES steps) of this point, for point_log
 
@@ -1229,6 +1234,23 @@ class Solver:
                 return log(X)
             self.lu_stats["fallbacks"] += 1
         wide = self.lu64_steps is not None and sweep
+        if wide and g.freq > 2 * self._f_poor:
+            # a lane far below the complex64 points in progress waits for one
+            # of them to converge, then predicts its own (a doomed complex64
+            # factorisation costs an LU and lu64_steps steps).  Its own
+            # intent is registered first, so that the lanes that start in the
+            # same round see each other; waits only point to frequencies at
+            # least twice as high, so no two lanes wait for each other.
+            self._c64_pending[lane] = g.freq
+            if self._active_lanes > 1 and not self._c64_refs_above(g.freq):
+                yield  # the other lanes start their first points
+                while not self._c64_refs_above(g.freq) and any(
+                        f >= 2 * g.freq for ln, f in self._c64_pending.items() if ln != lane):
+                    yield
+            self._c64_pending.pop(lane, None)
+            if self._c64_predicted_poor(g.freq):
+                self.lu_stats["predicted_poor"] += 1
+                self._f_poor = max(self._f_poor, g.freq)
         if not (wide and g.freq <= 2 * self._f_poor):
             if filled:
                 self._scale_copy(Z, Z32, lane)
@@ -1237,13 +1259,19 @@ class Solver:
                 filled = True
             # in a sweep, a complex64 solve still short of the target after
             # lu64_steps steps is abandoned for a complex128 anchor
-            X = yield from self._lu_solve(Z, B, Z32, lane, graph, cap=self.lu64_steps if wide else None)
+            if wide:
+                self._c64_pending[lane] = g.freq
+            try:
+                X = yield from self._lu_solve(Z, B, Z32, lane, graph, cap=self.lu64_steps if wide else None)
+            finally:
+                self._c64_pending.pop(lane, None)
             if mixed:
                 steps.append(("lu32", self.last_refine))
             if X is not None:
                 if mixed:
                     self.lu_stats["factorisations"] += 1
                     if sweep:
+                        self._c64_refs.append((float(g.freq), self.last_refine))
                         self._remember(lane, g.freq, X)
                         self._set_anchor(lane, g.freq, self._lane_pc[lane], self.last_refine)
                 return log(X)
@@ -1264,6 +1292,35 @@ class Solver:
         self.lu_stats["factorisations"] += 1
         self._set_anchor(lane, g.freq, pc)
         return log(X)
+
+    def _jumped(self, lane, freq):
+        """True when freq is more than three of the lane's last spacings away
+        from its last point (the lane was handed another part of the sweep)."""
+        h = self._history.get(lane, [])
+        if len(h) < 2:
+            return False
+        step = abs(h[-1][0] - h[-2][0])
+        return step > 0 and abs(freq - h[-1][0]) > 3 * step
+
+    def _c64_refs_above(self, freq):
+        return any(f > freq for f, _ in self._c64_refs)
+
+    # GMRES target of the step-count model below (relative residual reduction)
+    _C64_TARGET = 1e-15
+
+    def _c64_predicted_poor(self, freq):
+        """True when the nearest higher frequency whose own complex64
+        factorisation converged (in its_r GMRES steps) predicts more than
+        lu64_steps steps at freq: its contraction per step
+        rho_r = target^(1 / its_r) is scaled by (f_r / freq)^2 -- the charge
+        term of the MPIE grows as 1 / f^2 against the current term, and the
+        relative error of complex64 factors with the condition number."""
+        refs = [(f, its) for f, its in self._c64_refs if f > freq]
+        if not refs or self.lu64_steps is None:
+            return False
+        fr, its = min(refs)
+        rho = self._C64_TARGET ** (1.0 / max(its, 1)) * (fr / freq) ** 2
+        return rho >= 1.0 or np.log(self._C64_TARGET) / np.log(rho) > self.lu64_steps
 
     def _set_anchor(self, lane, freq, pc, its0=0):
         """The lane's factorisation (a Precond) that preconditions its next points."""
@@ -1798,6 +1855,8 @@ def _sweep_many(jobs, lanes, workers, keep_currents=False, graphs=False, on_poin
         s._anchor = {}  # no factorisation or solution carries over from an earlier call
         s._history = {}
         s._f_poor = 0.0
+        s._c64_refs = []
+        s._c64_pending = {}
     torch.cuda.synchronize(device)
     t1 = time.time()
     pool = fit_pool(workers)

@@ -151,6 +151,29 @@ def test_plan_item_bounds_hold_every_distance(name):
             assert 0.0 <= lo <= R.min() and R.max() <= hi, (tier, bi, off)
 
 
+def test_complex64_viability_prediction():
+    """The mixed sweep's prediction that complex64 factors will not reach the
+    target below a converged complex64 point: with a reference at 13.16 GHz
+    that took 7 steps (contraction 10^(-15/7) per step, times (f_r / f)^2),
+    1 GHz diverges and 2.14 GHz needs about 26 steps (> lu64_steps = 20),
+    while 5 GHz needs about 11; no reference above f, or no lu64_steps,
+    predicts nothing."""
+    from mom25d_amd import Solver
+
+    lay, st, mk, _ = W.tee()
+    s = Solver(lay.mesh(**mk), st, verbose=False, precision="mixed", device="cpu", lu64_steps=20)
+    assert not s._c64_predicted_poor(1e9)
+    s._c64_refs = [(13.16e9, 7)]
+    assert s._c64_predicted_poor(1e9) and s._c64_predicted_poor(2.14e9)
+    assert not s._c64_predicted_poor(5e9) and not s._c64_predicted_poor(14e9)
+    rho = 1e-15 ** (1 / 7) * (13.16 / 2.14) ** 2
+    assert 25 < np.log(1e-15) / np.log(rho) < 27
+    s._c64_refs.append((3e9, 30))  # the nearest reference above f is the one used
+    assert s._c64_predicted_poor(2.5e9)
+    s.lu64_steps = None
+    assert not s._c64_predicted_poor(1e9)
+
+
 def _declared_symbols():
     text = open(os.path.join(ROOT, "include", "mom25d.h")).read()
     return sorted(set(re.findall(r"^\s*(?:const char\*|int64_t|int)\s+(mom_\w+)\(", text, re.M)))
