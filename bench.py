@@ -56,7 +56,9 @@ def time_path(mesh, sub, freqs, precision, warmup, steps, torch, lanes=None, aut
         s.block = block
     s.geometry().upload(s.device)
     if warmup:
-        s.sweep(freqs[len(freqs) // 2 - warmup // 2:][:warmup], interp=None)
+        # untimed points spread over the band like the timed ones, so that every kind of point
+        # (complex64 and complex128 preconditioners, their lane buffers) has run once before the clock starts
+        s.sweep(sweep_points(freqs, warmup), interp=None)
     s.lu_stats = {k: 0 for k in s.lu_stats}
     s.point_log = []
     torch.cuda.synchronize()

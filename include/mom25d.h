@@ -49,6 +49,14 @@ int mom_j0(int n, const double* z, double* out, int real_axis, void* stream);
  * Replaces _fast.chunk_vals / group_vals / scatter (Engine/mom25d/_fast.py:82-158,
  * 190-211) and the block loop of assemble (Engine/mom25d/solver.py:474-509). */
 int mom_fill_tier(int tier, int n_items, const void* args, void* stream);
+/* The near tier's frequency-independent closed-form parts (S0 = I0 / 4 pi,
+ * S1 = ((r - o) I0 + I1) / 4 pi per test point and source triangle) of the
+ * n_items near work items in args->items, written to `out` at the offsets
+ * args->item_soff; pair_dz[lt * nl + ls] is the distance of the two levels.
+ * Once per mesh; the near tier of mom_fill_tier reads them from
+ * args->near_static.  Replaces the near-pair statics of AssemblyGeometry
+ * (Engine/mom25d/solver.py:373-376, integrals.py:24-67). */
+int mom_near_statics(int n_items, const void* args, const double* pair_dz, double* out, void* stream);
 int mom_fill_args_size(void);
 int mom_pair_tab_size(void);
 

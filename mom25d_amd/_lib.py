@@ -24,6 +24,7 @@ SIGNATURES = {
     "mom_sommerfeld": (_i, [_i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _p, _p]),
     "mom_j0": (_i, [_i, _p, _p, _i, _p]),
     "mom_fill_tier": (_i, [_i, _i, _p, _p]),
+    "mom_near_statics": (_i, [_i, _p, _p, _p, _p]),
     "mom_fill_args_size": (_i, []),
     "mom_pair_tab_size": (_i, []),
     "mom_gram_add": (_i, [_i, _p, _p, _d, _d, _p, _i64, _p]),

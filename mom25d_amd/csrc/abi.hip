@@ -2,6 +2,8 @@
 #include "common.h"
 
 #include <cstring>
+#include <map>
+#include <mutex>
 
 namespace mom {
 
@@ -17,6 +19,20 @@ int fail(const std::string& msg) {
 int check_launch(const char* what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(std::string(what) + ": " + hipGetErrorString(e));
+  return 0;
+}
+
+static std::mutex g_lds_mu;
+static std::map<const void*, size_t> g_lds;
+
+int reserve_lds(const void* kernel, size_t bytes, const char* what) {
+  std::lock_guard<std::mutex> lock(g_lds_mu);
+  size_t& have = g_lds[kernel];
+  if (bytes <= have) return 0;
+  const hipError_t e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e != hipSuccess) return fail(std::string(what) + ": cannot reserve " + std::to_string(bytes) + " bytes of LDS: " +
+                                   hipGetErrorString(e));
+  have = bytes;
   return 0;
 }
 

@@ -50,6 +50,10 @@ typedef cplx<float> c32;
 void set_error(const std::string& msg);
 int fail(const std::string& msg);
 int check_launch(const char* what);
+// Allow `kernel` (a __global__ function) at least `bytes` of dynamic LDS:
+// hipFuncSetAttribute when more than before, under a lock, so that launches
+// from several host threads agree.
+int reserve_lds(const void* kernel, size_t bytes, const char* what);
 
 }  // namespace mom
 

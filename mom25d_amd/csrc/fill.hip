@@ -99,6 +99,13 @@ struct FillArgs {
   const double* item_r;
   int32_t stage_max;
   int32_t pad2_;
+  // near tier: the frequency-independent closed-form parts S0 = I0 / 4 pi,
+  // S1 = ((r - o) I0 + I1) / 4 pi of every (test point, source triangle),
+  // computed once per mesh by near_static_kernel; work item it keeps them at
+  // near_static + item_soff[it] as [test triangle][test point][S0, S1x, S1y]
+  // [source lane] with a lane stride of the item's source count
+  const double* near_static;
+  const int64_t* item_soff;
 };
 
 constexpr int kStageMax = 64;        // spline intervals one workgroup can stage
@@ -325,6 +332,7 @@ __global__ __launch_bounds__(FillShape<TIER>::threads) void fill_kernel(FillArgs
       cl[q] = a.src_cloc[3 * (int64_t)(soff + lane) + q];
     }
     const bool pads = a.IVpad != nullptr;
+    const double* near_st = TIER == 0 ? a.near_static + a.item_soff[blockIdx.x] + lane : nullptr;
 
     for (int pl = wave; pl < kBlock; pl += NW) {
       const int p = s_tri[pl];
@@ -351,10 +359,8 @@ __global__ __launch_bounds__(FillShape<TIER>::threads) void fill_kernel(FillArgs
         const double rx = s_qp[pl][2 * k], ry = s_qp[pl][2 * k + 1];
         z64 k0a, k1x, k1y, k0v;
         if (TIER == 0) {
-          double I0, I1x, I1y;
-          wilton(Vs, rx, ry, dz, I0, I1x, I1y);
-          const double S0 = I0 / kFourPi;
-          const double S1x = ((rx - ox) * I0 + I1x) / kFourPi, S1y = ((ry - oy) * I0 + I1y) / kFourPi;
+          const double* st = near_st + (int64_t)(3 * (7 * pl + k)) * scnt;
+          const double S0 = st[0], S1x = st[scnt], S1y = st[2 * scnt];
           k0a = kA * S0; k1x = kA * S1x; k1y = kA * S1y; k0v = kV * S0;
         } else {
           k0a = {0.0, 0.0}; k1x = k0a; k1y = k0a; k0v = k0a;
@@ -446,6 +452,43 @@ __global__ __launch_bounds__(FillShape<TIER>::threads) void fill_kernel(FillArgs
     double* dst = a.Z + 2 * ((int64_t)s_rows[e / nc] * a.ldz + s_cols[e % nc]);
     unsafeAtomicAdd(dst, re);
     unsafeAtomicAdd(dst + 1, im);
+  }
+}
+
+// The near tier's closed-form parts, once per mesh (Engine/mom25d/solver.py:
+// 373-376 likewise computes them in its set-up): for every near work item,
+// test triangle pl of its block, test point k and source triangle (lane l),
+//   S0 = I0 / 4 pi,   S1 = ((r_k - o) I0 + I1) / 4 pi,
+// with I0, I1 the integrals of 1/R and (r'-r)/R over the source triangle at
+// the pair's level distance and o the test triangle's first vertex.  The
+// same arithmetic the pair kernel would do in line, so Z is unchanged.
+constexpr int kStaticThreads = 256;
+
+__global__ __launch_bounds__(kStaticThreads) void near_static_kernel(FillArgs a, const double* __restrict__ pair_dz,
+                                                                      double* __restrict__ out) {
+  const int* item = a.items + 6 * (int64_t)blockIdx.x;
+  const int blk = item[0], pp = item[1], soff = item[2], scnt = item[3];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane >= scnt) return;
+  const int s = a.src[soff + lane];
+  double Vs[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) Vs[q] = a.V[6 * (int64_t)s + q];
+  const double dz = pair_dz[pp];
+  double* dst = out + a.item_soff[blockIdx.x] + lane;
+  for (int pl = wave; pl < kBlock; pl += kStaticThreads / 64) {
+    const int p = a.blk_tris[blk * kBlock + pl];
+    if (p < 0) break;
+    const double ox = a.V[6 * (int64_t)p], oy = a.V[6 * (int64_t)p + 1];
+    for (int k = 0; k < 7; ++k) {
+      const double rx = a.qp7[14 * (int64_t)p + 2 * k], ry = a.qp7[14 * (int64_t)p + 2 * k + 1];
+      double I0, I1x, I1y;
+      wilton(Vs, rx, ry, dz, I0, I1x, I1y);
+      double* st = dst + (int64_t)(3 * (7 * pl + k)) * scnt;
+      st[0] = I0 / kFourPi;
+      st[scnt] = ((rx - ox) * I0 + I1x) / kFourPi;
+      st[2 * scnt] = ((ry - oy) * I0 + I1y) / kFourPi;
+    }
   }
 }
 
@@ -649,16 +692,10 @@ __global__ __launch_bounds__(256) void line_voltage_kernel(int npts, const doubl
 
 using namespace mom;
 
-static size_t g_fill_lds[3] = {0, 0, 0};
-
 template <int TIER>
 static int launch_fill(int n_items, const FillArgs& a, hipStream_t s) {
   const size_t lds = (size_t)a.acc_bytes;
-  if (lds > g_fill_lds[TIER]) {
-    MOM_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(fill_kernel<TIER>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    g_fill_lds[TIER] = lds;
-  }
+  if (int rc = reserve_lds(reinterpret_cast<const void*>(fill_kernel<TIER>), lds, "fill_kernel")) return rc;
   hipLaunchKernelGGL(fill_kernel<TIER>, dim3(n_items), dim3(FillShape<TIER>::threads), lds, s, a);
   return check_launch("fill_kernel");
 }
@@ -674,6 +711,7 @@ extern "C" int mom_fill_tier(int tier, int n_items, const void* args, void* stre
   // within the 160 KB of a CU
   if (a.acc_bytes < 0 || a.acc_bytes > 136 * 1024) return fail("mom_fill_tier: LDS accumulator out of range");
   if (!a.item_r) return fail("mom_fill_tier: null item bounds");
+  if (tier == 0 && (!a.near_static || !a.item_soff)) return fail("mom_fill_tier: near tier without its statics");
   hipStream_t s = (hipStream_t)stream;
   switch (tier) {
     case 0: return launch_fill<0>(n_items, a, s);
@@ -681,6 +719,17 @@ extern "C" int mom_fill_tier(int tier, int n_items, const void* args, void* stre
     case 2: return launch_fill<2>(n_items, a, s);
     default: return fail("mom_fill_tier: tier must be 0 (near), 1 (mid) or 2 (far)");
   }
+}
+
+extern "C" int mom_near_statics(int n_items, const void* args, const double* pair_dz, double* out, void* stream) {
+  if (n_items <= 0) return n_items == 0 ? 0 : fail("mom_near_statics: negative item count");
+  if (!args) return fail("mom_near_statics: null argument block");
+  FillArgs a = *reinterpret_cast<const FillArgs*>(args);
+  if (!a.items || !a.src || !a.V || !a.qp7 || !a.blk_tris || !a.item_soff || !pair_dz || !out)
+    return fail("mom_near_statics: null device pointer");
+  hipLaunchKernelGGL(near_static_kernel, dim3(n_items), dim3(kStaticThreads), 0, (hipStream_t)stream, a, pair_dz,
+                     out);
+  return check_launch("near_static_kernel");
 }
 
 extern "C" int mom_fill_args_size(void) { return (int)sizeof(FillArgs); }

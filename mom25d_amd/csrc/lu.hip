@@ -1250,20 +1250,12 @@ static int lookahead_for(hipStream_t s, size_t need, Lookahead** out) {
   return 0;
 }
 
-static size_t g_panel_lds_max = 0;
-
 template <typename R>
 static int launch_panel(cplx<R>* A, int64_t lda, int n, int k0, int b, int32_t* ipiv, int32_t* info,
                         PivotSlot* slots, cplx<R>* rowbuf, int gmax, hipStream_t s) {
   const int m = n - k0;
   const size_t lds_max = 152 * 1024;  // of the 160 KB per workgroup, beside the static rows
-  if (g_panel_lds_max == 0) {
-    MOM_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(panel_kernel<double>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max));
-    MOM_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(panel_kernel<float>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max));
-    g_panel_lds_max = lds_max;
-  }
+  if (int rc0 = reserve_lds(reinterpret_cast<const void*>(panel_kernel<R>), lds_max, "panel_kernel")) return rc0;
   // as few workgroups as the LDS allows (fewer slots to poll per column)
   int rp = std::max(1, (int)(lds_max / ((size_t)b * sizeof(cplx<R>))));
   int G = (m + rp - 1) / rp;
@@ -1370,14 +1362,8 @@ template <typename R>
 static int launch_panel_nopiv(cplx<R>* A, int64_t lda, int n, int k0, int b, int32_t* ipiv, int32_t* info,
                               hipStream_t s, bool sym = false) {
   if (b > kNoPivPanelMax<R>()) return fail("mom_getrf_nopiv: panel wider than the diagonal-block kernel");
-  static bool attr_set = false;
-  if (!attr_set) {
-    const size_t most = sizeof(cplx<R>) * (size_t)kNoPivPanelMax<R>() * kNoPivPanelMax<R>();
-    if (hipFuncSetAttribute((const void*)diag_lu_kernel<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)most) !=
-        hipSuccess)
-      return fail("diag_lu_kernel: cannot reserve LDS");
-    attr_set = true;
-  }
+  const size_t most = sizeof(cplx<R>) * (size_t)kNoPivPanelMax<R>() * kNoPivPanelMax<R>();
+  if (int rc0 = reserve_lds((const void*)diag_lu_kernel<R>, most, "diag_lu_kernel")) return rc0;
   hipLaunchKernelGGL(diag_lu_kernel<R>, dim3(1), dim3(1024), sizeof(cplx<R>) * (size_t)b * b, s, A, lda, k0, b, ipiv,
                      info, (int)sym);
   int rc = check_launch("diag_lu_kernel");
@@ -1678,37 +1664,19 @@ static int trtri_blocks(int n, const cplx<R>* LU, int64_t lda, void* work, hipSt
   const int nblk = (n + kTrsvBS - 1) / kTrsvBS;
   cplx<R>* D = reinterpret_cast<cplx<R>*>(reinterpret_cast<unsigned char*>(work) + getrs_work_bytes<R>(n));
   const size_t lds = 2 * sizeof(z64) * kTrsvBS * kTrsvBS;
-  static bool attr_set = false;
-  if (!attr_set) {
-    if (hipFuncSetAttribute((const void*)trtri_blocks_kernel<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds) != hipSuccess)
-      return fail("trtri_blocks_kernel: cannot reserve LDS");
-    attr_set = true;
-  }
+  if (int rc0 = reserve_lds((const void*)trtri_blocks_kernel<R>, lds, "trtri_blocks_kernel")) return rc0;
   hipLaunchKernelGGL(trtri_blocks_kernel<R>, dim3(nblk, 2), dim3(kTrsvBS), lds, s, n, LU, lda, D);
   int rc = check_launch("trtri_blocks_kernel");
   if (rc) return rc;
   const size_t ldsm = 2 * sizeof(cplx<R>) * kTrsvBS * kTrsvBS;
-  static bool attrm_set = false;
-  if (!attrm_set) {
-    if (hipFuncSetAttribute((const void*)chain_matrix_kernel<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)ldsm) != hipSuccess)
-      return fail("chain_matrix_kernel: cannot reserve LDS");
-    attrm_set = true;
-  }
+  if ((rc = reserve_lds((const void*)chain_matrix_kernel<R>, ldsm, "chain_matrix_kernel"))) return rc;
   hipLaunchKernelGGL(chain_matrix_kernel<R>, dim3(nblk, 2), dim3(256), ldsm, s, n, LU, lda, D, nblk,
                      chain_matrices<R>(work, n));
   if ((rc = check_launch("chain_matrix_kernel")) || !inv128<R>()) return rc;
   const int nblk2 = (n + kTrsvBS2 - 1) / kTrsvBS2;
   cplx<R>* D2 = D + (int64_t)2 * nblk * kTrsvBS * kTrsvBS;
   const size_t lds2 = 2 * sizeof(cplx<R>) * kTrsvBS * kTrsvBS;
-  static bool attr2_set = false;
-  if (!attr2_set) {
-    if (hipFuncSetAttribute((const void*)compose_inverse_kernel<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds2) != hipSuccess)
-      return fail("compose_inverse_kernel: cannot reserve LDS");
-    attr2_set = true;
-  }
+  if ((rc = reserve_lds((const void*)compose_inverse_kernel<R>, lds2, "compose_inverse_kernel"))) return rc;
   hipLaunchKernelGGL(compose_inverse_kernel<R>, dim3(nblk2, 2), dim3(256), lds2, s, n, LU, lda, D, nblk, D2);
   return check_launch("compose_inverse_kernel");
 }
@@ -2184,16 +2152,8 @@ static int sbsolve_setup(int n, const cplx<R>* LU, int64_t lda, int sb, void* wo
   cplx<R>* DU = reinterpret_cast<cplx<R>*>(w + L.du);
   cplx<R>* T = reinterpret_cast<cplx<R>*>(w + L.t);
   const size_t lds64 = 2 * sizeof(z64) * kTrsvBS * kTrsvBS;
-  static bool attr = false;
-  if (!attr) {
-    if (hipFuncSetAttribute((const void*)trtri_blocks_kernel<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds64) != hipSuccess)
-      return fail("trtri_blocks_kernel: cannot reserve LDS");
-    if (hipFuncSetAttribute((const void*)sb_inverse_kernel<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds64) != hipSuccess)
-      return fail("sb_inverse_kernel: cannot reserve LDS");
-    attr = true;
-  }
+  if (int rc0 = reserve_lds((const void*)trtri_blocks_kernel<R>, lds64, "trtri_blocks_kernel")) return rc0;
+  if (int rc0 = reserve_lds((const void*)sb_inverse_kernel<R>, lds64, "sb_inverse_kernel")) return rc0;
   hipLaunchKernelGGL(trtri_blocks_kernel<R>, dim3(L.nblk64, 2), dim3(kTrsvBS), lds64, s, n, LU, lda, D64);
   int rc = check_launch("trtri_blocks_kernel");
   if (rc) return rc;

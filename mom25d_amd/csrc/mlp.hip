@@ -378,17 +378,12 @@ __global__ __launch_bounds__(256) void forward_f32_kernel(int n, const float* __
   }
 }
 
-static bool g_attr = false;
-
 static int launch(bool train, int grid, const Args& a, hipStream_t s) {
   const size_t lds = sizeof(Smem);
-  if (!g_attr) {
-    MOM_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(mlp_kernel<true>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    MOM_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(mlp_kernel<false>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    g_attr = true;
-  }
+  if (int rc = mom::reserve_lds(train ? reinterpret_cast<const void*>(mlp_kernel<true>)
+                                      : reinterpret_cast<const void*>(mlp_kernel<false>),
+                                lds, "mlp_kernel"))
+    return rc;
   if (train)
     hipLaunchKernelGGL(mlp_kernel<true>, dim3(grid), dim3(256), lds, s, a);
   else

@@ -16,14 +16,33 @@ device.  The plan is then flattened into work items of <= 64 source
 triangles for the GPU fill, and uploaded once.
 """
 
+import ctypes
+
 import numpy as np
 import torch
 
+from . import _lib
 from .rules import TRI3_BARY, TRI3_W, TRI7_BARY, TRI7_W, quad_points
 
 BLOCK = 32
 CHUNK = 64
 NEAR, MID, FAR = 0, 1, 2
+
+_PTR_FIELDS = ("V", "qp7", "qw7", "qp3", "qw3", "cen", "area", "cA", "cV", "tb", "pad_pos", "blk_tris", "items",
+               "src")
+
+
+class FillArgs(ctypes.Structure):
+    """Mirror of the FillArgs block of csrc/fill.hip (size checked at first use)."""
+
+    _fields_ = [(n, ctypes.c_void_p) for n in _PTR_FIELDS] + [
+        ("nt", ctypes.c_int32), ("pairs", ctypes.c_void_p), ("tab_x", ctypes.c_void_p), ("tab_c", ctypes.c_void_p),
+        ("tab_b", ctypes.c_void_p), ("jwmu_re", ctypes.c_double), ("jwmu_im", ctypes.c_double),
+        ("ijwe_re", ctypes.c_double), ("ijwe_im", ctypes.c_double), ("Z", ctypes.c_void_p), ("ldz", ctypes.c_int64),
+        ("IVpad", ctypes.c_void_p), ("blk_row_off", ctypes.c_void_p), ("blk_rows", ctypes.c_void_p),
+        ("blk_rloc", ctypes.c_void_p), ("item_cols", ctypes.c_void_p), ("src_cloc", ctypes.c_void_p),
+        ("acc_bytes", ctypes.c_int64), ("item_r", ctypes.c_void_p), ("stage_max", ctypes.c_int32),
+        ("pad2_", ctypes.c_int32), ("near_static", ctypes.c_void_p), ("item_soff", ctypes.c_void_p)]
 
 
 def morton_order(xy):
@@ -120,6 +139,12 @@ class FillPlan:
         self.n_levels = nl
         self._scatter_maps(blocks, per_tier, block)
         self._item_bounds(blocks)
+        # near work items: offsets of their closed-form statics (test triangles x 7 points x 3 values x sources)
+        it = self.items[NEAR].astype(np.int64).reshape(-1, 6)
+        ntest = np.array([pb.size for pb in blocks], np.int64)[it[:, 0]]
+        size = 21 * ntest * it[:, 3]
+        self.item_soff = np.concatenate([[0], np.cumsum(size)[:-1]]).astype(np.int64)
+        self.near_static_len = int(size.sum())
 
         # conductor-loss Gram matrix per triangle, times cA_i cA_j
         X = self.qp7[:, None, :, :] - V[:, :, None, :]
@@ -260,7 +285,7 @@ class FillPlan:
             V=f64(self.V), qp7=f64(self.qp7), qw7=f64(self.qw7), qp3=f64(self.qp3), qw3=f64(self.qw3),
             cen=f64(self.cen), area=f64(self.area), cA=f64(self.cA), cV=f64(self.cV), tb=i32(self.tb),
             pad_pos=i32(self.pad_pos), blk_tris=i32(self.blk_tris), src=i32(self.src if self.src.size else [0]),
-            gram=f64(self.gram), tp=i32(m.tp), vp=i32(m.vp), tm=i32(m.tm), vm=i32(m.vm),
+            gram=f64(self.gram), tp=i32(m.tp), vp=i32(m.vp), tm=i32(m.tm), vm=i32(m.vm), tri_layer=i32(m.tri_layer),
         )
         for t, it in self.items.items():
             d[f"items{t}"] = i32(it if it.size else np.zeros((1, 6)))
@@ -271,8 +296,29 @@ class FillPlan:
         if self.nv:
             d.update(d_off=i32(self.d_off), d_pad=i32(self.d_pad), d_val=f64(self.d_val),
                      pad_tri=i32(self.pad_tris))
+        d["item_soff"] = torch.from_numpy(self.item_soff if self.item_soff.size else np.zeros(1, np.int64)).to(dev)
+        d["near_static"] = torch.empty(max(self.near_static_len, 1), dtype=torch.float64, device=dev)
+        n_near = int(self.items[NEAR].shape[0])
+        if n_near:
+            a = self.args(d)
+            a.items = d["items0"].data_ptr()
+            pair_dz = f64(np.abs(self.zl[:, None] - self.zl[None, :]).ravel())
+            with torch.cuda.device(dev):
+                _lib.call("mom_near_statics", n_near, ctypes.byref(a), pair_dz.data_ptr(), d["near_static"].data_ptr(),
+                          _lib.stream_ptr())
+                torch.cuda.current_stream().synchronize()
         self.dev = d
         return d
+
+    def args(self, d):
+        """A FillArgs block with the plan's device arrays `d` (upload()) set;
+        the caller adds the frequency data, Z and the work items of a tier."""
+        a = FillArgs()
+        for name in ("V", "qp7", "qw7", "qp3", "qw3", "cen", "area", "cA", "cV", "tb", "pad_pos", "blk_tris", "src",
+                     "blk_row_off", "blk_rows", "blk_rloc", "item_cols", "src_cloc", "near_static", "item_soff"):
+            setattr(a, name, d[name].data_ptr())
+        a.nt = self.nt
+        return a
 
     # Algorithmic FP64 flops per triangle pair, counted from the pair
     # kernel's loop body with sqrt and divide as one flop each: per test point

@@ -30,6 +30,7 @@ from . import _lib, analytic
 from .constants import C0, EPS0, MU0
 from .deembed import VOLTAGE_WEIGHT, FitContext, fit_waves, reciprocal_z0, sparams_point
 from .greens import SweepTables
+from .plan import FillArgs as _FillArgs
 from .plan import FillPlan
 
 
@@ -145,7 +146,11 @@ def via_impedance(vias, spans, omega, sections, cache=None):
     sum.  sections: [(z_bottom, z_top, k_tem or None)].  cache: a dict the
     caller keeps for one set of vias and spans, holding the
     frequency-independent inductances of the open sections."""
-    dist = _via_geometry(vias)
+    dist = None if cache is None else cache.get("dist")
+    if dist is None:
+        dist = _via_geometry(vias)
+        if cache is not None:
+            cache["dist"] = dist
     spans = np.asarray(spans, float).reshape(-1, 2)
     Z = np.zeros((len(spans), len(spans)), complex)
     for zb, zt, k in sections:
@@ -155,18 +160,18 @@ def via_impedance(vias, spans, omega, sections, cache=None):
         inside = np.flatnonzero(hi - lo > 1e-12 * height)
         if inside.size == 0:
             continue
-        r = dist[np.ix_(inside, inside)]
+        sel = slice(None) if inside.size == len(spans) else np.ix_(inside, inside)
         a1, a2 = lo[inside], hi[inside]
         if k is None:
             key = (zb, zt)
             L = None if cache is None else cache.get(key)
             if L is None:
-                L = _with_image(a1[:, None], a2[:, None], a1[None, :], a2[None, :], r)
+                L = _with_image(a1[:, None], a2[:, None], a1[None, :], a2[None, :], dist[sel])
                 if cache is not None:
                     cache[key] = L
-            Z[np.ix_(inside, inside)] += 1j * omega * L
+            Z[sel] += (1j * omega) * L
         else:
-            Z[np.ix_(inside, inside)] += _parallel_plate_z(a1, a2, r, height, k, omega)
+            Z[sel] += _parallel_plate_z(a1, a2, dist[sel], height, k, omega)
     return Z
 
 
@@ -316,23 +321,6 @@ class SweepResult:
             for f, Sf in zip(self.freqs, S):
                 vals = Sf.T.ravel() if n == 2 else Sf.ravel()
                 fh.write(f"{f:.9e} " + " ".join(f"{v.real:.9e} {v.imag:.9e}" for v in vals) + "\n")
-
-
-_PTR_FIELDS = ("V", "qp7", "qw7", "qp3", "qw3", "cen", "area", "cA", "cV", "tb", "pad_pos", "blk_tris", "items",
-               "src")
-
-
-class _FillArgs(ctypes.Structure):
-    """Mirror of the FillArgs block of csrc/fill.hip (size checked at first use)."""
-
-    _fields_ = [(n, ctypes.c_void_p) for n in _PTR_FIELDS] + [
-        ("nt", ctypes.c_int32), ("pairs", ctypes.c_void_p), ("tab_x", ctypes.c_void_p), ("tab_c", ctypes.c_void_p),
-        ("tab_b", ctypes.c_void_p), ("jwmu_re", ctypes.c_double), ("jwmu_im", ctypes.c_double),
-        ("ijwe_re", ctypes.c_double), ("ijwe_im", ctypes.c_double), ("Z", ctypes.c_void_p), ("ldz", ctypes.c_int64),
-        ("IVpad", ctypes.c_void_p), ("blk_row_off", ctypes.c_void_p), ("blk_rows", ctypes.c_void_p),
-        ("blk_rloc", ctypes.c_void_p), ("item_cols", ctypes.c_void_p), ("src_cloc", ctypes.c_void_p),
-        ("acc_bytes", ctypes.c_int64), ("item_r", ctypes.c_void_p), ("stage_max", ctypes.c_int32),
-        ("pad2_", ctypes.c_int32)]
 
 
 def _check_abi():
@@ -649,6 +637,29 @@ class Solver:
             self._bufs[key] = b
         return b
 
+    def _upload(self, name, a, lane=0):
+        """A host array in the lane's device buffer `name`, enqueued on the
+        current stream without waiting for the stream's earlier work (as a
+        copy from pageable memory would): staged through a pinned buffer of
+        the lane, whose previous copy is waited for before it is overwritten
+        (long done when the lane comes back to it)."""
+        a = np.ascontiguousarray(a)
+        dt = torch.from_numpy(a[:0]).dtype
+        key = ("stage", name, lane)
+        h, ev = self._bufs.get(key, (None, None))
+        if h is None or h.numel() < a.nbytes:
+            h, ev = torch.empty(max(a.nbytes, 1 << 12), dtype=torch.uint8, pin_memory=True), None
+        if ev is not None:
+            ev.synchronize()
+        hv = h[:a.nbytes]
+        hv.numpy()[:] = a.reshape(-1).view(np.uint8)
+        d = self._buf(name, a.shape, dt, lane)
+        d.view(torch.uint8).view(-1).copy_(hv, non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record()
+        self._bufs[key] = (h, ev)
+        return d
+
     PANEL_LDS = 152 * 1024  # LDS of one LU panel workgroup (the library's bound)
     # GMRES products from the tiles on and above the diagonal of Z (mom_scaled_symv):
     # valid because every Z the solver builds is symmetric (the fill ends with
@@ -665,7 +676,7 @@ class Solver:
         """Processes for the wave fits of a sweep (0: in this process)."""
         if self.workers is not None:
             return max(0, int(self.workers))
-        return min(8, max(1, (os.cpu_count() or 2) // 2)) if nf >= 16 else 0
+        return min(8, max(1, (os.cpu_count() or 2) // 2)) if nf >= 4 else 0
 
     def _lane_count(self):
         """Lanes of a sweep: as asked (default 4 up to N = 4000, else 3 in
@@ -755,11 +766,7 @@ class Solver:
             ivpad.zero_()
         P = _lib.ptr
         tabs = g.tables
-        a = _FillArgs()
-        for name in ("V", "qp7", "qw7", "qp3", "qw3", "cen", "area", "cA", "cV", "tb", "pad_pos", "blk_tris",
-                     "src"):
-            setattr(a, name, d[name].data_ptr())
-        a.nt = plan.nt
+        a = plan.args(d)
         a.pairs = g.pairs_dev.data_ptr()
         a.tab_x, a.tab_c, a.tab_b = tabs.tab_x.data_ptr(), tabs.tab_c.data_ptr(), tabs.tab_b.data_ptr()
         a.jwmu_re, a.jwmu_im = jwmu.real, jwmu.imag
@@ -767,8 +774,6 @@ class Solver:
         a.Z = Z.data_ptr()
         a.ldz = Z.stride(0)
         a.IVpad = ivpad.data_ptr() if ivpad is not None else None
-        for name in ("blk_row_off", "blk_rows", "blk_rloc", "item_cols", "src_cloc"):
-            setattr(a, name, d[name].data_ptr())
         stream = _lib.stream_ptr()
         for tier in (0, 1, 2):
             items = plan.items[tier]
@@ -784,7 +789,7 @@ class Solver:
             _lib.call("mom_gram_add", plan.nt, P(d["gram"]), P(d["tb"]), float(np.real(zs)), float(np.imag(zs)),
                       P(Z), Z.stride(0), stream)
         if nv:
-            E = torch.from_numpy(self._via_extra(g, zs)).to(self.device)
+            E = self._upload("via_extra", self._via_extra(g, zs), lane)
             _lib.call("mom_via_blocks", plan.nb, nv, plan.nt, P(ivpad), P(d["d_off"]), P(d["d_pad"]), P(d["d_val"]),
                       P(d["pad_tri"]), P(d["tp"]), P(d["vp"]), P(d["tm"]), P(d["vm"]), P(d["cV"]), P(E),
                       ijwe.real, ijwe.imag, P(Z), Z.stride(0), stream)
@@ -917,7 +922,7 @@ class Solver:
             else:
                 calls()
         else:
-            calls()
+            yield from _on_launcher(calls, self.device)
         if Z32 is None:
             self.last_refine = 0
             return X
@@ -1091,7 +1096,7 @@ class Solver:
             y = np.zeros((k, nrhs), complex)
             for c in range(nrhs):
                 y[:, c] = np.linalg.solve(np.triu(H[:k, :k, c]), g[:k, c])
-            yd = torch.from_numpy(-y).to(self.device)
+            yd = self._upload("gmres_y", -y, lane)
             _lib.call("mom_basis_sub", n, nrhs, k, P(Wb), P(yd), P(Y), stream)
             R = residual(Y)
         else:
@@ -1280,12 +1285,12 @@ class Solver:
         # a band where complex64 factors precondition poorly: complex128 anchor
         if not filled:
             fill(None)
-        pc = self._lu64_factor(Z, lane)
+        pc = yield from self._lu64_factor(Z, lane)
         X = yield from self._gmres_chunks(Z, pc, B, lane, max_its=self.NOPIV_CAP if self.nopiv else None)
         steps.append(("lu64", self.last_refine))
         if X is None:  # unpivoted factors too poor here: factor again with pivoting
             self.lu_stats["pivoted_retries"] += 1
-            pc = self._lu64_factor(Z, lane, pivot=True)
+            pc = yield from self._lu64_factor(Z, lane, pivot=True)
             X = yield from self._gmres_chunks(Z, pc, B, lane)
             steps.append(("lu64-pivoted", self.last_refine))
         self._remember(lane, g.freq, X)
@@ -1329,7 +1334,7 @@ class Solver:
                                   last=None)
 
     def _lu64_factor(self, Z, lane, pivot=False):
-        """complex128 LU of D Z D (D = diag(|Z_ii|^-1/2), into the lane's
+        """Generator: complex128 LU of D Z D (D = diag(|Z_ii|^-1/2), into the lane's
         dscale) with the solve data of its factors (super-block inverses
         and scaled couplings, or the 64-row inverted blocks), enqueued on the
         current stream; returns the Precond of the mixed path's GMRES.  Z
@@ -1360,15 +1365,19 @@ class Solver:
             sw = self._buf("sbwork64", (swb,), torch.uint8, lane, zero=True)
         else:
             sw = self._buf("swork64i", (int(L.mom_getrs_inv_work_bytes(64, n)),), torch.uint8, lane, zero=True)
-        if nopiv:
-            _lib.call(self._nopiv_entry, 64, n, P(A), A.stride(0), P(ipiv), P(info), P(work), wbytes, jb, stream)
-        else:
-            _lib.call("mom_getrf", 64, n, P(A), A.stride(0), P(ipiv), P(info), P(work), wbytes, jb, groups, stream)
-        _lib.call("mom_build_perm", n, P(ipiv), P(perm), stream)
-        if sb:
-            _lib.call("mom_sbsolve_setup", 64, n, P(A), A.stride(0), sb, P(sw), swb, stream)
-        else:
-            _lib.call("mom_trtri_blocks", 64, n, P(A), A.stride(0), P(sw), stream)
+        def calls():
+            if nopiv:
+                _lib.call(self._nopiv_entry, 64, n, P(A), A.stride(0), P(ipiv), P(info), P(work), wbytes, jb, stream)
+            else:
+                _lib.call("mom_getrf", 64, n, P(A), A.stride(0), P(ipiv), P(info), P(work), wbytes, jb, groups,
+                          stream)
+            _lib.call("mom_build_perm", n, P(ipiv), P(perm), stream)
+            if sb:
+                _lib.call("mom_sbsolve_setup", 64, n, P(A), A.stride(0), sb, P(sw), swb, stream)
+            else:
+                _lib.call("mom_trtri_blocks", 64, n, P(A), A.stride(0), P(sw), stream)
+
+        yield from _on_launcher(calls, self.device)
         self.lu_stats["fp64_lu"] += 1
         return Precond(64, A, perm, sw, sb)
 
@@ -1479,26 +1488,30 @@ class Solver:
     def line_voltage(self, greens, q, pts, layer="top"):
         """Voltage to the ground plane at points of a metal layer from the
         triangle charges q (Nt, Nexc) (GPU kernel)."""
+        return torch.cat(self._line_voltage_dev(greens, q, pts, layer), 1).cpu().numpy()
+
+    def _line_voltage_dev(self, greens, q, pts, layer, lane=0):
+        """line_voltage enqueued on the current stream: the device results,
+        in column groups of at most 4 (the kernel's limit); the inputs go up
+        through pinned memory, so nothing here waits for the stream."""
         m = self.mesh
         plan = self.geometry()
         d = plan.upload(self.device)
         li = m.layer_names.index(layer)
         tabs = greens.tables
         pts = np.ascontiguousarray(pts, float)
-        qd = torch.from_numpy(np.ascontiguousarray(q, complex)).to(self.device)
-        pd = torch.from_numpy(pts).to(self.device)
-        tl = self._buf("tri_layer", (plan.nt,), torch.int32)
-        tl.copy_(torch.from_numpy(m.tri_layer.astype(np.int32)))
+        qd = self._upload("charges", np.ascontiguousarray(q, complex), lane)
+        pd = self._upload("vpts", pts, lane)
         P = _lib.ptr
-        cols = []
+        outs = []
         for c0, c1 in _column_chunks(q.shape[1]):  # the kernel takes up to 4 columns
             qc = qd if q.shape[1] <= 4 else qd[:, c0:c1].contiguous()
             out = torch.empty((pts.shape[0], c1 - c0), dtype=torch.complex128, device=self.device)
-            _lib.call("mom_line_voltage", pts.shape[0], P(pd), li, len(m.layer_names), plan.nt, c1 - c0, P(tl),
-                      P(d["V"]), P(d["qp7"]), P(d["qw7"]), P(greens.pairs_dev), P(tabs.tab_x), P(tabs.tab_cw),
-                      P(tabs.tab_b), P(qc), 1.0 / EPS0, P(out), _lib.stream_ptr())
-            cols.append(out.cpu().numpy())
-        return np.hstack(cols)
+            _lib.call("mom_line_voltage", pts.shape[0], P(pd), li, len(m.layer_names), plan.nt, c1 - c0,
+                      P(d["tri_layer"]), P(d["V"]), P(d["qp7"]), P(d["qw7"]), P(greens.pairs_dev), P(tabs.tab_x),
+                      P(tabs.tab_cw), P(tabs.tab_b), P(qc), 1.0 / EPS0, P(out), _lib.stream_ptr())
+            outs.append(out)
+        return outs
 
     def _basis_positions(self):
         """Centres of the basis functions (N x 3): RWG edge midpoints on their
@@ -1535,12 +1548,23 @@ class Solver:
     def _port_iv(self, g, x):
         """Currents and voltages at every port's sample cuts (the voltages
         from the charges through the line-voltage kernel)."""
+        return _drain(self._port_iv_task(g, x))
+
+    def _port_iv_task(self, g, x, lane=0):
+        """Generator form of _port_iv: yields while the line-voltage kernels
+        run (the lane scheduler feeds the other lanes meanwhile)."""
         npt = len(self.ports)
         q = self.charges(x, g.omega)
+        layers = list(dict.fromkeys(p.layer for p in self.ports))
+        groups = [[p for p in range(npt) if self.ports[p].layer == lay] for lay in layers]
+        outs = [self._line_voltage_dev(g, q, np.concatenate([self.pm[p].vpts for p in ps]), lay, lane)
+                for lay, ps in zip(layers, groups)]
+        host = yield from _host(*[o for grp in outs for o in grp])
         phi = [None] * npt
-        for lay in dict.fromkeys(p.layer for p in self.ports):
-            ps = [p for p in range(npt) if self.ports[p].layer == lay]
-            vp = self.line_voltage(g, q, np.concatenate([self.pm[p].vpts for p in ps]), lay)
+        k = 0
+        for ps, grp in zip(groups, outs):
+            vp = np.hstack(host[k:k + len(grp)])
+            k += len(grp)
             for p, v in zip(ps, np.split(vp, np.cumsum([len(self.pm[p].vpts) for p in ps])[:-1])):
                 phi[p] = v
         I = [self.pm[p].currents(x) for p in range(npt)]
@@ -1658,6 +1682,40 @@ def _host(*tensors):
     while not ev.query():
         yield
     return [h.numpy().copy() for h in bufs]
+
+
+_LAUNCHER = None
+
+
+def _on_launcher(fn, device):
+    """Generator: fn() -- library calls that only enqueue work on the streams
+    they are given, such as a factorisation's several hundred launches -- on
+    a launcher thread (ctypes releases the GIL), yielding until it has
+    returned, so that the lane scheduler keeps feeding the other lanes
+    meanwhile.  The caller enqueues nothing more on those streams until then."""
+    global _LAUNCHER
+    if _LAUNCHER is None:
+        from concurrent.futures import ThreadPoolExecutor
+
+        _LAUNCHER = ThreadPoolExecutor(max_workers=8, thread_name_prefix="mom-launch")
+
+    def run():
+        with torch.cuda.device(device):
+            return fn()
+
+    fut = _LAUNCHER.submit(run)
+    while not fut.done():
+        yield
+    return fut.result()
+
+
+def _drain(gen):
+    """Run a generator task to its end (no other task to switch to)."""
+    try:
+        while True:
+            next(gen)
+    except StopIteration as e:
+        return e.value
 
 
 def _column_chunks(n, width=4):
@@ -1878,8 +1936,8 @@ def _sweep_many(jobs, lanes, workers, keep_currents=False, graphs=False, on_poin
         s._check_solve(lane, status)
         if keep_currents:
             o["cur"][fi] = x
+        I, V = yield from s._port_iv_task(g, x, lane)
         h0 = time.perf_counter()
-        I, V = s._port_iv(g, x)
         Ii = s._internal_currents(x)
         if pool is None:
             row = sparams_point(o["ctx"], f, I, V, Ii)
