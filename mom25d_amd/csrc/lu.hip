@@ -23,6 +23,8 @@
 
 #include <algorithm>
 #include <map>
+#include <type_traits>
+#include <utility>
 #include <mutex>
 #include <vector>
 
@@ -794,6 +796,183 @@ __device__ __forceinline__ float lane_value(float v, int k) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
 }
 
+template <typename R>
+__device__ __forceinline__ cplx<R> lane_value(cplx<R> v, int k) {
+  return {lane_value(v.re, k), lane_value(v.im, k)};
+}
+
+// a - x y in four fused multiply-adds
+template <typename R>
+__device__ __forceinline__ cplx<R> fms(cplx<R> a, cplx<R> x, cplx<R> y) {
+  return {fma(-x.re, y.re, fma(x.im, y.im, a.re)), fma(-x.re, y.im, fma(-x.im, y.re, a.im))};
+}
+
+// ------------------------------------ unpivoted panel, one row per lane
+// The unpivoted panel of at most 64 columns with its rows in registers:
+// lane i of every wavefront holds row i, wavefront w the columns
+// [w C, w C + C) of it (C = 64 / W: W = 1 for complex64, 2 for complex128,
+// whose 64 columns would not fit one lane's registers).  A step reads the
+// pivot row's values from lane j through the scalar unit (v_readlane), so
+// only the multipliers of the step's column cross wavefronts (through LDS,
+// one barrier per column; none for W = 1).  Against the block-in-LDS
+// kernels above, a 64-column panel of the 12 428-unknown preconditioner
+// takes a fraction of the time; both stay on the latency-bound chain of the
+// factorisation, whose late panels no longer hide behind the update GEMM.
+
+// f(integral_constant<int, 0>) ... f(integral_constant<int, N - 1>), each
+// call inlined with its index a compile-time constant (register arrays stay
+// in registers: no loop the unroller might keep rolled).
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// The b x b diagonal block (b <= 64; rows and columns past b act as the
+// identity) factored in place, A = L U, by one workgroup of 64 W threads.
+// sym: the block of a complex-symmetric matrix whose updates reached only its
+// lower triangle; its upper one is read mirrored.  The body is instantiated
+// per wavefront (WV), so that every column index is a compile-time constant.
+template <typename R, int W, int WV>
+__device__ __forceinline__ void diag_rows_body(cplx<R>* A, int64_t lda, int k0, int b, int32_t* ipiv, int32_t* info,
+                                               int sym, cplx<R> (*lx)[64]) {
+  constexpr int B = 64, C = B / W, C0 = WV * C;
+  const int i = threadIdx.x & 63;
+  constexpr cplx<R> zero = {(R)0, (R)0}, one = {(R)1, (R)0};
+  cplx<R> a[C];
+  static_for<C>([&](auto cc) {
+    const int c = C0 + cc;
+    cplx<R> v = {(R)(i == c), (R)0};
+    if (i < b && c < b) {
+      const bool up = sym && c > i;
+      v = A[(int64_t)(k0 + (up ? c : i)) * lda + k0 + (up ? i : c)];
+    }
+    a[cc] = v;
+  });
+  if (WV == 0 && i < b) ipiv[k0 + i] = k0 + i;
+  static_for<B>([&](auto jc) {
+    constexpr int j = jc;
+    cplx<R> l = zero;
+    if constexpr (j / C == WV) {
+      constexpr int jj = j % C;
+      const cplx<R> p = lane_value(a[jj], j);  // u_jj, final since step j - 1
+      if (i == 0 && j < b && p.re == (R)0 && p.im == (R)0 && info[0] == 0) info[0] = k0 + j + 1;
+      if (i > j) {
+        l = a[jj] * cdiv(one, nonzero_pivot(p));
+        a[jj] = l;
+      } else if (i == j) {
+        a[jj] = nonzero_pivot(p);
+      }
+      if constexpr (W > 1) lx[j & 1][i] = l;
+    }
+    if constexpr (W > 1) {
+      __syncthreads();
+      if constexpr (j / C != WV) l = lx[j & 1][i];
+    }
+    if (i > j) {
+      static_for<C>([&](auto cc) {
+        if constexpr (C0 + cc > j) a[cc] = fms(a[cc], l, lane_value(a[cc], j));
+      });
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  });
+  static_for<C>([&](auto cc) {
+    const int c = C0 + cc;
+    if (i < b && c < b) A[(int64_t)(k0 + i) * lda + k0 + c] = a[cc];
+  });
+}
+
+template <typename R, int W>
+__global__ __launch_bounds__(64 * W) void diag_rows_kernel(cplx<R>* A, int64_t lda, int k0, int b, int32_t* ipiv,
+                                                         int32_t* info, int sym) {
+  // on the factorisation's critical path, beside the look-ahead's update
+  // GEMM on the same CUs: its instructions go first
+  __builtin_amdgcn_s_setprio(3);
+  __shared__ cplx<R> lx[2][64];
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  static_for<W>([&](auto wv) {
+    if (w == wv) diag_rows_body<R, W, wv>(A, lda, k0, b, ipiv, info, sym, lx);
+  });
+}
+
+// L21 = A21 U11^-1 for the rows [k0 + b, n) below a factored diagonal block
+// (b <= 64): 64 rows per workgroup of 64 W threads, row i in lane i as in
+// diag_rows_kernel; U11 and the reciprocals of its diagonal are staged in LDS
+// and read at uniform addresses (broadcast).  Column c is finished in step c,
+// x_c = a_c / u_cc, and the columns after it updated, a_k -= x_c u_ck, in
+// groups of 16 whose reads of U stay next to their use.
+template <typename R, int W, int WV>
+__device__ __forceinline__ void below_rows_body(cplx<R>* A, int64_t lda, int n, int k0, int b, const cplx<R>* U,
+                                                const cplx<R>* dinv, cplx<R> (*lx)[64]) {
+  constexpr int B = 64, C = B / W, C0 = WV * C, G = 16;
+  const int i = threadIdx.x & 63;
+  constexpr cplx<R> zero = {(R)0, (R)0};
+  const int r = k0 + b + blockIdx.x * B + i;
+  const bool row = r < n;
+  cplx<R> a[C];
+  static_for<C>([&](auto cc) {
+    const int c = C0 + cc;
+    cplx<R> v = zero;
+    if (row && c < b) v = A[(int64_t)r * lda + k0 + c];
+    a[cc] = v;
+  });
+  static_for<B>([&](auto cs) {
+    constexpr int c = cs;
+    cplx<R> x = zero;
+    if constexpr (c / C == WV) {
+      constexpr int cl = c % C;
+      x = a[cl] * dinv[c];
+      a[cl] = x;
+      if constexpr (W > 1) lx[c & 1][i] = x;
+    }
+    if constexpr (W > 1) {
+      __syncthreads();
+      if constexpr (c / C != WV) x = lx[c & 1][i];
+    }
+    static_for<(C + G - 1) / G>([&](auto g) {
+      // an offset the compiler cannot see to be zero, so that the group's
+      // reads of U stay behind it (hoisted, as invariant loads, they would
+      // all be live at once)
+      int z;
+      asm volatile("s_mov_b32 %0, 0" : "=s"(z));
+      static_for<G>([&](auto e) {
+        constexpr int cc = g * G + e;
+        if constexpr (cc < C && C0 + cc > c) a[cc] = fms(a[cc], x, U[c * B + C0 + cc + z]);
+      });
+      __builtin_amdgcn_sched_barrier(0);
+    });
+  });
+  static_for<C>([&](auto cc) {
+    const int c = C0 + cc;
+    if (row && c < b) A[(int64_t)r * lda + k0 + c] = a[cc];
+  });
+}
+
+template <typename R, int W>
+__global__ __launch_bounds__(64 * W) void below_rows_kernel(cplx<R>* A, int64_t lda, int n, int k0, int b) {
+  constexpr int B = 64;
+  extern __shared__ unsigned char smem_raw[];
+  cplx<R>* U = reinterpret_cast<cplx<R>*>(smem_raw);  // [B][B]
+  cplx<R>* dinv = U + B * B;                           // [B]
+  __shared__ cplx<R> lx[2][B];
+  __builtin_amdgcn_s_setprio(3);  // on the critical path, as diag_rows_kernel
+  const int tid = threadIdx.x;
+  for (int e = tid; e < B * B; e += 64 * W) {
+    const int r = e / B, c = e % B;
+    U[e] = (r < b && c < b) ? A[(int64_t)(k0 + r) * lda + k0 + c] : cplx<R>{(R)(r == c), (R)0};
+  }
+  __syncthreads();
+  for (int c = tid; c < B; c += 64 * W) dinv[c] = cdiv(cplx<R>{(R)1, (R)0}, nonzero_pivot(U[c * B + c]));
+  __syncthreads();
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  static_for<W>([&](auto wv) {
+    if (w == wv) below_rows_body<R, W, wv>(A, lda, n, k0, b, U, dinv, lx);
+  });
+}
+
 // Set-up of a solve in one launch: the two block tickets cleared (words 0
 // and 2), X = P B (rows gathered by perm, n x nrhs, rounded to R) and Y (the
 // lower solve's output) filled with the sentinel.  The failure word (word 4)
@@ -1362,6 +1541,20 @@ template <typename R>
 static int launch_panel_nopiv(cplx<R>* A, int64_t lda, int n, int k0, int b, int32_t* ipiv, int32_t* info,
                               hipStream_t s, bool sym = false) {
   if (b > kNoPivPanelMax<R>()) return fail("mom_getrf_nopiv: panel wider than the diagonal-block kernel");
+  if (b <= 64) {  // rows in registers
+    // wavefronts sharing a row: its 64 columns (and, below, a row of U11 per
+    // step) within one lane's registers
+    constexpr int WD = sizeof(R) == sizeof(double) ? 2 : 1, WB = 2;
+    hipLaunchKernelGGL((diag_rows_kernel<R, WD>), dim3(1), dim3(64 * WD), 0, s, A, lda, k0, b, ipiv, info, (int)sym);
+    int rc = check_launch("diag_rows_kernel");
+    if (rc) return rc;
+    const int below = n - k0 - b;
+    if (below <= 0) return 0;
+    const size_t lds = sizeof(cplx<R>) * (64 * 64 + 64);
+    if ((rc = reserve_lds((const void*)below_rows_kernel<R, WB>, lds, "below_rows_kernel"))) return rc;
+    hipLaunchKernelGGL((below_rows_kernel<R, WB>), dim3((below + 63) / 64), dim3(64 * WB), lds, s, A, lda, n, k0, b);
+    return check_launch("below_rows_kernel");
+  }
   const size_t most = sizeof(cplx<R>) * (size_t)kNoPivPanelMax<R>() * kNoPivPanelMax<R>();
   if (int rc0 = reserve_lds((const void*)diag_lu_kernel<R>, most, "diag_lu_kernel")) return rc0;
   hipLaunchKernelGGL(diag_lu_kernel<R>, dim3(1), dim3(1024), sizeof(cplx<R>) * (size_t)b * b, s, A, lda, k0, b, ipiv,
