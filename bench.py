@@ -45,9 +45,51 @@ def _cpu_model():
     return platform.processor() or "unknown"
 
 
+# kernels timed with HIP events inside the timed sweep (library kinds of mom_timing_begin):
+# kind -> (kernel, bound, peak, unit of achieved)
+TIMED = {
+    0: ("cgemm_sub_kernel<float, {bm}, {bn}> (complex64 LU trailing update)", "mfma", 157.3e12, "TFLOP/s"),
+    4: ("cgemm_sub_kernel<double, {bm}, {bn}> (complex128 LU trailing update)", "mfma", 78.6e12, "TFLOP/s"),
+    1: ("symv_strip_kernel<2, 2, 8> (GMRES product D Z D x, symmetric half of Z)", "hbm", 8e12, "GB/s"),
+    2: ("sb_chain_kernel<float, 2, *> (complex64 preconditioner, one triangular chain)", "hbm", 8e12, "GB/s"),
+    3: ("sb_chain_kernel<double, 2, *> (complex128 preconditioner, one triangular chain)", "hbm", 8e12, "GB/s"),
+}
+FILL_KERNEL = "fill_kernel<{t}> (FP64 fill, {name} tier)"
+
+
+def kernel_table(launch_stats, fill_events, tile):
+    """Per timed kernel: launches, summed and average duration, algorithmic
+    work per launch, achieved rate and fraction of its peak (fill tiers
+    against the FP64 vector peak)."""
+    rows = []
+    for kind, (n, ms, work) in launch_stats.items():
+        if n <= 0:
+            continue
+        name, bound, peak, unit = TIMED[kind]
+        rows.append(dict(kernel=name.format(bm=tile[0], bn=tile[1]), bound=bound, launches=int(n), total_ms=ms,
+                         avg_launch_ms=ms / n, work_per_launch=work / n, achieved=work / (ms * 1e-3), peak=peak,
+                         unit=unit))
+    by_tier = {}
+    for tier, e0, e1, flops in fill_events:
+        b = by_tier.setdefault(tier, [0, 0.0, 0.0])
+        b[0] += 1
+        b[1] += e0.elapsed_time(e1)
+        b[2] += flops
+    for tier, (n, ms, work) in by_tier.items():
+        rows.append(dict(kernel=FILL_KERNEL.format(t=tier, name=("near", "mid", "far")[tier]), bound="valu (fp64)",
+                         launches=n, total_ms=ms, avg_launch_ms=ms / n, work_per_launch=work / n,
+                         achieved=work / (ms * 1e-3), peak=PEAK["fp64_valu"], unit="TFLOP/s"))
+    for r in rows:
+        r["frac"] = r["achieved"] / r["peak"]
+    return sorted(rows, key=lambda r: -r["total_ms"])
+
+
 def time_path(mesh, sub, freqs, precision, warmup, steps, torch, lanes=None, autotune=True, panel=None, tile=None,
-              block=None, interp=None, **kw):
-    from mom25d_amd import Solver
+              block=None, interp=None, timing=False, **kw):
+    """Solver set-up, warm-up sweep, then the timed sweep; timing: HIP
+    events around the launches of the kernels in TIMED and of the fill tiers
+    during the timed sweep (solver.kernel_stats, solver.fill_events)."""
+    from mom25d_amd import Solver, _lib
 
     s = Solver(mesh, sub, verbose=False, precision=precision, lanes=lanes, autotune=autotune and tile is None,
                panel=panel, **kw)
@@ -61,11 +103,21 @@ def time_path(mesh, sub, freqs, precision, warmup, steps, torch, lanes=None, aut
         s.sweep(sweep_points(freqs, warmup), interp=None)
     s.lu_stats = {k: 0 for k in s.lu_stats}
     s.point_log = []
+    if timing:
+        for kind in TIMED:
+            _lib.call("mom_timing_begin", kind, 65536)
+        s.fill_events = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     r = s.sweep(sweep_points(freqs, steps), interp=interp)  # None: exact fill at every point
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    if timing:
+        s.kernel_stats = {}
+        out = np.zeros(3)
+        for kind in TIMED:
+            _lib.call("mom_timing_end", kind, out.ctypes.data_as(__import__("ctypes").c_void_p))
+            s.kernel_stats[kind] = tuple(out)
     return s, r, t1 - t0
 
 
@@ -77,19 +129,6 @@ def sweep_points(freqs, steps):
     if steps >= nf:
         return freqs[np.arange(steps) % nf]
     return freqs[np.round(np.linspace(0, nf - 1, steps)).astype(int)]
-
-
-def gemm_roofline(solver, freq, torch):
-    """Event-timed trailing-update GEMM launches of one more solve."""
-    from mom25d_amd import _lib
-
-    g = solver.tables([freq])[0]
-    _lib.call("mom_gemm_timing_begin", 4096)
-    solver._solve_dev(g, solver.port_excitations())
-    torch.cuda.synchronize()
-    out = np.zeros(3)
-    _lib.call("mom_gemm_timing_end", out.ctypes.data_as(__import__("ctypes").c_void_p))
-    return int(out[0]), out[1] * 1e-3, out[2]
 
 
 def fill_time(solver, freq, torch, reps=3):
@@ -107,16 +146,17 @@ def fill_time(solver, freq, torch, reps=3):
     return ev[0].elapsed_time(ev[1]) * 1e-3 / reps
 
 
-def measured_traffic(kernel, workload, panel, block):
-    """HBM bytes per launch of `kernel` from the committed counter summary
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` (its name up to the first " (") in
+    the timed sweep of `workload`, from the committed counter summary
     (profiles/traffic.json: FETCH_SIZE + WRITE_SIZE per dispatch, scaled by
-    the same counters on a plain copy), or None for another configuration."""
+    the same counters on a plain copy of known size), or None."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
         table = json.load(fh)
-    return table.get(f"{kernel} {workload} panel {panel} block {block}", {}).get("bytes_per_launch")
+    return table.get(f"{kernel.split(' (')[0]} {workload}", {}).get("bytes_per_launch")
 
 
 def batch_random16(torch, precision):
@@ -230,7 +270,7 @@ def main():
     mixed_kw = dict(lu_equiv=args.lu_equiv, lu64_steps=args.lu64_steps if args.lu64_steps >= 0 else None,
                     solve_sb=args.solve_sb)
     sm, rm, tm = time_path(mesh, sub, freqs, prec, args.warmup, args.steps, torch, args.lanes, not args.no_tune,
-                           args.panel, tile, args.block, **(mixed_kw if prec == "mixed" else {}))
+                           args.panel, tile, args.block, timing=True, **(mixed_kw if prec == "mixed" else {}))
     for rec in sm.point_log:
         print(f"  point {rec['freq'] / 1e9:6.3f} GHz lane {rec['lane']}: " +
               ", ".join(f"{k} {v}" for k, v in rec["solves"]), file=sys.stderr)
@@ -261,23 +301,30 @@ def main():
         del si
 
     f_probe = freqs[len(freqs) // 2]
-    calls, secs, flops = gemm_roofline(sm, f_probe, torch)
-    achieved = flops / secs if secs > 0 else 0.0
-    peak = PEAK["fp64_mfma"] if prec == "fp64" else PEAK["fp32_mfma"]
     from mom25d_amd.autotune import get_tile
 
     bm, bn = sm.tile or get_tile(64 if prec == "fp64" else 32)
-    kname = f"cgemm_sub_kernel<{'double' if prec == 'fp64' else 'float'}, {bm}, {bn}>"
-    traffic = measured_traffic(kname, args.workload, sm.panel, sm.block or sm.panel)
-    roof = dict(bound="mfma", achieved=achieved / 1e12, peak=peak / 1e12, unit="TFLOP/s", frac=achieved / peak,
-                traffic=traffic, kernel=f"{kname} (LU trailing update)",
-                launches=calls, avg_launch_ms=1e3 * secs / max(calls, 1), flops_per_launch=flops / max(calls, 1))
+    # the roofline of the kernel with the largest summed time in the timed sweep (HIP events on
+    # each launch's stream); the other timed kernels beside it
+    table = kernel_table(sm.kernel_stats, sm.fill_events, (bm, bn))
+    sm.fill_events = None
+    top = table[0]
+    scale = 1e12 if top["unit"] == "TFLOP/s" else 1e9
+    roof = dict(bound=top["bound"], achieved=top["achieved"] / scale, peak=top["peak"] / scale, unit=top["unit"],
+                frac=top["frac"], traffic=measured_traffic(top["kernel"], args.workload), kernel=top["kernel"],
+                launches=top["launches"], avg_launch_ms=top["avg_launch_ms"], summed_ms=top["total_ms"],
+                work_per_launch=top["work_per_launch"],
+                work_unit="flop" if top["unit"] == "TFLOP/s" else "algorithmic bytes")
+    for r in table:
+        print(f"  timed kernel {r['kernel']}: {r['launches']} launches, {r['total_ms']:.1f} ms summed, "
+              f"{r['avg_launch_ms'] * 1e3:.1f} us avg, {r['frac']:.3f} of peak", file=sys.stderr)
     tf = fill_time(sm, f_probe, torch)
     fill_flops = sm.geometry().algorithmic_flops()
     extra = dict(fill=dict(seconds=tf, algorithmic_flops=fill_flops, achieved_tflops=fill_flops / tf / 1e12,
                            frac_fp64_valu=fill_flops / tf / PEAK["fp64_valu"]))
     extra["lu_config"] = dict(panel=sm.panel, block=sm.block or sm.panel, tile=[bm, bn],
                               autotuned=not args.no_tune and tile is None)
+    extra["timed_kernels"] = [{k: (round(v, 6) if isinstance(v, float) else v) for k, v in r.items()} for r in table]
     extra["lu_stats"] = sweep_lu_stats  # factorisations / reused LUs / GMRES steps in the timed sweep
     if prec == "mixed":
         extra["mixed_policy"] = dict(lu_equiv=sm._lu_equiv(), lu64_steps=sm.lu64_steps, solve_sb=sm.solve_sb)

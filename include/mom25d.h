@@ -288,11 +288,17 @@ int mom_interp_combine(int prec_w, int N, int nb, int K, const void* const* W, c
 int mom_cgemm_sub(int prec, int m, int n, int k, const void* A, int64_t lda, const void* B, int64_t ldb, void* C,
                   int64_t ldc, void* stream);
 
-/* Event timing of the LU's trailing-update GEMM launches (roofline report):
- * begin(capacity) starts recording up to `capacity` launches; end(out) waits
- * and writes out[0] = launches, out[1] = total ms, out[2] = total flops. */
-int mom_gemm_timing_begin(int capacity);
-int mom_gemm_timing_end(double* out);
+/* Event timing of selected launches (roofline report; no reference
+ * counterpart): kind 0 / 4 the complex64 / complex128 LU's trailing-update
+ * GEMMs (work in flops), 1 the strip pass of mom_scaled_symv (bytes: the
+ * symmetric half of Z and the columns), 2 / 3 each triangular chain of the
+ * complex64 / complex128 super-block solves (bytes: one triangle of the
+ * factors and the columns).
+ * begin(kind, capacity) records up to `capacity` launches of that kind,
+ * each between two HIP events on its own stream; end(kind, out) waits and
+ * writes out[0] = launches, out[1] = total ms, out[2] = total work. */
+int mom_timing_begin(int kind, int capacity);
+int mom_timing_end(int kind, double* out);
 
 /* FP64 iterative refinement pieces: R = B - Z X (nrhs <= 4; rowsum, when
  * non-null, receives the row sums of |Re z| + |Im z| of diag(d) Z diag(d),

@@ -67,8 +67,8 @@ SIGNATURES = {
     "mom_residual": (_i, [_i, _i, _p, _i64, _p, _p, _p, _p, _p, _p]),
     "mom_cvt_z_to_c": (_i, [_i64, _p, _p, _p]),
     "mom_refine_update": (_i, [_i, _i, _p, _p, _p, _p, _p]),
-    "mom_gemm_timing_begin": (_i, [_i]),
-    "mom_gemm_timing_end": (_i, [_p]),
+    "mom_timing_begin": (_i, [_i, _i]),
+    "mom_timing_end": (_i, [_i, _p]),
     "mom_basis_dot": (_i, [_i, _i, _i, _p, _p, _p, _p]),
     "mom_basis_sub": (_i, [_i, _i, _i, _p, _p, _p, _p]),
     "mom_line_voltage": (_i, [_i, _p, _i, _i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _d, _p, _p]),

@@ -304,6 +304,12 @@ __global__ __launch_bounds__(256) void panel_kernel(PanelArgs<R> a) {
   }
 }
 
+// The kernels of a GMRES step (preconditioner solves, products,
+// orthogonalisation) form a latency-bound chain per lane, while other lanes'
+// factorisations and fills run beside them on the same CUs: their waves
+// issue first (wave priority 2 of 3; the factorisation's panels take 3).
+__device__ __forceinline__ void gmres_step_priority() { __builtin_amdgcn_s_setprio(2); }
+
 // ------------------------------------------------- unpivoted panel
 template <typename R>
 static constexpr int kNoPivPanelMax() {  // widest unpivoted panel (columns held by the register layout of panel_below_kernel)
@@ -711,7 +717,7 @@ static double cgemm_flops(int m, int n, int k, int skip_above) {
   return f;
 }
 
-static double g_last_gemm_flops = 0;  // of the last launch_cgemm (timing report)
+static thread_local double g_last_gemm_flops = 0;  // of this thread's last launch_cgemm (timing report)
 
 template <typename R, int BM, int BN, bool ASSIGN = false>
 static int launch_cgemm_tile(int m, int n, int k, const cplx<R>* A, int64_t lda, const cplx<R>* B, int64_t ldb,
@@ -982,6 +988,7 @@ __global__ __launch_bounds__(64 * W) void below_rows_kernel(cplx<R>* A, int64_t 
 template <typename R, typename TB>
 __global__ void getrs_prep_kernel(int n, int nrhs, const int32_t* perm, const cplx<TB>* B, cplx<R>* X, cplx<R>* Y,
                                   unsigned* flags) {
+  gmres_step_priority();
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q == 0 || q == 2) flags[q] = 0u;
   if (q >= n * nrhs) return;
@@ -1013,6 +1020,7 @@ __global__ __launch_bounds__(BS * BS / 16) void trsv_kernel(int n, const cplx<R>
                                                             const cplx<R>* __restrict__ Mchain, cplx<R>* B_reset,
                                                             z64* __restrict__ out64, unsigned* ticket,
                                                             unsigned* error, unsigned spin_limit) {
+  gmres_step_priority();
   static_assert(INV || BS == kTrsvBS, "the substitution works on 64-row blocks");
   constexpr int Q = BS / 16, NT = BS * Q;
   constexpr int TRI = INV ? 1 : (LOWER ? BS * (BS - 1) / 2 : BS * (BS + 1) / 2);
@@ -1331,6 +1339,7 @@ __global__ __launch_bounds__(256) void basis_dot_kernel(int n, int nrhs, const z
 // w[r, c] -= sum_{i < j1} V[i, r, c] h[i, c]
 __global__ void basis_sub_kernel(int n, int nrhs, int j1, const z64* __restrict__ V, const z64* __restrict__ h,
                                  z64* __restrict__ w) {
+  gmres_step_priority();
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n * nrhs) return;
   const int c = q % nrhs;
@@ -1366,25 +1375,42 @@ __global__ void zero_u32_kernel(unsigned* p, int n) {
 // ----------------------------------------------------------------- drivers
 static int g_ncu = 0;
 
-// Optional timing of the trailing-update launches (for the roofline report):
-// while enabled, getrf brackets every update GEMM with a pair of HIP events.
-struct GemmTiming {
+// Optional event timing of selected launches (the benchmark's roofline
+// report): while a kind is on, each of its launches is bracketed by a pair
+// of HIP events on the launch's own stream, and its algorithmic work (flops
+// or bytes) is kept beside them.  Launches come from several host threads
+// (one per factorisation in flight, and the lanes' solves), so slots are
+// taken under a lock.
+enum TimingKind { kTimeGemm32 = 0, kTimeSymv = 1, kTimeChain32 = 2, kTimeChain64 = 3, kTimeGemm64 = 4, kTimeKinds = 5 };
+struct KernelTiming {
   bool on = false;
   std::vector<hipEvent_t> ev;
-  std::vector<double> flops;
+  std::vector<double> work;
   size_t used = 0;
 };
-static GemmTiming g_timing;
+static KernelTiming g_kt[kTimeKinds];
+static std::mutex g_kt_mu;
 
-static void timing_mark_begin(hipStream_t s) {
-  if (!g_timing.on || g_timing.used + 2 > g_timing.ev.size()) return;
-  (void)hipEventRecord(g_timing.ev[g_timing.used], s);
+// slot of a timed launch of `kind` (its start event recorded), or -1
+static int timing_begin(int kind, hipStream_t s) {
+  KernelTiming& t = g_kt[kind];
+  int slot;
+  {
+    std::lock_guard<std::mutex> lock(g_kt_mu);
+    if (!t.on || t.used + 2 > t.ev.size()) return -1;
+    slot = (int)t.used;
+    t.used += 2;
+    t.work.push_back(0.0);
+  }
+  (void)hipEventRecord(t.ev[slot], s);
+  return slot;
 }
-static void timing_mark_end(hipStream_t s, double flops) {
-  if (!g_timing.on || g_timing.used + 2 > g_timing.ev.size()) return;
-  (void)hipEventRecord(g_timing.ev[g_timing.used + 1], s);
-  g_timing.flops.push_back(flops);
-  g_timing.used += 2;
+static void timing_end(int kind, int slot, hipStream_t s, double work) {
+  if (slot < 0) return;
+  KernelTiming& t = g_kt[kind];
+  (void)hipEventRecord(t.ev[slot + 1], s);
+  std::lock_guard<std::mutex> lock(g_kt_mu);
+  t.work[slot / 2] = work;
 }
 
 static int device_cus() {
@@ -1527,11 +1553,12 @@ static int update_block(cplx<R>* A, int64_t lda, int n, int k0, int b, int r0, i
   }
   const int mm = n - r0, nn = c1 - c0;
   if (mm <= 0) return 0;
-  timing_mark_begin(s);
+  const int tkind = sizeof(R) == sizeof(double) ? kTimeGemm64 : kTimeGemm32;
+  const int tslot = timing_begin(tkind, s);
   if ((rc = launch_cgemm<R>(mm, nn, b, A + (int64_t)r0 * lda + k0, lda, A + (int64_t)k0 * lda + c0, lda,
                             A + (int64_t)r0 * lda + c0, lda, s, sym ? c0 - r0 : -1)))
     return rc;
-  timing_mark_end(s, g_last_gemm_flops);
+  timing_end(tkind, tslot, s, g_last_gemm_flops);
   return 0;
 }
 
@@ -2082,6 +2109,7 @@ __global__ __launch_bounds__(256) void sb_inverse_kernel(int n, const cplx<R>* _
 template <typename R, typename TB>
 __global__ void sb_prep_kernel(int n, int nrhs, const int32_t* perm, const cplx<TB>* B, cplx<R>* X1, cplx<R>* Y,
                                cplx<R>* X2, unsigned* flags) {
+  gmres_step_priority();
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q == 0 || q == 2) flags[q] = 0u;
   if (q >= n * nrhs) return;
@@ -2138,6 +2166,7 @@ __global__ __launch_bounds__(256, 2) void sb_chain_kernel(int n, int nsb, const 
                                                        const cplx<R>* __restrict__ D, const cplx<R>* __restrict__ in,
                                                        cplx<R>* out, z64* __restrict__ out64, unsigned* ticket,
                                                        unsigned* error, unsigned spin_limit) {
+  gmres_step_priority();
   constexpr int SB = kSbMax, CPL = 8;
   constexpr int RPW = kSbRowsPerWave<R>();
   constexpr int ROWS = 4 * RPW, G = SB / ROWS;
@@ -2391,13 +2420,21 @@ static int sbsolve(int n, int nrhs, int sb, const int32_t* perm, const cplx<TB>*
   if (rc) return rc;
   constexpr int ROWS = 4 * kSbRowsPerWave<R>();
   const dim3 grid(L.nsb * (kSbMax / ROWS));
+  // timed per chain: algorithmic bytes, one triangle of the factors and the columns in and out
+  const int tkind = sizeof(R) == sizeof(double) ? kTimeChain64 : kTimeChain32;
+  const double tbytes = sizeof(cplx<R>) * (0.5 * n * (n + 1.0) + 2.0 * n * nrhs);
+  int ts;
 #define MOM_SB(NR)                                                                                                    \
   case NR:                                                                                                            \
+    ts = timing_begin(tkind, s);                                                                                      \
     hipLaunchKernelGGL((sb_chain_kernel<R, NR, false>), grid, dim3(256), 0, s, n, L.nsb, T, (int64_t)L.ldt, DL,      \
                        (const cplx<R>*)X1, Y, (z64*)nullptr, flags, flags + 4, g_spin_limit);                         \
+    timing_end(tkind, ts, s, tbytes);                                                                                 \
     if ((rc = check_launch("sb_chain_kernel"))) return rc;                                                            \
+    ts = timing_begin(tkind, s);                                                                                      \
     hipLaunchKernelGGL((sb_chain_kernel<R, NR, true>), grid, dim3(256), 0, s, n, L.nsb, T, (int64_t)L.ldt, DU,       \
                        (const cplx<R>*)Y, X2, X64, flags + 2, flags + 4, g_spin_limit);                               \
+    timing_end(tkind, ts, s, tbytes);                                                                                 \
     break;
   switch (nrhs) {
     MOM_SB(1)
@@ -2735,6 +2772,7 @@ template <int NR, int NBUF, int LPR>
 __global__ __launch_bounds__(256) void symv_strip_kernel(int n, const z64* __restrict__ Z, int64_t ldz,
                                                          const z64* __restrict__ X, int ldx,
                                                          const double* __restrict__ dsc, z64* __restrict__ Q) {
+  gmres_step_priority();
   constexpr int RPI = 64 / LPR;              // rows per load instruction
   constexpr int U = kSymvT / RPI;            // rows per lane
   constexpr int SPT = kSymvT / (4 * LPR);    // steps per tile
@@ -2861,6 +2899,7 @@ template <int NR>
 __global__ __launch_bounds__(1024) void symv_reduce_kernel(int n, int nb, int chunk, const z64* __restrict__ Q,
                                                            const double* __restrict__ dsc, z64* __restrict__ W,
                                                            int ldw) {
+  gmres_step_priority();
   constexpr int E = kSymvT * NR, PS = 1024 / E;  // entries per block, partitions of S
   __shared__ z64 part[PS][E];
   const int K = blockIdx.x, e = threadIdx.x % E, p = threadIdx.x / E;
@@ -2902,15 +2941,22 @@ extern "C" int mom_scaled_symv(int n, int nrhs, const void* Z, int64_t ldz, cons
   // right-hand sides in passes of at most 2 (the strip kernel's registers)
   for (int k0 = 0; k0 < nrhs; k0 += 2) {
     const int nr = std::min(2, nrhs - k0);
+    // timed: the strip pass (the product's pass over Z); algorithmic bytes: the
+    // symmetric half of Z and the columns in and out
+    const double bytes = 8.0 * n * (n + 1.0) + 32.0 * n * nr;
     if (nr == 2) {
+      const int ts = timing_begin(kTimeSymv, s);
       hipLaunchKernelGGL((symv_strip_kernel<2, kSymvBufs<2>, kSymvLanesPerRow>), dim3(nc, nb), dim3(256), 0, s, n, z, ldz, x + k0, nrhs,
                          dsc, q);
+      timing_end(kTimeSymv, ts, s, bytes);
       if (int rc = check_launch("symv_strip_kernel")) return rc;
       hipLaunchKernelGGL((symv_reduce_kernel<2>), dim3(nb), dim3(1024), 0, s, n, nb, ch, (const z64*)q, dsc, w + k0,
                          nrhs);
     } else {
+      const int ts = timing_begin(kTimeSymv, s);
       hipLaunchKernelGGL((symv_strip_kernel<1, kSymvBufs<1>, kSymvLanesPerRow>), dim3(nc, nb), dim3(256), 0, s, n, z, ldz, x + k0, nrhs,
                          dsc, q);
+      timing_end(kTimeSymv, ts, s, bytes);
       if (int rc = check_launch("symv_strip_kernel")) return rc;
       hipLaunchKernelGGL((symv_reduce_kernel<1>), dim3(nb), dim3(1024), 0, s, n, nb, ch, (const z64*)q, dsc, w + k0,
                          nrhs);
@@ -2938,29 +2984,41 @@ extern "C" int mom_refine_update(int n, int nrhs, const void* D, void* X, void* 
 // starts recording; end() waits for the recorded launches and returns, in
 // out[0..2], the number of launches, their total duration [ms] and their
 // total algorithmic flops (8 m n k per complex update).
-extern "C" int mom_gemm_timing_begin(int capacity) {
-  for (hipEvent_t e : g_timing.ev) (void)hipEventDestroy(e);
-  g_timing.ev.assign(2 * (size_t)capacity, nullptr);
-  for (auto& e : g_timing.ev) MOM_TRY(hipEventCreate(&e));
-  g_timing.flops.clear();
-  g_timing.used = 0;
-  g_timing.on = true;
+extern "C" int mom_timing_begin(int kind, int capacity) {
+  if (kind < 0 || kind >= kTimeKinds) return fail("mom_timing_begin: kind must be 0..4");
+  if (capacity < 0) return fail("mom_timing_begin: negative capacity");
+  KernelTiming& t = g_kt[kind];
+  std::lock_guard<std::mutex> lock(g_kt_mu);
+  for (hipEvent_t e : t.ev) (void)hipEventDestroy(e);
+  t.ev.assign(2 * (size_t)capacity, nullptr);
+  for (auto& e : t.ev) MOM_TRY(hipEventCreate(&e));
+  t.work.clear();
+  t.used = 0;
+  t.on = true;
   return 0;
 }
 
-extern "C" int mom_gemm_timing_end(double* out) {
-  g_timing.on = false;
-  double ms = 0.0, fl = 0.0;
-  for (size_t i = 0; i + 1 < g_timing.used; i += 2) {
-    MOM_TRY(hipEventSynchronize(g_timing.ev[i + 1]));
-    float t = 0.f;
-    MOM_TRY(hipEventElapsedTime(&t, g_timing.ev[i], g_timing.ev[i + 1]));
-    ms += t;
-    fl += g_timing.flops[i / 2];
+extern "C" int mom_timing_end(int kind, double* out) {
+  if (kind < 0 || kind >= kTimeKinds) return fail("mom_timing_end: kind must be 0..4");
+  if (!out) return fail("mom_timing_end: null output");
+  KernelTiming& t = g_kt[kind];
+  size_t used;
+  {
+    std::lock_guard<std::mutex> lock(g_kt_mu);
+    t.on = false;
+    used = t.used;
   }
-  out[0] = (double)(g_timing.used / 2);
+  double ms = 0.0, work = 0.0;
+  for (size_t i = 0; i + 1 < used; i += 2) {
+    MOM_TRY(hipEventSynchronize(t.ev[i + 1]));
+    float e = 0.f;
+    MOM_TRY(hipEventElapsedTime(&e, t.ev[i], t.ev[i + 1]));
+    ms += e;
+    work += t.work[i / 2];
+  }
+  out[0] = (double)(used / 2);
   out[1] = ms;
-  out[2] = fl;
+  out[2] = work;
   return 0;
 }
 
@@ -2995,6 +3053,7 @@ constexpr int kOrthSplit = 8;  // row ranges of the step's dot products and norm
 // workgroup (i, s), the ranges cutting n into kOrthSplit pieces.
 __global__ __launch_bounds__(256) void dot_part_kernel(int n, int nrhs, int j1, const z64* __restrict__ V,
                                                        const z64* __restrict__ w, z64* __restrict__ part) {
+  gmres_step_priority();
   const int i = blockIdx.x, sp = blockIdx.y, tid = threadIdx.x;
   const int chunk = (n + kOrthSplit - 1) / kOrthSplit, r0 = sp * chunk, r1 = min(n, r0 + chunk);
   z64 acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
@@ -3026,6 +3085,7 @@ __global__ __launch_bounds__(256) void dot_part_kernel(int n, int nrhs, int j1, 
 
 // h[i][c] = sum_s part[s][i][c] in order; with H also H[i][c] += h[i][c]
 __global__ void dot_reduce_kernel(int cnt, const z64* __restrict__ part, z64* __restrict__ h, z64* __restrict__ H) {
+  gmres_step_priority();
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= cnt) return;
   z64 t = part[q];
@@ -3037,6 +3097,7 @@ __global__ void dot_reduce_kernel(int cnt, const z64* __restrict__ part, z64* __
 // npart[s][c] = sum over the rows of range s of |w[r, c]|^2: workgroup (c, s)
 __global__ __launch_bounds__(256) void norm_part_kernel(int n, int nrhs, const z64* __restrict__ w,
                                                         double* __restrict__ npart) {
+  gmres_step_priority();
   const int c = blockIdx.x, sp = blockIdx.y, tid = threadIdx.x;
   const int chunk = (n + kOrthSplit - 1) / kOrthSplit, r0 = sp * chunk, r1 = min(n, r0 + chunk);
   double t = 0.0;
@@ -3055,6 +3116,7 @@ __global__ __launch_bounds__(256) void norm_part_kernel(int n, int nrhs, const z
 // and hn[c] = (|w[:, c]|, 0) written once
 __global__ void col_scale_kernel(int n, int nrhs, z64* __restrict__ w, const double* __restrict__ npart,
                                  z64* __restrict__ hn) {
+  gmres_step_priority();
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n * nrhs) return;
   const int c = q % nrhs;

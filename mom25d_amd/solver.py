@@ -666,6 +666,9 @@ class Solver:
     # Z <- (Z + Z^T) / 2 as Engine/mom25d/solver.py:522-524 does; the frequency
     # interpolation preserves it); False: the row-wise product over all of Z
     symmetric_products = True
+    # a list: every fill tier's launch is bracketed by two events on its stream,
+    # appended here as (tier, start, end, algorithmic flops) (the bench's roofline report)
+    fill_events = None
     # spline intervals a fill work item may stage in LDS (those between its
     # smallest and largest source-test distance; an item that needs more
     # reads the table from global memory; 0: never stage).  The values are
@@ -783,7 +786,14 @@ class Solver:
             a.item_r = d[f"item_r{tier}"].data_ptr()
             a.stage_max = self.fill_stage
             a.acc_bytes = plan.acc_bytes[tier]
+            ev = None
+            if self.fill_events is not None:  # timing for a roofline report, on the launch's stream
+                ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                ev[0].record()
             _lib.call("mom_fill_tier", tier, int(items.shape[0]), ctypes.byref(a), stream)
+            if ev is not None:
+                ev[1].record()
+                self.fill_events.append((tier, ev[0], ev[1], float(plan.FLOPS_PER_PAIR[tier] * plan.tier_pairs[tier])))
         zs = self.stack.surface_impedance(g.freq)
         if zs:
             _lib.call("mom_gram_add", plan.nt, P(d["gram"]), P(d["tb"]), float(np.real(zs)), float(np.imag(zs)),
@@ -1377,6 +1387,8 @@ class Solver:
             else:
                 _lib.call("mom_trtri_blocks", 64, n, P(A), A.stride(0), P(sw), stream)
 
+        if self.block is not None:  # the tuned update depth serves the complex128 factors too
+            _lib.call("mom_set_lu_block", 64, self.block)
         yield from _on_launcher(calls, self.device)
         self.lu_stats["fp64_lu"] += 1
         return Precond(64, A, perm, sw, sb)
