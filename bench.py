@@ -238,6 +238,8 @@ def main():
                     help="mixed path: GMRES steps after which a complex64 solve gives way to a complex128 anchor")
     ap.add_argument("--solve-sb", type=int, default=512,
                     help="mixed path: super-block rows of the preconditioner solves (0: the 64-row chain)")
+    ap.add_argument("--no-kernel-timing", action="store_true",
+                    help="no HIP events around the timed sweep's launches (then no roofline object)")
     ap.add_argument("--hw-queues", type=int, default=16,
                     help="GPU_MAX_HW_QUEUES of this process (HIP's default is 4)")
     args = ap.parse_args()
@@ -270,7 +272,8 @@ def main():
     mixed_kw = dict(lu_equiv=args.lu_equiv, lu64_steps=args.lu64_steps if args.lu64_steps >= 0 else None,
                     solve_sb=args.solve_sb)
     sm, rm, tm = time_path(mesh, sub, freqs, prec, args.warmup, args.steps, torch, args.lanes, not args.no_tune,
-                           args.panel, tile, args.block, timing=True, **(mixed_kw if prec == "mixed" else {}))
+                           args.panel, tile, args.block, timing=not args.no_kernel_timing,
+                           **(mixed_kw if prec == "mixed" else {}))
     for rec in sm.point_log:
         print(f"  point {rec['freq'] / 1e9:6.3f} GHz lane {rec['lane']}: " +
               ", ".join(f"{k} {v}" for k, v in rec["solves"]), file=sys.stderr)
@@ -306,15 +309,17 @@ def main():
     bm, bn = sm.tile or get_tile(64 if prec == "fp64" else 32)
     # the roofline of the kernel with the largest summed time in the timed sweep (HIP events on
     # each launch's stream); the other timed kernels beside it
-    table = kernel_table(sm.kernel_stats, sm.fill_events, (bm, bn))
+    table = kernel_table(sm.kernel_stats, sm.fill_events, (bm, bn)) if not args.no_kernel_timing else []
     sm.fill_events = None
-    top = table[0]
-    scale = 1e12 if top["unit"] == "TFLOP/s" else 1e9
-    roof = dict(bound=top["bound"], achieved=top["achieved"] / scale, peak=top["peak"] / scale, unit=top["unit"],
-                frac=top["frac"], traffic=measured_traffic(top["kernel"], args.workload), kernel=top["kernel"],
-                launches=top["launches"], avg_launch_ms=top["avg_launch_ms"], summed_ms=top["total_ms"],
-                work_per_launch=top["work_per_launch"],
-                work_unit="flop" if top["unit"] == "TFLOP/s" else "algorithmic bytes")
+    roof = None
+    if table:
+        top = table[0]
+        scale = 1e12 if top["unit"] == "TFLOP/s" else 1e9
+        roof = dict(bound=top["bound"], achieved=top["achieved"] / scale, peak=top["peak"] / scale, unit=top["unit"],
+                    frac=top["frac"], traffic=measured_traffic(top["kernel"], args.workload), kernel=top["kernel"],
+                    launches=top["launches"], avg_launch_ms=top["avg_launch_ms"], summed_ms=top["total_ms"],
+                    work_per_launch=top["work_per_launch"],
+                    work_unit="flop" if top["unit"] == "TFLOP/s" else "algorithmic bytes")
     for r in table:
         print(f"  timed kernel {r['kernel']}: {r['launches']} launches, {r['total_ms']:.1f} ms summed, "
               f"{r['avg_launch_ms'] * 1e3:.1f} us avg, {r['frac']:.3f} of peak", file=sys.stderr)

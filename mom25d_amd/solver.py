@@ -682,8 +682,8 @@ class Solver:
         return min(8, max(1, (os.cpu_count() or 2) // 2)) if nf >= 4 else 0
 
     def _lane_count(self):
-        """Lanes of a sweep: as asked (default 4 up to N = 4000, else 3 in
-        mixed precision and 2 in FP64, as many as fit in a third of the free
+        """Lanes of a sweep: as asked (default 4 in mixed precision and, up
+        to N = 4000, in FP64, else 2; as many as fit in a third of the free
         device memory), and no more than allow every lane's LU panel to be
         resident at once."""
         n = self.mesh.n_unknowns
@@ -692,9 +692,10 @@ class Solver:
         else:
             per = n * n * (16 + (8 if self.precision == "mixed" else 0))
             free, _total = torch.cuda.mem_get_info(self.device)
-            # mixed precision spends most of a sweep in latency-bound GMRES steps: a third lane
-            # overlaps them (measured at N = 12 428: 4.33 s vs 4.46-4.66 s for 51 points)
-            want = 4 if n <= 4000 else (3 if self.precision == "mixed" else 2)
+            # mixed precision spends most of a sweep in latency-bound GMRES steps and factorisations
+            # bound by their panels, which further lanes overlap (measured at N = 12 428, 20 points:
+            # 0.98 / 0.94 / 0.86 / 0.88 / 0.92 s with 2 / 3 / 4 / 5 / 6 lanes)
+            want = 4 if n <= 4000 or self.precision == "mixed" else 2
             lanes = max(1, min(want, int(free / 3 // per)))
         if self.precision == "mixed" and self.nopiv:
             # unpivoted factorisations wait for no other workgroup; a pivoted
@@ -1798,6 +1799,20 @@ class LaneCursor:
         return self.lo[r] - 1
 
 
+_LANE_STREAMS = {}
+
+
+def _lane_streams(device, lanes):
+    """The lanes' streams, kept for the process's lifetime and reused by
+    every sweep (the library keeps a look-ahead stream and events per caller
+    stream, created on first use)."""
+    key = torch.device(device)
+    pool = _LANE_STREAMS.setdefault(key, [])
+    while len(pool) < lanes:
+        pool.append(torch.cuda.Stream(key))
+    return pool[:lanes]
+
+
 def run_lanes(task, count, lanes, device, solvers, graphs=False):
     """Run task(i, lane) for i < count, `lanes` at a time, round robin.
     Each task is a generator that enqueues GPU work on the current stream and
@@ -1809,7 +1824,7 @@ def run_lanes(task, count, lanes, device, solvers, graphs=False):
     meanwhile."""
     prev = torch.cuda.current_stream(device)
     own = lanes > 1 or graphs  # graphs are captured on a stream of their own
-    streams = [torch.cuda.Stream(device) for _ in range(lanes)] if own else [prev]
+    streams = _lane_streams(device, lanes) if own else [prev]
     for st in streams:
         if st is not prev:
             st.wait_stream(prev)
