@@ -142,3 +142,22 @@ def tune_lu(n, prec, device="cuda", panels=PANELS, blocks=BLOCKS, tiles=TILES, r
     _CACHE[key] = best
     _save()
     return best
+
+
+def main(argv=None):
+    """Print the autotuner's timing table for one size and factorisation:
+    python -m mom25d_amd.autotune --n 12428 --prec 32 --kind sym."""
+    import argparse
+
+    ap = argparse.ArgumentParser(description=main.__doc__.split(":")[0])
+    ap.add_argument("--n", type=int, default=12428)
+    ap.add_argument("--prec", type=int, default=32, choices=(32, 64))
+    ap.add_argument("--kind", default="sym", choices=("pivot", "nopiv", "sym"),
+                    help="pivot: mom_getrf; nopiv: mom_getrf_nopiv; sym: mom_sytrf_nopiv (L D L^T)")
+    a = ap.parse_args(argv)
+    best = tune_lu(a.n, a.prec, pivot=a.kind == "pivot", symmetric=a.kind == "sym", verbose=True)
+    print(f"best: panel {best['jb']}, block {best['nb']}, tile {best['tile']}: {best['ms']:.2f} ms (n = {best['n']})")
+
+
+if __name__ == "__main__":
+    main()
