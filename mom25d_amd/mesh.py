@@ -7,17 +7,21 @@ Engine/mom25d/mesh.py (Port :62-188, Via :191-241, Layout :309-693,
 _grid_lines :696-703, Mesh :706-1116); lumped gaps (add_lumped,
 add_component) are axis-aligned bridges as in the reference's grid meshes;
 internal S-parameter ports (add_gap_port, add_via_port) as well;
-pours, polygons, thick metal, apertures and triangular meshing are not part
-of this build.
+polygon copper and copper pours (add_polygon, add_pour) are meshed as a
+staircase on the same grid, their axis-aligned edges becoming grid lines,
+with the plane geometry of planar.py in place of shapely; thick metal,
+apertures and triangular meshing are not part of this build.
 
 Conventions kept from the reference because the numbers depend on them:
   * grid lines: every rectangle edge, every port's reference / gap
     coordinate and every lumped bridge's two ends and gap, optional
     refinement lines +-edge_cell, then uniform
     subdivision of each interval into ceil(len / max_cell) parts;
-  * a cell is copper if its centre lies strictly inside a rectangle or a
-    via pad of the layer; it is split into two triangles along a diagonal
-    that alternates in a checkerboard;
+  * polygon and pour edges parallel to an axis are grid lines too, unless
+    closer than `snap` (default max_cell / 4) to one already there;
+  * a cell is copper if its centre lies strictly inside a rectangle, a
+    polygon, a pour or a via pad of the layer; it is split into two
+    triangles along a diagonal that alternates in a checkerboard;
   * node numbering: per layer, node (i, j) of the grid has id
     layer * nx * ny + i * ny + j, then the used ids are renumbered in
     ascending order;
@@ -29,6 +33,8 @@ Conventions kept from the reference because the numbers depend on them:
 from dataclasses import dataclass, field
 
 import numpy as np
+
+from . import planar
 
 _AXIS_DIRS = {"+x": (1, 0), "-x": (-1, 0), "+y": (0, 1), "-y": (0, -1)}
 
@@ -134,9 +140,62 @@ class Via:
 
 
 @dataclass
+class Pour:
+    """Copper pour (Engine/mom25d/mesh.py:244-306): `region` minus every
+    other copper feature of its layer grown by `clearance` (mitred, so
+    Manhattan copper stays Manhattan), optionally stitched to `via_to` by a
+    via fence along all of its edges (`via_pitch`) and a square grid of vias
+    inside (`fill_pitch`), fence via centres `via_offset` (default pad radius
+    + drill / 2) inside the pour edge.  region: a list of planar shapes."""
+
+    region: list
+    clearance: float
+    via_pitch: float = None
+    via_drill: float = 0.3e-3
+    via_pad: float = None
+    via_offset: float = None
+    fill_pitch: float = None
+    layer: str = "top"
+    via_to: str = "gnd"
+
+    def resolve(self, signal):
+        """(pour shape, its outline, [Via, ...]) against the other copper
+        `signal` (a planar shape or None)."""
+        geom = planar.Region(self.region)
+        if signal is not None:
+            geom = planar.Region(self.region, [planar.mitre_buffer(signal, self.clearance)])
+        polys = planar.outline(geom)
+        vias = []
+        if (self.via_pitch or self.fill_pitch) and polys:
+            pad = self.via_pad or 2 * self.via_drill
+            off = self.via_offset if self.via_offset is not None else 0.5 * (pad + self.via_drill)
+            core = planar.mitre_buffer(geom, -off, polys)
+            rings = planar.outline(core)
+            pts = []
+            if self.via_pitch:
+                for r in planar.fence_rings(rings):
+                    pts += planar.ring_points(r, self.via_pitch)
+            if self.fill_pitch and rings:
+                x0, y0, x1, y1 = planar.bounds_of(rings)
+                xs = np.arange(x0, x1 + 1e-12, self.fill_pitch)
+                ys = np.arange(y0, y1 + 1e-12, self.fill_pitch)
+                X, Y = np.meshgrid(xs + 0.5 * (x1 - xs[-1]), ys + 0.5 * (y1 - ys[-1]))
+                inside = core.contains(X.ravel(), Y.ravel())
+                pts += list(zip(X.ravel()[inside], Y.ravel()[inside]))
+            # near-duplicates (fence corners, fill beside the fence) are dropped
+            dmin = 0.5 * min(p for p in (self.via_pitch, self.fill_pitch) if p)
+            kept = []
+            for q in pts:
+                if all(np.hypot(q[0] - k[0], q[1] - k[1]) >= dmin for k in kept):
+                    kept.append(q)
+            vias = [Via(q, self.via_drill, pad, top=self.layer, bottom=self.via_to) for q in kept]
+        return geom, polys, vias
+
+
+@dataclass
 class Layout:
-    """Axis-aligned copper rectangles, feed ports and vias on named metal
-    layers (Engine/mom25d/mesh.py:309-693, rectangle / port / via subset)."""
+    """Copper rectangles, polygons and pours, feed ports and vias on named
+    metal layers (Engine/mom25d/mesh.py:309-693)."""
 
     rects: list = field(default_factory=list)
     ports: list = field(default_factory=list)
@@ -146,6 +205,9 @@ class Layout:
     lumped: list = field(default_factory=list)
     components: list = field(default_factory=list)
     internal_ports: list = field(default_factory=list)
+    polys: list = field(default_factory=list)
+    poly_layers: list = field(default_factory=list)
+    pours: list = field(default_factory=list)
 
     def add_rect(self, x0, y0, x1, y1, layer="top"):
         self.rects.append((min(x0, x1), min(y0, y1), max(x0, x1), max(y0, y1)))
@@ -233,15 +295,61 @@ class Layout:
         return v
 
     def add_polygon(self, geom, layer="top"):
-        """Polygon copper (Engine/mom25d/mesh.py:543-560) is not part of this
-        build: it needs shapely geometry, which this image lacks."""
-        raise ValueError("polygon copper is not supported by this build (needs shapely); use add_rect")
+        """Polygon copper (Engine/mom25d/mesh.py:543-552): a planar.Polygon,
+        a vertex list, a list of rings (exterior first) or a shapely-like
+        polygon / multipolygon [m].  Axis-aligned edges become grid lines;
+        other edges are meshed as a staircase at the grid resolution."""
+        for g in planar.as_polygons(geom):
+            self.polys.append(g)
+            self.poly_layers.append(layer)
+        self._pours = None
+        return self
 
-    def add_pour(self, region, clearance, **kw):
-        """Copper pours (Engine/mom25d/mesh.py:488-506) are not part of this
-        build: they need shapely geometry, which this image lacks."""
-        raise ValueError("copper pours are not supported by this build (needs shapely); build the copper from "
-                         "rectangles and add the stitching vias with add_via")
+    def add_pour(self, region, clearance, via_pitch=None, via_drill=0.3e-3, via_pad=None, via_offset=None,
+                 fill_pitch=None, layer="top", via_to="gnd"):
+        """Copper pour (normally ground) over `region` (a rectangle
+        (x0, y0, x1, y1) or any shape add_polygon takes, metres) on `layer`
+        (Engine/mom25d/mesh.py:488-506).  All other copper of that layer --
+        rectangles, polygons, port feed lines and via pads -- is cut out of
+        it with `clearance` when the layout is meshed, so ports added later
+        are respected.  With `via_pitch` a via fence follows every pour edge
+        (with `fill_pitch` a grid of vias fills the inside), from the pour to
+        `via_to`; without vias the pour floats."""
+        self.pours.append(Pour(planar.as_polygons(region), clearance, via_pitch, via_drill, via_pad, via_offset,
+                               fill_pitch, layer, via_to))
+        self._pours = None
+        return self
+
+    def resolved_pours(self):
+        """[(shape, outline, vias), ...] of every pour against the current
+        copper of its layer (Engine/mom25d/mesh.py:516-537); cached until
+        copper is added."""
+        key = (len(self.rects), len(self.polys), len(self.vias), len(self.pours))
+        cache = getattr(self, "_pours", None)
+        if cache is not None and cache[0] == key:
+            return cache[1]
+        out = []
+        for p in self.pours:
+            sig = [planar.Polygon.box(*r) for r, n in zip(self.rects, self.rect_layers) if n == p.layer]
+            sig += [g for g, n in zip(self.polys, self.poly_layers) if n == p.layer]
+            sig += [planar.Polygon.disc(v.xy, 0.5 * v.pad, 32) for v in self.vias if p.layer in v.pad_layers()]
+            out.append(p.resolve(planar.Region(sig) if sig else None))
+        self._pours = (key, out)
+        return out
+
+    def _shapes(self, layer=None):
+        """Polygons and resolved pours of one layer (None: all layers)."""
+        shapes = [g for g, n in zip(self.polys, self.poly_layers) if layer is None or n == layer]
+        shapes += [g for (g, _, _), p in zip(self.resolved_pours(), self.pours) if layer is None or p.layer == layer]
+        return shapes
+
+    def polygon_grid_coords(self, tol):
+        """x of the vertical and y of the horizontal edges of the union of all
+        polygons and pours (Engine/mom25d/mesh.py:610-624)."""
+        shapes = self._shapes()
+        if not shapes:
+            return [], []
+        return planar.axis_coords(planar.outline(planar.Region(shapes)), tol)
 
     def add_plane(self, name):
         if name not in self.planes:
@@ -249,16 +357,24 @@ class Layout:
         return self
 
     def layer_names(self):
-        """Metal layers in order of first appearance (rectangles, then via pads)."""
-        names = list(self.rect_layers) + [n for v in self.vias for n in v.pad_layers()]
+        """Metal layers in order of first appearance (rectangles, polygons,
+        pours, via pads, then the layers pour vias go to)."""
+        names = list(self.rect_layers) + list(self.poly_layers) + [p.layer for p in self.pours]
+        names += [n for v in self.vias for n in v.pad_layers()]
+        names += [p.via_to for p in self.pours if p.via_to != "gnd" and (p.via_pitch or p.fill_pitch)]
         return [n for n in dict.fromkeys(names) if n not in self.planes]
 
     def all_vias(self):
-        return list(self.vias)
+        """Explicit vias, then the stitching vias of every pour."""
+        return list(self.vias) + [v for _, _, vs in self.resolved_pours() for v in vs]
 
     def bbox(self):
         boxes = [tuple(r) for r in self.rects]
-        for v in self.vias:
+        outl = [q for p in self.polys for q in planar.outline(p)] if self.polys else []
+        outl += [q for _, o, _ in self.resolved_pours() for q in o]
+        if outl:
+            boxes.append(planar.bounds_of(outl))
+        for v in self.all_vias():
             r = 0.5 * v.pad
             boxes.append((v.xy[0] - r, v.xy[1] - r, v.xy[0] + r, v.xy[1] + r))
         b = np.array(boxes)
@@ -270,7 +386,9 @@ class Layout:
         for (x0, y0, x1, y1), n in zip(self.rects, self.rect_layers):
             if layer is None or n == layer:
                 hit |= (x > x0) & (x < x1) & (y > y0) & (y < y1)
-        for v in self.vias:
+        for g in self._shapes(layer):
+            hit |= g.contains(x, y)
+        for v in self.all_vias():  # via pads: round copper that adds no grid lines
             if layer is None or layer in v.pad_layers():
                 hit |= (x - v.xy[0]) ** 2 + (y - v.xy[1]) ** 2 < (0.5 * v.pad) ** 2
         return hit
@@ -278,7 +396,7 @@ class Layout:
     def mesh(self, max_cell, edge_cell=None, snap=None, kind="grid"):
         if kind != "grid":
             raise ValueError("only kind='grid' meshes are supported by this build")
-        return Mesh.from_layout(self, max_cell, edge_cell)
+        return Mesh.from_layout(self, max_cell, edge_cell, snap)
 
 
 def _grid_lines(fixed, refine, lo, hi, max_cell, tol):
@@ -314,7 +432,7 @@ class Mesh:
         self._attach_vias(layout.all_vias() if layout is not None else [])
 
     @classmethod
-    def from_layout(cls, layout, max_cell, edge_cell=None):
+    def from_layout(cls, layout, max_cell, edge_cell=None, snap=None):
         """Structured mesh of a layout (Engine/mom25d/mesh.py:728-781)."""
         x0, y0, x1, y1 = layout.bbox()
         tol = 1e-9 * max(x1 - x0, y1 - y0)
@@ -324,6 +442,13 @@ class Mesh:
             (fx if p.axis == 0 else fy).extend([p.coord(0), p.coord(p.s_gap)])
         for p in getattr(layout, "lumped", []):  # a lumped gap is a grid line, and so are the bridge's ends
             (fx if p.axis == 0 else fy).extend([p.coord(0), p.coord(p.s_gap), p.coord(p.length)])
+        # polygon and pour edges: merged into an existing line closer than snap
+        snap = 0.25 * max_cell if snap is None else snap
+        px, py = layout.polygon_grid_coords(1e3 * tol) if hasattr(layout, "polygon_grid_coords") else ([], [])
+        for f, extra in ((fx, px), (fy, py)):
+            for c in sorted(extra):
+                if not f or np.min(np.abs(np.asarray(f) - c)) > snap:
+                    f.append(c)
         rx, ry = [], []
         if edge_cell:
             ax, ay = np.array(fx), np.array(fy)

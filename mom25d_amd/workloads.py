@@ -10,6 +10,7 @@ same geometry can be built by any implementation of that API.  Returns
   random16   16 seeded random chains of rectangles, 500-3000 RWG each, 64 f in 1-20 GHz
   via2       2-layer top -> via -> inner transition, ~6k RWG, 81 f in 1-15 GHz
   gcpw40     40 x 20 mm via-stitched grounded-coplanar pour, ~12k RWG, 51 f in 1-20 GHz
+  gcpw_pour  the reference's pour model: a line in a via-fenced pour, ~5k RWG + 88 vias, 15 f in 1-15 GHz
 """
 
 import types
@@ -156,4 +157,22 @@ def gcpw40(ns=None, max_cell=0.5 * mm, edge_cell=0.1 * mm, pitch=1.5 * mm):
     return lay, _ro4003(ns), dict(max_cell=max_cell, edge_cell=edge_cell), np.linspace(1, 20, 51) * GHz
 
 
-ALL = {"tiny": tiny, "tee": tee, "lpf7": lpf7, "via2": via2, "gcpw40": gcpw40}
+def gcpw_pour(ns=None, max_cell=0.25 * mm, edge_cell=0.05 * mm):
+    """50-ohm microstrip (eps_r 3.66, h 0.254 mm) in a copper pour on both
+    sides (gap 0.2 mm, 1.2 mm wide) stitched by via fences (1 mm pitch, 0.3 /
+    0.5 mm vias), reference planes 5 mm either side of the centre
+    (Models/gcpw_pour/geometry/gcpw_pour.py:8-24, meshed as in
+    Models/gcpw_pour/analysis/vs_openems.py:23-25: 15 f in 1-15 GHz)."""
+    ns = ns or _default_ns()
+    W, G, WP, REF, FEED = 0.5561 * mm, 0.2 * mm, 1.2 * mm, 5 * mm, 6 * mm
+    lay = ns.Layout()
+    lay.add_rect(-REF, -W / 2, REF, W / 2)
+    lay.add_port("P1", (-REF, 0), "-x", W, FEED)
+    lay.add_port("P2", (REF, 0), "+x", W, FEED)
+    E, y1 = REF + FEED, W / 2 + G + WP
+    lay.add_pour((-E, -y1, E, y1), G, via_pitch=1 * mm, via_drill=0.3 * mm, via_pad=0.5 * mm)
+    return lay, ns.Substrate(3.66, 0.254 * mm), dict(max_cell=max_cell, edge_cell=edge_cell), \
+        np.linspace(1, 15, 15) * GHz
+
+
+ALL = {"tiny": tiny, "tee": tee, "lpf7": lpf7, "via2": via2, "gcpw40": gcpw40, "gcpw_pour": gcpw_pour}

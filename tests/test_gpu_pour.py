@@ -1,0 +1,53 @@
+"""The reference's pour model (Models/gcpw_pour: a 50-ohm line in a
+via-fenced copper pour, 4 980 RWG + 88 via unknowns) solved on the device:
+against the openEMS simulation the reference holds for it
+(Engine/tests/data/gcpw_pour_openems.txt: f, re/im S11, re/im S21), with
+the line constants the reference states for it (eeff 2.72-2.76, Z0
+47.8-48.5 ohm: Engine/tests/test_gcpw.py:77-79), and mixed precision
+against FP64 (|dS| < 1e-4)."""
+
+import os
+
+import numpy as np
+import pytest
+
+from mom25d_amd import Solver
+from mom25d_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(__file__)
+
+
+@pytest.fixture(scope="module")
+def pour_runs():
+    d = np.loadtxt(os.path.join(HERE, "golden", "gcpw_pour_openems.txt"))[[10, 42]]  # 3 and 11 GHz
+    lay, st, mk, _ = W.gcpw_pour()
+    m = lay.mesh(**mk)
+    assert (m.n_basis, m.n_vias) == (4980, 88)
+    out = {p: Solver(m, st, verbose=False, precision=p).sweep(d[:, 0], interp=None) for p in ("fp64", "mixed")}
+    return d, out
+
+
+def test_pour_line_vs_openems(pour_runs):
+    d, out = pour_runs
+    r = out["fp64"]
+    s11r, s21r = d[:, 1] + 1j * d[:, 2], d[:, 3] + 1j * d[:, 4]
+    S = r.renormalize(50.0)
+    print("eeff", r.eeff[:, 0], "Z0", r.z0_line[:, 0].real, "|S11|", np.abs(S[:, 0, 0]), "openEMS", np.abs(s11r),
+          "dphase21", np.angle(S[:, 1, 0] / s21r, deg=True))
+    assert np.all((r.eeff[:, 0] > 2.70) & (r.eeff[:, 0] < 2.78))
+    assert np.all((r.z0_line[:, 0].real > 47.5) & (r.z0_line[:, 0].real < 48.8))
+    assert np.all(np.abs(np.angle(S[:, 1, 0] / s21r, deg=True)) < 2.5)
+    np.testing.assert_allclose(np.abs(S[:, 0, 0]), np.abs(s11r), atol=0.015)
+    # reciprocal and passive; the lossless substrate leaves only radiation
+    assert np.allclose(S[:, 0, 1], S[:, 1, 0], atol=1e-5)
+    power = np.sum(np.abs(S) ** 2, axis=1)
+    assert np.all((power < 1 + 1e-6) & (power > 0.97))
+
+
+def test_pour_mixed_matches_fp64(pour_runs):
+    _, out = pour_runs
+    dS = np.max(np.abs(out["mixed"].S - out["fp64"].S))
+    print(f"gcpw_pour mixed vs fp64 |dS| = {dS:.2e}")
+    assert dS < 1e-4

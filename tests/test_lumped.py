@@ -97,10 +97,11 @@ def test_parse_grammar_errors():
 def test_layout_rejects_what_this_build_lacks():
     from mom25d_amd.mesh import Layout
 
-    with pytest.raises(ValueError, match="shapely"):
-        Layout().add_pour((0, 0, 1e-3, 1e-3), 0.2e-3)
-    with pytest.raises(ValueError, match="shapely"):
-        Layout().add_polygon(None)
+    with pytest.raises(ValueError, match="antipads"):
+        Layout().add_via(0, 0, antipad=0.8e-3)
+    lay = Layout().add_rect(0, 0, 1e-3, 1e-3)
+    with pytest.raises(ValueError, match="grid"):
+        lay.mesh(0.2e-3, kind="tri")
 
 
 def test_layout_rejects_oblique_and_duplicate_gaps():

@@ -59,3 +59,141 @@ def test_rwg_topology_invariants():
     assert np.all(counts == 2)
     # total area equals the copper area
     assert m.area.sum() == pytest.approx((3 + 5) * mm * 1 * mm, rel=1e-12)
+
+
+# ------------------------------------------------------- polygons and pours
+def same_mesh(a, b):
+    assert a.n_unknowns == b.n_unknowns and a.n_vias == b.n_vias
+    np.testing.assert_array_equal(a.xg, b.xg)
+    np.testing.assert_array_equal(a.yg, b.yg)
+    assert digest(a) == digest(b)
+    np.testing.assert_array_equal([v.xy for v in a.vias], [v.xy for v in b.vias])
+
+
+def test_polygon_copper_meshes_like_rectangles():
+    """An L-shaped polygon and the two rectangles it is made of give the same
+    grid lines, cells and RWG functions."""
+    from mom25d_amd import planar
+
+    def build(poly):
+        lay = Layout()
+        if poly:
+            lay.add_polygon(planar.Polygon([[(0, 0), (4 * mm, 0), (4 * mm, 1 * mm), (1 * mm, 1 * mm),
+                                             (1 * mm, 3 * mm), (0, 3 * mm)]]))
+        else:
+            lay.add_rect(0, 0, 4 * mm, 1 * mm)
+            lay.add_rect(0, 1 * mm, 1 * mm, 3 * mm)
+        lay.add_port("P1", (4 * mm, 0.5 * mm), "+x", 1 * mm, 3 * mm)
+        return lay.mesh(0.3 * mm, 0.05 * mm)
+
+    same_mesh(build(True), build(False))
+
+
+def test_oblique_polygon_is_a_staircase():
+    """Cells of a triangle drawn as a polygon: exactly those whose centre lies
+    strictly inside its three half-planes (Engine/mom25d/mesh.py:543-552)."""
+    lay = Layout()
+    lay.add_polygon([(0, 0), (4 * mm, 0), (0, 3 * mm)])
+    m = lay.mesh(0.2 * mm)
+    xc, yc = 0.5 * (m.xg[1:] + m.xg[:-1]), 0.5 * (m.yg[1:] + m.yg[:-1])
+    X, Y = np.meshgrid(xc, yc, indexing="ij")
+    want = (X > 0) & (Y > 0) & (3 * X + 4 * Y < 12 * mm)
+    assert len(m.tris) == 2 * int(want.sum())
+    # the two axis-aligned legs are grid lines, the hypotenuse is not
+    assert 0.0 in m.xg and 0.0 in m.yg
+
+
+def pour_line(pour=True, **kw):
+    """Line with feeds in a pour (Engine/tests/test_vias.py:62-80)."""
+    w = 0.5561 * mm
+    lay = Layout()
+    lay.add_rect(-3 * mm, -w / 2, 3 * mm, w / 2)
+    lay.add_port("P1", (-3 * mm, 0), "-x", w, 4 * mm)
+    lay.add_port("P2", (3 * mm, 0), "+x", w, 4 * mm)
+    if pour:
+        lay.add_pour((-7 * mm, -2 * mm, 7 * mm, 2 * mm), 0.2 * mm, **kw)
+    return lay, w
+
+
+def test_pour_clearance_and_fence():
+    lay, w = pour_line(via_pitch=1 * mm, via_drill=0.3 * mm, via_pad=0.5 * mm)
+    G = 0.2 * mm
+    (geom, polys, vias), = lay.resolved_pours()
+    assert len(polys) == 2
+    ys = sorted({float(y) for p in polys for y in p.rings[0][:, 1]})
+    assert ys == pytest.approx([-2 * mm, -(w / 2 + G), w / 2 + G, 2 * mm], abs=1e-15)
+    # fence: each core rectangle (13.2 mm x (2 - 0.4 - w/2 - G - 0.4) mm) carries
+    # ceil(perimeter / pitch) vias
+    per = 2 * (13.2 * mm + (1.2 * mm - w / 2 - G))
+    assert len(vias) == 2 * int(np.ceil(per / (1 * mm)))
+    a = np.linspace(0, 2 * np.pi, 64, endpoint=False)
+    for v in vias:  # pads inside the pour
+        r = 0.99 * v.pad / 2
+        assert np.all(geom.contains(v.xy[0] + r * np.cos(a), v.xy[1] + r * np.sin(a)))
+        assert v.top == "top" and v.bottom == "gnd" and v.drill == 0.3 * mm
+    m = lay.mesh(0.4 * mm)
+    assert m.n_vias == len(vias)
+    assert np.any(np.isclose(m.yg, w / 2 + G))
+    assert lay.layer_names() == ["top"]
+
+
+def test_pour_fill_equals_rectangles_and_via_grid():
+    """A pour with a fill grid meshes exactly like its two rectangles plus
+    the vias of the fill formula (Engine/mom25d/mesh.py:293-298: a pitch grid
+    over the core's bounds, centred, kept where inside the core)."""
+    pitch, drill, pad = 0.9 * mm, 0.3 * mm, 0.5 * mm
+    lay, w = pour_line(fill_pitch=pitch, via_drill=drill, via_pad=pad)
+    G, off = 0.2 * mm, 0.5 * (pad + drill)
+    ref, _ = pour_line(False)
+    ref.add_rect(-7 * mm, -2 * mm, 7 * mm, -(w / 2 + G))
+    ref.add_rect(-7 * mm, w / 2 + G, 7 * mm, 2 * mm)
+    x0, x1 = -7 * mm + off, 7 * mm - off
+    y0, y1 = -2 * mm + off, 2 * mm - off
+    xs = np.arange(x0, x1 + 1e-12, pitch)
+    ys = np.arange(y0, y1 + 1e-12, pitch)
+    X, Y = np.meshgrid(xs + 0.5 * (x1 - xs[-1]), ys + 0.5 * (y1 - ys[-1]))
+    keep = (X > x0) & (X < x1) & (np.abs(Y) > w / 2 + G + off) & (np.abs(Y) < y1)
+    for x, y in zip(X[keep], Y[keep]):
+        ref.add_via(float(x), float(y), drill=drill, pad=pad)
+    assert len(lay.all_vias()) == int(keep.sum()) > 20
+    same_mesh(lay.mesh(0.4 * mm, 0.1 * mm), ref.mesh(0.4 * mm, 0.1 * mm))
+
+
+def test_pour_clears_via_pads_and_reaches_other_layers():
+    """A via pad on the pour layer is cleared by G around its 32-gon; pour
+    vias to an inner layer add that layer (Engine/mom25d/mesh.py:508-513)."""
+    lay, w = pour_line()
+    lay.vias.clear()
+    v = lay.add_via(0.0, 1.2 * mm, drill=0.3 * mm, pad=0.6 * mm)
+    lay.add_pour((-5 * mm, -1 * mm, 5 * mm, 1 * mm), 0.15 * mm, layer="inner", fill_pitch=2 * mm, via_to="top")
+    geoms = lay.resolved_pours()
+    g0 = geoms[0][0]
+    a = np.linspace(0, 2 * np.pi, 200)
+    apothem = 0.3 * mm * np.cos(np.pi / 32) + 0.2 * mm
+    assert not np.any(g0.contains(v.xy[0] + 0.99 * apothem * np.cos(a), v.xy[1] + 0.99 * apothem * np.sin(a)))
+    r = 1.01 * apothem / np.cos(np.pi / 32)
+    assert np.all(g0.contains(v.xy[0] + r * np.cos(a), v.xy[1] + r * np.sin(a)))
+    assert lay.layer_names() == ["top", "inner"]
+    inner = geoms[1][2]
+    assert inner and all(q.top == "inner" and q.bottom == "top" for q in inner)
+
+
+def test_pour_fence_matches_reference_positions():
+    """The 160 fence vias of a 40 mm GCPW pour (line and feeds spanning the
+    pour, gap 0.2 mm, 1.2 mm wide, 1 mm pitch, 0.3 / 0.5 mm via) at the
+    positions the reference's Pour.resolve wrote for its openEMS model
+    (Engine/tests/data/gcpw_pour_openems_vias.txt, micrometres; the case
+    'gcpw' of Engine/tests/data/ref_vias_openems.py:26-34)."""
+    import os
+
+    ref = np.loadtxt(os.path.join(os.path.dirname(__file__), "golden", "gcpw_pour_fence_vias.txt"))
+    w, G, WP = 0.5561 * mm, 0.2 * mm, 1.2 * mm
+    lay = Layout()
+    lay.add_rect(-5 * mm, -w / 2, 5 * mm, w / 2)
+    lay.add_port("P1", (-5 * mm, 0), "-x", w, 15 * mm)
+    lay.add_port("P2", (5 * mm, 0), "+x", w, 15 * mm)
+    y1 = w / 2 + G + WP
+    lay.add_pour((-20 * mm, -y1, 20 * mm, y1), G, via_pitch=1 * mm, via_drill=0.3 * mm, via_pad=0.5 * mm)
+    got = np.array([v.xy for v in lay.all_vias()]) / 1e-6
+    assert got.shape == ref.shape
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-6)
