@@ -36,8 +36,9 @@ def test_pour_line_vs_openems(pour_runs):
     S = r.renormalize(50.0)
     print("eeff", r.eeff[:, 0], "Z0", r.z0_line[:, 0].real, "|S11|", np.abs(S[:, 0, 0]), "openEMS", np.abs(s11r),
           "dphase21", np.angle(S[:, 1, 0] / s21r, deg=True))
-    assert np.all((r.eeff[:, 0] > 2.70) & (r.eeff[:, 0] < 2.78))
-    assert np.all((r.z0_line[:, 0].real > 47.5) & (r.z0_line[:, 0].real < 48.8))
+    # the reference's values for this line at 3 and 11 GHz, with its tolerances
+    np.testing.assert_allclose(r.eeff[:, 0], [2.725, 2.749], rtol=3e-3)
+    np.testing.assert_allclose(r.z0_line[:, 0].real, [47.8, 48.2], rtol=5e-3)
     assert np.all(np.abs(np.angle(S[:, 1, 0] / s21r, deg=True)) < 2.5)
     np.testing.assert_allclose(np.abs(S[:, 0, 0]), np.abs(s11r), atol=0.015)
     # reciprocal and passive; the lossless substrate leaves only radiation
