@@ -60,6 +60,103 @@ int mom_near_statics(int n_items, const void* args, const double* pair_dz, doubl
 int mom_fill_args_size(void);
 int mom_pair_tab_size(void);
 
+/* ---- plan object of the fill (one per mesh) ----------------------------
+ * The arrays of the reference's frequency-independent set-up,
+ * AssemblyGeometry (Engine/mom25d/solver.py:307-426), flattened: host
+ * pointers, row-major, read during mom_plan_create only.
+ *   V (nt,3,2), qpts (nt,7,2), qw (nt,7), mpts (nt,3,2), mw (nt,3),
+ *   fpts (nt,2) centroids, fw (nt) areas, cA / cV (nt,3), tri_basis (nt,3)
+ *   (geom.b, -1: no basis), zl (n_levels);
+ *   test blocks (geom.blocks[k][:2]): blk_off (n_blocks+1) into blk_tris,
+ *   blk_layer (n_blocks) = lt;
+ *   source groups (geom.blocks[k][3], in order): grp_block, grp_layer (ls),
+ *   grp_tier (0: near, the group has statics; 1: mid, 3-point rule;
+ *   2: far, centroid), grp_off (n_groups+1) into grp_src (= sel);
+ *   pad_pos (nt): row of each triangle in IVpad, -1, or NULL without vias. */
+typedef struct MomAssemblyGeometry {
+  int32_t nt, nb, n_levels, n_blocks, n_groups, pad_;
+  const double* V;
+  const double* qpts;
+  const double* qw;
+  const double* mpts;
+  const double* mw;
+  const double* fpts;
+  const double* fw;
+  const double* cA;
+  const double* cV;
+  const int64_t* tri_basis;
+  const double* zl;
+  const int64_t* blk_off;
+  const int64_t* blk_tris;
+  const int32_t* blk_layer;
+  const int32_t* grp_block;
+  const int32_t* grp_layer;
+  const int32_t* grp_tier;
+  const int64_t* grp_off;
+  const int64_t* grp_src;
+  const int64_t* pad_pos;
+} MomAssemblyGeometry;
+
+/* One fill: jwmu = j omega mu0, ijwe = 1 / (j omega eps0) (the reference's
+ * jwmu / inv_jwe, solver.py:462-463); the spline tables of the frequency as
+ * device arrays of this library's layout (pairs, tab_x, tab_c, tab_b), or
+ * pairs = NULL for the plan's own (mom_plan_set_pair); Z the complex128 RWG
+ * block (nb x nb at row stride ldz), IVpad complex128 (n_pad, nt) or NULL
+ * without pads; stage_max (0: 64) and tier_mask (0: all three tiers). */
+typedef struct MomFillCall {
+  double jwmu_re, jwmu_im, ijwe_re, ijwe_im;
+  const void* pairs;
+  const double* tab_x;
+  const double* tab_c;
+  const void* tab_b;
+  void* Z;
+  int64_t ldz;
+  void* IVpad;
+  int32_t stage_max, tier_mask;
+} MomFillCall;
+
+typedef struct MomPlanInfo {
+  int32_t n_items[3];
+  int32_t n_pad;
+  int64_t acc_bytes[3];
+  int64_t tier_pairs[3];
+  int64_t near_static_len;
+  int64_t n_src;
+} MomPlanInfo;
+
+/* sizeof of the three blocks above, for bindings to check their mirrors. */
+int mom_plan_struct_sizes(int32_t* out3);
+/* Builds the plan of one mesh (work items of <= 64 sources per block and
+ * group, scatter maps, distance bounds); with on_device, uploads it on the
+ * current device and computes the near tier's statics on `stream`
+ * (synchronised before returning).  Replaces the per-sweep set-up that
+ * assemble's loop of _fast.chunk_vals re-derives per block
+ * (Engine/mom25d/solver.py:307-426, 472-483). */
+int mom_plan_create(const MomAssemblyGeometry* g, int on_device, void* stream, void** plan);
+int mom_plan_destroy(void* plan);
+int mom_plan_info(const void* plan, MomPlanInfo* info);
+/* Read access to the plan's host arrays by name (src, items0..2, item_r0..2,
+ * blk_tris, blk_row_off, blk_rows, blk_rloc, item_cols, src_cloc,
+ * item_soff), valid until mom_plan_destroy. */
+int mom_plan_host_array(const void* plan, const char* name, const void** data, int64_t* count);
+/* The spline of level pair (lt, ls) for the plan's own tables, in the
+ * reference's layout (_fast.spline_data, Engine/mom25d/_fast.py:35-45):
+ * nodes x (n), PPoly coefficients c (4, n-1, 2) complex128 of the smooth
+ * kernels [S_A, S_V], the pair's dz and the coefficients cA, cV of its
+ * 1/(4 pi R) terms (pair.cA, pair.cV); zero marks an uncoupled pair. */
+int mom_plan_set_pair(void* plan, int lt, int ls, int n, const double* x, const double* c, double dz,
+                      double cA_re, double cA_im, double cV_re, double cV_im, int zero);
+/* Adds the RWG block of Z and IVpad (device pointers, zeroed by the caller)
+ * on `stream`: what assemble's loop of _fast.chunk_vals + scatter adds into
+ * Z[:nb, :nb] and IVpad (Engine/mom25d/solver.py:472-483,
+ * Engine/mom25d/_fast.py:145-211). */
+int mom_plan_fill(void* plan, const MomFillCall* call, void* stream);
+/* The same into host arrays (numpy: Z at row stride ldz, IVpad (n_pad, nt));
+ * stores the values (as chunk_vals + scatter into the zeroed Z of
+ * assemble do) and returns when they are there.  Device staging is
+ * plan-owned and reused. */
+int mom_plan_fill_host(void* plan, const MomFillCall* call);
+
 /* Z[b_i, b_j] += zs * g[t][i][j]: surface-impedance (conductor loss) term,
  * Engine/mom25d/solver.py:511-513. */
 int mom_gram_add(int nt, const double* g, const int32_t* tb, double zs_re, double zs_im, double* Z, int64_t ldz,

@@ -27,6 +27,14 @@ SIGNATURES = {
     "mom_near_statics": (_i, [_i, _p, _p, _p, _p]),
     "mom_fill_args_size": (_i, []),
     "mom_pair_tab_size": (_i, []),
+    "mom_plan_struct_sizes": (_i, [_p]),
+    "mom_plan_create": (_i, [_p, _i, _p, _p]),
+    "mom_plan_destroy": (_i, [_p]),
+    "mom_plan_info": (_i, [_p, _p]),
+    "mom_plan_host_array": (_i, [_p, ctypes.c_char_p, _p, _p]),
+    "mom_plan_set_pair": (_i, [_p, _i, _i, _i, _p, _p, _d, _d, _d, _d, _d, _i]),
+    "mom_plan_fill": (_i, [_p, _p, _p]),
+    "mom_plan_fill_host": (_i, [_p, _p]),
     "mom_gram_add": (_i, [_i, _p, _p, _d, _d, _p, _i64, _p]),
     "mom_via_blocks": (_i, [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _d, _d, _p, _i64, _p]),
     "mom_symmetrize": (_i, [_i, _p, _i64, _p, _i64, _p, _p]),

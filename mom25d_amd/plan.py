@@ -1,4 +1,4 @@
-"""Frequency-independent fill plan of one mesh (host set-up, device arrays).
+"""Frequency-independent fill plan of one mesh.
 
 Mirrors the set-up of Engine/mom25d/solver.py:294-426 (`_morton_order`,
 `AssemblyGeometry`): quadrature points and area-scaled weights, RWG
@@ -12,8 +12,11 @@ coefficients cA = s l / (2A) and cV = s l / A, Morton-ordered test blocks of
 The tier decides which source rule enters Z, so it is computed here with the
 same floating-point expressions as the reference (the 3D distance is the
 planar norm, squared again and combined with dz) and never re-derived on the
-device.  The plan is then flattened into work items of <= 64 source
-triangles for the GPU fill, and uploaded once.
+device.  These arrays -- the reference's AssemblyGeometry, flattened -- go
+through the library's plan entry point (mom_plan_create), which turns them
+into work items of <= 64 source triangles for the GPU fill, uploads them
+and computes the near tier's closed forms; the same entry point serves a
+caller holding the reference's own AssemblyGeometry.
 """
 
 import ctypes
@@ -28,21 +31,50 @@ BLOCK = 32
 CHUNK = 64
 NEAR, MID, FAR = 0, 1, 2
 
-_PTR_FIELDS = ("V", "qp7", "qw7", "qp3", "qw3", "cen", "area", "cA", "cV", "tb", "pad_pos", "blk_tris", "items",
-               "src")
+class MomAssemblyGeometry(ctypes.Structure):
+    """Mirror of MomAssemblyGeometry (include/mom25d.h)."""
+
+    _fields_ = [(n, ctypes.c_int32) for n in ("nt", "nb", "n_levels", "n_blocks", "n_groups", "pad_")] + [
+        (n, ctypes.c_void_p) for n in ("V", "qpts", "qw", "mpts", "mw", "fpts", "fw", "cA", "cV", "tri_basis", "zl",
+                                       "blk_off", "blk_tris", "blk_layer", "grp_block", "grp_layer", "grp_tier",
+                                       "grp_off", "grp_src", "pad_pos")]
 
 
-class FillArgs(ctypes.Structure):
-    """Mirror of the FillArgs block of csrc/fill.hip (size checked at first use)."""
+class MomFillCall(ctypes.Structure):
+    """Mirror of MomFillCall (include/mom25d.h)."""
 
-    _fields_ = [(n, ctypes.c_void_p) for n in _PTR_FIELDS] + [
-        ("nt", ctypes.c_int32), ("pairs", ctypes.c_void_p), ("tab_x", ctypes.c_void_p), ("tab_c", ctypes.c_void_p),
-        ("tab_b", ctypes.c_void_p), ("jwmu_re", ctypes.c_double), ("jwmu_im", ctypes.c_double),
-        ("ijwe_re", ctypes.c_double), ("ijwe_im", ctypes.c_double), ("Z", ctypes.c_void_p), ("ldz", ctypes.c_int64),
-        ("IVpad", ctypes.c_void_p), ("blk_row_off", ctypes.c_void_p), ("blk_rows", ctypes.c_void_p),
-        ("blk_rloc", ctypes.c_void_p), ("item_cols", ctypes.c_void_p), ("src_cloc", ctypes.c_void_p),
-        ("acc_bytes", ctypes.c_int64), ("item_r", ctypes.c_void_p), ("stage_max", ctypes.c_int32),
-        ("pad2_", ctypes.c_int32), ("near_static", ctypes.c_void_p), ("item_soff", ctypes.c_void_p)]
+    _fields_ = [(n, ctypes.c_double) for n in ("jwmu_re", "jwmu_im", "ijwe_re", "ijwe_im")] + [
+        (n, ctypes.c_void_p) for n in ("pairs", "tab_x", "tab_c", "tab_b", "Z")] + [
+        ("ldz", ctypes.c_int64), ("IVpad", ctypes.c_void_p), ("stage_max", ctypes.c_int32),
+        ("tier_mask", ctypes.c_int32)]
+
+
+class MomPlanInfo(ctypes.Structure):
+    """Mirror of MomPlanInfo (include/mom25d.h)."""
+
+    _fields_ = [("n_items", ctypes.c_int32 * 3), ("n_pad", ctypes.c_int32), ("acc_bytes", ctypes.c_int64 * 3),
+                ("tier_pairs", ctypes.c_int64 * 3), ("near_static_len", ctypes.c_int64), ("n_src", ctypes.c_int64)]
+
+
+def check_struct_sizes():
+    """The three plan blocks have the library's sizes."""
+    out = (ctypes.c_int32 * 3)()
+    _lib.call("mom_plan_struct_sizes", out)
+    want = [ctypes.sizeof(t) for t in (MomAssemblyGeometry, MomFillCall, MomPlanInfo)]
+    if list(out) != want:
+        raise RuntimeError(f"libmom25d plan structures differ from the Python mirrors: {list(out)} != {want}")
+
+
+def geometry_struct(arrays):
+    """A MomAssemblyGeometry over the dict of contiguous numpy arrays
+    `arrays` (the caller keeps them alive while the struct is used)."""
+    g = MomAssemblyGeometry()
+    for k in ("nt", "nb", "n_levels", "n_blocks", "n_groups"):
+        setattr(g, k, int(arrays[k]))
+    for k, _ in MomAssemblyGeometry._fields_[6:]:
+        a = arrays.get(k)
+        setattr(g, k, None if a is None else a.ctypes.data)
+    return g
 
 
 def morton_order(xy):
@@ -59,7 +91,11 @@ def morton_order(xy):
 
 
 class FillPlan:
-    """Host arrays of the plan; `upload()` makes the device copies."""
+    """Host arrays of the plan; `upload()` makes the device plan."""
+
+    # LDS budget of one fill workgroup's accumulator (complex128 entries),
+    # as the library splits work items
+    ACC_MAX_BYTES = 136 * 1024
 
     def __init__(self, mesh, zl=None, near_factor=3.0, far_factor=10.0, block=BLOCK):
         V, A = mesh.verts, mesh.area
@@ -84,20 +120,10 @@ class FillPlan:
         self.cA = mesh.tri_sign * lab / (2 * A[:, None])
         self.cV = mesh.tri_sign * lab / A[:, None]
 
+        # test blocks and, per block and source layer, the tiers' source lists
         blocks, blk_layer = [], []
-        src_lists = []
-        per_tier = {NEAR: [], MID: [], FAR: []}
-        n_src = 0
-        self.tier_pairs = np.zeros(3, np.int64)  # triangle pairs per tier
+        grp_block, grp_layer, grp_tier, grp_src = [], [], [], []
         cen = self.cen
-        # Morton rank of every triangle within its layer: the source lists are
-        # chunked in this order, so that a chunk is a compact patch whose
-        # edges (the columns it adds into) are few and close together.
-        mrank = np.zeros(nt, np.int64)
-        for ls in range(nl):
-            on_s = np.nonzero(tl == ls)[0]
-            if on_s.size:
-                mrank[on_s[morton_order(cen[on_s])]] = np.arange(on_s.size)
         for lt in range(nl):
             on_t = np.nonzero(tl == lt)[0]
             if not on_t.size:
@@ -119,37 +145,14 @@ class FillPlan:
                     tiers = (src[rel < near_factor], src[(rel >= near_factor) & (rel < far_factor)],
                              src[rel >= far_factor])
                     for tier, sel in zip((NEAR, MID, FAR), tiers):
-                        if not sel.size:
-                            continue
-                        sel = sel[np.argsort(mrank[sel], kind="stable")]
-                        self.tier_pairs[tier] += sel.size * pb.size
-                        src_lists.append(sel.astype(np.int32))
-                        for c0 in range(0, sel.size, CHUNK):
-                            cnt = min(CHUNK, sel.size - c0)
-                            per_tier[tier].append((bi, lt * nl + ls, n_src + c0, cnt))
-                        n_src += sel.size
+                        if sel.size:
+                            grp_block.append(bi)
+                            grp_layer.append(ls)
+                            grp_tier.append(tier)
+                            grp_src.append(sel)
         self.n_blocks = len(blocks)
-        bt = -np.ones((len(blocks), block), np.int32)
-        for i, pb in enumerate(blocks):
-            bt[i, :pb.size] = pb
-        self.blk_tris = bt
-        self.blk_layer = np.array(blk_layer, np.int32)
         self.blocks = blocks
-        self.src = np.concatenate(src_lists) if src_lists else np.zeros(0, np.int32)
         self.n_levels = nl
-        self._scatter_maps(blocks, per_tier, block)
-        self._item_bounds(blocks)
-        # near work items: offsets of their closed-form statics (test triangles x 7 points x 3 values x sources)
-        it = self.items[NEAR].astype(np.int64).reshape(-1, 6)
-        ntest = np.array([pb.size for pb in blocks], np.int64)[it[:, 0]]
-        size = 21 * ntest * it[:, 3]
-        self.item_soff = np.concatenate([[0], np.cumsum(size)[:-1]]).astype(np.int64)
-        self.near_static_len = int(size.sum())
-
-        # conductor-loss Gram matrix per triangle, times cA_i cA_j
-        X = self.qp7[:, None, :, :] - V[:, :, None, :]
-        G = np.einsum("tk,tikd,tjkd->tij", self.qw7, X, X)
-        self.gram = (self.cA[:, :, None] * self.cA[:, None, :] * G).reshape(nt, 9)
 
         # via segments: pad triangles and the drain matrix D (pad x segment)
         segs = getattr(mesh, "via_segments", [])
@@ -187,138 +190,100 @@ class FillPlan:
             self.d_val = np.array(vals, float)
         else:
             self.pad_tris = np.zeros(0, int)
+
+        # the reference's AssemblyGeometry, flattened for mom_plan_create
+        c64 = lambda a, t=np.float64: np.ascontiguousarray(a, t)  # noqa: E731
+        grp_off = np.concatenate([[0], np.cumsum([g.size for g in grp_src])]).astype(np.int64)
+        self.geometry_arrays = dict(
+            nt=nt, nb=self.nb, n_levels=nl, n_blocks=len(blocks), n_groups=len(grp_src),
+            V=c64(V), qpts=c64(self.qp7), qw=c64(self.qw7), mpts=c64(self.qp3), mw=c64(self.qw3), fpts=c64(cen),
+            fw=c64(A), cA=c64(self.cA), cV=c64(self.cV), tri_basis=c64(b, np.int64), zl=c64(self.zl),
+            blk_off=np.concatenate([[0], np.cumsum([pb.size for pb in blocks])]).astype(np.int64),
+            blk_tris=c64(np.concatenate(blocks), np.int64), blk_layer=c64(blk_layer, np.int32),
+            grp_block=c64(grp_block, np.int32), grp_layer=c64(grp_layer, np.int32),
+            grp_tier=c64(grp_tier, np.int32), grp_off=grp_off,
+            grp_src=c64(np.concatenate(grp_src) if grp_src else np.zeros(1), np.int64),
+            pad_pos=c64(self.pad_pos, np.int64) if nv else None)
+        self._read_host_plan()
+
+        # conductor-loss Gram matrix per triangle, times cA_i cA_j
+        X = self.qp7[:, None, :, :] - V[:, :, None, :]
+        G = np.einsum("tk,tikd,tjkd->tij", self.qw7, X, X)
+        self.gram = (self.cA[:, :, None] * self.cA[:, None, :] * G).reshape(nt, 9)
         self.dev = None
+        self.handle = None
 
-    # LDS budget of one fill workgroup's accumulator (complex128 entries)
-    ACC_MAX_BYTES = 136 * 1024
+    def _create(self, on_device, stream=None):
+        check_struct_sizes()
+        g = geometry_struct(self.geometry_arrays)
+        h = ctypes.c_void_p()
+        _lib.call("mom_plan_create", ctypes.byref(g), int(on_device), stream, ctypes.byref(h))
+        return h
 
-    def _scatter_maps(self, blocks, per_tier, block):
-        """Distinct rows (test basis functions) of every block and columns
-        (source basis functions) of every work item, with each triangle edge's
-        position in them: the fill kernel sums a work item's contributions per
-        (row, column) in LDS and adds every entry to Z once.  A work item whose
-        accumulator would exceed ACC_MAX_BYTES is split."""
-        tb = self.tb
-        nb = len(blocks)
-        row_off = np.zeros(nb + 1, np.int64)
-        rows, nrow = [], np.zeros(nb, np.int64)
-        rloc = -np.ones((nb, block, 3), np.int32)
-        for bi, pb in enumerate(blocks):
-            e = tb[pb]
-            u = np.unique(e[e >= 0])
-            loc = np.searchsorted(u, np.maximum(e, 0)).astype(np.int32)
-            loc[e < 0] = -1
-            rloc[bi, :pb.size] = loc
-            rows.append(u)
-            nrow[bi] = u.size
-            row_off[bi + 1] = row_off[bi] + u.size
-        self.blk_row_off = row_off.astype(np.int32)
-        self.blk_rows = np.concatenate(rows).astype(np.int32) if rows else np.zeros(0, np.int32)
-        self.blk_rloc = rloc
-        src_cloc = -np.ones((self.src.size, 3), np.int32)
-        cols, col_off = [], 0
-        self.items, self.acc_bytes = {}, {}
-        for tier, lst in per_tier.items():
-            out, acc = [], 0
-            todo = list(lst)
-            while todo:
-                bi, pp, so, cnt = todo.pop(0)
-                e = tb[self.src[so:so + cnt]]
-                u = np.unique(e[e >= 0])
-                nbytes = 16 * int(nrow[bi]) * u.size
-                if nbytes > self.ACC_MAX_BYTES and cnt > 1:
-                    h = cnt // 2
-                    todo[:0] = [(bi, pp, so, h), (bi, pp, so + h, cnt - h)]
-                    continue
-                loc = np.searchsorted(u, np.maximum(e, 0)).astype(np.int32)
-                loc[e < 0] = -1
-                src_cloc[so:so + cnt] = loc
-                out.append((bi, pp, so, cnt, col_off, u.size))
-                cols.append(u)
-                col_off += u.size
-                acc = max(acc, nbytes)
-            self.items[tier] = np.array(out, np.int32).reshape(-1, 6)
-            self.acc_bytes[tier] = acc
-        self.item_cols = np.concatenate(cols).astype(np.int32) if cols else np.zeros(0, np.int32)
-        self.src_cloc = src_cloc
+    def _read_host_plan(self):
+        """The library's work items, scatter maps and bounds, read from a
+        host-only plan (also what the device plan holds)."""
+        h = self._create(False)
+        try:
+            info = MomPlanInfo()
+            _lib.call("mom_plan_info", h, ctypes.byref(info))
 
-    def _item_bounds(self, blocks):
-        """Per work item, bounds [lo, hi] on the in-plane distance R between
-        any test point of its block (7-point rule) and any source point of
-        its tier (7-point rule near, 3-point mid, centroid far): from the
-        bounding boxes' centres and half-diagonals, padded by 1e-9 relative so
-        that the kernel's own rounding of R stays inside.  The fill kernel
-        stages the spline intervals of [lo, hi] in LDS."""
-        def circles(pts, counts):
-            # pts: (sum(counts), 2) grouped by item; centre and radius of each group's bounding box
-            starts = np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(np.int64)
-            lo = np.minimum.reduceat(pts, starts, axis=0)
-            hi = np.maximum.reduceat(pts, starts, axis=0)
-            return 0.5 * (lo + hi), 0.5 * np.hypot(*(hi - lo).T)
+            def arr(name, dtype, shape=(-1,)):
+                p, n = ctypes.c_void_p(), ctypes.c_int64()
+                _lib.call("mom_plan_host_array", h, name.encode(), ctypes.byref(p), ctypes.byref(n))
+                if n.value == 0:
+                    return np.zeros((0,) + tuple(shape[1:]), dtype)
+                buf = (ctypes.c_char * (n.value * np.dtype(dtype).itemsize)).from_address(p.value)
+                return np.frombuffer(buf, dtype).copy().reshape(shape)
 
-        tp = [self.qp7[pb].reshape(-1, 2) for pb in blocks]
-        bc, br = circles(np.concatenate(tp), [t.shape[0] for t in tp]) if tp else (np.zeros((0, 2)), np.zeros(0))
-        src_pts = {NEAR: self.qp7, MID: self.qp3, FAR: self.cen[:, None, :]}
-        self.item_r = {}
-        for tier, it in self.items.items():
-            if not it.size:
-                self.item_r[tier] = np.zeros((0, 2))
-                continue
-            so, cnt = it[:, 2].astype(np.int64), it[:, 3].astype(np.int64)
-            idx = np.repeat(so - np.concatenate([[0], np.cumsum(cnt)[:-1]]), cnt) + np.arange(cnt.sum())
-            pts = src_pts[tier][self.src[idx]].reshape(-1, 2)
-            sc, sr = circles(pts, cnt * src_pts[tier].shape[1])
-            D = np.hypot(*(sc - bc[it[:, 0]]).T)
-            r = br[it[:, 0]] + sr
-            lo = np.maximum(0.0, (D - r) * (1 - 1e-9) - 1e-12)
-            self.item_r[tier] = np.stack([np.where(D - r > 0, lo, 0.0), (D + r) * (1 + 1e-9) + 1e-12], 1)
+            self.items = {t: arr(f"items{t}", np.int32, (-1, 6)) for t in (NEAR, MID, FAR)}
+            self.item_r = {t: arr(f"item_r{t}", np.float64, (-1, 2)) for t in (NEAR, MID, FAR)}
+            self.src = arr("src", np.int32)
+            self.blk_tris = arr("blk_tris", np.int32, (-1, BLOCK))
+            self.blk_row_off = arr("blk_row_off", np.int32)
+            self.blk_rows = arr("blk_rows", np.int32)
+            self.blk_rloc = arr("blk_rloc", np.int32, (-1, BLOCK, 3))
+            self.item_cols = arr("item_cols", np.int32)
+            self.src_cloc = arr("src_cloc", np.int32, (-1, 3))
+            self.item_soff = arr("item_soff", np.int64)
+            self.acc_bytes = {t: int(info.acc_bytes[t]) for t in (NEAR, MID, FAR)}
+            self.tier_pairs = np.array(list(info.tier_pairs), np.int64)
+            self.near_static_len = int(info.near_static_len)
+        finally:
+            _lib.call("mom_plan_destroy", h)
 
     def upload(self, device="cuda"):
-        """Device copies of every array the kernels read (kept on the plan)."""
+        """The device plan (mom_plan_create) and device copies of the arrays
+        the other kernels read (Gram matrix, via drains, line voltages)."""
         if self.dev is not None:
             return self.dev
         dev = torch.device(device)
         f64 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(dev)  # noqa: E731
         i32 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.int32)).to(dev)  # noqa: E731
         m = self.mesh
-        d = dict(
-            V=f64(self.V), qp7=f64(self.qp7), qw7=f64(self.qw7), qp3=f64(self.qp3), qw3=f64(self.qw3),
-            cen=f64(self.cen), area=f64(self.area), cA=f64(self.cA), cV=f64(self.cV), tb=i32(self.tb),
-            pad_pos=i32(self.pad_pos), blk_tris=i32(self.blk_tris), src=i32(self.src if self.src.size else [0]),
-            gram=f64(self.gram), tp=i32(m.tp), vp=i32(m.vp), tm=i32(m.tm), vm=i32(m.vm), tri_layer=i32(m.tri_layer),
-        )
-        for t, it in self.items.items():
-            d[f"items{t}"] = i32(it if it.size else np.zeros((1, 6)))
-            d[f"item_r{t}"] = f64(self.item_r[t] if self.item_r[t].size else np.zeros((1, 2)))
-        d.update(blk_row_off=i32(self.blk_row_off), blk_rows=i32(self.blk_rows if self.blk_rows.size else [0]),
-                 blk_rloc=i32(self.blk_rloc), item_cols=i32(self.item_cols if self.item_cols.size else [0]),
-                 src_cloc=i32(self.src_cloc if self.src_cloc.size else np.zeros((1, 3))))
+        d = dict(V=f64(self.V), qp7=f64(self.qp7), qw7=f64(self.qw7), tb=i32(self.tb), cV=f64(self.cV),
+                 gram=f64(self.gram), tp=i32(m.tp), vp=i32(m.vp), tm=i32(m.tm), vm=i32(m.vm),
+                 tri_layer=i32(m.tri_layer))
         if self.nv:
-            d.update(d_off=i32(self.d_off), d_pad=i32(self.d_pad), d_val=f64(self.d_val),
-                     pad_tri=i32(self.pad_tris))
-        d["item_soff"] = torch.from_numpy(self.item_soff if self.item_soff.size else np.zeros(1, np.int64)).to(dev)
-        d["near_static"] = torch.empty(max(self.near_static_len, 1), dtype=torch.float64, device=dev)
-        n_near = int(self.items[NEAR].shape[0])
-        if n_near:
-            a = self.args(d)
-            a.items = d["items0"].data_ptr()
-            pair_dz = f64(np.abs(self.zl[:, None] - self.zl[None, :]).ravel())
-            with torch.cuda.device(dev):
-                _lib.call("mom_near_statics", n_near, ctypes.byref(a), pair_dz.data_ptr(), d["near_static"].data_ptr(),
-                          _lib.stream_ptr())
-                torch.cuda.current_stream().synchronize()
+            d.update(d_off=i32(self.d_off), d_pad=i32(self.d_pad), d_val=f64(self.d_val), pad_tri=i32(self.pad_tris))
+        with torch.cuda.device(dev):
+            self.handle = self._create(True, _lib.stream_ptr())
         self.dev = d
         return d
 
-    def args(self, d):
-        """A FillArgs block with the plan's device arrays `d` (upload()) set;
-        the caller adds the frequency data, Z and the work items of a tier."""
-        a = FillArgs()
-        for name in ("V", "qp7", "qw7", "qp3", "qw3", "cen", "area", "cA", "cV", "tb", "pad_pos", "blk_tris", "src",
-                     "blk_row_off", "blk_rows", "blk_rloc", "item_cols", "src_cloc", "near_static", "item_soff"):
-            setattr(a, name, d[name].data_ptr())
-        a.nt = self.nt
-        return a
+    def fill(self, call, stream, tier_mask=0):
+        """mom_plan_fill of the device plan with a MomFillCall."""
+        call.tier_mask = tier_mask
+        _lib.call("mom_plan_fill", self.handle, ctypes.byref(call), stream)
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                _lib.call("mom_plan_destroy", h)
+            except Exception:
+                pass
+            self.handle = None
 
     # Algorithmic FP64 flops per triangle pair, counted from the pair
     # kernel's loop body with sqrt and divide as one flop each: per test point

@@ -30,8 +30,7 @@ from . import _lib, analytic
 from .constants import C0, EPS0, MU0
 from .deembed import VOLTAGE_WEIGHT, FitContext, fit_waves, reciprocal_z0, sparams_point
 from .greens import SweepTables
-from .plan import FillArgs as _FillArgs
-from .plan import FillPlan
+from .plan import FillPlan, MomFillCall, check_struct_sizes
 
 
 @dataclass
@@ -325,8 +324,7 @@ class SweepResult:
 
 def _check_abi():
     L = _lib.lib()
-    if L.mom_fill_args_size() != ctypes.sizeof(_FillArgs):
-        raise RuntimeError("libmom25d FillArgs layout differs from the Python mirror")
+    check_struct_sizes()
     from .greens import PAIR_DTYPE
 
     if L.mom_pair_tab_size() != PAIR_DTYPE.itemsize:
@@ -770,29 +768,20 @@ class Solver:
             ivpad.zero_()
         P = _lib.ptr
         tabs = g.tables
-        a = plan.args(d)
-        a.pairs = g.pairs_dev.data_ptr()
-        a.tab_x, a.tab_c, a.tab_b = tabs.tab_x.data_ptr(), tabs.tab_c.data_ptr(), tabs.tab_b.data_ptr()
-        a.jwmu_re, a.jwmu_im = jwmu.real, jwmu.imag
-        a.ijwe_re, a.ijwe_im = ijwe.real, ijwe.imag
-        a.Z = Z.data_ptr()
-        a.ldz = Z.stride(0)
-        a.IVpad = ivpad.data_ptr() if ivpad is not None else None
+        call = MomFillCall(jwmu_re=jwmu.real, jwmu_im=jwmu.imag, ijwe_re=ijwe.real, ijwe_im=ijwe.imag,
+                           pairs=g.pairs_dev.data_ptr(), tab_x=tabs.tab_x.data_ptr(), tab_c=tabs.tab_c.data_ptr(),
+                           tab_b=tabs.tab_b.data_ptr(), Z=Z.data_ptr(), ldz=Z.stride(0),
+                           IVpad=ivpad.data_ptr() if ivpad is not None else None, stage_max=self.fill_stage)
         stream = _lib.stream_ptr()
-        for tier in (0, 1, 2):
-            items = plan.items[tier]
-            if not items.size:
-                continue
-            a.items = d[f"items{tier}"].data_ptr()
-            a.item_r = d[f"item_r{tier}"].data_ptr()
-            a.stage_max = self.fill_stage
-            a.acc_bytes = plan.acc_bytes[tier]
-            ev = None
-            if self.fill_events is not None:  # timing for a roofline report, on the launch's stream
+        if self.fill_events is None:
+            plan.fill(call, stream)
+        else:  # one launch per tier, timed on the launch's stream for a roofline report
+            for tier in (0, 1, 2):
+                if not plan.items[tier].size:
+                    continue
                 ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                 ev[0].record()
-            _lib.call("mom_fill_tier", tier, int(items.shape[0]), ctypes.byref(a), stream)
-            if ev is not None:
+                plan.fill(call, stream, 1 << tier)
                 ev[1].record()
                 self.fill_events.append((tier, ev[0], ev[1], float(plan.FLOPS_PER_PAIR[tier] * plan.tier_pairs[tier])))
         zs = self.stack.surface_impedance(g.freq)

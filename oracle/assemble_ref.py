@@ -52,10 +52,12 @@ class Geometry:
         self.qw = qw = TRI7_W[None, :] * A[:, None]
         mp, mw = points(V, TRI3_BARY), TRI3_W[None, :] * A[:, None]
         fp, fw = mesh.centroid[:, None, :], A[:, None]
+        self.mpts, self.mw, self.fpts, self.fw = mp, mw, fp, fw  # attribute names of solver.py:345
         size = np.max(np.linalg.norm(V - np.roll(V, 1, axis=1), axis=-1), axis=1)
         cen, tl = mesh.centroid, mesh.tri_layer
         nl = len(mesh.layer_names)
         zl = np.zeros(nl) if zl is None else np.asarray(zl, float)
+        self.zl = zl
         self.b = b = mesh.tri_basis
         lab = np.where(b >= 0, mesh.edge_len[np.maximum(b, 0)], 0.0)
         self.cA = cA = mesh.tri_sign * lab / (2 * A[:, None])

@@ -198,12 +198,13 @@ def test_struct_layouts_match():
     import ctypes
 
     from mom25d_amd import _lib
-    from mom25d_amd.solver import _FillArgs
+    from mom25d_amd.plan import check_struct_sizes
 
     if not os.path.exists(_lib.LIB_PATH):
         pytest.skip("libmom25d.so not built")
     L = _lib.lib()
-    assert L.mom_fill_args_size() == ctypes.sizeof(_FillArgs)
+    check_struct_sizes()
+    assert L.mom_fill_args_size() > 0
     assert L.mom_pair_tab_size() == PAIR_DTYPE.itemsize
     assert L.mom_abi_version() == 1
     assert L.mom_getrf_work_bytes(64, 1000, 64) > 0
@@ -318,3 +319,120 @@ def test_given_matrix_symmetry_flag():
     B = A + A.T
     B[1, 3] += 1e-15
     assert not _GivenZ(B).symmetric
+
+
+def _host_arrays(handle, names):
+    import ctypes
+
+    from mom25d_amd import _lib
+
+    out = {}
+    for name, dtype in names:
+        p, n = ctypes.c_void_p(), ctypes.c_int64()
+        _lib.call("mom_plan_host_array", handle, name.encode(), ctypes.byref(p), ctypes.byref(n))
+        buf = (ctypes.c_char * max(n.value * np.dtype(dtype).itemsize, 1)).from_address(p.value)
+        out[name] = np.frombuffer(buf, dtype)[:n.value].copy()
+    return out
+
+
+@pytest.mark.parametrize("name", ["tee", "via2"])
+def test_reference_binding_builds_the_product_plan(name):
+    """The reference-side binding (tests/ref_fill_binding.py), handed an
+    AssemblyGeometry of the reference's shape (the oracle's restatement),
+    makes the library build exactly the plan the product builds from its
+    own set-up: same work items, distance bounds and scatter maps."""
+    from mom25d_amd import _lib
+    from mom25d_amd.plan import FillPlan
+    from oracle.assemble_ref import Geometry
+    from tests.ref_fill_binding import GpuFill
+
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libmom25d.so not built")
+    lay, st, mk, _ = getattr(W, name)()
+    m = lay.mesh(**mk)
+    zl = np.array([(st.to_stackup() if hasattr(st, "to_stackup") else st).level_z(n) for n in m.layer_names])
+    plan = FillPlan(m, zl=zl)
+    b = GpuFill(_lib.LIB_PATH, Geometry(m, zl=zl), on_device=False)
+    try:
+        names = [(f"items{t}", np.int32) for t in range(3)] + [(f"item_r{t}", np.float64) for t in range(3)]
+        names += [(k, np.int32) for k in ("src", "blk_tris", "blk_rows", "blk_rloc", "item_cols", "src_cloc")]
+        got = _host_arrays(b.plan, names + [("item_soff", np.int64)])
+    finally:
+        b.close()
+    for t in range(3):
+        np.testing.assert_array_equal(got[f"items{t}"].reshape(-1, 6), plan.items[t])
+        np.testing.assert_array_equal(got[f"item_r{t}"].reshape(-1, 2), plan.item_r[t])
+    for k in ("src", "blk_tris", "blk_rows", "blk_rloc", "item_cols", "src_cloc", "item_soff"):
+        np.testing.assert_array_equal(got[k], getattr(plan, k).ravel())
+
+
+def test_plan_rejects_bad_geometry():
+    """Malformed AssemblyGeometry arrays are refused on the host with a
+    message, before anything reaches a device."""
+    import ctypes
+
+    from mom25d_amd import _lib
+    from mom25d_amd.plan import geometry_struct
+
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libmom25d.so not built")
+    lay, st, mk, _ = W.tiny()
+    from mom25d_amd.plan import FillPlan
+
+    good = FillPlan(lay.mesh(**mk)).geometry_arrays
+    cases = {"triangle is in two test blocks": dict(blk_tris=np.zeros_like(good["blk_tris"])),
+             "tri_basis entry outside": dict(tri_basis=good["tri_basis"] + good["nb"] + 5),
+             "group block, layer or tier out of range": dict(grp_tier=good["grp_tier"] + 3),
+             "source triangle outside": dict(grp_src=good["grp_src"] + good["nt"])}
+    for msg, change in cases.items():
+        a = dict(good, **change)
+        g = geometry_struct(a)
+        h = ctypes.c_void_p()
+        with pytest.raises(RuntimeError, match=msg):
+            _lib.call("mom_plan_create", ctypes.byref(g), 0, None, ctypes.byref(h))
+
+
+def test_plan_own_tables_from_reference_splines():
+    """Splines handed over pair by pair in the layout of _fast.spline_data
+    (nodes x, PPoly coefficients (4, n-1, 2) complex) become the fill
+    kernel's tables: the same nodes, each interval's coefficients as
+    [power][kernel][re, im], the interval lookup of greens.bucket_table at
+    half the smallest node spacing, and the pair records."""
+    from mom25d_amd import _lib
+    from mom25d_amd.greens import BUCKET_DTYPE, PAIR_DTYPE, bucket_table
+    from oracle.solve_ref import OracleSolver
+    from tests.ref_fill_binding import GpuFill
+
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libmom25d.so not built")
+    lay, st, mk, _ = W.via2(max_cell=0.5e-3, edge_cell=0.15e-3)
+    m = lay.mesh(**mk)
+    o = OracleSolver(m, st)
+    greens = o.greens(7e9)
+    b = GpuFill(_lib.LIB_PATH, o.geometry(), on_device=False)
+    try:
+        b.set_greens(greens)
+        got = _host_arrays(b.plan, [("tab_x", np.float64), ("tab_c", np.float64), ("tab_b", BUCKET_DTYPE),
+                                    ("pairs", PAIR_DTYPE)])
+    finally:
+        b.close()
+    nl = len(m.layer_names)
+    xs, cs, bs = [], [], []
+    for q, rec in enumerate(got["pairs"]):
+        pr = greens.pair(q // nl, q % nl)
+        x, c = pr._spl.x, pr._spl.c
+        assert rec["x_off"] == sum(a.size for a in xs) and rec["c_off"] == sum(a.size for a in cs)
+        assert rec["b_off"] == sum(a.size for a in bs)
+        assert rec["n"] == x.size and rec["dz"] == pr.dz and rec["zero"] == 0
+        assert (rec["cA_re"], rec["cA_im"], rec["cV_re"], rec["cV_im"]) == (complex(pr.cA).real, complex(pr.cA).imag,
+                                                                          complex(pr.cV).real, complex(pr.cV).imag)
+        dxb = 0.5 * np.diff(x).min()
+        assert rec["inv_dx"] == 1.0 / dxb
+        xs.append(x)
+        cs.append(np.ascontiguousarray(c.transpose(1, 0, 2)).view(np.float64).reshape(-1))
+        bs.append(bucket_table(x, dxb))
+    np.testing.assert_array_equal(got["tab_x"], np.concatenate(xs))
+    np.testing.assert_array_equal(got["tab_c"], np.concatenate(cs))
+    want_b = np.concatenate(bs)
+    np.testing.assert_array_equal(got["tab_b"]["i0"], want_b["i0"])
+    np.testing.assert_array_equal(got["tab_b"]["split"], want_b["split"])
