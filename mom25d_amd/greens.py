@@ -215,7 +215,7 @@ class SweepTables:
     """
 
     def __init__(self, stack, freqs, rho_max, levels, single, device="cuda", gl_order=12, surrogate=None,
-                 surrogate_tol=2e-3):
+                 surrogate_tol=5e-5):
         self._setup(stack, rho_max, levels, single, device)
         self.freqs = np.atleast_1d(np.asarray(freqs, float))
         jobs = [self._prepare(f, gl_order) for f in self.freqs]
@@ -376,26 +376,48 @@ class SweepTables:
             o += r.size
         return parts
 
-    def _surrogate_remainders(self, jobs, sur, tol, n_check=16):
+    @staticmethod
+    def surrogate_nodes(rho, n_fit, n_check):
+        """Table-node indices where the surrogate's tables take the integrals:
+        fit nodes log-spaced in rho over the whole table (both ends, about
+        n_fit of them once coincident ones merge), and up to n_check check
+        nodes, each midway between two fit nodes, spread over the table."""
+        n = rho.size
+        x = np.log(np.maximum(rho, 0.5 * rho[1]))
+        fit = np.unique(np.round(np.interp(np.linspace(x[0], x[-1], n_fit), x, np.arange(n))).astype(int))
+        gaps = np.nonzero(np.diff(fit) >= 2)[0]
+        mids = (fit[gaps] + fit[gaps + 1]) // 2
+        chk = mids[np.unique(np.round(np.linspace(0, mids.size - 1, min(n_check, mids.size))).astype(int))] \
+            if mids.size else np.zeros(0, int)
+        return fit, chk, x
+
+    def _surrogate_remainders(self, jobs, sur, tol, n_fit=64, n_check=16):
         """Remainder columns of every job from the surrogate networks (R_A,
-        R_V, R_W at every table node), judged by the Sommerfeld integrals at
-        n_check nodes per frequency (log-spaced over the table, ends
-        included): a frequency whose largest error there, relative to the
-        largest |R| of the same kernel and level pair, exceeds tol takes all
-        its columns from the integrals."""
+        R_V, R_W at every table node), corrected by the integrals at the fit
+        nodes and judged by them at the check nodes (surrogate_nodes):
+
+          - the networks' error in the demodulated remainder T at the fit
+            nodes, interpolated over log rho by a cubic spline, is taken off
+            every node (the network is smooth in log rho, so is its error),
+            and the fit nodes keep the integrals' values;
+          - a frequency whose largest error at the check nodes, relative to
+            the largest |R| of the same kernel and level pair, exceeds tol
+            takes all its columns from the integrals."""
         import time
+
+        from scipy.interpolate import CubicSpline
 
         L = self.L
         LL = L * L
         f_lo, f_hi = sur.f_range
         if self.rho_max > sur.rho_range[1] or any(not (f_lo * (1 - 1e-9) <= j["f"] <= f_hi * (1 + 1e-9)) for j in jobs):
             self.surrogate_report = dict(check_err=np.full(len(jobs), np.inf), fallback=list(range(len(jobs))),
-                                         tol=tol, n_check=n_check, seconds={}, integral_nodes=None,
+                                         tol=tol, n_fit=n_fit, n_check=n_check, seconds={}, integral_nodes=None,
                                          reason="outside the trained rho / frequency range")
             return self._sommerfeld(jobs)
         t0 = time.perf_counter()
-        chk_idx = [np.unique(np.round(np.geomspace(1, j["rho"].size, n_check)).astype(int) - 1) for j in jobs]
-        resC = self._sommerfeld(jobs, rhos=[j["rho"][c] for j, c in zip(jobs, chk_idx)])
+        nodes = [self.surrogate_nodes(j["rho"], n_fit, n_check) for j in jobs]
+        exact = self._sommerfeld(jobs, rhos=[j["rho"][np.concatenate([fi, ci])] for j, (fi, ci, _) in zip(jobs, nodes)])
         t1 = time.perf_counter()
         cols = []
         for j in jobs:
@@ -411,14 +433,23 @@ class SweepTables:
         o = 0
         for fi, j in enumerate(jobs):
             n = j["rho"].size
+            r = j["rho"]
             R = np.empty((n, 3 * LL), complex)
             for p in range(LL):
                 sl = slice(o + p * n, o + (p + 1) * n)
                 R[:, p], R[:, LL + p], R[:, 2 * LL + p] = RA[sl], RV[sl], RW[sl]
             o += LL * n
-            ref = resC[fi]
-            scale = np.maximum(np.abs(ref).max(0), 1e-300)
-            err = float(np.max(np.abs(R[chk_idx[fi]] - ref).max(0) / scale))
+            # demodulation of each column: A with kx_A, V and W with kx_V of its pair
+            kx = np.concatenate([j["kx"][0].ravel(), j["kx"][1].ravel(), j["kx"][1].ravel()])
+            dem = (4 * np.pi * np.sqrt(r ** 2 + sur.h ** 2))[:, None] * np.exp(1j * np.outer(r, kx))
+            fit, chk, x = nodes[fi]
+            ex_fit, ex_chk = exact[fi][:fit.size], exact[fi][fit.size:]
+            E = (ex_fit - R[fit]) * dem[fit]
+            sp = CubicSpline(x[fit], np.concatenate([E.real, E.imag], 1))(x)
+            R = R + (sp[:, :3 * LL] + 1j * sp[:, 3 * LL:]) / dem
+            R[fit] = ex_fit
+            scale = np.maximum(np.abs(exact[fi]).max(0), 1e-300)
+            err = float(np.max(np.abs(R[chk] - ex_chk).max(0) / scale)) if chk.size else np.inf
             errs.append(err)
             if not err <= tol:
                 fallback.append(fi)
@@ -428,11 +459,11 @@ class SweepTables:
             for fi, F in zip(fallback, full):
                 res[fi] = F
         t3 = time.perf_counter()
-        nodes = sum(c.size * (j["kc"].size + j["kr"].size) for j, c in zip(jobs, chk_idx))
-        nodes += sum(jobs[fi]["rho"].size * (jobs[fi]["kc"].size + jobs[fi]["kr"].size) for fi in fallback)
-        self.surrogate_report = dict(check_err=np.array(errs), fallback=fallback, tol=tol, n_check=n_check,
-                                     seconds=dict(integrals=t1 - t0 + (t3 - t2), network=t2 - t1),
-                                     integral_nodes=nodes, reason=None)
+        used = sum((f.size + c.size) * (j["kc"].size + j["kr"].size) for j, (f, c, _) in zip(jobs, nodes))
+        used += sum(jobs[fi]["rho"].size * (jobs[fi]["kc"].size + jobs[fi]["kr"].size) for fi in fallback)
+        self.surrogate_report = dict(check_err=np.array(errs), fallback=fallback, tol=tol, n_fit=n_fit,
+                                     n_check=n_check, seconds=dict(integrals=t1 - t0 + (t3 - t2), network=t2 - t1),
+                                     integral_nodes=used, reason=None)
         return res
 
     def integral_nodes(self, jobs):

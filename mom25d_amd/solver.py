@@ -470,7 +470,7 @@ class Solver:
 
     def __init__(self, mesh, substrate, z0=None, verbose=True, method="auto", v_weight=None, z0_recip_min=None,
                  tand_floor=None, precision="fp64", device="cuda", panel=None, refine_tol=1e-15, refine_maxit=20,
-                 lanes=None, surrogate=None, surrogate_tol=2e-3, workers=None, graphs=False, autotune=False,
+                 lanes=None, surrogate=None, surrogate_tol=5e-5, workers=None, graphs=False, autotune=False,
                  gmres_restart=120, reuse_lu=True, lu_equiv=None, lu64_steps=20, nopiv=True, solve_sb=512,
                  ldlt=True):
         from .layered import Stackup
