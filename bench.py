@@ -10,7 +10,12 @@ chosen with --precision: "mixed" (default: FP64 fill, complex64 LU on the
 matrix cores as the preconditioner of FP64 GMRES refinement, each lane's LU
 reused for its neighbouring frequencies while that pays) or "fp64" (the
 plain path, a complex128 LU per point); the other path is timed beside it on
-the same frequencies and |dS| between the two is reported.  Inputs (mesh, fill plan) are resident on the
+the same frequencies and |dS| between the two is reported.  The headline
+path takes the Green's remainders of its tables from the trained networks of
+the workload's stack when their weights exist (mom25d_amd/data/
+surrogate_<workload>.npz; corrected and judged by the integrals at a few
+nodes per frequency, --surrogate off for the plain tables); the other path
+always uses the plain tables.  Inputs (mesh, fill plan) are resident on the
 GPU before the timed region; the Green's tables of the timed frequencies are
 built inside it.
 
@@ -156,7 +161,7 @@ def measured_traffic(kernel, workload):
         return None
     with open(path) as fh:
         table = json.load(fh)
-    return table.get(f"{kernel.split(' (')[0]} {workload}", {}).get("bytes_per_launch")
+    return table.get("kernels", {}).get(f"{kernel.split(' (')[0]} {workload}", {}).get("bytes_per_launch")
 
 
 def batch_random16(torch, precision):
@@ -238,6 +243,10 @@ def main():
                     help="mixed path: GMRES steps after which a complex64 solve gives way to a complex128 anchor")
     ap.add_argument("--solve-sb", type=int, default=512,
                     help="mixed path: super-block rows of the preconditioner solves (0: the 64-row chain)")
+    ap.add_argument("--surrogate", default="auto", choices=("auto", "off"),
+                    help="headline path: Green's remainders from the trained networks of the workload's stack "
+                         "(mom25d_amd/data/surrogate_<workload>.npz, corrected and judged by the integrals); "
+                         "auto: when the weights exist")
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="no HIP events around the timed sweep's launches (then no roofline object)")
     ap.add_argument("--hw-queues", type=int, default=16,
@@ -271,9 +280,23 @@ def main():
     tile = tuple(int(x) for x in args.tile.split(",")) if args.tile else None
     mixed_kw = dict(lu_equiv=args.lu_equiv, lu64_steps=args.lu64_steps if args.lu64_steps >= 0 else None,
                     solve_sb=args.solve_sb)
+    head_kw = dict(mixed_kw if prec == "mixed" else {})
+    sur_path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "mom25d_amd", "data",
+                            f"surrogate_{args.workload}.npz")
+    if args.surrogate == "auto" and os.path.exists(sur_path):
+        from mom25d_amd.surrogate import GreensSurrogate
+
+        head_kw["surrogate"] = GreensSurrogate.load(sur_path, sub.to_stackup() if hasattr(sub, "to_stackup") else sub)
     sm, rm, tm = time_path(mesh, sub, freqs, prec, args.warmup, args.steps, torch, args.lanes, not args.no_tune,
-                           args.panel, tile, args.block, timing=not args.no_kernel_timing,
-                           **(mixed_kw if prec == "mixed" else {}))
+                           args.panel, tile, args.block, timing=not args.no_kernel_timing, **head_kw)
+    sur_line = None
+    if "surrogate" in head_kw:
+        rep = sm.last_tables.surrogate_report
+        sur_line = dict(weights=os.path.relpath(sur_path, os.path.dirname(os.path.abspath(__file__))),
+                        accepted=int(len(rep["check_err"]) - len(rep["fallback"])), tables=int(len(rep["check_err"])),
+                        tol=rep["tol"], check_err_max=float(np.max(rep["check_err"])),
+                        bessel_ratio=float(rep["integral_nodes"] / sm.last_tables.integral_nodes_total))
+        print(f"surrogate remainders: {sur_line}", file=sys.stderr, flush=True)
     for rec in sm.point_log:
         print(f"  point {rec['freq'] / 1e9:6.3f} GHz lane {rec['lane']}: " +
               ", ".join(f"{k} {v}" for k, v in rec["solves"]), file=sys.stderr)
@@ -333,6 +356,8 @@ def main():
     extra["lu_stats"] = sweep_lu_stats  # factorisations / reused LUs / GMRES steps in the timed sweep
     if prec == "mixed":
         extra["mixed_policy"] = dict(lu_equiv=sm._lu_equiv(), lu64_steps=sm.lu64_steps, solve_sb=sm.solve_sb)
+    if sur_line is not None:
+        extra["surrogate"] = sur_line  # the headline's tables: networks corrected and judged per frequency
     if interp_line is not None:
         extra["interp_auto"] = interp_line  # the reference's default sweep mode (interpolated Z between anchors)
     if not args.no_batch:
@@ -356,7 +381,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 (complex128 fill and LU)" if prec == "fp64" else "f64 fill / c64 LU + f64 GMRES refinement",
+        "dtype": ("f64 (complex128 fill and LU)" if prec == "fp64" else "f64 fill / c64 LU + f64 GMRES refinement")
+                 + ("; Green's remainders from the surrogate networks" if sur_line else ""),
         "data": "synthetic (seeded / deterministic layouts)",
         "config": {"workload": f"{args.workload}: 40x20 mm via-stitched GCPW pour, N={N} unknowns, "
                                f"{len(freqs)} f 1-20 GHz" if args.workload == "gcpw40" else f"{args.workload}: N={N}",

@@ -219,6 +219,7 @@ class SweepTables:
         self._setup(stack, rho_max, levels, single, device)
         self.freqs = np.atleast_1d(np.asarray(freqs, float))
         jobs = [self._prepare(f, gl_order) for f in self.freqs]
+        self.integral_nodes_total = self.integral_nodes(jobs)
         if surrogate is None:
             res = self._sommerfeld(jobs)
             self.surrogate_report = None

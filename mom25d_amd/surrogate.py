@@ -301,6 +301,7 @@ def train_for_workload(name, out=None, n_samples=1 << 21, epochs=120, batch=4096
     from . import workloads as W
 
     lay, st, mk, freqs = W.ALL[name]()
+    st = st.to_stackup() if hasattr(st, "to_stackup") else st
     levels = sorted(st.metals, key=st.level_z)
     nets = [RemainderSurrogate(st, levels, (freqs[0], freqs[-1]), seed=seed + k, kind=kind)
             for k, kind in enumerate(("AV", "W"))]

@@ -124,3 +124,28 @@ def test_random_batch_matches_reference(golden):
         dS = np.max(np.abs(r.S - fx[f"{name}_S"]))
         print(f"batch {name}: |dS| {dS:.2e}")
         assert dS < 1e-5
+
+
+def test_headline_with_surrogate_matches_reference(golden):
+    """The headline configuration as benchmarked: the 12 428-unknown pour,
+    mixed precision, Green's remainders from the trained networks of its
+    stack (mom25d_amd/data/surrogate_gcpw40.npz, corrected and judged by the
+    integrals per frequency), against the reference's S from its plain
+    tables: the required 1e-4, and the mixed path's own 1e-5."""
+    import os
+
+    from mom25d_amd.surrogate import GreensSurrogate
+
+    fx = golden("config_gcpw40.npz")
+    m, st = _mesh("gcpw40", fx)
+    path = os.path.join(os.path.dirname(__file__), "..", "mom25d_amd", "data", "surrogate_gcpw40.npz")
+    sur = GreensSurrogate.load(path, st.to_stackup())
+    freqs = fx["gcpw40_freqs"]
+    s = Solver(m, st, verbose=False, precision="mixed", surrogate=sur)
+    r = s.sweep(freqs, interp=None)
+    rep = s.last_tables.surrogate_report
+    dS = np.max(np.abs(r.S - fx["gcpw40_S"]))
+    print(f"gcpw40 mixed + surrogate: |dS| {dS:.2e} at {freqs / 1e9} GHz; networks accepted at "
+          f"{len(freqs) - len(rep['fallback'])} of {len(freqs)}, check error {rep['check_err']}")
+    assert not rep["fallback"]
+    assert dS < 1e-5
