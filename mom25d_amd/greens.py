@@ -536,6 +536,96 @@ class SweepTables:
         return len(self.greens)
 
 
+class ZeroPair:
+    """Kernels between levels of different stack sections: a solid plane
+    between them, so none (Engine/mom25d/layered.py:418-434)."""
+
+    zero = True
+
+    def __init__(self, dz):
+        self.dz = float(dz)
+        self.cA = self.cV = self.cW = self.cv = 0j
+
+    def smooth(self, rho):
+        z = np.zeros(np.shape(rho), complex)
+        return z, z
+
+    def smooth_w(self, rho):
+        return np.zeros(np.shape(rho), complex)
+
+    gA = gV = gW = smooth_w
+
+
+class SectionedTables:
+    """Tables of a stack with inner planes (Engine/mom25d/layered.py:437-474):
+    one SweepTables per section -- its own stack over the plane below it,
+    the levels of that section -- merged into one set of device arrays in
+    which the records of level pairs in different sections are marked as
+    uncoupled.  The fill forms no work for those pairs (its plan keeps
+    sources in the test level's section) and the line-voltage kernel skips
+    them; on the host they are ZeroPair."""
+
+    def __init__(self, stack, freqs, rho_max, levels, device="cuda", gl_order=12):
+        self.stack = stack
+        self.levels = list(levels)
+        self.freqs = np.atleast_1d(np.asarray(freqs, float))
+        self.L = L = len(self.levels)
+        self.zl = np.array([stack.level_z(n) for n in self.levels])
+        sec = [stack.section(n) for n in self.levels]
+        subs = {}
+        for q in sorted(set(sec)):
+            names = [n for n, t in zip(self.levels, sec) if t == q]
+            subs[q] = (names, SweepTables(stack.section_stack(q), self.freqs, rho_max, names, False, device, gl_order))
+        base, xs, cs, cws, bs = {}, [], [], [], []
+        x0 = c0 = b0 = 0
+        for q, (names, t) in subs.items():
+            base[q] = (x0, c0, b0)
+            xs.append(t.tab_x)
+            cs.append(t.tab_c)
+            cws.append(t.tab_cw)
+            bs.append(t.tab_b)
+            x0 += t.tab_x.numel()
+            c0 += t.tab_c.numel()
+            b0 += t.tab_b.numel() // BUCKET_DTYPE.itemsize
+        self.tab_x, self.tab_c, self.tab_cw, self.tab_b = (torch.cat(a) for a in (xs, cs, cws, bs))
+        self.greens, rows = [], []
+        for fi, f in enumerate(self.freqs):
+            recs = np.zeros(L * L, PAIR_DTYPE)
+            pairs = {}
+            for i in range(L):
+                for k in range(L):
+                    if sec[i] != sec[k]:
+                        recs[i * L + k]["dz"] = abs(self.zl[i] - self.zl[k])
+                        recs[i * L + k]["zero"] = 1
+                        pairs[i, k] = ZeroPair(abs(self.zl[i] - self.zl[k]))
+                        continue
+                    names, t = subs[sec[i]]
+                    a, b = names.index(self.levels[i]), names.index(self.levels[k])
+                    r = t.pair_records[fi][a * len(names) + b].copy()
+                    xb, cb, bb = base[sec[i]]
+                    r["x_off"] += xb
+                    r["c_off"] += cb
+                    r["b_off"] += bb
+                    recs[i * L + k] = r
+                    pairs[i, k] = t.greens[fi].pair(a, b)
+            rows.append(recs)
+            self.greens.append(FreqGreens(f, float(stack.z[-1]), self.levels, pairs, stack, self, fi))
+        allrec = np.concatenate(rows)
+        self._pairs_blob = torch.from_numpy(allrec.view(np.uint8).copy()).to(device)
+        rec_bytes = PAIR_DTYPE.itemsize * L * L
+        self.pairs_dev = [self._pairs_blob[fi * rec_bytes:(fi + 1) * rec_bytes] for fi in range(len(self.freqs))]
+        self.pair_records = rows
+        self.surrogate_report = None
+        self.integral_nodes_total = sum(t.integral_nodes_total for _, t in subs.values())
+        self.sections = subs
+
+    def __getitem__(self, fi):
+        return self.greens[fi]
+
+    def __len__(self):
+        return len(self.greens)
+
+
 def sommerfeld_remainders(stack, levels, freqs, rhos, rho_max, device="cuda"):
     """Remainders of a (multi-level) stack at arbitrary points: list of
     (n_i, 3, L, L) arrays for rhos[i] at freqs[i], and the kx (2, L, L)."""

@@ -9,8 +9,10 @@ add_component) are axis-aligned bridges as in the reference's grid meshes;
 internal S-parameter ports (add_gap_port, add_via_port) as well;
 polygon copper and copper pours (add_polygon, add_pour) are meshed as a
 staircase on the same grid, their axis-aligned edges becoming grid lines,
-with the plane geometry of planar.py in place of shapely; thick metal,
-apertures and triangular meshing are not part of this build.
+with the plane geometry of planar.py in place of shapely; solid inner
+planes (add_plane) take via ends and split the stack into sections; thick
+metal, antipads, apertures and triangular meshing are not part of this
+build.
 
 Conventions kept from the reference because the numbers depend on them:
   * grid lines: every rectangle edge, every port's reference / gap
@@ -350,6 +352,12 @@ class Layout:
         if not shapes:
             return [], []
         return planar.axis_coords(planar.outline(planar.Region(shapes)), tol)
+
+    def add_aperture(self, plane, region, max_cell=None, edge_cell=None, size=None):
+        """Openings in inner planes (Engine/mom25d/mesh.py:458-486) carry
+        magnetic-current unknowns of their own (aperture.py), which are not
+        part of this build."""
+        raise ValueError("apertures in inner planes are not supported by this build (solid planes only)")
 
     def add_plane(self, name):
         if name not in self.planes:

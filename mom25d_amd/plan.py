@@ -4,7 +4,8 @@ Mirrors the set-up of Engine/mom25d/solver.py:294-426 (`_morton_order`,
 `AssemblyGeometry`): quadrature points and area-scaled weights, RWG
 coefficients cA = s l / (2A) and cV = s l / A, Morton-ordered test blocks of
 32 triangles per metal layer, and the near / mid / far tier of every
-(test block, source triangle) pair:
+(test block, source triangle) pair of levels in one stack section (a solid
+plane between two levels leaves their pairs out):
 
     rel = min over the block of |c_p - c_s|_3D / max(size_p, size_s)
     near: rel < 3,   mid: 3 <= rel < 10,   far: rel >= 10
@@ -97,7 +98,7 @@ class FillPlan:
     # as the library splits work items
     ACC_MAX_BYTES = 136 * 1024
 
-    def __init__(self, mesh, zl=None, near_factor=3.0, far_factor=10.0, block=BLOCK):
+    def __init__(self, mesh, zl=None, near_factor=3.0, far_factor=10.0, block=BLOCK, section=None):
         V, A = mesh.verts, mesh.area
         nt = len(A)
         self.mesh = mesh
@@ -136,7 +137,7 @@ class FillPlan:
                 blk_layer.append(lt)
                 for ls in range(nl):
                     src = np.nonzero(tl == ls)[0]
-                    if not src.size:
+                    if not src.size or (section is not None and section[ls] != section[lt]):
                         continue
                     dz = abs(self.zl[lt] - self.zl[ls])
                     dist = np.linalg.norm(cen[pb][:, None, :] - cen[None, src, :], axis=-1)

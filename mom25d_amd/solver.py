@@ -29,7 +29,7 @@ import torch
 from . import _lib, analytic
 from .constants import C0, EPS0, MU0
 from .deembed import VOLTAGE_WEIGHT, FitContext, fit_waves, reciprocal_z0, sparams_point
-from .greens import SweepTables
+from .greens import SectionedTables, SweepTables
 from .plan import FillPlan, MomFillCall, check_struct_sizes
 
 
@@ -485,10 +485,17 @@ class Solver:
         st = self.stack
         missing = [n for n in mesh.layer_names if n not in st.metals]
         if missing:
-            raise ValueError(f"mesh layers {missing} are not metal levels of the stackup {list(st.metals)}")
+            raise ValueError(f"mesh layers {missing} are not metal levels of the stackup {list(st.metals)} "
+                             "(declare inner planes with Layout.add_plane)")
+        planes = list(getattr(mesh, "planes", []))
+        missing = [n for n in planes if n not in st.planes]
+        if missing:
+            raise ValueError(f"layout planes {missing} are not planes of the stackup {list(st.planes)}")
         self.single = len(st.layers) == 1 and st.top == "air" and mesh.layer_names == ["top"]
         self.zl = np.array([st.level_z(n) for n in mesh.layer_names])
-        order = sorted(mesh.layer_names, key=st.level_z)
+        # levels of different stack sections (a solid plane between them) do not couple
+        self.section = [st.section(n) for n in mesh.layer_names] if st.planes else None
+        order = sorted(mesh.layer_names + planes, key=st.level_z)
         if getattr(mesh, "level_order", None) != order:
             mesh.assign_levels(order)
         self.ports = mesh.layout.ports
@@ -590,13 +597,18 @@ class Solver:
     # ------------------------------------------------------------- set-up
     def geometry(self):
         if self._plan is None:
-            self._plan = FillPlan(self.mesh, zl=self.zl)
+            self._plan = FillPlan(self.mesh, zl=self.zl, section=self.section)
         return self._plan
 
     def tables(self, freqs):
         """Green's-function tables of all frequencies (one GPU launch)."""
-        t = SweepTables(self.stack, freqs, self.rho_max, self.mesh.layer_names, self.single, self.device,
-                        surrogate=self.surrogate, surrogate_tol=self.surrogate_tol)
+        if self.stack.planes:
+            if self.surrogate is not None:
+                raise ValueError("the remainder surrogate is trained for stacks without inner planes")
+            t = SectionedTables(self.stack, freqs, self.rho_max, self.mesh.layer_names, self.device)
+        else:
+            t = SweepTables(self.stack, freqs, self.rho_max, self.mesh.layer_names, self.single, self.device,
+                            surrogate=self.surrogate, surrogate_tol=self.surrogate_tol)
         self.last_tables = t
         return t
 
@@ -741,7 +753,7 @@ class Solver:
         E = (-inv_jwe * 1j) * (S.T @ M @ S).astype(complex)
         st = self.stack
         segs = self.mesh.via_segments
-        vz = {n: st.level_z(n) for n in self.mesh.layer_names}
+        vz = {n: st.level_z(n) for n in self.mesh.layer_names + list(st.planes)}
         vz["gnd"] = 0.0
         spans = [(vz[s["bottom"]], vz[s["top"]]) for s in segs]
         E += via_impedance(plan.vias, spans, omega, g.via_sections(), self._via_cache)
@@ -1521,7 +1533,7 @@ class Solver:
         m = self.mesh
         pos = np.c_[m.edge_mid[:m.n_basis], self.zl[m.edge_layer[:m.n_basis]]]
         if m.n_vias:
-            vz = {n: self.stack.level_z(n) for n in m.layer_names}
+            vz = {n: self.stack.level_z(n) for n in m.layer_names + list(self.stack.planes)}
             vz["gnd"] = 0.0
             zc = [0.5 * (vz[g["bottom"]] + vz[g["top"]]) for g in m.via_segments]
             xy = [m.vias[g["via"]].xy for g in m.via_segments]

@@ -297,3 +297,50 @@ class LayeredGreensRef:
 
     def via_sections(self):
         return self.stack.via_sections(self.freq)
+
+
+class _ZeroPairRef:
+    """No kernels across a solid plane (layered.py:418-434)."""
+
+    zero = True
+
+    def __init__(self, dz):
+        self.dz, self.cA, self.cV, self.cW = dz, 0.0, 0.0, 0.0
+
+    def smooth(self, rho):
+        z = np.zeros(np.shape(rho), complex)
+        return z, z
+
+    def smooth_w(self, rho):
+        return np.zeros(np.shape(rho), complex)
+
+    gA = gV = gW = smooth_w
+
+
+class SectionedGreensRef:
+    """Kernels of a stack with inner planes (layered.py:437-474): one
+    LayeredGreensRef per section over its own section stack (levels measured
+    from the plane below), none between sections."""
+
+    def __init__(self, freq, stack, rho_max=0.1, gl_order=12, zs_ground=0.0, levels=None):
+        self.freq, self.stack = freq, stack
+        self.omega = 2 * np.pi * freq
+        self.k0 = self.omega / C0
+        self.levels = list(levels) if levels is not None else list(stack.metals)
+        self.zl = np.array([stack.level_z(n) for n in self.levels])
+        self.h = float(stack.z[-1])
+        self.sec = [stack.section(n) for n in self.levels]
+        self._sub = {}
+        for q in sorted(set(self.sec)):
+            names = [n for n, t in zip(self.levels, self.sec) if t == q]
+            self._sub[q] = LayeredGreensRef(freq, stack.section_stack(q), rho_max, gl_order, zs_ground, names)
+
+    def pair(self, i, j):
+        i = i if isinstance(i, (int, np.integer)) else self.levels.index(i)
+        j = j if isinstance(j, (int, np.integer)) else self.levels.index(j)
+        if self.sec[i] != self.sec[j]:
+            return _ZeroPairRef(abs(self.zl[i] - self.zl[j]))
+        return self._sub[self.sec[i]].pair(self.levels[i], self.levels[j])
+
+    def via_sections(self):
+        return self.stack.via_sections(self.freq)

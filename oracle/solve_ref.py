@@ -12,7 +12,7 @@ import numpy as np
 import scipy.linalg as sla
 
 from .assemble_ref import EPS0, Geometry, assemble
-from .greens_ref import LayeredGreensRef, SlabGreens
+from .greens_ref import LayeredGreensRef, SectionedGreensRef, SlabGreens
 from .integrals_ref import TRI7_BARY, TRI7_W, points, wilton
 
 
@@ -22,7 +22,8 @@ def make_greens(stack, mesh, f, rho_max):
     if single:
         d = stack.layers[0]
         return SlabGreens(f, d.eps_r, d.t, d.tand, rho_max, zs_ground=zg)
-    return LayeredGreensRef(f, stack, rho_max, zs_ground=zg, levels=mesh.layer_names)
+    cls = SectionedGreensRef if stack.planes else LayeredGreensRef
+    return cls(f, stack, rho_max, zs_ground=zg, levels=mesh.layer_names)
 
 
 class OracleSolver:
@@ -36,7 +37,8 @@ class OracleSolver:
         self.mesh = mesh
         self.stack = substrate if isinstance(substrate, Stackup) else substrate.to_stackup()
         st = self.stack
-        order = sorted(mesh.layer_names, key=st.level_z)
+        planes = list(getattr(mesh, "planes", []))
+        order = sorted(mesh.layer_names + planes, key=st.level_z)
         if getattr(mesh, "level_order", None) != order:
             mesh.assign_levels(order)
         self.ports = mesh.layout.ports
@@ -44,8 +46,9 @@ class OracleSolver:
         v = mesh.nodes
         self.rho_max = 1.02 * np.hypot(*(v.max(0) - v.min(0)))
         self.zl = np.array([st.level_z(n) for n in mesh.layer_names])
-        via_z = {n: st.level_z(n) for n in mesh.layer_names}
+        via_z = {n: st.level_z(n) for n in mesh.layer_names + planes}
         via_z["gnd"] = 0.0
+        self.section = [st.section(n) for n in mesh.layer_names] if st.planes else None
         self.via_z = via_z
         self.use_c = use_c
         self._geom = None
@@ -58,7 +61,7 @@ class OracleSolver:
 
     def geometry(self):
         if self._geom is None:
-            self._geom = Geometry(self.mesh, zl=self.zl, via_z=self.via_z)
+            self._geom = Geometry(self.mesh, zl=self.zl, via_z=self.via_z, section=self.section)
         return self._geom
 
     def greens(self, f):
