@@ -176,6 +176,31 @@ class KernelPair:
         return self.cW / (4 * np.pi * self._R(rho)) + self.smooth_w(rho)
 
 
+def sommerfeld_path(k0, kk, t, rho_max, order=12, tail=True):
+    """Nodes and weights of the Sommerfeld integrals of a stack
+    (Engine/mom25d/layered.py:212-236): a path lifted into the complex plane
+    by a sin(pi k / T) bump of height a = min(k0 / 2, 1 / rho_max) over
+    [0, T], T = 1.3 max |k_i| + k0, in order-point Gauss panels (100..3000),
+    then (tail) the real axis from T to kmax = max(40 max |k_i|, 20 / t_min,
+    1.5 T) in 8-point panels of width min(pi / rho_max, 0.5 / t_min,
+    max |k_i|).  Returns kc, wc, kr, wr (kr, wr empty without the tail)."""
+    kmaxr = np.abs(kk).max()
+    tmin = np.min(t)
+    T = 1.3 * kmaxr + k0
+    a = min(0.5 * k0, 1.0 / rho_max)
+    n1 = int(np.clip(np.ceil(2 * T / a), 100, 3000))
+    tt, ww = gauss_panels(0.0, T, n1, order)
+    kc = tt + 1j * a * np.sin(np.pi * tt / T)
+    wc = ww * (1 + 1j * a * np.pi / T * np.cos(np.pi * tt / T))
+    if not tail:
+        return kc, wc, np.zeros(0), np.zeros(0)
+    kmax = max(40 * kmaxr, 20.0 / tmin, T * 1.5)
+    dk = min(np.pi / rho_max, 0.5 / tmin, kmaxr)
+    n2 = int(np.ceil((kmax - T) / dk))
+    kr, wr = gauss_panels(T, kmax, n2, 8)
+    return kc, wc, kr, wr
+
+
 class FreqGreens:
     """Green's functions of all level pairs at one frequency (host view plus
     the location of its tables in the sweep's device arrays)."""
@@ -258,16 +283,7 @@ class SweepTables:
         kk = k0 * np.sqrt(self.epsc)
         tmin = self.t.min()
         kmaxr = np.abs(kk).max()
-        T = 1.3 * kmaxr + k0
-        a = min(0.5 * k0, 1.0 / self.rho_max)
-        n1 = int(np.clip(np.ceil(2 * T / a), 100, 3000))
-        tt, ww = gauss_panels(0.0, T, n1, order)
-        kc = tt + 1j * a * np.sin(np.pi * tt / T)
-        wc = ww * (1 + 1j * a * np.pi / T * np.cos(np.pi * tt / T))
-        kmax = max(40 * kmaxr, 20.0 / tmin, T * 1.5)
-        dk = min(np.pi / self.rho_max, 0.5 / tmin, kmaxr)
-        n2 = int(np.ceil((kmax - T) / dk))
-        kr, wr = gauss_panels(T, kmax, n2, 8)
+        kc, wc, kr, wr = sommerfeld_path(k0, kk, self.t, self.rho_max, order)
 
         dz = np.abs(self.zl[:, None] - self.zl[None, :])
         c = np.zeros((3, L, L), complex)

@@ -10,9 +10,9 @@ internal S-parameter ports (add_gap_port, add_via_port) as well;
 polygon copper and copper pours (add_polygon, add_pour) are meshed as a
 staircase on the same grid, their axis-aligned edges becoming grid lines,
 with the plane geometry of planar.py in place of shapely; solid inner
-planes (add_plane) take via ends and split the stack into sections; thick
-metal, antipads, apertures and triangular meshing are not part of this
-build.
+planes (add_plane) take via ends and split the stack into sections, and
+vias pass them through modelled antipads (frills); thick metal, apertures
+and triangular meshing are not part of this build.
 
 Conventions kept from the reference because the numbers depend on them:
   * grid lines: every rectangle edge, every port's reference / gap
@@ -39,6 +39,7 @@ import numpy as np
 from . import planar
 
 _AXIS_DIRS = {"+x": (1, 0), "-x": (-1, 0), "+y": (0, 1), "-y": (0, -1)}
+FRILL_RINGS = 4  # radial potential functions per antipad (antipad.py)
 
 
 @dataclass
@@ -114,7 +115,10 @@ class Via:
     """Plated via between two levels (Engine/mom25d/mesh.py:191-241).
 
     xy : centre [m]; drill : hole diameter [m]; pad : pad diameter (default
-    2 drill), added on `top` and on `bottom` unless that is "gnd"."""
+    2 drill), added on `top` and on `bottom` unless that is "gnd";
+    connect : inner planes between top and bottom it connects to; antipad :
+    clearance diameter [m] in the planes it passes (None: no modelled hole,
+    the plane solid around the via), or {plane: diameter}."""
 
     xy: tuple
     drill: float
@@ -128,8 +132,9 @@ class Via:
     def __post_init__(self):
         self.xy = (float(self.xy[0]), float(self.xy[1]))
         self.connect = tuple(self.connect)
-        if self.antipad is not None:
-            raise ValueError("antipads are not supported by this build")
+        sizes = self.antipad.values() if isinstance(self.antipad, dict) else [self.antipad]
+        if any(d is not None and d <= self.drill for d in sizes):
+            raise ValueError("via antipad must be larger than its drill")
         if self.pad is None:
             self.pad = 2 * self.drill
         if self.pad < self.drill:
@@ -139,6 +144,12 @@ class Via:
 
     def pad_layers(self):
         return [n for n in (self.top, self.bottom) if n != "gnd"]
+
+    def antipad_at(self, plane):
+        """Clearance diameter [m] in `plane` (None: no modelled hole)."""
+        if isinstance(self.antipad, dict):
+            return self.antipad.get(plane)
+        return self.antipad
 
 
 @dataclass
@@ -578,26 +589,41 @@ class Mesh:
 
     def assign_levels(self, order):
         """Split vias into vertical segments for a stack order, bottom to top
-        (Engine/mom25d/mesh.py:932-976; antipad frills excluded).  Unknown
-        n_basis + k is the downward current of segment k."""
+        (Engine/mom25d/mesh.py:932-976): at attached copper, at connected
+        planes and at planes passed through a modelled antipad, where a
+        frill node (FRILL_RINGS ring unknowns, antipad.py) joins the segments
+        above and below.  Unknown n_basis + k is the downward current of
+        segment k; the frill rings follow the segments."""
         rank = {n: i for i, n in enumerate(order)}
         rank["gnd"] = -1
         self.via_segments = []
+        self.frills = []  # antipads: FRILL_RINGS unknowns each, after the via segments
         for k, (v, att) in enumerate(zip(self.vias, self.via_attach)):
             for n in (v.top, v.bottom) + v.connect:
                 if n not in rank:
                     raise ValueError(f"via layer {n!r} is not in the stack order {order}")
+            for n in v.connect:
+                if n not in self.planes:
+                    raise ValueError(f"via connects to {n!r}, which is not a declared plane")
             hi, lo = sorted((rank[v.top], rank[v.bottom]), reverse=True)
-            mid = [n for n in order if lo < rank[n] < hi and (n in att or n in v.connect)]
+            between = [n for n in order if lo < rank[n] < hi]
+            # planes passed through a modelled antipad split the via too (no charge there)
+            holes = {n for n in between if n in self.planes and n not in v.connect and v.antipad_at(n) is not None}
+            mid = [n for n in between if n in att or n in v.connect or n in holes]
             chain = [order[hi]] + sorted(mid, key=lambda n: -rank[n]) + [order[lo] if lo >= 0 else "gnd"]
             for a, b in zip(chain[:-1], chain[1:]):
+                if a in holes:
+                    self.frills[-1]["seg_low"] = len(self.via_segments)
                 self.via_segments.append(dict(via=k, top=a, bottom=b, top_tris=att.get(a),
                                               bottom_tris=att.get(b)))
+                if b in holes:
+                    self.frills.append(dict(via=k, plane=b, seg_up=len(self.via_segments) - 1, seg_low=None,
+                                            a=0.5 * v.drill, b=0.5 * v.antipad_at(b), xy=v.xy, nr=FRILL_RINGS))
         self.level_order = list(order)
-        self.frills = []
         self.n_vias = len(self.via_segments)
+        self.n_frills = sum(fr["nr"] for fr in self.frills)
         self.via_tris = [g["top_tris"] for g in self.via_segments]
-        self.n_unknowns = self.n_basis + self.n_vias
+        self.n_unknowns = self.n_basis + self.n_vias + self.n_frills
 
     # --------------------------------------------------------------- ports
     def flow_direction(self):

@@ -569,6 +569,7 @@ class Solver:
         self.surrogate = surrogate
         self.surrogate_tol = surrogate_tol
         self.last_tables = None
+        self._frills = None  # antipad.FrillModel, built at the first fill with antipads
         self._plan = None
         self._bufs = {}
         self._keep = {}
@@ -806,11 +807,28 @@ class Solver:
                       P(d["pad_tri"]), P(d["tp"]), P(d["vp"]), P(d["tm"]), P(d["vm"]), P(d["cV"]), P(E),
                       ijwe.real, ijwe.imag, P(Z), Z.stride(0), stream)
             self._keep[lane] = E
+        if self.mesh.n_frills:
+            self._frill_blocks(g, Z, lane)
         if Z32 is not None:
             self._scale_copy(Z, Z32, lane)
         else:
             _lib.call("mom_symmetrize", n, P(Z), Z.stride(0), None, 0, None, stream)
         return Z
+
+    def _frill_blocks(self, g, Z, lane=0):
+        """The antipad rows and columns of Z (antipad.FrillModel, host):
+        Z[:nj, nj:] = -C, Z[nj:, :nj] = -C^T, Z[nj:, nj:] = -(F + F^T) / 2
+        with nj the RWG and via unknowns (Engine/mom25d/solver.py:946-961)."""
+        from .antipad import FrillModel
+
+        if self._frills is None:
+            self._frills = FrillModel(self)
+        C, F = self._frills.matrix_blocks(g.omega)
+        nj = C.shape[0]
+        blk = self._upload("frill", np.ascontiguousarray(np.vstack([C, F])), lane)
+        Z[:nj, nj:].copy_(blk[:nj])
+        Z[nj:, :nj].copy_(blk[:nj].T)
+        Z[nj:, nj:].copy_(blk[nj:])
 
     def _scale_copy(self, Z, Z32, lane=0):
         """Z <- (Z + Z^T) / 2 and Z32 <- D Z D in complex64 with
@@ -1494,6 +1512,17 @@ class Solver:
                                          shape=(len(m.tris), m.n_unknowns))
         return self._div_op
 
+    def via_currents(self, x):
+        """Downward current of each via segment (mesh.via_segments) in the
+        solution x (Engine/mom25d/solver.py:1093-1095)."""
+        return x[self.mesh.n_basis:self.mesh.n_basis + self.mesh.n_vias]
+
+    def frill_voltages(self, x):
+        """Ring voltages of each antipad (mesh.frills; the first of each the
+        barrel-to-plane voltage) in the solution x
+        (Engine/mom25d/solver.py:1097-1099)."""
+        return x[self.mesh.n_basis + self.mesh.n_vias:]
+
     def charges(self, x, omega):
         """Surface charge per triangle, (Nt, Nexc), from continuity:
         q = -div(J) / (j omega) (Engine/mom25d/solver.py:1076-1091)."""
@@ -1529,7 +1558,8 @@ class Solver:
 
     def _basis_positions(self):
         """Centres of the basis functions (N x 3): RWG edge midpoints on their
-        level, via segments at mid span (Engine/mom25d/freqinterp.py:53-60)."""
+        level, via segments at mid span, antipad rings on the via axis at the
+        plane (Engine/mom25d/freqinterp.py:53-64)."""
         m = self.mesh
         pos = np.c_[m.edge_mid[:m.n_basis], self.zl[m.edge_layer[:m.n_basis]]]
         if m.n_vias:
@@ -1538,6 +1568,8 @@ class Solver:
             zc = [0.5 * (vz[g["bottom"]] + vz[g["top"]]) for g in m.via_segments]
             xy = [m.vias[g["via"]].xy for g in m.via_segments]
             pos = np.r_[pos, np.c_[np.asarray(xy, float).reshape(-1, 2), zc]]
+        for fr in getattr(m, "frills", []):  # antipad rings: on the via axis at the plane
+            pos = np.r_[pos, [[*fr["xy"], self.stack.level_z(fr["plane"])]] * fr["nr"]]
         return np.ascontiguousarray(pos, float)
 
     def _fit_context(self):

@@ -11,6 +11,7 @@ same geometry can be built by any implementation of that API.  Returns
   via2       2-layer top -> via -> inner transition, ~6k RWG, 81 f in 1-15 GHz
   gcpw40     40 x 20 mm via-stitched grounded-coplanar pour, ~12k RWG, 51 f in 1-20 GHz
   gcpw_pour  the reference's pour model: a line in a via-fenced pour, ~5k RWG + 88 vias, 15 f in 1-15 GHz
+  plane_transition  top -> via through an inner plane's antipad -> stripline, ~2k unknowns, 59 f in 0.5-15 GHz
 """
 
 import types
@@ -175,4 +176,27 @@ def gcpw_pour(ns=None, max_cell=0.25 * mm, edge_cell=0.05 * mm):
         np.linspace(1, 15, 15) * GHz
 
 
-ALL = {"tiny": tiny, "tee": tee, "lpf7": lpf7, "via2": via2, "gcpw40": gcpw40, "gcpw_pour": gcpw_pour}
+def plane_transition(ns=None, antipad=0.7 * mm, stitch=True, max_cell=0.3 * mm, edge_cell=0.05 * mm):
+    """Top microstrip (W 0.56 mm) over the inner plane L2 -> via (0.3 / 0.6
+    mm) through an antipad -> stripline (W 0.24 mm) between L2 and ground,
+    four stitching vias L2 -> ground; three 0.254 mm layers, eps_r 3.66
+    (Models/plane_transition/geometry/plane_transition.py:12-29; meshed as in
+    Engine/tests/test_antipad.py:21-35).  antipad None: the via passes the
+    plane without a modelled hole."""
+    ns = ns or _default_ns()
+    H, WT, WI = 0.254 * mm, 0.56 * mm, 0.24 * mm
+    lay = ns.Layout()
+    lay.add_plane("L2")
+    lay.add_rect(-3 * mm, -WT / 2, 0, WT / 2, layer="top")
+    lay.add_rect(0, -WI / 2, 3 * mm, WI / 2, layer="inner")
+    lay.add_port("P1", (-3 * mm, 0), "-x", WT, 12 * mm, layer="top")
+    lay.add_port("P2", (3 * mm, 0), "+x", WI, 12 * mm, layer="inner")
+    lay.add_via(0, 0, drill=0.3 * mm, pad=0.6 * mm, top="top", bottom="inner", antipad=antipad)
+    for a in np.pi / 4 + np.arange(4 if stitch else 0) * np.pi / 2:
+        lay.add_via(1 * mm * np.cos(a), 1 * mm * np.sin(a), drill=0.3 * mm, pad=0.6 * mm, top="L2", bottom="gnd")
+    st = ns.Stackup([ns.Dielectric(H, 3.66)] * 3, {"inner": 1, "top": 3}, planes={"L2": 2})
+    return lay, st, dict(max_cell=max_cell, edge_cell=edge_cell), np.linspace(0.5, 15, 59) * GHz
+
+
+ALL = {"tiny": tiny, "tee": tee, "lpf7": lpf7, "via2": via2, "gcpw40": gcpw40, "gcpw_pour": gcpw_pour,
+       "plane_transition": plane_transition}

@@ -207,6 +207,37 @@ def do_random():
     np.savez_compressed(os.path.join(HERE, "config_random.npz"), **out)
 
 
+def do_planes():
+    """Stacks with an inner plane (the plane transition of
+    Models/plane_transition): S of the reference's plain path for the via
+    passing the plane without a modelled hole (with and without stitching
+    vias) and through antipads of 0.4 / 0.7 / 1.2 mm; for the 0.7 mm antipad
+    the antipad (frill) blocks C and F of the impedance matrix at 2 and
+    15 GHz (antipad.FrillModel.blocks) and the via segments."""
+    from mom25d.antipad import FrillModel
+
+    out = {}
+    cases = (("open", dict(antipad=None, stitch=False), [3.5e9, 11.5e9]),
+             ("stitch", dict(antipad=None, stitch=True), [3.5e9, 11.5e9]),
+             ("anti400", dict(antipad=0.4 * mm), [9.5e9, 15e9]),
+             ("anti700", dict(antipad=0.7 * mm), [9.5e9, 15e9]),
+             ("anti1200", dict(antipad=1.2 * mm), [9.5e9, 15e9]))
+    for name, kw, fs in cases:
+        out.update(_dense_sweep(f"plane_{name}", lambda ns, kw=kw: W.plane_transition(ns, **kw), np.array(fs)))
+    lay, st, mk, _ = W.plane_transition(NS, antipad=0.7 * mm)
+    m = lay.mesh(**mk)
+    s = ref.Solver(m, st, verbose=False, method="dense")
+    out["frill_segments"] = np.array([(g["top"], g["bottom"]) for g in m.via_segments])
+    out["frill_meta"] = np.array([(fr["seg_up"], fr["seg_low"], fr["nr"]) for fr in m.frills])
+    fm = FrillModel(s)
+    for f in (2e9, 15e9):
+        g = s.greens(f)
+        C, F = fm.blocks(g, g.omega)
+        out[f"frill_C_{f / 1e9:g}"] = C
+        out[f"frill_F_{f / 1e9:g}"] = F
+    np.savez_compressed(os.path.join(HERE, "config_planes.npz"), **out)
+
+
 if __name__ == "__main__":
     sets = sys.argv[1:] or ["mesh", "tables", "z", "sweep", "random", "lpf7", "gcpw40"]
     for s in sets:
