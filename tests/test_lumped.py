@@ -97,8 +97,10 @@ def test_parse_grammar_errors():
 def test_layout_rejects_what_this_build_lacks():
     from mom25d_amd.mesh import Layout
 
-    with pytest.raises(ValueError, match="antipads"):
-        Layout().add_via(0, 0, antipad=0.8e-3)
+    with pytest.raises(ValueError, match="apertures"):
+        Layout().add_aperture("L2", (0, 0, 1e-3, 1e-3))
+    with pytest.raises(ValueError, match="antipad"):
+        Layout().add_via(0, 0, drill=0.3e-3, antipad=0.2e-3)  # smaller than the drill
     lay = Layout().add_rect(0, 0, 1e-3, 1e-3)
     with pytest.raises(ValueError, match="grid"):
         lay.mesh(0.2e-3, kind="tri")
