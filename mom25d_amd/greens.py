@@ -23,7 +23,10 @@ interval: c0..c3 of S_A, S_V, re / im) together with a bucket table that
 turns the interval search into one lookup and one comparison (bucket_table).
 """
 
+import functools
+
 import numpy as np
+from scipy.linalg import solve_banded
 import torch
 
 from . import _lib
@@ -35,9 +38,14 @@ PAIR_DTYPE = np.dtype([("x_off", "<i8"), ("c_off", "<i8"), ("b_off", "<i8"), ("n
                        ("cV_im", "<f8"), ("cW_re", "<f8"), ("cW_im", "<f8"), ("zero", "<i4"), ("pad_", "<i4")])
 
 
+@functools.lru_cache(maxsize=None)
+def _leggauss(order):
+    return np.polynomial.legendre.leggauss(order)
+
+
 def gauss_panels(a, b, n_panels, order):
     """Composite Gauss-Legendre nodes / weights on [a, b]."""
-    x, w = np.polynomial.legendre.leggauss(order)
+    x, w = _leggauss(order)
     edges = np.linspace(a, b, n_panels + 1)
     half = 0.5 * np.diff(edges)
     mid = 0.5 * (edges[1:] + edges[:-1])
@@ -46,9 +54,11 @@ def gauss_panels(a, b, n_panels, order):
 
 def rho_grid(lam, h, rmax):
     """Table nodes: fine near the source, never coarser than lam / 80."""
-    r = [0.0]
-    while r[-1] < rmax:
-        r.append(r[-1] + min(lam / 80, max(h / 25, r[-1] / 25)))
+    r, x = [0.0], 0.0
+    fine, floor = lam / 80, h / 25
+    while x < rmax:
+        x = x + min(fine, max(floor, x / 25))
+        r.append(x)
     return np.array(r)
 
 
@@ -114,19 +124,12 @@ def spline_coeffs(x, Y):
     d1 = x[-1] - x[-3]
     dia[-1], sub[-1] = dx[-2], d1
     rhs[-1] = (dx[-1] ** 2 * slope[-2] + (2 * d1 + dx[-1]) * dx[-2] * slope[-1]) / d1
-    # Thomas elimination (the eliminated diagonal stays positive for these systems)
-    cp = np.empty(n)
-    dp = np.empty(Y.shape, complex)
-    cp[0] = sup[0] / dia[0]
-    dp[0] = rhs[0] / dia[0]
-    for i in range(1, n):
-        den = dia[i] - sub[i] * cp[i - 1]
-        cp[i] = sup[i] / den
-        dp[i] = (rhs[i] - sub[i] * dp[i - 1]) / den
-    s = np.empty(Y.shape, complex)
-    s[-1] = dp[-1]
-    for i in range(n - 2, -1, -1):
-        s[i] = dp[i] - cp[i] * s[i + 1]
+    # the tridiagonal system (first and last rows carry the not-a-knot conditions), banded LAPACK solve
+    ab = np.zeros((3, n))
+    ab[0, 1:] = sup[:-1]
+    ab[1] = dia
+    ab[2, :-1] = sub[1:]
+    s = solve_banded((1, 1), ab, rhs, check_finite=False)
     t = (s[:-1] + s[1:] - 2 * slope) / dx[:, None]
     c = np.empty((4, n - 1, Y.shape[1]), complex)
     c[0] = t / dx[:, None]
