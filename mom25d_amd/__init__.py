@@ -10,7 +10,7 @@ import importlib
 
 from . import analytic
 from .layered import GND, Dielectric, Stackup
-from .mesh import Layout, Mesh, Port, Via
+from .mesh import CoplanarGround, Layout, Mesh, Port, Via
 
 mm = 1e-3
 GHz = 1e9
@@ -18,7 +18,7 @@ GHz = 1e9
 _LAZY = {"Solver": "solver", "Substrate": "solver", "SweepResult": "solver", "fit_waves": "deembed",
          "via_impedance": "solver", "via_inductance": "solver", "sweep_batch": "solver"}
 
-__all__ = ["analytic", "GND", "Dielectric", "Stackup", "Layout", "Mesh", "Port", "Via", "Solver", "Substrate",
+__all__ = ["analytic", "GND", "Dielectric", "Stackup", "CoplanarGround", "Layout", "Mesh", "Port", "Via", "Solver", "Substrate",
            "SweepResult", "fit_waves", "via_impedance", "via_inductance", "sweep_batch", "mm", "GHz"]
 
 

@@ -9,7 +9,8 @@ add_component) are axis-aligned bridges as in the reference's grid meshes;
 internal S-parameter ports (add_gap_port, add_via_port) as well;
 polygon copper and copper pours (add_polygon, add_pour) are meshed as a
 staircase on the same grid, their axis-aligned edges becoming grid lines,
-with the plane geometry of planar.py in place of shapely; solid inner
+with the plane geometry of planar.py in place of shapely; GCPW feeds
+(Port.ground) carry side-ground strips and fence vias; solid inner
 planes (add_plane) take via ends and split the stack into sections, and
 vias pass them through modelled antipads (frills); thick metal, apertures
 and triangular meshing are not part of this build.
@@ -43,6 +44,47 @@ FRILL_RINGS = 4  # radial potential functions per antipad (antipad.py)
 
 
 @dataclass
+class CoplanarGround:
+    """Side grounds of a grounded coplanar (GCPW) feed
+    (Engine/mom25d/mesh.py:25-59).
+
+    gap, width : spacing from the line to each ground strip and the strip's
+                 width [m]; one number for both sides or a pair (+t, -t),
+                 where t is the feed's normal (direction turned +90 deg);
+                 a side with gap None or width 0 has no strip
+    via_pitch  : fence via spacing along each strip's centre line (None: no
+                 fence, the strips must reach ground through other copper)
+    via_drill, via_pad : fence via drill and pad diameters [m]; the pad
+                 defaults to min(2 drill, 0.9 strip width)
+    via_to     : level the fence vias end on ("gnd": the ground plane)
+    via_connect : inner planes the fence vias also connect to
+    """
+
+    gap: object
+    width: object
+    via_pitch: float = None
+    via_drill: float = 0.3e-3
+    via_pad: float = None
+    via_to: str = "gnd"
+    via_connect: tuple = ()
+
+    def sides(self):
+        """[(sign of t, gap, width), ...] of the sides that carry a strip."""
+        gaps = tuple(self.gap) if isinstance(self.gap, (tuple, list)) else (self.gap,) * 2
+        widths = tuple(self.width) if isinstance(self.width, (tuple, list)) else (self.width,) * 2
+        out = []
+        for sg, g, w in zip((1, -1), gaps, widths):
+            if g is not None and w:
+                out.append((sg, float(g), float(w)))
+        return out
+
+    def key(self, scale):
+        """Hashable form with lengths in units of `scale` (ports with equal
+        keys share one line calibration)."""
+        return tuple((sg, round(g / scale, 6), round(w / scale, 6)) for sg, g, w in self.sides())
+
+
+@dataclass
 class Port:
     """Microstrip feed with a delta-gap source (Engine/mom25d/mesh.py:62-188).
 
@@ -54,6 +96,8 @@ class Port:
     margin     : distance kept free of near fields in the wave fit (default
                  max(1.5 width, 4 h), applied by the solver)
     layer      : metal layer of the feed
+    ground     : CoplanarGround for a GCPW feed (side ground strips along the
+                 whole feed, from the reference plane out, with fence vias)
     z0_from    : "voltage" or "reciprocity" (how the line impedance is fixed)
     """
 
@@ -71,8 +115,8 @@ class Port:
     def __post_init__(self):
         if self.z0_from not in ("voltage", "reciprocity"):
             raise ValueError("z0_from must be 'voltage' or 'reciprocity'")
-        if self.ground is not None:
-            raise ValueError("coplanar feed grounds are not supported by this build")
+        if self.ground is not None and not isinstance(self.ground, CoplanarGround):
+            raise ValueError("port ground must be a CoplanarGround")
         if not isinstance(self.direction, str) or self.direction not in _AXIS_DIRS:
             raise ValueError(f"direction must be one of {list(_AXIS_DIRS)} (axis-aligned feeds only)")
         self.u = np.array(_AXIS_DIRS[self.direction], float)
@@ -102,12 +146,45 @@ class Port:
         d = np.asarray(xy, float) - np.asarray(self.ref)
         return d @ self.u, d @ self.n
 
-    def rect(self):
+    def _span(self, t0, t1):
+        """Axis-aligned rectangle over the whole feed length between the
+        feed offsets t0 and t1."""
         a = self.axis
         c0, c1 = sorted((self.ref[a], self.coord(self.length)))
-        t = self.ref[1 - a]
-        lo, hi = t - 0.5 * self.width, t + 0.5 * self.width
-        return (c0, lo, c1, hi) if a == 0 else (lo, c0, hi, c1)
+        q0, q1 = sorted(float(self.ref[1 - a] + self.n[1 - a] * t) for t in (t0, t1))
+        return (c0, q0, c1, q1) if a == 0 else (q0, c0, q1, c1)
+
+    def rect(self):
+        return self._span(-0.5 * self.width, 0.5 * self.width)
+
+    def ground_strips(self):
+        """Rectangles (x0, y0, x1, y1) of the GCPW side-ground strips, one
+        per grounded side (none for a microstrip feed)
+        (Engine/mom25d/mesh.py:151-160)."""
+        if self.ground is None:
+            return []
+        W2 = 0.5 * self.width
+        return [self._span(sg * (W2 + g), sg * (W2 + g + w)) for sg, g, w in self.ground.sides()]
+
+    def ground_vias(self):
+        """Fence vias down the centre line of each side-ground strip, at
+        (i + 1/2) via_pitch from the reference plane, leaving out any whose
+        pad would cross the feed's open end (Engine/mom25d/mesh.py:162-178)."""
+        G = self.ground
+        if G is None or not G.via_pitch:
+            return []
+        s = (np.arange(int(np.floor(self.length / G.via_pitch))) + 0.5) * G.via_pitch
+        out = []
+        for sg, g, w in G.sides():
+            pad = min(G.via_pad or 2 * G.via_drill, 0.9 * w)
+            if pad < G.via_drill:
+                raise ValueError(f"port {self.name}: ground strip {w * 1e3:.3g} mm too narrow "
+                                 f"for {G.via_drill * 1e3:.3g} mm fence vias")
+            t = sg * (0.5 * self.width + g + 0.5 * w)
+            for si in s[s <= self.length - 0.5 * pad]:
+                out.append(Via(tuple(self.point(si, t)), G.via_drill, pad, top=self.layer, bottom=G.via_to,
+                               connect=tuple(G.via_connect)))
+        return out
 
 
 @dataclass
@@ -221,6 +298,7 @@ class Layout:
     polys: list = field(default_factory=list)
     poly_layers: list = field(default_factory=list)
     pours: list = field(default_factory=list)
+    ground_rects: set = field(default_factory=set)  # rectangles that are GCPW feed grounds
 
     def add_rect(self, x0, y0, x1, y1, layer="top"):
         self.rects.append((min(x0, x1), min(y0, y1), max(x0, x1), max(y0, y1)))
@@ -231,6 +309,12 @@ class Layout:
         p = Port(name, tuple(ref), direction, width, length, **kw)
         self.ports.append(p)
         self.add_rect(*p.rect(), layer=p.layer)
+        # a GCPW feed's side strips are ground copper: pours join them rather
+        # than clearing around them (Engine/mom25d/mesh.py:570-598)
+        for r in p.ground_strips():
+            self.ground_rects.add(len(self.rects))
+            self.add_rect(*r, layer=p.layer)
+        self.vias += p.ground_vias()
         return p
 
     def add_lumped(self, name, a, b, width, value=None, layer="top"):
@@ -335,17 +419,20 @@ class Layout:
 
     def resolved_pours(self):
         """[(shape, outline, vias), ...] of every pour against the current
-        copper of its layer (Engine/mom25d/mesh.py:516-537); cached until
-        copper is added."""
+        copper of its layer, except GCPW feed grounds and their fence vias
+        (Engine/mom25d/mesh.py:516-537); cached until copper is added."""
         key = (len(self.rects), len(self.polys), len(self.vias), len(self.pours))
         cache = getattr(self, "_pours", None)
         if cache is not None and cache[0] == key:
             return cache[1]
         out = []
+        fence = {v.xy for pt in self.ports for v in pt.ground_vias()}
         for p in self.pours:
-            sig = [planar.Polygon.box(*r) for r, n in zip(self.rects, self.rect_layers) if n == p.layer]
+            sig = [planar.Polygon.box(*r) for k, (r, n) in enumerate(zip(self.rects, self.rect_layers))
+                   if n == p.layer and k not in self.ground_rects]
             sig += [g for g, n in zip(self.polys, self.poly_layers) if n == p.layer]
-            sig += [planar.Polygon.disc(v.xy, 0.5 * v.pad, 32) for v in self.vias if p.layer in v.pad_layers()]
+            sig += [planar.Polygon.disc(v.xy, 0.5 * v.pad, 32) for v in self.vias
+                    if p.layer in v.pad_layers() and v.xy not in fence]
             out.append(p.resolve(planar.Region(sig) if sig else None))
         self._pours = (key, out)
         return out

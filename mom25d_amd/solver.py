@@ -578,7 +578,8 @@ class Solver:
         self.groups = {}
         for p, port in enumerate(self.ports):
             hp = st.level_h(port.layer)
-            key = (round(port.width / hp, 9), port.layer, None, port.z0_from)
+            key = (round(port.width / hp, 9), port.layer,
+                   port.ground.key(hp) if port.ground is not None else None, port.z0_from)
             self.groups.setdefault(key, []).append(p)
 
     def __del__(self):

@@ -25,11 +25,11 @@ GHz = 1e9
 def _default_ns():
     from . import analytic
     from .layered import Dielectric, Stackup
-    from .mesh import Layout
+    from .mesh import CoplanarGround, Layout
     from .solver import Substrate
 
     return types.SimpleNamespace(Layout=Layout, Substrate=Substrate, Stackup=Stackup, Dielectric=Dielectric,
-                                 analytic=analytic)
+                                 CoplanarGround=CoplanarGround, analytic=analytic)
 
 
 RO4003 = dict(eps_r=3.55, h=0.508 * mm, tand=0.0027, sigma=5.8e7)
@@ -176,6 +176,25 @@ def gcpw_pour(ns=None, max_cell=0.25 * mm, edge_cell=0.05 * mm):
         np.linspace(1, 15, 15) * GHz
 
 
+def gcpw_feed(ns=None, max_cell=0.25 * mm, edge_cell=0.05 * mm):
+    """The gcpw_pour line with its pour only between the reference planes
+    and GCPW feeds outside them: side-ground strips (gap 0.2 mm, 1.2 mm
+    wide) with 1 mm-pitch fence vias along each 6 mm feed
+    (Engine/tests/test_gcpw.py:16-25, grid mesh in place of its triangles;
+    N = 4 980 RWG + 64 via unknowns)."""
+    ns = ns or _default_ns()
+    W, G, WP, REF, FEED = 0.5561 * mm, 0.2 * mm, 1.2 * mm, 5 * mm, 6 * mm
+    lay = ns.Layout()
+    lay.add_rect(-REF, -W / 2, REF, W / 2)
+    gnd = ns.CoplanarGround(G, WP, via_pitch=1 * mm, via_drill=0.3 * mm, via_pad=0.5 * mm)
+    lay.add_port("P1", (-REF, 0), "-x", W, FEED, ground=gnd)
+    lay.add_port("P2", (REF, 0), "+x", W, FEED, ground=gnd)
+    y1 = W / 2 + G + WP
+    lay.add_pour((-REF, -y1, REF, y1), G, via_pitch=1 * mm, via_drill=0.3 * mm, via_pad=0.5 * mm)
+    return lay, ns.Substrate(3.66, 0.254 * mm), dict(max_cell=max_cell, edge_cell=edge_cell), \
+        np.linspace(1, 15, 15) * GHz
+
+
 def plane_transition(ns=None, antipad=0.7 * mm, stitch=True, max_cell=0.3 * mm, edge_cell=0.05 * mm):
     """Top microstrip (W 0.56 mm) over the inner plane L2 -> via (0.3 / 0.6
     mm) through an antipad -> stripline (W 0.24 mm) between L2 and ground,
@@ -199,4 +218,4 @@ def plane_transition(ns=None, antipad=0.7 * mm, stitch=True, max_cell=0.3 * mm, 
 
 
 ALL = {"tiny": tiny, "tee": tee, "lpf7": lpf7, "via2": via2, "gcpw40": gcpw40, "gcpw_pour": gcpw_pour,
-       "plane_transition": plane_transition}
+       "gcpw_feed": gcpw_feed, "plane_transition": plane_transition}

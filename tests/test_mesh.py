@@ -197,3 +197,81 @@ def test_pour_fence_matches_reference_positions():
     got = np.array([v.xy for v in lay.all_vias()]) / 1e-6
     assert got.shape == ref.shape
     np.testing.assert_allclose(got, ref, rtol=0, atol=1e-6)
+
+
+def gcpw_feed_layout(fence=True):
+    """10 mm GCPW line whose pour (with a fence) spans only the reference
+    planes; outside them the GCPW feeds carry the side grounds
+    (Engine/tests/test_gcpw.py:16-25)."""
+    from mom25d_amd import CoplanarGround
+
+    w, G, WP = 0.5561 * mm, 0.2 * mm, 1.2 * mm
+    lay = Layout()
+    lay.add_rect(-5 * mm, -w / 2, 5 * mm, w / 2)
+    gnd = CoplanarGround(G, WP, via_pitch=1 * mm, via_drill=0.3 * mm, via_pad=0.5 * mm) if fence else None
+    lay.add_port("P1", (-5 * mm, 0), "-x", w, 6 * mm, ground=gnd)
+    lay.add_port("P2", (5 * mm, 0), "+x", w, 6 * mm, ground=gnd)
+    y1 = w / 2 + G + WP
+    lay.add_pour((-5 * mm, -y1, 5 * mm, y1), G, via_pitch=1 * mm, via_drill=0.3 * mm, via_pad=0.5 * mm)
+    return lay
+
+
+def test_gcpw_feed_geometry():
+    """Side strips G from the feed, WP wide along the whole 6 mm feed, a
+    fence via per millimetre inside each strip, and the pour joined to the
+    strips at the reference plane instead of cleared around them
+    (Engine/tests/test_gcpw.py:28-48)."""
+    from mom25d_amd import CoplanarGround
+
+    w, G, WP = 0.5561 * mm, 0.2 * mm, 1.2 * mm
+    lay = gcpw_feed_layout()
+    p = lay.ports[0]
+    strips = p.ground_strips()
+    assert len(strips) == 2
+    x0, y0, x1, y1 = p.rect()
+    for sx0, sy0, sx1, sy1 in strips:
+        assert (sx0, sx1) == pytest.approx((x0, x1), abs=1e-15)
+        assert min(abs(sy0 - y1), abs(y0 - sy1)) == pytest.approx(G, rel=1e-9)  # gap to the feed
+        assert (sx1 - sx0) * (sy1 - sy0) == pytest.approx(WP * 6 * mm, rel=1e-9)
+    vias = p.ground_vias()
+    assert len(vias) == 2 * 6
+    for v in vias:  # every fence pad inside a strip
+        r = 0.99 * v.pad / 2
+        assert any(sx0 < v.xy[0] - r and v.xy[0] + r < sx1 and sy0 < v.xy[1] - r and v.xy[1] + r < sy1
+                   for sx0, sy0, sx1, sy1 in strips)
+        assert (v.drill, v.pad, v.top, v.bottom) == (0.3 * mm, 0.5 * mm, "top", "gnd")
+    assert len(lay.vias) == 4 * 6  # both feeds' fences are layout vias
+    # the pour meets each strip edge-on at x = -5 mm over the strip's full width
+    (geom, _, _), = lay.resolved_pours()
+    ys = np.linspace(w / 2 + G, w / 2 + G + WP, 9)[1:-1]
+    for sg in (1, -1):
+        assert np.all(geom.contains(np.full(7, -5 * mm + 1e-6), sg * ys))
+        assert np.all(lay.contains(np.full(7, -5 * mm - 1e-6), sg * ys, "top"))
+    m = lay.mesh(0.25 * mm, 0.05 * mm)
+    mid = m.edge_mid[:m.n_basis]
+    joined = np.isclose(mid[:, 0], -5 * mm) & (np.abs(mid[:, 1]) > w / 2 + G) & (np.abs(mid[:, 1]) < w / 2 + G + WP)
+    assert joined.sum() >= 2 * int(WP / (0.25 * mm))  # RWGs across the joint on both sides
+    # a one-sided ground: only the +t strip (+t of a +x feed is +y)
+    q = Layout().add_port("Q", (0, 0), "+x", w, 6 * mm, ground=CoplanarGround((G, None), (WP, 0)))
+    (c,) = q.ground_strips()
+    assert c[1] == pytest.approx(w / 2 + G)
+    with pytest.raises(ValueError, match="too narrow"):  # pad 0.9 x 0.2 mm < drill 0.3 mm
+        Layout().add_port("R", (0, 0), "+x", w, 6 * mm, ground=CoplanarGround(G, 0.2 * mm, via_pitch=1 * mm))
+
+
+def test_gcpw_fence_vias_are_not_cleared():
+    """A pour over the feed too: it clears the feed line but neither the
+    side strips nor their fence via pads, which are ground copper, while an
+    ordinary via pad on the layer is cleared (Engine/mom25d/mesh.py:528-533)."""
+    w, G, WP = 0.5561 * mm, 0.2 * mm, 1.2 * mm
+    lay = gcpw_feed_layout()
+    lay.pours.clear()
+    y1 = w / 2 + G + WP
+    lay.add_pour((-11 * mm, -y1, 5 * mm, y1), G)
+    lay.add_via(0.0, 1.1 * mm, drill=0.3 * mm, pad=0.5 * mm)
+    (geom, _, _), = lay.resolved_pours()
+    a = np.linspace(0, 2 * np.pi, 16, endpoint=False)
+    for v in lay.ports[0].ground_vias():
+        assert np.all(geom.contains(v.xy[0] + 0.3 * mm * np.cos(a), v.xy[1] + 0.3 * mm * np.sin(a)))
+    assert not np.any(geom.contains(0.3 * mm * np.cos(a), 1.1 * mm + 0.3 * mm * np.sin(a)))
+    assert not np.any(geom.contains(np.array([-8 * mm, -8 * mm]), np.array([w / 2 + 0.5 * G, -w / 2 - 0.5 * G])))
