@@ -80,6 +80,13 @@ struct FillArgs {
   // [source lane] with a lane stride of the item's source count
   const double* near_static;
   const int64_t* item_soff;
+  // ownership of Z entries within one tier's launch: entry (row, col) of a
+  // work item is touched by no other item of the launch when both its block
+  // row (row_own, per blk_rows entry) and its item column (col_own, per
+  // item_cols entry) are 1; the flush then adds it with a plain load and
+  // store instead of atomics.  Null: every entry through atomics.
+  const uint8_t* row_own;
+  const uint8_t* col_own;
 };
 
 }  // namespace mom

@@ -24,8 +24,11 @@
 // interval at the start of the bucket and the one node inside it, so the
 // lookup is one load and one comparison (no data-dependent loop), giving
 // exactly scipy's find_interval result (clamped end intervals).
-// Contributions are summed into Z with hardware FP64 atomics; the matrix is
-// symmetrised by a later pass, as the reference does.
+// Contributions are summed into Z with hardware FP64 atomics, except for the
+// entries that only one work item of a launch touches (basis functions whose
+// triangles all sit in the item's test block and source chunk), which are
+// added with plain loads and stores; the matrix is symmetrised by a later
+// pass, as the reference does.
 
 #include "fill.h"
 
@@ -215,8 +218,13 @@ __global__ __launch_bounds__(FillShape<TIER>::threads) void fill_kernel(FillArgs
       }
     }
   }
-  for (int e = tid; e < nr; e += kFillThreads) s_rows[e] = a.blk_rows[r0 + e];
-  for (int e = tid; e < nc; e += kFillThreads) s_cols[e] = a.item_cols[coff + e];
+  // rows and columns with bit 31 set when another work item of this launch
+  // may add to their entries too
+  const bool owned = a.row_own != nullptr && a.col_own != nullptr;
+  for (int e = tid; e < nr; e += kFillThreads)
+    s_rows[e] = a.blk_rows[r0 + e] | (owned && a.row_own[r0 + e] ? 0 : INT32_MIN);
+  for (int e = tid; e < nc; e += kFillThreads)
+    s_cols[e] = a.item_cols[coff + e] | (owned && a.col_own[coff + e] ? 0 : INT32_MIN);
   for (int e = tid; e < 2 * nr * nc; e += kFillThreads) acc[e] = 0.0;
   __syncthreads();
 
@@ -371,12 +379,36 @@ __global__ __launch_bounds__(FillShape<TIER>::threads) void fill_kernel(FillArgs
   else
     pairs(std::false_type{});
   __syncthreads();
-  for (int e = tid; e < nr * nc; e += kFillThreads) {
-    const double re = acc[2 * e], im = acc[2 * e + 1];
-    if (re == 0.0 && im == 0.0) continue;
-    double* dst = a.Z + 2 * ((int64_t)s_rows[e / nc] * a.ldz + s_cols[e % nc]);
-    unsafeAtomicAdd(dst, re);
-    unsafeAtomicAdd(dst + 1, im);
+  // FU entries per thread at a time: the Z values of the owned ones are all
+  // loaded before the first is stored, so that a workgroup's flush waits one
+  // memory round trip rather than one per entry
+  constexpr int FU = 8;
+  for (int e0 = tid; e0 < nr * nc; e0 += FU * kFillThreads) {
+    double2 z[FU];
+#pragma unroll
+    for (int u = 0; u < FU; ++u) {
+      const int e = e0 + u * kFillThreads;
+      if (e >= nr * nc) break;
+      const int32_t rr = s_rows[e / nc], cc = s_cols[e % nc];
+      if ((rr | cc) >= 0)  // this work item is the launch's only writer of the entry
+        z[u] = *reinterpret_cast<const double2*>(a.Z + 2 * ((int64_t)rr * a.ldz + cc));
+    }
+#pragma unroll
+    for (int u = 0; u < FU; ++u) {
+      const int e = e0 + u * kFillThreads;
+      if (e >= nr * nc) break;
+      const double re = acc[2 * e], im = acc[2 * e + 1];
+      const int32_t rr = s_rows[e / nc], cc = s_cols[e % nc];
+      double* dst = a.Z + 2 * ((int64_t)(rr & INT32_MAX) * a.ldz + (cc & INT32_MAX));
+      if ((rr | cc) >= 0) {
+        z[u].x += re;
+        z[u].y += im;
+        *reinterpret_cast<double2*>(dst) = z[u];
+      } else if (re != 0.0 || im != 0.0) {
+        unsafeAtomicAdd(dst, re);
+        unsafeAtomicAdd(dst + 1, im);
+      }
+    }
   }
 }
 
@@ -635,6 +667,7 @@ extern "C" int mom_fill_tier(int tier, int n_items, const void* args, void* stre
   // LDS: the accumulator plus about 18 KB of block data and staged spline,
   // within the 160 KB of a CU
   if (a.acc_bytes < 0 || a.acc_bytes > 136 * 1024) return fail("mom_fill_tier: LDS accumulator out of range");
+  if ((uintptr_t)a.Z % 16) return fail("mom_fill_tier: Z must be 16-byte aligned (complex128)");
   if (!a.item_r) return fail("mom_fill_tier: null item bounds");
   if (tier == 0 && (!a.near_static || !a.item_soff)) return fail("mom_fill_tier: near tier without its statics");
   hipStream_t s = (hipStream_t)stream;

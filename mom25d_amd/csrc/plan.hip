@@ -50,6 +50,7 @@ struct Plan {
   int64_t acc_bytes[kTiers] = {0, 0, 0};
   int64_t tier_pairs[kTiers] = {0, 0, 0};
   std::vector<int32_t> blk_row_off, blk_rows, blk_rloc, item_cols, src_cloc;
+  std::vector<uint8_t> row_own, col_own;  // entries with one writer per launch (FillArgs)
   std::vector<int64_t> item_soff;
   int64_t near_static_len = 0;
   std::vector<char> pair_used;  // (nl * nl) pairs some work item reads
@@ -70,6 +71,7 @@ struct Plan {
   Dev d_V, d_qp7, d_qw7, d_qp3, d_qw3, d_cen, d_area, d_cA, d_cV, d_tb, d_pad_pos, d_blk_tris, d_src;
   Dev d_items[kTiers], d_item_r[kTiers];
   Dev d_blk_row_off, d_blk_rows, d_blk_rloc, d_item_cols, d_src_cloc, d_item_soff, d_near_static;
+  Dev d_row_own, d_col_own;
   Dev d_pairs, d_tab_x, d_tab_c, d_tab_b;  // own tables
   Dev d_Z, d_IV;                           // staging of mom_plan_fill_host
   hipStream_t host_stream = nullptr;
@@ -78,7 +80,7 @@ struct Plan {
     std::vector<Dev*> v = {&d_V, &d_qp7, &d_qw7, &d_qp3, &d_qw3, &d_cen, &d_area, &d_cA, &d_cV, &d_tb,
                            &d_pad_pos, &d_blk_tris, &d_src, &d_blk_row_off, &d_blk_rows, &d_blk_rloc,
                            &d_item_cols, &d_src_cloc, &d_item_soff, &d_near_static, &d_pairs, &d_tab_x,
-                           &d_tab_c, &d_tab_b, &d_Z, &d_IV};
+                           &d_tab_c, &d_tab_b, &d_Z, &d_IV, &d_row_own, &d_col_own};
     for (int t = 0; t < kTiers; ++t) {
       v.push_back(&d_items[t]);
       v.push_back(&d_item_r[t]);
@@ -298,6 +300,35 @@ int build(Plan& P, const MomAssemblyGeometry& g) {
     P.acc_bytes[tier] = acc;
   }
 
+  // ownership: a block row whose basis function no other block holds, and
+  // an item column whose basis function no other item of the same block and
+  // tier holds (items of other blocks reach the entry only through their own
+  // rows), so that the item is the only writer of such an entry in its
+  // tier's launch
+  {
+    std::vector<int32_t> cnt((size_t)P.nb, 0);
+    for (int32_t m : P.blk_rows) ++cnt[m];
+    P.row_own.resize(P.blk_rows.size());
+    for (size_t e = 0; e < P.blk_rows.size(); ++e) P.row_own[e] = cnt[P.blk_rows[e]] == 1;
+    P.col_own.assign(P.item_cols.size(), 0);
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (int tier = 0; tier < kTiers; ++tier) {
+      const int32_t* it = P.items[tier].data();
+      const size_t n_items = P.items[tier].size() / 6;
+      std::vector<std::vector<int32_t>> by_blk((size_t)nblk);
+      for (size_t k = 0; k < n_items; ++k) by_blk[it[6 * k]].push_back((int32_t)k);
+      for (const auto& ks : by_blk) {
+        for (int32_t k : ks)
+          for (int32_t e = it[6 * k + 4]; e < it[6 * k + 4] + it[6 * k + 5]; ++e) ++cnt[P.item_cols[e]];
+        for (int32_t k : ks)
+          for (int32_t e = it[6 * k + 4]; e < it[6 * k + 4] + it[6 * k + 5]; ++e)
+            P.col_own[e] = cnt[P.item_cols[e]] == 1;
+        for (int32_t k : ks)
+          for (int32_t e = it[6 * k + 4]; e < it[6 * k + 4] + it[6 * k + 5]; ++e) cnt[P.item_cols[e]] = 0;
+      }
+    }
+  }
+
   // per work item, bounds [lo, hi] on the distance between any test point of
   // its block and any source point of its tier, padded for rounding
   std::vector<double> bc(2 * (size_t)nblk), br(nblk);
@@ -399,6 +430,8 @@ FillArgs base_args(Plan& P) {
   a.src_cloc = (const int32_t*)P.d_src_cloc.p;
   a.near_static = (const double*)P.d_near_static.p;
   a.item_soff = (const int64_t*)P.d_item_soff.p;
+  a.row_own = (const uint8_t*)P.d_row_own.p;
+  a.col_own = (const uint8_t*)P.d_col_own.p;
   return a;
 }
 
@@ -427,6 +460,8 @@ int to_device(Plan& P, hipStream_t s) {
   if (int rc = upload(P.d_item_cols, P.item_cols, s)) return rc;
   if (int rc = upload(P.d_src_cloc, P.src_cloc, s)) return rc;
   if (int rc = upload(P.d_item_soff, P.item_soff, s)) return rc;
+  if (int rc = upload(P.d_row_own, P.row_own, s)) return rc;
+  if (int rc = upload(P.d_col_own, P.col_own, s)) return rc;
   if (int rc = reserve(P.d_near_static, std::max<int64_t>(P.near_static_len, 1) * sizeof(double))) return rc;
   const int n_near = (int)(P.items[0].size() / 6);
   Dev dz;
@@ -617,6 +652,8 @@ extern "C" int mom_plan_host_array(const void* plan, const char* name, const voi
   if (n == "item_cols") return out(P.item_cols);
   if (n == "src_cloc") return out(P.src_cloc);
   if (n == "item_soff") return out(P.item_soff);
+  if (n == "row_own") return out(P.row_own);
+  if (n == "col_own") return out(P.col_own);
   if (n == "tab_x" || n == "tab_c" || n == "tab_b" || n == "pairs") {  // the own tables (mom_plan_set_pair)
     Plan& Q = const_cast<Plan&>(P);
     if (int rc = own_tables(Q)) return rc;
@@ -678,6 +715,7 @@ extern "C" int mom_plan_fill(void* plan, const MomFillCall* c, void* stream) {
   if (c->ldz < P.nb) return fail("mom_plan_fill: ldz below the number of RWG functions");
   if (P.n_pad && !c->IVpad) return fail("mom_plan_fill: the mesh has pad triangles: IVpad is required");
   if (c->stage_max < 0 || c->stage_max > 64) return fail("mom_plan_fill: stage_max outside [0, 64]");
+  if ((uintptr_t)c->Z % 16) return fail("mom_plan_fill: Z must be 16-byte aligned (complex128)");
   FillArgs a = base_args(P);
   if (c->pairs) {
     if (!c->tab_x || !c->tab_c || !c->tab_b) return fail("mom_plan_fill: incomplete device tables");

@@ -247,6 +247,8 @@ class FillPlan:
             self.item_cols = arr("item_cols", np.int32)
             self.src_cloc = arr("src_cloc", np.int32, (-1, 3))
             self.item_soff = arr("item_soff", np.int64)
+            self.row_own = arr("row_own", np.uint8).astype(bool)
+            self.col_own = arr("col_own", np.uint8).astype(bool)
             self.acc_bytes = {t: int(info.acc_bytes[t]) for t in (NEAR, MID, FAR)}
             self.tier_pairs = np.array(list(info.tier_pairs), np.int64)
             self.near_static_len = int(info.near_static_len)

@@ -127,6 +127,41 @@ def test_plan_scatter_maps_point_at_the_basis_functions(name):
             assert 16 * nr * nc <= plan.acc_bytes[tier] <= plan.ACC_MAX_BYTES
 
 
+@pytest.mark.parametrize("name", ["tee", "via2", "gcpw_pour"])
+def test_plan_owned_entries_have_one_writer(name):
+    """An entry the fill adds with a plain load and store (owned block row
+    and owned item column) is reached by no other work item of its tier's
+    launch: brute force over every (row, column) pair each item flushes.
+    The flags are also not vacuous: a good share of the entries is owned."""
+    from collections import Counter
+
+    from mom25d_amd.plan import FillPlan
+
+    lay, st, mk, _ = getattr(W, name)()
+    m = lay.mesh(**mk)
+    zl = np.array([(st.to_stackup() if hasattr(st, "to_stackup") else st).level_z(n) for n in m.layer_names])
+    plan = FillPlan(m, zl=zl)
+    assert plan.row_own.shape == plan.blk_rows.shape and plan.col_own.shape == plan.item_cols.shape
+    owned_total = flushed = 0
+    for tier, items in plan.items.items():
+        writers = Counter()
+        owned = []
+        for bi, pp, off, cnt, coff, nc in items:
+            r = slice(plan.blk_row_off[bi], plan.blk_row_off[bi + 1])
+            rows, cols = plan.blk_rows[r], plan.item_cols[coff:coff + nc]
+            R, C = np.meshgrid(rows, cols, indexing="ij")
+            keys = (R.astype(np.int64) * m.n_basis + C).ravel()
+            writers.update(keys.tolist())
+            own = (plan.row_own[r][:, None] & plan.col_own[coff:coff + nc][None, :]).ravel()
+            owned.append(keys[own])
+            flushed += keys.size
+        owned = np.concatenate(owned) if owned else np.zeros(0, np.int64)
+        assert all(writers[k] == 1 for k in owned.tolist()), tier
+        owned_total += owned.size
+    print(f"{name}: {owned_total / flushed:.2f} of the flushed entries owned")
+    assert owned_total > 0.2 * flushed
+
+
 @pytest.mark.parametrize("name", ["tee", "via2"])
 def test_plan_item_bounds_hold_every_distance(name):
     """The distance bounds of every work item (from which the fill kernel
