@@ -1415,25 +1415,28 @@ class Solver:
         return Precond(64, A, perm, sw, sb)
 
     def _remember(self, lane, freq, X):
-        """Keep the lane's last three solutions (device) for _extrapolate."""
+        """Keep the lane's last four solutions (device) for _extrapolate."""
         h = self._history.setdefault(lane, [])
         h.append((float(freq), X))
-        del h[:-3]
+        del h[:-4]
 
     def _extrapolate(self, lane, freq, B):
         """Initial guess at freq from the lane's last solutions: the Lagrange
-        polynomial through up to three of them, evaluated at freq, when freq
-        lies within one spacing of the last one (None otherwise).  The
-        solution varies smoothly with frequency away from resonances; GMRES
-        corrects whatever the extrapolation misses."""
-        h = self._history.get(lane, [])
-        if not h or h[-1][1].shape != B.shape:
+        polynomial through the (up to three) nearest of them within four
+        spacings, evaluated at freq, when one lies within two spacings of
+        freq (None otherwise; the spacing is the smallest between the kept
+        solutions, which may lie on both sides of freq).  The solution varies
+        smoothly with frequency away from resonances; GMRES corrects whatever
+        the extrapolation misses."""
+        h = list({f: X for f, X in self._history.get(lane, []) if X.shape == B.shape}.items())  # latest per f
+        fs = sorted(f for f, _ in h)
+        if len(fs) < 2:
             return None
-        fs = [f for f, _ in h]
-        step = abs(fs[-1] - fs[-2]) if len(fs) > 1 else None
-        if step is None or step == 0 or abs(freq - fs[-1]) > 1.5 * step:
+        step = min(b - a for a, b in zip(fs, fs[1:]))
+        if step <= 0 or min(abs(freq - f) for f in fs) > 2.0 * step:
             return None
-        pts = [(f, X) for f, X in h if abs(freq - f) <= 3.5 * step]
+        near = sorted((abs(freq - f), k) for k, (f, _) in enumerate(h) if abs(freq - f) <= 4.0 * step)
+        pts = [h[k] for _, k in near[:3]]
         X0 = torch.zeros_like(B)
         for i, (fi, Xi) in enumerate(pts):
             w = 1.0
@@ -1787,6 +1790,96 @@ class LaneCursor:
     """Hands the indices 0 .. count-1 to `lanes` lanes so that each lane
     works through neighbouring indices (neighbouring frequencies, whose
     factorisations can precondition each other): the range is cut into one
+    segment per lane, and a lane starts at its segment's middle and takes
+    the neighbours on alternate sides outward (above first), so that the
+    points that reuse its first factorisation lie within half a segment of
+    it.  The lowest segment is taken upward from its first index instead:
+    at the low end of a sweep complex64 factors may fail, and the complex128
+    one that then anchors it preconditions over octaves.  A lane whose
+    segment is used up takes the untaken outer end (of any segment with at
+    least two indices left) nearest to its own last index and works inward
+    from there, toward that segment's owner, who keeps working outward; it
+    gets None meanwhile when every segment has at most one index left (the
+    owners take those).  Every index is handed out exactly once."""
+
+    def __init__(self, count, lanes):
+        nseg = min(lanes, count)  # lanes beyond the count have no segment of their own
+        cuts = np.round(np.arange(nseg + 1) / max(nseg, 1) * count).astype(int)
+        # per segment [lo, a, b, hi]: the untaken indices are lo..a and b..hi
+        self.seg = []
+        for k in range(nseg):
+            lo, hi = int(cuts[k]), int(cuts[k + 1]) - 1
+            c = (lo + hi) // 2 if k > 0 else lo
+            self.seg.append([lo, c, c + 1, hi])
+        self.down = [True] * lanes  # the owner's next side (the first, the middle, is the lower side's top)
+        self.thief = [None] * lanes  # (segment, side) a lane works inward from, once its own is used up
+        self.last = [None] * lanes
+
+    def _left(self, r):
+        lo, a, b, hi = self.seg[r]
+        return max(a - lo + 1, 0) + max(hi - b + 1, 0)
+
+    def _own(self, lane):
+        lo, a, b, hi = self.seg[lane]
+        for _ in range(2):
+            if self.down[lane] and a >= lo:
+                self.seg[lane][1] -= 1
+                self.down[lane] = b > hi  # stays below once the upper side is used up
+                return a
+            if not self.down[lane] and b <= hi:
+                self.seg[lane][2] += 1
+                self.down[lane] = a >= lo
+                return b
+            self.down[lane] = not self.down[lane]
+        return None
+
+    def _steal(self, lane):
+        ends = []
+        for r, (lo, a, b, hi) in enumerate(self.seg):
+            if self._left(r) < 2:  # the owner's next index stays with the owner (its factors are near)
+                continue
+            if lo <= a:
+                ends.append((lo, r, "lo"))
+            if b <= hi:
+                ends.append((hi, r, "hi"))
+        if not ends:
+            return None
+        ref = self.last[lane]
+        if ref is None:  # a lane without a segment: the end of the segment with the most left
+            i, r, side = max(ends, key=lambda e: (self._left(e[1]), -e[0]))
+        else:
+            i, r, side = min(ends, key=lambda e: (abs(e[0] - ref), e[0]))
+        self.thief[lane] = (r, side)
+        return self._inward(lane)
+
+    def _inward(self, lane):
+        r, side = self.thief[lane]
+        lo, a, b, hi = self.seg[r]
+        if side == "lo" and lo <= a:
+            self.seg[r][0] += 1
+            return lo
+        if side == "hi" and b <= hi:
+            self.seg[r][3] -= 1
+            return hi
+        self.thief[lane] = None
+        return None
+
+    def take(self, lane):
+        i = None
+        if self.thief[lane] is not None:
+            i = self._inward(lane)
+        elif lane < len(self.seg):
+            i = self._own(lane)
+        if i is None:
+            i = self._steal(lane)
+        if i is not None:
+            self.last[lane] = i
+        return i
+
+
+class RunCursor:
+    """LaneCursor's alternative for batches of several meshes, whose points
+    are listed mesh after mesh: the range is cut into one
     run per pair of lanes, and the two lanes of a run take its indices from
     the two ends toward the middle (a lone lane takes its run upward).  A
     lane whose run is used up splits the run with the most indices left and
@@ -1847,13 +1940,14 @@ def _lane_streams(device, lanes):
     return pool[:lanes]
 
 
-def run_lanes(task, count, lanes, device, solvers, graphs=False):
+def run_lanes(task, count, lanes, device, solvers, graphs=False, cursor=None):
     """Run task(i, lane) for i < count, `lanes` at a time, round robin.
     Each task is a generator that enqueues GPU work on the current stream and
     yields before it waits for it; the lane's stream is made current whenever
     its task resumes, so the GPU work of all lanes overlaps.  Tasks complete
-    in order within a lane, and a lane takes its next index (LaneCursor:
-    neighbouring indices stay on one lane) as soon as its task ends.  The
+    in order within a lane, and a lane takes its next index (cursor, by
+    default LaneCursor: neighbouring indices stay on one lane) as soon as
+    its task ends.  The
     solvers bound their LU panels for `lanes` factorisations in flight
     meanwhile."""
     prev = torch.cuda.current_stream(device)
@@ -1865,7 +1959,7 @@ def run_lanes(task, count, lanes, device, solvers, graphs=False):
     for s in solvers:
         s._active_lanes = lanes
     active = [None] * lanes
-    cursor = LaneCursor(count, lanes)
+    cursor = (cursor or LaneCursor)(count, lanes)
     try:
         while True:
             busy = False
@@ -2012,7 +2106,8 @@ def _sweep_many(jobs, lanes, workers, keep_currents=False, graphs=False, on_poin
         if s.verbose:
             print(f"PROGRESS {done[0]}/{len(points)}", flush=True)
 
-    run_lanes(point, len(points), lanes, device, [o["solver"] for o in out], graphs)
+    run_lanes(point, len(points), lanes, device, [o["solver"] for o in out], graphs,
+              cursor=RunCursor if len(out) > 1 else LaneCursor)
     for o in out:
         for fi, fut in o["futures"].items():
             store(o, fi, fut.result())

@@ -313,16 +313,62 @@ def test_fit_pool_does_not_rerun_unguarded_scripts(tmp_path):
     assert out.count("MAIN RUN") == 1 and "sum 10" in out
 
 
-@pytest.mark.parametrize("count,lanes", [(0, 2), (1, 3), (10, 1), (51, 2), (64, 3), (7, 4), (100, 5)])
-def test_lane_cursor_hands_out_runs_of_neighbours(count, lanes):
-    """Every index once; a lane's indices are consecutive (up or down)
-    except where it splits another lane's run, so neighbouring frequencies
-    share a lane."""
+@pytest.mark.parametrize("count,lanes", [(0, 2), (1, 3), (10, 1), (51, 2), (64, 3), (7, 4), (20, 4), (100, 5), (3, 8)])
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_lane_cursor_works_outward_from_segment_middles(count, lanes, seed):
+    """Every index once, under any interleaving of the lanes.  A lane starts
+    at its segment's middle (the lowest segment: at its first index) and
+    takes its own indices in order of distance from there, so that the
+    points reusing its first factorisation stay close; once its segment is
+    used up it takes other segments' outer indices, working inward one
+    index at a time."""
     import random
 
     from mom25d_amd.solver import LaneCursor
 
     c = LaneCursor(count, lanes)
+    nseg = min(lanes, count)
+    cuts = np.round(np.arange(nseg + 1) / max(nseg, 1) * count).astype(int)
+    rng = random.Random(seed * 1000 + count * 31 + lanes)
+    seen, per_lane, active = [], {k: [] for k in range(lanes)}, list(range(lanes))
+    while active:
+        lane = rng.choice(active)
+        i = c.take(lane)
+        if i is None:
+            active.remove(lane)
+            continue
+        seen.append(i)
+        per_lane[lane].append(i)
+    assert sorted(seen) == list(range(count))
+    for lane, idx in per_lane.items():
+        if lane >= nseg:
+            continue
+        lo, hi = cuts[lane], cuts[lane + 1] - 1
+        mid = (lo + hi) // 2 if lane > 0 else lo  # the lowest segment runs upward from its first index
+        own = [i for i in idx if lo <= i <= hi]
+        assert mid not in own or own[0] == mid  # (others may reach the middle before a late lane starts)
+        d = [abs(i - mid) + (i < mid) * 0.5 for i in own]  # above before below at equal distance
+        assert d == sorted(d), (lane, idx)
+        stolen = [i for i in idx if not lo <= i <= hi]
+        runs = sum(int(abs(b - a) != 1) for a, b in zip(stolen, stolen[1:]))
+        assert runs <= lanes * max(1, int(np.ceil(np.log2(count + 2))))
+    # taken round robin, as the sweeps do, every lane opens at its segment's middle
+    c = LaneCursor(count, lanes)
+    first = [c.take(lane) for lane in range(lanes)]
+    for lane in range(nseg):
+        assert first[lane] == ((cuts[lane] + cuts[lane + 1] - 1) // 2 if lane > 0 else cuts[0])
+
+
+@pytest.mark.parametrize("count,lanes", [(0, 2), (1, 3), (10, 1), (51, 2), (64, 3), (7, 4), (100, 5)])
+def test_run_cursor_hands_out_runs_of_neighbours(count, lanes):
+    """The cursor of multi-mesh batches: every index once; a lane's indices
+    are consecutive (up or down) except where it splits another lane's run,
+    so neighbouring frequencies share a lane."""
+    import random
+
+    from mom25d_amd.solver import RunCursor
+
+    c = RunCursor(count, lanes)
     rng = random.Random(count * 31 + lanes)
     seen, per_lane, active = [], {k: [] for k in range(lanes)}, list(range(lanes))
     while active:
