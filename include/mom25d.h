@@ -45,7 +45,10 @@ int mom_j0(int n, const double* z, double* out, int real_axis, void* stream);
 
 /* FP64 impedance-matrix fill of one tier (0 near, 1 mid, 2 far) for n_items
  * work items; `args` points to a host FillArgs block (layout: csrc/fill.hip,
- * size mom_fill_args_size()).  Adds into Z (zeroed by the caller).
+ * size mom_fill_args_size()).  Adds into Z (zeroed by the caller; 16-byte
+ * aligned): through atomics, or with plain loads and stores for the entries
+ * the ownership arrays row_own / col_own mark as one work item's alone
+ * (null arrays: atomics everywhere).
  * Replaces _fast.chunk_vals / group_vals / scatter (Engine/mom25d/_fast.py:82-158,
  * 190-211) and the block loop of assemble (Engine/mom25d/solver.py:474-509). */
 int mom_fill_tier(int tier, int n_items, const void* args, void* stream);
